@@ -1,0 +1,206 @@
+/*
+ * smearfem.h -- C ABI of the MI355X (gfx950) assemble-and-solve library.
+ *
+ * Every entry point is `extern "C"`, takes plain pointers and sizes, and
+ * returns an int status (SFEM_OK == 0, negative == error class).  No entry
+ * point prints or exits.  The message of the last failure on a context is
+ * available from sfem_last_error().
+ *
+ * Array conventions follow the Julia caller (column-major, 1-based ids):
+ *   NodeList  ndim x n_nodes  Float64   x[d + ndim*node]
+ *   IEN       ien_rows x nen  Int64     ien[a*ien_rows + e]   (1-based node ids)
+ *   ID        id_rows x n_dof Int64     id[d*id_rows + node]  (1-based dof ids)
+ * so a Julia caller passes its arrays as they are (zero copy on the host).
+ *
+ * Reference interfaces replaced (file:line in soshalaw/smearFEM.jl):
+ *   sfem_gaussian_quadrature  <- gaussian_quadrature        src/fem.jl:21
+ *   sfem_basis_function       <- basis_function             src/fem.jl:48
+ *   sfem_assemble             <- assemble_system            src/fem.jl:135 (returns CSC via sfem_export_csc1)
+ *   sfem_bc_create + sfem_eliminate
+ *                             <- setboundaryCond + C'*K*C   examples/vector3D.jl:133-173, 314-316
+ *   sfem_system_solve         <- inv(Matrix(K_free))*C'*(-K*q_d)   examples/vector3D.jl:318-320
+ *   sfem_system_solution      <- q = q_d + C*q_f            examples/vector3D.jl:322
+ *
+ * Threading: a context owns one HIP stream.  Calls on one context are not
+ * re-entrant; different contexts may be used from different threads.
+ */
+#ifndef SMEARFEM_H
+#define SMEARFEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes (mapped by the Julia shim to the exceptions the
+ *      reference raises on the same inputs) ---------------------------- */
+#define SFEM_OK                 0
+#define SFEM_ERR_ARGUMENT      -1  /* ArgumentError: dof id < 1 reaching sparse() (src/fem.jl:253), bad option  */
+#define SFEM_ERR_BOUNDS        -2  /* BoundsError: IEN/ID index out of range (src/fem.jl:180,243), Q2 with nDof>=2 */
+#define SFEM_ERR_DIMENSION     -3  /* DimensionMismatch: ndim==1 (src/fem.jl:75,192), nDof != ndim, nen mismatch */
+#define SFEM_ERR_SINGULAR      -4  /* SingularException: det(Jac)==0 (src/fem.jl:195)                            */
+#define SFEM_ERR_UNDEFVAR      -5  /* UndefVarError: nGaussPoints not in {2,3} (src/fem.jl:30), unknown class   */
+#define SFEM_ERR_NOT_CONVERGED -6  /* solver reached max_iter before rtol                                       */
+#define SFEM_ERR_MEMORY        -7  /* device allocation failed or HBM budget exceeded                           */
+#define SFEM_ERR_DEVICE        -8  /* HIP runtime failure                                                       */
+#define SFEM_ERR_UNSUPPORTED   -9  /* beyond the library's index limits (e.g. >= 2^31 dofs, >= 2^32 COO entries) */
+#define SFEM_ERR_STATE        -10  /* handle used with the wrong context or after destroy                       */
+
+/* FunctionClass ("Q1" / "Q2", src/fem.jl:50,77) */
+#define SFEM_Q1 1
+#define SFEM_Q2 2
+
+typedef struct sfem_ctx    sfem_ctx;     /* device, stream, allocator, last error       */
+typedef struct sfem_mesh   sfem_mesh;    /* device-resident mesh + material inputs     */
+typedef struct sfem_csr    sfem_csr;     /* device-resident assembled K (block CSC)    */
+typedef struct sfem_bc     sfem_bc;      /* device-resident loads + Dirichlet data     */
+typedef struct sfem_system sfem_system;  /* device-resident K_ff, rhs, PCG state       */
+
+/* ---- lifecycle ------------------------------------------------------- */
+int         sfem_ctx_create(int device, sfem_ctx** out);
+void        sfem_ctx_destroy(sfem_ctx* ctx);
+const char* sfem_last_error(const sfem_ctx* ctx);
+/* device name, CU count, HBM bytes; any pointer may be NULL */
+int         sfem_device_info(sfem_ctx* ctx, char* name, size_t name_len, int* n_cu,
+                             int64_t* hbm_bytes);
+/* bytes the context's allocator currently holds on the device */
+int64_t     sfem_ctx_device_bytes(const sfem_ctx* ctx);
+
+/* ---- scalar utilities (host) ----------------------------------------- */
+/* gaussian_quadrature(a,b,n), src/fem.jl:21-31.  n in {2,3}, else
+ * SFEM_ERR_UNDEFVAR (the reference leaves xi undefined).  xi, w: n doubles. */
+int sfem_gaussian_quadrature(double a, double b, int n, double* xi, double* w);
+
+/* basis_function(xi,eta,zeta,FunctionClass), src/fem.jl:48-114.
+ * nargs = number of coordinates given (1: xi, 2: xi,eta, 3: xi,eta,zeta).
+ * N gets nfun values, dN gets nfun x ncol values column-major (Julia layout),
+ * where ncol = the reference's ΔN column count (Q1: 1D -> the 1x2 row, so
+ * nfun=2 and dN = [-0.5, 0.5] with *ncol = 2 and *nrow = 1).
+ * Q2 needs nargs >= 2 and ignores zeta, as the reference does. */
+int sfem_basis_function(int nargs, double xi, double eta, double zeta, int function_class,
+                        double* N, double* dN, int* nfun, int* nrow, int* ncol);
+
+/* ---- assembly: assemble_system, src/fem.jl:135-256 -------------------- */
+/* Upload mesh inputs once (H2D); assembly then runs from HBM.
+ *   n_elem      = ne^ndim in the reference (element count actually used)
+ *   young       = n_young values: 1 (the reference's scalar Young) or n_elem
+ *                 (per-element modulus; Ke is linear in E)
+ *   id          = may be NULL when n_dof == 1 (reference ignores ID then)  */
+int sfem_mesh_create(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
+                     const double* node_list, int64_t n_nodes,
+                     const int64_t* ien, int64_t ien_rows, int nen, int64_t n_elem,
+                     const int64_t* id, int64_t id_rows,
+                     const double* young, int64_t n_young, double nu,
+                     sfem_mesh** out);
+void sfem_mesh_destroy(sfem_mesh* mesh);
+
+/* flags for sfem_assemble_mesh */
+#define SFEM_ASM_DEFAULT       0
+#define SFEM_ASM_FORCE_SCALAR  1   /* sort dof-pair keys even when ID is node-blocked */
+
+/* Assemble K = sparse(E,J,V) on the device.  Pattern is the union of the
+ * element couplings (explicit zeros kept), duplicates are summed in
+ * ascending COO index order, dims = max row/col id -- the semantics of
+ * Julia's sparse(I,J,V) called at src/fem.jl:253. */
+int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* mesh, int flags, sfem_csr** out);
+
+/* One-shot drop-in for assemble_system: sfem_mesh_create + sfem_assemble_mesh. */
+int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
+                  const double* node_list, int64_t n_nodes,
+                  const int64_t* ien, int64_t ien_rows, int nen, int64_t n_elem,
+                  const int64_t* id, int64_t id_rows,
+                  const double* young, int64_t n_young, double nu,
+                  sfem_csr** out);
+
+/* m, n, stored entries, and the block size the device copy is kept in
+ * (n_dof when ID is node-blocked, else 1).  Any pointer may be NULL. */
+int  sfem_csr_info(const sfem_csr* K, int64_t* m, int64_t* n, int64_t* nnz, int* block);
+
+/* Fill caller-allocated host buffers with the 1-based CSC arrays of
+ * SparseMatrixCSC{Float64,Int64}: colptr (n+1), rowval (nnz), nzval (nnz). */
+int  sfem_export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval,
+                      double* nzval);
+void sfem_csr_destroy(sfem_csr* K);
+/* Device time of the last assembly's stages (ms): validate, element Ke,
+ * keys, sort, segmented reduction, total.  ms: 6 doubles. */
+int  sfem_csr_stage_ms(const sfem_csr* K, double* ms);
+
+/* ---- Dirichlet elimination, examples/vector3D.jl:133-173, 314-322 ----- */
+/* n = size(K,1).  f: load vector (NULL = 0).  fixed: n bytes, 1 = prescribed
+ * dof (the complement of the columns C = I[:,free] keeps).  q_d: prescribed
+ * values on fixed dofs (NULL = 0; entries on free dofs are ignored). */
+int  sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fixed,
+                    const double* q_d, sfem_bc** out);
+void sfem_bc_destroy(sfem_bc* bc);
+
+/* Build K_ff = C' K C and rhs = C'(f - K q_d) on the device.  The solver
+ * formats are built here: FP64 CSR always, and, when K is node-blocked and
+ * every node is either fully fixed or fully free, FP64/FP32 block formats. */
+int  sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out);
+int  sfem_system_info(const sfem_system* sys, int64_t* n_free, int64_t* nnz_ff, int* block);
+void sfem_system_destroy(sfem_system* sys);
+
+/* ---- solve ------------------------------------------------------------ */
+#define SFEM_PREC_FP64   0   /* plain FP64 Jacobi-PCG on FP64 CSR                       */
+#define SFEM_PREC_MIXED  1   /* FP32 operator/CG with FP64 residual refinement          */
+
+#define SFEM_FMT_AUTO    0
+#define SFEM_FMT_CSR     1   /* scalar CSR, int32 columns                               */
+#define SFEM_FMT_BSR     2   /* node-block (n_dof x n_dof) sliced format, int32 columns */
+
+typedef struct {
+    int     precision;       /* SFEM_PREC_*                                              */
+    int     format;          /* SFEM_FMT_*                                               */
+    double  rtol;            /* stop when ||rhs - K_ff x||_2 <= rtol ||rhs||_2 (FP64)   */
+    int64_t max_iter;        /* cap on CG iterations (all precisions)                    */
+    double  reliable_delta;  /* mixed: FP64 update when the FP32 residual drops by this  */
+    int     check_every;     /* iterations enqueued between host convergence polls      */
+    int     time_kernels;    /* 1: time every SpMV launch with HIP events                */
+    int     x0_zero;         /* 1: start from x = 0 (else from the system's current x)   */
+} sfem_solve_opts;
+
+typedef struct {
+    int64_t iterations;      /* CG iterations performed                                  */
+    int64_t fp64_updates;    /* mixed: FP64 residual replacements performed              */
+    int64_t spmv_launches;   /* operator applications in the iteration (any precision)   */
+    int64_t spmv64_launches; /* FP64 residual SpMVs                                      */
+    double  rel_residual;    /* final ||rhs - K_ff x|| / ||rhs|| evaluated in FP64       */
+    double  solve_ms;        /* device time of the solve (HIP events)                    */
+    double  spmv_ms;         /* summed device time of timed SpMV launches                */
+    double  spmv_bytes;      /* algorithmic bytes of one iteration SpMV                  */
+    int64_t spmv_timed;      /* number of SpMV launches that were timed                  */
+    int     converged;
+    int     format;          /* format actually used                                     */
+} sfem_solve_stats;
+
+void sfem_solve_opts_default(sfem_solve_opts* opts);
+
+int  sfem_system_solve(sfem_ctx* ctx, sfem_system* sys, const sfem_solve_opts* opts,
+                       sfem_solve_stats* stats);
+/* q = q_d + C q_f (n values, FP64), and optionally reactions K q - f on the
+ * fixed dofs (n values; zero on free dofs). */
+int  sfem_system_solution(sfem_ctx* ctx, const sfem_system* sys, double* q,
+                          double* reactions);
+
+/* One-shot: bc_create + eliminate + solve + solution. */
+int  sfem_solve(sfem_ctx* ctx, const sfem_csr* K, const double* f, const uint8_t* fixed,
+                const double* q_d, const sfem_solve_opts* opts, double* q, double* reactions,
+                sfem_solve_stats* stats);
+
+/* ---- measurement helpers --------------------------------------------- */
+/* y = K_ff x in the given precision/format, `reps` times, on the system's
+ * own vectors; returns the average device time per launch (HIP events). */
+int  sfem_system_spmv_bench(sfem_ctx* ctx, sfem_system* sys, int precision, int format,
+                            int reps, double* avg_ms, double* bytes);
+/* Stream-copy `bytes` bytes device-to-device with the library's copy kernel
+ * `reps` times; returns average ms per launch (streaming HBM ceiling). */
+int  sfem_stream_copy_bench(sfem_ctx* ctx, int64_t bytes, int reps, double* avg_ms);
+/* Wait for all work on the context's stream. */
+int  sfem_ctx_synchronize(sfem_ctx* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SMEARFEM_H */

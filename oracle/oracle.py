@@ -1,0 +1,287 @@
+"""CPU oracle for smearFEM's assemble-and-solve path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of bench.py, as the checker -- never by the product.
+
+ctypes wrapper over ``sfem_oracle.c`` (a C restatement of the reference, see
+its header for file:line citations) plus small pure-Python restatements:
+
+* ``meshgrid_ref``  -- examples/vector3D.jl:10-130 (cube meshes, as written)
+* ``sparse_ref``    -- Julia ``sparse(I,J,V)`` semantics with pure-Python
+                       loops (tiny inputs only), an independent check of the
+                       C combine
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+_SO = HERE / "libsfem_oracle.so"
+
+
+class OracleError(Exception):
+    def __init__(self, code, where=None):
+        super().__init__(f"oracle status {code} at element {where}")
+        self.code = code
+        self.where = where
+
+
+def build() -> Path:
+    subprocess.run(["make", "-s", "-C", str(HERE)], check=True)
+    return _SO
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not _SO.exists():
+            build()
+        L = C.CDLL(os.fspath(_SO))
+        d, i64, i32, vp = C.c_double, C.c_int64, C.c_int32, C.c_void_p
+        P = C.POINTER
+        L.or_gaussian_quadrature.argtypes = [d, d, C.c_int, P(d), P(d)]
+        L.or_basis_function.argtypes = [C.c_int, d, d, d, C.c_int, P(d), P(d), P(C.c_int), P(C.c_int),
+                                        P(C.c_int)]
+        L.or_element_stiffness.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int, P(d), d, d, P(d)]
+        L.or_coo_length.restype = i64
+        L.or_coo_length.argtypes = [C.c_int, C.c_int, C.c_int, i64]
+        L.or_assemble_coo.argtypes = [C.c_int, C.c_int, C.c_int, P(d), i64, P(i64), i64, C.c_int, i64, P(i64),
+                                      i64, P(d), i64, d, P(i64), P(i64), P(d), P(i64)]
+        L.or_sparse_dims.argtypes = [i64, P(i64), P(i64), P(i64), P(i64)]
+        L.or_sparse_csc.argtypes = [i64, P(i64), P(i64), P(d), i64, i64, P(i64), P(i64), P(d), P(i64)]
+        L.or_eliminate_nnz.restype = i64
+        L.or_eliminate_nnz.argtypes = [i64, P(i64), P(i64), P(i64)]
+        L.or_eliminate.argtypes = [i64, P(i64), P(i64), P(d), P(i64), P(d), P(d), P(i64), P(i32), P(d), P(d)]
+        L.or_spmv.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d)]
+        L.or_dense_spd_solve.argtypes = [i64, P(d), P(d), P(d)]
+        L.or_pcg.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d), d, i64, P(i64), P(d)]
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(C.POINTER(t))
+
+
+# ------------------------------------------------------------ L1 --
+def gaussian_quadrature(a, b, n=2):
+    xi = np.zeros(3)
+    w = np.zeros(3)
+    st = lib().or_gaussian_quadrature(float(a), float(b), int(n), _p(xi, C.c_double), _p(w, C.c_double))
+    if st:
+        raise OracleError(st)
+    return list(xi[:n]), list(w[:n])
+
+
+def basis_function(xi, eta=None, zeta=None, FunctionClass="Q1"):
+    nargs = 1 if eta is None else (2 if zeta is None else 3)
+    cls = {"Q1": 1, "Q2": 2}.get(FunctionClass, 0)
+    N = np.zeros(9)
+    dN = np.zeros(27)
+    nf, nr, nc = C.c_int(), C.c_int(), C.c_int()
+    st = lib().or_basis_function(nargs, float(xi), float(eta or 0), float(zeta or 0), cls, _p(N, C.c_double),
+                                 _p(dN, C.c_double), C.byref(nf), C.byref(nr), C.byref(nc))
+    if st:
+        raise OracleError(st)
+    return N[: nf.value].copy(), dN[: nr.value * nc.value].reshape(nc.value, nr.value).T.copy()
+
+
+def element_stiffness(ndim, FunctionClass, nDof, coords, E=1.0, nu=0.3):
+    """Ke summed over the Gauss loop; coords: ndim x nen.  Row-major L x L."""
+    coords = np.asfortranarray(coords, dtype=np.float64)
+    nen = coords.shape[1]
+    L = nen * nDof
+    Ke = np.zeros(L * L)
+    st = lib().or_element_stiffness(ndim, {"Q1": 1, "Q2": 2}[FunctionClass], nDof, nen, _p(coords, C.c_double),
+                                    float(E), float(nu), _p(Ke, C.c_double))
+    if st:
+        raise OracleError(st)
+    return Ke.reshape(L, L)
+
+
+# ------------------------------------------------------------ L2 --
+def assemble_coo(n_elem, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=None, Young=1, nu=0.3):
+    nodes = np.asfortranarray(NodeList, dtype=np.float64)
+    ien = np.asfortranarray(IEN, dtype=np.int64)
+    young = np.ascontiguousarray(np.atleast_1d(np.asarray(Young, dtype=np.float64)))
+    idm = None if ID is None else np.asfortranarray(ID, dtype=np.int64)
+    nen = ien.shape[1]
+    ncoo = lib().or_coo_length(ndim, nDof, nen, n_elem)
+    E = np.zeros(max(ncoo, 0), dtype=np.int64)
+    J = np.zeros(max(ncoo, 0), dtype=np.int64)
+    V = np.zeros(max(ncoo, 0), dtype=np.float64)
+    err = C.c_int64(-1)
+    st = lib().or_assemble_coo(ndim, {"Q1": 1, "Q2": 2}.get(FunctionClass, 0), nDof, _p(nodes, C.c_double),
+                               nodes.shape[1], _p(ien, C.c_int64), ien.shape[0], nen, n_elem,
+                               _p(idm, C.c_int64) if idm is not None else None,
+                               idm.shape[0] if idm is not None else 0, _p(young, C.c_double), young.size,
+                               float(nu), _p(E, C.c_int64), _p(J, C.c_int64), _p(V, C.c_double), C.byref(err))
+    if st:
+        raise OracleError(st, err.value)
+    return E, J, V
+
+
+def sparse_csc(I, J, V):
+    """Julia sparse(I,J,V) -> (m, n, colptr, rowval, nzval), 1-based arrays."""
+    I = np.ascontiguousarray(I, dtype=np.int64)
+    J = np.ascontiguousarray(J, dtype=np.int64)
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    m, n = C.c_int64(0), C.c_int64(0)
+    st = lib().or_sparse_dims(I.size, _p(I, C.c_int64), _p(J, C.c_int64), C.byref(m), C.byref(n))
+    if st:
+        raise OracleError(st)
+    colptr = np.zeros(n.value + 1, dtype=np.int64)
+    rowval = np.zeros(max(I.size, 1), dtype=np.int64)
+    nzval = np.zeros(max(I.size, 1), dtype=np.float64)
+    nnz = C.c_int64(0)
+    st = lib().or_sparse_csc(I.size, _p(I, C.c_int64), _p(J, C.c_int64), _p(V, C.c_double), m.value, n.value,
+                             _p(colptr, C.c_int64), _p(rowval, C.c_int64), _p(nzval, C.c_double), C.byref(nnz))
+    if st:
+        raise OracleError(st)
+    if n.value == 0:
+        colptr[0] = 1
+    return m.value, n.value, colptr, rowval[: nnz.value].copy(), nzval[: nnz.value].copy()
+
+
+def assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=None, Young=1, nu=0.3,
+                    n_elem=None):
+    """The reference's assemble_system on the CPU -> scipy csc_matrix."""
+    import scipy.sparse as sp
+
+    if n_elem is None:
+        n_elem = int(ne) ** int(ndim)
+    E, J, V = assemble_coo(n_elem, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, nu)
+    m, n, colptr, rowval, nzval = sparse_csc(E, J, V)
+    return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(m, n))
+
+
+def sparse_ref(I, J, V):
+    """Pure-Python Julia sparse(): dict keyed by (i,j), values folded left to
+    right in input order.  Returns a dict {(i, j): value} and (m, n)."""
+    if any(i < 1 for i in I) or any(j < 1 for j in J):
+        raise OracleError(-1)
+    acc = {}
+    for i, j, v in zip(I, J, V):
+        k = (int(i), int(j))
+        acc[k] = acc[k] + v if k in acc else v
+    m = max(I) if len(I) else 0
+    n = max(J) if len(J) else 0
+    return acc, (int(m), int(n))
+
+
+def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim):
+    """examples/vector3D.jl:10-130 restated with plain loops (NodeList, IEN, ID)."""
+    NodeList = np.zeros((ndim, (ne + 1) ** ndim))
+    IEN = np.zeros((ne ** ndim, 2 ** ndim), dtype=np.int64)
+    ID = np.zeros(((ne + 1) ** ndim, ndim), dtype=np.int64)
+    x = np.linspace(x0, x1, ne + 1)
+    y = np.linspace(y0, y1, ne + 1)
+    if ndim == 2:
+        m = 1
+        for j in range(1, ne + 2):
+            for i in range(1, ne + 2):
+                NodeList[0, m - 1] = x[i - 1]
+                NodeList[1, m - 1] = y[j - 1]
+                for l in range(1, ndim + 1):
+                    ID[m - 1, l - 1] = ndim * (m - 1) + l
+                m += 1
+        n = 1
+        for j in range(1, ne + 1):
+            for i in range(1, ne + 1):
+                IEN[n - 1] = [(j - 1) * (ne + 1) + i, (j - 1) * (ne + 1) + i + 1, j * (ne + 1) + i + 1,
+                              j * (ne + 1) + i]
+                n += 1
+    else:
+        z = np.linspace(z0, z1, ne + 1)
+        m = 1
+        for k in range(1, ne + 2):
+            for j in range(1, ne + 2):
+                for i in range(1, ne + 2):
+                    NodeList[:, m - 1] = [x[i - 1], y[j - 1], z[k - 1]]
+                    for l in range(1, ndim + 1):
+                        ID[m - 1, l - 1] = ndim * (m - 1) + l
+                    m += 1
+        n = 1
+        p = (ne + 1) ** 2
+        for k in range(1, ne + 1):
+            for j in range(1, ne + 1):
+                for i in range(1, ne + 1):
+                    IEN[n - 1] = [(k - 1) * p + (j - 1) * (ne + 1) + i, (k - 1) * p + (j - 1) * (ne + 1) + i + 1,
+                                  (k - 1) * p + j * (ne + 1) + i + 1, (k - 1) * p + j * (ne + 1) + i,
+                                  k * p + (j - 1) * (ne + 1) + i, k * p + (j - 1) * (ne + 1) + i + 1,
+                                  k * p + j * (ne + 1) + i + 1, k * p + j * (ne + 1) + i]
+                    n += 1
+    return NodeList, IEN, ID
+
+
+# ---------------------------------------------------- elimination / solve --
+def eliminate(K, fixed, f=None, q_d=None):
+    """(K_ff as scipy csc, rhs) following examples/vector3D.jl:315-316,320."""
+    import scipy.sparse as sp
+
+    K = K.tocsc()
+    n = K.shape[0]
+    colptr = (K.indptr + 1).astype(np.int64)
+    rowval = (K.indices + 1).astype(np.int64)
+    nzval = K.data.astype(np.float64)
+    fixed = np.asarray(fixed, dtype=bool)
+    free_map = np.where(~fixed, np.cumsum(~fixed) - 1, -1).astype(np.int64)
+    nf = int((~fixed).sum())
+    nnz = lib().or_eliminate_nnz(n, _p(colptr, C.c_int64), _p(rowval, C.c_int64), _p(free_map, C.c_int64))
+    ffp = np.zeros(nf + 1, dtype=np.int64)
+    ffr = np.zeros(max(nnz, 1), dtype=np.int32)
+    ffv = np.zeros(max(nnz, 1), dtype=np.float64)
+    rhs = np.zeros(nf)
+    fv = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+    qv = None if q_d is None else np.ascontiguousarray(np.where(fixed, q_d, 0.0), dtype=np.float64)
+    st = lib().or_eliminate(n, _p(colptr, C.c_int64), _p(rowval, C.c_int64), _p(nzval, C.c_double),
+                            _p(free_map, C.c_int64), _p(fv, C.c_double) if fv is not None else None,
+                            _p(qv, C.c_double) if qv is not None else None, _p(ffp, C.c_int64),
+                            _p(ffr, C.c_int32), _p(ffv, C.c_double), _p(rhs, C.c_double))
+    if st:
+        raise OracleError(st)
+    Kff = sp.csc_matrix((ffv[:nnz], ffr[:nnz], ffp), shape=(nf, nf))
+    return Kff, rhs
+
+
+def dense_solve(Kff, rhs):
+    A = np.ascontiguousarray(Kff.toarray(), dtype=np.float64)
+    x = np.zeros_like(rhs)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    st = lib().or_dense_spd_solve(A.shape[0], _p(A, C.c_double), _p(b, C.c_double), _p(x, C.c_double))
+    if st:
+        raise OracleError(st)
+    return x
+
+
+def pcg(Kff, rhs, rtol=1e-10, max_iter=100000, x0=None):
+    """FP64 Jacobi-PCG, single thread; returns (x, iterations, relres)."""
+    A = Kff.tocsc()
+    ptr = A.indptr.astype(np.int64)
+    idx = A.indices.astype(np.int32)
+    val = A.data.astype(np.float64)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros_like(b) if x0 is None else np.ascontiguousarray(x0, dtype=np.float64).copy()
+    it = C.c_int64(0)
+    rr = C.c_double(0)
+    st = lib().or_pcg(b.size, _p(ptr, C.c_int64), _p(idx, C.c_int32), _p(val, C.c_double), _p(b, C.c_double),
+                      _p(x, C.c_double), float(rtol), int(max_iter), C.byref(it), C.byref(rr))
+    if st not in (0, -6):
+        raise OracleError(st)
+    return x, it.value, rr.value
+
+
+def recombine(fixed, q_d, x):
+    """q = q_d + C q_f (examples/vector3D.jl:322)."""
+    fixed = np.asarray(fixed, dtype=bool)
+    q = np.where(fixed, 0.0 if q_d is None else q_d, 0.0).astype(np.float64)
+    q[~fixed] = x
+    return q

@@ -1,0 +1,306 @@
+// assemble.hip -- K = sparse(E, J, V) on the device by sort-by-key and an
+// in-order segmented reduction.
+//
+// Reference: smearFEM src/fem.jl:135-256; the combine semantics are those of
+// Julia's SparseArrays.sparse(I,J,V) at src/fem.jl:253: entries with equal
+// (row, col) are summed left to right in COO index order, explicit zeros are
+// kept (the pattern is the union of element couplings), rows ascend within
+// each column, and the dimensions are the largest row / column index.
+//
+// Pipeline (one stream, stage times recorded with HIP events):
+//   validate -> element Ke (COO values) -> keys -> stable radix sort ->
+//   segment heads -> scan -> in-order segment sums -> block CSC pointers.
+#include "sfem_objects.hpp"
+
+namespace sfem {
+
+namespace {
+
+__global__ __launch_bounds__(256) void seg_flags_kernel(const uint64_t* __restrict__ keys, int64_t n,
+                                                        uint32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = (i == 0 || keys[i] != keys[i - 1]) ? 1u : 0u;
+}
+
+// pos = exclusive scan of the flags; heads of segments record where they start.
+__global__ __launch_bounds__(256) void seg_heads_kernel(const uint64_t* __restrict__ keys,
+                                                        const uint32_t* __restrict__ pos, int64_t n,
+                                                        uint32_t nseg, uint32_t* __restrict__ head,
+                                                        uint64_t* __restrict__ ukey) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint64_t k = keys[i];
+    if (i == 0 || k != keys[i - 1]) {
+        head[pos[i]] = (uint32_t)i;
+        ukey[pos[i]] = k;
+    }
+    if (i == n - 1) head[nseg] = (uint32_t)n;
+}
+
+// One thread per unique key: values of the segment summed in sorted order,
+// which is ascending COO index (stable sort), starting from the first value
+// as Julia's combine does.  Block values for payload p sit at
+// V[p + i*ndof + j] (contiguous when bs == ndof).
+template <int BS>
+__global__ __launch_bounds__(256) void seg_sum_kernel(const double* __restrict__ V,
+                                                      const uint32_t* __restrict__ payload,
+                                                      const uint32_t* __restrict__ head,
+                                                      const uint64_t* __restrict__ ukey, int64_t nseg,
+                                                      int ndof, int row_bits,
+                                                      double* __restrict__ bval,
+                                                      int32_t* __restrict__ browidx,
+                                                      int32_t* __restrict__ bcolidx) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nseg) return;
+    const uint32_t s = head[u], e = head[u + 1];
+    double acc[BS * BS];
+    {
+        const uint32_t p = payload[s];
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) acc[i * BS + j] = V[(int64_t)p + i * ndof + j];
+    }
+    for (uint32_t q = s + 1; q < e; ++q) {
+        const uint32_t p = payload[q];
+#pragma unroll
+        for (int i = 0; i < BS; ++i)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) acc[i * BS + j] += V[(int64_t)p + i * ndof + j];
+    }
+#pragma unroll
+    for (int v = 0; v < BS * BS; ++v) bval[u * BS * BS + v] = acc[v];
+    const uint64_t k = ukey[u];
+    browidx[u] = (int32_t)(k & ((1ull << row_bits) - 1));
+    bcolidx[u] = (int32_t)(k >> row_bits);
+}
+
+// bcolptr[c] = first block whose column unit is >= c.
+__global__ __launch_bounds__(256) void colptr_kernel(const int32_t* __restrict__ bcolidx, int64_t nnzb,
+                                                     int64_t nunits, int64_t* __restrict__ bcolptr) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nnzb) return;
+    const int64_t c = bcolidx[u];
+    const int64_t cp = u == 0 ? -1 : bcolidx[u - 1];
+    for (int64_t cc = cp + 1; cc <= c; ++cc) bcolptr[cc] = u;
+    if (u == nnzb - 1)
+        for (int64_t cc = c + 1; cc <= nunits; ++cc) bcolptr[cc] = nnzb;
+}
+
+__global__ void fill_i64_kernel(int64_t* p, int64_t n, int64_t v) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+// Scalar CSC (1-based, Int64 indices) from the block CSC.
+__global__ __launch_bounds__(256) void export_colptr_kernel(const int64_t* __restrict__ bcolptr, int bs,
+                                                            int64_t n, int64_t nnz,
+                                                            int64_t* __restrict__ colptr) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > n) return;
+    if (s == n) { colptr[n] = nnz + 1; return; }
+    const int64_t cu = s / bs, j = s % bs;
+    const int64_t b0 = bcolptr[cu], nbc = bcolptr[cu + 1] - b0;
+    colptr[s] = b0 * bs * bs + j * nbc * bs + 1;
+}
+
+__global__ __launch_bounds__(256) void export_entries_kernel(
+    const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+    const int32_t* __restrict__ bcolidx, const double* __restrict__ bval, int bs, int64_t nnz,
+    int64_t* __restrict__ rowval, double* __restrict__ nzval) {
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnz) return;
+    const int bs2 = bs * bs;
+    const int64_t u = t / bs2;
+    const int v = (int)(t - u * bs2), i = v / bs, j = v % bs;
+    const int64_t cu = bcolidx[u];
+    const int64_t b0 = bcolptr[cu], nbc = bcolptr[cu + 1] - b0, k = u - b0;
+    const int64_t pos = b0 * bs2 + (int64_t)j * nbc * bs + k * bs + i;
+    rowval[pos] = (int64_t)browidx[u] * bs + i + 1;
+    nzval[pos] = bval[t];
+}
+
+int bits_for(int64_t nunits) {
+    int b = 0;
+    while ((int64_t(1) << b) < nunits) ++b;
+    return b;
+}
+
+struct Timer {
+    hipEvent_t ev[8];
+    int n = 0;
+    hipStream_t s;
+    explicit Timer(hipStream_t st) : s(st) {
+        for (auto& e : ev) SFEM_HIP(hipEventCreate(&e));
+    }
+    ~Timer() {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+    }
+    void mark() { SFEM_HIP(hipEventRecord(ev[n++], s)); }
+    double ms(int a, int b) const {
+        float t = 0;
+        SFEM_HIP(hipEventElapsedTime(&t, ev[a], ev[b]));
+        return t;
+    }
+};
+
+int code_of_step(int step) {
+    switch (step) {
+        case 0: case 4: return SFEM_ERR_BOUNDS;
+        case 2: case 5: return SFEM_ERR_SINGULAR;
+        default: return SFEM_ERR_DIMENSION;
+    }
+}
+
+const char* name_of_code(int code) {
+    switch (code) {
+        case SFEM_ERR_BOUNDS: return "BoundsError";
+        case SFEM_ERR_SINGULAR: return "SingularException";
+        case SFEM_ERR_ARGUMENT: return "ArgumentError";
+        case SFEM_ERR_DIMENSION: return "DimensionMismatch";
+        case SFEM_ERR_UNDEFVAR: return "UndefVarError";
+        default: return "error";
+    }
+}
+
+}  // namespace
+
+void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
+    K->ctx = ctx;
+    Timer tm(ctx->stream);
+    tm.mark();
+    DevBuf<ValidateOut> vout(ctx, 1);
+    {
+        ValidateOut init{~0ull, -1, INT32_MAX, 1, 0};
+        h2d(ctx, vout.get(), &init, sizeof(init));
+    }
+    launch_validate(ctx, m, vout.get());
+    tm.mark();
+    const int nen = m->nen, ndof = m->ndof;
+    const int64_t KE = (int64_t)nen * nen * ndof * ndof;
+    const int64_t ncoo = m->n_elem * KE;
+    if (ncoo >= (int64_t(1) << 32)) fail(SFEM_ERR_UNSUPPORTED, "assemble_system: 2^32 or more COO entries");
+    DevBuf<double> V(ctx, (size_t)ncoo);
+    launch_element_stiffness(ctx, m, V.get(), vout.get());
+    tm.mark();
+    ValidateOut vr;
+    d2h(ctx, &vr, vout.get(), sizeof(vr));
+
+    // first error in the reference's order: element-level errors, then the
+    // bounds check inside sparse() on the dof ids (ArgumentError)
+    unsigned long long key = vr.err_key;
+    int code = key == ~0ull ? 0 : code_of_step((int)(key & 7));
+    if (m->pre_code && (unsigned long long)m->pre_key < key) {
+        key = (unsigned long long)m->pre_key;
+        code = m->pre_code;
+    }
+    if (code) {
+        char buf[160];
+        snprintf(buf, sizeof buf, "assemble_system: %s at element %lld", name_of_code(code),
+                 (long long)(key >> 3) + 1);
+        fail(code, buf);
+    }
+    if (ndof > 1 && vr.min_dof == -1) fail(SFEM_ERR_ARGUMENT, "sparse: row/column index < 1 (ID entry < 1)");
+    if (ndof > 1 && vr.min_dof < -1) fail(SFEM_ERR_UNSUPPORTED, "ID entry >= 2^31");
+
+    int bs = 1;
+    int64_t nunits = vr.max_unit + 1;  // nodes (ndof == 1) or dofs
+    if (ndof > 1 && vr.blocked && !(flags & SFEM_ASM_FORCE_SCALAR)) {
+        bs = ndof;
+        nunits = vr.max_unit / ndof + 1;
+    }
+    K->bs = bs;
+    K->nunits = m->n_elem ? nunits : 0;
+    K->m = K->n = K->nunits * bs;
+    const int row_bits = bits_for(K->nunits);
+    if (2 * row_bits > 64) fail(SFEM_ERR_UNSUPPORTED, "too many dofs for 64-bit keys");
+
+    const int64_t npairs = (bs == 1 && ndof > 1) ? ncoo : m->n_elem * nen * nen;
+    DevBuf<uint64_t> keys(ctx, (size_t)npairs), keys_alt(ctx, (size_t)npairs);
+    DevBuf<uint32_t> pay(ctx, (size_t)npairs), pay_alt(ctx, (size_t)npairs);
+    launch_keys(ctx, m, bs, row_bits, keys.get(), pay.get());
+    tm.mark();
+    uint64_t* kp = keys.get();
+    uint32_t* vp = pay.get();
+    radix_sort_pairs(ctx, &kp, keys_alt.get(), &vp, pay_alt.get(), npairs, 2 * row_bits);
+    tm.mark();
+
+    // segments
+    K->bcolptr.reset(ctx, (size_t)K->nunits + 1);
+    if (npairs == 0) {
+        K->nnzb = K->nnz = 0;
+        hipLaunchKernelGGL(fill_i64_kernel, dim3(grid_for(K->nunits + 1, 256)), dim3(256), 0, ctx->stream,
+                           K->bcolptr.get(), K->nunits + 1, (int64_t)0);
+        SFEM_LAUNCH_CHECK();
+    } else {
+        DevBuf<uint32_t> pos(ctx, (size_t)npairs);
+        DevBuf<uint32_t> dtot(ctx, 1);
+        hipLaunchKernelGGL(seg_flags_kernel, dim3(grid_for(npairs, 256)), dim3(256), 0, ctx->stream, kp, npairs,
+                           pos.get());
+        SFEM_LAUNCH_CHECK();
+        exclusive_scan_u32_dev(ctx, pos.get(), pos.get(), npairs, dtot.get());
+        uint32_t nseg = 0;
+        d2h(ctx, &nseg, dtot.get(), sizeof(nseg));
+        K->nnzb = nseg;
+        K->nnz = K->nnzb * bs * bs;
+        DevBuf<uint32_t> head(ctx, (size_t)nseg + 1);
+        DevBuf<uint64_t> ukey(ctx, (size_t)nseg);
+        hipLaunchKernelGGL(seg_heads_kernel, dim3(grid_for(npairs, 256)), dim3(256), 0, ctx->stream, kp,
+                           pos.get(), npairs, nseg, head.get(), ukey.get());
+        SFEM_LAUNCH_CHECK();
+        K->browidx.reset(ctx, nseg);
+        K->bcolidx.reset(ctx, nseg);
+        K->bval.reset(ctx, (size_t)nseg * bs * bs);
+        const dim3 g(grid_for(nseg, 256));
+        switch (bs) {
+            case 1:
+                hipLaunchKernelGGL(seg_sum_kernel<1>, g, dim3(256), 0, ctx->stream, V.get(), vp, head.get(),
+                                   ukey.get(), (int64_t)nseg, ndof, row_bits, K->bval.get(), K->browidx.get(),
+                                   K->bcolidx.get());
+                break;
+            case 2:
+                hipLaunchKernelGGL(seg_sum_kernel<2>, g, dim3(256), 0, ctx->stream, V.get(), vp, head.get(),
+                                   ukey.get(), (int64_t)nseg, ndof, row_bits, K->bval.get(), K->browidx.get(),
+                                   K->bcolidx.get());
+                break;
+            default:
+                hipLaunchKernelGGL(seg_sum_kernel<3>, g, dim3(256), 0, ctx->stream, V.get(), vp, head.get(),
+                                   ukey.get(), (int64_t)nseg, ndof, row_bits, K->bval.get(), K->browidx.get(),
+                                   K->bcolidx.get());
+                break;
+        }
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(colptr_kernel, g, dim3(256), 0, ctx->stream, K->bcolidx.get(), (int64_t)nseg,
+                           K->nunits, K->bcolptr.get());
+        SFEM_LAUNCH_CHECK();
+    }
+    tm.mark();
+    sync(ctx);
+    K->t_validate = tm.ms(0, 1);
+    K->t_element = tm.ms(1, 2);
+    K->t_keys = tm.ms(2, 3);
+    K->t_sort = tm.ms(3, 4);
+    K->t_reduce = tm.ms(4, 5);
+    K->t_total = tm.ms(0, 5);
+}
+
+void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval) {
+    const int64_t n = K->n, nnz = K->nnz;
+    DevBuf<int64_t> dcp(ctx, (size_t)n + 1);
+    hipLaunchKernelGGL(export_colptr_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream,
+                       K->bcolptr.get(), K->bs, n, nnz, dcp.get());
+    SFEM_LAUNCH_CHECK();
+    if (nnz) {
+        DevBuf<int64_t> drv(ctx, (size_t)nnz);
+        DevBuf<double> dnz(ctx, (size_t)nnz);
+        hipLaunchKernelGGL(export_entries_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, ctx->stream,
+                           K->bcolptr.get(), K->browidx.get(), K->bcolidx.get(), K->bval.get(), K->bs, nnz,
+                           drv.get(), dnz.get());
+        SFEM_LAUNCH_CHECK();
+        if (rowval) d2h(ctx, rowval, drv.get(), sizeof(int64_t) * nnz);
+        if (nzval) d2h(ctx, nzval, dnz.get(), sizeof(double) * nnz);
+    }
+    if (colptr) d2h(ctx, colptr, dcp.get(), sizeof(int64_t) * (n + 1));
+}
+
+}  // namespace sfem

@@ -1,0 +1,371 @@
+// eliminate.hip -- Dirichlet elimination K_ff = C' K C, rhs = C'(f - K q_d)
+// and recombination q = q_d + C q_f on the device, plus the solver formats.
+//
+// Reference: smearFEM examples/vector3D.jl:151,170 (C = I[:, free]), :315-316
+// (K_free = C' K_bar C, which picks the submatrix K[free, free] exactly),
+// :320 (rhs = C'(-K_bar q_d); a load vector f adds in), :322 (q = q_d + C q_f).
+//
+// K arrives as a block CSC (column unit c, blocks (r, bs x bs)).  Because K is
+// bitwise symmetric, column c of K is row c, so every format below is built
+// by walking columns and treating them as rows:
+//   * scalar CSR of K_ff, int32 columns (always);
+//   * when bs > 1 and every node is fully fixed or fully free: a node-block
+//     sliced format ("SELL"): 64 block rows per slice, one lane per block row,
+//     slot k of all 64 rows stored contiguously, values as bs*bs planes of 64.
+#include "sfem_objects.hpp"
+
+namespace sfem {
+
+namespace {
+
+__global__ void free_flags_kernel(const uint8_t* __restrict__ fixed, int64_t n, uint32_t* __restrict__ flag) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flag[i] = fixed[i] ? 0u : 1u;
+}
+
+__global__ void free_map_kernel(const uint8_t* __restrict__ fixed, const uint32_t* __restrict__ pos, int64_t n,
+                                int32_t* __restrict__ free_of) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) free_of[i] = fixed[i] ? -1 : (int32_t)pos[i];
+}
+
+// per unit: 1 if all bs dofs free; clears *complete if a unit is partly fixed
+__global__ void unit_status_kernel(const uint8_t* __restrict__ fixed, int64_t nunits, int bs,
+                                   uint32_t* __restrict__ unit_free, int* complete) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    int c = 0;
+    for (int i = 0; i < bs; ++i) c += fixed[u * bs + i] ? 1 : 0;
+    unit_free[u] = c == 0 ? 1u : 0u;
+    if (c != 0 && c != bs) atomicAnd(complete, 0);
+}
+
+__global__ void unit_fixed_kernel(const uint32_t* __restrict__ ufree, uint8_t* __restrict__ ufixed, int64_t m) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < m) ufixed[i] = ufree[i] ? 0 : 1;
+}
+
+__global__ void csr_count_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                 const uint8_t* __restrict__ fixed, const int32_t* __restrict__ free_of,
+                                 int64_t nunits, int bs, uint32_t* __restrict__ rowcnt) {
+    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cu >= nunits) return;
+    uint32_t cnt = 0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t r = browidx[u];
+        for (int i = 0; i < bs; ++i) cnt += fixed[r * bs + i] ? 0u : 1u;
+    }
+    for (int j = 0; j < bs; ++j) {
+        const int32_t row = free_of[cu * bs + j];
+        if (row >= 0) rowcnt[row] = cnt;
+    }
+}
+
+// One thread per dof of K (a column of K = a row of K_ff when free).
+__global__ void csr_fill_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                const double* __restrict__ bval, const int32_t* __restrict__ free_of,
+                                const double* __restrict__ qd, const double* __restrict__ f, int64_t n, int bs,
+                                const uint32_t* __restrict__ rowptr, int32_t* __restrict__ col,
+                                double* __restrict__ val, double* __restrict__ rhs, double* __restrict__ dinv) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t row = free_of[s];
+    if (row < 0) return;
+    const int64_t cu = s / bs;
+    const int j = (int)(s - cu * bs);
+    const int bs2 = bs * bs;
+    uint32_t pos = rowptr[row];
+    double lift = 0.0, diag = 0.0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t r = browidx[u];
+        for (int i = 0; i < bs; ++i) {
+            const int64_t d = r * bs + i;
+            const double v = bval[u * bs2 + i * bs + j];
+            const int32_t c = free_of[d];
+            if (c >= 0) {
+                col[pos] = c;
+                val[pos] = v;
+                ++pos;
+                if (d == s) diag = v;
+            } else {
+                lift += v * qd[d];  // (K q_d)[s], columns ascending as the CSC matvec sums
+            }
+        }
+    }
+    rhs[row] = f[s] - lift;
+    dinv[row] = diag != 0.0 ? 1.0 / diag : 1.0;
+}
+
+__global__ void blk_count_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                 const int32_t* __restrict__ ufree_of, int64_t nunits, int32_t* __restrict__ blen) {
+    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cu >= nunits) return;
+    const int32_t row = ufree_of[cu];
+    if (row < 0) return;
+    int cnt = 0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) cnt += ufree_of[browidx[u]] >= 0 ? 1 : 0;
+    blen[row] = cnt;
+}
+
+__global__ void slice_width_kernel(const int32_t* __restrict__ blen, int64_t nfb, int64_t nslices,
+                                   int32_t* __restrict__ sl_w, int64_t* __restrict__ sl_cnt) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nslices) return;
+    int w = 0;
+    for (int64_t r = s * 64; r < min(nfb, s * 64 + 64); ++r) w = max(w, blen[r]);
+    sl_w[s] = w;
+    sl_cnt[s] = (int64_t)w * 64;
+}
+
+template <int BS>
+__global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                 const double* __restrict__ bval, const int32_t* __restrict__ ufree_of,
+                                 int64_t nunits, const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
+                                 int32_t* __restrict__ scol, double* __restrict__ sval64,
+                                 float* __restrict__ sval32) {
+    constexpr int B2 = BS * BS;
+    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cu >= nunits) return;
+    const int32_t row = ufree_of[cu];
+    if (row < 0) return;
+    const int64_t s = row / 64;
+    const int lane = row % 64;
+    const int64_t base = sl_ptr[s];
+    int k = 0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int32_t c = ufree_of[browidx[u]];
+        if (c < 0) continue;
+        const int64_t slot = base + (int64_t)k * 64;
+        scol[slot + lane] = c;
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                // plane j*BS+i holds M[j][i] = K[(row,j),(c,i)] = K[(c,i),(row,j)]
+                const double v = bval[u * B2 + i * BS + j];
+                sval64[slot * B2 + (j * BS + i) * 64 + lane] = v;
+                sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
+            }
+        ++k;
+    }
+    for (; k < sl_w[s]; ++k) {  // padding slots: own column, zero values
+        const int64_t slot = base + (int64_t)k * 64;
+        scol[slot + lane] = row;
+#pragma unroll
+        for (int v = 0; v < B2; ++v) {
+            sval64[slot * B2 + v * 64 + lane] = 0.0;
+            sval32[slot * B2 + v * 64 + lane] = 0.0f;
+        }
+    }
+}
+
+// Rows of a partially filled last slice (row >= nfb) still get full padding
+// so the SpMV can run every lane without bounds checks on the values.
+template <int BS>
+__global__ void sell_pad_tail_kernel(int64_t nfb, int64_t nslices, const int64_t* __restrict__ sl_ptr,
+                                     const int32_t* __restrict__ sl_w, int32_t* __restrict__ scol,
+                                     double* __restrict__ sval64, float* __restrict__ sval32) {
+    constexpr int B2 = BS * BS;
+    const int lane = threadIdx.x;
+    const int64_t s = nslices - 1;
+    const int64_t row = s * 64 + lane;
+    if (lane >= 64 || row < nfb) return;
+    for (int k = 0; k < sl_w[s]; ++k) {
+        const int64_t slot = sl_ptr[s] + (int64_t)k * 64;
+        scol[slot + lane] = (int32_t)(nfb - 1);
+        for (int v = 0; v < B2; ++v) {
+            sval64[slot * B2 + v * 64 + lane] = 0.0;
+            sval32[slot * B2 + v * 64 + lane] = 0.0f;
+        }
+    }
+}
+
+__global__ void to_f32_kernel(const double* __restrict__ a, float* __restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
+__global__ void recombine_kernel(const int32_t* __restrict__ free_of, const double* __restrict__ x,
+                                 const double* __restrict__ qd, int64_t n, double* __restrict__ q) {
+    const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (d < n) q[d] = free_of[d] >= 0 ? x[free_of[d]] : qd[d];
+}
+
+// R = K q - f on the fixed dofs (zero on free dofs).
+__global__ void reactions_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                 const double* __restrict__ bval, const int32_t* __restrict__ free_of,
+                                 const double* __restrict__ q, const double* __restrict__ f, int64_t n, int bs,
+                                 double* __restrict__ R) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    if (free_of[s] >= 0) { R[s] = 0.0; return; }
+    const int64_t cu = s / bs;
+    const int j = (int)(s - cu * bs);
+    double acc = 0.0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t r = browidx[u];
+        for (int i = 0; i < bs; ++i) acc += bval[u * bs * bs + i * bs + j] * q[r * bs + i];
+    }
+    R[s] = acc - f[s];
+}
+
+}  // namespace
+
+void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S) {
+    if (bc->n != K->n || K->m != K->n) fail(SFEM_ERR_DIMENSION, "eliminate: BC length differs from size(K)");
+    hipEvent_t e0, e1;
+    SFEM_HIP(hipEventCreate(&e0));
+    SFEM_HIP(hipEventCreate(&e1));
+    SFEM_HIP(hipEventRecord(e0, ctx->stream));
+    const int64_t n = K->n;
+    const int bs = K->bs;
+    S->ctx = ctx;
+    S->K = K;
+    S->n = n;
+    S->bs = bs;
+    S->free_of.reset(ctx, (size_t)n);
+    S->qd.reset(ctx, (size_t)n);
+    S->f.reset(ctx, (size_t)n);
+    SFEM_HIP(hipMemcpyAsync(S->qd.get(), bc->qd.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    SFEM_HIP(hipMemcpyAsync(S->f.get(), bc->f.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+
+    DevBuf<uint32_t> pos(ctx, (size_t)n), tot(ctx, 1);
+    const dim3 gn(grid_for(n, 256));
+    hipLaunchKernelGGL(free_flags_kernel, gn, dim3(256), 0, ctx->stream, bc->fixed.get(), n, pos.get());
+    SFEM_LAUNCH_CHECK();
+    exclusive_scan_u32_dev(ctx, pos.get(), pos.get(), n, tot.get());
+    hipLaunchKernelGGL(free_map_kernel, gn, dim3(256), 0, ctx->stream, bc->fixed.get(), pos.get(), n,
+                       S->free_of.get());
+    SFEM_LAUNCH_CHECK();
+    uint32_t nf = 0;
+    d2h(ctx, &nf, tot.get(), sizeof(nf));
+    S->nf = nf;
+
+    // scalar CSR
+    DevBuf<uint32_t> rowptr(ctx, (size_t)nf + 1);
+    memset_d(ctx, rowptr.get(), 0, sizeof(uint32_t) * (nf + 1));
+    const int64_t nunits = K->nunits;
+    const dim3 gu(grid_for(nunits, 256));
+    hipLaunchKernelGGL(csr_count_kernel, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                       bc->fixed.get(), S->free_of.get(), nunits, bs, rowptr.get());
+    SFEM_LAUNCH_CHECK();
+    {
+        DevBuf<uint32_t> t2(ctx, 1);
+        exclusive_scan_u32_dev(ctx, rowptr.get(), rowptr.get(), (int64_t)nf + 1, t2.get());
+    }
+    uint32_t nnz = 0;
+    if (nf) d2h(ctx, &nnz, rowptr.get() + nf, sizeof(nnz));
+    if ((uint64_t)nnz >= (1ull << 31)) fail(SFEM_ERR_UNSUPPORTED, "K_ff has 2^31 or more entries");
+    S->nnz = nnz;
+    S->rowptr.reset(ctx, (size_t)nf + 1);
+    SFEM_HIP(hipMemcpyAsync(S->rowptr.get(), rowptr.get(), sizeof(int32_t) * (nf + 1), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    S->col.reset(ctx, nnz);
+    S->val64.reset(ctx, nnz);
+    S->rhs.reset(ctx, nf);
+    S->x.reset(ctx, nf);
+    S->dinv64.reset(ctx, nf);
+    S->dinv32.reset(ctx, nf);
+    memset_d(ctx, S->x.get(), 0, sizeof(double) * nf);
+    hipLaunchKernelGGL(csr_fill_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                       K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, bs, rowptr.get(), S->col.get(),
+                       S->val64.get(), S->rhs.get(), S->dinv64.get());
+    SFEM_LAUNCH_CHECK();
+    if (nf) {
+        hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(nf, 256)), dim3(256), 0, ctx->stream, S->dinv64.get(),
+                           S->dinv32.get(), (int64_t)nf);
+        SFEM_LAUNCH_CHECK();
+    }
+
+    // node-block sliced format
+    S->has_block = false;
+    if (bs > 1 && nf > 0) {
+        DevBuf<uint32_t> ufree(ctx, (size_t)nunits), upos(ctx, (size_t)nunits), utot(ctx, 1);
+        DevBuf<int> complete(ctx, 1);
+        const int one = 1;
+        h2d(ctx, complete.get(), &one, sizeof(int));
+        hipLaunchKernelGGL(unit_status_kernel, gu, dim3(256), 0, ctx->stream, bc->fixed.get(), nunits, bs,
+                           ufree.get(), complete.get());
+        SFEM_LAUNCH_CHECK();
+        int comp = 0;
+        d2h(ctx, &comp, complete.get(), sizeof(int));
+        if (comp) {
+            exclusive_scan_u32_dev(ctx, ufree.get(), upos.get(), nunits, utot.get());
+            uint32_t nfb = 0;
+            d2h(ctx, &nfb, utot.get(), sizeof(nfb));
+            DevBuf<int32_t> ufree_of(ctx, (size_t)nunits);
+            // unit map: fixed unit <=> ufree == 0
+            DevBuf<uint8_t> ufixed(ctx, (size_t)nunits);
+            hipLaunchKernelGGL(unit_fixed_kernel, gu, dim3(256), 0, ctx->stream, ufree.get(), ufixed.get(), nunits);
+            SFEM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(free_map_kernel, gu, dim3(256), 0, ctx->stream, ufixed.get(), upos.get(), nunits,
+                               ufree_of.get());
+            SFEM_LAUNCH_CHECK();
+            DevBuf<int32_t> blen(ctx, nfb);
+            hipLaunchKernelGGL(blk_count_kernel, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                               ufree_of.get(), nunits, blen.get());
+            SFEM_LAUNCH_CHECK();
+            const int64_t nslices = (nfb + 63) / 64;
+            S->nfb = nfb;
+            S->nslices = nslices;
+            S->nnzb = nnz / (bs * bs);
+            S->sl_ptr.reset(ctx, (size_t)nslices + 1);
+            S->sl_w.reset(ctx, (size_t)nslices);
+            DevBuf<int64_t> cnt(ctx, (size_t)nslices + 1);
+            memset_d(ctx, cnt.get(), 0, sizeof(int64_t) * (nslices + 1));
+            hipLaunchKernelGGL(slice_width_kernel, dim3(grid_for(nslices, 256)), dim3(256), 0, ctx->stream,
+                               blen.get(), (int64_t)nfb, nslices, S->sl_w.get(), cnt.get());
+            SFEM_LAUNCH_CHECK();
+            const int64_t total = exclusive_scan_i64(ctx, cnt.get(), S->sl_ptr.get(), nslices + 1);
+            if (total >= (int64_t(1) << 31)) fail(SFEM_ERR_UNSUPPORTED, "block format too large");
+            S->sell_blocks = total;
+            S->scol.reset(ctx, (size_t)total);
+            S->sval64.reset(ctx, (size_t)total * bs * bs);
+            S->sval32.reset(ctx, (size_t)total * bs * bs);
+            if (bs == 3) {
+                hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                                   K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                SFEM_LAUNCH_CHECK();
+                hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
+                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+            } else {
+                hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                                   K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                SFEM_LAUNCH_CHECK();
+                hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
+                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+            }
+            SFEM_LAUNCH_CHECK();
+            S->has_block = true;
+        }
+    }
+    SFEM_HIP(hipEventRecord(e1, ctx->stream));
+    sync(ctx);
+    float ms = 0;
+    SFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    S->t_eliminate_ms = ms;
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+}
+
+void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions) {
+    const int64_t n = S->n;
+    DevBuf<double> dq(ctx, (size_t)n);
+    const dim3 gn(grid_for(n, 256));
+    hipLaunchKernelGGL(recombine_kernel, gn, dim3(256), 0, ctx->stream, S->free_of.get(), S->x.get(), S->qd.get(), n,
+                       dq.get());
+    SFEM_LAUNCH_CHECK();
+    if (reactions) {
+        DevBuf<double> dr(ctx, (size_t)n);
+        const sfem_csr* K = S->K;
+        hipLaunchKernelGGL(reactions_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                           K->bval.get(), S->free_of.get(), dq.get(), S->f.get(), n, S->bs, dr.get());
+        SFEM_LAUNCH_CHECK();
+        d2h(ctx, reactions, dr.get(), sizeof(double) * n);
+    }
+    if (q) d2h(ctx, q, dq.get(), sizeof(double) * n);
+}
+
+}  // namespace sfem

@@ -1,0 +1,131 @@
+// sfem_objects.hpp -- device-resident objects behind the opaque C-ABI handles,
+// and the host launchers each translation unit provides.
+#pragma once
+
+#include "sfem_internal.hpp"
+
+// Mesh and material inputs, uploaded once.
+struct sfem_mesh {
+    sfem_ctx* ctx = nullptr;
+    int ndim = 0, cls = 0, ndof = 0, nen = 0;
+    int64_t n_nodes = 0, n_elem = 0, ien_rows = 0, id_rows = 0, n_young = 0;
+    double nu = 0.3;
+    sfem::DevBuf<double> coords;   // ndim x n_nodes, node-major (x[d + ndim*node])
+    sfem::DevBuf<int32_t> ien;     // n_elem x nen element-major, 0-based; -1 = outside NodeList
+    sfem::DevBuf<int32_t> dof;     // id_rows x ndof node-major, 0-based; -1 = id < 1, -2 = id > 2^31
+    sfem::DevBuf<double> young;    // n_young
+    // Error the reference raises before any element work can be checked on the
+    // device (structural mismatches found at element 1, or IEN too short):
+    // (code, key) with key = element*8 + step, or code 0.
+    int pre_code = 0;
+    int64_t pre_key = INT64_MAX;
+};
+
+// Assembled K, kept as a block CSC: column-unit c holds blocks (row-unit r,
+// bs x bs values with val[i*bs + j] = K[r*bs+i, c*bs+j]).  bs = ndof when ID is
+// node-blocked (ID[m,l] = ndof*g(m) + l), else 1.
+struct sfem_csr {
+    sfem_ctx* ctx = nullptr;
+    int64_t m = 0, n = 0, nnz = 0;
+    int bs = 1;
+    int64_t nunits = 0, nnzb = 0;
+    sfem::DevBuf<int64_t> bcolptr;  // nunits + 1
+    sfem::DevBuf<int32_t> browidx;  // nnzb
+    sfem::DevBuf<int32_t> bcolidx;  // nnzb (column unit of each block)
+    sfem::DevBuf<double> bval;      // nnzb * bs * bs
+    // device time per assembly stage (ms)
+    double t_validate = 0, t_element = 0, t_keys = 0, t_sort = 0, t_reduce = 0, t_total = 0;
+};
+
+struct sfem_bc {
+    sfem_ctx* ctx = nullptr;
+    int64_t n = 0;
+    sfem::DevBuf<double> f;        // n
+    sfem::DevBuf<uint8_t> fixed;   // n
+    sfem::DevBuf<double> qd;       // n (0 on free dofs)
+};
+
+namespace sfem {
+
+// ---- element.hip ----
+struct ValidateOut {
+    unsigned long long err_key;   // min over elements of element*8 + step
+    int max_unit;                 // max 0-based unit (node / dof / node-block) referenced
+    int min_dof;                  // min stored dof code over used nodes (-1: id<1, -2: too large)
+    int blocked;                  // 1 while every used node's ID is ndof*g + l
+    int pad;
+};
+void launch_validate(sfem_ctx* ctx, const sfem_mesh* m, ValidateOut* d_out);
+void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out);
+// Emit (key, payload) for every unit pair; returns pair count.
+int64_t launch_keys(sfem_ctx* ctx, const sfem_mesh* m, int bs, int row_bits, uint64_t* keys,
+                    uint32_t* payload);
+
+// ---- radix_sort.hip ----
+// Stable LSD sort of (key, value) pairs on the low `key_bits` bits.  Buffers
+// are ping-ponged; on return *keys / *vals point at the sorted data (either
+// the input or the alternate buffer).
+void radix_sort_pairs(sfem_ctx* ctx, uint64_t** keys, uint64_t* keys_alt, uint32_t** vals,
+                      uint32_t* vals_alt, int64_t n, int key_bits);
+// Exclusive scan of n uint32 values (in may equal out); returns the total.
+uint64_t exclusive_scan_u32(sfem_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n);
+// Same without a host sync; total written to d_total if non-null.
+void exclusive_scan_u32_dev(sfem_ctx* ctx, const uint32_t* in, uint32_t* out, int64_t n,
+                            uint32_t* d_total);
+// Exclusive scan into 64-bit offsets; returns the total.
+int64_t exclusive_scan_i64(sfem_ctx* ctx, const int64_t* in, int64_t* out, int64_t n);
+
+// ---- assemble.hip ----
+void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K);
+void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval);
+
+}  // namespace sfem
+
+// Device scalars of one CG solve; updated only by the last block of each
+// reduction, read by every kernel of the next step.
+struct CgState {
+    double rz, rz_prev, beta, alpha, rr, bb, maxrr, tol2bb, delta2, rr_true;
+    int done, need_update, pad0, pad1;
+    long long iters, updates;
+    unsigned int counter;
+    unsigned int pad2;
+};
+
+// K_ff and its solver formats, plus the data to recombine the solution.
+struct sfem_system {
+    sfem_ctx* ctx = nullptr;
+    const sfem_csr* K = nullptr;   // must outlive the system (reactions)
+    int64_t n = 0, nf = 0;         // all dofs, free dofs
+    int bs = 1;                    // block size of K
+    sfem::DevBuf<int32_t> free_of; // n: free index or -1
+    sfem::DevBuf<double> qd, f;    // n
+    sfem::DevBuf<double> rhs;      // nf: C'(f - K q_d)
+    sfem::DevBuf<double> x;        // nf: FP64 solution
+    // scalar CSR of K_ff (always built)
+    int64_t nnz = 0;
+    sfem::DevBuf<int32_t> rowptr, col;
+    sfem::DevBuf<double> val64;
+    sfem::DevBuf<float> val32;     // built on first mixed CSR use
+    sfem::DevBuf<double> dinv64;
+    sfem::DevBuf<float> dinv32;
+    // node-block sliced format (when bs > 1 and every node is all-fixed or all-free)
+    bool has_block = false;
+    int64_t nfb = 0, nslices = 0, nnzb = 0, sell_blocks = 0;
+    sfem::DevBuf<int64_t> sl_ptr;  // nslices + 1, in blocks (64 per slot)
+    sfem::DevBuf<int32_t> sl_w;    // nslices: slots per slice
+    sfem::DevBuf<int32_t> scol;    // sell_blocks
+    sfem::DevBuf<double> sval64;   // sell_blocks * bs*bs, planes of 64
+    sfem::DevBuf<float> sval32;
+    double t_eliminate_ms = 0;
+};
+
+namespace sfem {
+// ---- eliminate.hip ----
+void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S);
+void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions);
+// ---- pcg.hip ----
+void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
+void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
+                       double* bytes);
+void stream_copy_bench(sfem_ctx* ctx, int64_t bytes, int reps, double* avg_ms);
+}  // namespace sfem

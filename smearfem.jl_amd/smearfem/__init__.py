@@ -1,0 +1,19 @@
+"""smearfem -- MI355X (gfx950) assemble-and-solve for smearFEM.jl's hot path.
+
+Python mirror of the reference's entry points over the C ABI in
+include/smearfem.h (library: ``libsmearfem_hip.so`` in this directory).
+"""
+from .errors import (ArgumentError, BoundsError, DeviceError, DeviceMemoryError, DimensionMismatch,
+                     NotConvergedError, SingularException, SmearFEMError, StateError, UndefVarError,
+                     UnsupportedError)
+from .fem import (Context, Mesh, StiffnessMatrix, assemble_system, basis_function, default_context,
+                  gaussian_quadrature)
+from .mesh import StructuredMesh, clamp_x0, face_load, random_face_load, structured_mesh
+
+__all__ = [
+    "ArgumentError", "BoundsError", "DeviceError", "DeviceMemoryError", "DimensionMismatch",
+    "NotConvergedError", "SingularException", "SmearFEMError", "StateError", "UndefVarError",
+    "UnsupportedError", "Context", "Mesh", "StiffnessMatrix", "assemble_system", "basis_function",
+    "default_context", "gaussian_quadrature", "StructuredMesh", "clamp_x0", "face_load",
+    "random_face_load", "structured_mesh",
+]

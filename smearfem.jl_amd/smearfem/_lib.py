@@ -1,0 +1,114 @@
+"""ctypes binding of the gfx950 C ABI (include/smearfem.h).
+
+The shared library is built in-tree by ``smearfem.jl_amd/build.py``.  There is
+no CPU fallback: if the library is missing or no GPU is visible, the calls
+that need it raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+_LIB_PATH = Path(__file__).resolve().parent / "libsmearfem_hip.so"
+
+c_i64p = C.POINTER(C.c_int64)
+c_dblp = C.POINTER(C.c_double)
+c_u8p = C.POINTER(C.c_uint8)
+c_intp = C.POINTER(C.c_int)
+c_vp = C.c_void_p
+
+
+class SolveOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("format", C.c_int),
+        ("rtol", C.c_double),
+        ("max_iter", C.c_int64),
+        ("reliable_delta", C.c_double),
+        ("check_every", C.c_int),
+        ("time_kernels", C.c_int),
+        ("x0_zero", C.c_int),
+    ]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("fp64_updates", C.c_int64),
+        ("spmv_launches", C.c_int64),
+        ("spmv64_launches", C.c_int64),
+        ("rel_residual", C.c_double),
+        ("solve_ms", C.c_double),
+        ("spmv_ms", C.c_double),
+        ("spmv_bytes", C.c_double),
+        ("spmv_timed", C.c_int64),
+        ("converged", C.c_int),
+        ("format", C.c_int),
+    ]
+
+
+# (name, restype, argtypes)
+_PROTOS = [
+    ("sfem_ctx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
+    ("sfem_ctx_destroy", None, [c_vp]),
+    ("sfem_last_error", C.c_char_p, [c_vp]),
+    ("sfem_device_info", C.c_int, [c_vp, C.c_char_p, C.c_size_t, c_intp, c_i64p]),
+    ("sfem_ctx_device_bytes", C.c_int64, [c_vp]),
+    ("sfem_ctx_synchronize", C.c_int, [c_vp]),
+    ("sfem_gaussian_quadrature", C.c_int, [C.c_double, C.c_double, C.c_int, c_dblp, c_dblp]),
+    ("sfem_basis_function", C.c_int,
+     [C.c_int, C.c_double, C.c_double, C.c_double, C.c_int, c_dblp, c_dblp, c_intp, c_intp, c_intp]),
+    ("sfem_mesh_create", C.c_int,
+     [c_vp, C.c_int, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, C.c_int, C.c_int64,
+      c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
+    ("sfem_mesh_destroy", None, [c_vp]),
+    ("sfem_assemble_mesh", C.c_int, [c_vp, c_vp, C.c_int, C.POINTER(c_vp)]),
+    ("sfem_assemble", C.c_int,
+     [c_vp, C.c_int, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, C.c_int, C.c_int64,
+      c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
+    ("sfem_csr_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_intp]),
+    ("sfem_csr_stage_ms", C.c_int, [c_vp, c_dblp]),
+    ("sfem_export_csc1", C.c_int, [c_vp, c_vp, c_i64p, c_i64p, c_dblp]),
+    ("sfem_csr_destroy", None, [c_vp]),
+    ("sfem_bc_create", C.c_int, [c_vp, C.c_int64, c_dblp, c_u8p, c_dblp, C.POINTER(c_vp)]),
+    ("sfem_bc_destroy", None, [c_vp]),
+    ("sfem_eliminate", C.c_int, [c_vp, c_vp, c_vp, C.POINTER(c_vp)]),
+    ("sfem_system_info", C.c_int, [c_vp, c_i64p, c_i64p, c_intp]),
+    ("sfem_system_destroy", None, [c_vp]),
+    ("sfem_solve_opts_default", None, [C.POINTER(SolveOpts)]),
+    ("sfem_system_solve", C.c_int, [c_vp, c_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("sfem_system_solution", C.c_int, [c_vp, c_vp, c_dblp, c_dblp]),
+    ("sfem_solve", C.c_int,
+     [c_vp, c_vp, c_dblp, c_u8p, c_dblp, C.POINTER(SolveOpts), c_dblp, c_dblp, C.POINTER(SolveStats)]),
+    ("sfem_system_spmv_bench", C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_dblp, c_dblp]),
+    ("sfem_stream_copy_bench", C.c_int, [c_vp, C.c_int64, C.c_int, c_dblp]),
+]
+
+EXPORTED = [p[0] for p in _PROTOS]
+
+_lib = None
+
+
+def lib_path() -> Path:
+    return _LIB_PATH
+
+
+def load() -> C.CDLL:
+    """Load the library (once).  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not _LIB_PATH.exists():
+        raise ImportError(
+            f"{_LIB_PATH} is missing: build it with `python smearfem.jl_amd/build.py` "
+            "(there is no CPU fallback)")
+    lib = C.CDLL(os.fspath(_LIB_PATH))
+    for name, res, args in _PROTOS:
+        fn = getattr(lib, name, None)
+        if fn is None:  # reported by the exports test; calling it raises AttributeError
+            continue
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
