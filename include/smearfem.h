@@ -123,6 +123,11 @@ int  sfem_csr_info(const sfem_csr* K, int64_t* m, int64_t* n, int64_t* nnz, int*
 int  sfem_export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval,
                       double* nzval);
 void sfem_csr_destroy(sfem_csr* K);
+/* Upload a host SparseMatrixCSC (1-based, e.g. K_bar = K + beta*b built on
+ * the host as examples/vector3D.jl:308 does) for sfem_bc_create/eliminate/
+ * solve.  Stored as scalar rows (block 1); export is refused for it. */
+int  sfem_csr_from_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr,
+                        const int64_t* rowval, const double* nzval, sfem_csr** out);
 /* Device time of the last assembly's stages (ms): validate, element Ke,
  * keys, sort, segmented reduction, total.  ms: 6 doubles. */
 int  sfem_csr_stage_ms(const sfem_csr* K, double* ms);
@@ -140,11 +145,14 @@ void sfem_bc_destroy(sfem_bc* bc);
  * every node is either fully fixed or fully free, FP64/FP32 block formats. */
 int  sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out);
 int  sfem_system_info(const sfem_system* sys, int64_t* n_free, int64_t* nnz_ff, int* block);
+/* Device time of the elimination that built the system (ms). */
+int  sfem_system_eliminate_ms(const sfem_system* sys, double* ms);
 void sfem_system_destroy(sfem_system* sys);
 
 /* ---- solve ------------------------------------------------------------ */
 #define SFEM_PREC_FP64   0   /* plain FP64 Jacobi-PCG on FP64 CSR                       */
-#define SFEM_PREC_MIXED  1   /* FP32 operator/CG with FP64 residual refinement          */
+#define SFEM_PREC_MIXED  1   /* FP32 operator and CG vectors, FP64 residual replacement  */
+#define SFEM_PREC_FP32_MATRIX 2 /* FP32 operator values, FP64 CG vectors and residuals    */
 
 #define SFEM_FMT_AUTO    0
 #define SFEM_FMT_CSR     1   /* scalar CSR, int32 columns                               */
@@ -173,6 +181,7 @@ typedef struct {
     int64_t spmv_timed;      /* number of SpMV launches that were timed                  */
     int     converged;
     int     format;          /* format actually used                                     */
+    int     stop_reason;     /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap  */
 } sfem_solve_stats;
 
 void sfem_solve_opts_default(sfem_solve_opts* opts);

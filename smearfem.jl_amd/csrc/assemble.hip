@@ -284,7 +284,91 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     K->t_total = tm.ms(0, 5);
 }
 
+namespace {
+__global__ void csc_keys_kernel(const int64_t* __restrict__ colptr, const int64_t* __restrict__ rowval, int64_t n,
+                                int64_t m, int row_bits, uint64_t* __restrict__ keys, uint32_t* __restrict__ payload,
+                                int* __restrict__ bad) {
+    const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= n) return;
+    for (int64_t k = colptr[j] - 1; k < colptr[j + 1] - 1; ++k) {
+        const int64_t r = rowval[k] - 1;
+        if (r < 0 || r >= m) { atomicOr(bad, 1); continue; }
+        // transposed key: (row of K) is the column unit of the stored CSC of K'
+        keys[k] = ((uint64_t)r << row_bits) | (uint64_t)j;
+        payload[k] = (uint32_t)k;
+    }
+}
+}  // namespace
+
+// Stores a host CSC of K as the block CSC (bs = 1) of K' -- i.e. the CSR of K
+// -- so the solver's "columns as rows" products apply K itself even when K is
+// not exactly symmetric.
+void upload_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                 const double* nzval, sfem_csr* K) {
+    if (m != n) fail(SFEM_ERR_DIMENSION, "solve needs a square matrix");
+    const int64_t nnz = colptr[n] - 1;
+    if (nnz < 0 || nnz >= (int64_t(1) << 31)) fail(SFEM_ERR_UNSUPPORTED, "matrix has 2^31 or more entries");
+    K->ctx = ctx;
+    K->bs = 1;
+    K->m = m;
+    K->n = n;
+    K->nunits = n;
+    K->transposed = true;
+    K->bcolptr.reset(ctx, (size_t)n + 1);
+    if (nnz == 0) {
+        K->nnz = K->nnzb = 0;
+        hipLaunchKernelGGL(fill_i64_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                           n + 1, (int64_t)0);
+        SFEM_LAUNCH_CHECK();
+        sync(ctx);
+        return;
+    }
+    DevBuf<int64_t> dcp(ctx, (size_t)n + 1), drv(ctx, (size_t)nnz);
+    DevBuf<double> dval(ctx, (size_t)nnz);
+    h2d(ctx, dcp.get(), colptr, sizeof(int64_t) * (n + 1));
+    h2d(ctx, drv.get(), rowval, sizeof(int64_t) * nnz);
+    h2d(ctx, dval.get(), nzval, sizeof(double) * nnz);
+    const int row_bits = bits_for(n);
+    DevBuf<uint64_t> keys(ctx, (size_t)nnz), keys_alt(ctx, (size_t)nnz);
+    DevBuf<uint32_t> pay(ctx, (size_t)nnz), pay_alt(ctx, (size_t)nnz);
+    DevBuf<int> bad(ctx, 1);
+    memset_d(ctx, bad.get(), 0, sizeof(int));
+    hipLaunchKernelGGL(csc_keys_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, dcp.get(), drv.get(), n, m,
+                       row_bits, keys.get(), pay.get(), bad.get());
+    SFEM_LAUNCH_CHECK();
+    int hbad = 0;
+    d2h(ctx, &hbad, bad.get(), sizeof(int));
+    if (hbad) fail(SFEM_ERR_ARGUMENT, "CSC row index out of range");
+    uint64_t* kp = keys.get();
+    uint32_t* vp = pay.get();
+    radix_sort_pairs(ctx, &kp, keys_alt.get(), &vp, pay_alt.get(), nnz, 2 * row_bits);
+    DevBuf<uint32_t> pos(ctx, (size_t)nnz), dtot(ctx, 1);
+    hipLaunchKernelGGL(seg_flags_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, ctx->stream, kp, nnz, pos.get());
+    SFEM_LAUNCH_CHECK();
+    exclusive_scan_u32_dev(ctx, pos.get(), pos.get(), nnz, dtot.get());
+    uint32_t nseg = 0;
+    d2h(ctx, &nseg, dtot.get(), sizeof(nseg));
+    K->nnzb = K->nnz = nseg;  // repeated (row, col) pairs are summed in stored order
+    DevBuf<uint32_t> head(ctx, (size_t)nseg + 1);
+    DevBuf<uint64_t> ukey(ctx, (size_t)nseg);
+    hipLaunchKernelGGL(seg_heads_kernel, dim3(grid_for(nnz, 256)), dim3(256), 0, ctx->stream, kp, pos.get(), nnz, nseg,
+                       head.get(), ukey.get());
+    SFEM_LAUNCH_CHECK();
+    K->browidx.reset(ctx, nseg);
+    K->bcolidx.reset(ctx, nseg);
+    K->bval.reset(ctx, nseg);
+    hipLaunchKernelGGL(seg_sum_kernel<1>, dim3(grid_for(nseg, 256)), dim3(256), 0, ctx->stream, dval.get(), vp,
+                       head.get(), ukey.get(), (int64_t)nseg, 1, row_bits, K->bval.get(), K->browidx.get(),
+                       K->bcolidx.get());
+    SFEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(colptr_kernel, dim3(grid_for(nseg, 256)), dim3(256), 0, ctx->stream, K->bcolidx.get(),
+                       (int64_t)nseg, n, K->bcolptr.get());
+    SFEM_LAUNCH_CHECK();
+    sync(ctx);
+}
+
 void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval) {
+    if (K->transposed) fail(SFEM_ERR_STATE, "export: matrix was uploaded from host CSC (the caller holds it)");
     const int64_t n = K->n, nnz = K->nnz;
     DevBuf<int64_t> dcp(ctx, (size_t)n + 1);
     hipLaunchKernelGGL(export_colptr_kernel, dim3(grid_for(n + 1, 256)), dim3(256), 0, ctx->stream,

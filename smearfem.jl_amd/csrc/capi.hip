@@ -4,6 +4,7 @@
 // message retrievable with sfem_last_error(); nothing prints or exits.
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -90,6 +91,19 @@ void memset_d(sfem_ctx* ctx, void* dst, int v, size_t bytes) {
     if (bytes) SFEM_HIP(hipMemsetAsync(dst, v, bytes, ctx->stream));
 }
 void sync(sfem_ctx* ctx) { SFEM_HIP(hipStreamSynchronize(ctx->stream)); }
+
+void ctx_retain(sfem_ctx* ctx) {
+    if (ctx) ctx->refs.fetch_add(1);
+}
+
+void ctx_release(sfem_ctx* ctx) {
+    if (!ctx || ctx->refs.fetch_sub(1) != 1) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    delete ctx->alloc;
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
 
 namespace {
 
@@ -189,20 +203,15 @@ int sfem_ctx_create(int device, sfem_ctx** out) {
         ctx->n_cu = prop.multiProcessorCount;
         SFEM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->alloc = new Allocator(device);
+        const char* nt = std::getenv("SFEM_SPMV_NT");
+        ctx->tune_nt = nt && nt[0] == '1';
     });
     if (st) { delete ctx; return st; }
     *out = ctx;
     return SFEM_OK;
 }
 
-void sfem_ctx_destroy(sfem_ctx* ctx) {
-    if (!ctx) return;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    delete ctx->alloc;
-    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
-    delete ctx;
-}
+void sfem_ctx_destroy(sfem_ctx* ctx) { ctx_release(ctx); }
 
 const char* sfem_last_error(const sfem_ctx* ctx) { return ctx ? ctx->last_error.c_str() : "no context"; }
 
@@ -362,11 +371,18 @@ int sfem_mesh_create(sfem_ctx* ctx, int ndim, int function_class, int n_dof, con
         sync(ctx);
     });
     if (st) { delete m; return st; }
+    m->ctx = ctx;
+    ctx_retain(ctx);
     *out = m;
     return SFEM_OK;
 }
 
-void sfem_mesh_destroy(sfem_mesh* m) { delete m; }
+void sfem_mesh_destroy(sfem_mesh* m) {
+    if (!m) return;
+    sfem_ctx* ctx = m->ctx;
+    delete m;
+    ctx_release(ctx);
+}
 
 int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr** out) {
     if (!ctx) return SFEM_ERR_ARGUMENT;
@@ -387,6 +403,8 @@ int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr** 
         assemble(ctx, m, flags, K);
     });
     if (st) { delete K; return st; }
+    K->ctx = ctx;
+    ctx_retain(ctx);
     *out = K;
     return SFEM_OK;
 }
@@ -432,7 +450,27 @@ int sfem_export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t*
     });
 }
 
-void sfem_csr_destroy(sfem_csr* K) { delete K; }
+void sfem_csr_destroy(sfem_csr* K) {
+    if (!K) return;
+    sfem_ctx* ctx = K->ctx;
+    delete K;
+    ctx_release(ctx);
+}
+
+int sfem_csr_from_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                       const double* nzval, sfem_csr** out) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!colptr || !out || m < 0 || n < 0) { ctx->last_error = "csr_from_csc1: bad arguments"; return SFEM_ERR_ARGUMENT; }
+    *out = nullptr;
+    sfem_csr* K = new (std::nothrow) sfem_csr();
+    if (!K) return SFEM_ERR_MEMORY;
+    int st = guard(ctx, [&] { upload_csc1(ctx, m, n, colptr, rowval, nzval, K); });
+    if (st) { delete K; return st; }
+    K->ctx = ctx;
+    ctx_retain(ctx);
+    *out = K;
+    return SFEM_OK;
+}
 
 // ---------------------------------------------------------------- solve --
 int sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fixed, const double* q_d,
@@ -464,11 +502,17 @@ int sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fix
         sync(ctx);
     });
     if (st) { delete bc; return st; }
+    ctx_retain(ctx);
     *out = bc;
     return SFEM_OK;
 }
 
-void sfem_bc_destroy(sfem_bc* bc) { delete bc; }
+void sfem_bc_destroy(sfem_bc* bc) {
+    if (!bc) return;
+    sfem_ctx* ctx = bc->ctx;
+    delete bc;
+    ctx_release(ctx);
+}
 
 int sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out) {
     if (!ctx) return SFEM_ERR_ARGUMENT;
@@ -478,6 +522,8 @@ int sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_sys
     if (!S) return SFEM_ERR_MEMORY;
     int st = guard(ctx, [&] { eliminate(ctx, K, bc, S); });
     if (st) { delete S; return st; }
+    S->ctx = ctx;
+    ctx_retain(ctx);
     *out = S;
     return SFEM_OK;
 }
@@ -490,7 +536,18 @@ int sfem_system_info(const sfem_system* S, int64_t* n_free, int64_t* nnz_ff, int
     return SFEM_OK;
 }
 
-void sfem_system_destroy(sfem_system* S) { delete S; }
+int sfem_system_eliminate_ms(const sfem_system* S, double* ms) {
+    if (!S || !ms) return SFEM_ERR_ARGUMENT;
+    *ms = S->t_eliminate_ms;
+    return SFEM_OK;
+}
+
+void sfem_system_destroy(sfem_system* S) {
+    if (!S) return;
+    sfem_ctx* ctx = S->ctx;
+    delete S;
+    ctx_release(ctx);
+}
 
 void sfem_solve_opts_default(sfem_solve_opts* o) {
     if (!o) return;

@@ -2,23 +2,29 @@
 //
 // Replaces the reference's dense inverse (smearFEM examples/vector3D.jl:318,
 // inv(Matrix(K_free)) * C' * (-K q_d)) with an iterative solve that keeps K_ff
-// sparse and resident in HBM.  Two precisions share one driver:
+// sparse and resident in HBM.  Three precisions share one driver:
 //
-//   FP64  : vectors, operator and reductions in FP64 on the scalar CSR.
-//   MIXED : FP32 operator (block-sliced or CSR) and FP32 vectors for the CG
-//           recurrences, FP64 reductions; the solution accumulates in FP64 and
-//           the true residual rhs - K_ff x is recomputed in FP64 whenever the
-//           FP32 residual has dropped by `reliable_delta` since the last
-//           replacement, or claims convergence ("reliable updates").  The
-//           stopping test is always on that FP64 residual.
+//   FP64        : operator, vectors and reductions in FP64.
+//   MIXED       : FP32 operator values and FP32 CG vectors; the solution
+//                 increment is folded into an FP64 solution.
+//   FP32_MATRIX : FP32 operator values, FP64 CG vectors.
+// In every mode the true residual rhs - K_ff x is recomputed with the FP64
+// operator whenever the recurrence residual has dropped by `reliable_delta`
+// since the last replacement, or claims convergence ("reliable updates"), and
+// the search direction is re-orthogonalised against the new residual.  In the
+// low-precision modes this is what carries the solve to FP64 accuracy; in
+// FP64 it costs one extra SpMV per 10x drop and keeps the recurrence honest.
+// Every mode stops only on that FP64 residual.
 //
-// Per iteration two kernels run, with no host round trip:
-//   spmv_cg : p_new = z + beta p_old (formed on the fly, also while gathering),
-//             q = K_ff p_new, partial p_new.q  -> last block: alpha
+// Per iteration four kernels run, with no host round trip:
+//   p_update: p = zcoef z + beta p (streaming)
+//   spmv_cg : q = K_ff p, partial p.q -> alpha
 //   update  : x += alpha p, r -= alpha q, z = D^-1 r, partial r.z, r.r
-//             -> last block: beta, flags
-// Every `check_every` iterations the FP64 residual kernels run (they exit at
-// once unless a flag is set) and the host reads the state once.
+//             -> beta, replacement flag
+//   resid   : exits at once unless a replacement is due (FP64 residual);
+//             replacements that stop making progress end the solve
+//             ("stagnation": the tolerance is below what FP64 can reach)
+// The host reads the 100-byte state every `check_every` iterations.
 //
 // Reductions are deterministic: wave64 butterfly shuffles, a fixed 4-wave
 // block combine, per-block partials, and the last block to arrive sums the
@@ -54,15 +60,18 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
     return is_last;
 }
 
-// Fixed-order sum of nparts partial pairs by one block.
-__device__ __forceinline__ void sum_partials2(const double* partials, int nparts, double* a, double* b, double* sh) {
-    double s0 = 0.0, s1 = 0.0;
+// Fixed-order sum of nparts partial triples (stride 3) by one block.
+__device__ __forceinline__ void sum_partials3(const double* partials, int nparts, double* a, double* b, double* c,
+                                              double* sh) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
     for (int i = threadIdx.x; i < nparts; i += kThreads) {
-        s0 += ld_coherent(partials + 2 * i);
-        s1 += ld_coherent(partials + 2 * i + 1);
+        s0 += ld_coherent(partials + 3 * i);
+        s1 += ld_coherent(partials + 3 * i + 1);
+        s2 += ld_coherent(partials + 3 * i + 2);
     }
     *a = block_sum_256(s0, sh);
     *b = block_sum_256(s1, sh + 4);
+    *c = block_sum_256(s2, sh);
 }
 
 // Logical block id: consecutive logical blocks land on the same XCD (the
@@ -90,8 +99,16 @@ __device__ __forceinline__ ACC csr_row(const int32_t* __restrict__ rowptr, const
     return s;
 }
 
+// Matrix streams are read once per SpMV: with NT the loads carry the
+// non-temporal hint so they do not evict the gathered vector from the caches.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
 // Node-block sliced format: this lane's block row of slice `sl`.
-template <int BS, typename TV, typename ACC, typename Gather>
+template <int BS, typename TV, typename ACC, bool NT = false, typename Gather>
 __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                          const int32_t* __restrict__ scol, const TV* __restrict__ sval, int64_t sl,
                                          int lane, Gather gx, ACC out[BS]) {
@@ -103,7 +120,7 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
 #pragma unroll 2
     for (int k = 0; k < w; ++k) {
         const int64_t slot = base + (int64_t)k * 64;
-        const int64_t c = scol[slot + lane];
+        const int64_t c = ld_stream<NT>(scol + slot + lane);
         ACC xv[BS];
 #pragma unroll
         for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
@@ -111,41 +128,51 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
 #pragma unroll
         for (int j = 0; j < BS; ++j)
 #pragma unroll
-            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)v[(j * BS + i) * 64], xv[i], out[j]);
+            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)ld_stream<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
     }
 }
 
+// ---------------------------------------------------------- p_update ---
+// p = zcoef*z + beta*p in place (zcoef = 1 except right after a residual
+// replacement, where it also makes p orthogonal to the new residual).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void p_update_kernel(T* __restrict__ p, const T* __restrict__ z, int64_t n,
+                                                            int64_t per_block, const CgState* st) {
+    if (st->done) return;
+    const T beta = (T)st->beta, zc = (T)st->zcoef;
+    const int64_t i0 = (int64_t)blockIdx.x * per_block;
+    const int64_t i1 = min(n, i0 + per_block);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) p[i] = fmaT<T>(beta, p[i], zc * z[i]);
+}
+
 // ---------------------------------------------------------- spmv_cg ----
+// q = K_ff p (accumulated in T), partial p.q -> last block: alpha = rz / pq.
 template <int G, typename TV, typename T>
 __global__ __launch_bounds__(kThreads) void csr_spmv_cg_kernel(const int32_t* __restrict__ rowptr,
                                                                const int32_t* __restrict__ col,
                                                                const TV* __restrict__ val, int64_t nrows,
-                                                               int64_t rows_per_block, const T* __restrict__ z,
-                                                               const T* __restrict__ pold, T* __restrict__ pnew,
+                                                               int64_t rows_per_block, const T* __restrict__ p,
                                                                T* __restrict__ q, CgState* st,
                                                                double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done) return;
-    const T beta = (T)st->beta;
     const int64_t r0 = logical_block() * rows_per_block;
     const int64_t r1 = min(nrows, r0 + rows_per_block);
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
     double dot = 0.0;
-    auto gx = [=](int64_t c) { return fmaT<T>(beta, pold[c], z[c]); };
+    auto gx = [=](int64_t c) { return p[c]; };
     for (int64_t row = r0 + g; row < r1; row += kThreads / G) {
         const T s = csr_row<G, TV, T>(rowptr, col, val, row, lane, gx);
         if (lane == 0) {
-            const T pn = fmaT<T>(beta, pold[row], z[row]);
-            pnew[row] = pn;
             q[row] = s;
-            dot += (double)pn * (double)s;
+            dot += (double)p[row] * (double)s;
         }
     }
     const double bsum = block_sum_256(dot, sh);
-    if (threadIdx.x == 0) partials[2 * blockIdx.x] = bsum;
+    if (threadIdx.x == 0) partials[3 * blockIdx.x] = bsum;
     if (last_block_arrive(&st->counter)) {
-        double pq, unused;
-        sum_partials2(partials, gridDim.x, &pq, &unused, sh);
+        double pq, u1, u2;
+        sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz / pq;
             if (!(pq > 0.0) || !std::isfinite(st->alpha)) st->done = 2;  // breakdown
@@ -154,39 +181,36 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_cg_kernel(const int32_t* __
     }
 }
 
-template <int BS, typename TV, typename T>
+template <int BS, typename TV, typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
     const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w, const int32_t* __restrict__ scol,
-    const TV* __restrict__ sval, int64_t nslices, int64_t nrows, int64_t slices_per_block, const T* __restrict__ z,
-    const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q, CgState* st, double* __restrict__ partials) {
+    const TV* __restrict__ sval, int64_t nslices, int64_t nrows, int64_t slices_per_block, const T* __restrict__ p,
+    T* __restrict__ q, CgState* st, double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done) return;
-    const T beta = (T)st->beta;
     const int64_t s0 = logical_block() * slices_per_block;
     const int64_t s1 = min(nslices, s0 + slices_per_block);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     double dot = 0.0;
-    auto gx = [=](int64_t c) { return fmaT<T>(beta, pold[c], z[c]); };
+    auto gx = [=](int64_t c) { return p[c]; };
     for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
         T out[BS];
-        sell_row<BS, TV, T>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
+        sell_row<BS, TV, T, NT>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
         const int64_t row = sl * 64 + lane;
         if (row < nrows) {
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 const int64_t d = row * BS + j;
-                const T pn = fmaT<T>(beta, pold[d], z[d]);
-                pnew[d] = pn;
                 q[d] = out[j];
-                dot += (double)pn * (double)out[j];
+                dot += (double)p[d] * (double)out[j];
             }
         }
     }
     const double bsum = block_sum_256(dot, sh);
-    if (threadIdx.x == 0) partials[2 * blockIdx.x] = bsum;
+    if (threadIdx.x == 0) partials[3 * blockIdx.x] = bsum;
     if (last_block_arrive(&st->counter)) {
-        double pq, unused;
-        sum_partials2(partials, gridDim.x, &pq, &unused, sh);
+        double pq, u1, u2;
+        sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz / pq;
             if (!(pq > 0.0) || !std::isfinite(st->alpha)) st->done = 2;
@@ -196,23 +220,32 @@ __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
 }
 
 // ----------------------------------------------------------- update ----
-template <typename T>
-__global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, T* __restrict__ r, T* __restrict__ z,
+// x += alpha p, r -= alpha q, z = D^-1 r, partial r.z and r.r.  With FOLD
+// (FP32 solution increment xs) a pending fold moves xs into x64 first.  The
+// last block sets beta and decides whether the FP64 residual is due.
+template <typename T, bool FOLD>
+__global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, double* __restrict__ x64,
+                                                             T* __restrict__ r, T* __restrict__ z,
                                                              const T* __restrict__ p, const T* __restrict__ q,
                                                              const T* __restrict__ dinv, int64_t n,
-                                                             int64_t per_block, int mixed, CgState* st,
+                                                             int64_t per_block, CgState* st,
                                                              double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done) return;
     const T a = (T)st->alpha;
+    const bool fold = FOLD && st->fold_pending;
     const int64_t i0 = (int64_t)blockIdx.x * per_block;
     const int64_t i1 = min(n, i0 + per_block);
     double rz = 0.0, rr = 0.0;
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
-        const T xi = fmaT<T>(a, p[i], x[i]);
+        T xi = x[i];
+        if (fold) {
+            x64[i] += (double)xi;
+            xi = 0;
+        }
         const T ri = fmaT<T>(-a, q[i], r[i]);
         const T zi = dinv[i] * ri;
-        x[i] = xi;
+        x[i] = fmaT<T>(a, p[i], xi);
         r[i] = ri;
         z[i] = zi;
         rz += (double)ri * (double)zi;
@@ -221,19 +254,29 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
     const double b0 = block_sum_256(rz, sh);
     const double b1 = block_sum_256(rr, sh + 4);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = b0;
-        partials[2 * blockIdx.x + 1] = b1;
+        partials[3 * blockIdx.x] = b0;
+        partials[3 * blockIdx.x + 1] = b1;
     }
     if (last_block_arrive(&st->counter)) {
-        double trz, trr;
-        sum_partials2(partials, gridDim.x, &trz, &trr, sh);
+        double trz, trr, u;
+        sum_partials3(partials, gridDim.x, &trz, &trr, &u, sh);
         if (threadIdx.x == 0) {
             st->rz_prev = st->rz;
             st->rz = trz;
             st->beta = trz / st->rz_prev;
             st->rr = trr;
             st->iters += 1;
-            if (trr <= st->tol2bb || (mixed && trr < st->delta2 * st->maxrr)) st->need_update = 1;
+            st->fold_pending = 0;
+            st->zcoef = 1.0;
+            if (trr <= st->tol2bb || trr < st->delta2 * st->maxrr) st->need_update = 1;
+            // backstop for a recurrence that stops decreasing: no 2x drop of |r|
+            // within max(10000, 4x the iterations it took to reach the best)
+            if (trr < 0.25 * st->best_rr) {
+                st->best_rr = trr;
+                st->best_it = st->iters;
+            } else if (st->iters - st->best_it > (4 * st->best_it > 10000 ? 4 * st->best_it : 10000)) {
+                st->done = 3;
+            }
             if (!std::isfinite(trr)) st->done = 2;
             st->counter = 0;
         }
@@ -241,53 +284,61 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
 }
 
 // ------------------------------------------- FP64 residual replacement --
-// x64 += xs, xs = 0 (mixed only), when an update is pending.
-__global__ __launch_bounds__(kThreads) void fold_kernel(double* __restrict__ x64, float* __restrict__ xs, int64_t n,
-                                                        const CgState* st) {
-    if (st->done || !st->need_update) return;
+// Runs after every update; exits at once unless the FP64 residual is due.
+// r = rhs - K_ff x with x = x64 (+ xs, the FP32 increment, when FOLD), in
+// FP64; writes r and z = D^-1 r in the CG precision.  The search direction
+// is made orthogonal to the new residual (p <- p - gamma z, gamma = r.p/r.z),
+// folded into zcoef of the next spmv_cg.
+__global__ __launch_bounds__(kThreads) void fold_kernel(double* __restrict__ x64, float* __restrict__ xs, int64_t n) {
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         x64[i] += (double)xs[i];
         xs[i] = 0.0f;
     }
 }
 
+template <bool FOLD>
 __device__ __forceinline__ void resid_finish(CgState* st, double* partials, double* sh) {
-    double trr, trz;
-    sum_partials2(partials, gridDim.x, &trr, &trz, sh);
+    double trr, trz, trp;
+    sum_partials3(partials, gridDim.x, &trr, &trz, &trp, sh);
     if (threadIdx.x == 0) {
         st->rr_true = trr;
         st->rr = trr;
         st->updates += 1;
         st->need_update = 0;
+        st->fold_pending = FOLD ? 1 : 0;
         if (trr <= st->tol2bb) {
             st->done = 1;
         } else {
+            const double beta = trz / st->rz_prev;
+            st->beta = beta;
+            st->zcoef = 1.0 - beta * (trp / trz);
             st->rz = trz;
-            st->beta = trz / st->rz_prev;
             st->maxrr = trr;
+            st->stall = trr > 0.81 * st->prev_true ? st->stall + 1 : 0;
+            st->prev_true = trr;
+            if (st->stall >= 5) st->done = 3;
         }
         if (!std::isfinite(trr)) st->done = 2;
         st->counter = 0;
     }
 }
 
-// r = rhs - K_ff x64 in FP64; writes r, z = D^-1 r in the CG precision T.
-template <int G, typename T>
+template <int G, typename T, bool FOLD>
 __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __restrict__ rowptr,
                                                              const int32_t* __restrict__ col,
                                                              const double* __restrict__ val, int64_t nrows,
                                                              int64_t rows_per_block, const double* __restrict__ x64,
-                                                             const double* __restrict__ rhs,
-                                                             const T* __restrict__ dinv, T* __restrict__ r,
-                                                             T* __restrict__ z, CgState* st,
+                                                             const T* __restrict__ xs, const double* __restrict__ rhs,
+                                                             const T* __restrict__ dinv, const T* __restrict__ p,
+                                                             T* __restrict__ r, T* __restrict__ z, CgState* st,
                                                              double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done || !st->need_update) return;
     const int64_t r0 = logical_block() * rows_per_block;
     const int64_t r1 = min(nrows, r0 + rows_per_block);
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
-    double rr = 0.0, rz = 0.0;
-    auto gx = [=](int64_t c) { return x64[c]; };
+    double rr = 0.0, rz = 0.0, rp = 0.0;
+    auto gx = [=](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
     for (int64_t row = r0 + g; row < r1; row += kThreads / G) {
         const double s = csr_row<G, double, double>(rowptr, col, val, row, lane, gx);
         if (lane == 0) {
@@ -298,30 +349,34 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
             z[row] = zt;
             rr += res * res;
             rz += (double)rt * (double)zt;
+            rp += (double)rt * (double)p[row];
         }
     }
     const double b0 = block_sum_256(rr, sh);
     const double b1 = block_sum_256(rz, sh + 4);
+    const double b2 = block_sum_256(rp, sh);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = b0;
-        partials[2 * blockIdx.x + 1] = b1;
+        partials[3 * blockIdx.x] = b0;
+        partials[3 * blockIdx.x + 1] = b1;
+        partials[3 * blockIdx.x + 2] = b2;
     }
-    if (last_block_arrive(&st->counter)) resid_finish(st, partials, sh);
+    if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
 
-template <int BS, typename T>
+template <int BS, typename T, bool FOLD>
 __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
     const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w, const int32_t* __restrict__ scol,
     const double* __restrict__ sval, int64_t nslices, int64_t nrows, int64_t slices_per_block,
-    const double* __restrict__ x64, const double* __restrict__ rhs, const T* __restrict__ dinv, T* __restrict__ r,
-    T* __restrict__ z, CgState* st, double* __restrict__ partials) {
+    const double* __restrict__ x64, const T* __restrict__ xs, const double* __restrict__ rhs,
+    const T* __restrict__ dinv, const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z, CgState* st,
+    double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done || !st->need_update) return;
     const int64_t s0 = logical_block() * slices_per_block;
     const int64_t s1 = min(nslices, s0 + slices_per_block);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double rr = 0.0, rz = 0.0;
-    auto gx = [=](int64_t c) { return x64[c]; };
+    double rr = 0.0, rz = 0.0, rp = 0.0;
+    auto gx = [=](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
     for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
         double out[BS];
         sell_row<BS, double, double>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
@@ -337,16 +392,19 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
                 z[d] = zt;
                 rr += res * res;
                 rz += (double)rt * (double)zt;
+                rp += (double)rt * (double)p[d];
             }
         }
     }
     const double b0 = block_sum_256(rr, sh);
     const double b1 = block_sum_256(rz, sh + 4);
+    const double b2 = block_sum_256(rp, sh);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = b0;
-        partials[2 * blockIdx.x + 1] = b1;
+        partials[3 * blockIdx.x] = b0;
+        partials[3 * blockIdx.x + 1] = b1;
+        partials[3 * blockIdx.x + 2] = b2;
     }
-    if (last_block_arrive(&st->counter)) resid_finish(st, partials, sh);
+    if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
 
 // bb = rhs.rhs (FP64), once per solve
@@ -360,12 +418,13 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double* __restric
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s += v[i] * v[i];
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) {
-        partials[2 * blockIdx.x] = b;
-        partials[2 * blockIdx.x + 1] = 0.0;
+        partials[3 * blockIdx.x] = b;
+        partials[3 * blockIdx.x + 1] = 0.0;
+        partials[3 * blockIdx.x + 2] = 0.0;
     }
     if (last_block_arrive(&st->counter)) {
-        double bb, unused;
-        sum_partials2(partials, gridDim.x, &bb, &unused, sh);
+        double bb, u1, u2;
+        sum_partials3(partials, gridDim.x, &bb, &u1, &u2, sh);
         if (threadIdx.x == 0) {
             st->bb = bb;
             st->tol2bb = rtol * rtol * bb;
@@ -393,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_kernel(const int32_t* __res
     }
 }
 
-template <int BS, typename TV, typename T>
+template <int BS, typename TV, typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void sell_spmv_kernel(const int64_t* __restrict__ sl_ptr,
                                                              const int32_t* __restrict__ sl_w,
                                                              const int32_t* __restrict__ scol,
@@ -406,7 +465,7 @@ __global__ __launch_bounds__(kThreads) void sell_spmv_kernel(const int64_t* __re
     auto gx = [=](int64_t c) { return x[c]; };
     for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
         T out[BS];
-        sell_row<BS, TV, T>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
+        sell_row<BS, TV, T, NT>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
         const int64_t row = sl * 64 + lane;
         if (row < nrows)
 #pragma unroll
@@ -419,10 +478,29 @@ __global__ void f64_to_f32_kernel(const double* __restrict__ a, float* __restric
         b[i] = (float)a[i];
 }
 
-__global__ __launch_bounds__(kThreads) void copy_kernel(const float4* __restrict__ a, float4* __restrict__ b,
-                                                        int64_t n) {
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
-        b[i] = a[i];
+// Streaming copy used as the measured HBM ceiling: UNROLL 16-byte loads in
+// flight per thread before the stores, optionally non-temporal.
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+template <int UNROLL, bool NT>
+__global__ __launch_bounds__(kThreads) void copy_kernel(const v4f* __restrict__ a, v4f* __restrict__ b, int64_t n) {
+    const int64_t stride = (int64_t)gridDim.x * kThreads * UNROLL;
+    for (int64_t base = (int64_t)blockIdx.x * kThreads * UNROLL + threadIdx.x; base < n; base += stride) {
+        v4f v[UNROLL];
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int64_t i = base + (int64_t)u * kThreads;
+            if (i < n) v[u] = ld_stream<NT>(a + i);
+        }
+#pragma unroll
+        for (int u = 0; u < UNROLL; ++u) {
+            const int64_t i = base + (int64_t)u * kThreads;
+            if (i < n) {
+                if constexpr (NT) __builtin_nontemporal_store(v[u], b + i);
+                else b[i] = v[u];
+            }
+        }
+    }
 }
 
 // --------------------------------------------------------- host side ---
@@ -448,69 +526,73 @@ Launch plan(int64_t items, int quantum, unsigned cap) {
 struct Ops {
     sfem_ctx* ctx;
     sfem_system* S;
-    int fmt;   // SFEM_FMT_CSR / SFEM_FMT_BSR
-    bool mixed;
+    int fmt;  // SFEM_FMT_CSR / SFEM_FMT_BSR
     unsigned cap;
     Launch csr, sell, vec;
+    bool nt;  // non-temporal matrix loads in the sliced SpMV
 };
 
-template <typename T>
-void launch_spmv_cg(const Ops& o, const T* z, const T* pold, T* pnew, T* q, CgState* st, double* partials) {
+template <typename TV>
+const TV* csr_vals(const sfem_system* S);
+template <>
+const double* csr_vals<double>(const sfem_system* S) { return S->val64.get(); }
+template <>
+const float* csr_vals<float>(const sfem_system* S) { return S->val32.get(); }
+template <typename TV>
+const TV* sell_vals(const sfem_system* S);
+template <>
+const double* sell_vals<double>(const sfem_system* S) { return S->sval64.get(); }
+template <>
+const float* sell_vals<float>(const sfem_system* S) { return S->sval32.get(); }
+
+template <typename TV, typename T>
+void launch_spmv_cg(const Ops& o, const T* p, T* q, CgState* st, double* partials) {
     sfem_system* S = o.S;
     hipStream_t s = o.ctx->stream;
     if (o.fmt == SFEM_FMT_BSR) {
-        const void* v = o.mixed ? (const void*)S->sval32.get() : (const void*)S->sval64.get();
-        using TV = T;
-        if (S->bs == 3)
-            hipLaunchKernelGGL((sell_spmv_cg_kernel<3, TV, T>), dim3(o.sell.grid), dim3(kThreads), 0, s,
-                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), (const TV*)v, S->nslices, S->nfb,
-                               o.sell.per_block, z, pold, pnew, q, st, partials);
-        else
-            hipLaunchKernelGGL((sell_spmv_cg_kernel<2, TV, T>), dim3(o.sell.grid), dim3(kThreads), 0, s,
-                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), (const TV*)v, S->nslices, S->nfb,
-                               o.sell.per_block, z, pold, pnew, q, st, partials);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(o.sell.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
+                               S->scol.get(), sell_vals<TV>(S), S->nslices, S->nfb, o.sell.per_block, p, q, st,
+                               partials);
+        };
+        if (S->bs == 3) {
+            if (o.nt) go(sell_spmv_cg_kernel<3, TV, T, true>);
+            else go(sell_spmv_cg_kernel<3, TV, T, false>);
+        } else {
+            if (o.nt) go(sell_spmv_cg_kernel<2, TV, T, true>);
+            else go(sell_spmv_cg_kernel<2, TV, T, false>);
+        }
     } else {
-        const void* v = o.mixed ? (const void*)S->val32.get() : (const void*)S->val64.get();
-        hipLaunchKernelGGL((csr_spmv_cg_kernel<kCsrLanes, T, T>), dim3(o.csr.grid), dim3(kThreads), 0, s,
-                           S->rowptr.get(), S->col.get(), (const T*)v, S->nf, o.csr.per_block, z, pold, pnew, q, st,
+        hipLaunchKernelGGL((csr_spmv_cg_kernel<kCsrLanes, TV, T>), dim3(o.csr.grid), dim3(kThreads), 0, s,
+                           S->rowptr.get(), S->col.get(), csr_vals<TV>(S), S->nf, o.csr.per_block, p, q, st,
                            partials);
     }
     SFEM_LAUNCH_CHECK();
 }
 
-template <typename T>
-void launch_resid(const Ops& o, const T* dinv, T* r, T* z, CgState* st, double* partials) {
+template <typename T, bool FOLD>
+void launch_resid(const Ops& o, const T* xs, const T* dinv, const T* p, T* r, T* z, CgState* st, double* partials) {
     sfem_system* S = o.S;
     hipStream_t s = o.ctx->stream;
     if (o.fmt == SFEM_FMT_BSR) {
         if (S->bs == 3)
-            hipLaunchKernelGGL((sell_resid_kernel<3, T>), dim3(o.sell.grid), dim3(kThreads), 0, s, S->sl_ptr.get(),
-                               S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb, o.sell.per_block,
-                               S->x.get(), S->rhs.get(), dinv, r, z, st, partials);
+            hipLaunchKernelGGL((sell_resid_kernel<3, T, FOLD>), dim3(o.sell.grid), dim3(kThreads), 0, s,
+                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb,
+                               o.sell.per_block, S->x.get(), xs, S->rhs.get(), dinv, p, r, z, st, partials);
         else
-            hipLaunchKernelGGL((sell_resid_kernel<2, T>), dim3(o.sell.grid), dim3(kThreads), 0, s, S->sl_ptr.get(),
-                               S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb, o.sell.per_block,
-                               S->x.get(), S->rhs.get(), dinv, r, z, st, partials);
+            hipLaunchKernelGGL((sell_resid_kernel<2, T, FOLD>), dim3(o.sell.grid), dim3(kThreads), 0, s,
+                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb,
+                               o.sell.per_block, S->x.get(), xs, S->rhs.get(), dinv, p, r, z, st, partials);
     } else {
-        hipLaunchKernelGGL((csr_resid_kernel<kCsrLanes, T>), dim3(o.csr.grid), dim3(kThreads), 0, s, S->rowptr.get(),
-                           S->col.get(), S->val64.get(), S->nf, o.csr.per_block, S->x.get(), S->rhs.get(), dinv, r, z,
-                           st, partials);
+        hipLaunchKernelGGL((csr_resid_kernel<kCsrLanes, T, FOLD>), dim3(o.csr.grid), dim3(kThreads), 0, s,
+                           S->rowptr.get(), S->col.get(), S->val64.get(), S->nf, o.csr.per_block, S->x.get(), xs,
+                           S->rhs.get(), dinv, p, r, z, st, partials);
     }
     SFEM_LAUNCH_CHECK();
 }
 
-double spmv_cg_bytes(const sfem_system* S, int fmt, bool mixed) {
-    const double sv = mixed ? 4.0 : 8.0;
-    const double vec = 4.0 * (double)S->nf * sv;  // z, p_old read; p_new, q written
-    if (fmt == SFEM_FMT_BSR)
-        return (double)S->nnzb * (S->bs * S->bs * sv + 4.0) + (double)(S->nslices + 1) * 8.0 +
-               (double)S->nslices * 4.0 + vec;
-    return (double)S->nnz * (sv + 4.0) + (double)(S->nf + 1) * 4.0 + vec;
-}
-
-double spmv_plain_bytes(const sfem_system* S, int fmt, bool mixed) {
-    const double sv = mixed ? 4.0 : 8.0;
-    const double vec = 2.0 * (double)S->nf * sv;  // x read once, y written once
+double spmv_bytes_for(const sfem_system* S, int fmt, double sv, double sx, int nvec) {
+    const double vec = (double)nvec * (double)S->nf * sx;
     if (fmt == SFEM_FMT_BSR)
         return (double)S->nnzb * (S->bs * S->bs * sv + 4.0) + (double)(S->nslices + 1) * 8.0 +
                (double)S->nslices * 4.0 + vec;
@@ -531,33 +613,39 @@ int resolve_format(const sfem_system* S, int fmt) {
     return fmt;
 }
 
-template <typename T>
+// TV: operator value type, T: CG vector type.  FOLD (T == float): the
+// solution increment lives in FP32 and is folded into the FP64 solution at
+// every residual replacement.
+template <typename TV, typename T>
 void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* stats, int fmt) {
-    const bool mixed = std::is_same<T, float>::value;
+    constexpr bool FOLD = std::is_same<T, float>::value;
     const int64_t nf = S->nf;
-    Ops ops{ctx, S, fmt, mixed, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}};
+    Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt};
     ops.csr = plan(nf, kThreads / kCsrLanes, ops.cap);
     if (S->has_block) ops.sell = plan(S->nslices, kThreads / 64, ops.cap);
     ops.vec = plan(nf, kThreads, std::max(8, ctx->n_cu * 4));
     const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid});
 
-    DevBuf<T> r(ctx, nf), z(ctx, nf), q(ctx, nf), p0(ctx, nf), p1(ctx, nf);
-    DevBuf<float> xs;
-    DevBuf<double> partials(ctx, 2 * (size_t)maxgrid);
+    DevBuf<T> r(ctx, nf), z(ctx, nf), q(ctx, nf), pv(ctx, nf);
+    DevBuf<T> xsbuf;
+    DevBuf<double> partials(ctx, 3 * (size_t)maxgrid);
     DevBuf<CgState> dst(ctx, 1);
-    const T* dinv = mixed ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get();
-    T* x = mixed ? nullptr : (T*)S->x.get();
-    if (mixed) {
-        xs.reset(ctx, nf);
-        memset_d(ctx, xs.get(), 0, sizeof(float) * nf);
-        x = (T*)xs.get();
+    const T* dinv = FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get();
+    T* x = (T*)S->x.get();  // FP64 paths update the FP64 solution directly
+    if (FOLD) {
+        xsbuf.reset(ctx, nf);
+        memset_d(ctx, xsbuf.get(), 0, sizeof(T) * nf);
+        x = xsbuf.get();
     }
+    const T* xs = FOLD ? xsbuf.get() : nullptr;
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * nf);
-    memset_d(ctx, p0.get(), 0, sizeof(T) * nf);
-    memset_d(ctx, p1.get(), 0, sizeof(T) * nf);
+    memset_d(ctx, pv.get(), 0, sizeof(T) * nf);
     CgState init{};
-    init.rz = 0.0;
+    init.rr = HUGE_VAL;
     init.rz_prev = HUGE_VAL;
+    init.zcoef = 1.0;
+    init.best_rr = HUGE_VAL;
+    init.prev_true = HUGE_VAL;
     init.need_update = 1;
     h2d(ctx, dst.get(), &init, sizeof(init));
     CgState* st = dst.get();
@@ -573,43 +661,35 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     }
     SFEM_HIP(hipEventRecord(t0, ctx->stream));
     hipLaunchKernelGGL(norm2_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->rhs.get(), nf,
-                       ops.vec.per_block, st, partials.get(), o->rtol, mixed ? o->reliable_delta : 0.0);
+                       ops.vec.per_block, st, partials.get(), o->rtol, o->reliable_delta);
     SFEM_LAUNCH_CHECK();
-    launch_resid<T>(ops, dinv, r.get(), z.get(), st, partials.get());
+    T* p = pv.get();
+    launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
 
     CgState host{};
-    int64_t enq = 0, spmv = 0, timed = 0, resid_launches = 1;
+    int64_t enq = 0, counted = 0, timed = 0;
     double spmv_ms = 0.0;
-    T* pc = p0.get();
-    T* pn = p1.get();
-    int done = 0;
     d2h(ctx, &host, st, sizeof(host));
-    done = host.done;
-    while (!done && enq < o->max_iter) {
+    while (!host.done && enq < o->max_iter) {
         const int64_t chunk = std::min<int64_t>(check, o->max_iter - enq);
         for (int64_t k = 0; k < chunk; ++k) {
+            hipLaunchKernelGGL(p_update_kernel<T>, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, p, z.get(), nf,
+                               ops.vec.per_block, st);
+            SFEM_LAUNCH_CHECK();
             if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k], ctx->stream));
-            launch_spmv_cg<T>(ops, z.get(), pc, pn, q.get(), st, partials.get());
+            launch_spmv_cg<TV, T>(ops, p, q.get(), st, partials.get());
             if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k + 1], ctx->stream));
-            std::swap(pc, pn);
-            hipLaunchKernelGGL(cg_update_kernel<T>, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x, r.get(),
-                               z.get(), pc, q.get(), dinv, nf, ops.vec.per_block, mixed ? 1 : 0, st,
+            hipLaunchKernelGGL((cg_update_kernel<T, FOLD>), dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x,
+                               S->x.get(), r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
                                partials.get());
             SFEM_LAUNCH_CHECK();
+            launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
         }
         enq += chunk;
-        if (mixed) {
-            hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->x.get(), xs.get(),
-                               nf, st);
-            SFEM_LAUNCH_CHECK();
-        }
-        launch_resid<T>(ops, dinv, r.get(), z.get(), st, partials.get());
-        ++resid_launches;
         d2h(ctx, &host, st, sizeof(host));
-        done = host.done;
         if (o->time_kernels) {
-            // launches after convergence exit at once; count only the real ones
-            const int64_t real = std::min<int64_t>(chunk, std::max<int64_t>(0, host.iters - spmv));
+            // launches after the stop exit at once; time only the real ones
+            const int64_t real = std::min<int64_t>(chunk, std::max<int64_t>(0, host.iters - counted));
             for (int64_t k = 0; k < real; ++k) {
                 float ms = 0;
                 SFEM_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
@@ -617,24 +697,19 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
             }
             timed += real;
         }
-        spmv = host.iters;
+        counted = host.iters;
     }
-    if (!done) {  // iteration cap: fold and evaluate the true residual once more
-        if (mixed) {
-            CgState h2 = host;
-            h2.need_update = 1;
-            h2d(ctx, st, &h2, sizeof(h2));
-            hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->x.get(),
-                               xs.get(), nf, st);
-            SFEM_LAUNCH_CHECK();
-        } else {
-            CgState h2 = host;
-            h2.need_update = 1;
-            h2d(ctx, st, &h2, sizeof(h2));
-        }
-        launch_resid<T>(ops, dinv, r.get(), z.get(), st, partials.get());
-        ++resid_launches;
+    if (!host.done) {  // iteration cap: evaluate the true residual of the final iterate
+        CgState h2 = host;
+        h2.need_update = 1;
+        h2d(ctx, st, &h2, sizeof(h2));
+        launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
         d2h(ctx, &host, st, sizeof(host));
+    }
+    if (FOLD) {  // x = x64 + xs
+        hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->x.get(),
+                           (float*)xsbuf.get(), nf);
+        SFEM_LAUNCH_CHECK();
     }
     SFEM_HIP(hipEventRecord(t1, ctx->stream));
     sync(ctx);
@@ -652,10 +727,11 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     stats->solve_ms = total;
     stats->spmv_ms = spmv_ms;
     stats->spmv_timed = timed;
-    stats->spmv_bytes = spmv_cg_bytes(S, fmt, mixed);
-    stats->converged = host.done == 1;
+    // spmv_cg: matrix once, p read once, q written
+    stats->spmv_bytes = spmv_bytes_for(S, fmt, sizeof(TV), sizeof(T), 2);
+    stats->converged = host.done == 1 ? 1 : 0;
     stats->format = fmt;
-    (void)resid_launches;
+    stats->stop_reason = host.done == 0 ? 4 : host.done;
 }
 
 }  // namespace
@@ -664,70 +740,63 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
     *st = sfem_solve_stats{};
     if (S->nf == 0) {
         st->converged = 1;
+        st->stop_reason = 1;
         return;
     }
     if (!(o->rtol >= 0.0) || o->max_iter < 0) fail(SFEM_ERR_ARGUMENT, "solve: bad rtol or max_iter");
-    const bool mixed = o->precision == SFEM_PREC_MIXED;
-    if (!mixed && o->precision != SFEM_PREC_FP64) fail(SFEM_ERR_ARGUMENT, "solve: unknown precision");
     const int fmt = resolve_format(S, o->format);
-    if (mixed) {
-        if (fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
-        run_pcg<float>(ctx, S, o, st, fmt);
-    } else {
-        run_pcg<double>(ctx, S, o, st, fmt);
+    switch (o->precision) {
+        case SFEM_PREC_FP64: run_pcg<double, double>(ctx, S, o, st, fmt); break;
+        case SFEM_PREC_MIXED:
+            if (fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
+            run_pcg<float, float>(ctx, S, o, st, fmt);
+            break;
+        case SFEM_PREC_FP32_MATRIX:
+            if (fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
+            run_pcg<float, double>(ctx, S, o, st, fmt);
+            break;
+        default: fail(SFEM_ERR_ARGUMENT, "solve: unknown precision");
     }
     if (!st->converged) {
+        static const char* why[] = {"", "", "breakdown", "stagnation", "iteration cap reached"};
         char buf[200];
         std::snprintf(buf, sizeof buf, "solve: %s after %lld iterations (relative residual %.3e)",
-                      st->iterations >= o->max_iter ? "iteration cap reached" : "breakdown",
-                      (long long)st->iterations, st->rel_residual);
+                      why[st->stop_reason], (long long)st->iterations, st->rel_residual);
         fail(SFEM_ERR_NOT_CONVERGED, buf);
     }
 }
 
-void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
-                       double* bytes) {
-    const bool mixed = precision == SFEM_PREC_MIXED;
-    const int fmt = resolve_format(S, format);
-    if (mixed && fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
+namespace {
+template <typename TV, typename T>
+double spmv_bench_impl(sfem_ctx* ctx, sfem_system* S, int fmt, int reps, bool nt) {
     const int64_t nf = S->nf;
     const unsigned cap = (unsigned)std::max(8, ctx->n_cu * 8);
     const Launch lc = plan(nf, kThreads / kCsrLanes, cap);
     const Launch ls = S->has_block ? plan(S->nslices, kThreads / 64, cap) : Launch{8, 1};
-    DevBuf<double> x64(ctx, nf), y64(ctx, nf);
-    DevBuf<float> x32(ctx, nf), y32(ctx, nf);
+    DevBuf<double> x64(ctx, nf);
+    DevBuf<T> x(ctx, nf), y(ctx, nf);
     SFEM_HIP(hipMemcpyAsync(x64.get(), S->rhs.get(), sizeof(double) * nf, hipMemcpyDeviceToDevice, ctx->stream));
-    hipLaunchKernelGGL(f64_to_f32_kernel, dim3(1024), dim3(256), 0, ctx->stream, x64.get(), x32.get(), nf);
-    SFEM_LAUNCH_CHECK();
+    if (std::is_same<T, float>::value)
+        hipLaunchKernelGGL(f64_to_f32_kernel, dim3(1024), dim3(256), 0, ctx->stream, x64.get(), (float*)x.get(), nf);
+    else
+        SFEM_HIP(hipMemcpyAsync(x.get(), x64.get(), sizeof(double) * nf, hipMemcpyDeviceToDevice, ctx->stream));
     auto launch = [&]() {
         hipStream_t s = ctx->stream;
         if (fmt == SFEM_FMT_BSR) {
-            if (mixed) {
-                if (S->bs == 3)
-                    hipLaunchKernelGGL((sell_spmv_kernel<3, float, float>), dim3(ls.grid), dim3(kThreads), 0, s,
-                                       S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices,
-                                       S->nfb, ls.per_block, x32.get(), y32.get());
-                else
-                    hipLaunchKernelGGL((sell_spmv_kernel<2, float, float>), dim3(ls.grid), dim3(kThreads), 0, s,
-                                       S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices,
-                                       S->nfb, ls.per_block, x32.get(), y32.get());
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(ls.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
+                                   S->scol.get(), sell_vals<TV>(S), S->nslices, S->nfb, ls.per_block, x.get(), y.get());
+            };
+            if (S->bs == 3) {
+                if (nt) go(sell_spmv_kernel<3, TV, T, true>);
+                else go(sell_spmv_kernel<3, TV, T, false>);
             } else {
-                if (S->bs == 3)
-                    hipLaunchKernelGGL((sell_spmv_kernel<3, double, double>), dim3(ls.grid), dim3(kThreads), 0, s,
-                                       S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices,
-                                       S->nfb, ls.per_block, x64.get(), y64.get());
-                else
-                    hipLaunchKernelGGL((sell_spmv_kernel<2, double, double>), dim3(ls.grid), dim3(kThreads), 0, s,
-                                       S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices,
-                                       S->nfb, ls.per_block, x64.get(), y64.get());
+                if (nt) go(sell_spmv_kernel<2, TV, T, true>);
+                else go(sell_spmv_kernel<2, TV, T, false>);
             }
-        } else if (mixed) {
-            hipLaunchKernelGGL((csr_spmv_kernel<kCsrLanes, float, float>), dim3(lc.grid), dim3(kThreads), 0, s,
-                               S->rowptr.get(), S->col.get(), S->val32.get(), nf, lc.per_block, x32.get(), y32.get());
         } else {
-            hipLaunchKernelGGL((csr_spmv_kernel<kCsrLanes, double, double>), dim3(lc.grid), dim3(kThreads), 0, s,
-                               S->rowptr.get(), S->col.get(), S->val64.get(), nf, lc.per_block, x64.get(),
-                               y64.get());
+            hipLaunchKernelGGL((csr_spmv_kernel<kCsrLanes, TV, T>), dim3(lc.grid), dim3(kThreads), 0, s,
+                               S->rowptr.get(), S->col.get(), csr_vals<TV>(S), nf, lc.per_block, x.get(), y.get());
         }
         SFEM_LAUNCH_CHECK();
     };
@@ -743,32 +812,63 @@ void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format,
     SFEM_HIP(hipEventElapsedTime(&ms, a, b));
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
-    *avg_ms = reps > 0 ? ms / reps : 0.0;
-    *bytes = spmv_plain_bytes(S, fmt, mixed);
+    return reps > 0 ? ms / reps : 0.0;
+}
+}  // namespace
+
+void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
+                       double* bytes) {
+    const int fmt = resolve_format(S, format);
+    if (precision != SFEM_PREC_FP64 && fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
+    switch (precision) {
+        case SFEM_PREC_FP64:
+            *avg_ms = spmv_bench_impl<double, double>(ctx, S, fmt, reps, ctx->tune_nt);
+            *bytes = spmv_bytes_for(S, fmt, 8, 8, 2);
+            break;
+        case SFEM_PREC_MIXED:
+            *avg_ms = spmv_bench_impl<float, float>(ctx, S, fmt, reps, ctx->tune_nt);
+            *bytes = spmv_bytes_for(S, fmt, 4, 4, 2);
+            break;
+        case SFEM_PREC_FP32_MATRIX:
+            *avg_ms = spmv_bench_impl<float, double>(ctx, S, fmt, reps, ctx->tune_nt);
+            *bytes = spmv_bytes_for(S, fmt, 4, 8, 2);
+            break;
+        default: fail(SFEM_ERR_ARGUMENT, "spmv bench: unknown precision");
+    }
 }
 
 void stream_copy_bench(sfem_ctx* ctx, int64_t bytes, int reps, double* avg_ms) {
     const int64_t n4 = bytes / 16;
     if (n4 <= 0) fail(SFEM_ERR_ARGUMENT, "copy bench: size too small");
-    DevBuf<float4> a(ctx, n4), b(ctx, n4);
-    memset_d(ctx, a.get(), 0, sizeof(float4) * n4);
-    const unsigned grid = (unsigned)std::max(8, ctx->n_cu * 8);
-    hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, ctx->stream, a.get(), b.get(), n4);
-    SFEM_LAUNCH_CHECK();
+    DevBuf<v4f> a(ctx, n4), b(ctx, n4);
+    memset_d(ctx, a.get(), 0, sizeof(v4f) * n4);
     hipEvent_t e0, e1;
     SFEM_HIP(hipEventCreate(&e0));
     SFEM_HIP(hipEventCreate(&e1));
-    SFEM_HIP(hipEventRecord(e0, ctx->stream));
-    for (int i = 0; i < reps; ++i)
-        hipLaunchKernelGGL(copy_kernel, dim3(grid), dim3(kThreads), 0, ctx->stream, a.get(), b.get(), n4);
-    SFEM_LAUNCH_CHECK();
-    SFEM_HIP(hipEventRecord(e1, ctx->stream));
-    sync(ctx);
-    float ms = 0;
-    SFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+    double best = 1e30;
+    // best of a few launch shapes and of plain vs non-temporal access
+    auto trial = [&](auto kern, unsigned grid) {
+        hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, ctx->stream, a.get(), b.get(), n4);
+        SFEM_LAUNCH_CHECK();
+        SFEM_HIP(hipEventRecord(e0, ctx->stream));
+        for (int i = 0; i < reps; ++i)
+            hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), 0, ctx->stream, a.get(), b.get(), n4);
+        SFEM_LAUNCH_CHECK();
+        SFEM_HIP(hipEventRecord(e1, ctx->stream));
+        sync(ctx);
+        float ms = 0;
+        SFEM_HIP(hipEventElapsedTime(&ms, e0, e1));
+        best = std::min(best, (double)ms / std::max(1, reps));
+    };
+    for (unsigned mult : {4u, 8u, 16u}) {
+        const unsigned grid = (unsigned)std::max(8, ctx->n_cu * (int)mult);
+        trial(copy_kernel<1, false>, grid);
+        trial(copy_kernel<4, false>, grid);
+        trial(copy_kernel<4, true>, grid);
+    }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    *avg_ms = ms / std::max(1, reps);
+    *avg_ms = best;
 }
 
 }  // namespace sfem

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <cstdio>
 #include <map>
@@ -60,18 +61,25 @@ private:
 
 }  // namespace sfem
 
+// A context lives until sfem_ctx_destroy has been called and every object
+// made on it (mesh, matrix, bc, system) is destroyed: each object holds a
+// reference, so any destroy order is safe.
 struct sfem_ctx {
+    std::atomic<int> refs{1};
     int device = 0;
     int n_cu = 0;
     hipStream_t stream = nullptr;
     std::string last_error;
     sfem::Allocator* alloc = nullptr;
-    // pinned host staging for small device->host reads
-    void* pinned = nullptr;
-    size_t pinned_bytes = 0;
+    // tuning switch read from SFEM_SPMV_NT at creation: non-temporal matrix
+    // loads in the sliced SpMV
+    bool tune_nt = false;
 };
 
 namespace sfem {
+
+void ctx_retain(sfem_ctx* ctx);
+void ctx_release(sfem_ctx* ctx);
 
 // RAII device buffer backed by the context allocator.
 template <typename T>

@@ -28,6 +28,7 @@ struct sfem_csr {
     sfem_ctx* ctx = nullptr;
     int64_t m = 0, n = 0, nnz = 0;
     int bs = 1;
+    bool transposed = false;  // stores K' (uploaded from a host CSC)
     int64_t nunits = 0, nnzb = 0;
     sfem::DevBuf<int64_t> bcolptr;  // nunits + 1
     sfem::DevBuf<int32_t> browidx;  // nnzb
@@ -78,6 +79,8 @@ int64_t exclusive_scan_i64(sfem_ctx* ctx, const int64_t* in, int64_t* out, int64
 // ---- assemble.hip ----
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K);
 void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval);
+void upload_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval,
+                 const double* nzval, sfem_csr* K);
 
 }  // namespace sfem
 
@@ -85,10 +88,16 @@ void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* row
 // reduction, read by every kernel of the next step.
 struct CgState {
     double rz, rz_prev, beta, alpha, rr, bb, maxrr, tol2bb, delta2, rr_true;
-    int done, need_update, pad0, pad1;
-    long long iters, updates;
+    double zcoef;        // p_new = zcoef*z + beta*p; 1 except right after a residual replacement
+    double best_rr;      // smallest recurrence |r|^2 seen (stagnation check)
+    double prev_true;    // true |r|^2 at the previous replacement
+    int done;            // 1 converged, 2 breakdown, 3 stagnated
+    int need_update;     // the true FP64 residual is due
+    int fold_pending;    // FP32 solution increment to be folded into x64
+    int stall;           // replacements in a row without progress
+    long long iters, updates, best_it;
     unsigned int counter;
-    unsigned int pad2;
+    unsigned int pad;
 };
 
 // K_ff and its solver formats, plus the data to recombine the solution.

@@ -1,0 +1,141 @@
+# smearFEM_hip.jl -- ccall shim that routes smearFEM's assemble-and-solve path to
+# the gfx950 library (include/smearfem.h).  Include it after `using smearFEM`
+# (or in place of src/fem.jl's assemble_system) to get the same signatures:
+#
+#   assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=nothing, Young=1, ν=0.3)
+#       -> SparseMatrixCSC{Float64,Int64}                       (src/fem.jl:135)
+#   solve_system(K, f, fixed, q_d; precision=:fp64, rtol=1e-10)
+#       -> q = q_d + C * (K_free \ C'(f - K q_d))               (examples/vector3D.jl:314-322)
+#
+# gaussian_quadrature / basis_function stay pure Julia (src/fem.jl:21, :48).
+# Status codes become the exceptions the reference raises on the same inputs.
+module SmearFEMHip
+
+using SparseArrays
+using LinearAlgebra
+
+export assemble_system, solve_system
+
+const libsfem = get(ENV, "SMEARFEM_HIP_LIB",
+                    joinpath(@__DIR__, "..", "smearfem", "libsmearfem_hip.so"))
+
+const CTX = Ref{Ptr{Cvoid}}(C_NULL)
+
+function context()
+    if CTX[] == C_NULL
+        r = Ref{Ptr{Cvoid}}(C_NULL)
+        st = ccall((:sfem_ctx_create, libsfem), Cint, (Cint, Ref{Ptr{Cvoid}}), 0, r)
+        st == 0 || error("sfem_ctx_create failed with status $st")
+        CTX[] = r[]
+        atexit(() -> ccall((:sfem_ctx_destroy, libsfem), Cvoid, (Ptr{Cvoid},), CTX[]))
+    end
+    return CTX[]
+end
+
+function check(ctx, st)
+    st == 0 && return nothing
+    msg = unsafe_string(ccall((:sfem_last_error, libsfem), Cstring, (Ptr{Cvoid},), ctx))
+    st == -1 && throw(ArgumentError(msg))
+    st == -2 && throw(BoundsError(msg))
+    st == -3 && throw(DimensionMismatch(msg))
+    st == -4 && throw(LinearAlgebra.SingularException(0))
+    st == -5 && throw(UndefVarError(Symbol(msg)))
+    error("smearfem_hip status $st: $msg")
+end
+
+function export_csc(ctx, Kp)
+    m, n, nnz, bs = Ref{Int64}(0), Ref{Int64}(0), Ref{Int64}(0), Ref{Cint}(0)
+    ccall((:sfem_csr_info, libsfem), Cint, (Ptr{Cvoid}, Ref{Int64}, Ref{Int64}, Ref{Int64}, Ref{Cint}),
+          Kp, m, n, nnz, bs)
+    colptr = Vector{Int64}(undef, n[] + 1)
+    rowval = Vector{Int64}(undef, nnz[])
+    nzval = Vector{Float64}(undef, nnz[])
+    st = ccall((:sfem_export_csc1, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}),
+               ctx, Kp, colptr, rowval, nzval)
+    check(ctx, st)
+    return SparseMatrixCSC(m[], n[], colptr, rowval, nzval)
+end
+
+function assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=nothing, Young=1, ν=0.3)
+    ctx = context()
+    nodes = Matrix{Float64}(NodeList)
+    ien = Matrix{Int64}(IEN)
+    idm = ID === nothing ? Matrix{Int64}(undef, 0, 0) : Matrix{Int64}(ID)
+    young = Young isa AbstractVector ? Vector{Float64}(Young) : [Float64(Young)]
+    cls = FunctionClass == "Q1" ? 1 : FunctionClass == "Q2" ? 2 : 0
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    st = GC.@preserve nodes ien idm young ccall((:sfem_assemble, libsfem), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Int64, Ptr{Int64}, Int64, Cint, Int64,
+         Ptr{Int64}, Int64, Ptr{Float64}, Int64, Float64, Ref{Ptr{Cvoid}}),
+        ctx, ndim, cls, nDof, nodes, size(nodes, 2), ien, size(ien, 1), size(ien, 2), ne^ndim,
+        ID === nothing ? C_NULL : pointer(idm), size(idm, 1), young, length(young), ν, out)
+    check(ctx, st)
+    try
+        return export_csc(ctx, out[])
+    finally
+        ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), out[])
+    end
+end
+
+# Field order and types follow sfem_solve_opts / sfem_solve_stats.
+mutable struct SolveOpts
+    precision::Cint
+    format::Cint
+    rtol::Float64
+    max_iter::Int64
+    reliable_delta::Float64
+    check_every::Cint
+    time_kernels::Cint
+    x0_zero::Cint
+end
+
+mutable struct SolveStats
+    iterations::Int64
+    fp64_updates::Int64
+    spmv_launches::Int64
+    spmv64_launches::Int64
+    rel_residual::Float64
+    solve_ms::Float64
+    spmv_ms::Float64
+    spmv_bytes::Float64
+    spmv_timed::Int64
+    converged::Cint
+    format::Cint
+    stop_reason::Cint
+end
+
+"""
+    solve_system(K, f, fixed, q_d; precision=:fp64, rtol=1e-10, max_iter=200000)
+
+`fixed[i] == true` marks a prescribed dof (the columns C = I[:, free] drops);
+`q_d` holds the prescribed values.  Returns q = q_d + C*q_f with
+K_free q_f = C'(f - K q_d), solved by Jacobi-PCG on the GPU.
+"""
+function solve_system(K::SparseMatrixCSC{Float64,Int64}, f, fixed, q_d; precision=:fp64, rtol=1e-10,
+                      max_iter=200000)
+    ctx = context()
+    n = size(K, 1)
+    Kp = Ref{Ptr{Cvoid}}(C_NULL)
+    st = ccall((:sfem_csr_from_csc1, libsfem), Cint,
+               (Ptr{Cvoid}, Int64, Int64, Ptr{Int64}, Ptr{Int64}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+               ctx, size(K, 1), size(K, 2), K.colptr, K.rowval, K.nzval, Kp)
+    check(ctx, st)
+    fv = f === nothing ? zeros(n) : Vector{Float64}(vec(f))
+    mask = Vector{UInt8}(fixed .!= 0)
+    qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
+    opts = SolveOpts(precision == :mixed ? 1 : 0, 0, rtol, max_iter, 0.1, 64, 0, 1)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0)
+    q = zeros(n)
+    try
+        st = ccall((:sfem_solve, libsfem), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}, Ref{SolveOpts},
+                    Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
+                   ctx, Kp[], fv, mask, qd, opts, q, C_NULL, stats)
+        check(ctx, st)
+    finally
+        ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), Kp[])
+    end
+    return q
+end
+
+end # module

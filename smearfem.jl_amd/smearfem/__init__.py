@@ -9,11 +9,13 @@ from .errors import (ArgumentError, BoundsError, DeviceError, DeviceMemoryError,
 from .fem import (Context, Mesh, StiffnessMatrix, assemble_system, basis_function, default_context,
                   gaussian_quadrature)
 from .mesh import StructuredMesh, clamp_x0, face_load, random_face_load, structured_mesh
+from .solve import BoundaryConditions, System, solve_system, stream_copy_bench
 
 __all__ = [
     "ArgumentError", "BoundsError", "DeviceError", "DeviceMemoryError", "DimensionMismatch",
     "NotConvergedError", "SingularException", "SmearFEMError", "StateError", "UndefVarError",
     "UnsupportedError", "Context", "Mesh", "StiffnessMatrix", "assemble_system", "basis_function",
     "default_context", "gaussian_quadrature", "StructuredMesh", "clamp_x0", "face_load",
-    "random_face_load", "structured_mesh",
+    "random_face_load", "structured_mesh", "BoundaryConditions", "System", "solve_system",
+    "stream_copy_bench",
 ]

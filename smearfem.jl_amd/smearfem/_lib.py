@@ -45,6 +45,7 @@ class SolveStats(C.Structure):
         ("spmv_timed", C.c_int64),
         ("converged", C.c_int),
         ("format", C.c_int),
+        ("stop_reason", C.c_int),
     ]
 
 
@@ -71,10 +72,12 @@ _PROTOS = [
     ("sfem_csr_stage_ms", C.c_int, [c_vp, c_dblp]),
     ("sfem_export_csc1", C.c_int, [c_vp, c_vp, c_i64p, c_i64p, c_dblp]),
     ("sfem_csr_destroy", None, [c_vp]),
+    ("sfem_csr_from_csc1", C.c_int, [c_vp, C.c_int64, C.c_int64, c_i64p, c_i64p, c_dblp, C.POINTER(c_vp)]),
     ("sfem_bc_create", C.c_int, [c_vp, C.c_int64, c_dblp, c_u8p, c_dblp, C.POINTER(c_vp)]),
     ("sfem_bc_destroy", None, [c_vp]),
     ("sfem_eliminate", C.c_int, [c_vp, c_vp, c_vp, C.POINTER(c_vp)]),
     ("sfem_system_info", C.c_int, [c_vp, c_i64p, c_i64p, c_intp]),
+    ("sfem_system_eliminate_ms", C.c_int, [c_vp, c_dblp]),
     ("sfem_system_destroy", None, [c_vp]),
     ("sfem_solve_opts_default", None, [C.POINTER(SolveOpts)]),
     ("sfem_system_solve", C.c_int, [c_vp, c_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),

@@ -1,0 +1,151 @@
+"""Dirichlet elimination and the K u = f solve on the GPU.
+
+The reference solves only inside its example driver
+(examples/vector3D.jl:314-322):
+
+    q_d, C = setboundaryCond(...)            # prescribed values, C = I[:, free]
+    K_free = C' * K_bar * C
+    q_f    = inv(Matrix(K_free)) * C' * (-K_bar * q_d)
+    q      = q_d + C * q_f
+
+``solve_system`` keeps that meaning (plus an optional load vector f:
+rhs = C'(f - K q_d)) and replaces the dense inverse by Jacobi-PCG on the
+device, in FP64 or in mixed FP32/FP64 precision.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import _lib
+from .fem import Context, StiffnessMatrix, _ptr, default_context
+
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32_matrix": 2}
+FORMATS = {"auto": 0, "csr": 1, "bsr": 2}
+FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
+
+
+class BoundaryConditions:
+    """Loads and Dirichlet data on the device (``sfem_bc``)."""
+
+    def __init__(self, ctx: Context, n: int, f=None, fixed=None, q_d=None):
+        self.ctx = ctx
+        fv = None if f is None else np.ascontiguousarray(f, dtype=np.float64)
+        mv = None if fixed is None else np.ascontiguousarray(np.asarray(fixed) != 0, dtype=np.uint8)
+        qv = None if q_d is None else np.ascontiguousarray(np.ravel(q_d), dtype=np.float64)
+        for name, v in (("f", fv), ("fixed", mv), ("q_d", qv)):
+            if v is not None and v.size != n:
+                raise ValueError(f"{name} has {v.size} entries, size(K,1) = {n}")
+        h = C.c_void_p()
+        ctx.check(ctx._lib.sfem_bc_create(ctx.handle, int(n), None if fv is None else _ptr(fv, C.c_double),
+                                          None if mv is None else _ptr(mv, C.c_uint8),
+                                          None if qv is None else _ptr(qv, C.c_double), C.byref(h)))
+        self.handle = h
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx._lib.sfem_bc_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class System:
+    """K_ff, rhs and the PCG state on the device (``sfem_system``).  The
+    stiffness matrix it was built from must stay alive (reactions use it)."""
+
+    def __init__(self, K: StiffnessMatrix, bc: BoundaryConditions):
+        self.ctx = K.ctx
+        self.K = K
+        h = C.c_void_p()
+        self.ctx.check(self.ctx._lib.sfem_eliminate(self.ctx.handle, K.handle, bc.handle, C.byref(h)))
+        self.handle = h
+        nf, nnz, bs = C.c_int64(), C.c_int64(), C.c_int()
+        self.ctx.check(self.ctx._lib.sfem_system_info(h, C.byref(nf), C.byref(nnz), C.byref(bs)))
+        ms = C.c_double()
+        self.ctx.check(self.ctx._lib.sfem_system_eliminate_ms(h, C.byref(ms)))
+        self.eliminate_ms = ms.value
+        self.n = K.shape[0]
+        self.n_free = nf.value
+        self.nnz = nnz.value
+        self.block = bs.value
+
+    def solve(self, precision="fp64", fmt="auto", rtol=1e-10, max_iter=200000, reliable_delta=0.1,
+              check_every=64, time_kernels=False, x0_zero=True, raise_on_fail=True) -> dict:
+        lib = self.ctx._lib
+        o = _lib.SolveOpts()
+        lib.sfem_solve_opts_default(C.byref(o))
+        o.precision = PRECISIONS[precision]
+        o.format = FORMATS[fmt]
+        o.rtol = float(rtol)
+        o.max_iter = int(max_iter)
+        o.reliable_delta = float(reliable_delta)
+        o.check_every = int(check_every)
+        o.time_kernels = 1 if time_kernels else 0
+        o.x0_zero = 1 if x0_zero else 0
+        s = _lib.SolveStats()
+        st = lib.sfem_system_solve(self.ctx.handle, self.handle, C.byref(o), C.byref(s))
+        stats = {name: getattr(s, name) for name, _ in _lib.SolveStats._fields_}
+        stats["format"] = FORMAT_NAMES.get(stats["format"], stats["format"])
+        stats["precision"] = precision
+        if st and raise_on_fail:
+            self.ctx.check(st)
+        stats["status"] = st
+        return stats
+
+    def solution(self, reactions: bool = False):
+        q = np.zeros(self.n)
+        R = np.zeros(self.n) if reactions else None
+        self.ctx.check(self.ctx._lib.sfem_system_solution(self.ctx.handle, self.handle, _ptr(q, C.c_double),
+                                                          None if R is None else _ptr(R, C.c_double)))
+        return (q, R) if reactions else q
+
+    def spmv_bench(self, precision="fp64", fmt="auto", reps=20):
+        ms, nbytes = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx._lib.sfem_system_spmv_bench(self.ctx.handle, self.handle, PRECISIONS[precision],
+                                                            FORMATS[fmt], int(reps), C.byref(ms), C.byref(nbytes)))
+        return ms.value, nbytes.value
+
+    def close(self) -> None:
+        if self.handle:
+            self.ctx._lib.sfem_system_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def solve_system(K: StiffnessMatrix, f=None, fixed=None, q_d=None, *, precision="fp64", fmt="auto",
+                 rtol=1e-10, max_iter=200000, reliable_delta=0.1, reactions=False, return_stats=False):
+    """q = q_d + C K_ff^{-1} C'(f - K q_d) on the GPU for a device matrix K."""
+    bc = BoundaryConditions(K.ctx, K.shape[0], f, fixed, q_d)
+    try:
+        S = System(K, bc)
+    finally:
+        bc.close()
+    try:
+        stats = S.solve(precision, fmt, rtol, max_iter, reliable_delta)
+        out = S.solution(reactions=reactions)
+    finally:
+        S.close()
+    if reactions:
+        q, R = out
+        return (q, R, stats) if return_stats else (q, R)
+    return (out, stats) if return_stats else out
+
+
+def stream_copy_bench(ctx: Optional[Context] = None, nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Average ms of one device-to-device copy of nbytes with the library's copy kernel."""
+    ctx = ctx or default_context()
+    ms = C.c_double()
+    ctx.check(ctx._lib.sfem_stream_copy_bench(ctx.handle, int(nbytes), int(reps), C.byref(ms)))
+    return ms.value
