@@ -162,24 +162,6 @@ double first_gp_det(int ndim, int cls, int nen, const double* node_list, const i
            J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
 }
 
-template <typename F>
-int guard(sfem_ctx* ctx, F&& f) {
-    try {
-        f();
-        if (ctx) ctx->last_error.clear();
-        return SFEM_OK;
-    } catch (const Error& e) {
-        if (ctx) ctx->last_error = e.msg;
-        return e.code;
-    } catch (const std::bad_alloc&) {
-        if (ctx) ctx->last_error = "host allocation failed";
-        return SFEM_ERR_MEMORY;
-    } catch (...) {
-        if (ctx) ctx->last_error = "unexpected internal error";
-        return SFEM_ERR_DEVICE;
-    }
-}
-
 }  // namespace
 }  // namespace sfem
 

@@ -100,7 +100,8 @@ template <int NDIM, int NEN, int NDOF>
 __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__ coords,
                                                       const int32_t* __restrict__ ien,
                                                       const double* __restrict__ young,
-                                                      int64_t n_young, double nu, int64_t n_elem,
+                                                      int64_t n_young, double nu_scalar,
+                                                      const double* __restrict__ nu_e, int64_t n_elem,
                                                       double* __restrict__ V, ValidateOut* out) {
     constexpr int NGP = 1 << NDIM;
     constexpr int P = NEN * NEN;
@@ -204,6 +205,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
         double D11 = 0, D12 = 0, D44 = 0;
         if constexpr (NDOF > 1) {
             const double E = (e < n_elem) ? young[n_young == 1 ? 0 : e] : 0.0;
+            const double nu = (nu_e && e < n_elem) ? nu_e[e] : nu_scalar;
             if constexpr (NDIM == 2) {  // plane stress, :217
                 D11 = E / (1 - nu * nu);
                 D12 = nu * E / (1 - nu * nu);
@@ -308,8 +310,8 @@ void launch_elem(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* out)
     constexpr int EPB = 256 / (NEN * NEN);
     const unsigned grid = grid_for(m->n_elem, EPB);
     hipLaunchKernelGGL((element_kernel<NDIM, NEN, NDOF>), dim3(grid), dim3(256), 0, ctx->stream,
-                       m->coords.get(), m->ien.get(), m->young.get(), m->n_young, m->nu, m->n_elem, V,
-                       out);
+                       m->coords.get(), m->ien.get(), m->young.get(), m->n_young, m->nu, m->nu_e.get(),
+                       m->n_elem, V, out);
     SFEM_LAUNCH_CHECK();
 }
 

@@ -34,103 +34,11 @@
 #include <vector>
 
 #include "sfem_objects.hpp"
+#include "solver_common.hpp"
 
 namespace sfem {
 
 namespace {
-
-constexpr int kThreads = 256;
-
-// ------------------------------------------------------------ helpers --
-__device__ __forceinline__ double ld_coherent(const double* p) {
-    unsigned long long v =
-        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return __longlong_as_double((long long)v);
-}
-
-// True in every thread of the block that arrives last; its partial results
-// (written before the call) are then visible to that block.
-__device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
-    __shared__ int is_last;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) is_last = atomicAdd(counter, 1u) == gridDim.x - 1;
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last;
-}
-
-// Fixed-order sum of nparts partial triples (stride 3) by one block.
-__device__ __forceinline__ void sum_partials3(const double* partials, int nparts, double* a, double* b, double* c,
-                                              double* sh) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kThreads) {
-        s0 += ld_coherent(partials + 3 * i);
-        s1 += ld_coherent(partials + 3 * i + 1);
-        s2 += ld_coherent(partials + 3 * i + 2);
-    }
-    *a = block_sum_256(s0, sh);
-    *b = block_sum_256(s1, sh + 4);
-    *c = block_sum_256(s2, sh);
-}
-
-// Logical block id: consecutive logical blocks land on the same XCD (the
-// hardware deals workgroups round-robin over the 8 XCDs), so each XCD's L2
-// sees a contiguous part of the rows and of the gathered vector.
-__device__ __forceinline__ int64_t logical_block() {
-    const int64_t g = gridDim.x, b = blockIdx.x;
-    if (g % 8) return b;
-    return (b % 8) * (g / 8) + b / 8;
-}
-
-template <typename T>
-__device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
-
-// ------------------------------------------------------- operator rows --
-// CSR, G lanes per row.  Returns the row sum in every lane of the group.
-template <int G, typename TV, typename ACC, typename Gather>
-__device__ __forceinline__ ACC csr_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
-                                       const TV* __restrict__ val, int64_t row, int lane, Gather gx) {
-    ACC s = 0;
-    const int32_t e = rowptr[row + 1];
-    for (int32_t k = rowptr[row] + lane; k < e; k += G) s = fmaT<ACC>((ACC)val[k], gx(col[k]), s);
-#pragma unroll
-    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-    return s;
-}
-
-// Matrix streams are read once per SpMV: with NT the loads carry the
-// non-temporal hint so they do not evict the gathered vector from the caches.
-template <bool NT, typename V>
-__device__ __forceinline__ V ld_stream(const V* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
-}
-
-// Node-block sliced format: this lane's block row of slice `sl`.
-template <int BS, typename TV, typename ACC, bool NT = false, typename Gather>
-__device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
-                                         const int32_t* __restrict__ scol, const TV* __restrict__ sval, int64_t sl,
-                                         int lane, Gather gx, ACC out[BS]) {
-    constexpr int B2 = BS * BS;
-    const int64_t base = sl_ptr[sl];
-    const int w = sl_w[sl];
-#pragma unroll
-    for (int j = 0; j < BS; ++j) out[j] = 0;
-#pragma unroll 2
-    for (int k = 0; k < w; ++k) {
-        const int64_t slot = base + (int64_t)k * 64;
-        const int64_t c = ld_stream<NT>(scol + slot + lane);
-        ACC xv[BS];
-#pragma unroll
-        for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
-        const TV* v = sval + slot * B2 + lane;
-#pragma unroll
-        for (int j = 0; j < BS; ++j)
-#pragma unroll
-            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)ld_stream<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
-    }
-}
 
 // ---------------------------------------------------------- p_update ---
 // p = zcoef*z + beta*p in place (zcoef = 1 except right after a residual
@@ -169,7 +77,7 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_cg_kernel(const int32_t* __
         }
     }
     const double bsum = block_sum_256(dot, sh);
-    if (threadIdx.x == 0) partials[3 * blockIdx.x] = bsum;
+    if (threadIdx.x == 0) st_coherent(partials + 3 * blockIdx.x, bsum);
     if (last_block_arrive(&st->counter)) {
         double pq, u1, u2;
         sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
@@ -207,7 +115,7 @@ __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
         }
     }
     const double bsum = block_sum_256(dot, sh);
-    if (threadIdx.x == 0) partials[3 * blockIdx.x] = bsum;
+    if (threadIdx.x == 0) st_coherent(partials + 3 * blockIdx.x, bsum);
     if (last_block_arrive(&st->counter)) {
         double pq, u1, u2;
         sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
@@ -254,8 +162,8 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
     const double b0 = block_sum_256(rz, sh);
     const double b1 = block_sum_256(rr, sh + 4);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = b0;
-        partials[3 * blockIdx.x + 1] = b1;
+        st_coherent(partials + 3 * blockIdx.x, b0);
+        st_coherent(partials + 3 * blockIdx.x + 1, b1);
     }
     if (last_block_arrive(&st->counter)) {
         double trz, trr, u;
@@ -356,9 +264,9 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
     const double b1 = block_sum_256(rz, sh + 4);
     const double b2 = block_sum_256(rp, sh);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = b0;
-        partials[3 * blockIdx.x + 1] = b1;
-        partials[3 * blockIdx.x + 2] = b2;
+        st_coherent(partials + 3 * blockIdx.x, b0);
+        st_coherent(partials + 3 * blockIdx.x + 1, b1);
+        st_coherent(partials + 3 * blockIdx.x + 2, b2);
     }
     if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
@@ -400,9 +308,9 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
     const double b1 = block_sum_256(rz, sh + 4);
     const double b2 = block_sum_256(rp, sh);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = b0;
-        partials[3 * blockIdx.x + 1] = b1;
-        partials[3 * blockIdx.x + 2] = b2;
+        st_coherent(partials + 3 * blockIdx.x, b0);
+        st_coherent(partials + 3 * blockIdx.x + 1, b1);
+        st_coherent(partials + 3 * blockIdx.x + 2, b2);
     }
     if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
@@ -418,9 +326,9 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double* __restric
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s += v[i] * v[i];
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x] = b;
-        partials[3 * blockIdx.x + 1] = 0.0;
-        partials[3 * blockIdx.x + 2] = 0.0;
+        st_coherent(partials + 3 * blockIdx.x, b);
+        st_coherent(partials + 3 * blockIdx.x + 1, 0.0);
+        st_coherent(partials + 3 * blockIdx.x + 2, 0.0);
     }
     if (last_block_arrive(&st->counter)) {
         double bb, u1, u2;

@@ -122,6 +122,26 @@ void d2h(sfem_ctx* ctx, void* dst, const void* src, size_t bytes);  // synchrono
 void memset_d(sfem_ctx* ctx, void* dst, int v, size_t bytes);
 void sync(sfem_ctx* ctx);
 
+// Runs f, turning internal errors into a status code plus ctx->last_error.
+template <typename F>
+inline int guard(sfem_ctx* ctx, F&& f) {
+    try {
+        f();
+        if (ctx) ctx->last_error.clear();
+        return SFEM_OK;
+    } catch (const Error& e) {
+        if (ctx) ctx->last_error = e.msg;
+        return e.code;
+    } catch (const std::bad_alloc&) {
+        if (ctx) ctx->last_error = "host allocation failed";
+        return SFEM_ERR_MEMORY;
+    } catch (...) {
+        if (ctx) ctx->last_error = "unexpected internal error";
+        return SFEM_ERR_DEVICE;
+    }
+}
+
+
 inline unsigned grid_for(int64_t n, int block) {
     int64_t g = (n + block - 1) / block;
     if (g < 1) g = 1;

@@ -14,6 +14,7 @@ struct sfem_mesh {
     sfem::DevBuf<int32_t> ien;     // n_elem x nen element-major, 0-based; -1 = outside NodeList
     sfem::DevBuf<int32_t> dof;     // id_rows x ndof node-major, 0-based; -1 = id < 1, -2 = id > 2^31
     sfem::DevBuf<double> young;    // n_young
+    sfem::DevBuf<double> nu_e;     // per-element Poisson ratio (batches); empty = scalar nu
     // Error the reference raises before any element work can be checked on the
     // device (structural mismatches found at element 1, or IEN too short):
     // (code, key) with key = element*8 + step, or code 0.

@@ -1,0 +1,123 @@
+// solver_common.hpp -- device building blocks shared by the single-system
+// PCG (pcg.hip) and the batched persistent PCG (batch.hip): coherent loads,
+// the last-block reduction handshake, and the CSR / node-block sliced row
+// engines.
+#pragma once
+
+#include "sfem_internal.hpp"
+
+namespace sfem {
+namespace {
+
+constexpr int kThreads = 256;
+
+// ------------------------------------------------------------ helpers --
+__device__ __forceinline__ double ld_coherent(const double* p) {
+    unsigned long long v =
+        __hip_atomic_load(reinterpret_cast<const unsigned long long*>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return __longlong_as_double((long long)v);
+}
+
+// Agent-coherent store of a block's partial result: it bypasses the XCD's L2
+// (per-XCD L2s are not coherent with each other) so the last block, on any
+// XCD, reads it with ld_coherent.
+__device__ __forceinline__ void st_coherent(double* p, double v) {
+    __hip_atomic_store(reinterpret_cast<unsigned long long*>(p), (unsigned long long)__double_as_longlong(v),
+                       __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Waits until this wave's outstanding vector memory operations, stores
+// included, have completed (GFX9 s_waitcnt: vmcnt(0), expcnt and lgkmcnt
+// left at their maxima).
+__device__ __forceinline__ void wait_vm_complete() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+// True in every thread of the block that arrives last.  Thread 0 must have
+// written this block's partials with st_coherent before the call; it waits for
+// those stores to complete before the counter increment.  Unlike
+// __threadfence() this does not write back and invalidate the whole L2, which
+// would evict the vector every other block is still gathering from.
+__device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        wait_vm_complete();
+        is_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
+    }
+    __syncthreads();
+    return is_last;
+}
+
+// Fixed-order sum of nparts partial triples (stride 3) by one block.
+__device__ __forceinline__ void sum_partials3(const double* partials, int nparts, double* a, double* b, double* c,
+                                              double* sh) {
+    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kThreads) {
+        s0 += ld_coherent(partials + 3 * i);
+        s1 += ld_coherent(partials + 3 * i + 1);
+        s2 += ld_coherent(partials + 3 * i + 2);
+    }
+    *a = block_sum_256(s0, sh);
+    *b = block_sum_256(s1, sh + 4);
+    *c = block_sum_256(s2, sh);
+}
+
+// Logical block id: consecutive logical blocks land on the same XCD (the
+// hardware deals workgroups round-robin over the 8 XCDs), so each XCD's L2
+// sees a contiguous part of the rows and of the gathered vector.
+__device__ __forceinline__ int64_t logical_block() {
+    const int64_t g = gridDim.x, b = blockIdx.x;
+    if (g % 8) return b;
+    return (b % 8) * (g / 8) + b / 8;
+}
+
+template <typename T>
+__device__ __forceinline__ T fmaT(T a, T b, T c) { return fma(a, b, c); }
+
+// ------------------------------------------------------- operator rows --
+// CSR, G lanes per row.  Returns the row sum in every lane of the group.
+template <int G, typename TV, typename ACC, typename Gather>
+__device__ __forceinline__ ACC csr_row(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                       const TV* __restrict__ val, int64_t row, int lane, Gather gx) {
+    ACC s = 0;
+    const int32_t e = rowptr[row + 1];
+    for (int32_t k = rowptr[row] + lane; k < e; k += G) s = fmaT<ACC>((ACC)val[k], gx(col[k]), s);
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+    return s;
+}
+
+// Matrix streams are read once per SpMV: with NT the loads carry the
+// non-temporal hint so they do not evict the gathered vector from the caches.
+template <bool NT, typename V>
+__device__ __forceinline__ V ld_stream(const V* p) {
+    if constexpr (NT) return __builtin_nontemporal_load(p);
+    else return *p;
+}
+
+// Node-block sliced format: this lane's block row of slice `sl`.
+template <int BS, typename TV, typename ACC, bool NT = false, typename Gather>
+__device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
+                                         const int32_t* __restrict__ scol, const TV* __restrict__ sval, int64_t sl,
+                                         int lane, Gather gx, ACC out[BS]) {
+    constexpr int B2 = BS * BS;
+    const int64_t base = sl_ptr[sl];
+    const int w = sl_w[sl];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[j] = 0;
+#pragma unroll 2
+    for (int k = 0; k < w; ++k) {
+        const int64_t slot = base + (int64_t)k * 64;
+        const int64_t c = ld_stream<NT>(scol + slot + lane);
+        ACC xv[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
+        const TV* v = sval + slot * B2 + lane;
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)ld_stream<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
+    }
+}
+
+}  // namespace
+}  // namespace sfem
