@@ -198,6 +198,108 @@ int  sfem_solve(sfem_ctx* ctx, const sfem_csr* K, const double* f, const uint8_t
                 const double* q_d, const sfem_solve_opts* opts, double* q, double* reactions,
                 sfem_solve_stats* stats);
 
+/* ---- batches: many meshes, one persistent solver launch ---------------
+ * No counterpart in the reference (each assemble_system / backslash call
+ * there handles one mesh, src/fem.jl:135-256); every problem is the same
+ * maths as a single-mesh call.  Problems are Q4 (ndim 2) or Hex8 (ndim 3)
+ * vector elasticity with node-interleaved dofs (ID[m,l] = ndim*(m-1)+l, as
+ * examples/vector3D.jl:33 builds it); every node fully fixed or fully free.
+ *   node_list    ndim x N_total, node-major, problems concatenated
+ *   node_offsets nprob+1 (node_offsets[p] = first node of problem p, 0-based)
+ *   ien          E_total x nen column-major, 1-based node ids local to the
+ *                element's problem
+ *   elem_offsets nprob+1
+ *   young, nu    one value per element
+ *   f, fixed, q_d  ndim*N_total values in the interleaved numbering */
+typedef struct sfem_batch sfem_batch;
+
+typedef struct {
+    int64_t iterations;
+    int64_t fp64_updates;
+    double  rel_residual;     /* final FP64 ||rhs - K_ff x|| / ||rhs||                   */
+    int64_t n_free;
+    int     stop_reason;      /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap */
+} sfem_batch_problem_stats;
+
+typedef struct {
+    double  solve_ms;         /* device time of all launches (HIP events)                 */
+    int64_t launches;
+    int64_t max_iterations;
+    int64_t sum_iterations;
+    int64_t not_converged;
+    int     grid;             /* blocks of the persistent launch                         */
+    int64_t device_bytes;     /* operator + vectors held by the batch                     */
+} sfem_batch_stats;
+
+int  sfem_batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, const double* node_list,
+                       const int64_t* node_offsets, const int64_t* ien, int nen,
+                       const int64_t* elem_offsets, const double* young, const double* nu,
+                       const double* f, const uint8_t* fixed, const double* q_d, sfem_batch** out);
+/* n_total = ndim*N_total; n_padded = dofs of the padded operator; slices of 64 node rows. */
+int  sfem_batch_info(const sfem_batch* b, int64_t* n_total, int64_t* n_padded, int64_t* slices,
+                     double* assemble_ms, double* eliminate_ms);
+/* Solve every problem.  item_slices: slices per work item (changes the
+ * reduction order, so results are bitwise reproducible per item_slices);
+ * blocks_per_cu: persistent grid = blocks_per_cu * CUs (0 = default; capped
+ * at residency).  opts->check_every > 0 splits the solve into launches of
+ * that many iterations; <= 0 runs it in one launch.  per: nprob entries or
+ * NULL.  Returns SFEM_ERR_NOT_CONVERGED if any problem did not converge. */
+int  sfem_batch_solve(sfem_ctx* ctx, sfem_batch* b, const sfem_solve_opts* opts, int item_slices,
+                      int blocks_per_cu, sfem_batch_problem_stats* per, sfem_batch_stats* stats);
+/* q = q_d + C x for all problems (n_total values, FP64). */
+int  sfem_batch_solution(sfem_ctx* ctx, const sfem_batch* b, double* q);
+/* Largest resident blocks per CU of the persistent kernel for a precision. */
+int  sfem_batch_max_blocks_per_cu(sfem_ctx* ctx, int precision, int* blocks);
+void sfem_batch_destroy(sfem_batch* b);
+
+/* ---- serving: continuous batching over the persistent solver -----------
+ * No counterpart in the reference.  Requests (one Q4 or Hex8 problem each,
+ * inputs as a one-problem batch) queue on the host; each step admits queued
+ * requests in arrival order while their device footprint fits the HBM
+ * budget and a slot is free (assembly + elimination on the device), runs
+ * iters_per_step iterations of every active request in one persistent
+ * launch, and retires the finished ones (solution copied to the host, device
+ * memory released).  A request's result does not depend on what else is
+ * active: it is bitwise the result it gets alone. */
+typedef struct sfem_server sfem_server;
+
+typedef struct {
+    int     precision;        /* SFEM_PREC_*                                              */
+    double  rtol;
+    int64_t max_iter;
+    double  reliable_delta;
+    int     item_slices;
+    int     blocks_per_cu;
+    int64_t hbm_budget;       /* bytes admitted requests may hold on the device           */
+    int     max_active;       /* cap on concurrently active requests                      */
+    int     iters_per_step;   /* CG iterations per persistent launch                      */
+} sfem_server_opts;
+
+#define SFEM_REQ_QUEUED  0
+#define SFEM_REQ_ACTIVE  1
+#define SFEM_REQ_DONE    2
+#define SFEM_REQ_FAILED  3
+
+void sfem_server_opts_default(sfem_server_opts* opts);
+int  sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server** out);
+/* Queue one problem; copies the inputs.  young/nu: one value per element.
+ * f / q_d may be NULL (zero).  *id receives the request id (0, 1, ...). */
+int  sfem_server_submit(sfem_server* s, int ndim, const double* node_list, int64_t n_nodes,
+                        const int64_t* ien, int64_t n_elem, const double* young, const double* nu,
+                        const double* f, const uint8_t* fixed, const double* q_d, int64_t* id);
+/* One admit / iterate / retire step.  Outputs (may be NULL): requests that
+ * finished in this step, and device ms of the step's solver launch. */
+int  sfem_server_step(sfem_server* s, int* n_finished, double* launch_ms);
+/* State of a request (SFEM_REQ_*), its stats once done, and host times in
+ * ms since the server was created: [submitted, admitted, finished]. */
+int  sfem_server_poll(const sfem_server* s, int64_t id, int* state, sfem_batch_problem_stats* st,
+                      double* times3);
+/* Copy a finished request's solution (ndim*n_nodes values) and forget it. */
+int  sfem_server_take(sfem_server* s, int64_t id, double* q);
+/* queued / active request counts and device bytes held by active requests. */
+int  sfem_server_info(const sfem_server* s, int64_t* queued, int64_t* active, int64_t* bytes);
+void sfem_server_destroy(sfem_server* s);
+
 /* ---- measurement helpers --------------------------------------------- */
 /* y = K_ff x in the given precision/format, `reps` times, on the system's
  * own vectors; returns the average device time per launch (HIP events). */

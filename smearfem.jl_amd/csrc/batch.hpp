@@ -1,0 +1,111 @@
+// batch.hpp -- the table-driven persistent PCG engine shared by batches and
+// the serving loop: per-problem descriptors, work items, CG state.
+#pragma once
+
+#include <utility>
+#include <vector>
+
+#include "sfem_objects.hpp"
+
+namespace sfem {
+
+// CG state of one problem (device memory; written by the persistent kernel).
+struct ProbState {
+    double rtol, delta;  // set by the host
+    double bb, tol2bb, delta2, maxrr;
+    double rz, rz_prev, beta, zcoef;
+    double rr_true, best_rr, best_true, rr_rec;
+    int64_t max_iter, iters, updates, best_it, last_upd, best_true_it;
+    int32_t init_pending, need_update, fold_pending, done;  // done: 0 running, 1 converged, 2 breakdown,
+    int32_t stall, capped, pad0, pad1;                      // 3 stagnation, 4 iteration cap
+};
+
+// One problem of a table: its sliced operator and vectors (padded layout,
+// 64-row slices, vectors of nslices*64*bs entries).  Pointers are typed by the
+// launch's precision: sval/dinv/r/z/p/q/xs are float in mixed mode.
+struct ProbDesc {
+    const int64_t* sl_ptr;
+    const int32_t* sl_w;
+    const int32_t* scol;
+    const void* sval;
+    const double* sval64;
+    const double* rhs;
+    double* x64;
+    void* xs;
+    void* r;
+    void* z;
+    void* p;
+    void* q;
+    const void* dinv;
+    ProbState* st;
+    int32_t bs, item0, item1, pad;
+};
+
+// A run of slices [slice0, slice1) of one problem, processed by one block.
+struct WorkItem {
+    int32_t prob, pad;
+    int64_t slice0, slice1;
+};
+
+struct TableLaunch {
+    const ProbDesc* probs;
+    int nprob;
+    const WorkItem* items;
+    int nitems;
+    double* part;   // 4 per item
+    double* partr;  // 3 per item
+    int64_t iters;  // loop passes in this launch
+};
+
+struct BatchProb {
+    int64_t slice0, slice1, n_free;
+};
+
+size_t prec_vec_bytes(int precision);
+int table_max_blocks_per_cu(int precision);
+int table_grid(sfem_ctx* ctx, int precision, int blocks_per_cu);
+void table_launch(sfem_ctx* ctx, int precision, const TableLaunch& a, int grid);
+void table_fold(sfem_ctx* ctx, double* x64, void* xs, int64_t n);
+ProbState initial_state(const sfem_solve_opts& o);
+
+}  // namespace sfem
+
+struct sfem_batch {
+    sfem_ctx* ctx = nullptr;
+    int64_t nprob = 0;
+    int ndim = 0, ndof = 0, bs = 0;
+    int64_t n = 0, npad = 0, nslices = 0, sell_blocks = 0;
+    std::vector<int64_t> node_off;
+    std::vector<sfem::BatchProb> hprobs;
+    // inputs kept only while the padded operator is built
+    sfem_mesh* mesh = nullptr;
+    sfem_csr* K = nullptr;
+    sfem_bc* bc = nullptr;
+    sfem::DevBuf<int32_t> pad_of;  // node unit -> padded row (-1 fixed)
+    sfem::DevBuf<int64_t> sl_ptr;
+    sfem::DevBuf<int32_t> sl_w, scol;
+    sfem::DevBuf<double> sval64, rhs, x, dinv64, qd;
+    sfem::DevBuf<float> sval32, dinv32;
+    // CG work vectors, typed by the last precision (work_vb bytes per entry)
+    size_t work_vb = 0;
+    sfem::DevBuf<uint8_t> wr, wz, wp, wq, wxs;
+    sfem::DevBuf<sfem::ProbState> st;
+    double t_assemble_ms = 0, t_eliminate_ms = 0;
+    ~sfem_batch();
+};
+
+namespace sfem {
+
+sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, const double* node_list,
+                         const int64_t* node_offsets, const int64_t* ien, int nen, const int64_t* elem_offsets,
+                         const double* young, const double* nu, const double* f, const uint8_t* fixed,
+                         const double* q_d);
+void batch_items(const sfem_batch* B, int item_slices, std::vector<WorkItem>* items,
+                 std::vector<std::pair<int, int>>* ranges);
+void batch_ensure_work(sfem_ctx* ctx, sfem_batch* B, int precision);
+ProbDesc batch_desc(sfem_batch* B, int precision, ProbState* st, int item0, int item1);
+void batch_reset(sfem_ctx* ctx, sfem_batch* B, int precision, const sfem_solve_opts& o, ProbState* d_st);
+void batch_solution(sfem_ctx* ctx, const sfem_batch* B, double* q);
+int64_t batch_device_bytes(const sfem_batch* B, int precision);
+
+}  // namespace sfem

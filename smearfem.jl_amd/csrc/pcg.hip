@@ -176,7 +176,11 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
             st->iters += 1;
             st->fold_pending = 0;
             st->zcoef = 1.0;
-            if (trr <= st->tol2bb || trr < st->delta2 * st->maxrr) st->need_update = 1;
+            // replacement when the recurrence converged, dropped by delta, or
+            // made no such drop for max(256, iteration of the best true residual)
+            const long long since = st->iters - st->last_upd;
+            if (trr <= st->tol2bb || trr < st->delta2 * st->maxrr || since >= (st->best_true_it > 256 ? st->best_true_it : 256))
+                st->need_update = 1;
             // backstop for a recurrence that stops decreasing: no 2x drop of |r|
             // within max(10000, 4x the iterations it took to reach the best)
             if (trr < 0.25 * st->best_rr) {
@@ -209,8 +213,10 @@ __device__ __forceinline__ void resid_finish(CgState* st, double* partials, doub
     double trr, trz, trp;
     sum_partials3(partials, gridDim.x, &trr, &trz, &trp, sh);
     if (threadIdx.x == 0) {
+        const double rec = st->rr;  // recurrence |r|^2 that triggered this replacement
         st->rr_true = trr;
         st->rr = trr;
+        st->last_upd = st->iters;
         st->updates += 1;
         st->need_update = 0;
         st->fold_pending = FOLD ? 1 : 0;
@@ -222,8 +228,16 @@ __device__ __forceinline__ void resid_finish(CgState* st, double* partials, doub
             st->zcoef = 1.0 - beta * (trp / trz);
             st->rz = trz;
             st->maxrr = trr;
-            st->stall = trr > 0.81 * st->prev_true ? st->stall + 1 : 0;
-            st->prev_true = trr;
+            // a replacement that does not beat the best true residual by 10%
+            // counts once; one that does not while the recurrence ran 10x
+            // below the true residual (the residual gap: the attainable
+            // floor) ends the solve
+            const bool gap = st->iters > 0 && 100.0 * rec < trr;
+            st->stall = trr > 0.81 * st->best_true ? st->stall + (gap ? 5 : 1) : 0;
+            if (trr < st->best_true) {
+                st->best_true = trr;
+                st->best_true_it = st->iters;
+            }
             if (st->stall >= 5) st->done = 3;
         }
         if (!std::isfinite(trr)) st->done = 2;
@@ -553,7 +567,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     init.rz_prev = HUGE_VAL;
     init.zcoef = 1.0;
     init.best_rr = HUGE_VAL;
-    init.prev_true = HUGE_VAL;
+    init.best_true = HUGE_VAL;
     init.need_update = 1;
     h2d(ctx, dst.get(), &init, sizeof(init));
     CgState* st = dst.get();

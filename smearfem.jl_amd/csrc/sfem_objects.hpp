@@ -91,12 +91,14 @@ struct CgState {
     double rz, rz_prev, beta, alpha, rr, bb, maxrr, tol2bb, delta2, rr_true;
     double zcoef;        // p_new = zcoef*z + beta*p; 1 except right after a residual replacement
     double best_rr;      // smallest recurrence |r|^2 seen (stagnation check)
-    double prev_true;    // true |r|^2 at the previous replacement
+    double best_true;    // smallest true |r|^2 at a replacement
     int done;            // 1 converged, 2 breakdown, 3 stagnated
     int need_update;     // the true FP64 residual is due
     int fold_pending;    // FP32 solution increment to be folded into x64
     int stall;           // replacements in a row without progress
     long long iters, updates, best_it;
+    long long last_upd;  // iteration of the latest replacement
+    long long best_true_it;  // iteration of best_true
     unsigned int counter;
     unsigned int pad;
 };

@@ -10,6 +10,8 @@ from .fem import (Context, Mesh, StiffnessMatrix, assemble_system, basis_functio
                   gaussian_quadrature)
 from .mesh import StructuredMesh, clamp_x0, face_load, random_face_load, structured_mesh
 from .solve import BoundaryConditions, System, solve_system, stream_copy_bench
+from .batch import Batch, Problem, autotune, make_problem, uneven_hex8_problems
+from .serve import Server, random_request, run_poisson
 
 __all__ = [
     "ArgumentError", "BoundsError", "DeviceError", "DeviceMemoryError", "DimensionMismatch",
@@ -17,5 +19,6 @@ __all__ = [
     "UnsupportedError", "Context", "Mesh", "StiffnessMatrix", "assemble_system", "basis_function",
     "default_context", "gaussian_quadrature", "StructuredMesh", "clamp_x0", "face_load",
     "random_face_load", "structured_mesh", "BoundaryConditions", "System", "solve_system",
-    "stream_copy_bench",
+    "stream_copy_bench", "Batch", "Problem", "autotune", "make_problem", "uneven_hex8_problems", "Server",
+    "random_request", "run_poisson",
 ]

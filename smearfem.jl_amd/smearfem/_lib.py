@@ -49,6 +49,42 @@ class SolveStats(C.Structure):
     ]
 
 
+class BatchProblemStats(C.Structure):
+    _fields_ = [
+        ("iterations", C.c_int64),
+        ("fp64_updates", C.c_int64),
+        ("rel_residual", C.c_double),
+        ("n_free", C.c_int64),
+        ("stop_reason", C.c_int),
+    ]
+
+
+class BatchStats(C.Structure):
+    _fields_ = [
+        ("solve_ms", C.c_double),
+        ("launches", C.c_int64),
+        ("max_iterations", C.c_int64),
+        ("sum_iterations", C.c_int64),
+        ("not_converged", C.c_int64),
+        ("grid", C.c_int),
+        ("device_bytes", C.c_int64),
+    ]
+
+
+class ServerOpts(C.Structure):
+    _fields_ = [
+        ("precision", C.c_int),
+        ("rtol", C.c_double),
+        ("max_iter", C.c_int64),
+        ("reliable_delta", C.c_double),
+        ("item_slices", C.c_int),
+        ("blocks_per_cu", C.c_int),
+        ("hbm_budget", C.c_int64),
+        ("max_active", C.c_int),
+        ("iters_per_step", C.c_int),
+    ]
+
+
 # (name, restype, argtypes)
 _PROTOS = [
     ("sfem_ctx_create", C.c_int, [C.c_int, C.POINTER(c_vp)]),
@@ -86,6 +122,24 @@ _PROTOS = [
      [c_vp, c_vp, c_dblp, c_u8p, c_dblp, C.POINTER(SolveOpts), c_dblp, c_dblp, C.POINTER(SolveStats)]),
     ("sfem_system_spmv_bench", C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_dblp, c_dblp]),
     ("sfem_stream_copy_bench", C.c_int, [c_vp, C.c_int64, C.c_int, c_dblp]),
+    ("sfem_batch_create", C.c_int,
+     [c_vp, C.c_int64, C.c_int, C.c_int, c_dblp, c_i64p, c_i64p, C.c_int, c_i64p, c_dblp, c_dblp, c_dblp, c_u8p,
+      c_dblp, C.POINTER(c_vp)]),
+    ("sfem_batch_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_dblp, c_dblp]),
+    ("sfem_batch_solve", C.c_int,
+     [c_vp, c_vp, C.POINTER(SolveOpts), C.c_int, C.c_int, C.POINTER(BatchProblemStats), C.POINTER(BatchStats)]),
+    ("sfem_batch_solution", C.c_int, [c_vp, c_vp, c_dblp]),
+    ("sfem_batch_max_blocks_per_cu", C.c_int, [c_vp, C.c_int, c_intp]),
+    ("sfem_batch_destroy", None, [c_vp]),
+    ("sfem_server_opts_default", None, [C.POINTER(ServerOpts)]),
+    ("sfem_server_create", C.c_int, [c_vp, C.POINTER(ServerOpts), C.POINTER(c_vp)]),
+    ("sfem_server_submit", C.c_int,
+     [c_vp, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, c_dblp, c_dblp, c_dblp, c_u8p, c_dblp, c_i64p]),
+    ("sfem_server_step", C.c_int, [c_vp, c_intp, c_dblp]),
+    ("sfem_server_poll", C.c_int, [c_vp, C.c_int64, c_intp, C.POINTER(BatchProblemStats), c_dblp]),
+    ("sfem_server_take", C.c_int, [c_vp, C.c_int64, c_dblp]),
+    ("sfem_server_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p]),
+    ("sfem_server_destroy", None, [c_vp]),
 ]
 
 EXPORTED = [p[0] for p in _PROTOS]
