@@ -1,25 +1,28 @@
-"""Benchmark: solved DOF/s of assemble + eliminate + solve on one MI355X.
+"""Benchmark: solved DOF/s of assemble + eliminate + solve on one MI355X, plus
+the batched (512 uneven meshes, one persistent launch) and served (Poisson
+request stream, p50/p99 latency) forms.
 
-Workload: 3D Hex8 block 256x64x64 elements on [0,4]x[0,1]x[0,1] (3,244,800 free
-dofs), per-element E = 10^U(-1,1) (seed 1234), nu = 0.3, x = 0 face clamped,
-iid N(0,1) nodal loads on every free node of the top face z = 1 (seed 5678).
-One step = device assembly (Ke, keys, sort, segmented reduction) + Dirichlet
-elimination + PCG to relative residual 1e-10 from resident inputs.
+Headline workload: 3D Hex8 block 256x64x64 elements on [0,4]x[0,1]x[0,1]
+(3,244,800 free dofs), per-element E = 10^U(-1,1) (seed 1234), nu = 0.3, x = 0
+face clamped, iid N(0,1) nodal loads on every free node of the top face z = 1
+(seed 5678).  One step = device assembly (Ke, keys, sort, segmented
+reduction) + Dirichlet elimination + PCG to relative residual 1e-10 from
+resident inputs.
 
-Four solver paths are timed on the same inputs: the plain FP64 path (scalar
-CSR), FP64 on the node-block sliced format, FP32 operator values with FP64
-vectors, and the FP32-CG / FP64-residual mixed form.  Each path's displacement
-is compared with an FP64 solve run to 1e-13.  value = solved DOF/s
-(n_free / step time) of the fastest path whose relative error is <= 1e-8;
-every path is listed under "paths" with its speed, iterations and error.
+Solver paths timed on the same inputs: the plain FP64 path (scalar CSR), FP64
+on the node-block sliced format, FP32 operator values with FP64 vectors, and
+the FP32-CG / FP64-residual mixed form.  Each path's displacement is compared
+with an FP64 solve carried on to stagnation (relative residual ~1e-13).
+value = solved DOF/s (n_free / step time) of the fastest path whose relative
+error is <= 1e-8; every path is listed under "paths".
 
-Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu] [--no-fp64]
+Usage: python bench.py [--steps K] [--warmup W] [--gpus 1] [--no-cpu]
+       [--no-batch] [--no-serve] [--only fp64/bsr,...] [--kernel-probe fp64/bsr]
 """
 from __future__ import annotations
 
 import argparse
 import json
-import os
 import sys
 import time
 from pathlib import Path
@@ -35,6 +38,7 @@ DIMS = (256, 64, 64)
 LENGTHS = (4.0, 1.0, 1.0)
 SEED_E, SEED_F = 1234, 5678
 RTOL = 1e-10
+ERR_BAR = 1e-8
 
 
 def log(msg):
@@ -69,7 +73,7 @@ def cpu_baseline(n_free_full, iters_full, nz_layers=2):
     """The CPU port (oracle/sfem_oracle.c, single thread) on a slab of the same
     mesh: nz_layers of the 64 element layers.  Assembly and elimination scale
     with the element count, one PCG iteration with nnz; the total for the full
-    problem uses the FP64 iteration count of the GPU run (same algorithm)."""
+    problem uses the FP64 Jacobi iteration count of the GPU run (same algorithm)."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from smearfem.mesh import clamp_x0, element_young_loguniform, random_face_load, structured_mesh
@@ -124,14 +128,155 @@ def kernel_probe(ctx, mesh, bc, path):
     K.close()
 
 
+def run_headline(args, ctx, m, young, fixed, f):
+    import smearfem
+    from smearfem.solve import BoundaryConditions
+
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, young, 0.3)
+    bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
+    variants = [("fp64", "csr"), ("fp64", "bsr"), ("fp32_matrix", "bsr"), ("mixed", "bsr")]
+    if args.only:
+        variants = [v for v in variants if f"{v[0]}/{v[1]}" in args.only.split(",")]
+    results = {}
+    q_ref = None
+    st_ref = {"rel_residual": float("nan"), "iterations": 0}
+    for vi, (precision, fmt) in enumerate(variants):
+        steps = []
+        nwarm = args.warmup if vi == 0 else 0
+        for i in range(nwarm + args.steps):
+            dt, st, K, S = timed_step(ctx, mesh, bc, precision, fmt, RTOL, time_kernels=True)
+            phase = "warmup" if i < nwarm else "timed"
+            log(f"{precision}/{fmt} {phase} step {i}: {dt:.3f}s iters={st['iterations']} "
+                f"updates={st['fp64_updates']} relres={st['rel_residual']:.2e} solve={st['solve_ms']:.1f}ms "
+                f"asm={st['assemble_stage_ms']['total']:.1f}ms elim={S.eliminate_ms:.1f}ms")
+            if i >= nwarm:
+                steps.append((dt, st))
+            if i == nwarm + args.steps - 1:
+                q = S.solution()
+                ms, nbytes = S.spmv_bench(precision, fmt, 20)
+                results[(precision, fmt)] = {
+                    "steps": steps, "n_free": S.n_free, "nnz": S.nnz, "q": q,
+                    "spmv_plain_GBs": nbytes / (ms * 1e-3) / 1e9, "spmv_plain_ms": ms,
+                    "eliminate_ms": S.eliminate_ms,
+                }
+                if (precision, fmt) == ("fp64", "bsr") and not args.no_ref:
+                    # FP64 reference for the error bound: this solution carried
+                    # on to stagnation (untimed)
+                    st_ref = S.solve("fp64", "bsr", rtol=1e-14, x0_zero=False, raise_on_fail=False)
+                    q_ref = S.solution()
+                    log(f"reference: +{st_ref['iterations']} iterations, relres={st_ref['rel_residual']:.2e} "
+                        f"stop={st_ref['stop_reason']}")
+            S.close()
+            K.close()
+    bc.close()
+    mesh.close()
+    if q_ref is None:
+        q_ref = results[("fp64", "bsr") if ("fp64", "bsr") in results else next(iter(results))]["q"]
+    ref_norm = np.linalg.norm(q_ref)
+    for r in results.values():
+        r["err"] = float(np.linalg.norm(r.pop("q") - q_ref) / ref_norm)
+
+    def summary(key):
+        r = results[key]
+        t = float(np.mean([d for d, _ in r["steps"]]))
+        st = r["steps"][-1][1]
+        ms = st["spmv_ms"] / max(1, st["spmv_timed"])
+        return {"value": r["n_free"] / t, "ms_per_step": t * 1e3, "iterations": int(st["iterations"]),
+                "fp64_updates": int(st["fp64_updates"]), "rel_residual": float(st["rel_residual"]),
+                "rel_err_vs_fp64_ref": r["err"], "solve_ms": float(st["solve_ms"]),
+                "us_per_iteration": 1e3 * float(st["solve_ms"]) / max(1, int(st["iterations"])),
+                "spmv_cg_avg_ms": ms, "spmv_cg_GBs": st["spmv_bytes"] / (ms * 1e-3) / 1e9,
+                "spmv_cg_bytes": st["spmv_bytes"], "spmv_timed": int(st["spmv_timed"]),
+                "spmv_plain_GBs": r["spmv_plain_GBs"], "assemble_ms": st["assemble_stage_ms"],
+                "eliminate_ms": r["eliminate_ms"]}
+
+    paths = {f"{k[0]}/{k[1]}": summary(k) for k in results}
+    info = {"n_free": results[next(iter(results))]["n_free"], "nnz": results[next(iter(results))]["nnz"],
+            "ref_rel_residual": float(st_ref["rel_residual"])}
+    return paths, info
+
+
+def run_batch(ctx, nprob=512, seed=4):
+    """512 uneven Hex8 cantilevers in one persistent launch per solve."""
+    import smearfem
+    from smearfem.batch import Batch, uneven_hex8_problems
+
+    t0 = time.perf_counter()
+    probs = uneven_hex8_problems(nprob, seed)
+    t_gen = time.perf_counter() - t0
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    B = Batch(probs, ctx)
+    ctx.synchronize()
+    t_build = time.perf_counter() - t0
+    n_free = int(sum(int((p.fixed == 0).sum()) for p in probs))
+    log(f"batch: {nprob} problems, {n_free} free dofs, host gen {t_gen:.2f}s, build {t_build * 1e3:.0f}ms "
+        f"(assembly {B.assemble_ms:.1f}ms, elimination {B.eliminate_ms:.1f}ms)")
+    out = {"problems": nprob, "seed": seed, "n_free": n_free, "build_ms": t_build * 1e3,
+           "assemble_ms": B.assemble_ms, "eliminate_ms": B.eliminate_ms, "forms": {}}
+    ref = None
+    for precision in ("fp64", "mixed"):
+        tune = smearfem.autotune(B, precision, probe_iters=20)
+        st, per = B.solve(precision, rtol=RTOL, item_slices=tune["item_slices"],
+                          blocks_per_cu=tune["blocks_per_cu"], raise_on_fail=False)
+        qs = B.solution()
+        if ref is None:
+            # FP64 reference for the error bound: the same batch to 1e-13
+            B.solve("fp64", rtol=1e-13, item_slices=tune["item_slices"], blocks_per_cu=tune["blocks_per_cu"],
+                    raise_on_fail=False)
+            ref = B.solution()
+        errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(qs, ref)]
+        its = np.array([p["iterations"] for p in per])
+        t = (st["solve_ms"] + B.assemble_ms + B.eliminate_ms) * 1e-3
+        out["forms"][precision] = {
+            "solve_ms": st["solve_ms"], "launches": st["launches"], "grid": st["grid"],
+            "item_slices": tune["item_slices"], "blocks_per_cu": tune["blocks_per_cu"],
+            "tune_ms_per_iter": tune["ms_per_iter"], "tuned_from_cache": tune["cached"],
+            "iterations_max": int(its.max()), "iterations_median": float(np.median(its)),
+            "iterations_sum": int(its.sum()), "not_converged": int(st["not_converged"]),
+            "max_rel_err_vs_fp64_1e-13": max(errs), "device_bytes": st["device_bytes"],
+            "solved_dof_per_s": n_free / t, "solve_only_dof_per_s": n_free / (st["solve_ms"] * 1e-3),
+        }
+        log(f"batch {precision}: solve {st['solve_ms']:.1f}ms in {st['launches']} launch, iterations max "
+            f"{its.max()} median {np.median(its):.0f}, max err {max(errs):.2e}, tune isl={tune['item_slices']} "
+            f"bpc={tune['blocks_per_cu']}")
+    B.close()
+    return out
+
+
+def run_serve(ctx, precision, rates, nreq=48, seed=11):
+    """Poisson request streams of Q4/Hex8 cantilevers (DOF log-uniform in
+    [1e3, 1e6]); the same requests at every arrival rate."""
+    from smearfem.serve import Server, random_request, run_poisson
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t0 = time.perf_counter()
+    reqs = [random_request(rng, 1e3, 1e6) for _ in range(nreq)]
+    log(f"serve: {nreq} requests generated in {time.perf_counter() - t0:.1f}s, "
+        f"{sum(r.n_dof for r in reqs)} dofs total")
+    out = {"precision": precision, "requests": nreq, "dof_range": [1e3, 1e6], "seed": seed, "rates": []}
+    for lam in rates:
+        S = Server(ctx, precision=precision, rtol=RTOL, iters_per_step=64, hbm_budget=64 << 30, max_active=64)
+        res = run_poisson(S, reqs, rate_per_s=lam, seed=seed + 1)
+        S.close()
+        s = res.summary()
+        s["rate_per_s"] = lam
+        out["rates"].append(s)
+        log(f"serve {precision} rate {lam}/s: p50 {s['p50_ms']:.1f}ms p99 {s['p99_ms']:.1f}ms, "
+            f"{s['solved_dof_per_s']:.3e} DOF/s, max active {s['max_active']}, failed {s['failed']}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--steps", type=int, default=1)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
-    ap.add_argument("--dims", type=str, default=None, help="override mesh, e.g. 64x16x16 (not the headline)")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-precision", default="fp64")
+    ap.add_argument("--serve-rates", default="4,16,64")
     ap.add_argument("--only", type=str, default=None, help="comma list of precision/format paths to time")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 reference solve (profiling runs)")
     ap.add_argument("--kernel-probe", type=str, default=None,
@@ -144,82 +289,30 @@ def main():
         sys.exit(2)
 
     import smearfem
-    from smearfem.solve import BoundaryConditions, System, stream_copy_bench
+    from smearfem.solve import BoundaryConditions, stream_copy_bench
 
-    dims = tuple(int(v) for v in args.dims.split("x")) if args.dims else DIMS
-    log(f"building inputs {dims}")
-    m, young, fixed, f = make_inputs(dims)
+    log(f"building inputs {DIMS}")
+    m, young, fixed, f = make_inputs()
     ctx = smearfem.Context(0)
     info = ctx.device_info()
     log(f"device {info}")
-    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, young, 0.3)
-    bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
-
     copy_ms = stream_copy_bench(ctx, 1 << 31, 10)
     copy_gbs = 2 * (1 << 31) / copy_ms / 1e6
     log(f"stream copy: {copy_gbs:.0f} GB/s")
 
     if args.kernel_probe:
+        mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, young, 0.3)
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
         kernel_probe(ctx, mesh, bc, args.kernel_probe)
         return
 
-    variants = [("fp64", "csr"), ("fp64", "bsr"), ("fp32_matrix", "bsr"), ("mixed", "bsr")]
-    if args.only:
-        variants = [v for v in variants if f"{v[0]}/{v[1]}" in args.only.split(",")]
-    results = {}
-    for vi, (precision, fmt) in enumerate(variants):
-        steps = []
-        nwarm = args.warmup if vi == 0 else 0
-        for i in range(nwarm + args.steps):
-            dt, st, K, S = timed_step(ctx, mesh, bc, precision, fmt, RTOL, time_kernels=True)
-            phase = "warmup" if i < nwarm else "timed"
-            log(f"{precision}/{fmt} {phase} step {i}: {dt:.3f}s iters={st['iterations']} updates={st['fp64_updates']} "
-                f"relres={st['rel_residual']:.2e} solve={st['solve_ms']:.1f}ms "
-                f"asm={st['assemble_stage_ms']['total']:.1f}ms elim={S.eliminate_ms:.1f}ms")
-            if i >= nwarm:
-                steps.append((dt, st))
-            if i == nwarm + args.steps - 1:
-                q = S.solution()
-                ms, nbytes = S.spmv_bench(precision, fmt, 20)
-                results[(precision, fmt)] = {
-                    "steps": steps, "n_free": S.n_free, "nnz": S.nnz, "q": q,
-                    "spmv_plain_GBs": nbytes / (ms * 1e-3) / 1e9, "spmv_plain_ms": ms,
-                }
-                if (precision, fmt) == ("fp64", "bsr") and not args.no_ref:
-                    # FP64 reference for the error bound: this solution carried
-                    # on to stagnation (untimed)
-                    st_ref = S.solve("fp64", "bsr", rtol=1e-14, x0_zero=False, raise_on_fail=False)
-                    q_ref = S.solution()
-                    log(f"reference: +{st_ref['iterations']} iterations, relres={st_ref['rel_residual']:.2e} "
-                        f"stop={st_ref['stop_reason']}")
-            S.close()
-            K.close()
-    if args.no_ref or ("fp64", "bsr") not in results:
-        q_ref = results[("fp64", "bsr") if ("fp64", "bsr") in results else next(iter(results))]["q"]
-        st_ref = {"rel_residual": float("nan")}
-    ref_norm = np.linalg.norm(q_ref)
-    for r in results.values():
-        r["err"] = float(np.linalg.norm(r.pop("q") - q_ref) / ref_norm)
-
-    def summary(key):
-        r = results[key]
-        t = float(np.mean([d for d, _ in r["steps"]]))
-        st = r["steps"][-1][1]
-        ms = st["spmv_ms"] / max(1, st["spmv_timed"])
-        return {"value": r["n_free"] / t, "ms_per_step": t * 1e3, "iterations": int(st["iterations"]),
-                "fp64_updates": int(st["fp64_updates"]), "rel_residual": float(st["rel_residual"]),
-                "rel_err_vs_fp64_ref": r["err"], "spmv_cg_avg_ms": ms,
-                "spmv_cg_GBs": st["spmv_bytes"] / (ms * 1e-3) / 1e9, "spmv_cg_bytes": st["spmv_bytes"],
-                "spmv_timed": int(st["spmv_timed"]), "spmv_plain_GBs": r["spmv_plain_GBs"],
-                "assemble_ms": st["assemble_stage_ms"]}
-
-    paths = {f"{k[0]}/{k[1]}": summary(k) for k in results}
-    valid = {k: v for k, v in paths.items() if v["rel_err_vs_fp64_ref"] <= 1e-8}
-    best_name = max(valid or paths, key=lambda k: (valid or paths)[k]["value"])
+    paths, hinfo = run_headline(args, ctx, m, young, fixed, f)
+    valid = {k: v for k, v in paths.items() if v["rel_err_vs_fp64_ref"] <= ERR_BAR}
+    pool = valid or paths
+    best_name = max(pool, key=lambda k: pool[k]["value"])
     best = paths[best_name]
     plain = paths.get("fp64/csr")
-    n_free = results[next(iter(results))]["n_free"]
-    dtype = {"fp64": "fp64", "fp32_matrix": "fp32 operator / fp64 vectors", "mixed": "fp32 CG / fp64 residual"}
+    dtype = {"fp64": "f64", "fp32_matrix": "f32 operator / f64 vectors", "mixed": "f32 CG / f64 residual"}
     out = {
         "metric": METRIC,
         "value": best["value"],
@@ -234,11 +327,10 @@ def main():
         "dtype": dtype[best_name.split("/")[0]],
         "data": "synthetic",
         "config": {
-            "workload": f"hex8-{dims[0]}x{dims[1]}x{dims[2]}-cantilever {best_name}",
-            "elements": int(m.n_elem), "n_free": int(n_free),
-            "nnz_Kff": int(results[next(iter(results))]["nnz"]), "rtol": RTOL,
-            "E": "10^U(-1,1) per element", "nu": 0.3, "load": "N(0,1) on free top-face nodes",
-            "ref_rel_residual": float(st_ref["rel_residual"]), "device": info["name"],
+            "workload": f"hex8-{DIMS[0]}x{DIMS[1]}x{DIMS[2]}-cantilever assemble+eliminate+PCG {best_name}",
+            "elements": int(m.n_elem), "n_free": int(hinfo["n_free"]), "nnz_Kff": int(hinfo["nnz"]),
+            "rtol": RTOL, "E": "10^U(-1,1) per element", "nu": 0.3, "load": "N(0,1) on free top-face nodes",
+            "ref_rel_residual": hinfo["ref_rel_residual"], "device": info["name"],
         },
         "roofline": {
             "bound": "hbm",
@@ -256,10 +348,15 @@ def main():
         "paths": paths,
         "speedup_vs_plain_fp64_csr": (best["value"] / plain["value"]) if plain else None,
     }
-    if not args.no_cpu and dims == DIMS:
+    if not args.no_batch:
+        out["batch"] = run_batch(ctx)
+    if not args.no_serve:
+        rates = [float(v) for v in args.serve_rates.split(",") if v]
+        out["serving"] = run_serve(ctx, args.serve_precision, rates)
+    if not args.no_cpu:
         iters_full = (plain or best)["iterations"]
         log("cpu baseline sample")
-        out["cpu_baseline"] = cpu_baseline(n_free, iters_full)
+        out["cpu_baseline"] = cpu_baseline(int(hinfo["n_free"]), iters_full)
     print(json.dumps(out), flush=True)
 
 

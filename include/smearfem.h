@@ -182,6 +182,8 @@ typedef struct {
     int     converged;
     int     format;          /* format actually used                                     */
     int     stop_reason;     /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap  */
+    double  floor_residual;  /* attainable relative residual estimate at the last FP64
+                                residual: eps || |rhs| + |K_ff||x| || / ||rhs||          */
 } sfem_solve_stats;
 
 void sfem_solve_opts_default(sfem_solve_opts* opts);
@@ -219,6 +221,7 @@ typedef struct {
     double  rel_residual;     /* final FP64 ||rhs - K_ff x|| / ||rhs||                   */
     int64_t n_free;
     int     stop_reason;      /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap */
+    double  floor_residual;   /* attainable relative residual estimate (see sfem_solve_stats) */
 } sfem_batch_problem_stats;
 
 typedef struct {

@@ -299,15 +299,17 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 T* r = (T*)D.r;
                 T* z = (T*)D.z;
                 const int bs = D.bs;
-                double rr = 0.0, rz = 0.0, rp = 0.0;
+                double rr = 0.0, rz = 0.0, rp = 0.0, ff = 0.0;
                 auto gx = [&](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
                 for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
-                    double out[3];
-                    if (bs == 3) sell_row<3, double, double>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out);
-                    else sell_row<2, double, double>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out);
+                    double out[3], aout[3];
+                    if (bs == 3) sell_row_abs<3>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out, aout);
+                    else sell_row_abs<2>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out, aout);
                     for (int j = 0; j < bs; ++j) {
                         const int64_t d = (sl * 64 + lane) * bs + j;
                         const double res = D.rhs[d] - out[j];
+                        const double sc = fabs(D.rhs[d]) + aout[j];
+                        ff += sc * sc;
                         const T rt = (T)res;
                         const T zt = dinv[d] * rt;
                         r[d] = rt;
@@ -317,11 +319,13 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                         rp += (double)rt * (double)p[d];
                     }
                 }
-                const double b0 = block_sum_256(rr, sh), b1 = block_sum_256(rz, sh + 4), b2 = block_sum_256(rp, sh);
+                const double b0 = block_sum_256(rr, sh), b1 = block_sum_256(rz, sh + 4);
+                const double b2 = block_sum_256(rp, sh), b3 = block_sum_256(ff, sh + 4);
                 if (threadIdx.x == 0) {
-                    a.partr[3 * (int64_t)it] = b0;
-                    a.partr[3 * (int64_t)it + 1] = b1;
-                    a.partr[3 * (int64_t)it + 2] = b2;
+                    a.partr[4 * (int64_t)it] = b0;
+                    a.partr[4 * (int64_t)it + 1] = b1;
+                    a.partr[4 * (int64_t)it + 2] = b2;
+                    a.partr[4 * (int64_t)it + 3] = b3;
                 }
             }
             grid.sync();
@@ -329,12 +333,15 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 ProbState* st = probs[pr].st;
                 if (!st->need_update || st->done) continue;
                 const int i0 = probs[pr].item0, i1 = probs[pr].item1;
-                const double trr = items_sum(a.partr, 3, 0, i0, i1, sh);
-                const double trz = items_sum(a.partr, 3, 1, i0, i1, sh);
-                const double trp = items_sum(a.partr, 3, 2, i0, i1, sh);
+                const double trr = items_sum(a.partr, 4, 0, i0, i1, sh);
+                const double trz = items_sum(a.partr, 4, 1, i0, i1, sh);
+                const double trp = items_sum(a.partr, 4, 2, i0, i1, sh);
+                // 16x the attainable |r|^2, (eps * || |rhs| + |K||x| ||)^2
+                const double floor_rr = 16.0 * kEps64 * kEps64 * items_sum(a.partr, 4, 3, i0, i1, sh);
                 if (threadIdx.x == 0) {
                     ProbState S = *st;
                     S.rr_true = trr;
+                    S.floor_rr = floor_rr;
                     S.updates += 1;
                     S.last_upd = S.iters;
                     S.need_update = 0;
@@ -348,17 +355,21 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                         S.zcoef = 1.0 - S.beta * (trp / trz);
                         S.rz = trz;
                         S.maxrr = trr;
-                        // not beating the best true residual by 10% counts once; doing
-                        // so across a residual gap (recurrence 10x below the true
-                        // residual: the attainable floor) ends the solve
-                        const bool gap = S.iters > 0 && 100.0 * S.rr_rec < trr;
-                        S.stall = trr > 0.81 * S.best_true ? S.stall + (gap ? 5 : 1) : 0;
-                        if (trr < S.best_true) {
-                            S.best_true = trr;
+                        // attainable floor: no 10% gain on the best true residual
+                        // within 4x of the rounding floor, or twice in a row while
+                        // the recurrence ran 5x below it
+                        const bool gap = S.iters > 0 && 25.0 * S.rr_rec < trr;
+                        if (trr <= 0.81 * S.best_true) {
+                            S.stall = 0;
                             S.best_true_it = S.iters;
+                        } else if (trr <= floor_rr) {
+                            S.stall += 2;
+                        } else if (gap) {
+                            S.stall += 1;
                         }
-                        if (S.stall >= 5) S.done = 3;
+                        if (S.stall >= 2) S.done = 3;
                     }
+                    if (trr < S.best_true) S.best_true = trr;
                     if (!isfinite(trr)) S.done = 2;
                     *st = S;
                 }
@@ -393,8 +404,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             auto gx = [&](int64_t c) { return p[c]; };
             for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
                 T out[3];
-                if (bs == 3) sell_row<3, TV, T>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
-                else sell_row<2, TV, T>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
+                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
+                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
                 for (int j = 0; j < bs; ++j) {
                     const int64_t d = (sl * 64 + lane) * bs + j;
                     q[d] = out[j];
@@ -744,7 +755,7 @@ double batch_run(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts& o, int ite
         descs[(size_t)p] = batch_desc(B, prec, B->st.get() + p, ranges[(size_t)p].first, ranges[(size_t)p].second);
     DevBuf<ProbDesc> ddesc(ctx, descs.size());
     DevBuf<WorkItem> ditems(ctx, items.size());
-    DevBuf<double> part(ctx, 4 * items.size()), partr(ctx, 3 * items.size());
+    DevBuf<double> part(ctx, 4 * items.size()), partr(ctx, 4 * items.size());
     h2d(ctx, ddesc.get(), descs.data(), sizeof(ProbDesc) * descs.size());
     h2d(ctx, ditems.get(), items.data(), sizeof(WorkItem) * items.size());
     TableLaunch a{};
@@ -920,6 +931,7 @@ int sfem_batch_solve(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts* opts, 
                 per[p].rel_residual = S.bb > 0 ? std::sqrt(S.rr_true / S.bb) : 0.0;
                 per[p].n_free = B->hprobs[(size_t)p].n_free;
                 per[p].stop_reason = S.done == 0 ? 4 : S.done;
+                per[p].floor_residual = S.bb > 0 ? std::sqrt(S.floor_rr / 16.0 / S.bb) : 0.0;
             }
             max_it = std::max<int64_t>(max_it, S.iters);
             sum_it += S.iters;

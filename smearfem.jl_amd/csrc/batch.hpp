@@ -14,7 +14,7 @@ struct ProbState {
     double rtol, delta;  // set by the host
     double bb, tol2bb, delta2, maxrr;
     double rz, rz_prev, beta, zcoef;
-    double rr_true, best_rr, best_true, rr_rec;
+    double rr_true, best_rr, best_true, rr_rec, floor_rr;
     int64_t max_iter, iters, updates, best_it, last_upd, best_true_it;
     int32_t init_pending, need_update, fold_pending, done;  // done: 0 running, 1 converged, 2 breakdown,
     int32_t stall, capped, pad0, pad1;                      // 3 stagnation, 4 iteration cap
@@ -53,7 +53,7 @@ struct TableLaunch {
     const WorkItem* items;
     int nitems;
     double* part;   // 4 per item
-    double* partr;  // 3 per item
+    double* partr;  // 4 per item
     int64_t iters;  // loop passes in this launch
 };
 

@@ -185,8 +185,11 @@ int sfem_ctx_create(int device, sfem_ctx** out) {
         ctx->n_cu = prop.multiProcessorCount;
         SFEM_HIP(hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking));
         ctx->alloc = new Allocator(device);
+        // launch-shape overrides for measurements (defaults in sfem_ctx)
         const char* nt = std::getenv("SFEM_SPMV_NT");
-        ctx->tune_nt = nt && nt[0] == '1';
+        if (nt) ctx->tune_nt = nt[0] == '1';
+        const char* ws = std::getenv("SFEM_SELL_WAVE_SLICES");
+        if (ws) ctx->tune_wave_slices = std::max(0, std::atoi(ws));
     });
     if (st) { delete ctx; return st; }
     *out = ctx;

@@ -210,11 +210,15 @@ __global__ __launch_bounds__(kThreads) void fold_kernel(double* __restrict__ x64
 
 template <bool FOLD>
 __device__ __forceinline__ void resid_finish(CgState* st, double* partials, double* sh) {
-    double trr, trz, trp;
-    sum_partials3(partials, gridDim.x, &trr, &trz, &trp, sh);
+    double sums[4];
+    sum_partials4(partials, gridDim.x, sums, sh);
+    const double trr = sums[0], trz = sums[1], trp = sums[2];
+    // 16x the attainable |r|^2, (eps * || |rhs| + |K||x| ||)^2
+    const double floor_rr = 16.0 * kEps64 * kEps64 * sums[3];
     if (threadIdx.x == 0) {
         const double rec = st->rr;  // recurrence |r|^2 that triggered this replacement
         st->rr_true = trr;
+        st->floor_rr = floor_rr;
         st->rr = trr;
         st->last_upd = st->iters;
         st->updates += 1;
@@ -228,18 +232,25 @@ __device__ __forceinline__ void resid_finish(CgState* st, double* partials, doub
             st->zcoef = 1.0 - beta * (trp / trz);
             st->rz = trz;
             st->maxrr = trr;
-            // a replacement that does not beat the best true residual by 10%
-            // counts once; one that does not while the recurrence ran 10x
-            // below the true residual (the residual gap: the attainable
-            // floor) ends the solve
-            const bool gap = st->iters > 0 && 100.0 * rec < trr;
-            st->stall = trr > 0.81 * st->best_true ? st->stall + (gap ? 5 : 1) : 0;
-            if (trr < st->best_true) {
-                st->best_true = trr;
+            // Attainable-floor test.  A replacement whose true residual does
+            // not beat the best by 10% ends the solve when it is within 4x of
+            // the rounding floor eps*|| |rhs| + |K||x| ||, and counts once
+            // when the recurrence ran 5x below it (the residual gap of finite
+            // precision; twice in a row ends the solve).  Slow progress away
+            // from the floor never stops it: the CG residual norm can plateau
+            // for thousands of iterations.
+            const bool gap = st->iters > 0 && 25.0 * rec < trr;
+            if (trr <= 0.81 * st->best_true) {
+                st->stall = 0;
                 st->best_true_it = st->iters;
+            } else if (trr <= floor_rr) {
+                st->stall += 2;
+            } else if (gap) {
+                st->stall += 1;
             }
-            if (st->stall >= 5) st->done = 3;
+            if (st->stall >= 2) st->done = 3;
         }
+        if (trr < st->best_true) st->best_true = trr;
         if (!std::isfinite(trr)) st->done = 2;
         st->counter = 0;
     }
@@ -259,12 +270,15 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
     const int64_t r0 = logical_block() * rows_per_block;
     const int64_t r1 = min(nrows, r0 + rows_per_block);
     const int g = threadIdx.x / G, lane = threadIdx.x % G;
-    double rr = 0.0, rz = 0.0, rp = 0.0;
+    double rr = 0.0, rz = 0.0, rp = 0.0, ff = 0.0;
     auto gx = [=](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
     for (int64_t row = r0 + g; row < r1; row += kThreads / G) {
-        const double s = csr_row<G, double, double>(rowptr, col, val, row, lane, gx);
+        double a;
+        const double s = csr_row_abs<G>(rowptr, col, val, row, lane, gx, &a);
         if (lane == 0) {
             const double res = rhs[row] - s;
+            const double sc = fabs(rhs[row]) + a;
+            ff += sc * sc;
             const T rt = (T)res;
             const T zt = dinv[row] * rt;
             r[row] = rt;
@@ -277,10 +291,12 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
     const double b0 = block_sum_256(rr, sh);
     const double b1 = block_sum_256(rz, sh + 4);
     const double b2 = block_sum_256(rp, sh);
+    const double b3 = block_sum_256(ff, sh + 4);
     if (threadIdx.x == 0) {
-        st_coherent(partials + 3 * blockIdx.x, b0);
-        st_coherent(partials + 3 * blockIdx.x + 1, b1);
-        st_coherent(partials + 3 * blockIdx.x + 2, b2);
+        st_coherent(partials + 4 * blockIdx.x, b0);
+        st_coherent(partials + 4 * blockIdx.x + 1, b1);
+        st_coherent(partials + 4 * blockIdx.x + 2, b2);
+        st_coherent(partials + 4 * blockIdx.x + 3, b3);
     }
     if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
@@ -297,17 +313,19 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
     const int64_t s0 = logical_block() * slices_per_block;
     const int64_t s1 = min(nslices, s0 + slices_per_block);
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    double rr = 0.0, rz = 0.0, rp = 0.0;
+    double rr = 0.0, rz = 0.0, rp = 0.0, ff = 0.0;
     auto gx = [=](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
     for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
-        double out[BS];
-        sell_row<BS, double, double>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
+        double out[BS], aout[BS];
+        sell_row_abs<BS>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out, aout);
         const int64_t row = sl * 64 + lane;
         if (row < nrows) {
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 const int64_t d = row * BS + j;
                 const double res = rhs[d] - out[j];
+                const double sc = fabs(rhs[d]) + aout[j];
+                ff += sc * sc;
                 const T rt = (T)res;
                 const T zt = dinv[d] * rt;
                 r[d] = rt;
@@ -321,10 +339,12 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
     const double b0 = block_sum_256(rr, sh);
     const double b1 = block_sum_256(rz, sh + 4);
     const double b2 = block_sum_256(rp, sh);
+    const double b3 = block_sum_256(ff, sh + 4);
     if (threadIdx.x == 0) {
-        st_coherent(partials + 3 * blockIdx.x, b0);
-        st_coherent(partials + 3 * blockIdx.x + 1, b1);
-        st_coherent(partials + 3 * blockIdx.x + 2, b2);
+        st_coherent(partials + 4 * blockIdx.x, b0);
+        st_coherent(partials + 4 * blockIdx.x + 1, b1);
+        st_coherent(partials + 4 * blockIdx.x + 2, b2);
+        st_coherent(partials + 4 * blockIdx.x + 3, b3);
     }
     if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
@@ -445,6 +465,16 @@ Launch plan(int64_t items, int quantum, unsigned cap) {
     return {(unsigned)grid, per};
 }
 
+// Sliced-operator launches: `wave_slices` > 0 gives every wavefront that
+// many slices (grid = slices / (4 * wave_slices)); 0 spreads them over the cap.
+Launch plan_sell(int64_t nslices, unsigned cap, int wave_slices) {
+    if (wave_slices <= 0) return plan(nslices, kThreads / 64, cap);
+    const int64_t per = (int64_t)wave_slices * (kThreads / 64);
+    int64_t grid = (nslices + per - 1) / per;
+    grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+    return {(unsigned)grid, per};
+}
+
 struct Ops {
     sfem_ctx* ctx;
     sfem_system* S;
@@ -544,13 +574,13 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     const int64_t nf = S->nf;
     Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt};
     ops.csr = plan(nf, kThreads / kCsrLanes, ops.cap);
-    if (S->has_block) ops.sell = plan(S->nslices, kThreads / 64, ops.cap);
+    if (S->has_block) ops.sell = plan_sell(S->nslices, ops.cap, ctx->tune_wave_slices);
     ops.vec = plan(nf, kThreads, std::max(8, ctx->n_cu * 4));
     const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid});
 
     DevBuf<T> r(ctx, nf), z(ctx, nf), q(ctx, nf), pv(ctx, nf);
     DevBuf<T> xsbuf;
-    DevBuf<double> partials(ctx, 3 * (size_t)maxgrid);
+    DevBuf<double> partials(ctx, 4 * (size_t)maxgrid);
     DevBuf<CgState> dst(ctx, 1);
     const T* dinv = FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get();
     T* x = (T*)S->x.get();  // FP64 paths update the FP64 solution directly
@@ -654,6 +684,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     stats->converged = host.done == 1 ? 1 : 0;
     stats->format = fmt;
     stats->stop_reason = host.done == 0 ? 4 : host.done;
+    stats->floor_residual = host.bb > 0 ? std::sqrt(host.floor_rr / 16.0 / host.bb) : 0.0;
 }
 
 }  // namespace
@@ -694,7 +725,7 @@ double spmv_bench_impl(sfem_ctx* ctx, sfem_system* S, int fmt, int reps, bool nt
     const int64_t nf = S->nf;
     const unsigned cap = (unsigned)std::max(8, ctx->n_cu * 8);
     const Launch lc = plan(nf, kThreads / kCsrLanes, cap);
-    const Launch ls = S->has_block ? plan(S->nslices, kThreads / 64, cap) : Launch{8, 1};
+    const Launch ls = S->has_block ? plan_sell(S->nslices, cap, ctx->tune_wave_slices) : Launch{8, 1};
     DevBuf<double> x64(ctx, nf);
     DevBuf<T> x(ctx, nf), y(ctx, nf);
     SFEM_HIP(hipMemcpyAsync(x64.get(), S->rhs.get(), sizeof(double) * nf, hipMemcpyDeviceToDevice, ctx->stream));

@@ -178,6 +178,7 @@ void retire(sfem_server* S, int* n_finished) {
         r.stats.rel_residual = st.bb > 0 ? std::sqrt(st.rr_true / st.bb) : 0.0;
         r.stats.n_free = r.B->hprobs[0].n_free;
         r.stats.stop_reason = st.done;
+        r.stats.floor_residual = st.bb > 0 ? std::sqrt(st.floor_rr / 16.0 / st.bb) : 0.0;
         r.state = SFEM_REQ_DONE;
         r.t_fin = S->now_ms();
         S->bytes_in_use -= r.bytes;
@@ -214,7 +215,7 @@ void step(sfem_server* S, int* n_finished, double* ms) {
     if (S->ditems.size() < items.size()) {
         S->ditems.reset(ctx, items.size() * 2);
         S->part.reset(ctx, items.size() * 8);
-        S->partr.reset(ctx, items.size() * 6);
+        S->partr.reset(ctx, items.size() * 8);
     }
     h2d(ctx, S->ddesc.get(), descs.data(), sizeof(ProbDesc) * descs.size());
     h2d(ctx, S->ditems.get(), items.data(), sizeof(WorkItem) * items.size());

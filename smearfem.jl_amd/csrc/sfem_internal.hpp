@@ -71,9 +71,12 @@ struct sfem_ctx {
     hipStream_t stream = nullptr;
     std::string last_error;
     sfem::Allocator* alloc = nullptr;
-    // tuning switch read from SFEM_SPMV_NT at creation: non-temporal matrix
-    // loads in the sliced SpMV
-    bool tune_nt = false;
+    // Sliced-SpMV launch shape (measured on MI355X, 256x64x64 Hex8, FP64:
+    // 392 us with both defaults vs 494 us with neither).  Overrides at
+    // creation: SFEM_SPMV_NT=0/1, SFEM_SELL_WAVE_SLICES=n (0 = spread the
+    // slices over at most 8 blocks per CU).
+    bool tune_nt = true;        // non-temporal matrix loads
+    int tune_wave_slices = 1;   // slices per wavefront
 };
 
 namespace sfem {

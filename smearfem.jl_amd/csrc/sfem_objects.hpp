@@ -92,6 +92,7 @@ struct CgState {
     double zcoef;        // p_new = zcoef*z + beta*p; 1 except right after a residual replacement
     double best_rr;      // smallest recurrence |r|^2 seen (stagnation check)
     double best_true;    // smallest true |r|^2 at a replacement
+    double floor_rr;     // 16 (eps || |rhs| + |K||x| ||)^2 at the last replacement
     int done;            // 1 converged, 2 breakdown, 3 stagnated
     int need_update;     // the true FP64 residual is due
     int fold_pending;    // FP32 solution increment to be folded into x64

@@ -9,6 +9,8 @@
 namespace sfem {
 namespace {
 
+constexpr double kEps64 = 2.220446049250313e-16;  // FP64 unit roundoff x 2
+
 constexpr int kThreads = 256;
 
 // ------------------------------------------------------------ helpers --
@@ -45,6 +47,18 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
     }
     __syncthreads();
     return is_last;
+}
+
+// Fixed-order sum of nparts partial quadruples (stride 4) by one block.
+__device__ __forceinline__ void sum_partials4(const double* partials, int nparts, double out[4], double* sh) {
+    double s[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int i = threadIdx.x; i < nparts; i += kThreads)
+#pragma unroll
+        for (int k = 0; k < 4; ++k) s[k] += ld_coherent(partials + 4 * i + k);
+    out[0] = block_sum_256(s[0], sh);
+    out[1] = block_sum_256(s[1], sh + 4);
+    out[2] = block_sum_256(s[2], sh);
+    out[3] = block_sum_256(s[3], sh + 4);
 }
 
 // Fixed-order sum of nparts partial triples (stride 3) by one block.
@@ -86,6 +100,28 @@ __device__ __forceinline__ ACC csr_row(const int32_t* __restrict__ rowptr, const
     return s;
 }
 
+// csr_row plus sum_k |val_k x_k| (the scale of the rounding error of the row
+// sum, for the attainable-residual estimate).
+template <int G, typename Gather>
+__device__ __forceinline__ double csr_row_abs(const int32_t* __restrict__ rowptr, const int32_t* __restrict__ col,
+                                              const double* __restrict__ val, int64_t row, int lane, Gather gx,
+                                              double* aout) {
+    double s = 0.0, a = 0.0;
+    const int32_t e = rowptr[row + 1];
+    for (int32_t k = rowptr[row] + lane; k < e; k += G) {
+        const double v = val[k], x = gx(col[k]);
+        s = fma(v, x, s);
+        a = fma(fabs(v), fabs(x), a);
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) {
+        s += __shfl_xor(s, off, 64);
+        a += __shfl_xor(a, off, 64);
+    }
+    *aout = a;
+    return s;
+}
+
 // Matrix streams are read once per SpMV: with NT the loads carry the
 // non-temporal hint so they do not evict the gathered vector from the caches.
 template <bool NT, typename V>
@@ -116,6 +152,34 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
         for (int j = 0; j < BS; ++j)
 #pragma unroll
             for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)ld_stream<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
+    }
+}
+
+// sell_row in FP64 plus aout[j] = sum |M_ji x_i| over the row's blocks.
+template <int BS, typename Gather>
+__device__ __forceinline__ void sell_row_abs(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
+                                             const int32_t* __restrict__ scol, const double* __restrict__ sval,
+                                             int64_t sl, int lane, Gather gx, double out[BS], double aout[BS]) {
+    constexpr int B2 = BS * BS;
+    const int64_t base = sl_ptr[sl];
+    const int w = sl_w[sl];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[j] = aout[j] = 0.0;
+    for (int k = 0; k < w; ++k) {
+        const int64_t slot = base + (int64_t)k * 64;
+        const int64_t c = ld_stream<true>(scol + slot + lane);
+        double xv[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
+        const double* v = sval + slot * B2 + lane;
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) {
+                const double m = ld_stream<true>(v + (j * BS + i) * 64);
+                out[j] = fma(m, xv[i], out[j]);
+                aout[j] = fma(fabs(m), fabs(xv[i]), aout[j]);
+            }
     }
 }
 

@@ -102,6 +102,7 @@ mutable struct SolveStats
     converged::Cint
     format::Cint
     stop_reason::Cint
+    floor_residual::Float64
 end
 
 """
@@ -124,7 +125,7 @@ function solve_system(K::SparseMatrixCSC{Float64,Int64}, f, fixed, q_d; precisio
     mask = Vector{UInt8}(fixed .!= 0)
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
     opts = SolveOpts(precision == :mixed ? 1 : 0, 0, rtol, max_iter, 0.1, 64, 0, 1)
-    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
     q = zeros(n)
     try
         st = ccall((:sfem_solve, libsfem), Cint,

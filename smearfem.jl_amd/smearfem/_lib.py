@@ -46,6 +46,7 @@ class SolveStats(C.Structure):
         ("converged", C.c_int),
         ("format", C.c_int),
         ("stop_reason", C.c_int),
+        ("floor_residual", C.c_double),
     ]
 
 
@@ -56,6 +57,7 @@ class BatchProblemStats(C.Structure):
         ("rel_residual", C.c_double),
         ("n_free", C.c_int64),
         ("stop_reason", C.c_int),
+        ("floor_residual", C.c_double),
     ]
 
 

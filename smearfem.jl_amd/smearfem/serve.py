@@ -152,12 +152,15 @@ class StreamResult:
     steps: int = 0
     launch_ms: float = 0.0
     failed: int = 0
+    at_floor: int = 0
+    rel_residuals: List[float] = field(default_factory=list)
     max_active: int = 0
 
     def summary(self) -> dict:
         lat = np.array(self.latencies_ms) if self.latencies_ms else np.array([np.nan])
         return {
-            "requests": len(self.latencies_ms), "failed": self.failed,
+            "requests": len(self.latencies_ms), "failed": self.failed, "at_floor": self.at_floor,
+            "max_rel_residual": float(max(self.rel_residuals)) if self.rel_residuals else None,
             "p50_ms": float(np.percentile(lat, 50)), "p99_ms": float(np.percentile(lat, 99)),
             "mean_ms": float(lat.mean()), "wall_s": self.wall_s,
             "solved_dof_per_s": float(np.sum(self.dofs) / self.wall_s) if self.wall_s > 0 else 0.0,
@@ -197,16 +200,17 @@ def run_poisson(server: Server, requests: Sequence[Problem], rate_per_s: float, 
             if p["state"] in ("done", "failed"):
                 idx, t_arr = pending.pop(rid)
                 t_done = time.perf_counter()
-                if p["state"] == "failed" or p["stop_reason"] != 1:
+                # solved: converged to rtol (1), or stopped at the FP64
+                # rounding floor of the system (3) when rtol lies below it
+                if p["state"] == "failed" or p["stop_reason"] not in (1, 3):
                     res.failed += 1
-                    if p["state"] == "done":
+                    try:
                         server.take(rid, requests[idx].n_dof)
-                    else:
-                        try:
-                            server.take(rid, requests[idx].n_dof)
-                        except Exception:
-                            pass
+                    except Exception:
+                        pass
                     continue
+                res.at_floor += p["stop_reason"] == 3
+                res.rel_residuals.append(p["rel_residual"])
                 q = server.take(rid, requests[idx].n_dof)
                 if check is not None:
                     check(idx, q, p)

@@ -76,6 +76,7 @@ def test_batch_q4_with_prescribed_values(ctx):
     st, per = B.solve("fp64", rtol=1e-13, raise_on_fail=False)
     assert st["status"] == smearfem.NotConvergedError.code
     assert all(s["stop_reason"] in (1, 3) for s in per) and max(s["iterations"] for s in per) < 5000
+    assert all(s["rel_residual"] <= max(1e-13, 20 * s["floor_residual"]) for s in per)
     B.close()
 
 

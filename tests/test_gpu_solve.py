@@ -188,5 +188,7 @@ def test_unattainable_tolerance_stops_on_stagnation(ctx):
         assert st["status"] == smearfem.NotConvergedError.code
         assert st["stop_reason"] == 3 and st["iterations"] < 20000
         assert st["rel_residual"] < 1e-9  # still the best FP64-accurate answer
+        # it stopped at the rounding floor eps*|| |rhs| + |K||x| || / ||rhs||
+        assert 0 < st["floor_residual"] < 1e-10 and st["rel_residual"] <= 20 * st["floor_residual"]
     S.close()
     K.close()
