@@ -39,6 +39,10 @@ LENGTHS = (4.0, 1.0, 1.0)
 SEED_E, SEED_F = 1234, 5678
 RTOL = 1e-10
 ERR_BAR = 1e-8
+# HBM bytes per CG-SpMV launch on this workload, from rocprofv3 FETCH_SIZE and
+# WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.00, write
+# x1.00); profiles/traffic_spmv_cg_hex8_256x64x64.md has the commands and data.
+MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9}
 
 
 def log(msg):
@@ -132,8 +136,12 @@ def run_headline(args, ctx, m, young, fixed, f):
     import smearfem
     from smearfem.solve import BoundaryConditions
 
+    ctx.synchronize()
+    t0 = time.perf_counter()
     mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, young, 0.3)
     bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
+    ctx.synchronize()
+    upload_s = time.perf_counter() - t0
     variants = [("fp64", "csr"), ("fp64", "bsr"), ("fp32_matrix", "bsr"), ("mixed", "bsr")]
     if args.only:
         variants = [v for v in variants if f"{v[0]}/{v[1]}" in args.only.split(",")]
@@ -152,12 +160,14 @@ def run_headline(args, ctx, m, young, fixed, f):
             if i >= nwarm:
                 steps.append((dt, st))
             if i == nwarm + args.steps - 1:
+                t0 = time.perf_counter()
                 q = S.solution()
+                download_s = time.perf_counter() - t0
                 ms, nbytes = S.spmv_bench(precision, fmt, 20)
                 results[(precision, fmt)] = {
                     "steps": steps, "n_free": S.n_free, "nnz": S.nnz, "q": q,
                     "spmv_plain_GBs": nbytes / (ms * 1e-3) / 1e9, "spmv_plain_ms": ms,
-                    "eliminate_ms": S.eliminate_ms,
+                    "eliminate_ms": S.eliminate_ms, "download_s": download_s,
                 }
                 if (precision, fmt) == ("fp64", "bsr") and not args.no_ref:
                     # FP64 reference for the error bound: this solution carried
@@ -188,11 +198,11 @@ def run_headline(args, ctx, m, young, fixed, f):
                 "spmv_cg_avg_ms": ms, "spmv_cg_GBs": st["spmv_bytes"] / (ms * 1e-3) / 1e9,
                 "spmv_cg_bytes": st["spmv_bytes"], "spmv_timed": int(st["spmv_timed"]),
                 "spmv_plain_GBs": r["spmv_plain_GBs"], "assemble_ms": st["assemble_stage_ms"],
-                "eliminate_ms": r["eliminate_ms"]}
+                "eliminate_ms": r["eliminate_ms"], "download_s": r["download_s"]}
 
     paths = {f"{k[0]}/{k[1]}": summary(k) for k in results}
     info = {"n_free": results[next(iter(results))]["n_free"], "nnz": results[next(iter(results))]["nnz"],
-            "ref_rel_residual": float(st_ref["rel_residual"])}
+            "ref_rel_residual": float(st_ref["rel_residual"]), "upload_s": upload_s}
     return paths, info
 
 
@@ -339,7 +349,7 @@ def main():
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
             "frac": best["spmv_cg_GBs"] / PEAK_HBM_GBS,
-            "traffic": None,
+            "traffic": MEASURED_TRAFFIC.get(best_name),
             "bytes_per_launch": best["spmv_cg_bytes"],
             "avg_launch_ms": best["spmv_cg_avg_ms"],
             "launches_timed": best["spmv_timed"],
@@ -347,6 +357,13 @@ def main():
         },
         "paths": paths,
         "speedup_vs_plain_fp64_csr": (best["value"] / plain["value"]) if plain else None,
+        # host-buffer boundary: the same step with the mesh/BC upload and the
+        # solution download added (not the headline value)
+        "pcie_inclusive": {
+            "upload_s": hinfo["upload_s"], "download_s": best["download_s"],
+            "solved_dof_per_s": hinfo["n_free"] / (best["ms_per_step"] * 1e-3 + hinfo["upload_s"]
+                                                   + best["download_s"]),
+        },
     }
     if not args.no_batch:
         out["batch"] = run_batch(ctx)
