@@ -142,7 +142,7 @@ def run_headline(args, ctx, m, young, fixed, f):
     bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
     ctx.synchronize()
     upload_s = time.perf_counter() - t0
-    variants = [("fp64", "csr"), ("fp64", "bsr"), ("fp32_matrix", "bsr"), ("mixed", "bsr")]
+    variants = [("fp64", "csr"), ("fp64", "bsr"), ("fp32_matrix", "bsr"), ("mixed", "bsr"), ("split", "bsr")]
     if args.only:
         variants = [v for v in variants if f"{v[0]}/{v[1]}" in args.only.split(",")]
     results = {}
@@ -322,7 +322,8 @@ def main():
     best_name = max(pool, key=lambda k: pool[k]["value"])
     best = paths[best_name]
     plain = paths.get("fp64/csr")
-    dtype = {"fp64": "f64", "fp32_matrix": "f32 operator / f64 vectors", "mixed": "f32 CG / f64 residual"}
+    dtype = {"fp64": "f64", "fp32_matrix": "f32 operator / f64 vectors", "mixed": "f32 CG / f64 residual",
+             "split": "f64 vectors / f32+16-bit operator"}
     out = {
         "metric": METRIC,
         "value": best["value"],

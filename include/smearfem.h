@@ -153,6 +153,8 @@ void sfem_system_destroy(sfem_system* sys);
 #define SFEM_PREC_FP64   0   /* plain FP64 Jacobi-PCG on FP64 CSR                       */
 #define SFEM_PREC_MIXED  1   /* FP32 operator and CG vectors, FP64 residual replacement  */
 #define SFEM_PREC_FP32_MATRIX 2 /* FP32 operator values, FP64 CG vectors and residuals    */
+#define SFEM_PREC_SPLIT  3   /* operator as FP32 + 16-bit correction (6 B/value, 2^-39
+                                relative), FP64 CG vectors; node-block format only       */
 
 #define SFEM_FMT_AUTO    0
 #define SFEM_FMT_CSR     1   /* scalar CSR, int32 columns                               */

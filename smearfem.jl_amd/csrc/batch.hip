@@ -404,8 +404,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             auto gx = [&](int64_t c) { return p[c]; };
             for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
                 T out[3];
-                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
-                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, sval, sl, lane, gx, out);
+                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, Vals<TV>{sval}, sl, lane, gx, out);
+                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, Vals<TV>{sval}, sl, lane, gx, out);
                 for (int j = 0; j < bs; ++j) {
                     const int64_t d = (sl * 64 + lane) * bs + j;
                     q[d] = out[j];

@@ -117,12 +117,24 @@ __global__ void slice_width_kernel(const int32_t* __restrict__ blen, int64_t nfb
     sl_cnt[s] = (int64_t)w * 64;
 }
 
+// 16-bit correction of the FP32 rounding h of v: round((v - h) / (ulp(h) 2^-16)),
+// so that h + q ulp(h) 2^-16 is v to 2^-39 relative.  0 where h is 0, not
+// finite, or too small for the scale to be a normal float (|h| < 2^-87).
+__device__ __forceinline__ int16_t split_correction(double v) {
+    const float h = (float)v;
+    const int e = (__float_as_int(h) >> 23) & 0xFF;
+    if (e <= 39 || e == 255) return 0;
+    const float scale = __int_as_float((__float_as_int(h) & 0x7F800000) - (39 << 23));
+    const double t = rint((v - (double)h) / (double)scale);
+    return (int16_t)fmin(fmax(t, -32767.0), 32767.0);
+}
+
 template <int BS>
 __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
                                  const double* __restrict__ bval, const int32_t* __restrict__ ufree_of,
                                  int64_t nunits, const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                  int32_t* __restrict__ scol, double* __restrict__ sval64,
-                                 float* __restrict__ sval32) {
+                                 float* __restrict__ sval32, int16_t* __restrict__ sval16) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -145,6 +157,7 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
                 const double v = bval[u * B2 + i * BS + j];
                 sval64[slot * B2 + (j * BS + i) * 64 + lane] = v;
                 sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
+                sval16[slot * B2 + (j * BS + i) * 64 + lane] = split_correction(v);
             }
         ++k;
     }
@@ -155,6 +168,7 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
         for (int v = 0; v < B2; ++v) {
             sval64[slot * B2 + v * 64 + lane] = 0.0;
             sval32[slot * B2 + v * 64 + lane] = 0.0f;
+            sval16[slot * B2 + v * 64 + lane] = 0;
         }
     }
 }
@@ -164,7 +178,8 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
 template <int BS>
 __global__ void sell_pad_tail_kernel(int64_t nfb, int64_t nslices, const int64_t* __restrict__ sl_ptr,
                                      const int32_t* __restrict__ sl_w, int32_t* __restrict__ scol,
-                                     double* __restrict__ sval64, float* __restrict__ sval32) {
+                                     double* __restrict__ sval64, float* __restrict__ sval32,
+                                     int16_t* __restrict__ sval16) {
     constexpr int B2 = BS * BS;
     const int lane = threadIdx.x;
     const int64_t s = nslices - 1;
@@ -176,6 +191,7 @@ __global__ void sell_pad_tail_kernel(int64_t nfb, int64_t nslices, const int64_t
         for (int v = 0; v < B2; ++v) {
             sval64[slot * B2 + v * 64 + lane] = 0.0;
             sval32[slot * B2 + v * 64 + lane] = 0.0f;
+            sval16[slot * B2 + v * 64 + lane] = 0;
         }
     }
 }
@@ -322,20 +338,23 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             S->scol.reset(ctx, (size_t)total);
             S->sval64.reset(ctx, (size_t)total * bs * bs);
             S->sval32.reset(ctx, (size_t)total * bs * bs);
+            S->sval16.reset(ctx, (size_t)total * bs * bs);
             if (bs == 3) {
                 hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get());
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
-                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
+                                   S->sval16.get());
             } else {
                 hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get());
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
-                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get());
+                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
+                                   S->sval16.get());
             }
             SFEM_LAUNCH_CHECK();
             S->has_block = true;

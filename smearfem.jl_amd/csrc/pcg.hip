@@ -92,7 +92,7 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_cg_kernel(const int32_t* __
 template <int BS, typename TV, typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
     const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w, const int32_t* __restrict__ scol,
-    const TV* __restrict__ sval, int64_t nslices, int64_t nrows, int64_t slices_per_block, const T* __restrict__ p,
+    Vals<TV> sval, int64_t nslices, int64_t nrows, int64_t slices_per_block, const T* __restrict__ p,
     T* __restrict__ q, CgState* st, double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done) return;
@@ -397,8 +397,8 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_kernel(const int32_t* __res
 template <int BS, typename TV, typename T, bool NT>
 __global__ __launch_bounds__(kThreads) void sell_spmv_kernel(const int64_t* __restrict__ sl_ptr,
                                                              const int32_t* __restrict__ sl_w,
-                                                             const int32_t* __restrict__ scol,
-                                                             const TV* __restrict__ sval, int64_t nslices,
+                                                             const int32_t* __restrict__ scol, Vals<TV> sval,
+                                                             int64_t nslices,
                                                              int64_t nrows, int64_t slices_per_block,
                                                              const T* __restrict__ x, T* __restrict__ y) {
     const int64_t s0 = logical_block() * slices_per_block;
@@ -491,11 +491,17 @@ const double* csr_vals<double>(const sfem_system* S) { return S->val64.get(); }
 template <>
 const float* csr_vals<float>(const sfem_system* S) { return S->val32.get(); }
 template <typename TV>
-const TV* sell_vals(const sfem_system* S);
+Vals<TV> sell_vals(const sfem_system* S);
 template <>
-const double* sell_vals<double>(const sfem_system* S) { return S->sval64.get(); }
+Vals<double> sell_vals<double>(const sfem_system* S) { return {S->sval64.get()}; }
 template <>
-const float* sell_vals<float>(const sfem_system* S) { return S->sval32.get(); }
+Vals<float> sell_vals<float>(const sfem_system* S) { return {S->sval32.get()}; }
+template <>
+Vals<SplitVal> sell_vals<SplitVal>(const sfem_system* S) { return {S->sval32.get(), S->sval16.get()}; }
+
+// bytes per stored operator value
+template <typename TV>
+constexpr double val_bytes() { return std::is_same<TV, SplitVal>::value ? 6.0 : (double)sizeof(TV); }
 
 template <typename TV, typename T>
 void launch_spmv_cg(const Ops& o, const T* p, T* q, CgState* st, double* partials) {
@@ -514,10 +520,12 @@ void launch_spmv_cg(const Ops& o, const T* p, T* q, CgState* st, double* partial
             if (o.nt) go(sell_spmv_cg_kernel<2, TV, T, true>);
             else go(sell_spmv_cg_kernel<2, TV, T, false>);
         }
-    } else {
+    } else if constexpr (!std::is_same<TV, SplitVal>::value) {
         hipLaunchKernelGGL((csr_spmv_cg_kernel<kCsrLanes, TV, T>), dim3(o.csr.grid), dim3(kThreads), 0, s,
                            S->rowptr.get(), S->col.get(), csr_vals<TV>(S), S->nf, o.csr.per_block, p, q, st,
                            partials);
+    } else {
+        fail(SFEM_ERR_UNSUPPORTED, "split precision needs the node-block format");
     }
     SFEM_LAUNCH_CHECK();
 }
@@ -680,7 +688,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     stats->spmv_ms = spmv_ms;
     stats->spmv_timed = timed;
     // spmv_cg: matrix once, p read once, q written
-    stats->spmv_bytes = spmv_bytes_for(S, fmt, sizeof(TV), sizeof(T), 2);
+    stats->spmv_bytes = spmv_bytes_for(S, fmt, val_bytes<TV>(), sizeof(T), 2);
     stats->converged = host.done == 1 ? 1 : 0;
     stats->format = fmt;
     stats->stop_reason = host.done == 0 ? 4 : host.done;
@@ -707,6 +715,10 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
         case SFEM_PREC_FP32_MATRIX:
             if (fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
             run_pcg<float, double>(ctx, S, o, st, fmt);
+            break;
+        case SFEM_PREC_SPLIT:
+            if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "split precision needs the node-block format");
+            run_pcg<SplitVal, double>(ctx, S, o, st, fmt);
             break;
         default: fail(SFEM_ERR_ARGUMENT, "solve: unknown precision");
     }
@@ -747,7 +759,7 @@ double spmv_bench_impl(sfem_ctx* ctx, sfem_system* S, int fmt, int reps, bool nt
                 if (nt) go(sell_spmv_kernel<2, TV, T, true>);
                 else go(sell_spmv_kernel<2, TV, T, false>);
             }
-        } else {
+        } else if constexpr (!std::is_same<TV, SplitVal>::value) {
             hipLaunchKernelGGL((csr_spmv_kernel<kCsrLanes, TV, T>), dim3(lc.grid), dim3(kThreads), 0, s,
                                S->rowptr.get(), S->col.get(), csr_vals<TV>(S), nf, lc.per_block, x.get(), y.get());
         }
@@ -785,6 +797,11 @@ void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format,
         case SFEM_PREC_FP32_MATRIX:
             *avg_ms = spmv_bench_impl<float, double>(ctx, S, fmt, reps, ctx->tune_nt);
             *bytes = spmv_bytes_for(S, fmt, 4, 8, 2);
+            break;
+        case SFEM_PREC_SPLIT:
+            if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "split precision needs the node-block format");
+            *avg_ms = spmv_bench_impl<SplitVal, double>(ctx, S, fmt, reps, ctx->tune_nt);
+            *bytes = spmv_bytes_for(S, fmt, 6, 8, 2);
             break;
         default: fail(SFEM_ERR_ARGUMENT, "spmv bench: unknown precision");
     }

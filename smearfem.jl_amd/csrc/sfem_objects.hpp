@@ -129,6 +129,7 @@ struct sfem_system {
     sfem::DevBuf<int32_t> scol;    // sell_blocks
     sfem::DevBuf<double> sval64;   // sell_blocks * bs*bs, planes of 64
     sfem::DevBuf<float> sval32;
+    sfem::DevBuf<int16_t> sval16;  // correction of sval32 toward sval64 (split precision)
     double t_eliminate_ms = 0;
 };
 

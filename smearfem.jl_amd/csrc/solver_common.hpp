@@ -130,11 +130,39 @@ __device__ __forceinline__ V ld_stream(const V* p) {
     else return *p;
 }
 
+// Operator value sources of the sliced format.  Vals<double> / Vals<float>
+// read one plane array; Vals<SplitVal> reads the FP32 rounding of each FP64
+// value plus a 16-bit correction and rebuilds v = hi + lo * ulp(hi) * 2^-16
+// exactly in FP64 (|v' - v| <= 2^-39 |v|): 6 bytes per value.
+struct SplitVal {};
+
+template <typename TV>
+struct Vals {
+    const TV* v;
+    template <bool NT>
+    __device__ __forceinline__ TV get(int64_t i) const { return ld_stream<NT>(v + i); }
+};
+
+template <>
+struct Vals<SplitVal> {
+    const float* hi;
+    const int16_t* lo;
+    template <bool NT>
+    __device__ __forceinline__ double get(int64_t i) const {
+        const float h = ld_stream<NT>(hi + i);
+        const int16_t l = ld_stream<NT>(lo + i);
+        // ulp(h) * 2^-16 = 2^(exponent(h) - 23 - 16); below the normal range
+        // (|h| < 2^-87, where lo is 0 by construction) the scale is 0
+        const float scale = __int_as_float(max((__float_as_int(h) & 0x7F800000) - (39 << 23), 0));
+        return fma((double)l, (double)scale, (double)h);
+    }
+};
+
 // Node-block sliced format: this lane's block row of slice `sl`.
 template <int BS, typename TV, typename ACC, bool NT = false, typename Gather>
 __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
-                                         const int32_t* __restrict__ scol, const TV* __restrict__ sval, int64_t sl,
-                                         int lane, Gather gx, ACC out[BS]) {
+                                         const int32_t* __restrict__ scol, Vals<TV> vals, int64_t sl, int lane,
+                                         Gather gx, ACC out[BS]) {
     constexpr int B2 = BS * BS;
     const int64_t base = sl_ptr[sl];
     const int w = sl_w[sl];
@@ -147,11 +175,12 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
         ACC xv[BS];
 #pragma unroll
         for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
-        const TV* v = sval + slot * B2 + lane;
+        const int64_t v = slot * B2 + lane;
 #pragma unroll
         for (int j = 0; j < BS; ++j)
 #pragma unroll
-            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)ld_stream<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
+            for (int i = 0; i < BS; ++i)
+                out[j] = fmaT<ACC>((ACC)vals.template get<NT>(v + (j * BS + i) * 64), xv[i], out[j]);
     }
 }
 

@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from .fem import Context, StiffnessMatrix, _ptr, default_context
 
-PRECISIONS = {"fp64": 0, "mixed": 1, "fp32_matrix": 2}
+PRECISIONS = {"fp64": 0, "mixed": 1, "fp32_matrix": 2, "split": 3}
 FORMATS = {"auto": 0, "csr": 1, "bsr": 2}
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 

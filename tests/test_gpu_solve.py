@@ -42,7 +42,7 @@ def oracle_direct(m, fixed, f, q_d=None, young=1.0, nu=0.3, coords=None):
 
 
 @pytest.mark.parametrize("precision,fmt", [("fp64", "csr"), ("fp64", "bsr"), ("mixed", "bsr"), ("mixed", "csr"),
-                                           ("fp32_matrix", "bsr"), ("fp32_matrix", "csr")])
+                                           ("fp32_matrix", "bsr"), ("fp32_matrix", "csr"), ("split", "bsr")])
 def test_cfg1_q4_plane_strain_cantilever(ctx, precision, fmt):
     # unit square 32x32, E=1, nu=0.3 plane strain via (E/(1-nu^2), nu/(1-nu)),
     # left edge clamped, unit traction (+x) on the right edge
@@ -64,7 +64,7 @@ def test_cfg1_q4_plane_strain_cantilever(ctx, precision, fmt):
     assert np.abs(R[fixed == 0]).max() == 0.0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32_matrix"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32_matrix", "split"])
 def test_hex8_cantilever_per_element_young(ctx, precision):
     print(f"[case] hex8 12x4x4 {precision}", flush=True)
     m = structured_mesh((12, 4, 4), lengths=(3.0, 1.0, 1.0))
@@ -90,7 +90,8 @@ def test_cfg2_against_direct_fixture(ctx):
     S = System(K, bc)
     bc.close()
     assert S.n_free == gold["n_free"] and S.nnz == gold["nnz_Kff"]
-    for precision, fmt in (("fp64", "csr"), ("fp64", "bsr"), ("mixed", "bsr"), ("fp32_matrix", "bsr")):
+    for precision, fmt in (("fp64", "csr"), ("fp64", "bsr"), ("mixed", "bsr"), ("fp32_matrix", "bsr"),
+                           ("split", "bsr")):
         print(f"[case] cfg2 {precision} {fmt}", flush=True)
         st = S.solve(precision, fmt, rtol=1e-10)
         q, R = S.solution(reactions=True)
@@ -118,7 +119,7 @@ def test_patch_test_prescribed_linear_field(ctx):
     u_lin = (A @ coords).T.reshape(-1)
     fixed = np.repeat(~interior, 3).astype(np.uint8)
     K = device_K(ctx, m, coords=coords)
-    for precision in ("fp64", "mixed", "fp32_matrix"):
+    for precision in ("fp64", "mixed", "fp32_matrix", "split"):
         q = solve_system(K, None, fixed, u_lin, precision=precision, rtol=1e-13)
         assert np.abs(q - u_lin).max() <= 1e-10 * np.abs(u_lin).max()
     K.close()
@@ -141,6 +142,8 @@ def test_partial_dof_clamp_uses_scalar_path(ctx):
     S = System(K, bc)
     bc.close()
     assert S.block == 1  # a node with some dofs fixed: the block format does not apply
+    with pytest.raises(smearfem.UnsupportedError):
+        S.solve("split", rtol=1e-12)  # the split operator exists only in the block format
     st = S.solve("fp64", rtol=1e-12)
     q = S.solution()
     q_ref, _ = oracle_direct(m, fixed, None, q_d, young=40.0, nu=0.4)
@@ -154,7 +157,7 @@ def test_solve_bitwise_reproducible(ctx):
     fixed = clamp_x0(m)
     f = random_face_load(m, 2, 1, fixed, seed=1)
     K = device_K(ctx, m, element_young_loguniform(m.n_elem, 0.1, 10.0, seed=2))
-    for precision in ("fp64", "mixed", "fp32_matrix"):
+    for precision in ("fp64", "mixed", "fp32_matrix", "split"):
         a, sa = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True)
         b, sb = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True)
         assert np.array_equal(a, b) and sa["iterations"] == sb["iterations"]
