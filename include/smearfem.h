@@ -169,7 +169,11 @@ typedef struct {
     int     check_every;     /* iterations enqueued between host convergence polls      */
     int     time_kernels;    /* 1: time every SpMV launch with HIP events                */
     int     x0_zero;         /* 1: start from x = 0 (else from the system's current x)   */
+    int     preconditioner;  /* SFEM_PC_*                                                */
 } sfem_solve_opts;
+
+#define SFEM_PC_JACOBI        0   /* z = r / diag(K_ff)                                   */
+#define SFEM_PC_BLOCK_JACOBI  1   /* z = (node diagonal block)^-1 r; node-block format only */
 
 typedef struct {
     int64_t iterations;      /* CG iterations performed                                  */

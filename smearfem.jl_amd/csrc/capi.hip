@@ -544,6 +544,7 @@ void sfem_solve_opts_default(sfem_solve_opts* o) {
     o->check_every = 64;
     o->time_kernels = 0;
     o->x0_zero = 1;
+    o->preconditioner = SFEM_PC_JACOBI;
 }
 
 int sfem_system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* opts, sfem_solve_stats* stats) {

@@ -129,12 +129,50 @@ __device__ __forceinline__ int16_t split_correction(double v) {
     return (int16_t)fmin(fmax(t, -32767.0), 32767.0);
 }
 
+// Inverse of a node's diagonal block M (M[j][i] = m[j*BS+i]), symmetric by
+// construction (upper triangle mirrored).  A singular block falls back to
+// the inverse of its diagonal (1 where that is 0).
+template <int BS>
+__device__ void invert_block(const double* m, double* inv) {
+    double det = 0.0;
+    if (BS == 3) {
+        const double a = m[0], b = m[1], c = m[2], e = m[4], f = m[5], i = m[8];
+        const double A = e * i - f * f, B = c * f - b * i, C = b * f - c * e;
+        det = a * A + b * B + c * C;
+        if (det != 0.0 && isfinite(det)) {
+            const double r = 1.0 / det;
+            inv[0] = A * r;
+            inv[1] = inv[3] = B * r;
+            inv[2] = inv[6] = C * r;
+            inv[4] = (a * i - c * c) * r;
+            inv[5] = inv[7] = (b * c - a * f) * r;
+            inv[8] = (a * e - b * b) * r;
+            return;
+        }
+    } else if (BS == 2) {
+        const double a = m[0], b = m[1], d = m[3];
+        det = a * d - b * b;
+        if (det != 0.0 && isfinite(det)) {
+            const double r = 1.0 / det;
+            inv[0] = d * r;
+            inv[1] = inv[2] = -b * r;
+            inv[3] = a * r;
+            return;
+        }
+    }
+    for (int j = 0; j < BS; ++j)
+        for (int i = 0; i < BS; ++i) inv[j * BS + i] = (i == j && m[j * BS + j] != 0.0) ? 1.0 / m[j * BS + j] : 0.0;
+    for (int j = 0; j < BS; ++j)
+        if (inv[j * BS + j] == 0.0) inv[j * BS + j] = 1.0;
+}
+
 template <int BS>
 __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
                                  const double* __restrict__ bval, const int32_t* __restrict__ ufree_of,
                                  int64_t nunits, const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                  int32_t* __restrict__ scol, double* __restrict__ sval64,
-                                 float* __restrict__ sval32, int16_t* __restrict__ sval16) {
+                                 float* __restrict__ sval32, int16_t* __restrict__ sval16, int64_t nfb,
+                                 double* __restrict__ binv64, float* __restrict__ binv32) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -159,6 +197,19 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
                 sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
                 sval16[slot * B2 + (j * BS + i) * 64 + lane] = split_correction(v);
             }
+        if (c == row) {  // diagonal block: its inverse for block-Jacobi preconditioning
+            double m[B2], inv[B2];
+#pragma unroll
+            for (int j = 0; j < BS; ++j)
+#pragma unroll
+                for (int i = 0; i < BS; ++i) m[j * BS + i] = bval[u * B2 + i * BS + j];
+            invert_block<BS>(m, inv);
+#pragma unroll
+            for (int v = 0; v < B2; ++v) {
+                binv64[v * nfb + row] = inv[v];
+                binv32[v * nfb + row] = (float)inv[v];
+            }
+        }
         ++k;
     }
     for (; k < sl_w[s]; ++k) {  // padding slots: own column, zero values
@@ -339,10 +390,13 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             S->sval64.reset(ctx, (size_t)total * bs * bs);
             S->sval32.reset(ctx, (size_t)total * bs * bs);
             S->sval16.reset(ctx, (size_t)total * bs * bs);
+            S->binv64.reset(ctx, (size_t)nfb * bs * bs);
+            S->binv32.reset(ctx, (size_t)nfb * bs * bs);
             if (bs == 3) {
                 hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get());
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
+                                   (int64_t)nfb, S->binv64.get(), S->binv32.get());
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
                                    S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
@@ -350,7 +404,8 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             } else {
                 hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get());
+                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
+                                   (int64_t)nfb, S->binv64.get(), S->binv32.get());
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
                                    S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),

@@ -127,6 +127,37 @@ __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
     }
 }
 
+// Last block of an update: beta, the replacement triggers and the stopping
+// backstops from the partial sums of r.z and r.r.
+__device__ __forceinline__ void cg_update_finish(CgState* st, double* partials, double* sh) {
+    double trz, trr, u;
+    sum_partials3(partials, gridDim.x, &trz, &trr, &u, sh);
+    if (threadIdx.x == 0) {
+        st->rz_prev = st->rz;
+        st->rz = trz;
+        st->beta = trz / st->rz_prev;
+        st->rr = trr;
+        st->iters += 1;
+        st->fold_pending = 0;
+        st->zcoef = 1.0;
+        // replacement when the recurrence converged, dropped by delta, or
+        // made no such drop for max(256, iteration of the best true residual)
+        const long long since = st->iters - st->last_upd;
+        const long long every = st->best_true_it > 256 ? st->best_true_it : 256;
+        if (trr <= st->tol2bb || trr < st->delta2 * st->maxrr || since >= every) st->need_update = 1;
+        // backstop for a recurrence that stops decreasing: no 2x drop of |r|
+        // within max(10000, 4x the iterations it took to reach the best)
+        if (trr < 0.25 * st->best_rr) {
+            st->best_rr = trr;
+            st->best_it = st->iters;
+        } else if (st->iters - st->best_it > (4 * st->best_it > 10000 ? 4 * st->best_it : 10000)) {
+            st->done = 3;
+        }
+        if (!std::isfinite(trr)) st->done = 2;
+        st->counter = 0;
+    }
+}
+
 // ----------------------------------------------------------- update ----
 // x += alpha p, r -= alpha q, z = D^-1 r, partial r.z and r.r.  With FOLD
 // (FP32 solution increment xs) a pending fold moves xs into x64 first.  The
@@ -165,34 +196,56 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
         st_coherent(partials + 3 * blockIdx.x, b0);
         st_coherent(partials + 3 * blockIdx.x + 1, b1);
     }
-    if (last_block_arrive(&st->counter)) {
-        double trz, trr, u;
-        sum_partials3(partials, gridDim.x, &trz, &trr, &u, sh);
-        if (threadIdx.x == 0) {
-            st->rz_prev = st->rz;
-            st->rz = trz;
-            st->beta = trz / st->rz_prev;
-            st->rr = trr;
-            st->iters += 1;
-            st->fold_pending = 0;
-            st->zcoef = 1.0;
-            // replacement when the recurrence converged, dropped by delta, or
-            // made no such drop for max(256, iteration of the best true residual)
-            const long long since = st->iters - st->last_upd;
-            if (trr <= st->tol2bb || trr < st->delta2 * st->maxrr || since >= (st->best_true_it > 256 ? st->best_true_it : 256))
-                st->need_update = 1;
-            // backstop for a recurrence that stops decreasing: no 2x drop of |r|
-            // within max(10000, 4x the iterations it took to reach the best)
-            if (trr < 0.25 * st->best_rr) {
-                st->best_rr = trr;
-                st->best_it = st->iters;
-            } else if (st->iters - st->best_it > (4 * st->best_it > 10000 ? 4 * st->best_it : 10000)) {
-                st->done = 3;
+    if (last_block_arrive(&st->counter)) cg_update_finish(st, partials, sh);
+}
+
+// Node-block Jacobi form of cg_update: one thread per node; z = B^-1 r with
+// B^-1 the node's inverse diagonal block (planar: binv[(j*BS+i)*nnodes + n]).
+template <int BS, typename T, bool FOLD>
+__global__ __launch_bounds__(kThreads) void cg_update_block_kernel(T* __restrict__ x, double* __restrict__ x64,
+                                                                   T* __restrict__ r, T* __restrict__ z,
+                                                                   const T* __restrict__ p, const T* __restrict__ q,
+                                                                   const T* __restrict__ binv, int64_t nnodes,
+                                                                   int64_t per_block, CgState* st,
+                                                                   double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const T a = (T)st->alpha;
+    const bool fold = FOLD && st->fold_pending;
+    const int64_t n0 = (int64_t)blockIdx.x * per_block;
+    const int64_t n1 = min(nnodes, n0 + per_block);
+    double rz = 0.0, rr = 0.0;
+    for (int64_t n = n0 + threadIdx.x; n < n1; n += kThreads) {
+        T rv[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            const int64_t i = n * BS + j;
+            T xi = x[i];
+            if (fold) {
+                x64[i] += (double)xi;
+                xi = 0;
             }
-            if (!std::isfinite(trr)) st->done = 2;
-            st->counter = 0;
+            x[i] = fmaT<T>(a, p[i], xi);
+            rv[j] = fmaT<T>(-a, q[i], r[i]);
+            r[i] = rv[j];
+            rr += (double)rv[j] * (double)rv[j];
+        }
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            T zj = 0;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) zj = fmaT<T>(binv[(j * BS + i) * nnodes + n], rv[i], zj);
+            z[n * BS + j] = zj;
+            rz += (double)rv[j] * (double)zj;
         }
     }
+    const double b0 = block_sum_256(rz, sh);
+    const double b1 = block_sum_256(rr, sh + 4);
+    if (threadIdx.x == 0) {
+        st_coherent(partials + 3 * blockIdx.x, b0);
+        st_coherent(partials + 3 * blockIdx.x + 1, b1);
+    }
+    if (last_block_arrive(&st->counter)) cg_update_finish(st, partials, sh);
 }
 
 // ------------------------------------------- FP64 residual replacement --
@@ -301,13 +354,15 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
     if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
 }
 
-template <int BS, typename T, bool FOLD>
+// BPC: z = B^-1 r with the node's inverse diagonal block (binv, planar with
+// stride nrows) instead of the point Jacobi dinv.
+template <int BS, typename T, bool FOLD, bool BPC>
 __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
     const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w, const int32_t* __restrict__ scol,
     const double* __restrict__ sval, int64_t nslices, int64_t nrows, int64_t slices_per_block,
     const double* __restrict__ x64, const T* __restrict__ xs, const double* __restrict__ rhs,
-    const T* __restrict__ dinv, const T* __restrict__ p, T* __restrict__ r, T* __restrict__ z, CgState* st,
-    double* __restrict__ partials) {
+    const T* __restrict__ dinv, const T* __restrict__ binv, const T* __restrict__ p, T* __restrict__ r,
+    T* __restrict__ z, CgState* st, double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done || !st->need_update) return;
     const int64_t s0 = logical_block() * slices_per_block;
@@ -320,19 +375,31 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
         sell_row_abs<BS>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out, aout);
         const int64_t row = sl * 64 + lane;
         if (row < nrows) {
+            T rt[BS];
 #pragma unroll
             for (int j = 0; j < BS; ++j) {
                 const int64_t d = row * BS + j;
                 const double res = rhs[d] - out[j];
                 const double sc = fabs(rhs[d]) + aout[j];
                 ff += sc * sc;
-                const T rt = (T)res;
-                const T zt = dinv[d] * rt;
-                r[d] = rt;
-                z[d] = zt;
+                rt[j] = (T)res;
+                r[d] = rt[j];
                 rr += res * res;
-                rz += (double)rt * (double)zt;
-                rp += (double)rt * (double)p[d];
+                rp += (double)rt[j] * (double)p[d];
+            }
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const int64_t d = row * BS + j;
+                T zt;
+                if constexpr (BPC) {
+                    zt = 0;
+#pragma unroll
+                    for (int i = 0; i < BS; ++i) zt = fmaT<T>(binv[(j * BS + i) * nrows + row], rt[i], zt);
+                } else {
+                    zt = dinv[d] * rt[j];
+                }
+                z[d] = zt;
+                rz += (double)rt[j] * (double)zt;
             }
         }
     }
@@ -481,7 +548,9 @@ struct Ops {
     int fmt;  // SFEM_FMT_CSR / SFEM_FMT_BSR
     unsigned cap;
     Launch csr, sell, vec;
-    bool nt;  // non-temporal matrix loads in the sliced SpMV
+    bool nt;        // non-temporal matrix loads in the sliced SpMV
+    bool block_pc;  // node-block Jacobi preconditioner
+    Launch node;    // one thread per node (block update)
 };
 
 template <typename TV>
@@ -531,18 +600,23 @@ void launch_spmv_cg(const Ops& o, const T* p, T* q, CgState* st, double* partial
 }
 
 template <typename T, bool FOLD>
-void launch_resid(const Ops& o, const T* xs, const T* dinv, const T* p, T* r, T* z, CgState* st, double* partials) {
+void launch_resid(const Ops& o, const T* xs, const T* dinv, const T* binv, const T* p, T* r, T* z, CgState* st,
+                  double* partials) {
     sfem_system* S = o.S;
     hipStream_t s = o.ctx->stream;
     if (o.fmt == SFEM_FMT_BSR) {
-        if (S->bs == 3)
-            hipLaunchKernelGGL((sell_resid_kernel<3, T, FOLD>), dim3(o.sell.grid), dim3(kThreads), 0, s,
-                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb,
-                               o.sell.per_block, S->x.get(), xs, S->rhs.get(), dinv, p, r, z, st, partials);
-        else
-            hipLaunchKernelGGL((sell_resid_kernel<2, T, FOLD>), dim3(o.sell.grid), dim3(kThreads), 0, s,
-                               S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb,
-                               o.sell.per_block, S->x.get(), xs, S->rhs.get(), dinv, p, r, z, st, partials);
+        auto go = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(o.sell.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
+                               S->scol.get(), S->sval64.get(), S->nslices, S->nfb, o.sell.per_block, S->x.get(), xs,
+                               S->rhs.get(), dinv, binv, p, r, z, st, partials);
+        };
+        if (S->bs == 3) {
+            if (o.block_pc) go(sell_resid_kernel<3, T, FOLD, true>);
+            else go(sell_resid_kernel<3, T, FOLD, false>);
+        } else {
+            if (o.block_pc) go(sell_resid_kernel<2, T, FOLD, true>);
+            else go(sell_resid_kernel<2, T, FOLD, false>);
+        }
     } else {
         hipLaunchKernelGGL((csr_resid_kernel<kCsrLanes, T, FOLD>), dim3(o.csr.grid), dim3(kThreads), 0, s,
                            S->rowptr.get(), S->col.get(), S->val64.get(), S->nf, o.csr.per_block, S->x.get(), xs,
@@ -580,17 +654,23 @@ template <typename TV, typename T>
 void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* stats, int fmt) {
     constexpr bool FOLD = std::is_same<T, float>::value;
     const int64_t nf = S->nf;
-    Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt};
+    const bool block_pc = o->preconditioner == SFEM_PC_BLOCK_JACOBI;
+    if (block_pc && fmt != SFEM_FMT_BSR)
+        fail(SFEM_ERR_UNSUPPORTED, "block-Jacobi preconditioning needs the node-block format");
+    if (o->preconditioner != SFEM_PC_JACOBI && !block_pc) fail(SFEM_ERR_ARGUMENT, "solve: unknown preconditioner");
+    Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt, block_pc, {}};
+    if (block_pc) ops.node = plan(S->nfb, kThreads, std::max(8, ctx->n_cu * 4));
     ops.csr = plan(nf, kThreads / kCsrLanes, ops.cap);
     if (S->has_block) ops.sell = plan_sell(S->nslices, ops.cap, ctx->tune_wave_slices);
     ops.vec = plan(nf, kThreads, std::max(8, ctx->n_cu * 4));
-    const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid});
+    const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid, ops.node.grid});
 
     DevBuf<T> r(ctx, nf), z(ctx, nf), q(ctx, nf), pv(ctx, nf);
     DevBuf<T> xsbuf;
     DevBuf<double> partials(ctx, 4 * (size_t)maxgrid);
     DevBuf<CgState> dst(ctx, 1);
     const T* dinv = FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get();
+    const T* binv = FOLD ? (const T*)S->binv32.get() : (const T*)S->binv64.get();
     T* x = (T*)S->x.get();  // FP64 paths update the FP64 solution directly
     if (FOLD) {
         xsbuf.reset(ctx, nf);
@@ -624,7 +704,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
                        ops.vec.per_block, st, partials.get(), o->rtol, o->reliable_delta);
     SFEM_LAUNCH_CHECK();
     T* p = pv.get();
-    launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
+    launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
 
     CgState host{};
     int64_t enq = 0, counted = 0, timed = 0;
@@ -639,11 +719,20 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
             if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k], ctx->stream));
             launch_spmv_cg<TV, T>(ops, p, q.get(), st, partials.get());
             if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k + 1], ctx->stream));
-            hipLaunchKernelGGL((cg_update_kernel<T, FOLD>), dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x,
-                               S->x.get(), r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
-                               partials.get());
+            if (!block_pc)
+                hipLaunchKernelGGL((cg_update_kernel<T, FOLD>), dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream,
+                                   x, S->x.get(), r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
+                                   partials.get());
+            else if (S->bs == 3)
+                hipLaunchKernelGGL((cg_update_block_kernel<3, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
+                                   ctx->stream, x, S->x.get(), r.get(), z.get(), p, q.get(), binv, S->nfb,
+                                   ops.node.per_block, st, partials.get());
+            else
+                hipLaunchKernelGGL((cg_update_block_kernel<2, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
+                                   ctx->stream, x, S->x.get(), r.get(), z.get(), p, q.get(), binv, S->nfb,
+                                   ops.node.per_block, st, partials.get());
             SFEM_LAUNCH_CHECK();
-            launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
+            launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
         }
         enq += chunk;
         d2h(ctx, &host, st, sizeof(host));
@@ -663,7 +752,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
         CgState h2 = host;
         h2.need_update = 1;
         h2d(ctx, st, &h2, sizeof(h2));
-        launch_resid<T, FOLD>(ops, xs, dinv, p, r.get(), z.get(), st, partials.get());
+        launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
         d2h(ctx, &host, st, sizeof(host));
     }
     if (FOLD) {  // x = x64 + xs

@@ -130,6 +130,8 @@ struct sfem_system {
     sfem::DevBuf<double> sval64;   // sell_blocks * bs*bs, planes of 64
     sfem::DevBuf<float> sval32;
     sfem::DevBuf<int16_t> sval16;  // correction of sval32 toward sval64 (split precision)
+    sfem::DevBuf<double> binv64;   // inverse node diagonal blocks, planar: [(j*bs+i)*nfb + node]
+    sfem::DevBuf<float> binv32;
     double t_eliminate_ms = 0;
 };
 

@@ -87,6 +87,7 @@ mutable struct SolveOpts
     check_every::Cint
     time_kernels::Cint
     x0_zero::Cint
+    preconditioner::Cint
 end
 
 mutable struct SolveStats
@@ -124,7 +125,7 @@ function solve_system(K::SparseMatrixCSC{Float64,Int64}, f, fixed, q_d; precisio
     fv = f === nothing ? zeros(n) : Vector{Float64}(vec(f))
     mask = Vector{UInt8}(fixed .!= 0)
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
-    opts = SolveOpts(precision == :mixed ? 1 : 0, 0, rtol, max_iter, 0.1, 64, 0, 1)
+    opts = SolveOpts(precision == :mixed ? 1 : 0, 0, rtol, max_iter, 0.1, 64, 0, 1, 0)
     stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
     q = zeros(n)
     try

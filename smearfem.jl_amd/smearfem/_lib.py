@@ -29,6 +29,7 @@ class SolveOpts(C.Structure):
         ("check_every", C.c_int),
         ("time_kernels", C.c_int),
         ("x0_zero", C.c_int),
+        ("preconditioner", C.c_int),
     ]
 
 

@@ -23,6 +23,7 @@ from . import _lib
 from .fem import Context, StiffnessMatrix, _ptr, default_context
 
 PRECISIONS = {"fp64": 0, "mixed": 1, "fp32_matrix": 2, "split": 3}
+PRECONDITIONERS = {"jacobi": 0, "block_jacobi": 1}
 FORMATS = {"auto": 0, "csr": 1, "bsr": 2}
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 
@@ -77,7 +78,8 @@ class System:
         self.block = bs.value
 
     def solve(self, precision="fp64", fmt="auto", rtol=1e-10, max_iter=200000, reliable_delta=0.1,
-              check_every=64, time_kernels=False, x0_zero=True, raise_on_fail=True) -> dict:
+              check_every=64, time_kernels=False, x0_zero=True, raise_on_fail=True,
+              preconditioner="jacobi") -> dict:
         lib = self.ctx._lib
         o = _lib.SolveOpts()
         lib.sfem_solve_opts_default(C.byref(o))
@@ -89,11 +91,13 @@ class System:
         o.check_every = int(check_every)
         o.time_kernels = 1 if time_kernels else 0
         o.x0_zero = 1 if x0_zero else 0
+        o.preconditioner = PRECONDITIONERS[preconditioner]
         s = _lib.SolveStats()
         st = lib.sfem_system_solve(self.ctx.handle, self.handle, C.byref(o), C.byref(s))
         stats = {name: getattr(s, name) for name, _ in _lib.SolveStats._fields_}
         stats["format"] = FORMAT_NAMES.get(stats["format"], stats["format"])
         stats["precision"] = precision
+        stats["preconditioner"] = preconditioner
         if st and raise_on_fail:
             self.ctx.check(st)
         stats["status"] = st
@@ -125,7 +129,8 @@ class System:
 
 
 def solve_system(K: StiffnessMatrix, f=None, fixed=None, q_d=None, *, precision="fp64", fmt="auto",
-                 rtol=1e-10, max_iter=200000, reliable_delta=0.1, reactions=False, return_stats=False):
+                 rtol=1e-10, max_iter=200000, reliable_delta=0.1, reactions=False, return_stats=False,
+                 preconditioner="jacobi"):
     """q = q_d + C K_ff^{-1} C'(f - K q_d) on the GPU for a device matrix K."""
     bc = BoundaryConditions(K.ctx, K.shape[0], f, fixed, q_d)
     try:
@@ -133,7 +138,7 @@ def solve_system(K: StiffnessMatrix, f=None, fixed=None, q_d=None, *, precision=
     finally:
         bc.close()
     try:
-        stats = S.solve(precision, fmt, rtol, max_iter, reliable_delta)
+        stats = S.solve(precision, fmt, rtol, max_iter, reliable_delta, preconditioner=preconditioner)
         out = S.solution(reactions=reactions)
     finally:
         S.close()

@@ -41,19 +41,22 @@ def oracle_direct(m, fixed, f, q_d=None, young=1.0, nu=0.3, coords=None):
     return O.recombine(fixed, q_d, u), K
 
 
-@pytest.mark.parametrize("precision,fmt", [("fp64", "csr"), ("fp64", "bsr"), ("mixed", "bsr"), ("mixed", "csr"),
-                                           ("fp32_matrix", "bsr"), ("fp32_matrix", "csr"), ("split", "bsr")])
-def test_cfg1_q4_plane_strain_cantilever(ctx, precision, fmt):
+@pytest.mark.parametrize("precision,fmt,pc", [("fp64", "csr", "jacobi"), ("fp64", "bsr", "jacobi"),
+                                              ("mixed", "bsr", "jacobi"), ("mixed", "csr", "jacobi"),
+                                              ("fp32_matrix", "bsr", "jacobi"), ("fp32_matrix", "csr", "jacobi"),
+                                              ("split", "bsr", "jacobi"), ("fp64", "bsr", "block_jacobi"),
+                                              ("mixed", "bsr", "block_jacobi"), ("split", "bsr", "block_jacobi")])
+def test_cfg1_q4_plane_strain_cantilever(ctx, precision, fmt, pc):
     # unit square 32x32, E=1, nu=0.3 plane strain via (E/(1-nu^2), nu/(1-nu)),
     # left edge clamped, unit traction (+x) on the right edge
-    print(f"[case] cfg1 {precision} {fmt}", flush=True)
+    print(f"[case] cfg1 {precision} {fmt} {pc}", flush=True)
     m = structured_mesh((32, 32))
     E, nu = 1.0 / (1 - 0.09), 0.3 / 0.7
     fixed = clamp_x0(m)
     f = face_load(m, 0, 1, 0, 1.0)
     K = device_K(ctx, m, E, nu)
     q, R, st = solve_system(K, f, fixed, precision=precision, fmt=fmt, rtol=1e-12, reactions=True,
-                            return_stats=True)
+                            return_stats=True, preconditioner=pc)
     K.close()
     q_ref, Kref = oracle_direct(m, fixed, f, young=E, nu=nu)
     assert st["converged"] and st["rel_residual"] <= 1e-12
@@ -64,15 +67,17 @@ def test_cfg1_q4_plane_strain_cantilever(ctx, precision, fmt):
     assert np.abs(R[fixed == 0]).max() == 0.0
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32_matrix", "split"])
-def test_hex8_cantilever_per_element_young(ctx, precision):
-    print(f"[case] hex8 12x4x4 {precision}", flush=True)
+@pytest.mark.parametrize("precision,pc", [("fp64", "jacobi"), ("mixed", "jacobi"), ("fp32_matrix", "jacobi"),
+                                          ("split", "jacobi"), ("fp64", "block_jacobi"), ("mixed", "block_jacobi"),
+                                          ("split", "block_jacobi")])
+def test_hex8_cantilever_per_element_young(ctx, precision, pc):
+    print(f"[case] hex8 12x4x4 {precision} {pc}", flush=True)
     m = structured_mesh((12, 4, 4), lengths=(3.0, 1.0, 1.0))
     young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=7)
     fixed = clamp_x0(m)
     f = random_face_load(m, 2, 1, fixed, seed=3)
     K = device_K(ctx, m, young)
-    q, st = solve_system(K, f, fixed, precision=precision, rtol=1e-12, return_stats=True)
+    q, st = solve_system(K, f, fixed, precision=precision, rtol=1e-12, return_stats=True, preconditioner=pc)
     K.close()
     q_ref, _ = oracle_direct(m, fixed, f, young=young)
     assert st["converged"]
@@ -90,10 +95,11 @@ def test_cfg2_against_direct_fixture(ctx):
     S = System(K, bc)
     bc.close()
     assert S.n_free == gold["n_free"] and S.nnz == gold["nnz_Kff"]
-    for precision, fmt in (("fp64", "csr"), ("fp64", "bsr"), ("mixed", "bsr"), ("fp32_matrix", "bsr"),
-                           ("split", "bsr")):
-        print(f"[case] cfg2 {precision} {fmt}", flush=True)
-        st = S.solve(precision, fmt, rtol=1e-10)
+    for precision, fmt, pc in (("fp64", "csr", "jacobi"), ("fp64", "bsr", "jacobi"), ("mixed", "bsr", "jacobi"),
+                               ("fp32_matrix", "bsr", "jacobi"), ("split", "bsr", "jacobi"),
+                               ("split", "bsr", "block_jacobi"), ("mixed", "bsr", "block_jacobi")):
+        print(f"[case] cfg2 {precision} {fmt} {pc}", flush=True)
+        st = S.solve(precision, fmt, rtol=1e-10, preconditioner=pc)
         q, R = S.solution(reactions=True)
         u = q[fixed == 0]
         assert abs(np.linalg.norm(u) - gold["norm_u"]) <= ERR * gold["norm_u"]
@@ -144,6 +150,8 @@ def test_partial_dof_clamp_uses_scalar_path(ctx):
     assert S.block == 1  # a node with some dofs fixed: the block format does not apply
     with pytest.raises(smearfem.UnsupportedError):
         S.solve("split", rtol=1e-12)  # the split operator exists only in the block format
+    with pytest.raises(smearfem.UnsupportedError):
+        S.solve("fp64", rtol=1e-12, preconditioner="block_jacobi")
     st = S.solve("fp64", rtol=1e-12)
     q = S.solution()
     q_ref, _ = oracle_direct(m, fixed, None, q_d, young=40.0, nu=0.4)
@@ -157,9 +165,10 @@ def test_solve_bitwise_reproducible(ctx):
     fixed = clamp_x0(m)
     f = random_face_load(m, 2, 1, fixed, seed=1)
     K = device_K(ctx, m, element_young_loguniform(m.n_elem, 0.1, 10.0, seed=2))
-    for precision in ("fp64", "mixed", "fp32_matrix", "split"):
-        a, sa = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True)
-        b, sb = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True)
+    for precision, pc in (("fp64", "jacobi"), ("mixed", "jacobi"), ("fp32_matrix", "jacobi"), ("split", "jacobi"),
+                          ("split", "block_jacobi"), ("mixed", "block_jacobi")):
+        a, sa = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True, preconditioner=pc)
+        b, sb = solve_system(K, f, fixed, precision=precision, rtol=1e-10, return_stats=True, preconditioner=pc)
         assert np.array_equal(a, b) and sa["iterations"] == sb["iterations"]
     K.close()
 
