@@ -42,7 +42,7 @@ ERR_BAR = 1e-8
 # HBM bytes per CG-SpMV launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.00, write
 # x1.00); profiles/traffic_spmv_cg_hex8_256x64x64.md has the commands and data.
-MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9}
+MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9, "split/bsr": 1.7175e9}
 
 
 def log(msg):
@@ -233,10 +233,12 @@ def run_batch(ctx, nprob=512, seed=4):
     out = {"problems": nprob, "seed": seed, "n_free": n_free, "build_ms": t_build * 1e3,
            "assemble_ms": B.assemble_ms, "eliminate_ms": B.eliminate_ms, "forms": {}}
     ref = None
-    for precision in ("fp64", "mixed"):
-        tune = smearfem.autotune(B, precision, probe_iters=20)
+    for precision, pc in (("fp64", "jacobi"), ("fp64", "block_jacobi"), ("mixed", "block_jacobi"),
+                          ("split", "block_jacobi")):
+        form = precision + ("/bj" if pc == "block_jacobi" else "")
+        tune = smearfem.autotune(B, precision, probe_iters=20, preconditioner=pc)
         st, per = B.solve(precision, rtol=RTOL, item_slices=tune["item_slices"],
-                          blocks_per_cu=tune["blocks_per_cu"], raise_on_fail=False)
+                          blocks_per_cu=tune["blocks_per_cu"], raise_on_fail=False, preconditioner=pc)
         qs = B.solution()
         if ref is None:
             # FP64 reference for the error bound: the same batch to 1e-13
@@ -246,23 +248,24 @@ def run_batch(ctx, nprob=512, seed=4):
         errs = [float(np.linalg.norm(a - b) / np.linalg.norm(b)) for a, b in zip(qs, ref)]
         its = np.array([p["iterations"] for p in per])
         t = (st["solve_ms"] + B.assemble_ms + B.eliminate_ms) * 1e-3
-        out["forms"][precision] = {
+        out["forms"][form] = {
             "solve_ms": st["solve_ms"], "launches": st["launches"], "grid": st["grid"],
             "item_slices": tune["item_slices"], "blocks_per_cu": tune["blocks_per_cu"],
             "tune_ms_per_iter": tune["ms_per_iter"], "tuned_from_cache": tune["cached"],
             "iterations_max": int(its.max()), "iterations_median": float(np.median(its)),
             "iterations_sum": int(its.sum()), "not_converged": int(st["not_converged"]),
             "max_rel_err_vs_fp64_1e-13": max(errs), "device_bytes": st["device_bytes"],
-            "solved_dof_per_s": n_free / t, "solve_only_dof_per_s": n_free / (st["solve_ms"] * 1e-3),
+            "solved_dof_per_s": n_free / (t_build + st["solve_ms"] * 1e-3),
+            "device_only_dof_per_s": n_free / t, "solve_only_dof_per_s": n_free / (st["solve_ms"] * 1e-3),
         }
-        log(f"batch {precision}: solve {st['solve_ms']:.1f}ms in {st['launches']} launch, iterations max "
+        log(f"batch {form}: solve {st['solve_ms']:.1f}ms in {st['launches']} launch, iterations max "
             f"{its.max()} median {np.median(its):.0f}, max err {max(errs):.2e}, tune isl={tune['item_slices']} "
             f"bpc={tune['blocks_per_cu']}")
     B.close()
     return out
 
 
-def run_serve(ctx, precision, rates, nreq=48, seed=11):
+def run_serve(ctx, precision, rates, nreq=48, seed=11, preconditioner="block_jacobi"):
     """Poisson request streams of Q4/Hex8 cantilevers (DOF log-uniform in
     [1e3, 1e6]); the same requests at every arrival rate."""
     from smearfem.serve import Server, random_request, run_poisson
@@ -272,9 +275,11 @@ def run_serve(ctx, precision, rates, nreq=48, seed=11):
     reqs = [random_request(rng, 1e3, 1e6) for _ in range(nreq)]
     log(f"serve: {nreq} requests generated in {time.perf_counter() - t0:.1f}s, "
         f"{sum(r.n_dof for r in reqs)} dofs total")
-    out = {"precision": precision, "requests": nreq, "dof_range": [1e3, 1e6], "seed": seed, "rates": []}
+    out = {"precision": precision, "preconditioner": preconditioner, "requests": nreq, "dof_range": [1e3, 1e6],
+           "seed": seed, "rates": []}
     for lam in rates:
-        S = Server(ctx, precision=precision, rtol=RTOL, iters_per_step=64, hbm_budget=64 << 30, max_active=64)
+        S = Server(ctx, precision=precision, rtol=RTOL, iters_per_step=64, hbm_budget=64 << 30, max_active=64,
+                   preconditioner=preconditioner)
         res = run_poisson(S, reqs, rate_per_s=lam, seed=seed + 1)
         S.close()
         s = res.summary()
@@ -293,6 +298,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--forms-only", action="store_true",
+                    help="run only the batch and serving forms (no headline, no JSON line)")
     ap.add_argument("--serve-precision", default="fp64")
     ap.add_argument("--serve-rates", default="4,16,64")
     ap.add_argument("--only", type=str, default=None, help="comma list of precision/format paths to time")
@@ -324,6 +331,13 @@ def main():
         kernel_probe(ctx, mesh, bc, args.kernel_probe)
         return
 
+    if args.forms_only:
+        if not args.no_batch:
+            print(json.dumps({"batch": run_batch(ctx)}), flush=True)
+        if not args.no_serve:
+            rates = [float(v) for v in args.serve_rates.split(",") if v]
+            print(json.dumps({"serving": run_serve(ctx, args.serve_precision, rates)}), flush=True)
+        return
     paths, hinfo = run_headline(args, ctx, m, young, fixed, f)
     valid = {k: v for k, v in paths.items() if v["rel_err_vs_fp64_ref"] <= ERR_BAR}
     pool = valid or paths

@@ -282,6 +282,7 @@ typedef struct {
     int64_t hbm_budget;       /* bytes admitted requests may hold on the device           */
     int     max_active;       /* cap on concurrently active requests                      */
     int     iters_per_step;   /* CG iterations per persistent launch                      */
+    int     preconditioner;   /* SFEM_PC_*                                                */
 } sfem_server_opts;
 
 #define SFEM_REQ_QUEUED  0

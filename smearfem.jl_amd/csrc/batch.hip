@@ -30,6 +30,7 @@
 #include <vector>
 
 #include "batch.hpp"
+#include "block_ops.hpp"
 #include "solver_common.hpp"
 
 namespace cg = cooperative_groups;
@@ -107,7 +108,9 @@ __global__ void sell_fill_pad_kernel(const int64_t* __restrict__ bcolptr, const 
                                      const double* __restrict__ bval, const int32_t* __restrict__ pad_of,
                                      int64_t nunits, const int64_t* __restrict__ sl_ptr,
                                      const int32_t* __restrict__ sl_w, int32_t* __restrict__ scol,
-                                     double* __restrict__ sval64, float* __restrict__ sval32) {
+                                     double* __restrict__ sval64, float* __restrict__ sval32,
+                                     int16_t* __restrict__ sval16, int64_t nrows, double* __restrict__ binv64,
+                                     float* __restrict__ binv32) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -129,7 +132,21 @@ __global__ void sell_fill_pad_kernel(const int64_t* __restrict__ bcolptr, const 
                 const double v = bval[u * B2 + i * BS + j];
                 sval64[slot * B2 + (j * BS + i) * 64 + lane] = v;
                 sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
+                sval16[slot * B2 + (j * BS + i) * 64 + lane] = split_correction(v);
             }
+        if (c == row) {  // diagonal block: its inverse for block-Jacobi preconditioning
+            double m[B2], inv[B2];
+#pragma unroll
+            for (int j = 0; j < BS; ++j)
+#pragma unroll
+                for (int i = 0; i < BS; ++i) m[j * BS + i] = bval[u * B2 + i * BS + j];
+            invert_block<BS>(m, inv);
+#pragma unroll
+            for (int v = 0; v < B2; ++v) {
+                binv64[v * nrows + row] = inv[v];
+                binv32[v * nrows + row] = (float)inv[v];
+            }
+        }
         ++k;
     }
     for (; k < sl_w[s]; ++k) {
@@ -138,8 +155,21 @@ __global__ void sell_fill_pad_kernel(const int64_t* __restrict__ bcolptr, const 
         for (int v = 0; v < B2; ++v) {
             sval64[slot * B2 + v * 64 + lane] = 0.0;
             sval32[slot * B2 + v * 64 + lane] = 0.0f;
+            sval16[slot * B2 + v * 64 + lane] = 0;
         }
     }
+}
+
+// identity blocks for every row (padding rows keep them)
+__global__ void binv_identity_kernel(double* __restrict__ binv64, float* __restrict__ binv32, int64_t nrows,
+                                     int bs) {
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    for (int j = 0; j < bs; ++j)
+        for (int i = 0; i < bs; ++i) {
+            binv64[(j * bs + i) * nrows + row] = i == j ? 1.0 : 0.0;
+            binv32[(j * bs + i) * nrows + row] = i == j ? 1.0f : 0.0f;
+        }
 }
 
 // padding rows (no source column unit, blen < 0): every slot is padding
@@ -147,7 +177,7 @@ template <int BS>
 __global__ void sell_pad_gap_kernel(const int32_t* __restrict__ blen, int64_t nrows,
                                     const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                     int32_t* __restrict__ scol, double* __restrict__ sval64,
-                                    float* __restrict__ sval32) {
+                                    float* __restrict__ sval32, int16_t* __restrict__ sval16) {
     constexpr int B2 = BS * BS;
     const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= nrows || blen[row] >= 0) return;
@@ -159,6 +189,7 @@ __global__ void sell_pad_gap_kernel(const int32_t* __restrict__ blen, int64_t nr
         for (int v = 0; v < B2; ++v) {
             sval64[slot * B2 + v * 64 + lane] = 0.0;
             sval32[slot * B2 + v * 64 + lane] = 0.0f;
+            sval16[slot * B2 + v * 64 + lane] = 0;
         }
     }
 }
@@ -239,6 +270,116 @@ __device__ __forceinline__ double items_sum(const double* part, int stride, int 
     return block_sum_256(s, sh);
 }
 
+// The operator values of a descriptor as the launch's value source.
+template <typename TV>
+__device__ __forceinline__ Vals<TV> desc_vals(const ProbDesc& D) {
+    if constexpr (std::is_same<TV, SplitVal>::value) return Vals<SplitVal>{(const float*)D.sval, D.sval16};
+    else return Vals<TV>{(const TV*)D.sval};
+}
+
+// z = preconditioner(r) for the BS dofs of node row `row`
+template <int BS, typename T>
+__device__ __forceinline__ void apply_pc(const ProbDesc& D, int64_t row, const T (&rv)[BS], T (&zv)[BS]) {
+    if (D.block_pc) {
+        const T* binv = (const T*)D.binv;
+        T m[BS * BS];
+#pragma unroll
+        for (int v = 0; v < BS * BS; ++v) m[v] = binv[v * D.binv_stride + row];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            T acc = 0;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) acc = fmaT<T>(m[j * BS + i], rv[i], acc);
+            zv[j] = acc;
+        }
+    } else {
+        const T* dinv = (const T*)D.dinv;
+#pragma unroll
+        for (int j = 0; j < BS; ++j) zv[j] = dinv[row * BS + j] * rv[j];
+    }
+}
+
+// Update phase of one work item: x += al p, r -= al q, z = M^-1 r, one node
+// per thread; returns the thread's r.z and r.r.
+template <int BS, typename T, bool FOLD>
+__device__ __forceinline__ void update_item(const ProbDesc& D, const WorkItem& I, T al, bool fold, double& rz,
+                                            double& rr) {
+    double* x64 = D.x64;
+    T* xs = (T*)D.xs;
+    T* r = (T*)D.r;
+    T* z = (T*)D.z;
+    const T* p = (const T*)D.p;
+    const T* q = (const T*)D.q;
+    for (int64_t n = I.slice0 * 64 + threadIdx.x; n < I.slice1 * 64; n += kThreads) {
+        T pv[BS], qv[BS], rv[BS], zv[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            pv[j] = p[n * BS + j];
+            qv[j] = q[n * BS + j];
+            rv[j] = r[n * BS + j];
+        }
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            const int64_t i = n * BS + j;
+            if constexpr (FOLD) {
+                T xi = xs[i];
+                if (fold) {
+                    x64[i] += (double)xi;
+                    xi = 0;
+                }
+                xs[i] = fmaT<T>(al, pv[j], xi);
+            } else {
+                x64[i] = fma((double)al, (double)pv[j], x64[i]);
+            }
+            rv[j] = fmaT<T>(-al, qv[j], rv[j]);
+            r[i] = rv[j];
+            rr += (double)rv[j] * (double)rv[j];
+        }
+        apply_pc<BS, T>(D, n, rv, zv);
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            z[n * BS + j] = zv[j];
+            rz += (double)rv[j] * (double)zv[j];
+        }
+    }
+}
+
+// FP64 residual of one work item: r = rhs - K x, z = M^-1 r; accumulates
+// |r|^2, r.z, r.p and the rounding-floor scale.
+template <int BS, typename T, bool FOLD>
+__device__ __forceinline__ void resid_item(const ProbDesc& D, const WorkItem& I, int w, int lane, double& rr,
+                                           double& rz, double& rp, double& ff) {
+    const double* x64 = D.x64;
+    const T* xs = (const T*)D.xs;
+    const T* p = (const T*)D.p;
+    T* r = (T*)D.r;
+    T* z = (T*)D.z;
+    auto gx = [&](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
+    for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
+        double out[BS], aout[BS];
+        sell_row_abs<BS>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out, aout);
+        const int64_t row = sl * 64 + lane;
+        T rt[BS], zt[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            const int64_t d = row * BS + j;
+            const double res = D.rhs[d] - out[j];
+            const double sc = fabs(D.rhs[d]) + aout[j];
+            ff += sc * sc;
+            rt[j] = (T)res;
+            r[d] = rt[j];
+            rr += res * res;
+            rp += (double)rt[j] * (double)p[d];
+        }
+        apply_pc<BS, T>(D, row, rt, zt);
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            z[row * BS + j] = zt[j];
+            rz += (double)rt[j] * (double)zt[j];
+        }
+    }
+}
+
 template <typename TV, typename T, bool FOLD>
 __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
     cg::grid_group grid = cg::this_grid();
@@ -292,33 +433,9 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 const WorkItem I = a.items[it];
                 const ProbDesc& D = probs[I.prob];
                 if (!D.st->need_update || D.st->done) continue;
-                const double* x64 = D.x64;
-                const T* xs = (const T*)D.xs;
-                const T* dinv = (const T*)D.dinv;
-                const T* p = (const T*)D.p;
-                T* r = (T*)D.r;
-                T* z = (T*)D.z;
-                const int bs = D.bs;
                 double rr = 0.0, rz = 0.0, rp = 0.0, ff = 0.0;
-                auto gx = [&](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
-                for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
-                    double out[3], aout[3];
-                    if (bs == 3) sell_row_abs<3>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out, aout);
-                    else sell_row_abs<2>(D.sl_ptr, D.sl_w, D.scol, D.sval64, sl, lane, gx, out, aout);
-                    for (int j = 0; j < bs; ++j) {
-                        const int64_t d = (sl * 64 + lane) * bs + j;
-                        const double res = D.rhs[d] - out[j];
-                        const double sc = fabs(D.rhs[d]) + aout[j];
-                        ff += sc * sc;
-                        const T rt = (T)res;
-                        const T zt = dinv[d] * rt;
-                        r[d] = rt;
-                        z[d] = zt;
-                        rr += res * res;
-                        rz += (double)rt * (double)zt;
-                        rp += (double)rt * (double)p[d];
-                    }
-                }
+                if (D.bs == 3) resid_item<3, T, FOLD>(D, I, w, lane, rr, rz, rp, ff);
+                else resid_item<2, T, FOLD>(D, I, w, lane, rr, rz, rp, ff);
                 const double b0 = block_sum_256(rr, sh), b1 = block_sum_256(rz, sh + 4);
                 const double b2 = block_sum_256(rp, sh), b3 = block_sum_256(ff, sh + 4);
                 if (threadIdx.x == 0) {
@@ -398,14 +515,14 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             if (D.st->done) continue;
             const T* p = (const T*)D.p;
             T* q = (T*)D.q;
-            const TV* sval = (const TV*)D.sval;
+            const Vals<TV> vals = desc_vals<TV>(D);
             const int bs = D.bs;
             double dot = 0.0;
             auto gx = [&](int64_t c) { return p[c]; };
             for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
                 T out[3];
-                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, Vals<TV>{sval}, sl, lane, gx, out);
-                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, Vals<TV>{sval}, sl, lane, gx, out);
+                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
+                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
                 for (int j = 0; j < bs; ++j) {
                     const int64_t d = (sl * 64 + lane) * bs + j;
                     q[d] = out[j];
@@ -417,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
         }
         grid.sync();
 
-        // ---- U: alpha; x += alpha p; r -= alpha q; z = D^-1 r ----
+        // ---- U: alpha; x += alpha p; r -= alpha q; z = M^-1 r (per node) ----
         for (int it = blockIdx.x; it < a.nitems; it += G) {
             const WorkItem I = a.items[it];
             const ProbDesc& D = probs[I.prob];
@@ -425,32 +542,9 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const double pq = items_sum(a.part, 4, 0, D.item0, D.item1, sh);
             const T al = (T)(D.st->rz / pq);
             const bool fold = FOLD && D.st->fold_pending;
-            double* x64 = D.x64;
-            T* xs = (T*)D.xs;
-            T* r = (T*)D.r;
-            T* z = (T*)D.z;
-            const T* p = (const T*)D.p;
-            const T* q = (const T*)D.q;
-            const T* dinv = (const T*)D.dinv;
             double rz = 0.0, rr = 0.0;
-            for (int64_t i = I.slice0 * 64 * D.bs + threadIdx.x; i < I.slice1 * 64 * D.bs; i += kThreads) {
-                if (FOLD) {
-                    T xi = xs[i];
-                    if (fold) {
-                        x64[i] += (double)xi;
-                        xi = 0;
-                    }
-                    xs[i] = fmaT<T>(al, p[i], xi);
-                } else {
-                    x64[i] = fma((double)al, (double)p[i], x64[i]);
-                }
-                const T ri = fmaT<T>(-al, q[i], r[i]);
-                const T zi = dinv[i] * ri;
-                r[i] = ri;
-                z[i] = zi;
-                rz += (double)ri * (double)zi;
-                rr += (double)ri * (double)ri;
-            }
+            if (D.bs == 3) update_item<3, T, FOLD>(D, I, al, fold, rz, rr);
+            else update_item<2, T, FOLD>(D, I, al, fold, rz, rr);
             const double b0 = block_sum_256(rz, sh), b1 = block_sum_256(rr, sh + 4);
             if (threadIdx.x == 0) {
                 a.part[4 * (int64_t)it + 1] = b0;
@@ -523,6 +617,7 @@ int table_max_blocks_per_cu(int precision) {
         case SFEM_PREC_FP64: return resident_blocks<double, double, false>();
         case SFEM_PREC_MIXED: return resident_blocks<float, float, true>();
         case SFEM_PREC_FP32_MATRIX: return resident_blocks<float, double, false>();
+        case SFEM_PREC_SPLIT: return resident_blocks<SplitVal, double, false>();
         default: fail(SFEM_ERR_ARGUMENT, "unknown precision");
     }
 }
@@ -533,6 +628,7 @@ void table_launch(sfem_ctx* ctx, int precision, const TableLaunch& a, int grid) 
         case SFEM_PREC_FP64: launch_table<double, double, false>(ctx, a, grid); break;
         case SFEM_PREC_MIXED: launch_table<float, float, true>(ctx, a, grid); break;
         case SFEM_PREC_FP32_MATRIX: launch_table<float, double, false>(ctx, a, grid); break;
+        case SFEM_PREC_SPLIT: launch_table<SplitVal, double, false>(ctx, a, grid); break;
         default: fail(SFEM_ERR_ARGUMENT, "unknown precision");
     }
 }
@@ -627,21 +723,29 @@ void batch_build(sfem_ctx* ctx, sfem_batch* B) {
     B->scol.reset(ctx, (size_t)slots);
     B->sval64.reset(ctx, (size_t)slots * bs * bs);
     B->sval32.reset(ctx, (size_t)slots * bs * bs);
+    B->sval16.reset(ctx, (size_t)slots * bs * bs);
+    B->binv64.reset(ctx, (size_t)base * bs * bs);
+    B->binv32.reset(ctx, (size_t)base * bs * bs);
     const dim3 gr(grid_for(base, 256));
+    hipLaunchKernelGGL(binv_identity_kernel, gr, dim3(256), 0, ctx->stream, B->binv64.get(), B->binv32.get(), base,
+                       bs);
+    SFEM_LAUNCH_CHECK();
     if (bs == 3) {
         hipLaunchKernelGGL(sell_fill_pad_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
                            K->bval.get(), B->pad_of.get(), nunits, B->sl_ptr.get(), B->sl_w.get(), B->scol.get(),
-                           B->sval64.get(), B->sval32.get());
+                           B->sval64.get(), B->sval32.get(), B->sval16.get(), base, B->binv64.get(),
+                           B->binv32.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(sell_pad_gap_kernel<3>, gr, dim3(256), 0, ctx->stream, blen.get(), base, B->sl_ptr.get(),
-                           B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get());
+                           B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get(), B->sval16.get());
     } else {
         hipLaunchKernelGGL(sell_fill_pad_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
                            K->bval.get(), B->pad_of.get(), nunits, B->sl_ptr.get(), B->sl_w.get(), B->scol.get(),
-                           B->sval64.get(), B->sval32.get());
+                           B->sval64.get(), B->sval32.get(), B->sval16.get(), base, B->binv64.get(),
+                           B->binv32.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(sell_pad_gap_kernel<2>, gr, dim3(256), 0, ctx->stream, blen.get(), base, B->sl_ptr.get(),
-                           B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get());
+                           B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get(), B->sval16.get());
     }
     SFEM_LAUNCH_CHECK();
     B->rhs.reset(ctx, (size_t)B->npad);
@@ -695,12 +799,13 @@ void batch_ensure_work(sfem_ctx* ctx, sfem_batch* B, int precision) {
         B->wxs.reset(ctx, sizeof(float) * B->npad);
 }
 
-ProbDesc batch_desc(sfem_batch* B, int precision, ProbState* st, int item0, int item1) {
+ProbDesc batch_desc(sfem_batch* B, int precision, int preconditioner, ProbState* st, int item0, int item1) {
     ProbDesc D{};
     D.sl_ptr = B->sl_ptr.get();
     D.sl_w = B->sl_w.get();
     D.scol = B->scol.get();
     D.sval = precision == SFEM_PREC_FP64 ? (const void*)B->sval64.get() : (const void*)B->sval32.get();
+    D.sval16 = B->sval16.get();
     D.sval64 = B->sval64.get();
     D.rhs = B->rhs.get();
     D.x64 = B->x.get();
@@ -710,6 +815,9 @@ ProbDesc batch_desc(sfem_batch* B, int precision, ProbState* st, int item0, int 
     D.p = B->wp.get();
     D.q = B->wq.get();
     D.dinv = precision == SFEM_PREC_MIXED ? (const void*)B->dinv32.get() : (const void*)B->dinv64.get();
+    D.binv = precision == SFEM_PREC_MIXED ? (const void*)B->binv32.get() : (const void*)B->binv64.get();
+    D.binv_stride = B->nslices * 64;
+    D.block_pc = preconditioner == SFEM_PC_BLOCK_JACOBI ? 1 : 0;
     D.st = st;
     D.bs = B->bs;
     D.item0 = item0;
@@ -741,8 +849,10 @@ void batch_reset(sfem_ctx* ctx, sfem_batch* B, int precision, const sfem_solve_o
 double batch_run(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts& o, int item_slices, int blocks_per_cu,
                  int64_t* launches) {
     const int prec = o.precision;
-    if (prec != SFEM_PREC_FP64 && prec != SFEM_PREC_MIXED && prec != SFEM_PREC_FP32_MATRIX)
+    if (prec != SFEM_PREC_FP64 && prec != SFEM_PREC_MIXED && prec != SFEM_PREC_FP32_MATRIX && prec != SFEM_PREC_SPLIT)
         fail(SFEM_ERR_ARGUMENT, "batch solve: unknown precision");
+    if (o.preconditioner != SFEM_PC_JACOBI && o.preconditioner != SFEM_PC_BLOCK_JACOBI)
+        fail(SFEM_ERR_ARGUMENT, "batch solve: unknown preconditioner");
     if (item_slices < 1) fail(SFEM_ERR_ARGUMENT, "batch solve: item_slices must be >= 1");
     if (o.max_iter < 0) fail(SFEM_ERR_ARGUMENT, "batch solve: max_iter must be >= 0");
     if (B->st.size() != (size_t)B->nprob) B->st.reset(ctx, (size_t)B->nprob);
@@ -752,7 +862,8 @@ double batch_run(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts& o, int ite
     batch_items(B, item_slices, &items, &ranges);
     std::vector<ProbDesc> descs((size_t)B->nprob);
     for (int64_t p = 0; p < B->nprob; ++p)
-        descs[(size_t)p] = batch_desc(B, prec, B->st.get() + p, ranges[(size_t)p].first, ranges[(size_t)p].second);
+        descs[(size_t)p] = batch_desc(B, prec, o.preconditioner, B->st.get() + p, ranges[(size_t)p].first,
+                                      ranges[(size_t)p].second);
     DevBuf<ProbDesc> ddesc(ctx, descs.size());
     DevBuf<WorkItem> ditems(ctx, items.size());
     DevBuf<double> part(ctx, 4 * items.size()), partr(ctx, 4 * items.size());
@@ -873,8 +984,8 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
 int64_t batch_device_bytes(const sfem_batch* B, int precision) {
     const int64_t b2 = (int64_t)B->bs * B->bs;
     const int64_t vb = (int64_t)prec_vec_bytes(precision);
-    return B->sell_blocks * (4 + b2 * 12) + B->nslices * 12 + B->npad * (8 * 3 + 4 + 4 * vb + (vb == 4 ? 4 : 0)) +
-           (int64_t)B->pad_of.size() * 4 + B->n * 8;
+    return B->sell_blocks * (4 + b2 * 14) + B->nslices * 12 + B->npad * (8 * 3 + 4 + 4 * vb + (vb == 4 ? 4 : 0)) +
+           B->nslices * 64 * b2 * 12 + (int64_t)B->pad_of.size() * 4 + B->n * 8;
 }
 
 }  // namespace sfem

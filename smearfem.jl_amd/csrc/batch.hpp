@@ -22,12 +22,16 @@ struct ProbState {
 
 // One problem of a table: its sliced operator and vectors (padded layout,
 // 64-row slices, vectors of nslices*64*bs entries).  Pointers are typed by the
-// launch's precision: sval/dinv/r/z/p/q/xs are float in mixed mode.
+// launch's precision: sval is the FP32 plane in mixed and split modes (plus
+// sval16 in split mode), the FP64 plane otherwise; dinv/binv/r/z/p/q/xs are
+// float in mixed mode.  binv: inverse node diagonal blocks, planar with
+// stride binv_stride (node rows of the operator); used when block_pc is 1.
 struct ProbDesc {
     const int64_t* sl_ptr;
     const int32_t* sl_w;
     const int32_t* scol;
     const void* sval;
+    const int16_t* sval16;
     const double* sval64;
     const double* rhs;
     double* x64;
@@ -37,8 +41,10 @@ struct ProbDesc {
     void* p;
     void* q;
     const void* dinv;
+    const void* binv;
     ProbState* st;
-    int32_t bs, item0, item1, pad;
+    int64_t binv_stride;
+    int32_t bs, item0, item1, block_pc;
 };
 
 // A run of slices [slice0, slice1) of one problem, processed by one block.
@@ -85,7 +91,9 @@ struct sfem_batch {
     sfem::DevBuf<int64_t> sl_ptr;
     sfem::DevBuf<int32_t> sl_w, scol;
     sfem::DevBuf<double> sval64, rhs, x, dinv64, qd;
-    sfem::DevBuf<float> sval32, dinv32;
+    sfem::DevBuf<float> sval32, dinv32, binv32;
+    sfem::DevBuf<int16_t> sval16;
+    sfem::DevBuf<double> binv64;  // planar, stride nslices * 64
     // CG work vectors, typed by the last precision (work_vb bytes per entry)
     size_t work_vb = 0;
     sfem::DevBuf<uint8_t> wr, wz, wp, wq, wxs;
@@ -103,7 +111,7 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
 void batch_items(const sfem_batch* B, int item_slices, std::vector<WorkItem>* items,
                  std::vector<std::pair<int, int>>* ranges);
 void batch_ensure_work(sfem_ctx* ctx, sfem_batch* B, int precision);
-ProbDesc batch_desc(sfem_batch* B, int precision, ProbState* st, int item0, int item1);
+ProbDesc batch_desc(sfem_batch* B, int precision, int preconditioner, ProbState* st, int item0, int item1);
 void batch_reset(sfem_ctx* ctx, sfem_batch* B, int precision, const sfem_solve_opts& o, ProbState* d_st);
 void batch_solution(sfem_ctx* ctx, const sfem_batch* B, double* q);
 int64_t batch_device_bytes(const sfem_batch* B, int precision);

@@ -94,8 +94,8 @@ void estimate_bytes(Request& r, int precision) {
     const int64_t rows = (r.n_nodes + 63) / 64 * 64;
     const int64_t slots = rows * maxdeg;
     const int64_t b2 = (int64_t)bs * bs, vb = (int64_t)prec_vec_bytes(precision);
-    r.est_resident = slots * (4 + 12 * b2) + rows / 64 * 12 + rows * bs * (8 * 3 + 4 + 4 * vb + 4) + r.n_nodes * 4 +
-                     r.n_nodes * bs * 8;
+    r.est_resident = slots * (4 + 14 * b2) + rows / 64 * 12 + rows * bs * (8 * 3 + 4 + 4 * vb + 4) +
+                     rows * b2 * 12 + r.n_nodes * 4 + r.n_nodes * bs * 8;
     const int64_t ke = (int64_t)nen * bs * nen * bs;
     r.est_transient = r.est_resident + r.n_elem * (ke * 8 + (int64_t)nen * nen * 24) + slots * (8 + b2 * 8);
 }
@@ -209,7 +209,8 @@ void step(sfem_server* S, int* n_finished, double* ms) {
             w.prob = (int32_t)descs.size();
             items.push_back(w);
         }
-        descs.push_back(batch_desc(r.B, S->o.precision, S->pool.get() + r.slot, base, (int)items.size()));
+        descs.push_back(batch_desc(r.B, S->o.precision, S->o.preconditioner, S->pool.get() + r.slot, base,
+                                   (int)items.size()));
     }
     if (S->ddesc.size() < descs.size()) S->ddesc.reset(ctx, descs.size() * 2);
     if (S->ditems.size() < items.size()) {
@@ -253,6 +254,7 @@ void sfem_server_opts_default(sfem_server_opts* o) {
     o->hbm_budget = (int64_t)64 << 30;
     o->max_active = 256;
     o->iters_per_step = 64;
+    o->preconditioner = SFEM_PC_JACOBI;
 }
 
 int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server** out) {
@@ -266,8 +268,10 @@ int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server*
         if (opts) S->o = *opts;
         if (S->o.max_active < 1 || S->o.iters_per_step < 1 || S->o.item_slices < 1 || S->o.hbm_budget <= 0)
             fail(SFEM_ERR_ARGUMENT, "server_create: max_active, iters_per_step, item_slices and hbm_budget must be > 0");
-        if (S->o.precision < SFEM_PREC_FP64 || S->o.precision > SFEM_PREC_FP32_MATRIX)
+        if (S->o.precision < SFEM_PREC_FP64 || S->o.precision > SFEM_PREC_SPLIT)
             fail(SFEM_ERR_ARGUMENT, "server_create: unknown precision");
+        if (S->o.preconditioner != SFEM_PC_JACOBI && S->o.preconditioner != SFEM_PC_BLOCK_JACOBI)
+            fail(SFEM_ERR_ARGUMENT, "server_create: unknown preconditioner");
         S->ctx = ctx;
         S->grid = table_grid(ctx, S->o.precision, S->o.blocks_per_cu);
         S->pool.reset(ctx, (size_t)S->o.max_active);

@@ -85,6 +85,7 @@ class ServerOpts(C.Structure):
         ("hbm_budget", C.c_int64),
         ("max_active", C.c_int),
         ("iters_per_step", C.c_int),
+        ("preconditioner", C.c_int),
     ]
 
 

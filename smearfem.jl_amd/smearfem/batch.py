@@ -22,7 +22,7 @@ import numpy as np
 from . import _lib
 from .fem import Context, _ptr, default_context
 from .mesh import StructuredMesh, clamp_x0, face_load, structured_mesh
-from .solve import PRECISIONS
+from .solve import PRECISIONS, PRECONDITIONERS
 
 
 @dataclass
@@ -107,7 +107,7 @@ class Batch:
         self.signature = {"ndim": ndim, "nprob": self.nprob, "slices": self.slices}
 
     def solve(self, precision="mixed", rtol=1e-10, max_iter=100000, item_slices=8, blocks_per_cu=0,
-              check_every=0, reliable_delta=0.1, raise_on_fail=True):
+              check_every=0, reliable_delta=0.1, raise_on_fail=True, preconditioner="jacobi"):
         """Returns (stats dict, list of per-problem stats dicts)."""
         lib = self.ctx._lib
         o = _lib.SolveOpts()
@@ -118,13 +118,14 @@ class Batch:
         o.reliable_delta = float(reliable_delta)
         o.check_every = int(check_every)
         o.x0_zero = 1
+        o.preconditioner = PRECONDITIONERS[preconditioner]
         per = (_lib.BatchProblemStats * self.nprob)()
         s = _lib.BatchStats()
         st = lib.sfem_batch_solve(self.ctx.handle, self.handle, C.byref(o), int(item_slices), int(blocks_per_cu),
                                   per, C.byref(s))
         stats = {name: getattr(s, name) for name, _ in _lib.BatchStats._fields_}
         stats.update(status=st, precision=precision, item_slices=int(item_slices),
-                     blocks_per_cu=int(blocks_per_cu))
+                     blocks_per_cu=int(blocks_per_cu), preconditioner=preconditioner)
         plist = [{name: getattr(p, name) for name, _ in _lib.BatchProblemStats._fields_} for p in per]
         if st and raise_on_fail:
             self.ctx.check(st)
@@ -156,14 +157,14 @@ def _bucket(v: int) -> int:
     return int(np.floor(np.log2(max(v, 1))))
 
 
-def tune_key(ctx: Context, batch: Batch, precision: str) -> str:
+def tune_key(ctx: Context, batch: Batch, precision: str, preconditioner: str = "jacobi") -> str:
     info = ctx.device_info()
-    return (f"{info['name'].strip()}|cu{info['n_cu']}|{precision}|d{batch.ndim}|p{_bucket(batch.nprob)}"
-            f"|s{_bucket(batch.slices)}")
+    return (f"{info['name'].strip()}|cu{info['n_cu']}|{precision}|{preconditioner}|d{batch.ndim}"
+            f"|p{_bucket(batch.nprob)}|s{_bucket(batch.slices)}")
 
 
 def autotune(batch: Batch, precision="mixed", item_slices=(2, 4, 8, 16, 32), blocks_per_cu=None,
-             probe_iters=40, use_cache=True, verbose=False) -> dict:
+             probe_iters=40, use_cache=True, verbose=False, preconditioner="jacobi") -> dict:
     """Pick (item_slices, blocks_per_cu) for a batch by timing short solves.
 
     Every candidate runs ``probe_iters`` CG iterations of every problem from
@@ -172,7 +173,7 @@ def autotune(batch: Batch, precision="mixed", item_slices=(2, 4, 8, 16, 32), blo
     precision, dimension and the batch's size class.
     """
     ctx = batch.ctx
-    key = tune_key(ctx, batch, precision)
+    key = tune_key(ctx, batch, precision, preconditioner)
     path = _cache_path()
     cache = {}
     if use_cache and path.exists():
@@ -191,7 +192,7 @@ def autotune(batch: Batch, precision="mixed", item_slices=(2, 4, 8, 16, 32), blo
     for isl in item_slices:
         for bpc in blocks_per_cu:
             st, _ = batch.solve(precision, rtol=0.0, max_iter=probe_iters, item_slices=isl, blocks_per_cu=bpc,
-                                raise_on_fail=False)
+                                raise_on_fail=False, preconditioner=preconditioner)
             ms_it = st["solve_ms"] / max(st["max_iterations"], 1)
             trials.append({"item_slices": isl, "blocks_per_cu": bpc, "ms_per_iter": ms_it})
             if verbose:

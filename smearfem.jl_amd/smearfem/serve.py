@@ -23,7 +23,7 @@ from . import _lib
 from .batch import Problem, make_problem
 from .fem import Context, _ptr, default_context
 from .mesh import clamp_x0, structured_mesh
-from .solve import PRECISIONS
+from .solve import PRECISIONS, PRECONDITIONERS
 
 REQ_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed"}
 
@@ -31,7 +31,7 @@ REQ_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed"}
 class Server:
     def __init__(self, ctx: Optional[Context] = None, precision="mixed", rtol=1e-10, max_iter=100000,
                  reliable_delta=0.1, item_slices=8, blocks_per_cu=0, hbm_budget: Optional[int] = None,
-                 max_active=256, iters_per_step=64):
+                 max_active=256, iters_per_step=64, preconditioner="jacobi"):
         self.ctx = ctx or default_context()
         lib = self.ctx._lib
         o = _lib.ServerOpts()
@@ -46,6 +46,7 @@ class Server:
             o.hbm_budget = int(hbm_budget)
         o.max_active = int(max_active)
         o.iters_per_step = int(iters_per_step)
+        o.preconditioner = PRECONDITIONERS[preconditioner]
         h = C.c_void_p()
         self.ctx.check(lib.sfem_server_create(self.ctx.handle, C.byref(o), C.byref(h)))
         self.handle = h

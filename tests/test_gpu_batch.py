@@ -38,12 +38,14 @@ def small_hex8_batch(n, seed):
     return uneven_hex8_problems(n, seed, lo=2, hi=7)
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "fp32_matrix"])
-def test_batch_hex8_against_oracle(ctx, precision):
-    print(f"[case] batch hex8 x10 {precision}", flush=True)
+@pytest.mark.parametrize("precision,pc", [("fp64", "jacobi"), ("mixed", "jacobi"), ("fp32_matrix", "jacobi"),
+                                          ("split", "jacobi"), ("fp64", "block_jacobi"), ("mixed", "block_jacobi"),
+                                          ("split", "block_jacobi")])
+def test_batch_hex8_against_oracle(ctx, precision, pc):
+    print(f"[case] batch hex8 x10 {precision} {pc}", flush=True)
     probs = small_hex8_batch(10, seed=11)
     B = Batch(probs, ctx)
-    st, per = B.solve(precision, rtol=1e-12, item_slices=2)
+    st, per = B.solve(precision, rtol=1e-12, item_slices=2, preconditioner=pc)
     qs = B.solution()
     B.close()
     assert st["not_converged"] == 0 and st["launches"] == 1
@@ -67,8 +69,8 @@ def test_batch_q4_with_prescribed_values(ctx):
         probs.append(make_problem(m, young, rng.uniform(0.2, 0.4), f, fixed, q_d))
     B = Batch(probs, ctx)
     # 1e-11: a 10x2 strip here cannot reach 1e-12 in FP64 (its floor is about 4e-12)
-    for precision in ("fp64", "mixed"):
-        st, per = B.solve(precision, rtol=1e-11)
+    for precision, pc in (("fp64", "jacobi"), ("mixed", "jacobi"), ("split", "block_jacobi")):
+        st, per = B.solve(precision, rtol=1e-11, preconditioner=pc)
         for p, q in zip(probs, B.solution()):
             assert relerr(q, oracle_solution(p)) <= ERR
             assert np.array_equal(q[p.fixed == 1], p.q_d[p.fixed == 1])
@@ -80,17 +82,19 @@ def test_batch_q4_with_prescribed_values(ctx):
     B.close()
 
 
-def test_batch_problem_bitwise_alone_and_in_mix(ctx):
-    print("[case] batch invariance", flush=True)
+@pytest.mark.parametrize("precision,pc", [("mixed", "jacobi"), ("split", "block_jacobi")])
+def test_batch_problem_bitwise_alone_and_in_mix(ctx, precision, pc):
+    print(f"[case] batch invariance {precision} {pc}", flush=True)
     probs = small_hex8_batch(7, seed=5)
     target = probs[3]
     A = Batch([target], ctx)
-    A.solve("mixed", rtol=1e-11, item_slices=1)
+    A.solve(precision, rtol=1e-11, item_slices=1, preconditioner=pc)
     q_alone = A.solution()[0]
     A.close()
     B = Batch(probs, ctx)
     for bpc, check_every in ((1, 0), (4, 0), (2, 7)):
-        st, per = B.solve("mixed", rtol=1e-11, item_slices=1, blocks_per_cu=bpc, check_every=check_every)
+        st, per = B.solve(precision, rtol=1e-11, item_slices=1, blocks_per_cu=bpc, check_every=check_every,
+                          preconditioner=pc)
         q_mix = B.solution()[3]
         assert np.array_equal(q_mix, q_alone), (bpc, check_every)
         if check_every:

@@ -37,12 +37,12 @@ def oracle_solution(p):
     return O.recombine(p.fixed, None, O.dense_solve(Kff, rhs))
 
 
-@pytest.mark.parametrize("precision", ["mixed", "fp64"])
-def test_alone_equals_in_mix(ctx, precision):
-    print(f"[case] serve alone==mix {precision}", flush=True)
+@pytest.mark.parametrize("precision,pc", [("mixed", "jacobi"), ("fp64", "jacobi"), ("split", "block_jacobi")])
+def test_alone_equals_in_mix(ctx, precision, pc):
+    print(f"[case] serve alone==mix {precision} {pc}", flush=True)
     target = request((6, 3, 3), seed=1)
     others = [request((8, 4, 3), 2), request((12, 5), 3), request((5, 5, 5), 4), request((20, 6), 5)]
-    S = Server(ctx, precision=precision, rtol=1e-11, iters_per_step=16, item_slices=2)
+    S = Server(ctx, precision=precision, rtol=1e-11, iters_per_step=16, item_slices=2, preconditioner=pc)
     rid = S.submit(target)
     S.drain()
     q_alone = S.take(rid, target.n_dof)
