@@ -63,6 +63,7 @@ def lib():
         L.or_spmv.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d)]
         L.or_dense_spd_solve.argtypes = [i64, P(d), P(d), P(d)]
         L.or_pcg.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d), d, i64, P(i64), P(d)]
+        L.or_pcg_mt.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d), d, i64, C.c_int, P(i64), P(d)]
         _lib = L
     return _lib
 
@@ -274,6 +275,23 @@ def pcg(Kff, rhs, rtol=1e-10, max_iter=100000, x0=None):
     rr = C.c_double(0)
     st = lib().or_pcg(b.size, _p(ptr, C.c_int64), _p(idx, C.c_int32), _p(val, C.c_double), _p(b, C.c_double),
                       _p(x, C.c_double), float(rtol), int(max_iter), C.byref(it), C.byref(rr))
+    if st not in (0, -6):
+        raise OracleError(st)
+    return x, it.value, rr.value
+
+
+def pcg_threads(Kff, rhs, nthreads, rtol=1e-10, max_iter=100000):
+    """pcg with the loops over nthreads OpenMP threads (CPU baseline timing)."""
+    A = Kff.tocsc()
+    ptr = A.indptr.astype(np.int64)
+    idx = A.indices.astype(np.int32)
+    val = A.data.astype(np.float64)
+    b = np.ascontiguousarray(rhs, dtype=np.float64)
+    x = np.zeros_like(b)
+    it = C.c_int64(0)
+    rr = C.c_double(0)
+    st = lib().or_pcg_mt(b.size, _p(ptr, C.c_int64), _p(idx, C.c_int32), _p(val, C.c_double), _p(b, C.c_double),
+                         _p(x, C.c_double), float(rtol), int(max_iter), int(nthreads), C.byref(it), C.byref(rr))
     if st not in (0, -6):
         raise OracleError(st)
     return x, it.value, rr.value

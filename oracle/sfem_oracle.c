@@ -575,3 +575,72 @@ int or_pcg(int64_t n, const int64_t* ptr, const int32_t* idx, const double* val,
     free(r); free(z); free(p); free(q); free(dinv);
     return st;
 }
+
+/* or_pcg with its loops split over nthreads OpenMP threads: the same
+ * algorithm, used only to time an all-core CPU baseline (the summation order,
+ * and so the last bits, depend on the thread count). */
+static void spmv_mt(int64_t n, const int64_t* ptr, const int32_t* idx, const double* val, const double* x,
+                    double* y, int nthreads) {
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double s = 0.0;
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) s += val[k] * x[idx[k]];
+        y[i] = s;
+    }
+}
+
+int or_pcg_mt(int64_t n, const int64_t* ptr, const int32_t* idx, const double* val, const double* b,
+              double* x, double rtol, int64_t max_iter, int nthreads, int64_t* iters, double* relres) {
+    double *r = malloc(sizeof(double) * n), *z = malloc(sizeof(double) * n), *p = malloc(sizeof(double) * n),
+           *q = malloc(sizeof(double) * n), *dinv = malloc(sizeof(double) * n);
+    if (!r || !z || !p || !q || !dinv) { free(r); free(z); free(p); free(q); free(dinv); return OR_ERR_MEMORY; }
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = 0; i < n; ++i) {
+        double d = 0.0;
+        for (int64_t k = ptr[i]; k < ptr[i + 1]; ++k) if (idx[k] == i) d = val[k];
+        dinv[i] = d != 0.0 ? 1.0 / d : 1.0;
+    }
+    spmv_mt(n, ptr, idx, val, x, q, nthreads);
+    double bb = 0.0, rz = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : bb, rz)
+    for (int64_t i = 0; i < n; ++i) {
+        r[i] = b[i] - q[i];
+        z[i] = dinv[i] * r[i];
+        p[i] = z[i];
+        rz += r[i] * z[i];
+        bb += b[i] * b[i];
+    }
+    double bn = sqrt(bb), rr = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rr)
+    for (int64_t i = 0; i < n; ++i) rr += r[i] * r[i];
+    int64_t it = 0;
+    int st = OR_ERR_NOT_CONVERGED;
+    if (bn == 0.0 || sqrt(rr) <= rtol * bn) st = OR_OK;
+    while (st != OR_OK && it < max_iter) {
+        spmv_mt(n, ptr, idx, val, p, q, nthreads);
+        double pq = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : pq)
+        for (int64_t i = 0; i < n; ++i) pq += p[i] * q[i];
+        double alpha = rz / pq;
+        double rz_new = 0.0;
+        rr = 0.0;
+#pragma omp parallel for num_threads(nthreads) schedule(static) reduction(+ : rz_new, rr)
+        for (int64_t i = 0; i < n; ++i) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * q[i];
+            z[i] = dinv[i] * r[i];
+            rz_new += r[i] * z[i];
+            rr += r[i] * r[i];
+        }
+        ++it;
+        if (sqrt(rr) <= rtol * bn) { st = OR_OK; break; }
+        double beta = rz_new / rz;
+        rz = rz_new;
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+        for (int64_t i = 0; i < n; ++i) p[i] = z[i] + beta * p[i];
+    }
+    *iters = it;
+    *relres = bn > 0 ? sqrt(rr) / bn : 0.0;
+    free(r); free(z); free(p); free(q); free(dinv);
+    return st;
+}

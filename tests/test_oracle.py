@@ -300,3 +300,16 @@ def test_patch_test_prescribed_linear_field():
     Kff, rhs = O.eliminate(K, fixed, None, np.where(fixed == 1, u_lin, 0.0))
     x = O.dense_solve(Kff, rhs)
     assert np.abs(x - u_lin[fixed == 0]).max() < 1e-12
+
+
+def test_threaded_pcg_matches_serial():
+    # the all-thread CPU baseline runs the same algorithm as the serial oracle
+    m = structured_mesh((12, 4, 4))
+    fixed = clamp_x0(m)
+    f = face_load(m, 0, 1, 2, -1.0)
+    K = O.assemble_system(0, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, 1.0, 0.3, n_elem=m.n_elem)
+    Kff, rhs = O.eliminate(K, fixed, f)
+    x1, i1, r1 = O.pcg(Kff, rhs, rtol=1e-11)
+    x4, i4, r4 = O.pcg_threads(Kff, rhs, 4, rtol=1e-11)
+    assert abs(i1 - i4) <= 2 and r4 <= 1e-11
+    assert np.linalg.norm(x1 - x4) <= 1e-9 * np.linalg.norm(x1)
