@@ -266,7 +266,7 @@ __device__ __forceinline__ bool block_any(int nprob, F pred) {
 // Sum of partial slot k over items [i0, i1), fixed order for a 256-thread block.
 __device__ __forceinline__ double items_sum(const double* part, int stride, int k, int i0, int i1, double* sh) {
     double s = 0.0;
-    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) s += part[(int64_t)stride * i + k];
+    for (int i = i0 + threadIdx.x; i < i1; i += kThreads) s += glb(part)[(int64_t)stride * i + k];
     return block_sum_256(s, sh);
 }
 
@@ -281,7 +281,7 @@ __device__ __forceinline__ Vals<TV> desc_vals(const ProbDesc& D) {
 template <int BS, typename T>
 __device__ __forceinline__ void apply_pc(const ProbDesc& D, int64_t row, const T (&rv)[BS], T (&zv)[BS]) {
     if (D.block_pc) {
-        const T* binv = (const T*)D.binv;
+        const SFEM_GLOBAL T* binv = glb((const T*)D.binv);
         T m[BS * BS];
 #pragma unroll
         for (int v = 0; v < BS * BS; ++v) m[v] = binv[v * D.binv_stride + row];
@@ -293,7 +293,7 @@ __device__ __forceinline__ void apply_pc(const ProbDesc& D, int64_t row, const T
             zv[j] = acc;
         }
     } else {
-        const T* dinv = (const T*)D.dinv;
+        const SFEM_GLOBAL T* dinv = glb((const T*)D.dinv);
 #pragma unroll
         for (int j = 0; j < BS; ++j) zv[j] = dinv[row * BS + j] * rv[j];
     }
@@ -304,12 +304,12 @@ __device__ __forceinline__ void apply_pc(const ProbDesc& D, int64_t row, const T
 template <int BS, typename T, bool FOLD>
 __device__ __forceinline__ void update_item(const ProbDesc& D, const WorkItem& I, T al, bool fold, double& rz,
                                             double& rr) {
-    double* x64 = D.x64;
-    T* xs = (T*)D.xs;
-    T* r = (T*)D.r;
-    T* z = (T*)D.z;
-    const T* p = (const T*)D.p;
-    const T* q = (const T*)D.q;
+    SFEM_GLOBAL double* x64 = glb(D.x64);
+    SFEM_GLOBAL T* xs = glb((T*)D.xs);
+    SFEM_GLOBAL T* r = glb((T*)D.r);
+    SFEM_GLOBAL T* z = glb((T*)D.z);
+    const SFEM_GLOBAL T* p = glb((const T*)D.p);
+    const SFEM_GLOBAL T* q = glb((const T*)D.q);
     for (int64_t n = I.slice0 * 64 + threadIdx.x; n < I.slice1 * 64; n += kThreads) {
         T pv[BS], qv[BS], rv[BS], zv[BS];
 #pragma unroll
@@ -349,11 +349,12 @@ __device__ __forceinline__ void update_item(const ProbDesc& D, const WorkItem& I
 template <int BS, typename T, bool FOLD>
 __device__ __forceinline__ void resid_item(const ProbDesc& D, const WorkItem& I, int w, int lane, double& rr,
                                            double& rz, double& rp, double& ff) {
-    const double* x64 = D.x64;
-    const T* xs = (const T*)D.xs;
-    const T* p = (const T*)D.p;
-    T* r = (T*)D.r;
-    T* z = (T*)D.z;
+    const SFEM_GLOBAL double* x64 = glb((const double*)D.x64);
+    const SFEM_GLOBAL T* xs = glb((const T*)D.xs);
+    const SFEM_GLOBAL T* p = glb((const T*)D.p);
+    const SFEM_GLOBAL double* rhs = glb(D.rhs);
+    SFEM_GLOBAL T* r = glb((T*)D.r);
+    SFEM_GLOBAL T* z = glb((T*)D.z);
     auto gx = [&](int64_t c) { return FOLD ? x64[c] + (double)xs[c] : x64[c]; };
     for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
         double out[BS], aout[BS];
@@ -363,8 +364,8 @@ __device__ __forceinline__ void resid_item(const ProbDesc& D, const WorkItem& I,
 #pragma unroll
         for (int j = 0; j < BS; ++j) {
             const int64_t d = row * BS + j;
-            const double res = D.rhs[d] - out[j];
-            const double sc = fabs(D.rhs[d]) + aout[j];
+            const double res = rhs[d] - out[j];
+            const double sc = fabs(rhs[d]) + aout[j];
             ff += sc * sc;
             rt[j] = (T)res;
             r[d] = rt[j];
@@ -394,10 +395,11 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const ProbDesc& D = probs[I.prob];
             if (!D.st->init_pending) continue;
             double s = 0.0;
+            const SFEM_GLOBAL double* rhs = glb(D.rhs);
             for (int64_t i = I.slice0 * 64 * D.bs + threadIdx.x; i < I.slice1 * 64 * D.bs; i += kThreads)
-                s = fma(D.rhs[i], D.rhs[i], s);
+                s = fma(rhs[i], rhs[i], s);
             const double b = block_sum_256(s, sh);
-            if (threadIdx.x == 0) a.part[4 * (int64_t)it + 3] = b;
+            if (threadIdx.x == 0) glb(a.part)[4 * (int64_t)it + 3] = b;
         }
         grid.sync();
         for (int pr = blockIdx.x; pr < a.nprob; pr += G) {
@@ -439,10 +441,10 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 const double b0 = block_sum_256(rr, sh), b1 = block_sum_256(rz, sh + 4);
                 const double b2 = block_sum_256(rp, sh), b3 = block_sum_256(ff, sh + 4);
                 if (threadIdx.x == 0) {
-                    a.partr[4 * (int64_t)it] = b0;
-                    a.partr[4 * (int64_t)it + 1] = b1;
-                    a.partr[4 * (int64_t)it + 2] = b2;
-                    a.partr[4 * (int64_t)it + 3] = b3;
+                    glb(a.partr)[4 * (int64_t)it] = b0;
+                    glb(a.partr)[4 * (int64_t)it + 1] = b1;
+                    glb(a.partr)[4 * (int64_t)it + 2] = b2;
+                    glb(a.partr)[4 * (int64_t)it + 3] = b3;
                 }
             }
             grid.sync();
@@ -501,8 +503,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const ProbDesc& D = probs[I.prob];
             if (D.st->done) continue;
             const T beta = (T)D.st->beta, zc = (T)D.st->zcoef;
-            T* p = (T*)D.p;
-            const T* z = (const T*)D.z;
+            SFEM_GLOBAL T* p = glb((T*)D.p);
+            const SFEM_GLOBAL T* z = glb((const T*)D.z);
             for (int64_t i = I.slice0 * 64 * D.bs + threadIdx.x; i < I.slice1 * 64 * D.bs; i += kThreads)
                 p[i] = fmaT<T>(beta, p[i], zc * z[i]);
         }
@@ -513,8 +515,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const WorkItem I = a.items[it];
             const ProbDesc& D = probs[I.prob];
             if (D.st->done) continue;
-            const T* p = (const T*)D.p;
-            T* q = (T*)D.q;
+            const SFEM_GLOBAL T* p = glb((const T*)D.p);
+            SFEM_GLOBAL T* q = glb((T*)D.q);
             const Vals<TV> vals = desc_vals<TV>(D);
             const int bs = D.bs;
             double dot = 0.0;
@@ -530,7 +532,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 }
             }
             const double b = block_sum_256(dot, sh);
-            if (threadIdx.x == 0) a.part[4 * (int64_t)it] = b;
+            if (threadIdx.x == 0) glb(a.part)[4 * (int64_t)it] = b;
         }
         grid.sync();
 
@@ -547,8 +549,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             else update_item<2, T, FOLD>(D, I, al, fold, rz, rr);
             const double b0 = block_sum_256(rz, sh), b1 = block_sum_256(rr, sh + 4);
             if (threadIdx.x == 0) {
-                a.part[4 * (int64_t)it + 1] = b0;
-                a.part[4 * (int64_t)it + 2] = b1;
+                glb(a.part)[4 * (int64_t)it + 1] = b0;
+                glb(a.part)[4 * (int64_t)it + 2] = b1;
             }
         }
         grid.sync();

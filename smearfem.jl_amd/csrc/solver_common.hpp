@@ -11,6 +11,15 @@ namespace {
 
 constexpr double kEps64 = 2.220446049250313e-16;  // FP64 unit roundoff x 2
 
+// Global-address-space view of a device pointer.  Pointers read from memory
+// (the batch descriptors) are generic to the compiler, which then emits flat
+// loads; accessing through these views gives global loads and stores.
+#define SFEM_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ __forceinline__ const SFEM_GLOBAL T* glb(const T* p) { return (const SFEM_GLOBAL T*)p; }
+template <typename T>
+__device__ __forceinline__ SFEM_GLOBAL T* glb(T* p) { return (SFEM_GLOBAL T*)p; }
+
 constexpr int kThreads = 256;
 
 // ------------------------------------------------------------ helpers --
@@ -126,8 +135,8 @@ __device__ __forceinline__ double csr_row_abs(const int32_t* __restrict__ rowptr
 // non-temporal hint so they do not evict the gathered vector from the caches.
 template <bool NT, typename V>
 __device__ __forceinline__ V ld_stream(const V* p) {
-    if constexpr (NT) return __builtin_nontemporal_load(p);
-    else return *p;
+    if constexpr (NT) return __builtin_nontemporal_load(glb(p));
+    else return *glb(p);
 }
 
 // Operator value sources of the sliced format.  Vals<double> / Vals<float>
@@ -164,8 +173,8 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
                                          const int32_t* __restrict__ scol, Vals<TV> vals, int64_t sl, int lane,
                                          Gather gx, ACC out[BS]) {
     constexpr int B2 = BS * BS;
-    const int64_t base = sl_ptr[sl];
-    const int w = sl_w[sl];
+    const int64_t base = glb(sl_ptr)[sl];
+    const int w = glb(sl_w)[sl];
 #pragma unroll
     for (int j = 0; j < BS; ++j) out[j] = 0;
 #pragma unroll 2
@@ -190,8 +199,8 @@ __device__ __forceinline__ void sell_row_abs(const int64_t* __restrict__ sl_ptr,
                                              const int32_t* __restrict__ scol, const double* __restrict__ sval,
                                              int64_t sl, int lane, Gather gx, double out[BS], double aout[BS]) {
     constexpr int B2 = BS * BS;
-    const int64_t base = sl_ptr[sl];
-    const int w = sl_w[sl];
+    const int64_t base = glb(sl_ptr)[sl];
+    const int w = glb(sl_w)[sl];
 #pragma unroll
     for (int j = 0; j < BS; ++j) out[j] = aout[j] = 0.0;
     for (int k = 0; k < w; ++k) {

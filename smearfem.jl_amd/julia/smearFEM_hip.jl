@@ -6,6 +6,9 @@
 #       -> SparseMatrixCSC{Float64,Int64}                       (src/fem.jl:135)
 #   solve_system(K, f, fixed, q_d; precision=:fp64, rtol=1e-10)
 #       -> q = q_d + C * (K_free \ C'(f - K q_d))               (examples/vector3D.jl:314-322)
+#   assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
+#                      precision=:split, preconditioner=:block_jacobi)
+#       -> q, with K kept on the device (no export), so the node-block formats apply
 #
 # gaussian_quadrature / basis_function stay pure Julia (src/fem.jl:21, :48).
 # Status codes become the exceptions the reference raises on the same inputs.
@@ -14,7 +17,7 @@ module SmearFEMHip
 using SparseArrays
 using LinearAlgebra
 
-export assemble_system, solve_system
+export assemble_system, solve_system, assemble_and_solve
 
 const libsfem = get(ENV, "SMEARFEM_HIP_LIB",
                     joinpath(@__DIR__, "..", "smearfem", "libsmearfem_hip.so"))
@@ -125,7 +128,59 @@ function solve_system(K::SparseMatrixCSC{Float64,Int64}, f, fixed, q_d; precisio
     fv = f === nothing ? zeros(n) : Vector{Float64}(vec(f))
     mask = Vector{UInt8}(fixed .!= 0)
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
-    opts = SolveOpts(precision == :mixed ? 1 : 0, 0, rtol, max_iter, 0.1, 64, 0, 1, 0)
+    opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1, 0)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
+    q = zeros(n)
+    try
+        st = ccall((:sfem_solve, libsfem), Cint,
+                   (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}, Ref{SolveOpts},
+                    Ptr{Float64}, Ptr{Float64}, Ref{SolveStats}),
+                   ctx, Kp[], fv, mask, qd, opts, q, C_NULL, stats)
+        check(ctx, st)
+    finally
+        ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), Kp[])
+    end
+    return q
+end
+
+precision_code(p::Symbol) = p === :fp64 ? 0 : p === :mixed ? 1 : p === :fp32_matrix ? 2 : p === :split ? 3 :
+                            throw(ArgumentError("unknown precision $p"))
+preconditioner_code(p::Symbol) = p === :jacobi ? 0 : p === :block_jacobi ? 1 :
+                                 throw(ArgumentError("unknown preconditioner $p"))
+
+"""
+    assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
+                       precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000)
+
+assemble_system (src/fem.jl:135) and the example's elimination and solve
+(examples/vector3D.jl:314-322) in one call; K stays on the device, so the
+node-block operator formats (split precision, block-Jacobi) are available when
+ID is node-interleaved and every node is fully fixed or fully free.
+"""
+function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
+                            precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000)
+    ctx = context()
+    nodes = Matrix{Float64}(NodeList)
+    ien = Matrix{Int64}(IEN)
+    idm = ID === nothing ? Matrix{Int64}(undef, 0, 0) : Matrix{Int64}(ID)
+    young = Young isa AbstractVector ? Vector{Float64}(Young) : [Float64(Young)]
+    cls = FunctionClass == "Q1" ? 1 : FunctionClass == "Q2" ? 2 : 0
+    Kp = Ref{Ptr{Cvoid}}(C_NULL)
+    st = GC.@preserve nodes ien idm young ccall((:sfem_assemble, libsfem), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Cint, Ptr{Float64}, Int64, Ptr{Int64}, Int64, Cint, Int64,
+         Ptr{Int64}, Int64, Ptr{Float64}, Int64, Float64, Ref{Ptr{Cvoid}}),
+        ctx, ndim, cls, nDof, nodes, size(nodes, 2), ien, size(ien, 1), size(ien, 2), ne^ndim,
+        ID === nothing ? C_NULL : pointer(idm), size(idm, 1), young, length(young), ν, Kp)
+    check(ctx, st)
+    m = Ref{Int64}(0)
+    ccall((:sfem_csr_info, libsfem), Cint, (Ptr{Cvoid}, Ref{Int64}, Ptr{Int64}, Ptr{Int64}, Ptr{Cint}),
+          Kp[], m, C_NULL, C_NULL, C_NULL)
+    n = m[]
+    fv = f === nothing ? zeros(n) : Vector{Float64}(vec(f))
+    mask = Vector{UInt8}(fixed .!= 0)
+    qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
+    opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1,
+                     preconditioner_code(preconditioner))
     stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
     q = zeros(n)
     try
