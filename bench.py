@@ -24,6 +24,7 @@ from __future__ import annotations
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 from pathlib import Path
@@ -88,7 +89,21 @@ def cpu_threads():
     return max(1, min(16, len(os.sched_getaffinity(0))))
 
 
-def cpu_baseline(n_free_full, iters_full, nz_layers=2):
+def cpu_model():
+    """CPU model name and logical CPU count of this host."""
+    name = platform.processor() or "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    name = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"model": name, "logical_cpus": os.cpu_count()}
+
+
+def cpu_baseline(n_free_full, iters_full, nz_layers=8, k_it=300, k_it_mt=1000):
     """The CPU port (oracle/sfem_oracle.c) on a slab of the same mesh:
     nz_layers of the 64 element layers.  Assembly and elimination (single
     thread, as the reference) scale with the element count, one PCG iteration
@@ -110,11 +125,10 @@ def cpu_baseline(n_free_full, iters_full, nz_layers=2):
     t1 = time.perf_counter()
     Kff, rhs = O.eliminate(K, fixed, f)
     t2 = time.perf_counter()
-    k_it = 30
     _, it, _ = O.pcg(Kff, rhs, rtol=0.0, max_iter=k_it)
     t3 = time.perf_counter()
     nth = cpu_threads()
-    _, it_mt, _ = O.pcg_threads(Kff, rhs, nth, rtol=0.0, max_iter=k_it)
+    _, it_mt, _ = O.pcg_threads(Kff, rhs, nth, rtol=0.0, max_iter=k_it_mt)
     t4 = time.perf_counter()
     scale_e = (DIMS[2]) / nz_layers
     full_nnz = 256794606.0
@@ -130,12 +144,14 @@ def cpu_baseline(n_free_full, iters_full, nz_layers=2):
         "kind": "port",
         "sample": (f"oracle (C restatement) on a {dims[0]}x{dims[1]}x{dims[2]} slab: assembly "
                    f"{t1 - t0:.2f}s + elimination {t2 - t1:.2f}s (1 thread) scaled x{scale_e:.0f} by elements, "
-                   f"{k_it} PCG iterations on {nth} threads {t4 - t3:.2f}s scaled by nnz to "
-                   f"{t_iter_mt * 1e3:.1f} ms/iteration x {iters_full} iterations (FP64 GPU count)"),
+                   f"{k_it} PCG iterations on 1 thread {t3 - t2:.2f}s and {k_it_mt} on {nth} threads "
+                   f"{t4 - t3:.2f}s, scaled by nnz to {t_iter_mt * 1e3:.1f} ms/iteration x {iters_full} "
+                   f"iterations (FP64 GPU count)"),
         "single_thread_value": n_free_full / total_1,
         "single_thread_ms_per_iteration": t_iter * 1e3,
         "extrapolated_seconds": total,
         "sample_seconds": t4 - t0,
+        "host": cpu_model(),
     }
 
 
