@@ -238,6 +238,7 @@ typedef struct {
     int64_t not_converged;
     int     grid;             /* blocks of the persistent launch                         */
     int64_t device_bytes;     /* operator + vectors held by the batch                     */
+    int     barrier_grouped;  /* 1: the launches used the per-XCD grouped grid barrier    */
 } sfem_batch_stats;
 
 int  sfem_batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, const double* node_list,

@@ -20,7 +20,6 @@
 // Reference: the per-problem maths is smearFEM's assemble_system
 // (src/fem.jl:135-256) and the example's elimination and solve
 // (examples/vector3D.jl:314-322); batching has no reference counterpart.
-#include <hip/hip_cooperative_groups.h>
 
 #include <algorithm>
 #include <cmath>
@@ -31,9 +30,9 @@
 
 #include "batch.hpp"
 #include "block_ops.hpp"
+#include "grid_barrier.hpp"
 #include "solver_common.hpp"
 
-namespace cg = cooperative_groups;
 
 using namespace sfem;
 
@@ -383,7 +382,8 @@ __device__ __forceinline__ void resid_item(const ProbDesc& D, const WorkItem& I,
 
 template <typename TV, typename T, bool FOLD>
 __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
-    cg::grid_group grid = cg::this_grid();
+    GridBarrier grid{a.bar_ws, a.bar_timeout, 0u, false};
+    if (!grid.start(a.bar_grouped != 0)) return;
     __shared__ double sh[8];
     const int G = gridDim.x, w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const ProbDesc* probs = a.probs;
@@ -401,7 +401,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const double b = block_sum_256(s, sh);
             if (threadIdx.x == 0) glb(a.part)[4 * (int64_t)it + 3] = b;
         }
-        grid.sync();
+        if (!grid.sync()) return;
         for (int pr = blockIdx.x; pr < a.nprob; pr += G) {
             ProbState* st = probs[pr].st;
             if (!st->init_pending) continue;
@@ -425,7 +425,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 *st = S;
             }
         }
-        grid.sync();
+        if (!grid.sync()) return;
     }
 
     for (int64_t k = 0; k < a.iters; ++k) {
@@ -447,7 +447,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                     glb(a.partr)[4 * (int64_t)it + 3] = b3;
                 }
             }
-            grid.sync();
+            if (!grid.sync()) return;
             for (int pr = blockIdx.x; pr < a.nprob; pr += G) {
                 ProbState* st = probs[pr].st;
                 if (!st->need_update || st->done) continue;
@@ -493,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                     *st = S;
                 }
             }
-            grid.sync();
+            if (!grid.sync()) return;
         }
         if (!block_any(a.nprob, [&](int i) { return !probs[i].st->done; })) break;
 
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             for (int64_t i = I.slice0 * 64 * D.bs + threadIdx.x; i < I.slice1 * 64 * D.bs; i += kThreads)
                 p[i] = fmaT<T>(beta, p[i], zc * z[i]);
         }
-        grid.sync();
+        if (!grid.sync()) return;
 
         // ---- S: q = K p and the item partials of p.q ----
         for (int it = blockIdx.x; it < a.nitems; it += G) {
@@ -523,8 +523,8 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             auto gx = [&](int64_t c) { return p[c]; };
             for (int64_t sl = I.slice0 + w; sl < I.slice1; sl += kThreads / 64) {
                 T out[3];
-                if (bs == 3) sell_row<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
-                else sell_row<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
+                if (bs == 3) sell_row_staged<3, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
+                else sell_row_staged<2, TV, T, true>(D.sl_ptr, D.sl_w, D.scol, vals, sl, lane, gx, out);
                 for (int j = 0; j < bs; ++j) {
                     const int64_t d = (sl * 64 + lane) * bs + j;
                     q[d] = out[j];
@@ -534,7 +534,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
             const double b = block_sum_256(dot, sh);
             if (threadIdx.x == 0) glb(a.part)[4 * (int64_t)it] = b;
         }
-        grid.sync();
+        if (!grid.sync()) return;
 
         // ---- U: alpha; x += alpha p; r -= alpha q; z = M^-1 r (per node) ----
         for (int it = blockIdx.x; it < a.nitems; it += G) {
@@ -553,7 +553,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 glb(a.part)[4 * (int64_t)it + 2] = b1;
             }
         }
-        grid.sync();
+        if (!grid.sync()) return;
 
         // ---- F: per-problem scalars and stopping tests ----
         for (int pr = blockIdx.x; pr < a.nprob; pr += G) {
@@ -591,7 +591,7 @@ __global__ __launch_bounds__(kThreads) void table_pcg_kernel(TableLaunch a) {
                 *st = S;
             }
         }
-        grid.sync();
+        if (!grid.sync()) return;
     }
 }
 
@@ -624,8 +624,14 @@ int table_max_blocks_per_cu(int precision) {
     }
 }
 
-void table_launch(sfem_ctx* ctx, int precision, const TableLaunch& a, int grid) {
-    if (a.nitems <= 0 || a.nprob <= 0) return;
+void table_launch(sfem_ctx* ctx, int precision, const TableLaunch& a0, int grid) {
+    if (a0.nitems <= 0 || a0.nprob <= 0) return;
+    if (!ctx->bar_ws) ctx->bar_ws = static_cast<unsigned*>(ctx->alloc->alloc(sizeof(unsigned) * kBarWords));
+    TableLaunch a = a0;
+    a.bar_ws = ctx->bar_ws;
+    a.bar_timeout = ctx->bar_timeout_ticks;
+    a.bar_grouped = ctx->bar_grouped ? 1 : 0;
+    SFEM_HIP(hipMemsetAsync(a.bar_ws, 0, sizeof(unsigned) * kBarWords, ctx->stream));
     switch (precision) {
         case SFEM_PREC_FP64: launch_table<double, double, false>(ctx, a, grid); break;
         case SFEM_PREC_MIXED: launch_table<float, float, true>(ctx, a, grid); break;
@@ -633,6 +639,10 @@ void table_launch(sfem_ctx* ctx, int precision, const TableLaunch& a, int grid) 
         case SFEM_PREC_SPLIT: launch_table<SplitVal, double, false>(ctx, a, grid); break;
         default: fail(SFEM_ERR_ARGUMENT, "unknown precision");
     }
+    unsigned flags[2] = {0u, 0u};
+    d2h(ctx, flags, a.bar_ws + kBarFlags, sizeof(flags));
+    ctx->bar_last_grouped = a.bar_grouped && flags[0] == 0u;
+    if (flags[1]) fail(SFEM_ERR_DEVICE, "persistent solver: a grid barrier wait exceeded its time limit");
 }
 
 void table_fold(sfem_ctx* ctx, double* x64, void* xs, int64_t n) {
@@ -1058,6 +1068,7 @@ int sfem_batch_solve(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts* opts, 
             stats->not_converged = bad;
             stats->grid = table_grid(ctx, o.precision, blocks_per_cu);
             stats->device_bytes = batch_device_bytes(B, o.precision);
+            stats->barrier_grouped = ctx->bar_last_grouped ? 1 : 0;
         }
         if (bad) fail(SFEM_ERR_NOT_CONVERGED, "batch_solve: " + std::to_string(bad) + " problems did not converge");
     });

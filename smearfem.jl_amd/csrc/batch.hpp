@@ -61,6 +61,9 @@ struct TableLaunch {
     double* part;   // 4 per item
     double* partr;  // 4 per item
     int64_t iters;  // loop passes in this launch
+    unsigned* bar_ws;                 // grid barrier workspace (kBarWords, zeroed per launch)
+    unsigned long long bar_timeout;   // barrier wait limit, 100 MHz ticks
+    int bar_grouped;                  // allow the per-XCD grouped barrier
 };
 
 struct BatchProb {

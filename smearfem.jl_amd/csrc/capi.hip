@@ -100,6 +100,7 @@ void ctx_release(sfem_ctx* ctx) {
     if (!ctx || ctx->refs.fetch_sub(1) != 1) return;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->bar_ws) ctx->alloc->free(ctx->bar_ws);
     delete ctx->alloc;
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
@@ -190,6 +191,10 @@ int sfem_ctx_create(int device, sfem_ctx** out) {
         if (nt) ctx->tune_nt = nt[0] == '1';
         const char* ws = std::getenv("SFEM_SELL_WAVE_SLICES");
         if (ws) ctx->tune_wave_slices = std::max(0, std::atoi(ws));
+        const char* bt = std::getenv("SFEM_BARRIER_TIMEOUT_US");
+        if (bt) ctx->bar_timeout_ticks = (unsigned long long)std::max(0LL, std::atoll(bt)) * 100ull;
+        const char* bg = std::getenv("SFEM_BARRIER_GROUPED");
+        if (bg) ctx->bar_grouped = bg[0] != '0';
     });
     if (st) { delete ctx; return st; }
     *out = ctx;

@@ -77,6 +77,13 @@ struct sfem_ctx {
     // slices over at most 8 blocks per CU).
     bool tune_nt = true;        // non-temporal matrix loads
     int tune_wave_slices = 1;   // slices per wavefront
+    // Grid barrier of the persistent solvers: workspace, wait limit (100 MHz
+    // ticks; SFEM_BARRIER_TIMEOUT_US) and whether the per-XCD grouped form may
+    // be used (SFEM_BARRIER_GROUPED=0 forces the per-block form).
+    unsigned* bar_ws = nullptr;
+    unsigned long long bar_timeout_ticks = 10ull * 100000000ull;
+    bool bar_grouped = true;
+    bool bar_last_grouped = false;  // form the last launch used
 };
 
 namespace sfem {

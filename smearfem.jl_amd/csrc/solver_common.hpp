@@ -145,9 +145,15 @@ __device__ __forceinline__ V ld_stream(const V* p) {
 // exactly in FP64 (|v' - v| <= 2^-39 |v|): 6 bytes per value.
 struct SplitVal {};
 
+// `raw` loads a value as stored and `decode` turns it into the arithmetic
+// type, so a caller can issue many loads before it uses any of them.
 template <typename TV>
 struct Vals {
     const TV* v;
+    using Raw = TV;
+    template <bool NT>
+    __device__ __forceinline__ Raw raw(int64_t i) const { return ld_stream<NT>(v + i); }
+    __device__ __forceinline__ TV decode(Raw r) const { return r; }
     template <bool NT>
     __device__ __forceinline__ TV get(int64_t i) const { return ld_stream<NT>(v + i); }
 };
@@ -156,15 +162,22 @@ template <>
 struct Vals<SplitVal> {
     const float* hi;
     const int16_t* lo;
+    struct Raw {
+        float h;
+        int l;
+    };
     template <bool NT>
-    __device__ __forceinline__ double get(int64_t i) const {
-        const float h = ld_stream<NT>(hi + i);
-        const int16_t l = ld_stream<NT>(lo + i);
+    __device__ __forceinline__ Raw raw(int64_t i) const {
+        return Raw{ld_stream<NT>(hi + i), (int)ld_stream<NT>(lo + i)};
+    }
+    __device__ __forceinline__ double decode(Raw r) const {
         // ulp(h) * 2^-16 = 2^(exponent(h) - 23 - 16); below the normal range
         // (|h| < 2^-87, where lo is 0 by construction) the scale is 0
-        const float scale = __int_as_float(max((__float_as_int(h) & 0x7F800000) - (39 << 23), 0));
-        return fma((double)l, (double)scale, (double)h);
+        const float scale = __int_as_float(max((__float_as_int(r.h) & 0x7F800000) - (39 << 23), 0));
+        return fma((double)r.l, (double)scale, (double)r.h);
     }
+    template <bool NT>
+    __device__ __forceinline__ double get(int64_t i) const { return decode(raw<NT>(i)); }
 };
 
 // Node-block sliced format: this lane's block row of slice `sl`.
@@ -193,7 +206,66 @@ __device__ __forceinline__ void sell_row(const int64_t* __restrict__ sl_ptr, con
     }
 }
 
-// sell_row in FP64 plus aout[j] = sum |M_ji x_i| over the row's blocks.
+// sell_row with explicit staging, for kernels whose register budget is set
+// elsewhere (the persistent solvers): per pair of block columns, the column
+// ids and all value loads are issued before any of them is used, then the
+// gathers, then the arithmetic.  Scheduling barriers keep the compiler from
+// interleaving loads with their uses, which would leave one load in flight
+// per wavefront.  Same arithmetic, in the same order, as sell_row.
+template <int BS, typename TV, typename ACC, bool NT = false, typename Gather>
+__device__ __forceinline__ void sell_row_staged(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
+                                                const int32_t* __restrict__ scol, Vals<TV> vals, int64_t sl,
+                                                int lane, Gather gx, ACC out[BS]) {
+    constexpr int B2 = BS * BS;
+    using Raw = typename Vals<TV>::Raw;
+    const int64_t base = glb(sl_ptr)[sl];
+    const int w = glb(sl_w)[sl];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[j] = 0;
+    int k = 0;
+    for (; k + 2 <= w; k += 2) {
+        const int64_t s0 = base + (int64_t)k * 64, s1 = s0 + 64;
+        const int64_t c0 = ld_stream<NT>(scol + s0 + lane), c1 = ld_stream<NT>(scol + s1 + lane);
+        Raw r0[B2], r1[B2];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) r0[m] = vals.template raw<NT>(s0 * B2 + lane + m * 64);
+#pragma unroll
+        for (int m = 0; m < B2; ++m) r1[m] = vals.template raw<NT>(s1 * B2 + lane + m * 64);
+        __builtin_amdgcn_sched_barrier(0);
+        ACC x0[BS], x1[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x0[i] = gx(c0 * BS + i);
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x1[i] = gx(c1 * BS + i);
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)vals.decode(r0[j * BS + i]), x0[i], out[j]);
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)vals.decode(r1[j * BS + i]), x1[i], out[j]);
+    }
+    if (k < w) {
+        const int64_t s0 = base + (int64_t)k * 64;
+        const int64_t c0 = ld_stream<NT>(scol + s0 + lane);
+        Raw r0[B2];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) r0[m] = vals.template raw<NT>(s0 * B2 + lane + m * 64);
+        ACC x0[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x0[i] = gx(c0 * BS + i);
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) out[j] = fmaT<ACC>((ACC)vals.decode(r0[j * BS + i]), x0[i], out[j]);
+    }
+}
+
+// sell_row in FP64 plus aout[j] = sum |M_ji x_i| over the row's blocks,
+// staged like sell_row_staged (pairs of block columns; the arithmetic order
+// per row is k, then j, then i).
 template <int BS, typename Gather>
 __device__ __forceinline__ void sell_row_abs(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                              const int32_t* __restrict__ scol, const double* __restrict__ sval,
@@ -203,21 +275,43 @@ __device__ __forceinline__ void sell_row_abs(const int64_t* __restrict__ sl_ptr,
     const int w = glb(sl_w)[sl];
 #pragma unroll
     for (int j = 0; j < BS; ++j) out[j] = aout[j] = 0.0;
-    for (int k = 0; k < w; ++k) {
-        const int64_t slot = base + (int64_t)k * 64;
-        const int64_t c = ld_stream<true>(scol + slot + lane);
-        double xv[BS];
-#pragma unroll
-        for (int i = 0; i < BS; ++i) xv[i] = gx(c * BS + i);
-        const double* v = sval + slot * B2 + lane;
+    auto step = [&](const double (&m)[B2], const double (&xv)[BS]) {
 #pragma unroll
         for (int j = 0; j < BS; ++j)
 #pragma unroll
             for (int i = 0; i < BS; ++i) {
-                const double m = ld_stream<true>(v + (j * BS + i) * 64);
-                out[j] = fma(m, xv[i], out[j]);
-                aout[j] = fma(fabs(m), fabs(xv[i]), aout[j]);
+                out[j] = fma(m[j * BS + i], xv[i], out[j]);
+                aout[j] = fma(fabs(m[j * BS + i]), fabs(xv[i]), aout[j]);
             }
+    };
+    int k = 0;
+    for (; k + 2 <= w; k += 2) {
+        const int64_t s0 = base + (int64_t)k * 64, s1 = s0 + 64;
+        const int64_t c0 = ld_stream<true>(scol + s0 + lane), c1 = ld_stream<true>(scol + s1 + lane);
+        double m0[B2], m1[B2];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) m0[m] = ld_stream<true>(sval + s0 * B2 + lane + m * 64);
+#pragma unroll
+        for (int m = 0; m < B2; ++m) m1[m] = ld_stream<true>(sval + s1 * B2 + lane + m * 64);
+        __builtin_amdgcn_sched_barrier(0);
+        double x0[BS], x1[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x0[i] = gx(c0 * BS + i);
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x1[i] = gx(c1 * BS + i);
+        __builtin_amdgcn_sched_barrier(0);
+        step(m0, x0);
+        step(m1, x1);
+    }
+    if (k < w) {
+        const int64_t s0 = base + (int64_t)k * 64;
+        const int64_t c0 = ld_stream<true>(scol + s0 + lane);
+        double m0[B2], x0[BS];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) m0[m] = ld_stream<true>(sval + s0 * B2 + lane + m * 64);
+#pragma unroll
+        for (int i = 0; i < BS; ++i) x0[i] = gx(c0 * BS + i);
+        step(m0, x0);
     }
 }
 

@@ -71,6 +71,7 @@ class BatchStats(C.Structure):
         ("not_converged", C.c_int64),
         ("grid", C.c_int),
         ("device_bytes", C.c_int64),
+        ("barrier_grouped", C.c_int),
     ]
 
 

@@ -102,6 +102,49 @@ def test_batch_problem_bitwise_alone_and_in_mix(ctx, precision, pc):
     B.close()
 
 
+def _context_with_env(monkeypatch, **env):
+    for k, v in env.items():
+        monkeypatch.setenv(k, v)
+    c = smearfem.Context(0)
+    for k in env:
+        monkeypatch.delenv(k)
+    return c
+
+
+def test_barrier_forms_bitwise(ctx, monkeypatch):
+    """The per-XCD grouped grid barrier (default) and the per-block form give
+    bitwise the same solutions; the default launch uses the grouped form."""
+    print("[case] batch barrier forms", flush=True)
+    probs = small_hex8_batch(5, seed=8)
+    B = Batch(probs, ctx)
+    st, _ = B.solve("fp64", rtol=1e-11, item_slices=1, blocks_per_cu=2)
+    q_grouped = B.solution()
+    B.close()
+    assert st["barrier_grouped"] == 1
+    c = _context_with_env(monkeypatch, SFEM_BARRIER_GROUPED="0")
+    B = Batch(probs, c)
+    st, _ = B.solve("fp64", rtol=1e-11, item_slices=1, blocks_per_cu=2)
+    q_block = B.solution()
+    B.close()
+    c.close()
+    assert st["barrier_grouped"] == 0
+    for a, b in zip(q_grouped, q_block):
+        assert np.array_equal(a, b)
+
+
+def test_barrier_wait_limit_ends_launch_with_error(monkeypatch):
+    """A grid-barrier wait longer than the limit aborts the launch (every block
+    returns at its next barrier) and the solve reports a device error.  The
+    limit is forced to 0 from the host; no wait is unbounded."""
+    print("[case] batch barrier wait limit", flush=True)
+    c = _context_with_env(monkeypatch, SFEM_BARRIER_TIMEOUT_US="0")
+    B = Batch(small_hex8_batch(3, seed=2), c)
+    with pytest.raises(smearfem.DeviceError):
+        B.solve("fp64", rtol=1e-10, item_slices=1)
+    B.close()
+    c.close()
+
+
 def test_batch_iteration_cap_and_errors(ctx):
     probs = small_hex8_batch(3, seed=9)
     B = Batch(probs, ctx)
