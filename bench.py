@@ -103,7 +103,7 @@ def cpu_model():
     return {"model": name, "logical_cpus": os.cpu_count()}
 
 
-def cpu_baseline(n_free_full, iters_full, nz_layers=8, k_it=300, k_it_mt=1000):
+def cpu_baseline(n_free_full, iters_full, nz_layers=8, k_it=400, k_it_mt=3000):
     """The CPU port (oracle/sfem_oracle.c) on a slab of the same mesh:
     nz_layers of the 64 element layers.  Assembly and elimination (single
     thread, as the reference) scale with the element count, one PCG iteration
@@ -286,6 +286,7 @@ def run_batch(ctx, nprob=512, seed=4):
         t = (st["solve_ms"] + B.assemble_ms + B.eliminate_ms) * 1e-3
         out["forms"][form] = {
             "solve_ms": st["solve_ms"], "launches": st["launches"], "grid": st["grid"],
+            "barrier_grouped": st["barrier_grouped"],
             "item_slices": tune["item_slices"], "blocks_per_cu": tune["blocks_per_cu"],
             "tune_ms_per_iter": tune["ms_per_iter"], "tuned_from_cache": tune["cached"],
             "iterations_max": int(its.max()), "iterations_median": float(np.median(its)),

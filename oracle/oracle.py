@@ -6,7 +6,10 @@ the ``cpu_baseline`` leg of bench.py, as the checker -- never by the product.
 ctypes wrapper over ``sfem_oracle.c`` (a C restatement of the reference, see
 its header for file:line citations) plus small pure-Python restatements:
 
-* ``meshgrid_ref``  -- examples/vector3D.jl:10-130 (cube meshes, as written)
+* ``meshgrid_ref``  -- examples/vector3D.jl:10-130 (cube meshes, as written,
+                       optionally with the top / bottom face connectivity)
+* ``inflate_sphere_ref`` -- src/PostProcess.jl:30-46
+* ``setboundaryCond_ref`` -- examples/vector3D.jl:133-173
 * ``sparse_ref``    -- Julia ``sparse(I,J,V)`` semantics with pure-Python
                        loops (tiny inputs only), an independent check of the
                        C combine
@@ -64,6 +67,8 @@ def lib():
         L.or_dense_spd_solve.argtypes = [i64, P(d), P(d), P(d)]
         L.or_pcg.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d), d, i64, P(i64), P(d)]
         L.or_pcg_mt.argtypes = [i64, P(i64), P(i32), P(d), P(d), P(d), d, i64, C.c_int, P(i64), P(d)]
+        L.or_surface_coo.argtypes = [P(d), i64, P(i64), P(i64), i64, C.c_int, C.c_int, C.c_int, P(i64), i64,
+                                     P(i64), P(i64), P(d), P(i64)]
         _lib = L
     return _lib
 
@@ -178,11 +183,15 @@ def sparse_ref(I, J, V):
     return acc, (int(m), int(n))
 
 
-def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim):
-    """examples/vector3D.jl:10-130 restated with plain loops (NodeList, IEN, ID)."""
+def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim, with_faces=False):
+    """examples/vector3D.jl:10-130 restated with plain loops.  Returns
+    (NodeList, IEN, ID), or with ``with_faces`` (NodeList, IEN, ID, IEN_top,
+    IEN_btm) as the reference does (:94-120; 2D: :45-56)."""
     NodeList = np.zeros((ndim, (ne + 1) ** ndim))
     IEN = np.zeros((ne ** ndim, 2 ** ndim), dtype=np.int64)
     ID = np.zeros(((ne + 1) ** ndim, ndim), dtype=np.int64)
+    IEN_top = np.zeros((ne ** (ndim - 1), 2 ** (ndim - 1)), dtype=np.int64)
+    IEN_btm = np.zeros((ne ** (ndim - 1), 2 ** (ndim - 1)), dtype=np.int64)
     x = np.linspace(x0, x1, ne + 1)
     y = np.linspace(y0, y1, ne + 1)
     if ndim == 2:
@@ -199,6 +208,10 @@ def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim):
             for i in range(1, ne + 1):
                 IEN[n - 1] = [(j - 1) * (ne + 1) + i, (j - 1) * (ne + 1) + i + 1, j * (ne + 1) + i + 1,
                               j * (ne + 1) + i]
+                if j == 1:
+                    IEN_btm[i - 1] = [IEN[n - 1, 0], IEN[n - 1, 1]]
+                elif j == ne:
+                    IEN_top[i - 1] = [IEN[n - 1, 3], IEN[n - 1, 2]]
                 n += 1
     else:
         z = np.linspace(z0, z1, ne + 1)
@@ -211,6 +224,7 @@ def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim):
                         ID[m - 1, l - 1] = ndim * (m - 1) + l
                     m += 1
         n = 1
+        nt = nb = 1
         p = (ne + 1) ** 2
         for k in range(1, ne + 1):
             for j in range(1, ne + 1):
@@ -219,8 +233,85 @@ def meshgrid_ref(x0, x1, y0, y1, z0, z1, ne, ndim):
                                   (k - 1) * p + j * (ne + 1) + i + 1, (k - 1) * p + j * (ne + 1) + i,
                                   k * p + (j - 1) * (ne + 1) + i, k * p + (j - 1) * (ne + 1) + i + 1,
                                   k * p + j * (ne + 1) + i + 1, k * p + j * (ne + 1) + i]
+                    if k == 1:
+                        IEN_btm[nb - 1] = IEN[n - 1, 0:4]
+                        nb += 1
+                    elif k == ne:
+                        IEN_top[nt - 1] = IEN[n - 1, 4:8]
+                        nt += 1
                     n += 1
+    if with_faces:
+        return NodeList, IEN, ID, IEN_top, IEN_btm
     return NodeList, IEN, ID
+
+
+def inflate_sphere_ref(NodeList, x0, x1, y0, y1):
+    """src/PostProcess.jl:30-46 with plain loops, on a copy (the reference
+    overwrites its argument in place, so in examples/vector3D.jl:292 NodeList
+    itself becomes the cylinder).  The (x, y) of each node is moved to radius
+    max(|x - cx|, |y - cy|) along its direction from the centre; the centre is
+    not added back, so the result is centred at the origin; a node at the
+    centre goes to (0, 0) (``scale ≈ 0`` holds only for scale == 0)."""
+    out = np.array(NodeList, dtype=np.float64, copy=True)
+    cx, cy = 0.5 * (x0 + x1), 0.5 * (y0 + y1)
+    for i in range(out.shape[1]):
+        dx, dy = out[0, i] - cx, out[1, i] - cy
+        scale = max(abs(dx), abs(dy))
+        if scale == 0.0:
+            out[0, i], out[1, i] = 0.0, 0.0
+        else:
+            r = np.sqrt(dx * dx + dy * dy)
+            out[0, i], out[1, i] = scale * dx / r, scale * dy / r
+    return out
+
+
+def setboundaryCond_ref(NodeList, ne, ndim, d, nDof=3):
+    """examples/vector3D.jl:133-173: z dof of nodes on z == 0 fixed to 0, on
+    z == 1 fixed to -d (exact comparisons, first match).  Returns (q_d, fixed)
+    with fixed the mask of the columns removed from C (rCol)."""
+    n = nDof * (ne + 1) ** ndim
+    q_d = np.zeros(n)
+    fixed = np.zeros(n, dtype=np.uint8)
+    for node in range(NodeList.shape[1]):
+        z = NodeList[2, node]
+        if z == 0:
+            q_d[3 * node + 2] = 0.0
+            fixed[3 * node + 2] = 1
+        elif z == 1:
+            q_d[3 * node + 2] = -d
+            fixed[3 * node + 2] = 1
+    return q_d, fixed
+
+
+def surface_coo(NodeList, IEN_btm, IEN_top, ID, nDof=3):
+    """apply_boundary_conditions (examples/vector3D.jl:175-264) up to sparse():
+    (E, J, V), bottom faces first, then top."""
+    nodes = np.asfortranarray(NodeList, dtype=np.float64)
+    top = np.asfortranarray(IEN_top, dtype=np.int64)
+    btm = np.asfortranarray(IEN_btm, dtype=np.int64)
+    idm = np.asfortranarray(ID, dtype=np.int64)
+    nf, nfn = btm.shape
+    L = nDof * nfn
+    n = 2 * nf * L * L
+    E = np.zeros(n, dtype=np.int64)
+    J = np.zeros(n, dtype=np.int64)
+    V = np.zeros(n, dtype=np.float64)
+    err = C.c_int64(-1)
+    st = lib().or_surface_coo(_p(nodes, C.c_double), nodes.shape[1], _p(btm, C.c_int64), _p(top, C.c_int64), nf,
+                              nfn, nodes.shape[0], nDof, _p(idm, C.c_int64), idm.shape[0], _p(E, C.c_int64),
+                              _p(J, C.c_int64), _p(V, C.c_double), C.byref(err))
+    if st:
+        raise OracleError(st, err.value)
+    return E, J, V
+
+
+def apply_boundary_conditions(ne, NodeList, IEN, IEN_top, IEN_btm, ndim, FunctionClass="Q1", ID=None, nDof=3):
+    """The reference's surface penalty b = sparse(E, J, V) -> scipy csc_matrix."""
+    import scipy.sparse as sp
+
+    E, J, V = surface_coo(NodeList, IEN_btm, IEN_top, ID, nDof)
+    m, n, colptr, rowval, nzval = sparse_csc(E, J, V)
+    return sp.csc_matrix((nzval, rowval - 1, colptr - 1), shape=(m, n))
 
 
 # ---------------------------------------------------- elimination / solve --

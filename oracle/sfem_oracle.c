@@ -17,6 +17,7 @@
  *                            stable counting sort by row, duplicates combined
  *                            with + in input order, explicit zeros kept,
  *                            transpose to CSC; dims = max index.
+ *   apply_boundary_conditions examples/vector3D.jl:175-264 (surface penalty b)
  *   Dirichlet elimination    examples/vector3D.jl:151,170,315-316,320-322
  *   dense solve              examples/vector3D.jl:318 (inv(Matrix(K_free)))
  * plus FP64 Jacobi-PCG (no reference counterpart; CPU baseline for sizes a
@@ -366,6 +367,86 @@ int or_assemble_coo(int ndim, int cls, int ndof, const double* nodes, int64_t n_
         }
     }
     *err_elem = -1;
+    return OR_OK;
+}
+
+/* ---- examples/vector3D.jl:175-264 (apply_boundary_conditions) ---------- *
+ * COO of the surface penalty b on the bottom and top faces, before sparse():
+ * length 2 * n_faces * (ndof * nfn)^2 (:182-184), bottom faces first (inz_btm,
+ * :235), then top faces (inz_top, :236).  Per face and Gauss point (x, y in
+ * the order [ξ1 ξ2 ξ2 ξ1], [η1 η1 η2 η2], weights wξ*wη, :203-208):
+ * N, ΔN = basis_function(x, y) (2D Q1), dxdξ = coords * ΔN (3x2, :219-220),
+ * w = wgp * |dxdξ[:,1] x dxdξ[:,2]| (:222-223), be = M'M with
+ * M[d, (a-1)*nDof + d] = N_a (:225-230), so be[(a,i),(b,j)] = N_a N_b if i == j
+ * else 0; V[inz] += w * be[i,j] in Gauss-point order (:241-249).
+ * Only ndim = 3, nfn = 4, nDof = 3 runs in the reference: ndim = 2 makes
+ * coords*ΔN a DimensionMismatch (1x2 ΔN, :213-220), and nDof != 3 a
+ * DimensionMismatch in M[1,1:nDof:end] = N (:227).  The norm is the 2-norm
+ * sqrt(c1^2 + c2^2 + c3^2) (Julia's norm calls BLAS nrm2, which may scale;
+ * the two agree to rounding). */
+int or_surface_coo(const double* nodes, int64_t n_nodes, const int64_t* ien_btm, const int64_t* ien_top,
+                   int64_t n_faces, int nfn, int ndim, int ndof, const int64_t* id, int64_t id_rows,
+                   int64_t* E, int64_t* J, double* V, int64_t* err_face) {
+    *err_face = -1;
+    if (ndim != 3 || nfn != 4 || ndof != 3) return OR_ERR_DIMENSION;
+    double xi[2], wx[2];
+    or_gaussian_quadrature(-1, 1, 2, xi, wx);
+    const double gxs[4] = {xi[0], xi[1], xi[1], xi[0]};
+    const double gys[4] = {xi[0], xi[0], xi[1], xi[1]};
+    const double gws[4] = {wx[0] * wx[0], wx[1] * wx[0], wx[1] * wx[1], wx[0] * wx[1]};
+    const int L = nfn * ndof;          /* 12 */
+    const int64_t LL = (int64_t)L * L; /* length(be) = 144 */
+    for (int64_t e = 0; e < n_faces; ++e) {
+        *err_face = e;
+        double ct[12], cb[12]; /* coords_top / coords_btm, 3 x 4 column-major */
+        for (int k = 0; k < nfn; ++k) {
+            const int64_t nt = ien_top[k * n_faces + e], nb = ien_btm[k * n_faces + e];
+            if (nt < 1 || nt > n_nodes || nb < 1 || nb > n_nodes) return OR_ERR_BOUNDS;
+            for (int d = 0; d < 3; ++d) {
+                ct[d + 3 * k] = nodes[d + 3 * (nt - 1)];
+                cb[d + 3 * k] = nodes[d + 3 * (nb - 1)];
+            }
+        }
+        for (int g = 0; g < 4; ++g) {
+            double N[4], dN[8];
+            int nf, nr, nc;
+            or_basis_function(2, gxs[g], gys[g], 0, 1, N, dN, &nf, &nr, &nc);
+            double tt[6], tb[6]; /* dxdξ, 3 x 2 column-major */
+            for (int k = 0; k < 2; ++k)
+                for (int d = 0; d < 3; ++d) {
+                    double st = 0, sb = 0;
+                    for (int q = 0; q < 4; ++q) {
+                        st += ct[d + 3 * q] * dN[q + 4 * k];
+                        sb += cb[d + 3 * q] * dN[q + 4 * k];
+                    }
+                    tt[d + 3 * k] = st;
+                    tb[d + 3 * k] = sb;
+                }
+            double xt[3] = {tt[1] * tt[5] - tt[2] * tt[4], tt[2] * tt[3] - tt[0] * tt[5], tt[0] * tt[4] - tt[1] * tt[3]};
+            double xb[3] = {tb[1] * tb[5] - tb[2] * tb[4], tb[2] * tb[3] - tb[0] * tb[5], tb[0] * tb[4] - tb[1] * tb[3]};
+            const double w_top = gws[g] * sqrt(xt[0] * xt[0] + xt[1] * xt[1] + xt[2] * xt[2]);
+            const double w_btm = gws[g] * sqrt(xb[0] * xb[0] + xb[1] * xb[1] + xb[2] * xb[2]);
+            for (int iN = 0; iN < nfn; ++iN)
+                for (int jN = 0; jN < nfn; ++jN)
+                    for (int iD = 0; iD < ndof; ++iD)
+                        for (int jD = 0; jD < ndof; ++jD) {
+                            const double be = iD == jD ? N[iN] * N[jN] : 0.0;
+                            const int64_t off = (int64_t)iN * ndof * L + (int64_t)jN * ndof * ndof + iD * ndof + jD;
+                            const int64_t inz_btm = LL * e + off;
+                            const int64_t inz_top = LL * n_faces + LL * e + off;
+                            const int64_t ti = ien_top[iN * n_faces + e], tj = ien_top[jN * n_faces + e];
+                            const int64_t bi = ien_btm[iN * n_faces + e], bj = ien_btm[jN * n_faces + e];
+                            if (ti > id_rows || tj > id_rows || bi > id_rows || bj > id_rows) return OR_ERR_BOUNDS;
+                            E[inz_top] = id[iD * id_rows + (ti - 1)];
+                            J[inz_top] = id[jD * id_rows + (tj - 1)];
+                            V[inz_top] += w_top * be;
+                            E[inz_btm] = id[iD * id_rows + (bi - 1)];
+                            J[inz_btm] = id[jD * id_rows + (bj - 1)];
+                            V[inz_btm] += w_btm * be;
+                        }
+        }
+    }
+    *err_face = -1;
     return OR_OK;
 }
 
