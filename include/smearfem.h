@@ -206,6 +206,65 @@ int  sfem_solve(sfem_ctx* ctx, const sfem_csr* K, const double* f, const uint8_t
                 const double* q_d, const sfem_solve_opts* opts, double* q, double* reactions,
                 sfem_solve_stats* stats);
 
+/* ---- the example's boundary machinery and displacement sweep -----------
+ * examples/vector3D.jl:10-338 and src/PostProcess.jl:30-46.  Meshes, face
+ * lists, penalty matrices and constraints stay on the device. */
+
+/* apply_boundary_conditions(ne, NodeList, IEN, IEN_top, IEN_btm, ndim,
+ * FunctionClass, ID, nDof) (examples/vector3D.jl:175-264): the surface
+ * penalty b = sparse(E,J,V) with, per face and 2x2 Gauss point,
+ * V += |t1 x t2| * N_a N_b on equal dof components (bottom faces first in
+ * COO order, then top).  IEN_btm / IEN_top: n_faces x nfn, column-major,
+ * 1-based; ID: id_rows x n_dof, column-major.  Only ndim = 3, nfn = 4,
+ * n_dof = 3 runs in the reference; anything else is SFEM_ERR_DIMENSION.
+ * Errors follow the reference's face loop (BoundsError at the first face
+ * whose nodes or ids are out of range). */
+int  sfem_surface_mesh_create(sfem_ctx* ctx, int ndim, int n_dof, const double* node_list, int64_t n_nodes,
+                              const int64_t* ien_btm, const int64_t* ien_top, int64_t n_faces, int nfn,
+                              const int64_t* id, int64_t id_rows, sfem_mesh** out);
+int  sfem_assemble_surface(sfem_ctx* ctx, int ndim, int n_dof, const double* node_list, int64_t n_nodes,
+                           const int64_t* ien_btm, const int64_t* ien_top, int64_t n_faces, int nfn,
+                           const int64_t* id, int64_t id_rows, sfem_csr** out);
+
+/* K_bar = K + beta*b (examples/vector3D.jl:308): union pattern, values
+ * K_ij + fl(beta*b_ij) (K_ij + 0.0 where b has no entry, 0.0 + fl(beta*b_ij)
+ * where K has none).  SparseArrays drops entries whose sum is exactly zero
+ * from the stored pattern; this block format keeps them as stored zeros.
+ * Different sizes: SFEM_ERR_DIMENSION (as Julia's +). */
+int  sfem_csr_add(sfem_ctx* ctx, const sfem_csr* K, double beta, const sfem_csr* b, sfem_csr** out);
+
+/* Structured mesh on the device: meshgrid(x0,x1,y0,y1,z0,z1,ne,ndim)
+ * (examples/vector3D.jl:10-130) generalised to ne[0] x ne[1] (x ne[2])
+ * elements; node numbering x fastest, Q4/Hex8 IEN in the reference's
+ * order, ID[m,l] = ndim(m-1)+l, coordinates x0 + i*(x1-x0)/ne with the end
+ * point exact.  box = {x0,x1,y0,y1[,z0,z1]}.  SFEM_MESH_INFLATE maps (x, y)
+ * as inflate_sphere(NodeList, x0, x1, y0, y1) (src/PostProcess.jl:30-46).
+ * faces (3D, may be NULL): the bottom (z = z0) and top (z = z1) face lists
+ * IEN_btm / IEN_top (:96-115) as a surface mesh for sfem_assemble_mesh. */
+#define SFEM_MESH_INFLATE 1
+int  sfem_meshgrid(sfem_ctx* ctx, int ndim, const int64_t* ne, const double* box, int flags,
+                   const double* young, int64_t n_young, double nu, sfem_mesh** mesh, sfem_mesh** faces);
+/* Sizes of a mesh, and its arrays in Julia's layouts (NodeList ndim x nNodes,
+ * IEN nElem x nen, ID nNodes x nDof; column-major, 1-based).  Any output
+ * pointer may be NULL. */
+int  sfem_mesh_info(const sfem_mesh* mesh, int* ndim, int* nen, int* n_dof, int64_t* n_nodes, int64_t* n_elem);
+int  sfem_mesh_export(sfem_ctx* ctx, const sfem_mesh* mesh, double* node_list, int64_t* ien, int64_t* id);
+
+/* Dirichlet constraints from coordinate planes, the rule of setboundaryCond
+ * (examples/vector3D.jl:133-173) generalised: rule r fixes dof dof[r]
+ * (1-based) of every node whose coordinate axis[r] (1-based) equals
+ * coord[r] exactly, with value value[r]; for each (node, dof) the first
+ * matching rule wins (the reference's if/elseif).  f: n_total loads or NULL. */
+int  sfem_bc_planes(sfem_ctx* ctx, const sfem_mesh* mesh, int64_t n_total, int nrules, const int* axis,
+                    const double* coord, const int* dof, const double* value, const double* f, sfem_bc** out);
+
+/* New loads and prescribed values for an eliminated system whose fixed set
+ * is unchanged (the displacement sweep, examples/vector3D.jl:310-322):
+ * rhs = C'(f - K q_d) summed as sfem_eliminate does (bitwise the same rhs);
+ * the solution is kept as the next solve's start (opts.x0_zero = 0).  A
+ * different fixed set is SFEM_ERR_ARGUMENT. */
+int  sfem_system_set_bc(sfem_ctx* ctx, sfem_system* sys, const sfem_bc* bc);
+
 /* ---- batches: many meshes, one persistent solver launch ---------------
  * No counterpart in the reference (each assemble_system / backslash call
  * there handles one mesh, src/fem.jl:135-256); every problem is the same

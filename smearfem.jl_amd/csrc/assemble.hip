@@ -196,8 +196,9 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     }
     if (code) {
         char buf[160];
-        snprintf(buf, sizeof buf, "assemble_system: %s at element %lld", name_of_code(code),
-                 (long long)(key >> 3) + 1);
+        snprintf(buf, sizeof buf, m->face ? "apply_boundary_conditions: %s at face %lld"
+                                          : "assemble_system: %s at element %lld",
+                 name_of_code(code), (long long)(key >> 3) + 1);
         fail(code, buf);
     }
     if (ndof > 1 && vr.min_dof == -1) fail(SFEM_ERR_ARGUMENT, "sparse: row/column index < 1 (ID entry < 1)");

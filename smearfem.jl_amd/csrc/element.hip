@@ -29,7 +29,7 @@ namespace {
 __global__ __launch_bounds__(256) void validate_kernel(const int32_t* __restrict__ ien, int nen,
                                                        int64_t n_elem, int ndof,
                                                        const int32_t* __restrict__ dof,
-                                                       int64_t id_rows, ValidateOut* out) {
+                                                       int64_t id_rows, int64_t period, ValidateOut* out) {
     __shared__ unsigned long long s_key[4];
     __shared__ int s_max[4], s_min[4], s_blk[4];
     const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -39,14 +39,15 @@ __global__ __launch_bounds__(256) void validate_kernel(const int32_t* __restrict
         bool bad = false;
         for (int a = 0; a < nen; ++a)
             if (ien[e * nen + a] < 0) bad = true;
+        const unsigned long long ek = (unsigned long long)(period > 0 ? e % period : e);
         if (bad) {
-            key = (unsigned long long)e * 8 + 0;
+            key = ek * 8 + 0;
         } else if (ndof == 1) {
             for (int a = 0; a < nen; ++a) mx = max(mx, ien[e * nen + a]);
         } else {
             for (int a = 0; a < nen; ++a) {
                 const int node = ien[e * nen + a];
-                if (node >= id_rows) { key = (unsigned long long)e * 8 + 4; break; }
+                if (node >= id_rows) { key = ek * 8 + 4; break; }
                 const int d0 = dof[(int64_t)node * ndof];
                 if (d0 < 0 || d0 % ndof != 0) blk = 0;
                 for (int d = 0; d < ndof; ++d) {
@@ -320,12 +321,16 @@ void launch_elem(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* out)
 void launch_validate(sfem_ctx* ctx, const sfem_mesh* m, ValidateOut* d_out) {
     if (m->n_elem == 0) return;
     hipLaunchKernelGGL(validate_kernel, dim3(grid_for(m->n_elem, 256)), dim3(256), 0, ctx->stream,
-                       m->ien.get(), m->nen, m->n_elem, m->ndof, m->dof.get(), m->id_rows, d_out);
+                       m->ien.get(), m->nen, m->n_elem, m->ndof, m->dof.get(), m->id_rows, m->key_period, d_out);
     SFEM_LAUNCH_CHECK();
 }
 
 void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out) {
     if (m->n_elem == 0) return;
+    if (m->face) {
+        launch_face_mass(ctx, m, V);
+        return;
+    }
     const int kind = m->ndim * 100 + m->nen * 10 + m->ndof;
     switch (kind) {
         case 242: launch_elem<2, 4, 2>(ctx, m, V, d_out); break;  // Q4 plane stress

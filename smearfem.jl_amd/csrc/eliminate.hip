@@ -97,6 +97,37 @@ __global__ void csr_fill_kernel(const int64_t* __restrict__ bcolptr, const int32
     dinv[row] = diag != 0.0 ? 1.0 / diag : 1.0;
 }
 
+// rhs[row] = f[s] - (K q_d)[s] over the fixed columns, the sum csr_fill_kernel
+// forms (same code, same order), for new loads / prescribed values.
+__global__ void rhs_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                           const double* __restrict__ bval, const int32_t* __restrict__ free_of,
+                           const double* __restrict__ qd, const double* __restrict__ f, int64_t n, int bs,
+                           double* __restrict__ rhs) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t row = free_of[s];
+    if (row < 0) return;
+    const int64_t cu = s / bs;
+    const int j = (int)(s - cu * bs);
+    const int bs2 = bs * bs;
+    double lift = 0.0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t r = browidx[u];
+        for (int i = 0; i < bs; ++i) {
+            const int64_t d = r * bs + i;
+            const double v = bval[u * bs2 + i * bs + j];
+            if (free_of[d] < 0) lift += v * qd[d];  // (K q_d)[s], columns ascending as the CSC matvec sums
+        }
+    }
+    rhs[row] = f[s] - lift;
+}
+
+__global__ void mask_diff_kernel(const uint8_t* __restrict__ fixed, const int32_t* __restrict__ free_of, int64_t n,
+                                 int* __restrict__ diff) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s < n && ((fixed[s] != 0) != (free_of[s] < 0))) atomicOr(diff, 1);
+}
+
 __global__ void blk_count_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
                                  const int32_t* __restrict__ ufree_of, int64_t nunits, int32_t* __restrict__ blen) {
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -374,6 +405,27 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     S->t_eliminate_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+}
+
+void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {
+    if (bc->n != S->n) fail(SFEM_ERR_DIMENSION, "system_set_bc: BC length differs from the system");
+    const int64_t n = S->n;
+    const dim3 gn(grid_for(n, 256));
+    DevBuf<int> diff(ctx, 1);
+    memset_d(ctx, diff.get(), 0, sizeof(int));
+    hipLaunchKernelGGL(mask_diff_kernel, gn, dim3(256), 0, ctx->stream, bc->fixed.get(), S->free_of.get(), n,
+                       diff.get());
+    SFEM_LAUNCH_CHECK();
+    int d = 0;
+    d2h(ctx, &d, diff.get(), sizeof(int));
+    if (d) fail(SFEM_ERR_ARGUMENT, "system_set_bc: the fixed dofs differ from the eliminated system's");
+    SFEM_HIP(hipMemcpyAsync(S->qd.get(), bc->qd.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    SFEM_HIP(hipMemcpyAsync(S->f.get(), bc->f.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
+    const sfem_csr* K = S->K;
+    hipLaunchKernelGGL(rhs_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(), K->bval.get(),
+                       S->free_of.get(), S->qd.get(), S->f.get(), n, S->bs, S->rhs.get());
+    SFEM_LAUNCH_CHECK();
+    sync(ctx);
 }
 
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions) {

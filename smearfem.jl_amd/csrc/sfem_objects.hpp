@@ -20,6 +20,11 @@ struct sfem_mesh {
     // (code, key) with key = element*8 + step, or code 0.
     int pre_code = 0;
     int64_t pre_key = INT64_MAX;
+    // Surface mesh of the penalty b: elements are the bottom faces then the
+    // top faces (nen = 4 in 3D); errors are reported per face index, i.e.
+    // element % key_period.
+    bool face = false;
+    int64_t key_period = 0;
 };
 
 // Assembled K, kept as a block CSC: column-unit c holds blocks (row-unit r,
@@ -76,6 +81,10 @@ void exclusive_scan_u32_dev(sfem_ctx* ctx, const uint32_t* in, uint32_t* out, in
                             uint32_t* d_total);
 // Exclusive scan into 64-bit offsets; returns the total.
 int64_t exclusive_scan_i64(sfem_ctx* ctx, const int64_t* in, int64_t* out, int64_t n);
+
+// ---- surface.hip ----
+// COO values of the surface penalty, same layout as launch_element_stiffness.
+void launch_face_mass(sfem_ctx* ctx, const sfem_mesh* m, double* V);
 
 // ---- assemble.hip ----
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K);
@@ -139,6 +148,7 @@ namespace sfem {
 // ---- eliminate.hip ----
 void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S);
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions);
+void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc);
 // ---- pcg.hip ----
 void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
 void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,

@@ -12,6 +12,8 @@ from .mesh import StructuredMesh, clamp_x0, face_load, random_face_load, structu
 from .solve import BoundaryConditions, System, solve_system, stream_copy_bench
 from .batch import Batch, Problem, autotune, make_problem, uneven_hex8_problems
 from .serve import Server, random_request, run_poisson
+from .surface import (DeviceMesh, PlaneRule, apply_boundary_conditions, bc_planes, csr_add, displacement_sweep,
+                      meshgrid, setboundaryCond)
 
 __all__ = [
     "ArgumentError", "BoundsError", "DeviceError", "DeviceMemoryError", "DimensionMismatch",
@@ -20,5 +22,7 @@ __all__ = [
     "default_context", "gaussian_quadrature", "StructuredMesh", "clamp_x0", "face_load",
     "random_face_load", "structured_mesh", "BoundaryConditions", "System", "solve_system",
     "stream_copy_bench", "Batch", "Problem", "autotune", "make_problem", "uneven_hex8_problems", "Server",
+    "DeviceMesh", "PlaneRule", "apply_boundary_conditions", "bc_planes", "csr_add", "displacement_sweep",
+    "meshgrid", "setboundaryCond",
     "random_request", "run_poisson",
 ]

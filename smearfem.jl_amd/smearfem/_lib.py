@@ -127,6 +127,20 @@ _PROTOS = [
      [c_vp, c_vp, c_dblp, c_u8p, c_dblp, C.POINTER(SolveOpts), c_dblp, c_dblp, C.POINTER(SolveStats)]),
     ("sfem_system_spmv_bench", C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_dblp, c_dblp]),
     ("sfem_stream_copy_bench", C.c_int, [c_vp, C.c_int64, C.c_int, c_dblp]),
+    ("sfem_surface_mesh_create", C.c_int,
+     [c_vp, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, c_i64p, C.c_int64, C.c_int, c_i64p, C.c_int64,
+      C.POINTER(c_vp)]),
+    ("sfem_assemble_surface", C.c_int,
+     [c_vp, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, c_i64p, C.c_int64, C.c_int, c_i64p, C.c_int64,
+      C.POINTER(c_vp)]),
+    ("sfem_csr_add", C.c_int, [c_vp, c_vp, C.c_double, c_vp, C.POINTER(c_vp)]),
+    ("sfem_meshgrid", C.c_int,
+     [c_vp, C.c_int, c_i64p, c_dblp, C.c_int, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp), C.POINTER(c_vp)]),
+    ("sfem_mesh_info", C.c_int, [c_vp, c_intp, c_intp, c_intp, c_i64p, c_i64p]),
+    ("sfem_mesh_export", C.c_int, [c_vp, c_vp, c_dblp, c_i64p, c_i64p]),
+    ("sfem_bc_planes", C.c_int,
+     [c_vp, c_vp, C.c_int64, C.c_int, c_intp, c_dblp, c_intp, c_dblp, c_dblp, C.POINTER(c_vp)]),
+    ("sfem_system_set_bc", C.c_int, [c_vp, c_vp, c_vp]),
     ("sfem_batch_create", C.c_int,
      [c_vp, C.c_int64, C.c_int, C.c_int, c_dblp, c_i64p, c_i64p, C.c_int, c_i64p, c_dblp, c_dblp, c_dblp, c_u8p,
       c_dblp, C.POINTER(c_vp)]),

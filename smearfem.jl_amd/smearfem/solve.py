@@ -44,6 +44,13 @@ class BoundaryConditions:
                                           None if mv is None else _ptr(mv, C.c_uint8),
                                           None if qv is None else _ptr(qv, C.c_double), C.byref(h)))
         self.handle = h
+        self.n = int(n)
+
+    @classmethod
+    def _wrap(cls, ctx: Context, handle: C.c_void_p, n: int) -> "BoundaryConditions":
+        obj = cls.__new__(cls)
+        obj.ctx, obj.handle, obj.n = ctx, handle, int(n)
+        return obj
 
     def close(self) -> None:
         if self.handle:
@@ -102,6 +109,12 @@ class System:
             self.ctx.check(st)
         stats["status"] = st
         return stats
+
+    def set_bc(self, bc: BoundaryConditions) -> None:
+        """New loads / prescribed values with the same fixed dofs
+        (``sfem_system_set_bc``); the current solution stays as the start of
+        the next ``solve(x0_zero=False)``."""
+        self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
     def solution(self, reactions: bool = False):
         q = np.zeros(self.n)
