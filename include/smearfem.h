@@ -20,6 +20,12 @@
  *                             <- setboundaryCond + C'*K*C   examples/vector3D.jl:133-173, 314-316
  *   sfem_system_solve         <- inv(Matrix(K_free))*C'*(-K*q_d)   examples/vector3D.jl:318-320
  *   sfem_system_solution      <- q = q_d + C*q_f            examples/vector3D.jl:322
+ *   sfem_meshgrid             <- meshgrid (+ inflate_sphere) examples/vector3D.jl:10-130, src/PostProcess.jl:30
+ *   sfem_assemble_surface     <- apply_boundary_conditions  examples/vector3D.jl:175-264
+ *   sfem_csr_add              <- K_bar = K + beta*b         examples/vector3D.jl:308
+ *   sfem_bc_planes            <- setboundaryCond            examples/vector3D.jl:133-173
+ *   sfem_system_set_bc / sfem_system_solve_many
+ *                             <- the displacement sweep     examples/vector3D.jl:310-322
  *
  * Threading: a context owns one HIP stream.  Calls on one context are not
  * re-entrant; different contexts may be used from different threads.
@@ -265,6 +271,7 @@ int  sfem_bc_planes(sfem_ctx* ctx, const sfem_mesh* mesh, int64_t n_total, int n
  * different fixed set is SFEM_ERR_ARGUMENT. */
 int  sfem_system_set_bc(sfem_ctx* ctx, sfem_system* sys, const sfem_bc* bc);
 
+
 /* ---- batches: many meshes, one persistent solver launch ---------------
  * No counterpart in the reference (each assemble_system / backslash call
  * there handles one mesh, src/fem.jl:135-256); every problem is the same
@@ -288,6 +295,19 @@ typedef struct {
     int     stop_reason;      /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap */
     double  floor_residual;   /* attainable relative residual estimate (see sfem_solve_stats) */
 } sfem_batch_problem_stats;
+
+/* k right-hand sides of one eliminated system at once (the sweep's
+ * K_free q_f = C'(f - K_bar q_d) for k displacements, examples/vector3D.jl:
+ * 310-322): column j uses q_d[:, j] and f[:, j] (n x k column-major; NULL =
+ * zeros, prescribed values read on the fixed dofs only) and returns
+ * q[:, j] = q_d + C q_f.  One FP64 point-Jacobi PCG per column, all columns
+ * sharing each pass over K_ff; each column stops when its FP64 true residual
+ * meets opts->rtol (restarting from it otherwise).  1 <= k <= 64; per: k
+ * entries (fp64_updates = restarts), may be NULL; solve_ms may be NULL.
+ * SFEM_ERR_NOT_CONVERGED if a column misses rtol. */
+int  sfem_system_solve_many(sfem_ctx* ctx, sfem_system* sys, int k, const double* q_d, const double* f,
+                            const sfem_solve_opts* opts, double* q, sfem_batch_problem_stats* per,
+                            double* solve_ms);
 
 typedef struct {
     double  solve_ms;         /* device time of all launches (HIP events)                 */

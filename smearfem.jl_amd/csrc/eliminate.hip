@@ -122,6 +122,37 @@ __global__ void rhs_kernel(const int64_t* __restrict__ bcolptr, const int32_t* _
     rhs[row] = f[s] - lift;
 }
 
+// rhs_many[row*kp + j] = f_j[s] - (K q_d_j)[s] for k columns (q_d, f: n x k
+// column-major, NULL = zeros); per column the same sum as rhs_kernel.
+__global__ void rhs_many_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                const double* __restrict__ bval, const int32_t* __restrict__ free_of,
+                                const double* __restrict__ qd, const double* __restrict__ f, int64_t n, int bs, int k,
+                                int kp, double* __restrict__ rhs) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t row = free_of[s];
+    if (row < 0) return;
+    const int64_t cu = s / bs;
+    const int jd = (int)(s - cu * bs);
+    const int bs2 = bs * bs;
+    for (int c = 0; c < k; ++c) {
+        double lift = 0.0;
+        if (qd) {
+            const double* q = qd + (int64_t)c * n;
+            for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+                const int64_t r = browidx[u];
+                for (int i = 0; i < bs; ++i) {
+                    const int64_t d = r * bs + i;
+                    const double v = bval[u * bs2 + i * bs + jd];
+                    if (free_of[d] < 0) lift += v * q[d];  // (K q_d)[s], columns ascending as the CSC matvec sums
+                }
+            }
+        }
+        rhs[(int64_t)row * kp + c] = (f ? f[(int64_t)c * n + s] : 0.0) - lift;
+    }
+    for (int c = k; c < kp; ++c) rhs[(int64_t)row * kp + c] = 0.0;
+}
+
 __global__ void mask_diff_kernel(const uint8_t* __restrict__ fixed, const int32_t* __restrict__ free_of, int64_t n,
                                  int* __restrict__ diff) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -426,6 +457,14 @@ void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {
                        S->free_of.get(), S->qd.get(), S->f.get(), n, S->bs, S->rhs.get());
     SFEM_LAUNCH_CHECK();
     sync(ctx);
+}
+
+void launch_rhs_many(sfem_ctx* ctx, const sfem_system* S, int k, int kp, const double* qd, const double* f,
+                     double* rhs) {
+    const sfem_csr* K = S->K;
+    hipLaunchKernelGGL(rhs_many_kernel, dim3(grid_for(S->n, 256)), dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                       K->browidx.get(), K->bval.get(), S->free_of.get(), qd, f, S->n, S->bs, k, kp, rhs);
+    SFEM_LAUNCH_CHECK();
 }
 
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions) {

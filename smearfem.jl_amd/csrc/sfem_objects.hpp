@@ -149,6 +149,9 @@ namespace sfem {
 void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S);
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions);
 void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc);
+// k right-hand sides C'(f_j - K q_d_j) into rhs (n_free x kp, row-major; columns >= k zero).
+void launch_rhs_many(sfem_ctx* ctx, const sfem_system* S, int k, int kp, const double* qd, const double* f,
+                     double* rhs);
 // ---- pcg.hip ----
 void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
 void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,

@@ -141,6 +141,8 @@ _PROTOS = [
     ("sfem_bc_planes", C.c_int,
      [c_vp, c_vp, C.c_int64, C.c_int, c_intp, c_dblp, c_intp, c_dblp, c_dblp, C.POINTER(c_vp)]),
     ("sfem_system_set_bc", C.c_int, [c_vp, c_vp, c_vp]),
+    ("sfem_system_solve_many", C.c_int,
+     [c_vp, c_vp, C.c_int, c_dblp, c_dblp, C.POINTER(SolveOpts), c_dblp, C.POINTER(BatchProblemStats), c_dblp]),
     ("sfem_batch_create", C.c_int,
      [c_vp, C.c_int64, C.c_int, C.c_int, c_dblp, c_i64p, c_i64p, C.c_int, c_i64p, c_dblp, c_dblp, c_dblp, c_u8p,
       c_dblp, C.POINTER(c_vp)]),

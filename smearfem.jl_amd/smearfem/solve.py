@@ -116,6 +116,33 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
+    def solve_many(self, q_d=None, f=None, rtol=1e-10, max_iter=200000, check_every=32, raise_on_fail=True):
+        """k right-hand sides at once (``sfem_system_solve_many``): column j of
+        the n x k arrays q_d / f gives q[:, j] = q_d + C K_ff^-1 C'(f - K q_d).
+        Returns (q n x k, per-column stats list, device ms)."""
+        cols = [a for a in (q_d, f) if a is not None]
+        if not cols:
+            raise ValueError("solve_many needs q_d or f")
+        k = np.asarray(cols[0]).reshape(self.n, -1).shape[1]
+        qv = None if q_d is None else np.asfortranarray(np.asarray(q_d, dtype=np.float64).reshape(self.n, k))
+        fv = None if f is None else np.asfortranarray(np.asarray(f, dtype=np.float64).reshape(self.n, k))
+        o = _lib.SolveOpts()
+        self.ctx._lib.sfem_solve_opts_default(C.byref(o))
+        o.rtol = float(rtol)
+        o.max_iter = int(max_iter)
+        o.check_every = int(check_every)
+        q = np.zeros((self.n, k), order="F")
+        per = (_lib.BatchProblemStats * k)()
+        ms = C.c_double()
+        st = self.ctx._lib.sfem_system_solve_many(self.ctx.handle, self.handle, k,
+                                                  None if qv is None else _ptr(qv, C.c_double),
+                                                  None if fv is None else _ptr(fv, C.c_double), C.byref(o),
+                                                  _ptr(q, C.c_double), per, C.byref(ms))
+        if st and raise_on_fail:
+            self.ctx.check(st)
+        stats = [{name: getattr(p, name) for name, _ in _lib.BatchProblemStats._fields_} for p in per]
+        return q, stats, ms.value
+
     def solution(self, reactions: bool = False):
         q = np.zeros(self.n)
         R = np.zeros(self.n) if reactions else None

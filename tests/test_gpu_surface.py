@@ -199,3 +199,56 @@ def test_bc_planes_rules(ctx):
         bc_planes(mesh, n, [PlaneRule(4, 0.0, 1, 0.0)])
     for h in (S, bc, K, mesh):
         h.close()
+
+
+@pytest.mark.parametrize("k", [1, 3, 5, 50])
+def test_solve_many_vs_oracle(ctx, k):
+    """k displacements of the sweep at once; every column within 1e-8 of the
+    oracle's direct solve, each column's rhs the one set_bc / eliminate form."""
+    print(f"[case] solve_many k={k}", flush=True)
+    ne = 4
+    deltas = [0.001 + 0.01 * i for i in range(k)]
+    mesh, faces = meshgrid(0, 1, 0, 1, 0, 1, ne, 3, inflate=True, Young=40.0, nu=0.4, ctx=ctx)
+    K, b = mesh.assemble(), faces.assemble()
+    Kbar = csr_add(K, 100.0, b)
+    bc = setboundaryCond(mesh, deltas[0])
+    S = System(Kbar, bc)
+    nodes, _, _ = mesh.export()
+    qd = np.zeros((S.n, k))
+    top = nodes[2] == 1.0
+    for j, d in enumerate(deltas):
+        qd[2::3, j][top] = -d
+    q, per, ms = S.solve_many(q_d=qd, rtol=1e-11)
+    want = oracle_sweep(ne, deltas) if k <= 5 else oracle_sweep(ne, deltas[:: max(1, k // 5)])
+    cols = range(k) if k <= 5 else range(0, k, max(1, k // 5))
+    for j, w in zip(cols, want):
+        assert per[j]["stop_reason"] == 1 and per[j]["rel_residual"] <= 1e-11
+        assert relerr(q[:, j], w) <= ERR
+    # one column alone equals the single-rhs solve to the tolerance
+    st = S.solve("fp64", rtol=1e-11)
+    assert relerr(q[:, 0], S.solution()) <= 1e-9
+    assert abs(per[0]["iterations"] - st["iterations"]) <= 0.1 * st["iterations"] + 32
+    for h in (S, bc, K, b, Kbar, mesh, faces):
+        h.close()
+
+
+def test_solve_many_loads_and_errors(ctx):
+    print("[case] solve_many loads", flush=True)
+    m = smearfem.structured_mesh((6, 3, 3))
+    fixed = smearfem.clamp_x0(m)
+    K = smearfem.assemble_system(0, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, 1.0, 0.3, n_elem=m.n_elem, ctx=ctx,
+                                 device=True)
+    rng = np.random.default_rng(0)
+    F = rng.standard_normal((m.n_dof, 4))
+    S = System(K, smearfem.BoundaryConditions(ctx, m.n_dof, None, fixed))
+    q, per, _ = S.solve_many(f=F, rtol=1e-12)
+    Ko = O.assemble_system(0, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, 1.0, 0.3, n_elem=m.n_elem)
+    for j in range(4):
+        Kff, rhs = O.eliminate(Ko, fixed, F[:, j])
+        assert relerr(q[:, j], O.recombine(fixed, None, O.dense_solve(Kff, rhs))) <= ERR
+    with pytest.raises(smearfem.ArgumentError):
+        S.solve_many(f=np.zeros((m.n_dof, 65)))
+    with pytest.raises(smearfem.NotConvergedError):
+        S.solve_many(f=F, rtol=1e-12, max_iter=5)
+    S.close()
+    K.close()
