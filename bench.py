@@ -327,6 +327,57 @@ def run_serve(ctx, precision, rates, nreq=48, seed=11, preconditioner="block_jac
     return out
 
 
+def run_sweep(ctx, ne=64, nd=50, beta=100.0):
+    """The example's displacement sweep (examples/vector3D.jl:270-322) at ne^3
+    elements: unit cube inflated to the cylinder, Young 40, nu 0.4,
+    K_bar = K + 100 b, u_z = 0 at z = 0 and u_z = -d at z = 1 for
+    d = 0.001:0.01:0.5.  Two forms: one solve per d (rhs replaced, started from
+    the previous solution) and all d at once (multi-rhs PCG).  Errors are
+    against a tight FP64 solve at the last d scaled by d (q is linear in d)."""
+    from smearfem.solve import System
+    from smearfem.surface import csr_add, displacement_sweep, meshgrid, setboundaryCond
+
+    deltas = [0.001 + 0.01 * i for i in range(nd)]
+    out = {"workload": f"cylinder {ne}^3 Hex8, {nd} prescribed displacements", "beta": beta, "rtol": RTOL}
+    # reference: the last displacement to 1e-13
+    mesh, faces = meshgrid(0, 1, 0, 1, 0, 1, ne, 3, inflate=True, Young=40.0, nu=0.4, ctx=ctx)
+    K, b = mesh.assemble(), faces.assemble()
+    Kbar = csr_add(K, beta, b)
+    bc = setboundaryCond(mesh, deltas[-1])
+    S = System(Kbar, bc)
+    S.solve("fp64", rtol=1e-13, raise_on_fail=False)
+    q_ref = S.solution() / deltas[-1]
+    out["n_free"] = S.n_free
+    # all displacements at once
+    nodes, _, _ = mesh.export()
+    qd = np.zeros((S.n, nd))
+    top = nodes[2] == 1.0
+    for j, d in enumerate(deltas):
+        qd[2::3, j][top] = -d
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    q, per, dev_ms = S.solve_many(q_d=qd, rtol=RTOL)
+    wall = time.perf_counter() - t0
+    err = max(float(np.linalg.norm(q[:, j] - d * q_ref) / np.linalg.norm(d * q_ref)) for j, d in enumerate(deltas))
+    out["multi_rhs"] = {"seconds": wall, "device_ms": dev_ms, "iterations_max": max(c["iterations"] for c in per),
+                        "solved_dof_per_s": S.n_free * nd / wall, "max_rel_err": err}
+    log(f"sweep multi-rhs: {nd} displacements in {wall:.2f} s ({S.n_free * nd / wall:.3e} DOF/s), err {err:.2e}")
+    for h in (S, bc, K, b, Kbar, mesh, faces):
+        h.close()
+    # one displacement at a time, warm-started, including the device build
+    t0 = time.perf_counter()
+    r = displacement_sweep(ne, deltas, beta=beta, rtol=RTOL, warm_start=True, ctx=ctx)
+    wall = time.perf_counter() - t0
+    err = max(float(np.linalg.norm(qj - d * q_ref) / np.linalg.norm(d * q_ref)) for qj, d in zip(r.q, deltas))
+    its = [st["iterations"] for st in r.stats]
+    out["sequential"] = {"seconds": wall, "build_ms": r.build_ms, "solve_ms": r.solve_ms,
+                         "iterations_mean": float(np.mean(its)), "solved_dof_per_s": r.n_free * nd / wall,
+                         "max_rel_err": err}
+    log(f"sweep sequential: {wall:.2f} s incl. build {r.build_ms:.0f} ms ({r.n_free * nd / wall:.3e} DOF/s), "
+        f"mean {np.mean(its):.0f} iterations, err {err:.2e}")
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,8 +386,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-sweep", action="store_true")
     ap.add_argument("--forms-only", action="store_true",
-                    help="run only the batch and serving forms (no headline, no JSON line)")
+                    help="run only the batch, serving and sweep forms (no headline, no JSON line)")
     ap.add_argument("--serve-precision", default="fp64")
     ap.add_argument("--serve-rates", default="4,16,64")
     ap.add_argument("--only", type=str, default=None, help="comma list of precision/format paths to time")
@@ -374,6 +426,8 @@ def main():
         if not args.no_serve:
             rates = [float(v) for v in args.serve_rates.split(",") if v]
             print(json.dumps({"serving": run_serve(ctx, args.serve_precision, rates)}), flush=True)
+        if not args.no_sweep:
+            print(json.dumps({"sweep": run_sweep(ctx)}), flush=True)
         return
     paths, hinfo = run_headline(args, ctx, m, young, fixed, f)
     valid = {k: v for k, v in paths.items() if v["rel_err_vs_fp64_ref"] <= ERR_BAR}
@@ -431,6 +485,8 @@ def main():
     if not args.no_serve:
         rates = [float(v) for v in args.serve_rates.split(",") if v]
         out["serving"] = run_serve(ctx, args.serve_precision, rates)
+    if not args.no_sweep:
+        out["sweep"] = run_sweep(ctx)
     if not args.no_cpu:
         iters_full = (plain or best)["iterations"]
         log("cpu baseline sample")
