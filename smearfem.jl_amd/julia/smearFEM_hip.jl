@@ -9,6 +9,11 @@
 #   assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
 #                      precision=:split, preconditioner=:block_jacobi)
 #       -> q, with K kept on the device (no export), so the node-block formats apply
+#   apply_boundary_conditions(ne, NodeList, IEN, IEN_top, IEN_btm, ndim, FunctionClass, ID, nDof=3)
+#       -> b, the surface penalty                                (examples/vector3D.jl:175-264)
+#   displacement_sweep(ne, deltas; Young=40, ν=0.4, β=100, rtol=1e-10)
+#       -> [q(d) for d in deltas], the example's loop with all d solved together
+#          (meshgrid + inflate_sphere + K + β*b + setboundaryCond, :270-322)
 #
 # gaussian_quadrature / basis_function stay pure Julia (src/fem.jl:21, :48).
 # Status codes become the exceptions the reference raises on the same inputs.
@@ -17,7 +22,7 @@ module SmearFEMHip
 using SparseArrays
 using LinearAlgebra
 
-export assemble_system, solve_system, assemble_and_solve
+export assemble_system, solve_system, assemble_and_solve, apply_boundary_conditions, displacement_sweep
 
 const libsfem = get(ENV, "SMEARFEM_HIP_LIB",
                     joinpath(@__DIR__, "..", "smearfem", "libsmearfem_hip.so"))
@@ -193,6 +198,112 @@ function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Yo
         ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), Kp[])
     end
     return q
+end
+
+"""
+    apply_boundary_conditions(ne, NodeList, IEN, IEN_top, IEN_btm, ndim, FunctionClass, ID=nothing, nDof=3)
+
+The surface penalty b of examples/vector3D.jl:175-264 assembled on the GPU
+(IEN and FunctionClass are unused there too).  Same errors: BoundsError at
+the first bad face, DimensionMismatch unless ndim = 3, 4-node faces, nDof = 3.
+"""
+function apply_boundary_conditions(ne, NodeList, IEN, IEN_top, IEN_btm, ndim, FunctionClass="Q1", ID=nothing,
+                                   nDof=3)
+    ctx = context()
+    nodes = Matrix{Float64}(NodeList)
+    top = Matrix{Int64}(IEN_top)
+    btm = Matrix{Int64}(IEN_btm)
+    idm = Matrix{Int64}(ID)
+    out = Ref{Ptr{Cvoid}}(C_NULL)
+    st = GC.@preserve nodes top btm idm ccall((:sfem_assemble_surface, libsfem), Cint,
+        (Ptr{Cvoid}, Cint, Cint, Ptr{Float64}, Int64, Ptr{Int64}, Ptr{Int64}, Int64, Cint, Ptr{Int64}, Int64,
+         Ref{Ptr{Cvoid}}),
+        ctx, ndim, nDof, nodes, size(nodes, 2), btm, top, size(btm, 1), size(btm, 2), idm, size(idm, 1), out)
+    check(ctx, st)
+    try
+        return export_csc(ctx, out[])
+    finally
+        ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), out[])
+    end
+end
+
+# sfem_batch_problem_stats (per right-hand side of sfem_system_solve_many)
+struct ColumnStats
+    iterations::Int64
+    restarts::Int64
+    rel_residual::Float64
+    n_free::Int64
+    stop_reason::Cint
+    floor_residual::Float64
+end
+
+"""
+    displacement_sweep(ne, deltas; Young=40, ν=0.4, β=100, rtol=1e-10)
+
+The example's solve loop (examples/vector3D.jl:270-322) on the GPU: the unit
+cube meshed with ne^3 Hex8 and inflated to the cylinder (K and b both on it,
+as inflate_sphere overwrites NodeList there), K_bar = K + β b, u_z = 0 on
+z = 0 and u_z = -d on z = 1, q(d) = q_d + C K_free^-1 C'(-K_bar q_d) for every d.
+The displacements are solved together, up to 64 per pass over K_free.
+Returns a vector of q (3 (ne+1)^3 values each, node-interleaved as ID).
+"""
+function displacement_sweep(ne, deltas; Young=40.0, ν=0.4, β=100.0, rtol=1e-10, max_iter=200000)
+    ctx = context()
+    mesh, faces, K, b, Kbar, bc, sys = (Ref{Ptr{Cvoid}}(C_NULL) for _ in 1:7)
+    dims = Int64[ne, ne, ne]
+    box = Float64[0, 1, 0, 1, 0, 1]
+    young = [Float64(Young)]
+    try
+        check(ctx, ccall((:sfem_meshgrid, libsfem), Cint,
+                         (Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Float64}, Cint, Ptr{Float64}, Int64, Float64,
+                          Ref{Ptr{Cvoid}}, Ref{Ptr{Cvoid}}),
+                         ctx, 3, dims, box, 1, young, 1, ν, mesh, faces))
+        check(ctx, ccall((:sfem_assemble_mesh, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}),
+                         ctx, mesh[], 0, K))
+        check(ctx, ccall((:sfem_assemble_mesh, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ref{Ptr{Cvoid}}),
+                         ctx, faces[], 0, b))
+        check(ctx, ccall((:sfem_csr_add, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Float64, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                         ctx, K[], β, b[], Kbar))
+        n = 3 * (ne + 1)^3
+        # setboundaryCond's fixed set (:155-166): u_z on z == 0 and on z == 1
+        axis, coord, dof = Cint[3, 3], Float64[0, 1], Cint[3, 3]
+        value = Float64[0, -first(deltas)]
+        check(ctx, ccall((:sfem_bc_planes, libsfem), Cint,
+                         (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Cint, Ptr{Cint}, Ptr{Float64}, Ptr{Cint}, Ptr{Float64},
+                          Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                         ctx, mesh[], n, 2, axis, coord, dof, value, C_NULL, bc))
+        check(ctx, ccall((:sfem_eliminate, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                         ctx, Kbar[], bc[], sys))
+        # node z coordinates of the generated mesh (x fastest, then y, then z)
+        z = [Float64(k) / ne for k in 0:ne for j in 0:ne for i in 0:ne]
+        z[end - (ne + 1)^2 + 1:end] .= 1.0
+        qs = Vector{Vector{Float64}}()
+        opts = SolveOpts(0, 0, rtol, max_iter, 0.1, 32, 0, 1, 0)
+        for group in Iterators.partition(collect(deltas), 64)
+            k = length(group)
+            qd = zeros(n, k)
+            for (j, d) in enumerate(group), (m, zm) in enumerate(z)
+                zm == 1.0 && (qd[3m, j] = -d)
+            end
+            q = zeros(n, k)
+            per = Vector{ColumnStats}(undef, k)
+            check(ctx, ccall((:sfem_system_solve_many, libsfem), Cint,
+                             (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Float64}, Ptr{Float64}, Ref{SolveOpts}, Ptr{Float64},
+                              Ptr{ColumnStats}, Ptr{Float64}),
+                             ctx, sys[], k, qd, C_NULL, opts, q, per, C_NULL))
+            append!(qs, [q[:, j] for j in 1:k])
+        end
+        return qs
+    finally
+        sys[] != C_NULL && ccall((:sfem_system_destroy, libsfem), Cvoid, (Ptr{Cvoid},), sys[])
+        bc[] != C_NULL && ccall((:sfem_bc_destroy, libsfem), Cvoid, (Ptr{Cvoid},), bc[])
+        for h in (Kbar, b, K)
+            h[] != C_NULL && ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), h[])
+        end
+        for h in (faces, mesh)
+            h[] != C_NULL && ccall((:sfem_mesh_destroy, libsfem), Cvoid, (Ptr{Cvoid},), h[])
+        end
+    end
 end
 
 end # module

@@ -14,6 +14,7 @@ from .batch import Batch, Problem, autotune, make_problem, uneven_hex8_problems
 from .serve import Server, random_request, run_poisson
 from .surface import (DeviceMesh, PlaneRule, apply_boundary_conditions, bc_planes, csr_add, displacement_sweep,
                       meshgrid, setboundaryCond)
+from .vtk import write_scene, write_vtk
 
 __all__ = [
     "ArgumentError", "BoundsError", "DeviceError", "DeviceMemoryError", "DimensionMismatch",
@@ -23,6 +24,6 @@ __all__ = [
     "random_face_load", "structured_mesh", "BoundaryConditions", "System", "solve_system",
     "stream_copy_bench", "Batch", "Problem", "autotune", "make_problem", "uneven_hex8_problems", "Server",
     "DeviceMesh", "PlaneRule", "apply_boundary_conditions", "bc_planes", "csr_add", "displacement_sweep",
-    "meshgrid", "setboundaryCond",
+    "meshgrid", "setboundaryCond", "write_scene", "write_vtk",
     "random_request", "run_poisson",
 ]

@@ -149,6 +149,10 @@ def test_displacement_sweep_vs_oracle(ctx, precision):
         assert relerr(q, w) <= ERR
     # q is linear in d
     assert relerr(res.q[3] / 0.2, res.q[1] / 0.011) <= 1e-8
+    if precision == "fp64":
+        tog = displacement_sweep(4, deltas, rtol=1e-12, together=True, ctx=ctx)
+        for q, w, st in zip(tog.q, want, tog.stats):
+            assert st["stop_reason"] == 1 and relerr(q, w) <= ERR
 
 
 def test_set_bc_matches_fresh_elimination(ctx):
