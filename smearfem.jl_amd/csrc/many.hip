@@ -80,6 +80,49 @@ __global__ __launch_bounds__(kThreadsM) void spmm_kernel(const int32_t* __restri
     if (part) block_partial<KP>(dot, sh, part + lb * KP);
 }
 
+// Q = K_ff P from K's node blocks (bs = 3).  K is bitwise symmetric, so
+// column unit u of its block CSC is node row u: rows 3u+jj of K_ff are
+// sum over blocks (r, v) of v[i*3+jj] x_i with x_i = P[free_of[3r+i]] (0 for a
+// fixed dof).  Each gathered node row of P serves three output rows, a third
+// of the gather traffic of the scalar rows; partially fixed nodes need no
+// separate format.
+template <int KP>
+__global__ __launch_bounds__(kThreadsM) void spmm_blk3_kernel(const int64_t* __restrict__ bcolptr,
+                                                              const int32_t* __restrict__ browidx,
+                                                              const double* __restrict__ bval,
+                                                              const int32_t* __restrict__ free_of, int64_t nunits,
+                                                              int64_t chunk, const double* __restrict__ P,
+                                                              double* __restrict__ Q, double* __restrict__ part) {
+    const int64_t lb = logical_block();
+    constexpr int UPB = kThreadsM / KP;
+    __shared__ double sh[kThreadsM];
+    const int j = threadIdx.x % KP, ul = threadIdx.x / KP;
+    double dot = 0.0;
+    const int64_t u1 = min(nunits, (lb + 1) * chunk);
+    for (int64_t u = lb * chunk + ul; u < u1; u += UPB) {
+        const int32_t f0 = free_of[3 * u], f1 = free_of[3 * u + 1], f2 = free_of[3 * u + 2];
+        if (f0 < 0 && f1 < 0 && f2 < 0) continue;
+        double a0 = 0.0, a1 = 0.0, a2 = 0.0;
+        const int64_t e1 = bcolptr[u + 1];
+#pragma unroll 2
+        for (int64_t e = bcolptr[u]; e < e1; ++e) {
+            const int64_t r = browidx[e];
+            const double* v = bval + e * 9;
+            const int32_t g0 = free_of[3 * r], g1 = free_of[3 * r + 1], g2 = free_of[3 * r + 2];
+            const double x0 = g0 >= 0 ? P[(int64_t)g0 * KP + j] : 0.0;
+            const double x1 = g1 >= 0 ? P[(int64_t)g1 * KP + j] : 0.0;
+            const double x2 = g2 >= 0 ? P[(int64_t)g2 * KP + j] : 0.0;
+            a0 = fma(v[0], x0, fma(v[3], x1, fma(v[6], x2, a0)));
+            a1 = fma(v[1], x0, fma(v[4], x1, fma(v[7], x2, a1)));
+            a2 = fma(v[2], x0, fma(v[5], x1, fma(v[8], x2, a2)));
+        }
+        if (f0 >= 0) { Q[(int64_t)f0 * KP + j] = a0; dot = fma(P[(int64_t)f0 * KP + j], a0, dot); }
+        if (f1 >= 0) { Q[(int64_t)f1 * KP + j] = a1; dot = fma(P[(int64_t)f1 * KP + j], a1, dot); }
+        if (f2 >= 0) { Q[(int64_t)f2 * KP + j] = a2; dot = fma(P[(int64_t)f2 * KP + j], a2, dot); }
+    }
+    if (part) block_partial<KP>(dot, sh, part + lb * KP);
+}
+
 // x += a p; r -= a q; z = r / diag; partials of r.z and r.r.
 template <int KP>
 __global__ __launch_bounds__(kThreadsM) void update_kernel(int64_t nf, int64_t chunk, const ColumnState* __restrict__ cs,
@@ -274,12 +317,26 @@ struct ManyRun {
                     const sfem_solve_opts& o, double* dq, std::vector<ColumnState>& hcs, double* ms) {
         const int64_t nf = S->nf, n = S->n;
         constexpr int RPB = kThreadsM / KP;
-        // enough blocks for 8 waves per SIMD over the row groups; partials
-        // (nb x KP) are reduced by one block per column
-        const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * 64, (nf + RPB - 1) / RPB));
+        // 16 blocks per CU keep every CU's resident slots filled through
+        // the row chunks; the partials (nb x KP) are reduced by one block
+        // per column, whose cost grows with nb
+        const int nb = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)ctx->n_cu * 16, (nf + RPB - 1) / RPB));
         // contiguous row chunks (a multiple of the rows per block pass); the
         // blocks of one XCD take consecutive chunks (logical_block)
         const int64_t chunk = ((nf + nb - 1) / nb + RPB - 1) / RPB * RPB;
+        // node-block SpMV when K is an assembled (bitwise symmetric) 3x3-block matrix
+        const sfem_csr* K = S->K;
+        const bool blk = K->bs == 3 && !K->transposed;
+        const int64_t nunits = K->nunits;
+        const int64_t uchunk = ((nunits + nb - 1) / nb + RPB - 1) / RPB * RPB;
+        auto spmm = [&](const double* src, double* dst, double* prt) {
+            if (blk)
+                hipLaunchKernelGGL(spmm_blk3_kernel<KP>, dim3(nb), dim3(kThreadsM), 0, ctx->stream, K->bcolptr.get(),
+                                   K->browidx.get(), K->bval.get(), S->free_of.get(), nunits, uchunk, src, dst, prt);
+            else
+                hipLaunchKernelGGL(spmm_kernel<KP>, dim3(nb), dim3(kThreadsM), 0, ctx->stream, S->rowptr.get(),
+                                   S->col.get(), S->val64.get(), nf, chunk, src, dst, prt);
+        };
         const size_t nv = (size_t)std::max<int64_t>(nf, 1) * KP;
         DevBuf<double> X(ctx, nv), R(ctx, nv), Z(ctx, nv), P(ctx, nv), Q(ctx, nv), RHS(ctx, nv);
         DevBuf<double> part(ctx, (size_t)nb * 2 * KP);
@@ -304,8 +361,7 @@ struct ManyRun {
         hcs.assign(KP, ColumnState{});
         for (int64_t check = 0;; ++check) {
             for (int it = 0; it < every; ++it) {
-                hipLaunchKernelGGL(spmm_kernel<KP>, g, b, 0, ctx->stream, S->rowptr.get(), S->col.get(),
-                                   S->val64.get(), nf, chunk, P.get(), Q.get(), part.get());
+                spmm(P.get(), Q.get(), part.get());
                 hipLaunchKernelGGL(fin_alpha_kernel, dim3(KP), dim3(256), 0, ctx->stream, cs.get(), KP, part.get(), nb);
                 hipLaunchKernelGGL(update_kernel<KP>, g, b, 0, ctx->stream, nf, chunk, cs.get(), X.get(), R.get(), Z.get(),
                                    P.get(), Q.get(), dinv, part.get());
@@ -322,8 +378,7 @@ struct ManyRun {
             }
             if (pending) {
                 // true residuals of every column; decisions only for pending ones
-                hipLaunchKernelGGL(spmm_kernel<KP>, g, b, 0, ctx->stream, S->rowptr.get(), S->col.get(),
-                                   S->val64.get(), nf, chunk, X.get(), Q.get(), nullptr);
+                spmm(X.get(), Q.get(), nullptr);
                 hipLaunchKernelGGL(true_resid_kernel<KP>, g, b, 0, ctx->stream, nf, chunk, RHS.get(), Q.get(), Q.get(),
                                    part.get());
                 hipLaunchKernelGGL(fin_true_kernel, dim3(KP), dim3(256), 0, ctx->stream, cs.get(), KP, part.get(), nb,

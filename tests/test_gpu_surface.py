@@ -250,6 +250,16 @@ def test_solve_many_loads_and_errors(ctx):
     for j in range(4):
         Kff, rhs = O.eliminate(Ko, fixed, F[:, j])
         assert relerr(q[:, j], O.recombine(fixed, None, O.dense_solve(Kff, rhs))) <= ERR
+    # the scalar-row multi-rhs SpMV (a K assembled with dof-pair keys, block 1)
+    Ks = smearfem.assemble_system(0, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, 1.0, 0.3, n_elem=m.n_elem, ctx=ctx,
+                                  device=True, force_scalar=True)
+    assert Ks.block == 1
+    Ss = System(Ks, smearfem.BoundaryConditions(ctx, m.n_dof, None, fixed))
+    qs, per_s, _ = Ss.solve_many(f=F, rtol=1e-12)
+    assert all(c["stop_reason"] == 1 for c in per_s)
+    assert max(relerr(qs[:, j], q[:, j]) for j in range(4)) <= 1e-9
+    Ss.close()
+    Ks.close()
     with pytest.raises(smearfem.ArgumentError):
         S.solve_many(f=np.zeros((m.n_dof, 65)))
     with pytest.raises(smearfem.NotConvergedError):
