@@ -147,8 +147,9 @@ int  sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fi
 void sfem_bc_destroy(sfem_bc* bc);
 
 /* Build K_ff = C' K C and rhs = C'(f - K q_d) on the device.  The solver
- * formats are built here: FP64 CSR always, and, when K is node-blocked and
- * every node is either fully fixed or fully free, FP64/FP32 block formats. */
+ * formats are built here: FP64 CSR always, and, when K is node-blocked, the
+ * node-block formats (a node with some dofs fixed keeps them as a zero row /
+ * column with a unit diagonal, so they stay exactly 0 in the solve). */
 int  sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out);
 int  sfem_system_info(const sfem_system* sys, int64_t* n_free, int64_t* nnz_ff, int* block);
 /* Device time of the elimination that built the system (ms). */
@@ -238,6 +239,13 @@ int  sfem_assemble_surface(sfem_ctx* ctx, int ndim, int n_dof, const double* nod
  * from the stored pattern; this block format keeps them as stored zeros.
  * Different sizes: SFEM_ERR_DIMENSION (as Julia's +). */
 int  sfem_csr_add(sfem_ctx* ctx, const sfem_csr* K, double beta, const sfem_csr* b, sfem_csr** out);
+
+/* y = K x (n values in and out, FP64) for an assembled matrix (K, b,
+ * K + beta*b: all bitwise symmetric) or an uploaded CSC.  With b the surface
+ * penalty and x = the traction t repeated per node, y is the consistent load
+ * vector of a uniform traction on those faces, int N_a t dA (no counterpart
+ * in the reference, which passes loads as arrays). */
+int  sfem_csr_apply(sfem_ctx* ctx, const sfem_csr* K, const double* x, double* y);
 
 /* Structured mesh on the device: meshgrid(x0,x1,y0,y1,z0,z1,ne,ndim)
  * (examples/vector3D.jl:10-130) generalised to ne[0] x ne[1] (x ne[2])

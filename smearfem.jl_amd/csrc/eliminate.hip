@@ -31,14 +31,28 @@ __global__ void free_map_kernel(const uint8_t* __restrict__ fixed, const uint32_
 }
 
 // per unit: 1 if all bs dofs free; clears *complete if a unit is partly fixed
+// unit_free[u] = 1 when any dof of node u is free (the node is a block row);
+// *partial = 1 when some node has both fixed and free dofs.
 __global__ void unit_status_kernel(const uint8_t* __restrict__ fixed, int64_t nunits, int bs,
-                                   uint32_t* __restrict__ unit_free, int* complete) {
+                                   uint32_t* __restrict__ unit_free, int* partial) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
     int c = 0;
     for (int i = 0; i < bs; ++i) c += fixed[u * bs + i] ? 1 : 0;
-    unit_free[u] = c == 0 ? 1u : 0u;
-    if (c != 0 && c != bs) atomicAnd(complete, 0);
+    unit_free[u] = c < bs ? 1u : 0u;
+    if (c != 0 && c != bs) atomicOr(partial, 1);
+}
+
+// Padded block vectors <-> compact free dofs: pad_of[row*bs + i] is the free
+// index of dof i of block row `row`, or -1 for a fixed dof of a partially
+// fixed node (kept at exactly 0 by a unit diagonal and zero couplings).
+__global__ void pad_map_kernel(const int32_t* __restrict__ ufree_of, const int32_t* __restrict__ free_of,
+                               int64_t nunits, int bs, int32_t* __restrict__ pad_of) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const int32_t row = ufree_of[u];
+    if (row < 0) return;
+    for (int i = 0; i < bs; ++i) pad_of[(int64_t)row * bs + i] = free_of[u * bs + i];
 }
 
 __global__ void unit_fixed_kernel(const uint32_t* __restrict__ ufree, uint8_t* __restrict__ ufixed, int64_t m) {
@@ -186,7 +200,8 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
                                  int64_t nunits, const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
                                  int32_t* __restrict__ scol, double* __restrict__ sval64,
                                  float* __restrict__ sval32, int16_t* __restrict__ sval16, int64_t nfb,
-                                 double* __restrict__ binv64, float* __restrict__ binv32) {
+                                 double* __restrict__ binv64, float* __restrict__ binv32,
+                                 const uint8_t* __restrict__ fixed, double* __restrict__ pdinv64) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -197,26 +212,32 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
     const int64_t base = sl_ptr[s];
     int k = 0;
     for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
-        const int32_t c = ufree_of[browidx[u]];
+        const int64_t ru = browidx[u];
+        const int32_t c = ufree_of[ru];
         if (c < 0) continue;
         const int64_t slot = base + (int64_t)k * 64;
         scol[slot + lane] = c;
+        // M[j][i] = K[(row,j),(c,i)] = K[(c,i),(row,j)]; a fixed dof of a
+        // partially fixed node gets a zero row / column and a unit diagonal
+        double m[B2];
 #pragma unroll
         for (int j = 0; j < BS; ++j)
 #pragma unroll
             for (int i = 0; i < BS; ++i) {
-                // plane j*BS+i holds M[j][i] = K[(row,j),(c,i)] = K[(c,i),(row,j)]
-                const double v = bval[u * B2 + i * BS + j];
-                sval64[slot * B2 + (j * BS + i) * 64 + lane] = v;
-                sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
-                sval16[slot * B2 + (j * BS + i) * 64 + lane] = split_correction(v);
+                const bool masked = fixed[cu * BS + j] || fixed[ru * BS + i];
+                m[j * BS + i] = masked ? ((ru == cu && i == j) ? 1.0 : 0.0) : bval[u * B2 + i * BS + j];
             }
+#pragma unroll
+        for (int v = 0; v < B2; ++v) {
+            sval64[slot * B2 + v * 64 + lane] = m[v];
+            sval32[slot * B2 + v * 64 + lane] = (float)m[v];
+            sval16[slot * B2 + v * 64 + lane] = split_correction(m[v]);
+        }
         if (c == row) {  // diagonal block: its inverse for block-Jacobi preconditioning
-            double m[B2], inv[B2];
+            double inv[B2];
+            if (pdinv64)
 #pragma unroll
-            for (int j = 0; j < BS; ++j)
-#pragma unroll
-                for (int i = 0; i < BS; ++i) m[j * BS + i] = bval[u * B2 + i * BS + j];
+                for (int j = 0; j < BS; ++j) pdinv64[(int64_t)row * BS + j] = m[j * BS + j] != 0.0 ? 1.0 / m[j * BS + j] : 1.0;
             invert_block<BS>(m, inv);
 #pragma unroll
             for (int v = 0; v < B2; ++v) {
@@ -292,6 +313,40 @@ __global__ void reactions_kernel(const int64_t* __restrict__ bcolptr, const int3
 
 }  // namespace
 
+namespace {
+__global__ void pad_gather_kernel(const int32_t* __restrict__ pad_of, int64_t n, const double* __restrict__ src,
+                                  double* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t f = pad_of[k];
+    dst[k] = f >= 0 ? src[f] : 0.0;
+}
+
+__global__ void pad_scatter_kernel(const int32_t* __restrict__ pad_of, int64_t n, const double* __restrict__ src,
+                                   double* __restrict__ dst) {
+    const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int32_t f = pad_of[k];
+    if (f >= 0) dst[f] = src[k];
+}
+}  // namespace
+
+void pad_gather(sfem_ctx* ctx, const sfem_system* S, const double* compact, double* padded) {
+    const int64_t n = S->nfb * S->bs;
+    if (n == 0) return;
+    hipLaunchKernelGGL(pad_gather_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, S->pad_of.get(), n,
+                       compact, padded);
+    SFEM_LAUNCH_CHECK();
+}
+
+void pad_scatter(sfem_ctx* ctx, const sfem_system* S, const double* padded, double* compact) {
+    const int64_t n = S->nfb * S->bs;
+    if (n == 0) return;
+    hipLaunchKernelGGL(pad_scatter_kernel, dim3(grid_for(n, 256)), dim3(256), 0, ctx->stream, S->pad_of.get(), n,
+                       padded, compact);
+    SFEM_LAUNCH_CHECK();
+}
+
 void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S) {
     if (bc->n != K->n || K->m != K->n) fail(SFEM_ERR_DIMENSION, "eliminate: BC length differs from size(K)");
     hipEvent_t e0, e1;
@@ -358,19 +413,20 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
         SFEM_LAUNCH_CHECK();
     }
 
-    // node-block sliced format
+    // node-block sliced format: every node with a free dof is a block row
     S->has_block = false;
+    S->partial = false;
     if (bs > 1 && nf > 0) {
         DevBuf<uint32_t> ufree(ctx, (size_t)nunits), upos(ctx, (size_t)nunits), utot(ctx, 1);
-        DevBuf<int> complete(ctx, 1);
-        const int one = 1;
-        h2d(ctx, complete.get(), &one, sizeof(int));
+        DevBuf<int> dpartial(ctx, 1);
+        memset_d(ctx, dpartial.get(), 0, sizeof(int));
         hipLaunchKernelGGL(unit_status_kernel, gu, dim3(256), 0, ctx->stream, bc->fixed.get(), nunits, bs,
-                           ufree.get(), complete.get());
+                           ufree.get(), dpartial.get());
         SFEM_LAUNCH_CHECK();
-        int comp = 0;
-        d2h(ctx, &comp, complete.get(), sizeof(int));
-        if (comp) {
+        int part = 0;
+        d2h(ctx, &part, dpartial.get(), sizeof(int));
+        S->partial = part != 0;
+        {
             exclusive_scan_u32_dev(ctx, ufree.get(), upos.get(), nunits, utot.get());
             uint32_t nfb = 0;
             d2h(ctx, &nfb, utot.get(), sizeof(nfb));
@@ -389,7 +445,25 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             const int64_t nslices = (nfb + 63) / 64;
             S->nfb = nfb;
             S->nslices = nslices;
-            S->nnzb = nnz / (bs * bs);
+            if (S->partial) {
+                DevBuf<uint32_t> btot(ctx, 1);
+                DevBuf<uint32_t> bscan(ctx, nfb);
+                exclusive_scan_u32_dev(ctx, (const uint32_t*)blen.get(), bscan.get(), nfb, btot.get());
+                uint32_t nb = 0;
+                d2h(ctx, &nb, btot.get(), sizeof(nb));
+                S->nnzb = nb;
+                S->pad_of.reset(ctx, (size_t)nfb * bs);
+                S->prhs.reset(ctx, (size_t)nfb * bs);
+                S->px.reset(ctx, (size_t)nfb * bs);
+                S->pdinv64.reset(ctx, (size_t)nfb * bs);
+                S->pdinv32.reset(ctx, (size_t)nfb * bs);
+                hipLaunchKernelGGL(pad_map_kernel, gu, dim3(256), 0, ctx->stream, ufree_of.get(), S->free_of.get(),
+                                   nunits, bs, S->pad_of.get());
+                SFEM_LAUNCH_CHECK();
+            } else {
+                S->nnzb = nnz / (bs * bs);
+            }
+            double* pdinv = S->partial ? S->pdinv64.get() : nullptr;
             S->sl_ptr.reset(ctx, (size_t)nslices + 1);
             S->sl_w.reset(ctx, (size_t)nslices);
             DevBuf<int64_t> cnt(ctx, (size_t)nslices + 1);
@@ -410,7 +484,7 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
                 hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
                                    S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
-                                   (int64_t)nfb, S->binv64.get(), S->binv32.get());
+                                   (int64_t)nfb, S->binv64.get(), S->binv32.get(), bc->fixed.get(), pdinv);
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
                                    S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
@@ -419,13 +493,19 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
                 hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
                                    K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
                                    S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
-                                   (int64_t)nfb, S->binv64.get(), S->binv32.get());
+                                   (int64_t)nfb, S->binv64.get(), S->binv32.get(), bc->fixed.get(), pdinv);
                 SFEM_LAUNCH_CHECK();
                 hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
                                    S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
                                    S->sval16.get());
             }
             SFEM_LAUNCH_CHECK();
+            if (S->partial) {
+                hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for((int64_t)nfb * bs, 256)), dim3(256), 0, ctx->stream,
+                                   S->pdinv64.get(), S->pdinv32.get(), (int64_t)nfb * bs);
+                SFEM_LAUNCH_CHECK();
+                pad_gather(ctx, S, S->rhs.get(), S->prhs.get());
+            }
             S->has_block = true;
         }
     }
@@ -456,6 +536,7 @@ void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {
     hipLaunchKernelGGL(rhs_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(), K->bval.get(),
                        S->free_of.get(), S->qd.get(), S->f.get(), n, S->bs, S->rhs.get());
     SFEM_LAUNCH_CHECK();
+    if (S->partial && S->has_block) pad_gather(ctx, S, S->rhs.get(), S->prhs.get());
     sync(ctx);
 }
 

@@ -551,6 +551,11 @@ struct Ops {
     bool nt;        // non-temporal matrix loads in the sliced SpMV
     bool block_pc;  // node-block Jacobi preconditioner
     Launch node;    // one thread per node (block update)
+    // vectors of the solve: the free dofs, or the padded block rows when
+    // the node-block format carries partially fixed nodes
+    double* x64;
+    const double* rhs;
+    int64_t n;
 };
 
 template <typename TV>
@@ -607,8 +612,8 @@ void launch_resid(const Ops& o, const T* xs, const T* dinv, const T* binv, const
     if (o.fmt == SFEM_FMT_BSR) {
         auto go = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(o.sell.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
-                               S->scol.get(), S->sval64.get(), S->nslices, S->nfb, o.sell.per_block, S->x.get(), xs,
-                               S->rhs.get(), dinv, binv, p, r, z, st, partials);
+                               S->scol.get(), S->sval64.get(), S->nslices, S->nfb, o.sell.per_block, o.x64, xs,
+                               o.rhs, dinv, binv, p, r, z, st, partials);
         };
         if (S->bs == 3) {
             if (o.block_pc) go(sell_resid_kernel<3, T, FOLD, true>);
@@ -626,7 +631,8 @@ void launch_resid(const Ops& o, const T* xs, const T* dinv, const T* binv, const
 }
 
 double spmv_bytes_for(const sfem_system* S, int fmt, double sv, double sx, int nvec) {
-    const double vec = (double)nvec * (double)S->nf * sx;
+    const int64_t nv = (fmt == SFEM_FMT_BSR && S->partial) ? S->nfb * S->bs : S->nf;
+    const double vec = (double)nvec * (double)nv * sx;
     if (fmt == SFEM_FMT_BSR)
         return (double)S->nnzb * (S->bs * S->bs * sv + 4.0) + (double)(S->nslices + 1) * 8.0 +
                (double)S->nslices * 4.0 + vec;
@@ -653,14 +659,17 @@ int resolve_format(const sfem_system* S, int fmt) {
 template <typename TV, typename T>
 void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* stats, int fmt) {
     constexpr bool FOLD = std::is_same<T, float>::value;
-    const int64_t nf = S->nf;
+    const bool padded = fmt == SFEM_FMT_BSR && S->partial;
+    const int64_t nf = padded ? S->nfb * S->bs : S->nf;  // length of the solve's vectors
     const bool block_pc = o->preconditioner == SFEM_PC_BLOCK_JACOBI;
     if (block_pc && fmt != SFEM_FMT_BSR)
         fail(SFEM_ERR_UNSUPPORTED, "block-Jacobi preconditioning needs the node-block format");
     if (o->preconditioner != SFEM_PC_JACOBI && !block_pc) fail(SFEM_ERR_ARGUMENT, "solve: unknown preconditioner");
-    Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt, block_pc, {}};
+    double* x64 = padded ? S->px.get() : S->x.get();
+    Ops ops{ctx, S, fmt, (unsigned)std::max(8, ctx->n_cu * 8), {}, {}, {}, ctx->tune_nt, block_pc, {},
+            x64, padded ? S->prhs.get() : S->rhs.get(), nf};
     if (block_pc) ops.node = plan(S->nfb, kThreads, std::max(8, ctx->n_cu * 4));
-    ops.csr = plan(nf, kThreads / kCsrLanes, ops.cap);
+    ops.csr = plan(S->nf, kThreads / kCsrLanes, ops.cap);
     if (S->has_block) ops.sell = plan_sell(S->nslices, ops.cap, ctx->tune_wave_slices);
     ops.vec = plan(nf, kThreads, std::max(8, ctx->n_cu * 4));
     const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid, ops.node.grid});
@@ -669,16 +678,18 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     DevBuf<T> xsbuf;
     DevBuf<double> partials(ctx, 4 * (size_t)maxgrid);
     DevBuf<CgState> dst(ctx, 1);
-    const T* dinv = FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get();
+    const T* dinv = padded ? (FOLD ? (const T*)S->pdinv32.get() : (const T*)S->pdinv64.get())
+                           : (FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get());
     const T* binv = FOLD ? (const T*)S->binv32.get() : (const T*)S->binv64.get();
-    T* x = (T*)S->x.get();  // FP64 paths update the FP64 solution directly
+    T* x = (T*)x64;  // FP64 paths update the FP64 solution directly
     if (FOLD) {
         xsbuf.reset(ctx, nf);
         memset_d(ctx, xsbuf.get(), 0, sizeof(T) * nf);
         x = xsbuf.get();
     }
     const T* xs = FOLD ? xsbuf.get() : nullptr;
-    if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * nf);
+    if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
+    if (padded) pad_gather(ctx, S, S->x.get(), x64);  // start from the current solution
     memset_d(ctx, pv.get(), 0, sizeof(T) * nf);
     CgState init{};
     init.rr = HUGE_VAL;
@@ -700,7 +711,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
         for (auto& e : ev) SFEM_HIP(hipEventCreate(&e));
     }
     SFEM_HIP(hipEventRecord(t0, ctx->stream));
-    hipLaunchKernelGGL(norm2_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->rhs.get(), nf,
+    hipLaunchKernelGGL(norm2_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, ops.rhs, nf,
                        ops.vec.per_block, st, partials.get(), o->rtol, o->reliable_delta);
     SFEM_LAUNCH_CHECK();
     T* p = pv.get();
@@ -721,15 +732,15 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
             if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k + 1], ctx->stream));
             if (!block_pc)
                 hipLaunchKernelGGL((cg_update_kernel<T, FOLD>), dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream,
-                                   x, S->x.get(), r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
+                                   x, x64, r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
                                    partials.get());
             else if (S->bs == 3)
                 hipLaunchKernelGGL((cg_update_block_kernel<3, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
-                                   ctx->stream, x, S->x.get(), r.get(), z.get(), p, q.get(), binv, S->nfb,
+                                   ctx->stream, x, x64, r.get(), z.get(), p, q.get(), binv, S->nfb,
                                    ops.node.per_block, st, partials.get());
             else
                 hipLaunchKernelGGL((cg_update_block_kernel<2, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
-                                   ctx->stream, x, S->x.get(), r.get(), z.get(), p, q.get(), binv, S->nfb,
+                                   ctx->stream, x, x64, r.get(), z.get(), p, q.get(), binv, S->nfb,
                                    ops.node.per_block, st, partials.get());
             SFEM_LAUNCH_CHECK();
             launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
@@ -756,10 +767,11 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
         d2h(ctx, &host, st, sizeof(host));
     }
     if (FOLD) {  // x = x64 + xs
-        hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, S->x.get(),
+        hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x64,
                            (float*)xsbuf.get(), nf);
         SFEM_LAUNCH_CHECK();
     }
+    if (padded) pad_scatter(ctx, S, x64, S->x.get());
     SFEM_HIP(hipEventRecord(t1, ctx->stream));
     sync(ctx);
     float total = 0;
@@ -823,13 +835,15 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
 namespace {
 template <typename TV, typename T>
 double spmv_bench_impl(sfem_ctx* ctx, sfem_system* S, int fmt, int reps, bool nt) {
-    const int64_t nf = S->nf;
+    const bool padded = fmt == SFEM_FMT_BSR && S->partial;
+    const int64_t nf = padded ? S->nfb * S->bs : S->nf;
     const unsigned cap = (unsigned)std::max(8, ctx->n_cu * 8);
     const Launch lc = plan(nf, kThreads / kCsrLanes, cap);
     const Launch ls = S->has_block ? plan_sell(S->nslices, cap, ctx->tune_wave_slices) : Launch{8, 1};
     DevBuf<double> x64(ctx, nf);
     DevBuf<T> x(ctx, nf), y(ctx, nf);
-    SFEM_HIP(hipMemcpyAsync(x64.get(), S->rhs.get(), sizeof(double) * nf, hipMemcpyDeviceToDevice, ctx->stream));
+    SFEM_HIP(hipMemcpyAsync(x64.get(), padded ? S->prhs.get() : S->rhs.get(), sizeof(double) * nf,
+                            hipMemcpyDeviceToDevice, ctx->stream));
     if (std::is_same<T, float>::value)
         hipLaunchKernelGGL(f64_to_f32_kernel, dim3(1024), dim3(256), 0, ctx->stream, x64.get(), (float*)x.get(), nf);
     else

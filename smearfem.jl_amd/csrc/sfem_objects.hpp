@@ -141,6 +141,13 @@ struct sfem_system {
     sfem::DevBuf<int16_t> sval16;  // correction of sval32 toward sval64 (split precision)
     sfem::DevBuf<double> binv64;   // inverse node diagonal blocks, planar: [(j*bs+i)*nfb + node]
     sfem::DevBuf<float> binv32;
+    // Partially fixed nodes: the block rows carry their fixed dofs with a zero
+    // row / column and a unit diagonal; the block solver then works on padded
+    // vectors of nfb*bs values, mapped to the free dofs by pad_of (-1 = fixed).
+    bool partial = false;
+    sfem::DevBuf<int32_t> pad_of;
+    sfem::DevBuf<double> prhs, px, pdinv64;
+    sfem::DevBuf<float> pdinv32;
     double t_eliminate_ms = 0;
 };
 
@@ -149,6 +156,9 @@ namespace sfem {
 void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S);
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions);
 void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc);
+// padded block vector <- compact free-dof vector (0 on padded fixed dofs), and back
+void pad_gather(sfem_ctx* ctx, const sfem_system* S, const double* compact, double* padded);
+void pad_scatter(sfem_ctx* ctx, const sfem_system* S, const double* padded, double* compact);
 // k right-hand sides C'(f_j - K q_d_j) into rhs (n_free x kp, row-major; columns >= k zero).
 void launch_rhs_many(sfem_ctx* ctx, const sfem_system* S, int k, int kp, const double* qd, const double* f,
                      double* rhs);

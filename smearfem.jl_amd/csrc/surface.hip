@@ -232,6 +232,25 @@ __global__ void bc_planes_kernel(const double* __restrict__ coords, int ndim, co
     }
 }
 
+// y = K x walking K's block columns as rows: exact for the assembled K and b
+// (bitwise symmetric) and for an uploaded CSC (which stores K').  One thread
+// per unit; the sum runs over the column's blocks in ascending row order.
+__global__ void csr_apply_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                 const double* __restrict__ bval, int64_t nunits, int bs, const double* __restrict__ x,
+                                 double* __restrict__ y) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nunits) return;
+    const int bs2 = bs * bs;
+    for (int j = 0; j < bs; ++j) {
+        double acc = 0.0;
+        for (int64_t e = bcolptr[u]; e < bcolptr[u + 1]; ++e) {
+            const int64_t r = browidx[e];
+            for (int i = 0; i < bs; ++i) acc += bval[e * bs2 + i * bs + j] * x[r * bs + i];
+        }
+        y[u * bs + j] = acc;
+    }
+}
+
 template <typename T>
 T* new_or_null() {
     return new (std::nothrow) T();
@@ -559,6 +578,22 @@ int sfem_bc_planes(sfem_ctx* ctx, const sfem_mesh* m, int64_t n_total, int nrule
     ctx_retain(ctx);
     *out = bc;
     return SFEM_OK;
+}
+
+int sfem_csr_apply(sfem_ctx* ctx, const sfem_csr* K, const double* x, double* y) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!K || !x || !y) { ctx->last_error = "csr_apply: NULL argument"; return SFEM_ERR_ARGUMENT; }
+    return guard(ctx, [&] {
+        if (K->m != K->n || K->m != K->nunits * K->bs) fail(SFEM_ERR_DIMENSION, "csr_apply: square matrices only");
+        const int64_t n = K->n;
+        if (n == 0) return;
+        DevBuf<double> dx(ctx, (size_t)n), dy(ctx, (size_t)n);
+        h2d(ctx, dx.get(), x, sizeof(double) * n);
+        hipLaunchKernelGGL(csr_apply_kernel, dim3(grid_for(K->nunits, 256)), dim3(256), 0, ctx->stream,
+                           K->bcolptr.get(), K->browidx.get(), K->bval.get(), K->nunits, K->bs, dx.get(), dy.get());
+        SFEM_LAUNCH_CHECK();
+        d2h(ctx, y, dy.get(), sizeof(double) * n);
+    });
 }
 
 int sfem_system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {

@@ -13,7 +13,7 @@ from .solve import BoundaryConditions, System, solve_system, stream_copy_bench
 from .batch import Batch, Problem, autotune, make_problem, uneven_hex8_problems
 from .serve import Server, random_request, run_poisson
 from .surface import (DeviceMesh, PlaneRule, apply_boundary_conditions, bc_planes, csr_add, displacement_sweep,
-                      meshgrid, setboundaryCond)
+                      face_traction_loads, meshgrid, setboundaryCond)
 from .vtk import write_scene, write_vtk
 
 __all__ = [
@@ -23,7 +23,7 @@ __all__ = [
     "default_context", "gaussian_quadrature", "StructuredMesh", "clamp_x0", "face_load",
     "random_face_load", "structured_mesh", "BoundaryConditions", "System", "solve_system",
     "stream_copy_bench", "Batch", "Problem", "autotune", "make_problem", "uneven_hex8_problems", "Server",
-    "DeviceMesh", "PlaneRule", "apply_boundary_conditions", "bc_planes", "csr_add", "displacement_sweep",
+    "DeviceMesh", "PlaneRule", "apply_boundary_conditions", "bc_planes", "csr_add", "displacement_sweep", "face_traction_loads",
     "meshgrid", "setboundaryCond", "write_scene", "write_vtk",
     "random_request", "run_poisson",
 ]

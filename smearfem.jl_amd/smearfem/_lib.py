@@ -134,6 +134,7 @@ _PROTOS = [
      [c_vp, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, c_i64p, C.c_int64, C.c_int, c_i64p, C.c_int64,
       C.POINTER(c_vp)]),
     ("sfem_csr_add", C.c_int, [c_vp, c_vp, C.c_double, c_vp, C.POINTER(c_vp)]),
+    ("sfem_csr_apply", C.c_int, [c_vp, c_vp, c_dblp, c_dblp]),
     ("sfem_meshgrid", C.c_int,
      [c_vp, C.c_int, c_i64p, c_dblp, C.c_int, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp), C.POINTER(c_vp)]),
     ("sfem_mesh_info", C.c_int, [c_vp, c_intp, c_intp, c_intp, c_i64p, c_i64p]),

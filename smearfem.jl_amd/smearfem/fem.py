@@ -137,6 +137,16 @@ class StiffnessMatrix:
                                                       _ptr(rowval, C.c_int64), _ptr(nzval, C.c_double)))
         return colptr, rowval, nzval
 
+    def apply(self, x) -> np.ndarray:
+        """y = K x on the device (``sfem_csr_apply``)."""
+        xv = np.ascontiguousarray(x, dtype=np.float64)
+        if xv.size != self.shape[1]:
+            raise ValueError(f"x has {xv.size} entries, K has {self.shape[1]} columns")
+        y = np.zeros(self.shape[0])
+        self.ctx.check(self.ctx._lib.sfem_csr_apply(self.ctx.handle, self.handle, _ptr(xv, C.c_double),
+                                                    _ptr(y, C.c_double)))
+        return y
+
     def to_scipy(self):
         import scipy.sparse as sp
 

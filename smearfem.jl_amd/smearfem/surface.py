@@ -117,6 +117,21 @@ def csr_add(K: StiffnessMatrix, beta: float, b: StiffnessMatrix) -> StiffnessMat
     return StiffnessMatrix(K.ctx, h)
 
 
+def face_traction_loads(faces: DeviceMesh, traction) -> np.ndarray:
+    """Consistent nodal loads of a uniform traction t on a face mesh:
+    f[3(m-1)+i] = t_i * int N_m dA over the faces, formed on the device as
+    b (1 (x) t) with b the faces' penalty matrix (sum_b N_b = 1).  For one
+    loaded face, build the face mesh from that face list only."""
+    t = np.asarray(traction, dtype=np.float64).ravel()
+    if t.size != faces.n_dof:
+        raise ValueError(f"traction needs {faces.n_dof} components")
+    b = faces.assemble()
+    try:
+        return b.apply(np.tile(t, b.shape[1] // t.size))
+    finally:
+        b.close()
+
+
 @dataclass
 class PlaneRule:
     """Fix dof ``dof`` (1-based) of the nodes with coordinate ``axis``

@@ -131,7 +131,7 @@ def test_patch_test_prescribed_linear_field(ctx):
     K.close()
 
 
-def test_partial_dof_clamp_uses_scalar_path(ctx):
+def test_partial_dof_clamp_every_path(ctx):
     # setboundaryCond-style constraints: only the z dof of the bottom and top
     # nodes prescribed (examples/vector3D.jl:159-168) plus one pinned corner
     m = structured_mesh((4, 4, 4))
@@ -147,15 +147,22 @@ def test_partial_dof_clamp_uses_scalar_path(ctx):
     bc = BoundaryConditions(ctx, K.shape[0], None, fixed, q_d)
     S = System(K, bc)
     bc.close()
-    assert S.block == 1  # a node with some dofs fixed: the block format does not apply
-    with pytest.raises(smearfem.UnsupportedError):
-        S.solve("split", rtol=1e-12)  # the split operator exists only in the block format
-    with pytest.raises(smearfem.UnsupportedError):
-        S.solve("fp64", rtol=1e-12, preconditioner="block_jacobi")
-    st = S.solve("fp64", rtol=1e-12)
-    q = S.solution()
+    # nodes with some dofs fixed: padded node blocks (unit diagonal on the
+    # fixed dofs) carry every block path; the scalar rows remain available
+    assert S.block == 3
     q_ref, _ = oracle_direct(m, fixed, None, q_d, young=40.0, nu=0.4)
-    assert relerr(q, q_ref) <= ERR
+    for prec, fmt, pc, rtol in (("fp64", "csr", "jacobi", 1e-12), ("fp64", "bsr", "jacobi", 1e-12),
+                                ("fp64", "bsr", "block_jacobi", 1e-12), ("split", "bsr", "block_jacobi", 1e-12),
+                                ("mixed", "bsr", "block_jacobi", 1e-11), ("mixed", "csr", "jacobi", 1e-11)):
+        st = S.solve(prec, fmt, rtol=rtol, preconditioner=pc)
+        assert st["format"] == fmt and st["stop_reason"] == 1
+        q = S.solution()
+        assert relerr(q, q_ref) <= ERR, (prec, fmt, pc)
+        assert np.all(q[fixed == 1] == q_d[fixed == 1])
+    # a warm start from a solution converged to the same tolerance needs no iteration
+    S.solve("fp64", "bsr", rtol=1e-12)
+    st = S.solve("fp64", "bsr", rtol=1e-12, x0_zero=False)
+    assert st["iterations"] <= 1
     S.close()
     K.close()
 

@@ -266,3 +266,30 @@ def test_solve_many_loads_and_errors(ctx):
         S.solve_many(f=F, rtol=1e-12, max_iter=5)
     S.close()
     K.close()
+
+
+def test_apply_and_face_traction_loads(ctx):
+    """y = K x on the device equals scipy's product; uniform traction loads on
+    the bottom and top faces of a box are the closed-form nodal areas times t."""
+    print("[case] apply and traction loads", flush=True)
+    from smearfem.surface import face_traction_loads
+
+    nx, ny, nz = 4, 3, 2
+    L = (2.0, 1.5, 1.0)
+    mesh, faces = meshgrid(0, L[0], 0, L[1], 0, L[2], (nx, ny, nz), 3, ctx=ctx)
+    K = mesh.assemble()
+    x = np.random.default_rng(3).standard_normal(K.shape[0])
+    y = K.apply(x)
+    assert relerr(y, K.to_scipy() @ x) <= 1e-14
+    t = np.array([0.5, -2.0, 3.0])
+    f = face_traction_loads(faces, t).reshape(-1, 3)
+    nodes, _, _ = mesh.export()
+    hx, hy = L[0] / nx, L[1] / ny
+    on_face = (nodes[2] == 0.0) | (nodes[2] == L[2])
+    wx = np.where((nodes[0] == 0.0) | (nodes[0] == L[0]), 0.5, 1.0)
+    wy = np.where((nodes[1] == 0.0) | (nodes[1] == L[1]), 0.5, 1.0)
+    area = np.where(on_face, wx * wy * hx * hy, 0.0)
+    assert np.allclose(f, area[:, None] * t[None, :], rtol=1e-13, atol=1e-15)
+    assert np.allclose(f.sum(axis=0), 2 * L[0] * L[1] * t, rtol=1e-13)
+    for h in (K, mesh, faces):
+        h.close()
