@@ -85,7 +85,9 @@ __global__ __launch_bounds__(kThreadsM) void spmm_kernel(const int32_t* __restri
 // sum over blocks (r, v) of v[i*3+jj] x_i with x_i = P[free_of[3r+i]] (0 for a
 // fixed dof).  Each gathered node row of P serves three output rows, a third
 // of the gather traffic of the scalar rows; partially fixed nodes need no
-// separate format.
+// separate format.  Four blocks' row ids, values and free-dof ids are loaded
+// before their gathers (scheduling barriers keep them batched); the
+// arithmetic order is that of one block at a time.
 template <int KP>
 __global__ __launch_bounds__(kThreadsM) void spmm_blk3_kernel(const int64_t* __restrict__ bcolptr,
                                                               const int32_t* __restrict__ browidx,
@@ -104,8 +106,36 @@ __global__ __launch_bounds__(kThreadsM) void spmm_blk3_kernel(const int64_t* __r
         if (f0 < 0 && f1 < 0 && f2 < 0) continue;
         double a0 = 0.0, a1 = 0.0, a2 = 0.0;
         const int64_t e1 = bcolptr[u + 1];
-#pragma unroll 2
-        for (int64_t e = bcolptr[u]; e < e1; ++e) {
+        int64_t e = bcolptr[u];
+        for (; e + 4 <= e1; e += 4) {
+            int64_t r[4];
+            double v[4][9];
+            int32_t g[4][3];
+            double x[4][3];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) r[t] = browidx[e + t];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int m = 0; m < 9; ++m) v[t][m] = bval[(e + t) * 9 + m];
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) g[t][i] = free_of[3 * r[t] + i];
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+#pragma unroll
+                for (int i = 0; i < 3; ++i) x[t][i] = g[t][i] >= 0 ? P[(int64_t)g[t][i] * KP + j] : 0.0;
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                a0 = fma(v[t][0], x[t][0], fma(v[t][3], x[t][1], fma(v[t][6], x[t][2], a0)));
+                a1 = fma(v[t][1], x[t][0], fma(v[t][4], x[t][1], fma(v[t][7], x[t][2], a1)));
+                a2 = fma(v[t][2], x[t][0], fma(v[t][5], x[t][1], fma(v[t][8], x[t][2], a2)));
+            }
+        }
+        for (; e < e1; ++e) {
             const int64_t r = browidx[e];
             const double* v = bval + e * 9;
             const int32_t g0 = free_of[3 * r], g1 = free_of[3 * r + 1], g2 = free_of[3 * r + 2];
