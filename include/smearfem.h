@@ -285,7 +285,8 @@ int  sfem_system_set_bc(sfem_ctx* ctx, sfem_system* sys, const sfem_bc* bc);
  * there handles one mesh, src/fem.jl:135-256); every problem is the same
  * maths as a single-mesh call.  Problems are Q4 (ndim 2) or Hex8 (ndim 3)
  * vector elasticity with node-interleaved dofs (ID[m,l] = ndim*(m-1)+l, as
- * examples/vector3D.jl:33 builds it); every node fully fixed or fully free.
+ * examples/vector3D.jl:33 builds it).  A node may have some dofs fixed and
+ * others free (a roller plane): its fixed dofs are held at q_d exactly.
  *   node_list    ndim x N_total, node-major, problems concatenated
  *   node_offsets nprob+1 (node_offsets[p] = first node of problem p, 0-based)
  *   ien          E_total x nen column-major, 1-based node ids local to the

@@ -46,14 +46,16 @@ namespace sfem {
 namespace {
 
 // ------------------------------------------------ padded elimination ---
+// A node with any free dof is a block row; its fixed dofs (a partially fixed
+// node) get a zero row / column and a unit diagonal, zero rhs and zero start,
+// so they stay exactly 0.
 __global__ void unit_free_kernel(const uint8_t* __restrict__ fixed, int64_t nunits, int bs,
-                                 uint32_t* __restrict__ ufree, int* __restrict__ complete) {
+                                 uint32_t* __restrict__ ufree) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= nunits) return;
     int c = 0;
     for (int i = 0; i < bs; ++i) c += fixed[u * bs + i] ? 1 : 0;
-    ufree[u] = c == 0 ? 1u : 0u;
-    if (c != 0 && c != bs) atomicAnd(complete, 0);
+    ufree[u] = c < bs ? 1u : 0u;
 }
 
 // out[i] = upos[idx[i]] (idx[i] == nunits gives the total)
@@ -109,7 +111,7 @@ __global__ void sell_fill_pad_kernel(const int64_t* __restrict__ bcolptr, const 
                                      const int32_t* __restrict__ sl_w, int32_t* __restrict__ scol,
                                      double* __restrict__ sval64, float* __restrict__ sval32,
                                      int16_t* __restrict__ sval16, int64_t nrows, double* __restrict__ binv64,
-                                     float* __restrict__ binv32) {
+                                     float* __restrict__ binv32, const uint8_t* __restrict__ fixed) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -119,26 +121,29 @@ __global__ void sell_fill_pad_kernel(const int64_t* __restrict__ bcolptr, const 
     const int lane = row % 64;
     int k = 0;
     for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
-        const int32_t c = pad_of[browidx[u]];
+        const int64_t ru = browidx[u];
+        const int32_t c = pad_of[ru];
         if (c < 0) continue;
         const int64_t slot = sl_ptr[s] + (int64_t)k * 64;
         scol[slot + lane] = c;
+        // plane j*BS+i holds M[j][i] = K[(row,j),(c,i)] = K[(c,i),(row,j)];
+        // fixed dofs of a partially fixed node: zero row / column, unit diagonal
+        double m[B2];
 #pragma unroll
         for (int j = 0; j < BS; ++j)
 #pragma unroll
             for (int i = 0; i < BS; ++i) {
-                // plane j*BS+i holds M[j][i] = K[(row,j),(c,i)] = K[(c,i),(row,j)]
-                const double v = bval[u * B2 + i * BS + j];
-                sval64[slot * B2 + (j * BS + i) * 64 + lane] = v;
-                sval32[slot * B2 + (j * BS + i) * 64 + lane] = (float)v;
-                sval16[slot * B2 + (j * BS + i) * 64 + lane] = split_correction(v);
+                const bool masked = fixed[cu * BS + j] || fixed[ru * BS + i];
+                m[j * BS + i] = masked ? ((ru == cu && i == j) ? 1.0 : 0.0) : bval[u * B2 + i * BS + j];
             }
+#pragma unroll
+        for (int v = 0; v < B2; ++v) {
+            sval64[slot * B2 + v * 64 + lane] = m[v];
+            sval32[slot * B2 + v * 64 + lane] = (float)m[v];
+            sval16[slot * B2 + v * 64 + lane] = split_correction(m[v]);
+        }
         if (c == row) {  // diagonal block: its inverse for block-Jacobi preconditioning
-            double m[B2], inv[B2];
-#pragma unroll
-            for (int j = 0; j < BS; ++j)
-#pragma unroll
-                for (int i = 0; i < BS; ++i) m[j * BS + i] = bval[u * B2 + i * BS + j];
+            double inv[B2];
             invert_block<BS>(m, inv);
 #pragma unroll
             for (int v = 0; v < B2; ++v) {
@@ -215,6 +220,13 @@ __global__ void pad_rhs_kernel(const int64_t* __restrict__ bcolptr, const int32_
     const int j = (int)(s - cu * bs);
     const int32_t prow = pad_of[cu];
     if (prow < 0) return;
+    if (fixed[s]) {  // fixed dof of a partially fixed node: stays 0
+        const int64_t pd = (int64_t)prow * bs + j;
+        rhs[pd] = 0.0;
+        dinv64[pd] = 1.0;
+        dinv32[pd] = 1.0f;
+        return;
+    }
     double lift = 0.0, diag = 0.0;
     for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
         const int64_t r = browidx[u];
@@ -231,12 +243,13 @@ __global__ void pad_rhs_kernel(const int64_t* __restrict__ bcolptr, const int32_
     dinv32[pd] = (float)dinv64[pd];
 }
 
-__global__ void batch_recombine_kernel(const int32_t* __restrict__ pad_of, const double* __restrict__ x,
-                                       const double* __restrict__ qd, int64_t n, int bs, double* __restrict__ q) {
+__global__ void batch_recombine_kernel(const int32_t* __restrict__ pad_of, const uint8_t* __restrict__ fixed,
+                                       const double* __restrict__ x, const double* __restrict__ qd, int64_t n, int bs,
+                                       double* __restrict__ q) {
     const int64_t d = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (d >= n) return;
     const int32_t pr = pad_of[d / bs];
-    q[d] = pr >= 0 ? x[(int64_t)pr * bs + d % bs] : qd[d];
+    q[d] = (pr >= 0 && !fixed[d]) ? x[(int64_t)pr * bs + d % bs] : qd[d];
 }
 
 __global__ void fold_all_kernel(double* __restrict__ x64, float* __restrict__ xs, int64_t n) {
@@ -672,16 +685,9 @@ void batch_build(sfem_ctx* ctx, sfem_batch* B) {
     SFEM_HIP(hipEventCreate(&e1));
     SFEM_HIP(hipEventRecord(e0, ctx->stream));
     DevBuf<uint32_t> ufree(ctx, (size_t)nunits), upos(ctx, (size_t)nunits), utot(ctx, 1);
-    DevBuf<int> complete(ctx, 1);
-    const int one = 1;
-    h2d(ctx, complete.get(), &one, sizeof(int));
     const dim3 gu(grid_for(nunits, 256));
-    hipLaunchKernelGGL(unit_free_kernel, gu, dim3(256), 0, ctx->stream, bc->fixed.get(), nunits, bs, ufree.get(),
-                       complete.get());
+    hipLaunchKernelGGL(unit_free_kernel, gu, dim3(256), 0, ctx->stream, bc->fixed.get(), nunits, bs, ufree.get());
     SFEM_LAUNCH_CHECK();
-    int comp = 0;
-    d2h(ctx, &comp, complete.get(), sizeof(int));
-    if (!comp) fail(SFEM_ERR_UNSUPPORTED, "batch: every node must be fully fixed or fully free");
     exclusive_scan_u32_dev(ctx, ufree.get(), upos.get(), nunits, utot.get());
     const int64_t np = B->nprob;
     DevBuf<int64_t> doff(ctx, (size_t)np + 1);
@@ -746,7 +752,7 @@ void batch_build(sfem_ctx* ctx, sfem_batch* B) {
         hipLaunchKernelGGL(sell_fill_pad_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
                            K->bval.get(), B->pad_of.get(), nunits, B->sl_ptr.get(), B->sl_w.get(), B->scol.get(),
                            B->sval64.get(), B->sval32.get(), B->sval16.get(), base, B->binv64.get(),
-                           B->binv32.get());
+                           B->binv32.get(), bc->fixed.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(sell_pad_gap_kernel<3>, gr, dim3(256), 0, ctx->stream, blen.get(), base, B->sl_ptr.get(),
                            B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get(), B->sval16.get());
@@ -754,7 +760,7 @@ void batch_build(sfem_ctx* ctx, sfem_batch* B) {
         hipLaunchKernelGGL(sell_fill_pad_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
                            K->bval.get(), B->pad_of.get(), nunits, B->sl_ptr.get(), B->sl_w.get(), B->scol.get(),
                            B->sval64.get(), B->sval32.get(), B->sval16.get(), base, B->binv64.get(),
-                           B->binv32.get());
+                           B->binv32.get(), bc->fixed.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(sell_pad_gap_kernel<2>, gr, dim3(256), 0, ctx->stream, blen.get(), base, B->sl_ptr.get(),
                            B->sl_w.get(), B->scol.get(), B->sval64.get(), B->sval32.get(), B->sval16.get());
@@ -924,7 +930,7 @@ double batch_run(sfem_ctx* ctx, sfem_batch* B, const sfem_solve_opts& o, int ite
 void batch_solution(sfem_ctx* ctx, const sfem_batch* B, double* q) {
     DevBuf<double> dq(ctx, (size_t)B->n);
     hipLaunchKernelGGL(batch_recombine_kernel, dim3(grid_for(B->n, 256)), dim3(256), 0, ctx->stream,
-                       B->pad_of.get(), B->x.get(), B->qd.get(), B->n, B->bs, dq.get());
+                       B->pad_of.get(), B->fixed.get(), B->x.get(), B->qd.get(), B->n, B->bs, dq.get());
     SFEM_LAUNCH_CHECK();
     d2h(ctx, q, dq.get(), sizeof(double) * B->n);
 }
@@ -983,6 +989,14 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
     batch_build(ctx, B.get());
     B->qd.reset(ctx, (size_t)B->n);
     SFEM_HIP(hipMemcpyAsync(B->qd.get(), bc->qd.get(), sizeof(double) * B->n, hipMemcpyDeviceToDevice, ctx->stream));
+    B->fixed.reset(ctx, (size_t)B->n);
+    SFEM_HIP(hipMemcpyAsync(B->fixed.get(), bc->fixed.get(), (size_t)B->n, hipMemcpyDeviceToDevice, ctx->stream));
+    // free dofs per problem (a partially fixed node keeps only its free dofs)
+    for (int64_t p = 0; p < nprob; ++p) {
+        int64_t nfree = 0;
+        for (int64_t d = node_offsets[p] * n_dof; d < node_offsets[p + 1] * n_dof; ++d) nfree += fixed[d] ? 0 : 1;
+        B->hprobs[(size_t)p].n_free = nfree;
+    }
     sync(ctx);
     sfem_csr_destroy(B->K);
     B->K = nullptr;
@@ -997,7 +1011,7 @@ int64_t batch_device_bytes(const sfem_batch* B, int precision) {
     const int64_t b2 = (int64_t)B->bs * B->bs;
     const int64_t vb = (int64_t)prec_vec_bytes(precision);
     return B->sell_blocks * (4 + b2 * 14) + B->nslices * 12 + B->npad * (8 * 3 + 4 + 4 * vb + (vb == 4 ? 4 : 0)) +
-           B->nslices * 64 * b2 * 12 + (int64_t)B->pad_of.size() * 4 + B->n * 8;
+           B->nslices * 64 * b2 * 12 + (int64_t)B->pad_of.size() * 4 + B->n * 9;
 }
 
 }  // namespace sfem

@@ -91,6 +91,7 @@ struct sfem_batch {
     sfem_csr* K = nullptr;
     sfem_bc* bc = nullptr;
     sfem::DevBuf<int32_t> pad_of;  // node unit -> padded row (-1 fixed)
+    sfem::DevBuf<uint8_t> fixed;   // dof mask (a node may have fixed and free dofs)
     sfem::DevBuf<int64_t> sl_ptr;
     sfem::DevBuf<int32_t> sl_w, scol;
     sfem::DevBuf<double> sval64, rhs, x, dinv64, qd;

@@ -73,7 +73,8 @@ def test_batch_q4_with_prescribed_values(ctx):
         st, per = B.solve(precision, rtol=1e-11, preconditioner=pc)
         for p, q in zip(probs, B.solution()):
             assert relerr(q, oracle_solution(p)) <= ERR
-            assert np.array_equal(q[p.fixed == 1], p.q_d[p.fixed == 1])
+            qd = p.q_d if p.q_d is not None else np.zeros(p.mesh.n_dof)
+        assert np.array_equal(q[p.fixed == 1], qd[p.fixed == 1])
     # below the floor: the solve ends on stagnation, not at max_iter
     st, per = B.solve("fp64", rtol=1e-13, raise_on_fail=False)
     assert st["status"] == smearfem.NotConvergedError.code
@@ -154,17 +155,58 @@ def test_batch_iteration_cap_and_errors(ctx):
     assert all(s["stop_reason"] == 4 and s["iterations"] == 3 for s in per)
     assert all(s["rel_residual"] > 1e-6 for s in per)
     B.close()
-    # a node with only some dofs fixed: the padded block layout does not apply
-    m = structured_mesh((3, 3, 3))
-    fixed = clamp_x0(m)
-    fixed[3 * 21 + 2] = 1  # node 21 is at x = 1
-    with pytest.raises(smearfem.UnsupportedError):
-        Batch([make_problem(m, 1.0, 0.3, face_load(m, 0, 1, 2, -1.0), fixed)], ctx)
     # an IEN entry outside its problem's nodes
     bad = structured_mesh((2, 2, 2))
     bad.IEN[1, 3] = bad.n_nodes + 1
     with pytest.raises(smearfem.BoundsError):
         Batch([make_problem(bad, 1.0, 0.3, np.zeros(bad.n_dof), clamp_x0(bad))], ctx)
+
+
+def _partial_clamp_problems():
+    """Nodes with only some dofs fixed: a roller plane (u_z = 0 on z = 0) next
+    to a clamped face, a single fixed dof on a loaded node, and a plain
+    cantilever in between."""
+    out = []
+    m = structured_mesh((3, 3, 3))
+    fixed = clamp_x0(m)
+    fixed[3 * 21 + 2] = 1  # node 21 is at x = 1: only u_z fixed
+    out.append(make_problem(m, 1.0, 0.3, face_load(m, 0, 1, 2, -1.0), fixed))
+    out.append(uneven_hex8_problems(1, 5, lo=3, hi=4)[0])
+    m = structured_mesh((4, 2, 3))
+    fixed = clamp_x0(m)
+    x = m.NodeList
+    for a in np.nonzero(x[2] == 0.0)[0]:
+        fixed[3 * a + 2] = 1
+    q_d = np.zeros(m.n_dof)
+    top = np.nonzero(x[2] == x[2].max())[0]
+    fixed[3 * top] = 1
+    q_d[3 * top] = 0.01  # prescribed u_x on the top face, u_y and u_z free
+    out.append(make_problem(m, 2.0, 0.25, face_load(m, 0, 1, 2, -0.5), fixed, q_d))
+    return out
+
+
+@pytest.mark.parametrize("precision,pc", [("fp64", "jacobi"), ("mixed", "jacobi"), ("split", "block_jacobi"),
+                                          ("fp64", "block_jacobi")])
+def test_batch_partially_fixed_nodes(ctx, precision, pc):
+    print(f"[case] batch partial clamp {precision} {pc}", flush=True)
+    probs = _partial_clamp_problems()
+    B = Batch(probs, ctx)
+    st, per = B.solve(precision, rtol=1e-12, item_slices=2, preconditioner=pc)
+    qs = B.solution()
+    B.close()
+    assert st["not_converged"] == 0
+    for p, q, s in zip(probs, qs, per):
+        assert s["n_free"] == int((p.fixed == 0).sum())
+        assert s["stop_reason"] == 1 and s["rel_residual"] <= 1e-12
+        assert relerr(q, oracle_solution(p)) <= ERR
+        qd = p.q_d if p.q_d is not None else np.zeros(p.mesh.n_dof)
+        assert np.array_equal(q[p.fixed == 1], qd[p.fixed == 1])
+    # a problem's solution is bitwise the same alone
+    A = Batch([probs[2]], ctx)
+    A.solve(precision, rtol=1e-12, item_slices=2, preconditioner=pc)
+    alone = A.solution()[0]
+    A.close()
+    assert np.array_equal(alone, qs[2])
 
 
 def test_autotune_picks_a_measured_candidate(ctx, tmp_path, monkeypatch):

@@ -67,6 +67,34 @@ def test_alone_equals_in_mix(ctx, precision, pc):
     S.close()
 
 
+def test_partially_fixed_request_in_mix(ctx):
+    """A request with a roller plane (only u_z fixed on z = 0) is served next
+    to plain cantilevers: same bits alone and in the mix, oracle parity."""
+    print("[case] serve partial clamp in mix", flush=True)
+    m = structured_mesh((5, 3, 3), lengths=(1.0, 0.6, 0.6))
+    fixed = clamp_x0(m)
+    fixed[3 * np.nonzero(m.NodeList[2] == 0.0)[0] + 2] = 1
+    f = random_face_load(m, 0, 1, fixed, seed=7)
+    target = make_problem(m, 10.0 ** np.random.default_rng(7).uniform(-1, 1, m.n_elem), 0.3, f, fixed)
+    S = Server(ctx, precision="fp64", rtol=1e-11, iters_per_step=16, item_slices=2, preconditioner="block_jacobi")
+    rid = S.submit(target)
+    S.drain()
+    q_alone = S.take(rid, target.n_dof)
+    others = [request((8, 4, 3), 2), request((12, 5), 3)]
+    ids = [S.submit(others[0])]
+    S.step()
+    rid = S.submit(target)
+    ids.append(S.submit(others[1]))
+    S.drain()
+    q_mix = S.take(rid, target.n_dof)
+    assert np.array_equal(q_alone, q_mix)
+    assert relerr(q_mix, oracle_solution(target)) <= ERR
+    assert np.all(q_mix[fixed == 1] == 0.0)
+    for i, p in zip(ids, others):
+        assert relerr(S.take(i, p.n_dof), oracle_solution(p)) <= ERR
+    S.close()
+
+
 def test_hbm_budget_admission(ctx):
     print("[case] serve budget", flush=True)
     reqs = [request((6, 4, 4), s) for s in range(4)]
