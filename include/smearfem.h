@@ -181,6 +181,8 @@ typedef struct {
 
 #define SFEM_PC_JACOBI        0   /* z = r / diag(K_ff)                                   */
 #define SFEM_PC_BLOCK_JACOBI  1   /* z = (node diagonal block)^-1 r; node-block format only */
+#define SFEM_PC_MULTIGRID     2   /* z = one geometric multigrid V-cycle (sfem_system_set_grid
+                                     first); precision FP64, SPLIT or FP32_MATRIX          */
 
 typedef struct {
     int64_t iterations;      /* CG iterations performed                                  */
@@ -207,6 +209,54 @@ int  sfem_system_solve(sfem_ctx* ctx, sfem_system* sys, const sfem_solve_opts* o
  * fixed dofs (n values; zero on free dofs). */
 int  sfem_system_solution(sfem_ctx* ctx, const sfem_system* sys, double* q,
                           double* reactions);
+
+/* ---- geometric multigrid preconditioner --------------------------------
+ * No counterpart in the reference (its solve is a dense inverse,
+ * examples/vector3D.jl:316-320); the solution is the same one, to rtol on the
+ * FP64 true residual.  For node-block systems (n_dof = ndim) whose nodes form
+ * the structured grid of the reference's meshgrid numbering
+ * (examples/vector3D.jl:67-101: node = i + (nx+1)(j + (ny+1) k), x fastest),
+ * with dims[d] elements along direction d.  Any geometry, material field and
+ * fixed set; K may hold only couplings between grid neighbours (element
+ * stiffness, surface penalty), else SFEM_ERR_ARGUMENT.
+ *   Levels: the grid coarsened by 2 along each direction whose element count
+ *   is even, until a level has at most coarse_nodes nodes (at least once);
+ *   coarse operators are Galerkin products P'AP with multilinear P on the
+ *   free dofs; the coarsest (at most 400 nodes, else SFEM_ERR_UNSUPPORTED)
+ *   is solved with a dense inverse.  Smoothing: `degree` Chebyshev steps in
+ *   (node diagonal block)^-1 K on [lo, hi] x lambda_max before and after the
+ *   coarse correction, lambda_max by power_iters power iterations.
+ * The hierarchy depends on K and the fixed set only: sfem_system_set_bc keeps
+ * it.  Solve with opts.preconditioner = SFEM_PC_MULTIGRID. */
+#define SFEM_MG_MAX_LEVELS 24
+typedef struct {
+    int     degree;        /* Chebyshev steps before and after the coarse correction (2)  */
+    double  lo, hi;        /* smoothing interval, x lambda_max (0.1, 1.1)                 */
+    int     power_iters;   /* power iterations per level for lambda_max (12)              */
+    int64_t coarse_nodes;  /* stop coarsening at this many nodes (64)                     */
+} sfem_mg_opts;
+
+typedef struct {
+    int     levels;
+    int     degree;
+    int64_t nodes[SFEM_MG_MAX_LEVELS];
+    int64_t dims[SFEM_MG_MAX_LEVELS][3];     /* elements per direction (0 beyond ndim)  */
+    double  lambda_max[SFEM_MG_MAX_LEVELS];  /* estimate; 0 on the coarsest level         */
+    int64_t coarse_dofs;
+    double  setup_ms;                        /* host wall time of the build             */
+} sfem_mg_info;
+
+void sfem_mg_opts_default(sfem_mg_opts* opts);
+/* Builds the hierarchy (opts NULL: defaults). */
+int  sfem_system_set_grid(sfem_ctx* ctx, sfem_system* sys, int ndim, const int64_t* dims,
+                          const sfem_mg_opts* opts);
+int  sfem_system_mg_info(const sfem_system* sys, sfem_mg_info* info);
+/* Level >= 1 operator as stored for the next Galerkin product (FP64, fixed
+ * rows and columns zero): stencil[(s*B2 + a*n_dof + b)*nodes + I] =
+ * A[(I,a),(I+off_s,b)], B2 = n_dof^2, off_s = (s%3-1, s/3%3-1, s/9-1) (3D) or
+ * (s%3-1, s/3-1) (2D); fixed: nodes*n_dof flags.  Either may be NULL. */
+int  sfem_system_mg_stencil(sfem_ctx* ctx, const sfem_system* sys, int level, double* stencil,
+                            uint8_t* fixed);
 
 /* One-shot: bc_create + eliminate + solve + solution. */
 int  sfem_solve(sfem_ctx* ctx, const sfem_csr* K, const double* f, const uint8_t* fixed,

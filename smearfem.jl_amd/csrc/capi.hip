@@ -562,6 +562,27 @@ int sfem_system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* opts
     return guard(ctx, [&] { system_solve(ctx, S, &o, stats ? stats : &local); });
 }
 
+void sfem_mg_opts_default(sfem_mg_opts* o) {
+    if (o) mg_opts_default(o);
+}
+
+int sfem_system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!S) { ctx->last_error = "set_grid: NULL system"; return SFEM_ERR_ARGUMENT; }
+    return guard(ctx, [&] { system_set_grid(ctx, S, ndim, dims, opts); });
+}
+
+int sfem_system_mg_info(const sfem_system* S, sfem_mg_info* info) {
+    if (!S || !info) return SFEM_ERR_ARGUMENT;
+    return guard(S->ctx, [&] { mg_info(S, info); });
+}
+
+int sfem_system_mg_stencil(sfem_ctx* ctx, const sfem_system* S, int level, double* stencil, uint8_t* fixed) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!S) { ctx->last_error = "mg_stencil: NULL system"; return SFEM_ERR_ARGUMENT; }
+    return guard(ctx, [&] { mg_stencil(ctx, S, level, stencil, fixed); });
+}
+
 int sfem_system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions) {
     if (!ctx) return SFEM_ERR_ARGUMENT;
     if (!S) { ctx->last_error = "solution: NULL system"; return SFEM_ERR_ARGUMENT; }

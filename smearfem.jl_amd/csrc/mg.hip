@@ -1,0 +1,1387 @@
+// mg.hip -- geometric multigrid preconditioned CG for node-block systems whose
+// nodes form a structured grid in the reference's meshgrid numbering
+// (examples/vector3D.jl:67-101: x fastest, then y, then z; 2D: x, then y).
+//
+// The reference solves K_free q_f = C'(f - K q_d) with a dense inverse
+// (examples/vector3D.jl:316-320).  This solver reaches the same solution (the
+// stopping test is on the FP64 true residual) in a number of iterations that
+// does not grow with the mesh: the preconditioner is one V-cycle over a
+// hierarchy of nested grids.
+//
+//   level 0    the system's node-block sliced operator with FP32 values and
+//              its inverse diagonal blocks; vectors in block-row order.
+//   level l+1  the grid coarsened by 2 along every direction whose element
+//              count is even (others kept).  Operator A_{l+1} = P' A_l P, P
+//              the multilinear interpolation between the nested grids,
+//              restricted to free dofs on both sides (a coarse dof is fixed
+//              when the fine dof at the same node is fixed).  Computed in
+//              FP64 on 3^d-point node-block stencils (the grid gives every
+//              column, so none is stored), symmetrised, and kept in FP32 for
+//              smoothing.
+//   coarsest   dense inverse in FP64, computed on the host.
+//
+// Smoother: Chebyshev polynomial in B^-1 A (B = the diagonal node blocks) on
+// [lo, hi] x lambda_max, lambda_max from power iteration on every level;
+// `degree` steps before and after the coarse correction.  The V-cycle is a
+// fixed symmetric linear operator up to FP32 rounding and preconditions an
+// FP64-vector CG whose operator is the FP64, split or FP32 node-block format.
+//
+// Every reduction is deterministic (fixed grids, per-block partials summed
+// in a fixed order by the last block), so solves are bitwise reproducible.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <vector>
+
+#include "block_ops.hpp"
+#include "sfem_objects.hpp"
+#include "solver_common.hpp"
+
+namespace sfem {
+
+constexpr int kMgMaxDegree = 8;
+constexpr int64_t kMgMaxCoarseNodes = 400;  // dense coarsest level (host inverse, O(dofs^3))
+
+struct MgLevel {
+    int64_t n[3] = {1, 1, 1};  // nodes per grid direction
+    int f[3] = {1, 1, 1};      // coarsening factor from the finer level
+    int64_t nn = 0;            // grid nodes
+    int64_t nrows = 0;         // rows of the level's vectors (level 0: block rows)
+    DevBuf<double> st64;       // masked symmetric FP64 stencil (levels >= 1)
+    DevBuf<float> st32;        // smoothing operator: st64 with a unit diagonal on fixed dofs
+    DevBuf<float> binv;        // inverse diagonal blocks, [(j*BS+i)*nrows + row]
+    DevBuf<uint8_t> fixed;     // nrows*bs: the dof is fixed
+    DevBuf<float> r, x, da, db, b;
+    double lmax = 0;
+    float c1[kMgMaxDegree] = {}, c2[kMgMaxDegree] = {};
+};
+
+struct MgHierarchy {
+    int nd = 0, bs = 0;
+    int degree = 2;
+    std::vector<MgLevel> L;
+    DevBuf<int32_t> row_of_node;  // fine grid node -> block row (-1: every dof fixed)
+    DevBuf<int32_t> node_of_row;  // block row -> fine grid node
+    DevBuf<double> cinv;          // coarsest level: dense inverse, nc x nc row-major
+    int64_t nc = 0;
+    double setup_ms = 0;
+};
+
+void MgDelete::operator()(MgHierarchy* h) const { delete h; }
+
+namespace {
+
+// ------------------------------------------------------------ grid --
+struct Grid {
+    int64_t n0, n1, n2;  // nodes per direction (n2 = 1 in 2D)
+};
+
+template <int ND>
+struct Sten {
+    static constexpr int NS = ND == 3 ? 27 : 9;  // stencil slots
+    static constexpr int C = NS / 2;             // slot of offset 0
+};
+
+template <int ND>
+__host__ __device__ __forceinline__ void coords_of(int64_t node, const Grid& g, int64_t c[3]) {
+    c[0] = node % g.n0;
+    const int64_t t = node / g.n0;
+    c[1] = t % g.n1;
+    c[2] = ND == 3 ? t / g.n1 : 0;
+}
+
+__host__ __device__ __forceinline__ int64_t node_at(const Grid& g, int64_t x, int64_t y, int64_t z) {
+    return x + g.n0 * (y + g.n1 * z);
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ bool inside(const Grid& g, int64_t x, int64_t y, int64_t z) {
+    return x >= 0 && x < g.n0 && y >= 0 && y < g.n1 && (ND == 2 ? z == 0 : (z >= 0 && z < g.n2));
+}
+
+// slot of the offset (dx, dy, dz), each in {-1, 0, 1} (dz = 0 in 2D); the
+// slot of the opposite offset is NS - 1 - slot
+template <int ND>
+__host__ __device__ __forceinline__ int slot_of(int dx, int dy, int dz) {
+    return (dx + 1) + 3 * (dy + 1) + (ND == 3 ? 9 * (dz + 1) : 0);
+}
+
+template <int ND>
+__host__ __device__ __forceinline__ void offset_of(int s, int& dx, int& dy, int& dz) {
+    dx = s % 3 - 1;
+    dy = (s / 3) % 3 - 1;
+    dz = ND == 3 ? s / 9 - 1 : 0;
+}
+
+// ------------------------------------------------------- hierarchy --
+// Level-0 maps: a node with any free dof is a block row (rank in node order,
+// as the elimination numbers them).
+__global__ void node_free_kernel(const int32_t* __restrict__ free_of, int64_t nn, int bs,
+                                 uint32_t* __restrict__ flag) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nn) return;
+    int c = 0;
+    for (int i = 0; i < bs; ++i) c += free_of[u * bs + i] >= 0 ? 1 : 0;
+    flag[u] = c > 0 ? 1u : 0u;
+}
+
+__global__ void row_maps_kernel(const uint32_t* __restrict__ flag, const uint32_t* __restrict__ pos,
+                                const int32_t* __restrict__ free_of, int64_t nn, int bs,
+                                int32_t* __restrict__ row_of_node, int32_t* __restrict__ node_of_row,
+                                uint8_t* __restrict__ fixed0, uint8_t* __restrict__ fixnode) {
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nn) return;
+    const int32_t row = flag[u] ? (int32_t)pos[u] : -1;
+    row_of_node[u] = row;
+    for (int i = 0; i < bs; ++i) fixnode[u * bs + i] = free_of[u * bs + i] < 0 ? 1 : 0;
+    if (row < 0) return;
+    node_of_row[row] = (int32_t)u;
+    for (int i = 0; i < bs; ++i) fixed0[(int64_t)row * bs + i] = free_of[u * bs + i] < 0 ? 1 : 0;
+}
+
+// Fine stencil from K's node blocks (one thread per stored block column):
+// st[(s*B2 + i*BS + j)*nn + row] = K[(row,i),(row+off_s,j)], 0 where either
+// dof is fixed.  A block between nodes that are not grid neighbours sets *bad.
+template <int ND>
+__global__ void stencil_from_k_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                      const double* __restrict__ bval, int64_t nn, int transposed,
+                                      const int32_t* __restrict__ free_of, Grid g, double* __restrict__ st,
+                                      int* __restrict__ bad) {
+    constexpr int BS = ND, B2 = BS * BS;
+    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cu >= nn) return;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t ru = browidx[u];
+        // K stores K[(ru,i),(cu,j)] at bval[u*B2 + i*BS + j]; transposed, K'
+        const int64_t row = transposed ? cu : ru, col = transposed ? ru : cu;
+        int64_t a[3], b[3];
+        coords_of<ND>(row, g, a);
+        coords_of<ND>(col, g, b);
+        const int64_t dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
+            atomicOr(bad, 1);
+            continue;
+        }
+        const int s = slot_of<ND>((int)dx, (int)dy, (int)dz);
+        for (int i = 0; i < BS; ++i)
+            for (int j = 0; j < BS; ++j) {
+                const double v = transposed ? bval[u * B2 + j * BS + i] : bval[u * B2 + i * BS + j];
+                const bool masked = free_of[row * BS + i] < 0 || free_of[col * BS + j] < 0;
+                st[((int64_t)s * B2 + i * BS + j) * nn + row] = masked ? 0.0 : v;
+            }
+    }
+}
+
+// Coarse dof fixed <=> the fine dof at the same node is fixed.
+template <int ND>
+__global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, Grid gc, int f0, int f1, int f2,
+                                    uint8_t* __restrict__ cfix) {
+    constexpr int BS = ND;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= gc.n0 * gc.n1 * gc.n2) return;
+    int64_t c[3];
+    coords_of<ND>(I, gc, c);
+    const int64_t fn = node_at(gf, f0 * c[0], f1 * c[1], f2 * c[2]);
+    for (int a = 0; a < BS; ++a) cfix[I * BS + a] = ffix[fn * BS + a];
+}
+
+// Galerkin product on stencils: one thread per (coarse node I, slot s),
+// A_c(I, I+D)[a][b] = sum_{i, j} P(i, I) A_f(i, j)[a][b] P(j, I+D) over the
+// fine nodes i = f I + alpha, j = f (I+D) + beta of the two supports, in a
+// fixed order.  P weights are products of 1 and 1/2 per direction.
+template <int ND>
+__global__ void galerkin_kernel(const double* __restrict__ stf, Grid gf, Grid gc, int f0, int f1, int f2,
+                                double* __restrict__ stc) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+    const int64_t nnc = gc.n0 * gc.n1 * gc.n2, nnf = gf.n0 * gf.n1 * gf.n2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nnc * NS) return;
+    const int s = (int)(t / nnc);
+    const int64_t I = t - (int64_t)s * nnc;
+    int64_t ci[3];
+    coords_of<ND>(I, gc, ci);
+    int D[3];
+    offset_of<ND>(s, D[0], D[1], D[2]);
+    const int64_t cj[3] = {ci[0] + D[0], ci[1] + D[1], ci[2] + D[2]};
+    double acc[B2];
+#pragma unroll
+    for (int m = 0; m < B2; ++m) acc[m] = 0.0;
+    if (inside<ND>(gc, cj[0], cj[1], cj[2])) {
+        const int f[3] = {f0, f1, f2};
+        int lo[3], hi[3];
+        for (int d = 0; d < 3; ++d) {
+            lo[d] = f[d] == 2 ? -1 : 0;
+            hi[d] = f[d] == 2 ? 1 : 0;
+        }
+        for (int az = lo[2]; az <= hi[2]; ++az)
+            for (int ay = lo[1]; ay <= hi[1]; ++ay)
+                for (int ax = lo[0]; ax <= hi[0]; ++ax) {
+                    const int64_t ix = f0 * ci[0] + ax, iy = f1 * ci[1] + ay, iz = f2 * ci[2] + az;
+                    if (!inside<ND>(gf, ix, iy, iz)) continue;
+                    const double wa = (ax ? 0.5 : 1.0) * (ay ? 0.5 : 1.0) * (az ? 0.5 : 1.0);
+                    const int64_t inode = node_at(gf, ix, iy, iz);
+                    for (int bz = lo[2]; bz <= hi[2]; ++bz)
+                        for (int by = lo[1]; by <= hi[1]; ++by)
+                            for (int bx = lo[0]; bx <= hi[0]; ++bx) {
+                                const int64_t jx = f0 * cj[0] + bx, jy = f1 * cj[1] + by, jz = f2 * cj[2] + bz;
+                                const int64_t ox = jx - ix, oy = jy - iy, oz = jz - iz;
+                                if (ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1) continue;
+                                if (!inside<ND>(gf, jx, jy, jz)) continue;
+                                const double w = wa * ((bx ? 0.5 : 1.0) * (by ? 0.5 : 1.0) * (bz ? 0.5 : 1.0));
+                                const int sf = slot_of<ND>((int)ox, (int)oy, (int)oz);
+                                const double* src = stf + (int64_t)sf * B2 * nnf + inode;
+#pragma unroll
+                                for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nnf], acc[m]);
+                            }
+                }
+    }
+#pragma unroll
+    for (int m = 0; m < B2; ++m) stc[((int64_t)s * B2 + m) * nnc + I] = acc[m];
+}
+
+// Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
+// side), zero the rows and columns of fixed dofs (st64, the next Galerkin
+// product's input) and put a unit diagonal on them (st32, the smoother's
+// operator).
+template <int ND>
+__global__ void sym_mask_kernel(const double* __restrict__ raw, Grid g, const uint8_t* __restrict__ fixed,
+                                double* __restrict__ st64, float* __restrict__ st32) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+    const int64_t nn = g.n0 * g.n1 * g.n2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * NS) return;
+    const int s = (int)(t / nn);
+    const int64_t I = t - (int64_t)s * nn;
+    int64_t c[3];
+    coords_of<ND>(I, g, c);
+    int dx, dy, dz;
+    offset_of<ND>(s, dx, dy, dz);
+    const bool valid = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
+    const int64_t J = valid ? node_at(g, c[0] + dx, c[1] + dy, c[2] + dz) : I;
+    const int sp = NS - 1 - s;
+    for (int a = 0; a < BS; ++a)
+        for (int b = 0; b < BS; ++b) {
+            double v = 0.0;
+            bool unit = false;
+            if (valid) {
+                const bool masked = fixed[I * BS + a] || fixed[J * BS + b];
+                v = masked ? 0.0
+                           : 0.5 * (raw[((int64_t)s * B2 + a * BS + b) * nn + I] +
+                                    raw[((int64_t)sp * B2 + b * BS + a) * nn + J]);
+                unit = masked && s == Sten<ND>::C && a == b;
+            }
+            const int64_t o = ((int64_t)s * B2 + a * BS + b) * nn + I;
+            st64[o] = v;
+            st32[o] = unit ? 1.0f : (float)v;
+        }
+}
+
+template <int ND>
+__global__ void stencil_binv_kernel(const double* __restrict__ st64, const uint8_t* __restrict__ fixed, int64_t nn,
+                                    float* __restrict__ binv) {
+    constexpr int BS = ND, B2 = BS * BS, C = Sten<ND>::C;
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    double m[B2], inv[B2];
+    for (int a = 0; a < BS; ++a)
+        for (int b = 0; b < BS; ++b) {
+            const double v = st64[((int64_t)C * B2 + a * BS + b) * nn + node];
+            m[a * BS + b] = (fixed[node * BS + a] && a == b) ? 1.0 : v;
+        }
+    invert_block<BS>(m, inv);
+    for (int k = 0; k < B2; ++k) binv[(int64_t)k * nn + node] = (float)inv[k];
+}
+
+// ------------------------------------------------------- V-cycle ----
+struct SellOp {
+    const int64_t* sl_ptr;
+    const int32_t* sl_w;
+    const int32_t* scol;
+    const float* val;
+    int64_t nslices, nrows, per_block;
+};
+
+struct StOp {
+    const float* val;
+    Grid g;
+    int64_t nn;
+};
+
+template <int ND>
+__device__ __forceinline__ void stencil_row(const StOp& op, int64_t node, const float* __restrict__ v,
+                                            float (&out)[ND]) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+    int64_t c[3];
+    coords_of<ND>(node, op.g, c);
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[j] = 0.0f;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+        int dx, dy, dz;
+        offset_of<ND>(s, dx, dy, dz);
+        const bool ok = inside<ND>(op.g, c[0] + dx, c[1] + dy, c[2] + dz);
+        const int64_t m = ok ? node + dx + op.g.n0 * (dy + op.g.n1 * (int64_t)dz) : node;
+        float xv[BS];
+#pragma unroll
+        for (int i = 0; i < BS; ++i) xv[i] = ok ? v[m * BS + i] : 0.0f;
+#pragma unroll
+        for (int j = 0; j < BS; ++j)
+#pragma unroll
+            for (int i = 0; i < BS; ++i)
+                out[j] = fmaf(op.val[((int64_t)s * B2 + j * BS + i) * op.nn + node], xv[i], out[j]);
+    }
+}
+
+// MODE 0: r -= A v; d_out = c1 d_in + c2 B^-1 r; x += d_out (a Chebyshev step)
+// MODE 1: r -= A v (the residual after the last step)
+template <int BS, int MODE>
+__device__ __forceinline__ void cheb_epilogue(int64_t row, int64_t nrows, const float (&av)[BS], float* __restrict__ r,
+                                              float* __restrict__ x, const float* d_in, float* __restrict__ d_out,
+                                              float c1, float c2, const float* __restrict__ binv) {
+    float rv[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        const int64_t d = row * BS + j;
+        rv[j] = r[d] - av[j];
+        r[d] = rv[j];
+    }
+    if (MODE == 1) return;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float z = 0.0f;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) z = fmaf(binv[(int64_t)(j * BS + i) * nrows + row], rv[i], z);
+        const int64_t d = row * BS + j;
+        const float dn = fmaf(c2, z, d_in ? c1 * d_in[d] : 0.0f);
+        d_out[d] = dn;
+        x[d] += dn;
+    }
+}
+
+template <int BS, int MODE, bool NT>
+__global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp op, const float* v, float* __restrict__ r,
+                                                             float* __restrict__ x, const float* d_in,
+                                                             float* __restrict__ d_out, float c1, float c2,
+                                                             const float* __restrict__ binv, const int* done) {
+    if (*done) return;
+    const int64_t s0 = logical_block() * op.per_block;
+    const int64_t s1 = min(op.nslices, s0 + op.per_block);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto gx = [=](int64_t c) { return v[c]; };
+    for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
+        float out[BS];
+        sell_row<BS, float, float, NT>(op.sl_ptr, op.sl_w, op.scol, Vals<float>{op.val}, sl, lane, gx, out);
+        const int64_t row = sl * 64 + lane;
+        if (row < op.nrows) cheb_epilogue<BS, MODE>(row, op.nrows, out, r, x, d_in, d_out, c1, c2, binv);
+    }
+}
+
+template <int ND, int MODE>
+__global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp op, const float* v, float* __restrict__ r,
+                                                                float* __restrict__ x, const float* d_in,
+                                                                float* __restrict__ d_out, float c1, float c2,
+                                                                const float* __restrict__ binv, const int* done) {
+    if (*done) return;
+    const int64_t node = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (node >= op.nn) return;
+    float out[ND];
+    stencil_row<ND>(op, node, v, out);
+    cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+}
+
+// First step from x = 0: r = b, d = c2 B^-1 b, x = d.
+template <int BS, typename TB>
+__global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restrict__ b, int64_t nrows,
+                                                              const float* __restrict__ binv, float c2,
+                                                              float* __restrict__ r, float* __restrict__ x,
+                                                              float* __restrict__ d, const int* done) {
+    if (*done) return;
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (row >= nrows) return;
+    float rv[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        rv[j] = (float)b[row * BS + j];
+        r[row * BS + j] = rv[j];
+    }
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float z = 0.0f;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) z = fmaf(binv[(int64_t)(j * BS + i) * nrows + row], rv[i], z);
+        const float dn = c2 * z;
+        d[row * BS + j] = dn;
+        x[row * BS + j] = dn;
+    }
+}
+
+// b_c = P' r_f on the coarse grid (0 on fixed coarse dofs); row_of_node maps
+// fine grid nodes to rows of r_f (NULL: the identity).
+template <int ND>
+__global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, Grid gf,
+                                                            const int32_t* __restrict__ row_of_node, Grid gc, int f0,
+                                                            int f1, int f2, const uint8_t* __restrict__ cfix,
+                                                            float* __restrict__ bc, const int* done) {
+    constexpr int BS = ND;
+    if (*done) return;
+    const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (I >= gc.n0 * gc.n1 * gc.n2) return;
+    int64_t ci[3];
+    coords_of<ND>(I, gc, ci);
+    const int f[3] = {f0, f1, f2};
+    int lo[3], hi[3];
+    for (int d = 0; d < 3; ++d) {
+        lo[d] = f[d] == 2 ? -1 : 0;
+        hi[d] = f[d] == 2 ? 1 : 0;
+    }
+    float acc[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
+    for (int az = lo[2]; az <= hi[2]; ++az)
+        for (int ay = lo[1]; ay <= hi[1]; ++ay)
+            for (int ax = lo[0]; ax <= hi[0]; ++ax) {
+                const int64_t ix = f0 * ci[0] + ax, iy = f1 * ci[1] + ay, iz = f2 * ci[2] + az;
+                if (!inside<ND>(gf, ix, iy, iz)) continue;
+                const int64_t fn = node_at(gf, ix, iy, iz);
+                const int64_t row = row_of_node ? (int64_t)row_of_node[fn] : fn;
+                if (row < 0) continue;
+                const float w = (ax ? 0.5f : 1.0f) * (ay ? 0.5f : 1.0f) * (az ? 0.5f : 1.0f);
+#pragma unroll
+                for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j], acc[j]);
+            }
+#pragma unroll
+    for (int j = 0; j < BS; ++j) bc[I * BS + j] = cfix[I * BS + j] ? 0.0f : acc[j];
+}
+
+// e = P x_c on the fine rows (0 on fixed fine dofs), x_f += e; node_of_row
+// maps rows to fine grid nodes (NULL: the identity).
+template <int ND>
+__global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restrict__ xc, Grid gc, Grid gf, int f0,
+                                                           int f1, int f2, const int32_t* __restrict__ node_of_row,
+                                                           int64_t nrows, const uint8_t* __restrict__ ffix,
+                                                           float* __restrict__ xf, float* __restrict__ e,
+                                                           const int* done) {
+    constexpr int BS = ND;
+    if (*done) return;
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (row >= nrows) return;
+    const int64_t node = node_of_row ? (int64_t)node_of_row[row] : row;
+    int64_t cf[3];
+    coords_of<ND>(node, gf, cf);
+    const int f[3] = {f0, f1, f2};
+    int64_t p[3][2];
+    float w[3][2];
+    int np[3];
+    for (int d = 0; d < 3; ++d) {
+        if (f[d] == 2 && (cf[d] & 1)) {
+            p[d][0] = (cf[d] - 1) / 2;
+            p[d][1] = p[d][0] + 1;
+            w[d][0] = w[d][1] = 0.5f;
+            np[d] = 2;
+        } else {
+            p[d][0] = p[d][1] = f[d] == 2 ? cf[d] / 2 : cf[d];
+            w[d][0] = w[d][1] = 1.0f;
+            np[d] = 1;
+        }
+    }
+    float acc[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
+    for (int kz = 0; kz < np[2]; ++kz)
+        for (int ky = 0; ky < np[1]; ++ky)
+            for (int kx = 0; kx < np[0]; ++kx) {
+                const int64_t I = node_at(gc, p[0][kx], p[1][ky], p[2][kz]);
+                const float ww = w[0][kx] * w[1][ky] * w[2][kz];
+#pragma unroll
+                for (int j = 0; j < BS; ++j) acc[j] = fmaf(ww, xc[I * BS + j], acc[j]);
+            }
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        const int64_t d = row * BS + j;
+        const float ev = ffix[d] ? 0.0f : acc[j];
+        e[d] = ev;
+        xf[d] += ev;
+    }
+}
+
+// coarsest level: x = A^-1 b with the dense FP64 inverse (one block)
+__global__ __launch_bounds__(kThreads) void dense_apply_kernel(const double* __restrict__ inv, int64_t nc,
+                                                               const float* __restrict__ b, float* __restrict__ x,
+                                                               const int* done) {
+    if (*done) return;
+    for (int64_t i = threadIdx.x; i < nc; i += kThreads) {
+        double s = 0.0;
+        for (int64_t j = 0; j < nc; ++j) s = fma(inv[i * nc + j], (double)b[j], s);
+        x[i] = (float)s;
+    }
+}
+
+// ------------------------------------------- power iteration (setup) --
+struct PowState {
+    double lam;  // |B^-1 A v| for the last unit vector v
+    double inv;  // 1 / |w| of the last step (scale of the next input)
+    unsigned counter;
+    unsigned pad;
+};
+
+__device__ __forceinline__ double sum_parts(const double* partials, int nparts, double* sh) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < nparts; i += kThreads) s += ld_coherent(partials + i);
+    return block_sum_256(s, sh);
+}
+
+__device__ __forceinline__ void pow_finish(PowState* ps, const double* partials, double* sh) {
+    const double s = sum_parts(partials, gridDim.x, sh);
+    if (threadIdx.x == 0) {
+        const double nrm = sqrt(s);
+        ps->lam = nrm;
+        ps->inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
+        ps->counter = 0;
+    }
+}
+
+__device__ __forceinline__ float hash_unit(int64_t i) {
+    uint32_t h = (uint32_t)i * 2654435761u ^ (uint32_t)(i >> 32);
+    h ^= h >> 15;
+    h *= 0x2c1b3c6dU;
+    h ^= h >> 12;
+    h *= 0x297a2d39U;
+    h ^= h >> 15;
+    return (float)(h & 0xFFFF) * (1.0f / 32768.0f) - 1.0f;
+}
+
+// start vector: fixed pseudo-random values, 0 on fixed dofs
+__global__ __launch_bounds__(kThreads) void pow_init_kernel(int64_t n, const uint8_t* __restrict__ fixed,
+                                                            float* __restrict__ v, PowState* ps,
+                                                            double* __restrict__ partials) {
+    __shared__ double sh[8];
+    double s = 0.0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const float x = fixed[i] ? 0.0f : hash_unit(i);
+        v[i] = x;
+        s += (double)x * (double)x;
+    }
+    const double b = block_sum_256(s, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+}
+
+template <int BS>
+__device__ __forceinline__ double pow_epilogue(int64_t row, int64_t nrows, const float (&av)[BS], float scale,
+                                               const float* __restrict__ binv, float* __restrict__ w) {
+    double s = 0.0;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float z = 0.0f;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) z = fmaf(binv[(int64_t)(j * BS + i) * nrows + row], av[i], z);
+        z *= scale;
+        w[row * BS + j] = z;
+        s += (double)z * (double)z;
+    }
+    return s;
+}
+
+// w = B^-1 A (v / |v|), partial |w|^2
+template <int BS>
+__global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp op, const float* __restrict__ v,
+                                                            const float* __restrict__ binv, float* __restrict__ w,
+                                                            PowState* ps, double* __restrict__ partials) {
+    __shared__ double sh[8];
+    const float scale = (float)ps->inv;
+    const int64_t s0 = logical_block() * op.per_block;
+    const int64_t s1 = min(op.nslices, s0 + op.per_block);
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto gx = [=](int64_t c) { return v[c]; };
+    double s = 0.0;
+    for (int64_t sl = s0 + wv; sl < s1; sl += kThreads / 64) {
+        float out[BS];
+        sell_row<BS, float, float, false>(op.sl_ptr, op.sl_w, op.scol, Vals<float>{op.val}, sl, lane, gx, out);
+        const int64_t row = sl * 64 + lane;
+        if (row < op.nrows) s += pow_epilogue<BS>(row, op.nrows, out, scale, binv, w);
+    }
+    const double b = block_sum_256(s, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+}
+
+template <int ND>
+__global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp op, const float* __restrict__ v,
+                                                               const float* __restrict__ binv, float* __restrict__ w,
+                                                               PowState* ps, double* __restrict__ partials) {
+    __shared__ double sh[8];
+    const float scale = (float)ps->inv;
+    double s = 0.0;
+    for (int64_t node = (int64_t)blockIdx.x * kThreads + threadIdx.x; node < op.nn;
+         node += (int64_t)gridDim.x * kThreads) {
+        float out[ND];
+        stencil_row<ND>(op, node, v, out);
+        s += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
+    }
+    const double b = block_sum_256(s, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+}
+
+// ------------------------------------------------------- outer CG ----
+struct MgState {
+    double bb, tol2bb;
+    double rz, rz_old, alpha, beta;
+    double rr, rr_true, best_true, floor_rr;
+    long long iters, trues;
+    int done;       // 1 converged, 2 breakdown, 3 stagnation
+    int restart;    // next direction is z (after a residual replacement)
+    int need_true;  // the recurrence claims convergence: check the FP64 residual
+    int stall;
+    unsigned counter;
+    unsigned pad;
+};
+
+__global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restrict__ rhs, int64_t n,
+                                                           int64_t per_block, MgState* st,
+                                                           double* __restrict__ partials, double rtol) {
+    __shared__ double sh[8];
+    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(rhs[i], rhs[i], s);
+    const double b = block_sum_256(s, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&st->counter)) {
+        const double bb = sum_parts(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->bb = bb;
+            st->tol2bb = rtol * rtol * bb;
+            if (bb == 0.0) {
+                st->done = 1;
+                st->rr_true = 0.0;
+            }
+            st->counter = 0;
+        }
+    }
+}
+
+// r = rhs - A x in FP64 when a check is due (else exits at once); decides
+// convergence, or replaces the residual and restarts the directions.
+template <int BS>
+__global__ __launch_bounds__(kThreads) void mg_true_kernel(const int64_t* __restrict__ sl_ptr,
+                                                           const int32_t* __restrict__ sl_w,
+                                                           const int32_t* __restrict__ scol,
+                                                           const double* __restrict__ sval, int64_t nslices,
+                                                           int64_t nrows, int64_t per_block,
+                                                           const double* __restrict__ x,
+                                                           const double* __restrict__ rhs, double* __restrict__ r,
+                                                           MgState* st, double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done || !st->need_true) return;
+    const int64_t s0 = logical_block() * per_block;
+    const int64_t s1 = min(nslices, s0 + per_block);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    auto gx = [=](int64_t c) { return x[c]; };
+    double rr = 0.0, ff = 0.0;
+    for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
+        double out[BS], aout[BS];
+        sell_row_abs<BS>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out, aout);
+        const int64_t row = sl * 64 + lane;
+        if (row < nrows) {
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const int64_t d = row * BS + j;
+                const double res = rhs[d] - out[j];
+                r[d] = res;
+                rr = fma(res, res, rr);
+                const double sc = fabs(rhs[d]) + aout[j];
+                ff = fma(sc, sc, ff);
+            }
+        }
+    }
+    const double b0 = block_sum_256(rr, sh);
+    const double b1 = block_sum_256(ff, sh + 4);
+    if (threadIdx.x == 0) {
+        st_coherent(partials + 2 * blockIdx.x, b0);
+        st_coherent(partials + 2 * blockIdx.x + 1, b1);
+    }
+    if (last_block_arrive(&st->counter)) {
+        double s0v = 0.0, s1v = 0.0;
+        for (int i = threadIdx.x; i < (int)gridDim.x; i += kThreads) {
+            s0v += ld_coherent(partials + 2 * i);
+            s1v += ld_coherent(partials + 2 * i + 1);
+        }
+        const double trr = block_sum_256(s0v, sh);
+        const double tff = block_sum_256(s1v, sh + 4);
+        if (threadIdx.x == 0) {
+            st->rr_true = trr;
+            st->rr = trr;
+            st->floor_rr = 16.0 * kEps64 * kEps64 * tff;
+            st->trues += 1;
+            st->need_true = 0;
+            if (trr <= st->tol2bb) {
+                st->done = 1;
+            } else if (!(trr == trr) || trr > 1e300) {
+                st->done = 2;
+            } else {
+                st->restart = 1;
+                if (trr <= 0.81 * st->best_true) {
+                    st->best_true = trr;
+                    st->stall = 0;
+                } else {
+                    // no 10% gain since the best check: at the rounding floor,
+                    // or for the third time in a row, the tolerance is out of reach
+                    st->stall += 1;
+                    if (trr <= st->floor_rr || st->stall >= 3) st->done = 3;
+                }
+            }
+            st->counter = 0;
+        }
+    }
+}
+
+// r.z -> beta (0 after a restart)
+__global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restrict__ r, const float* __restrict__ z,
+                                                         int64_t n, int64_t per_block, MgState* st,
+                                                         double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(r[i], (double)z[i], s);
+    const double b = block_sum_256(s, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&st->counter)) {
+        const double rz = sum_parts(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            if (!(rz > 0.0) || !isfinite(rz)) {
+                st->done = 2;
+            } else {
+                st->beta = st->restart ? 0.0 : rz / st->rz_old;
+                st->rz_old = rz;
+            }
+            st->rz = rz;
+            st->restart = 0;
+            st->counter = 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(kThreads) void mg_p_kernel(double* __restrict__ p, const float* __restrict__ z, int64_t n,
+                                                        int64_t per_block, const MgState* st) {
+    if (st->done) return;
+    const double beta = st->beta;
+    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) p[i] = fma(beta, p[i], (double)z[i]);
+}
+
+// q = A p (operator values TV, FP64 vectors), p.q -> alpha
+template <int BS, typename TV, bool NT>
+__global__ __launch_bounds__(kThreads) void mg_spmv_kernel(const int64_t* __restrict__ sl_ptr,
+                                                           const int32_t* __restrict__ sl_w,
+                                                           const int32_t* __restrict__ scol, Vals<TV> sval,
+                                                           int64_t nslices, int64_t nrows, int64_t per_block,
+                                                           const double* __restrict__ p, double* __restrict__ q,
+                                                           MgState* st, double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const int64_t s0 = logical_block() * per_block;
+    const int64_t s1 = min(nslices, s0 + per_block);
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    double dot = 0.0;
+    auto gx = [=](int64_t c) { return p[c]; };
+    for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
+        double out[BS];
+        sell_row<BS, TV, double, NT>(sl_ptr, sl_w, scol, sval, sl, lane, gx, out);
+        const int64_t row = sl * 64 + lane;
+        if (row < nrows) {
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const int64_t d = row * BS + j;
+                q[d] = out[j];
+                dot = fma(p[d], out[j], dot);
+            }
+        }
+    }
+    const double b = block_sum_256(dot, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&st->counter)) {
+        const double pq = sum_parts(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->alpha = st->rz_old / pq;
+            if (!(pq > 0.0) || !isfinite(st->alpha)) st->done = 2;
+            st->counter = 0;
+        }
+    }
+}
+
+// x += alpha p, r -= alpha q, r.r; a recurrence at the tolerance asks for
+// the FP64 check
+__global__ __launch_bounds__(kThreads) void mg_update_kernel(double* __restrict__ x, double* __restrict__ r,
+                                                             const double* __restrict__ p,
+                                                             const double* __restrict__ q, int64_t n,
+                                                             int64_t per_block, MgState* st,
+                                                             double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const double a = st->alpha;
+    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+    double rr = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
+        x[i] = fma(a, p[i], x[i]);
+        const double ri = fma(-a, q[i], r[i]);
+        r[i] = ri;
+        rr = fma(ri, ri, rr);
+    }
+    const double b = block_sum_256(rr, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&st->counter)) {
+        const double trr = sum_parts(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->rr = trr;
+            st->iters += 1;
+            if (trr <= st->tol2bb) st->need_true = 1;
+            if (!isfinite(trr)) st->done = 2;
+            st->counter = 0;
+        }
+    }
+}
+
+// ---------------------------------------------------------- host ----
+struct Launch {
+    unsigned grid;
+    int64_t per_block;
+};
+
+Launch plan_vec(int64_t items, unsigned cap) {
+    int64_t need = (items + kThreads - 1) / kThreads;
+    int64_t grid = std::min<int64_t>(need, cap);
+    grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+    int64_t per = (items + grid - 1) / grid;
+    per = (per + kThreads - 1) / kThreads * kThreads;
+    if (per == 0) per = kThreads;
+    return {(unsigned)grid, per};
+}
+
+// one slice per wavefront, four per block
+Launch plan_sell(int64_t nslices) {
+    const int64_t per = kThreads / 64;
+    int64_t grid = (nslices + per - 1) / per;
+    grid = std::max<int64_t>(8, (grid + 7) / 8 * 8);
+    return {(unsigned)grid, per};
+}
+
+unsigned rows_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+
+Grid grid_of(const MgLevel& l) { return Grid{l.n[0], l.n[1], l.n[2]}; }
+
+// Gauss-Jordan inverse with partial pivoting (FP64); false if singular.
+bool dense_inverse(std::vector<double>& a, int64_t n, std::vector<double>& inv) {
+    inv.assign((size_t)(n * n), 0.0);
+    for (int64_t i = 0; i < n; ++i) inv[(size_t)(i * n + i)] = 1.0;
+    for (int64_t c = 0; c < n; ++c) {
+        int64_t piv = c;
+        for (int64_t i = c + 1; i < n; ++i)
+            if (std::fabs(a[(size_t)(i * n + c)]) > std::fabs(a[(size_t)(piv * n + c)])) piv = i;
+        const double pv = a[(size_t)(piv * n + c)];
+        if (!(std::fabs(pv) > 0.0) || !std::isfinite(pv)) return false;
+        if (piv != c)
+            for (int64_t j = 0; j < n; ++j) {
+                std::swap(a[(size_t)(piv * n + j)], a[(size_t)(c * n + j)]);
+                std::swap(inv[(size_t)(piv * n + j)], inv[(size_t)(c * n + j)]);
+            }
+        const double rp = 1.0 / pv;
+        for (int64_t j = 0; j < n; ++j) {
+            a[(size_t)(c * n + j)] *= rp;
+            inv[(size_t)(c * n + j)] *= rp;
+        }
+        for (int64_t i = 0; i < n; ++i) {
+            if (i == c) continue;
+            const double m = a[(size_t)(i * n + c)];
+            if (m == 0.0) continue;
+            for (int64_t j = 0; j < n; ++j) {
+                a[(size_t)(i * n + j)] -= m * a[(size_t)(c * n + j)];
+                inv[(size_t)(i * n + j)] -= m * inv[(size_t)(c * n + j)];
+            }
+        }
+    }
+    return true;
+}
+
+void cheb_coefficients(MgLevel& l, int degree, double lo, double hi) {
+    const double a = lo * l.lmax, b = hi * l.lmax;
+    const double theta = 0.5 * (b + a), delta = 0.5 * (b - a), sigma = theta / delta;
+    double rho = 1.0 / sigma;
+    l.c1[0] = 0.0f;
+    l.c2[0] = (float)(1.0 / theta);
+    for (int k = 1; k < degree; ++k) {
+        const double rn = 1.0 / (2.0 * sigma - rho);
+        l.c1[k] = (float)(rn * rho);
+        l.c2[k] = (float)(2.0 * rn / delta);
+        rho = rn;
+    }
+}
+
+template <int ND>
+void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_opts& o) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+    hipStream_t s = ctx->stream;
+    const sfem_csr* K = S->K;
+    MgLevel& L0 = H.L[0];
+    const int64_t nn0 = L0.nn;
+    const Grid g0 = grid_of(L0);
+
+    // level-0 maps and masks
+    DevBuf<uint32_t> flag(ctx, (size_t)nn0), pos(ctx, (size_t)nn0), tot(ctx, 1);
+    hipLaunchKernelGGL(node_free_kernel, dim3(rows_grid(nn0)), dim3(kThreads), 0, s, S->free_of.get(), nn0, BS,
+                       flag.get());
+    SFEM_LAUNCH_CHECK();
+    exclusive_scan_u32_dev(ctx, flag.get(), pos.get(), nn0, tot.get());
+    uint32_t nrows = 0;
+    d2h(ctx, &nrows, tot.get(), sizeof(nrows));
+    if ((int64_t)nrows != S->nfb) fail(SFEM_ERR_STATE, "multigrid: block rows do not match the system");
+    L0.nrows = nrows;
+    H.row_of_node.reset(ctx, (size_t)nn0);
+    H.node_of_row.reset(ctx, (size_t)nrows);
+    L0.fixed.reset(ctx, (size_t)nrows * BS);
+    DevBuf<uint8_t> fixnode(ctx, (size_t)nn0 * BS);
+    hipLaunchKernelGGL(row_maps_kernel, dim3(rows_grid(nn0)), dim3(kThreads), 0, s, flag.get(), pos.get(),
+                       S->free_of.get(), nn0, BS, H.row_of_node.get(), H.node_of_row.get(), L0.fixed.get(),
+                       fixnode.get());
+    SFEM_LAUNCH_CHECK();
+
+    // fine stencil (FP64, fixed rows and columns zero)
+    DevBuf<double> stf(ctx, (size_t)nn0 * NS * B2);
+    memset_d(ctx, stf.get(), 0, stf.bytes());
+    DevBuf<int> bad(ctx, 1);
+    memset_d(ctx, bad.get(), 0, sizeof(int));
+    hipLaunchKernelGGL(stencil_from_k_kernel<ND>, dim3(rows_grid(nn0)), dim3(kThreads), 0, s, K->bcolptr.get(),
+                       K->browidx.get(), K->bval.get(), nn0, K->transposed ? 1 : 0, S->free_of.get(), g0, stf.get(),
+                       bad.get());
+    SFEM_LAUNCH_CHECK();
+    int hbad = 0;
+    d2h(ctx, &hbad, bad.get(), sizeof(int));
+    if (hbad) fail(SFEM_ERR_ARGUMENT, "multigrid: K couples nodes that are not neighbours on the given grid");
+
+    // coarse levels
+    const double* prev_st = stf.get();
+    const uint8_t* prev_fix = fixnode.get();
+    for (;;) {
+        const MgLevel& P = H.L.back();
+        int f[3] = {1, 1, 1};
+        bool any = false;
+        for (int d = 0; d < ND; ++d) {
+            const int64_t e = P.n[d] - 1;
+            if (e >= 2 && e % 2 == 0) {
+                f[d] = 2;
+                any = true;
+            }
+        }
+        if (!any) break;
+        if (H.L.size() > 1 && P.nn <= o.coarse_nodes) break;
+        MgLevel C;
+        for (int d = 0; d < 3; ++d) {
+            C.f[d] = f[d];
+            C.n[d] = d < ND ? (P.n[d] - 1) / f[d] + 1 : 1;
+        }
+        C.nn = C.n[0] * C.n[1] * C.n[2];
+        C.nrows = C.nn;
+        const Grid gf = grid_of(P), gc = grid_of(C);
+        C.fixed.reset(ctx, (size_t)C.nn * BS);
+        hipLaunchKernelGGL(coarse_fixed_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, prev_fix, gf, gc,
+                           f[0], f[1], f[2], C.fixed.get());
+        SFEM_LAUNCH_CHECK();
+        DevBuf<double> raw(ctx, (size_t)C.nn * NS * B2);
+        hipLaunchKernelGGL(galerkin_kernel<ND>, dim3(rows_grid(C.nn * NS)), dim3(kThreads), 0, s, prev_st, gf, gc,
+                           f[0], f[1], f[2], raw.get());
+        SFEM_LAUNCH_CHECK();
+        C.st64.reset(ctx, (size_t)C.nn * NS * B2);
+        C.st32.reset(ctx, (size_t)C.nn * NS * B2);
+        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn * NS)), dim3(kThreads), 0, s, raw.get(), gc,
+                           C.fixed.get(), C.st64.get(), C.st32.get());
+        SFEM_LAUNCH_CHECK();
+        C.binv.reset(ctx, (size_t)C.nn * B2);
+        hipLaunchKernelGGL(stencil_binv_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, C.st64.get(),
+                           C.fixed.get(), C.nn, C.binv.get());
+        SFEM_LAUNCH_CHECK();
+        H.L.push_back(std::move(C));
+        prev_st = H.L.back().st64.get();
+        prev_fix = H.L.back().fixed.get();
+        if ((int)H.L.size() >= SFEM_MG_MAX_LEVELS) break;
+    }
+    sync(ctx);
+    stf.release();
+    if (H.L.size() < 2)
+        fail(SFEM_ERR_UNSUPPORTED, "multigrid: no direction has an even element count to coarsen");
+    MgLevel& Lc = H.L.back();
+    if (Lc.nn > kMgMaxCoarseNodes)
+        fail(SFEM_ERR_UNSUPPORTED, "multigrid: the grid coarsens only to " + std::to_string(Lc.nn) +
+                                       " nodes (at most " + std::to_string(kMgMaxCoarseNodes) +
+                                       "); element counts need more factors of 2");
+
+    // coarsest: dense inverse on the host
+    {
+        const int64_t nn = Lc.nn, nc = nn * BS;
+        std::vector<double> st((size_t)(nn * NS * B2));
+        std::vector<uint8_t> fx((size_t)nc);
+        d2h(ctx, st.data(), Lc.st64.get(), st.size() * sizeof(double));
+        d2h(ctx, fx.data(), Lc.fixed.get(), fx.size());
+        std::vector<double> a((size_t)(nc * nc), 0.0), inv;
+        const Grid gc = grid_of(Lc);
+        for (int64_t I = 0; I < nn; ++I) {
+            int64_t c[3];
+            coords_of<ND>(I, gc, c);
+            for (int sl = 0; sl < NS; ++sl) {
+                int dx, dy, dz;
+                offset_of<ND>(sl, dx, dy, dz);
+                if (!inside<ND>(gc, c[0] + dx, c[1] + dy, c[2] + dz)) continue;
+                const int64_t J = node_at(gc, c[0] + dx, c[1] + dy, c[2] + dz);
+                for (int p = 0; p < BS; ++p)
+                    for (int q = 0; q < BS; ++q)
+                        a[(size_t)((I * BS + p) * nc + J * BS + q)] = st[(size_t)(((int64_t)sl * B2 + p * BS + q) * nn + I)];
+            }
+        }
+        for (int64_t d = 0; d < nc; ++d)
+            if (fx[(size_t)d]) a[(size_t)(d * nc + d)] = 1.0;
+        if (!dense_inverse(a, nc, inv)) fail(SFEM_ERR_SINGULAR, "multigrid: the coarsest operator is singular");
+        H.nc = nc;
+        H.cinv.reset(ctx, (size_t)(nc * nc));
+        h2d(ctx, H.cinv.get(), inv.data(), inv.size() * sizeof(double));
+    }
+
+    // work vectors
+    for (size_t l = 0; l < H.L.size(); ++l) {
+        MgLevel& v = H.L[l];
+        const size_t n = (size_t)v.nrows * BS;
+        v.r.reset(ctx, n);
+        v.x.reset(ctx, n);
+        v.da.reset(ctx, n);
+        v.db.reset(ctx, n);
+        if (l > 0) v.b.reset(ctx, n);
+    }
+
+    // lambda_max(B^-1 A) per level (the coarsest is solved, not smoothed)
+    const int nl = (int)H.L.size();
+    DevBuf<PowState> ps(ctx, (size_t)nl);
+    memset_d(ctx, ps.get(), 0, sizeof(PowState) * nl);
+    const Launch ls = plan_sell(S->nslices);
+    auto init_grid = [&](int l) { return std::min<unsigned>(rows_grid(H.L[(size_t)l].nrows * BS), 1024); };
+    unsigned maxgrid = ls.grid;
+    for (int l = 0; l < nl; ++l) maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn), init_grid(l)});
+    DevBuf<double> partials(ctx, (size_t)maxgrid);
+    for (int l = 0; l + 1 < nl; ++l) {
+        MgLevel& v = H.L[l];
+        const int64_t n = v.nrows * BS;
+        const unsigned gi = init_grid(l);
+        hipLaunchKernelGGL(pow_init_kernel, dim3(gi), dim3(kThreads), 0, s, n, v.fixed.get(), v.da.get(),
+                           ps.get() + l, partials.get());
+        SFEM_LAUNCH_CHECK();
+        float* vin = v.da.get();
+        float* vout = v.db.get();
+        for (int k = 0; k < o.power_iters; ++k) {
+            if (l == 0) {
+                const SellOp op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
+                                ls.per_block};
+                hipLaunchKernelGGL(pow_sell_kernel<BS>, dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
+                                   S->binv32.get(), vout, ps.get(), partials.get());
+            } else {
+                const StOp op{v.st32.get(), grid_of(v), v.nn};
+                hipLaunchKernelGGL(pow_stencil_kernel<ND>, dim3(rows_grid(v.nn)), dim3(kThreads), 0, s, op, vin,
+                                   v.binv.get(), vout, ps.get() + l, partials.get());
+            }
+            SFEM_LAUNCH_CHECK();
+            std::swap(vin, vout);
+        }
+    }
+    std::vector<PowState> hps((size_t)nl);
+    d2h(ctx, hps.data(), ps.get(), sizeof(PowState) * nl);
+    for (int l = 0; l + 1 < nl; ++l) {
+        H.L[l].lmax = hps[(size_t)l].lam;
+        if (!(H.L[l].lmax > 0.0) || !std::isfinite(H.L[l].lmax))
+            fail(SFEM_ERR_SINGULAR, "multigrid: no spectrum estimate on level " + std::to_string(l));
+        cheb_coefficients(H.L[l], H.degree, o.lo, o.hi);
+    }
+}
+
+// One preconditioned CG solve; TV: outer operator values.
+template <int ND, typename TV>
+struct MgRun {
+    static constexpr int BS = ND;
+    sfem_ctx* ctx;
+    sfem_system* S;
+    MgHierarchy& H;
+    hipStream_t s;
+    Launch ls;
+    const int* done;
+    bool nt;
+
+    void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2) {
+        MgLevel& L = H.L[(size_t)l];
+        if (l == 0) {
+            const SellOp op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
+                            ls.per_block};
+            auto go = [&](auto k) {
+                hipLaunchKernelGGL(k, dim3(ls.grid), dim3(kThreads), 0, s, op, v, L.r.get(), L.x.get(), d_in, d_out,
+                                   c1, c2, (const float*)S->binv32.get(), done);
+            };
+            if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true>) : go(cheb_sell_kernel<BS, 0, false>);
+            else nt ? go(cheb_sell_kernel<BS, 1, true>) : go(cheb_sell_kernel<BS, 1, false>);
+        } else {
+            const StOp op{L.st32.get(), grid_of(L), L.nn};
+            auto go = [&](auto k) {
+                hipLaunchKernelGGL(k, dim3(rows_grid(L.nn)), dim3(kThreads), 0, s, op, v, L.r.get(), L.x.get(), d_in,
+                                   d_out, c1, c2, (const float*)L.binv.get(), done);
+            };
+            if (mode == 0) go(cheb_stencil_kernel<ND, 0>);
+            else go(cheb_stencil_kernel<ND, 1>);
+        }
+        SFEM_LAUNCH_CHECK();
+    }
+
+    // x_l = V-cycle(b_l); level 0's right-hand side is the outer residual
+    void vcycle(int l, const double* r_outer) {
+        MgLevel& L = H.L[(size_t)l];
+        const int nl = (int)H.L.size();
+        if (l == nl - 1) {
+            hipLaunchKernelGGL(dense_apply_kernel, dim3(1), dim3(kThreads), 0, s, H.cinv.get(), H.nc, L.b.get(),
+                               L.x.get(), done);
+            SFEM_LAUNCH_CHECK();
+            return;
+        }
+        const float* binv = l == 0 ? S->binv32.get() : L.binv.get();
+        if (l == 0)
+            hipLaunchKernelGGL((cheb_first_kernel<BS, double>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, r_outer,
+                               L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(), done);
+        else
+            hipLaunchKernelGGL((cheb_first_kernel<BS, float>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s,
+                               (const float*)L.b.get(), L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(),
+                               done);
+        SFEM_LAUNCH_CHECK();
+        float* dp = L.da.get();
+        float* dn = L.db.get();
+        for (int k = 1; k < H.degree; ++k) {
+            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
+            std::swap(dp, dn);
+        }
+        step(l, 1, dp, nullptr, nullptr, 0.0f, 0.0f);
+        MgLevel& C = H.L[(size_t)l + 1];
+        const Grid gf = grid_of(L), gc = grid_of(C);
+        hipLaunchKernelGGL(restrict_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, (const float*)L.r.get(),
+                           gf, l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
+                           (const uint8_t*)C.fixed.get(), C.b.get(), done);
+        SFEM_LAUNCH_CHECK();
+        vcycle(l + 1, nullptr);
+        hipLaunchKernelGGL(prolong_kernel<ND>, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s,
+                           (const float*)C.x.get(), gc, gf, C.f[0], C.f[1], C.f[2],
+                           l == 0 ? (const int32_t*)H.node_of_row.get() : nullptr, L.nrows,
+                           (const uint8_t*)L.fixed.get(), L.x.get(), L.da.get(), done);
+        SFEM_LAUNCH_CHECK();
+        step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0]);
+        dp = L.db.get();
+        dn = L.da.get();
+        for (int k = 1; k < H.degree; ++k) {
+            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
+            std::swap(dp, dn);
+        }
+    }
+};
+
+template <typename TV>
+Vals<TV> outer_vals(const sfem_system* S) {
+    if constexpr (std::is_same<TV, SplitVal>::value) return {S->sval32.get(), S->sval16.get()};
+    else if constexpr (std::is_same<TV, float>::value) return {S->sval32.get()};
+    else return {S->sval64.get()};
+}
+
+template <int ND, typename TV>
+void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts* o, sfem_solve_stats* stats) {
+    constexpr int BS = ND;
+    hipStream_t s = ctx->stream;
+    const bool padded = S->partial;
+    const int64_t n = S->nfb * BS;
+    double* x = padded ? S->px.get() : S->x.get();
+    const double* rhs = padded ? S->prhs.get() : S->rhs.get();
+    const Launch ls = plan_sell(S->nslices);
+    const Launch lv = plan_vec(n, (unsigned)std::max(8, ctx->n_cu * 4));
+    DevBuf<double> r(ctx, (size_t)n), p(ctx, (size_t)n), q(ctx, (size_t)n);
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max(ls.grid, lv.grid));
+    DevBuf<MgState> dst(ctx, 1);
+    if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
+    if (padded) pad_gather(ctx, S, S->x.get(), x);
+    memset_d(ctx, p.get(), 0, sizeof(double) * n);
+    MgState init{};
+    init.best_true = HUGE_VAL;
+    init.need_true = 1;
+    init.restart = 1;
+    h2d(ctx, dst.get(), &init, sizeof(init));
+    MgState* st = dst.get();
+    MgRun<ND, TV> run{ctx, S, H, s, ls, &st->done, ctx->tune_nt};
+    const float* z = H.L[0].x.get();
+    Vals<TV> vals = outer_vals<TV>(S);
+
+    hipEvent_t t0, t1;
+    SFEM_HIP(hipEventCreate(&t0));
+    SFEM_HIP(hipEventCreate(&t1));
+    SFEM_HIP(hipEventRecord(t0, s));
+    hipLaunchKernelGGL(mg_norm_kernel, dim3(lv.grid), dim3(kThreads), 0, s, rhs, n, lv.per_block, st, partials.get(),
+                       o->rtol);
+    SFEM_LAUNCH_CHECK();
+    auto true_resid = [&] {
+        hipLaunchKernelGGL(mg_true_kernel<BS>, dim3(ls.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
+                           S->scol.get(), S->sval64.get(), S->nslices, S->nfb, ls.per_block, (const double*)x, rhs,
+                           r.get(), st, partials.get());
+        SFEM_LAUNCH_CHECK();
+    };
+    MgState host{};
+    int64_t enq = 0;
+    const int chunk_max = std::max(1, std::min(o->check_every, 8));
+    d2h(ctx, &host, st, sizeof(host));
+    while (!host.done && enq < o->max_iter) {
+        const int64_t chunk = std::min<int64_t>(chunk_max, o->max_iter - enq);
+        for (int64_t k = 0; k < chunk; ++k) {
+            true_resid();
+            run.vcycle(0, r.get());
+            hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
+                               lv.per_block, st, partials.get());
+            SFEM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(mg_p_kernel, dim3(lv.grid), dim3(kThreads), 0, s, p.get(), z, n, lv.per_block,
+                               (const MgState*)st);
+            SFEM_LAUNCH_CHECK();
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(ls.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
+                                   S->scol.get(), vals, S->nslices, S->nfb, ls.per_block, (const double*)p.get(),
+                                   q.get(), st, partials.get());
+            };
+            if (ctx->tune_nt) go(mg_spmv_kernel<BS, TV, true>);
+            else go(mg_spmv_kernel<BS, TV, false>);
+            SFEM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(mg_update_kernel, dim3(lv.grid), dim3(kThreads), 0, s, x, r.get(),
+                               (const double*)p.get(), (const double*)q.get(), n, lv.per_block, st, partials.get());
+            SFEM_LAUNCH_CHECK();
+        }
+        true_resid();  // the check a converged recurrence asked for
+        enq += chunk;
+        d2h(ctx, &host, st, sizeof(host));
+    }
+    if (!host.done) {  // iteration cap: the true residual of the final iterate
+        MgState h2 = host;
+        h2.need_true = 1;
+        h2d(ctx, st, &h2, sizeof(h2));
+        true_resid();
+        d2h(ctx, &host, st, sizeof(host));
+        if (host.done != 1) host.done = 0;  // reported as the iteration cap
+    }
+    if (padded) pad_scatter(ctx, S, x, S->x.get());
+    SFEM_HIP(hipEventRecord(t1, s));
+    sync(ctx);
+    float total = 0;
+    SFEM_HIP(hipEventElapsedTime(&total, t0, t1));
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+
+    stats->iterations = host.iters;
+    stats->fp64_updates = host.trues;
+    stats->spmv_launches = host.iters;
+    stats->spmv64_launches = host.trues;
+    stats->rel_residual = host.bb > 0 ? std::sqrt(host.rr_true / host.bb) : 0.0;
+    stats->solve_ms = total;
+    stats->spmv_ms = 0.0;
+    stats->spmv_timed = 0;
+    const double vb = std::is_same<TV, SplitVal>::value ? 6.0 : (double)sizeof(TV);
+    stats->spmv_bytes = (double)S->nnzb * (BS * BS * vb + 4.0) + (double)(S->nslices + 1) * 8.0 +
+                        (double)S->nslices * 4.0 + 2.0 * (double)n * 8.0;
+    stats->converged = host.done == 1 ? 1 : 0;
+    stats->format = SFEM_FMT_BSR;
+    stats->stop_reason = host.done == 0 ? 4 : host.done;
+    stats->floor_residual = host.bb > 0 ? std::sqrt(host.floor_rr / 16.0 / host.bb) : 0.0;
+}
+
+template <int ND>
+void mg_solve_nd(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st) {
+    MgHierarchy& H = *S->mg;
+    switch (o->precision) {
+        case SFEM_PREC_FP64: mg_pcg<ND, double>(ctx, S, H, o, st); break;
+        case SFEM_PREC_SPLIT: mg_pcg<ND, SplitVal>(ctx, S, H, o, st); break;
+        case SFEM_PREC_FP32_MATRIX: mg_pcg<ND, float>(ctx, S, H, o, st); break;
+        case SFEM_PREC_MIXED:
+            fail(SFEM_ERR_UNSUPPORTED, "multigrid: precision fp64, split or fp32_matrix (the V-cycle is FP32)");
+        default: fail(SFEM_ERR_ARGUMENT, "solve: unknown precision");
+    }
+}
+
+}  // namespace
+
+void mg_opts_default(sfem_mg_opts* o) {
+    o->degree = 2;
+    o->lo = 0.1;
+    o->hi = 1.1;
+    o->power_iters = 12;
+    o->coarse_nodes = 64;
+}
+
+void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
+    if (ndim != 2 && ndim != 3) fail(SFEM_ERR_DIMENSION, "set_grid: ndim must be 2 or 3");
+    if (!dims) fail(SFEM_ERR_ARGUMENT, "set_grid: dims is NULL");
+    sfem_mg_opts o;
+    mg_opts_default(&o);
+    if (opts) o = *opts;
+    if (o.degree < 1 || o.degree > kMgMaxDegree || !(o.lo > 0.0) || !(o.hi > o.lo) || o.power_iters < 1 ||
+        o.coarse_nodes < 1)
+        fail(SFEM_ERR_ARGUMENT, "set_grid: bad multigrid options");
+    if (!S->has_block || S->bs != ndim || !S->K || S->K->bs != ndim)
+        fail(SFEM_ERR_UNSUPPORTED, "multigrid: needs a node-block system with n_dof = ndim (vector elasticity)");
+    int64_t nn = 1;
+    for (int d = 0; d < ndim; ++d) {
+        if (dims[d] < 1) fail(SFEM_ERR_ARGUMENT, "set_grid: element counts must be >= 1");
+        nn *= dims[d] + 1;
+    }
+    if (nn != S->K->nunits)
+        fail(SFEM_ERR_ARGUMENT, "set_grid: the grid has " + std::to_string(nn) + " nodes, K has " +
+                                    std::to_string(S->K->nunits));
+    const auto t0 = std::chrono::steady_clock::now();
+    std::unique_ptr<MgHierarchy, MgDelete> H(new MgHierarchy());
+    H->nd = ndim;
+    H->bs = ndim;
+    H->degree = o.degree;
+    H->L.reserve(SFEM_MG_MAX_LEVELS);  // level references stay valid while levels are added
+    MgLevel L0;
+    for (int d = 0; d < 3; ++d) L0.n[d] = d < ndim ? dims[d] + 1 : 1;
+    L0.nn = nn;
+    H->L.push_back(std::move(L0));
+    S->mg.reset();
+    if (ndim == 3) build_levels<3>(ctx, S, *H, o);
+    else build_levels<2>(ctx, S, *H, o);
+    sync(ctx);
+    H->setup_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    S->grid_nd = ndim;
+    for (int d = 0; d < 3; ++d) S->grid_dims[d] = d < ndim ? dims[d] : 0;
+    S->mg_opts = o;
+    S->mg = std::move(H);
+}
+
+void mg_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st) {
+    if (!S->mg) fail(SFEM_ERR_STATE, "multigrid: call sfem_system_set_grid first");
+    if (S->mg->nd == 3) mg_solve_nd<3>(ctx, S, o, st);
+    else mg_solve_nd<2>(ctx, S, o, st);
+}
+
+void mg_info(const sfem_system* S, sfem_mg_info* info) {
+    *info = sfem_mg_info{};
+    if (!S->mg) fail(SFEM_ERR_STATE, "multigrid: no hierarchy (sfem_system_set_grid)");
+    const MgHierarchy& H = *S->mg;
+    info->levels = (int)H.L.size();
+    for (size_t l = 0; l < H.L.size(); ++l) {
+        info->nodes[l] = H.L[l].nn;
+        for (int d = 0; d < 3; ++d) info->dims[l][d] = H.L[l].n[d] - 1;
+        info->lambda_max[l] = H.L[l].lmax;
+    }
+    info->coarse_dofs = H.nc;
+    info->setup_ms = H.setup_ms;
+    info->degree = H.degree;
+}
+
+void mg_stencil(sfem_ctx* ctx, const sfem_system* S, int level, double* stencil, uint8_t* fixed) {
+    if (!S->mg) fail(SFEM_ERR_STATE, "multigrid: no hierarchy (sfem_system_set_grid)");
+    const MgHierarchy& H = *S->mg;
+    if (level < 1 || level >= (int)H.L.size()) fail(SFEM_ERR_ARGUMENT, "mg_stencil: level out of range");
+    const MgLevel& L = H.L[(size_t)level];
+    if (stencil) d2h(ctx, stencil, L.st64.get(), L.st64.bytes());
+    if (fixed) d2h(ctx, fixed, L.fixed.get(), L.fixed.bytes());
+}
+
+}  // namespace sfem

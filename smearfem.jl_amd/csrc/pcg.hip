@@ -807,7 +807,10 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
     }
     if (!(o->rtol >= 0.0) || o->max_iter < 0) fail(SFEM_ERR_ARGUMENT, "solve: bad rtol or max_iter");
     const int fmt = resolve_format(S, o->format);
-    switch (o->precision) {
+    if (o->preconditioner == SFEM_PC_MULTIGRID) {
+        if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "multigrid: needs the node-block format");
+        mg_solve(ctx, S, o, st);
+    } else switch (o->precision) {
         case SFEM_PREC_FP64: run_pcg<double, double>(ctx, S, o, st, fmt); break;
         case SFEM_PREC_MIXED:
             if (fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);

@@ -2,6 +2,8 @@
 // and the host launchers each translation unit provides.
 #pragma once
 
+#include <memory>
+
 #include "sfem_internal.hpp"
 
 // Mesh and material inputs, uploaded once.
@@ -114,6 +116,13 @@ struct CgState {
 };
 
 // K_ff and its solver formats, plus the data to recombine the solution.
+namespace sfem {
+struct MgHierarchy;  // mg.hip
+struct MgDelete {
+    void operator()(MgHierarchy* h) const;
+};
+}  // namespace sfem
+
 struct sfem_system {
     sfem_ctx* ctx = nullptr;
     const sfem_csr* K = nullptr;   // must outlive the system (reactions)
@@ -149,6 +158,13 @@ struct sfem_system {
     sfem::DevBuf<double> prhs, px, pdinv64;
     sfem::DevBuf<float> pdinv32;
     double t_eliminate_ms = 0;
+    // structured grid of the nodes (sfem_system_set_grid) and the multigrid
+    // hierarchy built on it; it depends on K and the fixed mask only, so new
+    // loads or prescribed values (system_set_bc) keep it
+    int grid_nd = 0;
+    int64_t grid_dims[3] = {0, 0, 0};
+    sfem_mg_opts mg_opts{};
+    std::unique_ptr<sfem::MgHierarchy, sfem::MgDelete> mg;
 };
 
 namespace sfem {
@@ -167,4 +183,10 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
 void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
                        double* bytes);
 void stream_copy_bench(sfem_ctx* ctx, int64_t bytes, int reps, double* avg_ms);
+// ---- mg.hip ----
+void mg_opts_default(sfem_mg_opts* o);
+void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts);
+void mg_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
+void mg_info(const sfem_system* S, sfem_mg_info* info);
+void mg_stencil(sfem_ctx* ctx, const sfem_system* S, int level, double* stencil, uint8_t* fixed);
 }  // namespace sfem

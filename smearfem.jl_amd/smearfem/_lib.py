@@ -51,6 +51,31 @@ class SolveStats(C.Structure):
     ]
 
 
+MG_MAX_LEVELS = 24
+
+
+class MgOpts(C.Structure):
+    _fields_ = [
+        ("degree", C.c_int),
+        ("lo", C.c_double),
+        ("hi", C.c_double),
+        ("power_iters", C.c_int),
+        ("coarse_nodes", C.c_int64),
+    ]
+
+
+class MgInfo(C.Structure):
+    _fields_ = [
+        ("levels", C.c_int),
+        ("degree", C.c_int),
+        ("nodes", C.c_int64 * MG_MAX_LEVELS),
+        ("dims", (C.c_int64 * 3) * MG_MAX_LEVELS),
+        ("lambda_max", C.c_double * MG_MAX_LEVELS),
+        ("coarse_dofs", C.c_int64),
+        ("setup_ms", C.c_double),
+    ]
+
+
 class BatchProblemStats(C.Structure):
     _fields_ = [
         ("iterations", C.c_int64),
@@ -123,6 +148,10 @@ _PROTOS = [
     ("sfem_solve_opts_default", None, [C.POINTER(SolveOpts)]),
     ("sfem_system_solve", C.c_int, [c_vp, c_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("sfem_system_solution", C.c_int, [c_vp, c_vp, c_dblp, c_dblp]),
+    ("sfem_mg_opts_default", None, [C.POINTER(MgOpts)]),
+    ("sfem_system_set_grid", C.c_int, [c_vp, c_vp, C.c_int, c_i64p, C.POINTER(MgOpts)]),
+    ("sfem_system_mg_info", C.c_int, [c_vp, C.POINTER(MgInfo)]),
+    ("sfem_system_mg_stencil", C.c_int, [c_vp, c_vp, C.c_int, c_dblp, c_u8p]),
     ("sfem_solve", C.c_int,
      [c_vp, c_vp, c_dblp, c_u8p, c_dblp, C.POINTER(SolveOpts), c_dblp, c_dblp, C.POINTER(SolveStats)]),
     ("sfem_system_spmv_bench", C.c_int, [c_vp, c_vp, C.c_int, C.c_int, C.c_int, c_dblp, c_dblp]),

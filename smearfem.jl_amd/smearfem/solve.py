@@ -23,7 +23,7 @@ from . import _lib
 from .fem import Context, StiffnessMatrix, _ptr, default_context
 
 PRECISIONS = {"fp64": 0, "mixed": 1, "fp32_matrix": 2, "split": 3}
-PRECONDITIONERS = {"jacobi": 0, "block_jacobi": 1}
+PRECONDITIONERS = {"jacobi": 0, "block_jacobi": 1, "multigrid": 2}
 FORMATS = {"auto": 0, "csr": 1, "bsr": 2}
 FORMAT_NAMES = {v: k for k, v in FORMATS.items()}
 
@@ -116,6 +116,45 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
+    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=12, coarse_nodes=64) -> dict:
+        """Declare the nodes a structured grid with ``dims`` elements per
+        direction (meshgrid numbering, x fastest) and build the geometric
+        multigrid hierarchy (``sfem_system_set_grid``); then
+        ``solve(preconditioner="multigrid")``.  Returns ``mg_info()``."""
+        d = np.asarray(dims, dtype=np.int64).ravel()
+        o = _lib.MgOpts()
+        self.ctx._lib.sfem_mg_opts_default(C.byref(o))
+        o.degree, o.lo, o.hi = int(degree), float(lo), float(hi)
+        o.power_iters, o.coarse_nodes = int(power_iters), int(coarse_nodes)
+        self.ctx.check(self.ctx._lib.sfem_system_set_grid(self.ctx.handle, self.handle, int(d.size),
+                                                          _ptr(d, C.c_int64), C.byref(o)))
+        self.grid = tuple(int(v) for v in d)
+        return self.mg_info()
+
+    def mg_info(self) -> dict:
+        info = _lib.MgInfo()
+        self.ctx.check(self.ctx._lib.sfem_system_mg_info(self.handle, C.byref(info)))
+        nl = info.levels
+        nd = len(getattr(self, "grid", (0, 0, 0)))
+        return {"levels": nl, "degree": info.degree, "nodes": [info.nodes[i] for i in range(nl)],
+                "dims": [tuple(info.dims[i][d] for d in range(nd)) for i in range(nl)],
+                "lambda_max": [info.lambda_max[i] for i in range(nl)], "coarse_dofs": info.coarse_dofs,
+                "setup_ms": info.setup_ms}
+
+    def mg_stencil(self, level: int):
+        """Level >= 1 Galerkin operator as stored (FP64, fixed rows / columns
+        zero): array [slot, a, b, node] with slot = (dx+1) + 3(dy+1) [+ 9(dz+1)],
+        and the fixed-dof flags [node, a]."""
+        info = self.mg_info()
+        nn = info["nodes"][level]
+        bs = self.block
+        ns = 3 ** len(self.grid)
+        st = np.zeros(ns * bs * bs * nn)
+        fx = np.zeros(nn * bs, dtype=np.uint8)
+        self.ctx.check(self.ctx._lib.sfem_system_mg_stencil(self.ctx.handle, self.handle, int(level),
+                                                            _ptr(st, C.c_double), _ptr(fx, C.c_uint8)))
+        return st.reshape(ns, bs, bs, nn), fx.reshape(nn, bs)
+
     def solve_many(self, q_d=None, f=None, rtol=1e-10, max_iter=200000, check_every=32, raise_on_fail=True):
         """k right-hand sides at once (``sfem_system_solve_many``): column j of
         the n x k arrays q_d / f gives q[:, j] = q_d + C K_ff^-1 C'(f - K q_d).
@@ -170,14 +209,20 @@ class System:
 
 def solve_system(K: StiffnessMatrix, f=None, fixed=None, q_d=None, *, precision="fp64", fmt="auto",
                  rtol=1e-10, max_iter=200000, reliable_delta=0.1, reactions=False, return_stats=False,
-                 preconditioner="jacobi"):
-    """q = q_d + C K_ff^{-1} C'(f - K q_d) on the GPU for a device matrix K."""
+                 preconditioner="jacobi", grid=None):
+    """q = q_d + C K_ff^{-1} C'(f - K q_d) on the GPU for a device matrix K.
+    preconditioner="multigrid" needs ``grid``: elements per direction of the
+    structured mesh the nodes come from."""
+    if preconditioner == "multigrid" and grid is None:
+        raise ValueError("multigrid needs grid= (elements per direction)")
     bc = BoundaryConditions(K.ctx, K.shape[0], f, fixed, q_d)
     try:
         S = System(K, bc)
     finally:
         bc.close()
     try:
+        if preconditioner == "multigrid":
+            S.set_grid(grid)
         stats = S.solve(precision, fmt, rtol, max_iter, reliable_delta, preconditioner=preconditioner)
         out = S.solution(reactions=reactions)
     finally:

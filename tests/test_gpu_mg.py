@@ -1,0 +1,246 @@
+"""Geometric multigrid preconditioner (structured grids) against the oracle.
+
+Coarse operators: every level's stored operator equals the Galerkin product
+P'AP of the level above, with P the multilinear interpolation restricted to
+free dofs, built here independently with scipy from the oracle's K (1e-12
+relative, Frobenius); the coarse fixed flags are those of the coincident fine
+dofs.  Solutions: within 1e-8 relative of a direct FP64 solve of the oracle's
+K_ff (solved to relative residual 1e-12), in every supported precision, with
+prescribed values, partially fixed nodes and 2D Q4 meshes; the iteration
+count stays bounded as the mesh is refined; solves are bitwise reproducible.
+"""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+import scipy.sparse.linalg  # noqa: F401  (sp.linalg.norm)
+
+import oracle as O
+import smearfem
+from smearfem.mesh import clamp_x0, element_young_loguniform, random_face_load, structured_mesh
+from smearfem.solve import BoundaryConditions, System, solve_system
+
+pytestmark = pytest.mark.gpu
+ERR = 1e-8
+
+
+def relerr(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def device_K(ctx, m, young, nu=0.3):
+    return smearfem.assemble_system(0, m.NodeList, m.IEN, m.ndim, "Q1", m.ndim, m.ID, young, nu, ctx=ctx,
+                                    device=True, n_elem=m.n_elem)
+
+
+def oracle_K(m, young, nu=0.3):
+    return sp.csr_matrix(O.assemble_system(0, m.NodeList, m.IEN, m.ndim, "Q1", m.ndim, m.ID, young, nu,
+                                           n_elem=m.n_elem))
+
+
+def oracle_direct(K, fixed, f, q_d=None):
+    Kff, rhs = O.eliminate(K, fixed, f, q_d)
+    return O.recombine(fixed, q_d, O.dense_solve(Kff, rhs))
+
+
+def roller(m, axis=2):
+    """x = 0 clamped, u_axis fixed on the plane x_axis = 0 (partially fixed nodes)."""
+    fixed = clamp_x0(m)
+    gi = m.node_grid_index()
+    fixed[m.ndim * np.nonzero(gi[axis] == 0)[0] + axis] = 1
+    return fixed
+
+
+def interp_1d(e, f):
+    if f == 1:
+        return sp.identity(e + 1, format="csr")
+    rows, cols, vals = [], [], []
+    for i in range(e + 1):
+        if i % 2 == 0:
+            rows.append(i)
+            cols.append(i // 2)
+            vals.append(1.0)
+        else:
+            rows += [i, i]
+            cols += [(i - 1) // 2, (i + 1) // 2]
+            vals += [0.5, 0.5]
+    return sp.csr_matrix((vals, (rows, cols)), shape=(e + 1, e // 2 + 1))
+
+
+def stencil_to_csr(st, dims):
+    """[slot, a, b, node] stencil of a structured level -> sparse matrix."""
+    ns, bs, _, nn = st.shape
+    n = [d + 1 for d in dims]
+    nd = len(dims)
+    node = np.arange(nn)
+    c = [node % n[0], (node // n[0]) % n[1]] + ([node // (n[0] * n[1])] if nd == 3 else [])
+    rows, cols, vals = [], [], []
+    for s in range(ns):
+        off = [s % 3 - 1, (s // 3) % 3 - 1] + ([s // 9 - 1] if nd == 3 else [])
+        ok = np.ones(nn, bool)
+        for d in range(nd):
+            ok &= (c[d] + off[d] >= 0) & (c[d] + off[d] < n[d])
+        J = node + off[0] + n[0] * (off[1] + (n[1] * off[2] if nd == 3 else 0))
+        for a in range(bs):
+            for b in range(bs):
+                rows.append(node[ok] * bs + a)
+                cols.append(J[ok] * bs + b)
+                vals.append(st[s, a, b, ok])
+    return sp.csr_matrix((np.concatenate(vals), (np.concatenate(rows), np.concatenate(cols))), shape=(nn * bs,) * 2)
+
+
+def galerkin_chain(K, fixed, dims, levels):
+    """Expected level operators and fixed flags, level 1 .. levels-1."""
+    bs = len(dims)
+    free = ~fixed.astype(bool)
+    A = sp.diags(free.astype(float)) @ K @ sp.diags(free.astype(float))
+    fx = fixed.astype(bool)
+    cur = list(dims)
+    out = []
+    for _ in range(1, levels):
+        f = [2 if (e >= 2 and e % 2 == 0) else 1 for e in cur]
+        P1 = [interp_1d(e, fi) for e, fi in zip(cur, f)]
+        Pn = P1[0]
+        for Pd in P1[1:]:
+            Pn = sp.kron(Pd, Pn)
+        P = sp.kron(Pn, sp.identity(bs)).tocsr()
+        new = [e // fi for e, fi in zip(cur, f)]
+        nc = [e + 1 for e in new]
+        idx = np.arange(int(np.prod(nc)))
+        cc = [idx % nc[0], (idx // nc[0]) % nc[1]] + ([idx // (nc[0] * nc[1])] if bs == 3 else [])
+        fn = cc[0] * f[0] + (cur[0] + 1) * (cc[1] * f[1] + ((cur[1] + 1) * cc[2] * f[2] if bs == 3 else 0))
+        cfx = fx.reshape(-1, bs)[fn].reshape(-1)
+        Pm = sp.diags((~fx).astype(float)) @ P @ sp.diags((~cfx).astype(float))
+        A = (Pm.T @ A @ Pm).tocsr()
+        out.append((A, cfx, tuple(new)))
+        fx, cur = cfx, new
+    return out
+
+
+@pytest.mark.parametrize("dims", [(8, 4, 4), (12, 6, 4), (16, 8)])
+def test_coarse_operators_are_galerkin_products(ctx, dims):
+    print(f"[case] mg galerkin {dims}", flush=True)
+    m = structured_mesh(dims)
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=3)
+    fixed = roller(m, axis=m.ndim - 1)
+    Kd = device_K(ctx, m, young)
+    bc = BoundaryConditions(ctx, m.n_dof, np.zeros(m.n_dof), fixed)
+    S = System(Kd, bc)
+    bc.close()
+    info = S.set_grid(dims, coarse_nodes=1)
+    assert info["levels"] >= 2 and info["coarse_dofs"] == info["nodes"][-1] * m.ndim
+    expect = galerkin_chain(oracle_K(m, young), fixed, dims, info["levels"])
+    for lvl, (A, cfx, new) in enumerate(expect, start=1):
+        assert info["dims"][lvl] == new
+        st, fx = S.mg_stencil(lvl)
+        assert np.array_equal(fx.reshape(-1).astype(bool), cfx)
+        G = stencil_to_csr(st, new)
+        assert abs(G - G.T).max() == 0.0  # symmetrised exactly
+        assert sp.linalg.norm(G - A) <= 1e-12 * sp.linalg.norm(A), lvl
+    S.close()
+    Kd.close()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "split", "fp32_matrix"])
+def test_mg_solve_vs_oracle(ctx, precision):
+    print(f"[case] mg solve 16x8x8 {precision}", flush=True)
+    m = structured_mesh((16, 8, 8), lengths=(2.0, 1.0, 1.0))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=5)
+    fixed = clamp_x0(m)
+    f = random_face_load(m, 2, 1, fixed, seed=5)
+    Kd = device_K(ctx, m, young)
+    q, st = solve_system(Kd, f, fixed, precision=precision, rtol=1e-12, return_stats=True,
+                                        preconditioner="multigrid", grid=(16, 8, 8))
+    Kd.close()
+    assert st["converged"] == 1 and st["stop_reason"] == 1 and st["rel_residual"] <= 1e-12
+    print(f"[mg] {precision}: {st['iterations']} iterations, {st['fp64_updates']} FP64 residuals", flush=True)
+    # an FP32 outer operator reaches 1e-12 through FP64 residual restarts
+    assert st["iterations"] <= (40 if precision != "fp32_matrix" else 150), st["iterations"]
+    assert relerr(q, oracle_direct(oracle_K(m, young), fixed, f)) <= ERR
+
+
+def test_mg_partial_nodes_prescribed_values_and_set_bc(ctx):
+    print("[case] mg roller + q_d + set_bc", flush=True)
+    m = structured_mesh((8, 4, 6), lengths=(2.0, 1.0, 1.5))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=7)
+    fixed = roller(m)
+    gi = m.node_grid_index()
+    top = np.nonzero(gi[2] == m.dims[2])[0]
+    fixed[3 * top + 2] = 1
+    q_d = np.zeros(m.n_dof)
+    q_d[3 * top + 2] = -0.02
+    Kd = device_K(ctx, m, young)
+    K = oracle_K(m, young)
+    bc = BoundaryConditions(ctx, m.n_dof, np.zeros(m.n_dof), fixed, q_d)
+    S = System(Kd, bc)
+    bc.close()
+    S.set_grid(m.dims)
+    st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+    q = S.solution()
+    assert st["stop_reason"] == 1 and st["iterations"] <= 40
+    assert relerr(q, oracle_direct(K, fixed, np.zeros(m.n_dof), q_d)) <= ERR
+    assert np.array_equal(q[fixed == 1], q_d[fixed == 1])
+    # new loads and values, same fixed set: the hierarchy is kept
+    f2 = random_face_load(m, 0, 1, fixed, seed=8)
+    bc2 = BoundaryConditions(ctx, m.n_dof, f2, fixed, 0.5 * q_d)
+    S.set_bc(bc2)
+    bc2.close()
+    st = S.solve("split", rtol=1e-12, preconditioner="multigrid")
+    q2 = S.solution()
+    assert st["stop_reason"] == 1
+    assert relerr(q2, oracle_direct(K, fixed, f2, 0.5 * q_d)) <= ERR
+    S.close()
+    Kd.close()
+
+
+def test_mg_q4_and_bounded_iterations(ctx):
+    print("[case] mg q4 + refinement", flush=True)
+    its = []
+    for ne in (16, 32, 64):
+        m = structured_mesh((ne, ne // 2))
+        young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=ne)
+        fixed = clamp_x0(m)
+        f = random_face_load(m, 1, 1, fixed, seed=ne)
+        Kd = device_K(ctx, m, young)
+        q, st = solve_system(Kd, f, fixed, rtol=1e-12, return_stats=True,
+                                            preconditioner="multigrid", grid=m.dims)
+        Kd.close()
+        its.append(st["iterations"])
+        if ne <= 32:
+            assert relerr(q, oracle_direct(oracle_K(m, young), fixed, f)) <= ERR
+    assert max(its) <= 40 and its[-1] <= its[0] + 6, its
+
+
+def test_mg_bitwise_reproducible_and_errors(ctx):
+    print("[case] mg reproducible + errors", flush=True)
+    m = structured_mesh((8, 4, 4))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=1)
+    fixed = clamp_x0(m)
+    f = random_face_load(m, 2, 1, fixed, seed=1)
+    Kd = device_K(ctx, m, young)
+    bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
+    S = System(Kd, bc)
+    bc.close()
+    with pytest.raises(smearfem.StateError):
+        S.solve("fp64", preconditioner="multigrid")
+    with pytest.raises(smearfem.ArgumentError):
+        S.set_grid((8, 4, 5))  # node count differs
+    with pytest.raises(smearfem.ArgumentError):
+        S.set_grid((4, 8, 4))  # same node count, K's couplings are not this grid's
+    S.set_grid((8, 4, 4))
+    with pytest.raises(smearfem.UnsupportedError):
+        S.solve("mixed", preconditioner="multigrid")
+    S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+    a = S.solution()
+    S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+    assert np.array_equal(a, S.solution())
+    S.close()
+    Kd.close()
+    m3 = structured_mesh((3, 3, 3))
+    K3 = device_K(ctx, m3, 1.0)
+    bc = BoundaryConditions(ctx, m3.n_dof, np.zeros(m3.n_dof), clamp_x0(m3))
+    S3 = System(K3, bc)
+    bc.close()
+    with pytest.raises(smearfem.UnsupportedError):
+        S3.set_grid((3, 3, 3))  # nothing to coarsen
+    S3.close()
+    K3.close()
