@@ -10,8 +10,10 @@ reduction) + Dirichlet elimination + PCG to relative residual 1e-10 from
 resident inputs.
 
 Solver paths timed on the same inputs: the plain FP64 path (scalar CSR), FP64
-on the node-block sliced format, FP32 operator values with FP64 vectors, and
-the FP32-CG / FP64-residual mixed form.  Each path's displacement is compared
+on the node-block sliced format, FP32 operator values with FP64 vectors, the
+FP32-CG / FP64-residual mixed form, node-block Jacobi preconditioning, and
+geometric multigrid preconditioning (/mg: the structured-grid hierarchy is
+built inside the timed step).  Each path's displacement is compared
 with an FP64 solve carried on to stagnation (relative residual ~1e-13).
 value = solved DOF/s (n_free / step time) of the fastest path whose relative
 error is <= 1e-8; every path is listed under "paths".
@@ -62,8 +64,9 @@ def make_inputs(dims=DIMS):
 
 
 def path_name(v):
-    """precision/format, with /bj for node-block Jacobi preconditioning."""
-    return f"{v[0]}/{v[1]}" + ("/bj" if v[2] == "block_jacobi" else "")
+    """precision/format, with /bj for node-block Jacobi and /mg for geometric
+    multigrid preconditioning."""
+    return f"{v[0]}/{v[1]}" + {"block_jacobi": "/bj", "multigrid": "/mg"}.get(v[2], "")
 
 
 def timed_step(ctx, mesh, bc, precision, fmt, pc, rtol, time_kernels):
@@ -73,10 +76,12 @@ def timed_step(ctx, mesh, bc, precision, fmt, pc, rtol, time_kernels):
     t0 = time.perf_counter()
     K = mesh.assemble()
     S = System(K, bc)
+    mg = S.set_grid(DIMS) if pc == "multigrid" else None
     st = S.solve(precision, fmt, rtol=rtol, time_kernels=time_kernels, raise_on_fail=False, preconditioner=pc)
     ctx.synchronize()
     dt = time.perf_counter() - t0
     st["assemble_stage_ms"] = K.stage_ms()
+    st["mg"] = mg
     return dt, st, K, S
 
 
@@ -160,10 +165,20 @@ def kernel_probe(ctx, mesh, bc, path):
     one 1 GiB stream copy (traffic calibration), 20 plain SpMVs, 50 CG iterations."""
     from smearfem.solve import System, stream_copy_bench
 
-    precision, fmt = path.split("/")
+    parts = path.split("/")
+    precision, fmt = parts[0], parts[1]
+    mg = len(parts) > 2 and parts[2] == "mg"
     stream_copy_bench(ctx, 1 << 30, 1)
     K = mesh.assemble()
     S = System(K, bc)
+    if mg:  # hierarchy, then 10 multigrid-PCG iterations (30 finest-level smoothing steps)
+        S.set_grid(DIMS)
+        st = S.solve(precision, fmt, rtol=1e-30, max_iter=10, raise_on_fail=False, preconditioner="multigrid")
+        ctx.synchronize()
+        log(f"probe {path}: 10 iterations {st['solve_ms']:.1f} ms, smoothing step bytes {st['smooth_bytes']:.4e}")
+        S.close()
+        K.close()
+        return
     ms, nbytes = S.spmv_bench(precision, fmt, 20)
     st = S.solve(precision, fmt, rtol=1e-10, max_iter=50, raise_on_fail=False)
     ctx.synchronize()
@@ -185,7 +200,8 @@ def run_headline(args, ctx, m, young, fixed, f):
     upload_s = time.perf_counter() - t0
     variants = [("fp64", "csr", "jacobi"), ("fp64", "bsr", "jacobi"), ("fp32_matrix", "bsr", "jacobi"),
                 ("mixed", "bsr", "jacobi"), ("split", "bsr", "jacobi"), ("fp64", "bsr", "block_jacobi"),
-                ("mixed", "bsr", "block_jacobi"), ("split", "bsr", "block_jacobi")]
+                ("mixed", "bsr", "block_jacobi"), ("split", "bsr", "block_jacobi"), ("fp64", "bsr", "multigrid"),
+                ("split", "bsr", "multigrid")]
     if args.only:
         variants = [v for v in variants if path_name(v) in args.only.split(",")]
     results = {}
@@ -193,13 +209,14 @@ def run_headline(args, ctx, m, young, fixed, f):
     st_ref = {"rel_residual": float("nan"), "iterations": 0}
     for vi, (precision, fmt, pc) in enumerate(variants):
         steps = []
-        nwarm = args.warmup if vi == 0 else 0
+        nwarm = args.warmup if (vi == 0 or pc == "multigrid") else 0
         for i in range(nwarm + args.steps):
             dt, st, K, S = timed_step(ctx, mesh, bc, precision, fmt, pc, RTOL, time_kernels=True)
             phase = "warmup" if i < nwarm else "timed"
             log(f"{path_name((precision, fmt, pc))} {phase} step {i}: {dt:.3f}s iters={st['iterations']} "
                 f"updates={st['fp64_updates']} relres={st['rel_residual']:.2e} solve={st['solve_ms']:.1f}ms "
-                f"asm={st['assemble_stage_ms']['total']:.1f}ms elim={S.eliminate_ms:.1f}ms")
+                f"asm={st['assemble_stage_ms']['total']:.1f}ms elim={S.eliminate_ms:.1f}ms"
+                + (f" mg_setup={st['mg']['setup_ms']:.1f}ms" if st.get("mg") else ""))
             if i >= nwarm:
                 steps.append((dt, st))
             if i == nwarm + args.steps - 1:
@@ -235,14 +252,21 @@ def run_headline(args, ctx, m, young, fixed, f):
         t = float(np.mean([d for d, _ in r["steps"]]))
         st = r["steps"][-1][1]
         ms = st["spmv_ms"] / max(1, st["spmv_timed"])
-        return {"value": r["n_free"] / t, "ms_per_step": t * 1e3, "iterations": int(st["iterations"]),
+        out = {"value": r["n_free"] / t, "ms_per_step": t * 1e3, "iterations": int(st["iterations"]),
                 "fp64_updates": int(st["fp64_updates"]), "rel_residual": float(st["rel_residual"]),
                 "rel_err_vs_fp64_ref": r["err"], "solve_ms": float(st["solve_ms"]),
                 "us_per_iteration": 1e3 * float(st["solve_ms"]) / max(1, int(st["iterations"])),
-                "spmv_cg_avg_ms": ms, "spmv_cg_GBs": st["spmv_bytes"] / (ms * 1e-3) / 1e9,
+                "spmv_cg_avg_ms": ms, "spmv_cg_GBs": st["spmv_bytes"] / (ms * 1e-3) / 1e9 if ms > 0 else None,
                 "spmv_cg_bytes": st["spmv_bytes"], "spmv_timed": int(st["spmv_timed"]),
                 "spmv_plain_GBs": r["spmv_plain_GBs"], "assemble_ms": st["assemble_stage_ms"],
                 "eliminate_ms": r["eliminate_ms"], "download_s": r["download_s"]}
+        if st.get("mg"):
+            sm = st["smooth_ms"] / max(1, st["smooth_timed"])
+            out.update({"mg_setup_ms": st["mg"]["setup_ms"], "mg_levels": st["mg"]["levels"],
+                        "mg_lambda_max": st["mg"]["lambda_max"][:-1], "smooth_avg_ms": sm,
+                        "smooth_GBs": st["smooth_bytes"] / (sm * 1e-3) / 1e9 if sm > 0 else None,
+                        "smooth_bytes": st["smooth_bytes"], "smooth_timed": int(st["smooth_timed"])})
+        return out
 
     paths = {path_name(k): summary(k) for k in results}
     info = {"n_free": results[next(iter(results))]["n_free"], "nnz": results[next(iter(results))]["nnz"],
@@ -437,7 +461,18 @@ def main():
     plain = paths.get("fp64/csr")
     dtype = {"fp64": "f64", "fp32_matrix": "f32 operator / f64 vectors", "mixed": "f32 CG / f64 residual",
              "split": "f64 vectors / f32+16-bit operator"}
-    mtraffic = MEASURED_TRAFFIC.get("/".join(best_name.split("/")[:2]))
+    is_mg = best_name.endswith("/mg")
+    if is_mg:
+        # dominant kernel of a multigrid iteration: the finest-level Chebyshev
+        # smoothing step (3 of the 5 finest-level operator passes)
+        rl = {"kernel": "cheb_sell_kernel (finest-level smoothing step, FP32 node-block operator)",
+              "achieved": best["smooth_GBs"], "bytes_per_launch": best["smooth_bytes"],
+              "avg_launch_ms": best["smooth_avg_ms"], "launches_timed": best["smooth_timed"],
+              "traffic": MEASURED_TRAFFIC.get("mg-smooth")}
+    else:
+        rl = {"kernel": f"spmv_cg {best_name}", "achieved": best["spmv_cg_GBs"],
+              "bytes_per_launch": best["spmv_cg_bytes"], "avg_launch_ms": best["spmv_cg_avg_ms"],
+              "launches_timed": best["spmv_timed"], "traffic": MEASURED_TRAFFIC.get("/".join(best_name.split("/")[:2]))}
     out = {
         "metric": METRIC,
         "value": best["value"],
@@ -449,7 +484,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": dtype[best_name.split("/")[0]],
+        "dtype": dtype[best_name.split("/")[0]] + (" (f32 multigrid V-cycle)" if is_mg else ""),
         "data": "synthetic",
         "config": {
             "workload": f"hex8-{DIMS[0]}x{DIMS[1]}x{DIMS[2]}-cantilever assemble+eliminate+PCG {best_name}",
@@ -459,15 +494,15 @@ def main():
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": f"spmv_cg {best_name}",
-            "achieved": best["spmv_cg_GBs"],
+            "kernel": rl["kernel"],
+            "achieved": rl["achieved"],
             "peak": PEAK_HBM_GBS,
             "unit": "GB/s",
-            "frac": best["spmv_cg_GBs"] / PEAK_HBM_GBS,
-            "traffic": mtraffic,
-            "bytes_per_launch": best["spmv_cg_bytes"],
-            "avg_launch_ms": best["spmv_cg_avg_ms"],
-            "launches_timed": best["spmv_timed"],
+            "frac": rl["achieved"] / PEAK_HBM_GBS,
+            "traffic": rl["traffic"],
+            "bytes_per_launch": rl["bytes_per_launch"],
+            "avg_launch_ms": rl["avg_launch_ms"],
+            "launches_timed": rl["launches_timed"],
             "stream_copy_GBs": copy_gbs,
         },
         "paths": paths,

@@ -147,9 +147,10 @@ int  sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fi
 void sfem_bc_destroy(sfem_bc* bc);
 
 /* Build K_ff = C' K C and rhs = C'(f - K q_d) on the device.  The solver
- * formats are built here: FP64 CSR always, and, when K is node-blocked, the
- * node-block formats (a node with some dofs fixed keeps them as a zero row /
- * column with a unit diagonal, so they stay exactly 0 in the solve). */
+ * formats are built here: the FP64 CSR row pointers (its columns and values
+ * on the first solve that uses the CSR format), and, when K is node-blocked,
+ * the node-block formats (a node with some dofs fixed keeps them as a zero
+ * row / column with a unit diagonal, so they stay exactly 0 in the solve). */
 int  sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out);
 int  sfem_system_info(const sfem_system* sys, int64_t* n_free, int64_t* nnz_ff, int* block);
 /* Device time of the elimination that built the system (ms). */
@@ -199,6 +200,10 @@ typedef struct {
     int     stop_reason;     /* 1 converged, 2 breakdown, 3 stagnation, 4 iteration cap  */
     double  floor_residual;  /* attainable relative residual estimate at the last FP64
                                 residual: eps || |rhs| + |K_ff||x| || / ||rhs||          */
+    double  smooth_ms;       /* multigrid, time_kernels: summed device time of the timed
+                                finest-level smoothing steps (FP32 operator)             */
+    int64_t smooth_timed;    /* number of those launches                                 */
+    double  smooth_bytes;    /* algorithmic bytes of one finest-level smoothing step     */
 } sfem_solve_stats;
 
 void sfem_solve_opts_default(sfem_solve_opts* opts);
@@ -232,7 +237,7 @@ int  sfem_system_solution(sfem_ctx* ctx, const sfem_system* sys, double* q,
 typedef struct {
     int     degree;        /* Chebyshev steps before and after the coarse correction (2)  */
     double  lo, hi;        /* smoothing interval, x lambda_max (0.1, 1.1)                 */
-    int     power_iters;   /* power iterations per level for lambda_max (12)              */
+    int     power_iters;   /* power iterations per level for lambda_max (8)               */
     int64_t coarse_nodes;  /* stop coarsening at this many nodes (64)                     */
 } sfem_mg_opts;
 

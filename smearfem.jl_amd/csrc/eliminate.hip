@@ -111,6 +111,36 @@ __global__ void csr_fill_kernel(const int64_t* __restrict__ bcolptr, const int32
     dinv[row] = diag != 0.0 ? 1.0 / diag : 1.0;
 }
 
+// rhs and 1/diag of csr_fill_kernel without the CSR arrays (same loop, same
+// order, so the same bits); the arrays are built on first use (ensure_csr).
+__global__ void rhs_dinv_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                const double* __restrict__ bval, const int32_t* __restrict__ free_of,
+                                const double* __restrict__ qd, const double* __restrict__ f, int64_t n, int bs,
+                                double* __restrict__ rhs, double* __restrict__ dinv) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t row = free_of[s];
+    if (row < 0) return;
+    const int64_t cu = s / bs;
+    const int j = (int)(s - cu * bs);
+    const int bs2 = bs * bs;
+    double lift = 0.0, diag = 0.0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
+        const int64_t r = browidx[u];
+        for (int i = 0; i < bs; ++i) {
+            const int64_t d = r * bs + i;
+            const double v = bval[u * bs2 + i * bs + j];
+            if (free_of[d] >= 0) {
+                if (d == s) diag = v;
+            } else {
+                lift += v * qd[d];
+            }
+        }
+    }
+    rhs[row] = f[s] - lift;
+    dinv[row] = diag != 0.0 ? 1.0 / diag : 1.0;
+}
+
 // rhs[row] = f[s] - (K q_d)[s] over the fixed columns, the sum csr_fill_kernel
 // forms (same code, same order), for new loads / prescribed values.
 __global__ void rhs_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
@@ -396,16 +426,17 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     S->rowptr.reset(ctx, (size_t)nf + 1);
     SFEM_HIP(hipMemcpyAsync(S->rowptr.get(), rowptr.get(), sizeof(int32_t) * (nf + 1), hipMemcpyDeviceToDevice,
                             ctx->stream));
-    S->col.reset(ctx, nnz);
-    S->val64.reset(ctx, nnz);
+    S->col.release();  // the CSR arrays are filled on first use (ensure_csr)
+    S->val64.release();
+    S->val32.release();
     S->rhs.reset(ctx, nf);
     S->x.reset(ctx, nf);
     S->dinv64.reset(ctx, nf);
     S->dinv32.reset(ctx, nf);
     memset_d(ctx, S->x.get(), 0, sizeof(double) * nf);
-    hipLaunchKernelGGL(csr_fill_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
-                       K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, bs, rowptr.get(), S->col.get(),
-                       S->val64.get(), S->rhs.get(), S->dinv64.get());
+    hipLaunchKernelGGL(rhs_dinv_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                       K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, bs, S->rhs.get(),
+                       S->dinv64.get());
     SFEM_LAUNCH_CHECK();
     if (nf) {
         hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(nf, 256)), dim3(256), 0, ctx->stream, S->dinv64.get(),
@@ -516,6 +547,19 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     S->t_eliminate_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+}
+
+void ensure_csr(sfem_ctx* ctx, sfem_system* S) {
+    if (S->nnz == 0 || S->col.size() == (size_t)S->nnz) return;
+    const sfem_csr* K = S->K;
+    S->col.reset(ctx, (size_t)S->nnz);
+    S->val64.reset(ctx, (size_t)S->nnz);
+    // the fill also forms rhs and 1/diag; those go to scratch (S keeps its own)
+    DevBuf<double> rhs(ctx, (size_t)S->nf), dinv(ctx, (size_t)S->nf);
+    hipLaunchKernelGGL(csr_fill_kernel, dim3(grid_for(S->n, 256)), dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                       K->browidx.get(), K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), S->n, S->bs,
+                       (const uint32_t*)S->rowptr.get(), S->col.get(), S->val64.get(), rhs.get(), dinv.get());
+    SFEM_LAUNCH_CHECK();
 }
 
 void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {

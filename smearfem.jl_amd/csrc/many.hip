@@ -357,6 +357,7 @@ struct ManyRun {
         // node-block SpMV when K is an assembled (bitwise symmetric) 3x3-block matrix
         const sfem_csr* K = S->K;
         const bool blk = K->bs == 3 && !K->transposed;
+        if (!blk) ensure_csr(ctx, S);
         const int64_t nunits = K->nunits;
         const int64_t uchunk = ((nunits + nb - 1) / nb + RPB - 1) / RPB * RPB;
         auto spmm = [&](const double* src, double* dst, double* prt) {

@@ -188,7 +188,11 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
 // Galerkin product on stencils: one thread per (coarse node I, slot s),
 // A_c(I, I+D)[a][b] = sum_{i, j} P(i, I) A_f(i, j)[a][b] P(j, I+D) over the
 // fine nodes i = f I + alpha, j = f (I+D) + beta of the two supports, in a
-// fixed order.  P weights are products of 1 and 1/2 per direction.
+// fixed order.  P weights are products of 1 and 1/2 per direction.  The
+// pairs (alpha, beta) are separable by direction: per direction the 9 pairs
+// get a fine coordinate of i, the offset j - i and a weight (0 when i or j
+// is off the grid or j - i is more than one apart); x and y pairs are
+// unrolled, z pairs looped.
 template <int ND>
 __global__ void galerkin_kernel(const double* __restrict__ stf, Grid gf, Grid gc, int f0, int f1, int f2,
                                 double* __restrict__ stc) {
@@ -208,32 +212,46 @@ __global__ void galerkin_kernel(const double* __restrict__ stf, Grid gf, Grid gc
     for (int m = 0; m < B2; ++m) acc[m] = 0.0;
     if (inside<ND>(gc, cj[0], cj[1], cj[2])) {
         const int f[3] = {f0, f1, f2};
-        int lo[3], hi[3];
-        for (int d = 0; d < 3; ++d) {
-            lo[d] = f[d] == 2 ? -1 : 0;
-            hi[d] = f[d] == 2 ? 1 : 0;
-        }
-        for (int az = lo[2]; az <= hi[2]; ++az)
-            for (int ay = lo[1]; ay <= hi[1]; ++ay)
-                for (int ax = lo[0]; ax <= hi[0]; ++ax) {
-                    const int64_t ix = f0 * ci[0] + ax, iy = f1 * ci[1] + ay, iz = f2 * ci[2] + az;
-                    if (!inside<ND>(gf, ix, iy, iz)) continue;
-                    const double wa = (ax ? 0.5 : 1.0) * (ay ? 0.5 : 1.0) * (az ? 0.5 : 1.0);
-                    const int64_t inode = node_at(gf, ix, iy, iz);
-                    for (int bz = lo[2]; bz <= hi[2]; ++bz)
-                        for (int by = lo[1]; by <= hi[1]; ++by)
-                            for (int bx = lo[0]; bx <= hi[0]; ++bx) {
-                                const int64_t jx = f0 * cj[0] + bx, jy = f1 * cj[1] + by, jz = f2 * cj[2] + bz;
-                                const int64_t ox = jx - ix, oy = jy - iy, oz = jz - iz;
-                                if (ox < -1 || ox > 1 || oy < -1 || oy > 1 || oz < -1 || oz > 1) continue;
-                                if (!inside<ND>(gf, jx, jy, jz)) continue;
-                                const double w = wa * ((bx ? 0.5 : 1.0) * (by ? 0.5 : 1.0) * (bz ? 0.5 : 1.0));
-                                const int sf = slot_of<ND>((int)ox, (int)oy, (int)oz);
-                                const double* src = stf + (int64_t)sf * B2 * nnf + inode;
+        const int64_t nfd[3] = {gf.n0, gf.n1, gf.n2};
+        auto pair = [&](int d, int k, int64_t& fi, int& off, double& w) {
+            const int a = k / 3 - 1, b = k % 3 - 1;
+            fi = f[d] * ci[d] + a;
+            const int64_t fj = f[d] * cj[d] + b;
+            off = (int)(fj - fi);
+            const bool ok = (f[d] == 2 || (a == 0 && b == 0)) && fi >= 0 && fi < nfd[d] && fj >= 0 &&
+                            fj < nfd[d] && off >= -1 && off <= 1;
+            w = ok ? (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) : 0.0;
+        };
+        int64_t fx[9], fy[9];
+        int ox[9], oy[9];
+        double wx[9], wy[9];
 #pragma unroll
-                                for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nnf], acc[m]);
-                            }
+        for (int k = 0; k < 9; ++k) {
+            pair(0, k, fx[k], ox[k], wx[k]);
+            pair(1, k, fy[k], oy[k], wy[k]);
+        }
+        for (int kz = 0; kz < 9; ++kz) {
+            int64_t iz;
+            int oz;
+            double wz;
+            pair(2, kz, iz, oz, wz);
+            if (wz == 0.0) continue;
+#pragma unroll
+            for (int ky = 0; ky < 9; ++ky) {
+                if (wy[ky] == 0.0) continue;
+                const int64_t rowy = gf.n0 * (fy[ky] + gf.n1 * iz);
+                const double wyz = wz * wy[ky];
+#pragma unroll
+                for (int kx = 0; kx < 9; ++kx) {
+                    if (wx[kx] == 0.0) continue;
+                    const int sf = slot_of<ND>(ox[kx], oy[ky], oz);
+                    const double w = wyz * wx[kx];
+                    const double* src = stf + (int64_t)sf * B2 * nnf + fx[kx] + rowy;
+#pragma unroll
+                    for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nnf], acc[m]);
                 }
+            }
+        }
     }
 #pragma unroll
     for (int m = 0; m < B2; ++m) stc[((int64_t)s * B2 + m) * nnc + I] = acc[m];
@@ -1109,9 +1127,13 @@ struct MgRun {
     Launch ls;
     const int* done;
     bool nt;
+    std::vector<hipEvent_t>* ev = nullptr;  // non-null: time the finest-level smoothing steps
+    size_t ev_next = 0;
 
     void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2) {
         MgLevel& L = H.L[(size_t)l];
+        const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
+        if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
         if (l == 0) {
             const SellOp op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
                             ls.per_block};
@@ -1131,6 +1153,7 @@ struct MgRun {
             else go(cheb_stencil_kernel<ND, 1>);
         }
         SFEM_LAUNCH_CHECK();
+        if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next++ + 1], s));
     }
 
     // x_l = V-cycle(b_l); level 0's right-hand side is the outer residual
@@ -1228,11 +1251,20 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         SFEM_LAUNCH_CHECK();
     };
     MgState host{};
-    int64_t enq = 0;
+    int64_t enq = 0, counted = 0, timed = 0;
+    double smooth_ms = 0.0;
     const int chunk_max = std::max(1, std::min(o->check_every, 8));
+    const int per_iter = 2 * H.degree - 1;  // finest-level smoothing steps per V-cycle
+    std::vector<hipEvent_t> ev;
+    if (o->time_kernels) {
+        ev.resize(2 * (size_t)per_iter * chunk_max);
+        for (auto& e : ev) SFEM_HIP(hipEventCreate(&e));
+        run.ev = &ev;
+    }
     d2h(ctx, &host, st, sizeof(host));
     while (!host.done && enq < o->max_iter) {
         const int64_t chunk = std::min<int64_t>(chunk_max, o->max_iter - enq);
+        run.ev_next = 0;
         for (int64_t k = 0; k < chunk; ++k) {
             true_resid();
             run.vcycle(0, r.get());
@@ -1257,7 +1289,19 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         true_resid();  // the check a converged recurrence asked for
         enq += chunk;
         d2h(ctx, &host, st, sizeof(host));
+        if (o->time_kernels) {
+            // V-cycles after the stop exit at once; time those of real iterations
+            const int64_t real = std::min<int64_t>(chunk, std::max<int64_t>(0, host.iters - counted));
+            for (int64_t k = 0; k < real * per_iter && k < (int64_t)run.ev_next; ++k) {
+                float ms = 0;
+                SFEM_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
+                smooth_ms += ms;
+                ++timed;
+            }
+        }
+        counted = host.iters;
     }
+    for (auto& e : ev) (void)hipEventDestroy(e);
     if (!host.done) {  // iteration cap: the true residual of the final iterate
         MgState h2 = host;
         h2.need_true = 1;
@@ -1289,6 +1333,12 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     stats->format = SFEM_FMT_BSR;
     stats->stop_reason = host.done == 0 ? 4 : host.done;
     stats->floor_residual = host.bb > 0 ? std::sqrt(host.floor_rr / 16.0 / host.bb) : 0.0;
+    // a smoothing step: FP32 operator once, v gathered, r and x read and
+    // written, d read and written, the 3x3 (2x2) inverse blocks read
+    stats->smooth_ms = smooth_ms;
+    stats->smooth_timed = timed;
+    stats->smooth_bytes = (double)S->nnzb * (BS * BS * 4.0 + 4.0) + (double)(S->nslices + 1) * 8.0 +
+                          (double)S->nslices * 4.0 + (double)n * 4.0 * (7.0 + BS);
 }
 
 template <int ND>
@@ -1310,7 +1360,7 @@ void mg_opts_default(sfem_mg_opts* o) {
     o->degree = 2;
     o->lo = 0.1;
     o->hi = 1.1;
-    o->power_iters = 12;
+    o->power_iters = 8;
     o->coarse_nodes = 64;
 }
 

@@ -640,6 +640,7 @@ double spmv_bytes_for(const sfem_system* S, int fmt, double sv, double sx, int n
 }
 
 void ensure_val32(sfem_ctx* ctx, sfem_system* S) {
+    ensure_csr(ctx, S);
     if (S->val32.size() == (size_t)S->nnz || S->nnz == 0) return;
     S->val32.reset(ctx, (size_t)S->nnz);
     hipLaunchKernelGGL(f64_to_f32_kernel, dim3(2048), dim3(256), 0, ctx->stream, S->val64.get(), S->val32.get(),
@@ -807,6 +808,7 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
     }
     if (!(o->rtol >= 0.0) || o->max_iter < 0) fail(SFEM_ERR_ARGUMENT, "solve: bad rtol or max_iter");
     const int fmt = resolve_format(S, o->format);
+    if (fmt == SFEM_FMT_CSR) ensure_csr(ctx, S);
     if (o->preconditioner == SFEM_PC_MULTIGRID) {
         if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "multigrid: needs the node-block format");
         mg_solve(ctx, S, o, st);
@@ -890,6 +892,7 @@ double spmv_bench_impl(sfem_ctx* ctx, sfem_system* S, int fmt, int reps, bool nt
 void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
                        double* bytes) {
     const int fmt = resolve_format(S, format);
+    if (fmt == SFEM_FMT_CSR) ensure_csr(ctx, S);
     if (precision != SFEM_PREC_FP64 && fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
     switch (precision) {
         case SFEM_PREC_FP64:

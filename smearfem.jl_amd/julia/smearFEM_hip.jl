@@ -112,6 +112,9 @@ mutable struct SolveStats
     format::Cint
     stop_reason::Cint
     floor_residual::Float64
+    smooth_ms::Float64
+    smooth_timed::Int64
+    smooth_bytes::Float64
 end
 
 """
@@ -134,7 +137,7 @@ function solve_system(K::SparseMatrixCSC{Float64,Int64}, f, fixed, q_d; precisio
     mask = Vector{UInt8}(fixed .!= 0)
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
     opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1, 0)
-    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
     q = zeros(n)
     try
         st = ccall((:sfem_solve, libsfem), Cint,
@@ -186,7 +189,7 @@ function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Yo
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
     opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1,
                      preconditioner_code(preconditioner))
-    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
     q = zeros(n)
     try
         st = ccall((:sfem_solve, libsfem), Cint,

@@ -48,6 +48,9 @@ class SolveStats(C.Structure):
         ("format", C.c_int),
         ("stop_reason", C.c_int),
         ("floor_residual", C.c_double),
+        ("smooth_ms", C.c_double),
+        ("smooth_timed", C.c_int64),
+        ("smooth_bytes", C.c_double),
     ]
 
 

@@ -116,7 +116,7 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
-    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=12, coarse_nodes=64) -> dict:
+    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
