@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "block_ops.hpp"
@@ -65,6 +66,13 @@ struct MgHierarchy {
     DevBuf<double> cinv;          // coarsest level: dense inverse, nc x nc row-major
     int64_t nc = 0;
     double setup_ms = 0;
+    // FP16 finest-level smoothing operator: s * K_ff (s = 2^-e, max |s a| in
+    // (1/2, 1]) in the system's sliced layout, and the inverses of its
+    // diagonal blocks; the finest-level smoother then works in units of s
+    bool fp16 = false;
+    double scale0 = 1.0;
+    DevBuf<_Float16> h0;
+    DevBuf<float> binv0;
 };
 
 void MgDelete::operator()(MgHierarchy* h) const { delete h; }
@@ -185,76 +193,53 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
     for (int a = 0; a < BS; ++a) cfix[I * BS + a] = ffix[fn * BS + a];
 }
 
-// Galerkin product on stencils: one thread per (coarse node I, slot s),
-// A_c(I, I+D)[a][b] = sum_{i, j} P(i, I) A_f(i, j)[a][b] P(j, I+D) over the
-// fine nodes i = f I + alpha, j = f (I+D) + beta of the two supports, in a
-// fixed order.  P weights are products of 1 and 1/2 per direction.  The
-// pairs (alpha, beta) are separable by direction: per direction the 9 pairs
-// get a fine coordinate of i, the offset j - i and a weight (0 when i or j
-// is off the grid or j - i is more than one apart); x and y pairs are
-// unrolled, z pairs looped.
+// Galerkin product along one direction: P = P_x (x) P_y (x) P_z, so P'AP
+// is three one-direction products, each coarsening direction d by 2 and
+// keeping the others.  One thread per (output node I, slot s):
+// A'(I, I+D)[a][b] = sum over the pairs (alpha, beta) of direction d of
+// w(alpha) w(beta) A(i, i+o)[a][b], with i = I except i_d = 2 I_d + alpha,
+// o = D except o_d = 2 (I_d + D_d) + beta - i_d (pairs with |o_d| > 1 or a
+// node off the grid add nothing); alpha, beta in {-1, 0, 1}, w(0) = 1,
+// w(+-1) = 1/2, summed in a fixed order.
 template <int ND>
-__global__ void galerkin_kernel(const double* __restrict__ stf, Grid gf, Grid gc, int f0, int f1, int f2,
-                                double* __restrict__ stc) {
+__global__ void galerkin_dir_kernel(const double* __restrict__ in, Grid gi, int d, Grid go,
+                                    double* __restrict__ out) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
-    const int64_t nnc = gc.n0 * gc.n1 * gc.n2, nnf = gf.n0 * gf.n1 * gf.n2;
+    const int64_t nno = go.n0 * go.n1 * go.n2, nni = gi.n0 * gi.n1 * gi.n2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nnc * NS) return;
-    const int s = (int)(t / nnc);
-    const int64_t I = t - (int64_t)s * nnc;
-    int64_t ci[3];
-    coords_of<ND>(I, gc, ci);
+    if (t >= nno * NS) return;
+    const int s = (int)(t / nno);
+    const int64_t I = t - (int64_t)s * nno;
+    int64_t c[3];
+    coords_of<ND>(I, go, c);
     int D[3];
     offset_of<ND>(s, D[0], D[1], D[2]);
-    const int64_t cj[3] = {ci[0] + D[0], ci[1] + D[1], ci[2] + D[2]};
     double acc[B2];
 #pragma unroll
     for (int m = 0; m < B2; ++m) acc[m] = 0.0;
-    if (inside<ND>(gc, cj[0], cj[1], cj[2])) {
-        const int f[3] = {f0, f1, f2};
-        const int64_t nfd[3] = {gf.n0, gf.n1, gf.n2};
-        auto pair = [&](int d, int k, int64_t& fi, int& off, double& w) {
-            const int a = k / 3 - 1, b = k % 3 - 1;
-            fi = f[d] * ci[d] + a;
-            const int64_t fj = f[d] * cj[d] + b;
-            off = (int)(fj - fi);
-            const bool ok = (f[d] == 2 || (a == 0 && b == 0)) && fi >= 0 && fi < nfd[d] && fj >= 0 &&
-                            fj < nfd[d] && off >= -1 && off <= 1;
-            w = ok ? (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0) : 0.0;
-        };
-        int64_t fx[9], fy[9];
-        int ox[9], oy[9];
-        double wx[9], wy[9];
+    if (inside<ND>(go, c[0] + D[0], c[1] + D[1], c[2] + D[2])) {
+        const int64_t nid = d == 0 ? gi.n0 : d == 1 ? gi.n1 : gi.n2;
+        for (int a = -1; a <= 1; ++a) {
+            const int64_t ia = 2 * c[d] + a;
+            if (ia < 0 || ia >= nid) continue;
+            for (int b = -1; b <= 1; ++b) {
+                const int64_t jb = 2 * (c[d] + D[d]) + b;
+                const int64_t o = jb - ia;
+                if (jb < 0 || jb >= nid || o < -1 || o > 1) continue;
+                int64_t ic[3] = {c[0], c[1], c[2]};
+                ic[d] = ia;
+                int od[3] = {D[0], D[1], D[2]};
+                od[d] = (int)o;
+                const double w = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0);
+                const int sf = slot_of<ND>(od[0], od[1], od[2]);
+                const double* src = in + (int64_t)sf * B2 * nni + node_at(gi, ic[0], ic[1], ic[2]);
 #pragma unroll
-        for (int k = 0; k < 9; ++k) {
-            pair(0, k, fx[k], ox[k], wx[k]);
-            pair(1, k, fy[k], oy[k], wy[k]);
-        }
-        for (int kz = 0; kz < 9; ++kz) {
-            int64_t iz;
-            int oz;
-            double wz;
-            pair(2, kz, iz, oz, wz);
-            if (wz == 0.0) continue;
-#pragma unroll
-            for (int ky = 0; ky < 9; ++ky) {
-                if (wy[ky] == 0.0) continue;
-                const int64_t rowy = gf.n0 * (fy[ky] + gf.n1 * iz);
-                const double wyz = wz * wy[ky];
-#pragma unroll
-                for (int kx = 0; kx < 9; ++kx) {
-                    if (wx[kx] == 0.0) continue;
-                    const int sf = slot_of<ND>(ox[kx], oy[ky], oz);
-                    const double w = wyz * wx[kx];
-                    const double* src = stf + (int64_t)sf * B2 * nnf + fx[kx] + rowy;
-#pragma unroll
-                    for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nnf], acc[m]);
-                }
+                for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nni], acc[m]);
             }
         }
     }
 #pragma unroll
-    for (int m = 0; m < B2; ++m) stc[((int64_t)s * B2 + m) * nnc + I] = acc[m];
+    for (int m = 0; m < B2; ++m) out[((int64_t)s * B2 + m) * nno + I] = acc[m];
 }
 
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
@@ -310,12 +295,53 @@ __global__ void stencil_binv_kernel(const double* __restrict__ st64, const uint8
     for (int k = 0; k < B2; ++k) binv[(int64_t)k * nn + node] = (float)inv[k];
 }
 
+// max |a| over the FP64 sliced values (exact: the bit patterns of
+// non-negative doubles order as unsigned integers)
+__global__ void maxabs_kernel(const double* __restrict__ v, int64_t n, unsigned long long* __restrict__ out) {
+    unsigned long long m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max(m, (unsigned long long)__double_as_longlong(fabs(v[i])));
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor((long long)m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+__global__ void to_half_kernel(const double* __restrict__ v, int64_t n, double scale, _Float16* __restrict__ h) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        h[i] = (_Float16)(float)(v[i] * scale);
+}
+
+// Inverse of each block row's diagonal block, read from the FP16 values.
+template <int BS>
+__global__ void binv_half_kernel(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
+                                 const int32_t* __restrict__ scol, const _Float16* __restrict__ h, int64_t nrows,
+                                 float* __restrict__ binv) {
+    constexpr int B2 = BS * BS;
+    const int64_t row = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= nrows) return;
+    const int64_t sl = row / 64;
+    const int lane = (int)(row % 64);
+    double m[B2], inv[B2];
+    for (int k = 0; k < B2; ++k) m[k] = (k % (BS + 1) == 0) ? 1.0 : 0.0;
+    const int w = sl_w[sl];
+    for (int k = 0; k < w; ++k) {
+        const int64_t slot = sl_ptr[sl] + (int64_t)k * 64;
+        if (scol[slot + lane] != row) continue;
+        for (int v = 0; v < B2; ++v) m[v] = (double)h[slot * B2 + (int64_t)v * 64 + lane];
+        break;
+    }
+    invert_block<BS>(m, inv);
+    for (int k = 0; k < B2; ++k) binv[(int64_t)k * nrows + row] = (float)inv[k];
+}
+
 // ------------------------------------------------------- V-cycle ----
+// finest-level smoothing operator: the system's sliced format with FP32
+// values, or FP16 values of the operator scaled by 1 / max |a|
+template <typename TV>
 struct SellOp {
     const int64_t* sl_ptr;
     const int32_t* sl_w;
     const int32_t* scol;
-    const float* val;
+    const TV* val;
     int64_t nslices, nrows, per_block;
 };
 
@@ -376,8 +402,8 @@ __device__ __forceinline__ void cheb_epilogue(int64_t row, int64_t nrows, const 
     }
 }
 
-template <int BS, int MODE, bool NT>
-__global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp op, const float* v, float* __restrict__ r,
+template <int BS, int MODE, bool NT, typename TV>
+__global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp<TV> op, const float* v, float* __restrict__ r,
                                                              float* __restrict__ x, const float* d_in,
                                                              float* __restrict__ d_out, float c1, float c2,
                                                              const float* __restrict__ binv, const int* done) {
@@ -388,7 +414,7 @@ __global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp op, const fl
     auto gx = [=](int64_t c) { return v[c]; };
     for (int64_t sl = s0 + w; sl < s1; sl += kThreads / 64) {
         float out[BS];
-        sell_row<BS, float, float, NT>(op.sl_ptr, op.sl_w, op.scol, Vals<float>{op.val}, sl, lane, gx, out);
+        sell_row<BS, TV, float, NT>(op.sl_ptr, op.sl_w, op.scol, Vals<TV>{op.val}, sl, lane, gx, out);
         const int64_t row = sl * 64 + lane;
         if (row < op.nrows) cheb_epilogue<BS, MODE>(row, op.nrows, out, r, x, d_in, d_out, c1, c2, binv);
     }
@@ -407,9 +433,10 @@ __global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp op, const f
     cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
 }
 
-// First step from x = 0: r = b, d = c2 B^-1 b, x = d.
+// First step from x = 0: r = b, d = c2 B^-1 b, x = d.  bscale: the units of
+// the level's operator (the FP16 finest level works on b / max |a|).
 template <int BS, typename TB>
-__global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restrict__ b, int64_t nrows,
+__global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restrict__ b, TB bscale, int64_t nrows,
                                                               const float* __restrict__ binv, float c2,
                                                               float* __restrict__ r, float* __restrict__ x,
                                                               float* __restrict__ d, const int* done) {
@@ -419,7 +446,7 @@ __global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restri
     float rv[BS];
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
-        rv[j] = (float)b[row * BS + j];
+        rv[j] = (float)(b[row * BS + j] * bscale);
         r[row * BS + j] = rv[j];
     }
 #pragma unroll
@@ -434,9 +461,10 @@ __global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restri
 }
 
 // b_c = P' r_f on the coarse grid (0 on fixed coarse dofs); row_of_node maps
-// fine grid nodes to rows of r_f (NULL: the identity).
+// fine grid nodes to rows of r_f (NULL: the identity); r_f is in units of
+// 1 / rscale.
 template <int ND>
-__global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, Grid gf,
+__global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, float rscale, Grid gf,
                                                             const int32_t* __restrict__ row_of_node, Grid gc, int f0,
                                                             int f1, int f2, const uint8_t* __restrict__ cfix,
                                                             float* __restrict__ bc, const int* done) {
@@ -465,7 +493,7 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
                 if (row < 0) continue;
                 const float w = (ax ? 0.5f : 1.0f) * (ay ? 0.5f : 1.0f) * (az ? 0.5f : 1.0f);
 #pragma unroll
-                for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j], acc[j]);
+                for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
             }
 #pragma unroll
     for (int j = 0; j < BS; ++j) bc[I * BS + j] = cfix[I * BS + j] ? 0.0f : acc[j];
@@ -601,8 +629,8 @@ __device__ __forceinline__ double pow_epilogue(int64_t row, int64_t nrows, const
 }
 
 // w = B^-1 A (v / |v|), partial |w|^2
-template <int BS>
-__global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp op, const float* __restrict__ v,
+template <int BS, typename TV>
+__global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp<TV> op, const float* __restrict__ v,
                                                             const float* __restrict__ binv, float* __restrict__ w,
                                                             PowState* ps, double* __restrict__ partials) {
     __shared__ double sh[8];
@@ -614,7 +642,7 @@ __global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp op, const flo
     double s = 0.0;
     for (int64_t sl = s0 + wv; sl < s1; sl += kThreads / 64) {
         float out[BS];
-        sell_row<BS, float, float, false>(op.sl_ptr, op.sl_w, op.scol, Vals<float>{op.val}, sl, lane, gx, out);
+        sell_row<BS, TV, float, false>(op.sl_ptr, op.sl_w, op.scol, Vals<TV>{op.val}, sl, lane, gx, out);
         const int64_t row = sl * 64 + lane;
         if (row < op.nrows) s += pow_epilogue<BS>(row, op.nrows, out, scale, binv, w);
     }
@@ -1004,10 +1032,31 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         hipLaunchKernelGGL(coarse_fixed_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, prev_fix, gf, gc,
                            f[0], f[1], f[2], C.fixed.get());
         SFEM_LAUNCH_CHECK();
-        DevBuf<double> raw(ctx, (size_t)C.nn * NS * B2);
-        hipLaunchKernelGGL(galerkin_kernel<ND>, dim3(rows_grid(C.nn * NS)), dim3(kThreads), 0, s, prev_st, gf, gc,
-                           f[0], f[1], f[2], raw.get());
-        SFEM_LAUNCH_CHECK();
+        // P'AP one direction at a time (x, then y, then z)
+        DevBuf<double> raw, tmp;
+        const double* cur = prev_st;
+        Grid gcur = gf;
+        int last = -1;
+        for (int d = 0; d < ND; ++d)
+            if (f[d] == 2) last = d;
+        for (int d = 0; d < ND; ++d) {
+            if (f[d] != 2) continue;
+            Grid gn = gcur;
+            int64_t* nd = d == 0 ? &gn.n0 : d == 1 ? &gn.n1 : &gn.n2;
+            *nd = (*nd - 1) / 2 + 1;
+            const int64_t nno = gn.n0 * gn.n1 * gn.n2;
+            DevBuf<double> o(ctx, (size_t)nno * NS * B2);
+            hipLaunchKernelGGL(galerkin_dir_kernel<ND>, dim3(rows_grid(nno * NS)), dim3(kThreads), 0, s, cur, gcur, d,
+                               gn, o.get());
+            SFEM_LAUNCH_CHECK();
+            if (d == last) {
+                raw = std::move(o);
+            } else {
+                tmp = std::move(o);  // the previous intermediate goes back to the allocator in stream order
+                cur = tmp.get();
+            }
+            gcur = gn;
+        }
         C.st64.reset(ctx, (size_t)C.nn * NS * B2);
         C.st32.reset(ctx, (size_t)C.nn * NS * B2);
         hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn * NS)), dim3(kThreads), 0, s, raw.get(), gc,
@@ -1062,6 +1111,32 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         h2d(ctx, H.cinv.get(), inv.data(), inv.size() * sizeof(double));
     }
 
+    // FP16 finest-level smoothing operator
+    if (o.fine_fp16) {
+        const int64_t nv = S->sell_blocks * B2;
+        DevBuf<unsigned long long> mx(ctx, 1);
+        memset_d(ctx, mx.get(), 0, sizeof(unsigned long long));
+        hipLaunchKernelGGL(maxabs_kernel, dim3(1024), dim3(kThreads), 0, s, S->sval64.get(), nv, mx.get());
+        SFEM_LAUNCH_CHECK();
+        unsigned long long hm = 0;
+        d2h(ctx, &hm, mx.get(), sizeof(hm));
+        double amax;
+        std::memcpy(&amax, &hm, sizeof(amax));
+        if (!(amax > 0.0) || !std::isfinite(amax)) fail(SFEM_ERR_SINGULAR, "multigrid: zero or non-finite operator");
+        int e = 0;
+        std::frexp(amax, &e);  // amax = f 2^e, f in [1/2, 1)
+        H.scale0 = std::ldexp(1.0, -e);
+        H.h0.reset(ctx, (size_t)nv);
+        hipLaunchKernelGGL(to_half_kernel, dim3(2048), dim3(kThreads), 0, s, S->sval64.get(), nv, H.scale0,
+                           H.h0.get());
+        SFEM_LAUNCH_CHECK();
+        H.binv0.reset(ctx, (size_t)S->nfb * B2);
+        hipLaunchKernelGGL(binv_half_kernel<BS>, dim3(rows_grid(S->nfb)), dim3(kThreads), 0, s, S->sl_ptr.get(),
+                           S->sl_w.get(), S->scol.get(), (const _Float16*)H.h0.get(), S->nfb, H.binv0.get());
+        SFEM_LAUNCH_CHECK();
+        H.fp16 = true;
+    }
+
     // work vectors
     for (size_t l = 0; l < H.L.size(); ++l) {
         MgLevel& v = H.L[l];
@@ -1092,10 +1167,15 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         float* vin = v.da.get();
         float* vout = v.db.get();
         for (int k = 0; k < o.power_iters; ++k) {
-            if (l == 0) {
-                const SellOp op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
-                                ls.per_block};
-                hipLaunchKernelGGL(pow_sell_kernel<BS>, dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
+            if (l == 0 && H.fp16) {
+                const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices,
+                                          S->nfb, ls.per_block};
+                hipLaunchKernelGGL((pow_sell_kernel<BS, _Float16>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
+                                   (const float*)H.binv0.get(), vout, ps.get(), partials.get());
+            } else if (l == 0) {
+                const SellOp<float> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices,
+                                       S->nfb, ls.per_block};
+                hipLaunchKernelGGL((pow_sell_kernel<BS, float>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
                                    S->binv32.get(), vout, ps.get(), partials.get());
             } else {
                 const StOp op{v.st32.get(), grid_of(v), v.nn};
@@ -1134,15 +1214,24 @@ struct MgRun {
         MgLevel& L = H.L[(size_t)l];
         const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
-        if (l == 0) {
-            const SellOp op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
-                            ls.per_block};
+        if (l == 0 && H.fp16) {
+            const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices, S->nfb,
+                                      ls.per_block};
+            auto go = [&](auto k) {
+                hipLaunchKernelGGL(k, dim3(ls.grid), dim3(kThreads), 0, s, op, v, L.r.get(), L.x.get(), d_in, d_out,
+                                   c1, c2, (const float*)H.binv0.get(), done);
+            };
+            if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true, _Float16>) : go(cheb_sell_kernel<BS, 0, false, _Float16>);
+            else nt ? go(cheb_sell_kernel<BS, 1, true, _Float16>) : go(cheb_sell_kernel<BS, 1, false, _Float16>);
+        } else if (l == 0) {
+            const SellOp<float> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices, S->nfb,
+                                   ls.per_block};
             auto go = [&](auto k) {
                 hipLaunchKernelGGL(k, dim3(ls.grid), dim3(kThreads), 0, s, op, v, L.r.get(), L.x.get(), d_in, d_out,
                                    c1, c2, (const float*)S->binv32.get(), done);
             };
-            if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true>) : go(cheb_sell_kernel<BS, 0, false>);
-            else nt ? go(cheb_sell_kernel<BS, 1, true>) : go(cheb_sell_kernel<BS, 1, false>);
+            if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true, float>) : go(cheb_sell_kernel<BS, 0, false, float>);
+            else nt ? go(cheb_sell_kernel<BS, 1, true, float>) : go(cheb_sell_kernel<BS, 1, false, float>);
         } else {
             const StOp op{L.st32.get(), grid_of(L), L.nn};
             auto go = [&](auto k) {
@@ -1166,14 +1255,14 @@ struct MgRun {
             SFEM_LAUNCH_CHECK();
             return;
         }
-        const float* binv = l == 0 ? S->binv32.get() : L.binv.get();
+        const float* binv = l > 0 ? L.binv.get() : H.fp16 ? H.binv0.get() : S->binv32.get();
         if (l == 0)
             hipLaunchKernelGGL((cheb_first_kernel<BS, double>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, r_outer,
-                               L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(), done);
+                               H.scale0, L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(), done);
         else
             hipLaunchKernelGGL((cheb_first_kernel<BS, float>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s,
-                               (const float*)L.b.get(), L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(),
-                               done);
+                               (const float*)L.b.get(), 1.0f, L.nrows, binv, L.c2[0], L.r.get(), L.x.get(),
+                               L.da.get(), done);
         SFEM_LAUNCH_CHECK();
         float* dp = L.da.get();
         float* dn = L.db.get();
@@ -1185,7 +1274,7 @@ struct MgRun {
         MgLevel& C = H.L[(size_t)l + 1];
         const Grid gf = grid_of(L), gc = grid_of(C);
         hipLaunchKernelGGL(restrict_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, (const float*)L.r.get(),
-                           gf, l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
+                           l == 0 ? (float)(1.0 / H.scale0) : 1.0f, gf, l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
                            (const uint8_t*)C.fixed.get(), C.b.get(), done);
         SFEM_LAUNCH_CHECK();
         vcycle(l + 1, nullptr);
@@ -1333,11 +1422,11 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     stats->format = SFEM_FMT_BSR;
     stats->stop_reason = host.done == 0 ? 4 : host.done;
     stats->floor_residual = host.bb > 0 ? std::sqrt(host.floor_rr / 16.0 / host.bb) : 0.0;
-    // a smoothing step: FP32 operator once, v gathered, r and x read and
+    // a smoothing step: FP16 or FP32 operator once, v gathered, r and x read and
     // written, d read and written, the 3x3 (2x2) inverse blocks read
     stats->smooth_ms = smooth_ms;
     stats->smooth_timed = timed;
-    stats->smooth_bytes = (double)S->nnzb * (BS * BS * 4.0 + 4.0) + (double)(S->nslices + 1) * 8.0 +
+    stats->smooth_bytes = (double)S->nnzb * (BS * BS * (H.fp16 ? 2.0 : 4.0) + 4.0) + (double)(S->nslices + 1) * 8.0 +
                           (double)S->nslices * 4.0 + (double)n * 4.0 * (7.0 + BS);
 }
 
@@ -1362,6 +1451,7 @@ void mg_opts_default(sfem_mg_opts* o) {
     o->hi = 1.1;
     o->power_iters = 8;
     o->coarse_nodes = 64;
+    o->fine_fp16 = 1;
 }
 
 void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
@@ -1423,6 +1513,7 @@ void mg_info(const sfem_system* S, sfem_mg_info* info) {
     info->coarse_dofs = H.nc;
     info->setup_ms = H.setup_ms;
     info->degree = H.degree;
+    info->fine_fp16 = H.fp16 ? 1 : 0;
 }
 
 void mg_stencil(sfem_ctx* ctx, const sfem_system* S, int level, double* stencil, uint8_t* fixed) {

@@ -116,7 +116,7 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
-    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64) -> dict:
+    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
@@ -126,6 +126,7 @@ class System:
         self.ctx._lib.sfem_mg_opts_default(C.byref(o))
         o.degree, o.lo, o.hi = int(degree), float(lo), float(hi)
         o.power_iters, o.coarse_nodes = int(power_iters), int(coarse_nodes)
+        o.fine_fp16 = 1 if fine_fp16 else 0
         self.ctx.check(self.ctx._lib.sfem_system_set_grid(self.ctx.handle, self.handle, int(d.size),
                                                           _ptr(d, C.c_int64), C.byref(o)))
         self.grid = tuple(int(v) for v in d)
@@ -136,7 +137,7 @@ class System:
         self.ctx.check(self.ctx._lib.sfem_system_mg_info(self.handle, C.byref(info)))
         nl = info.levels
         nd = len(getattr(self, "grid", (0, 0, 0)))
-        return {"levels": nl, "degree": info.degree, "nodes": [info.nodes[i] for i in range(nl)],
+        return {"levels": nl, "degree": info.degree, "fine_fp16": bool(info.fine_fp16), "nodes": [info.nodes[i] for i in range(nl)],
                 "dims": [tuple(info.dims[i][d] for d in range(nd)) for i in range(nl)],
                 "lambda_max": [info.lambda_max[i] for i in range(nl)], "coarse_dofs": info.coarse_dofs,
                 "setup_ms": info.setup_ms}

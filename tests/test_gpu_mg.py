@@ -158,6 +158,33 @@ def test_mg_solve_vs_oracle(ctx, precision):
     assert relerr(q, oracle_direct(oracle_K(m, young), fixed, f)) <= ERR
 
 
+def test_mg_fp32_smoother_matches_fp16_smoother_solution(ctx):
+    """Both finest-level smoother operators (FP16 scaled, FP32) reach the
+    oracle's solution; FP16 values change the preconditioner only."""
+    print("[case] mg fp16 vs fp32 smoother", flush=True)
+    m = structured_mesh((16, 4, 8), lengths=(2.0, 0.5, 1.0))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=9)
+    fixed = roller(m)
+    f = random_face_load(m, 2, 1, fixed, seed=9)
+    Kd = device_K(ctx, m, young)
+    ref = oracle_direct(oracle_K(m, young), fixed, f)
+    its = {}
+    for fp16 in (True, False):
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
+        S = System(Kd, bc)
+        bc.close()
+        info = S.set_grid(m.dims, fine_fp16=fp16)
+        assert info["fine_fp16"] == fp16
+        st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+        assert st["stop_reason"] == 1
+        assert relerr(S.solution(), ref) <= ERR
+        its[fp16] = st["iterations"]
+        S.close()
+    Kd.close()
+    print(f"[mg] iterations fp16 {its[True]} fp32 {its[False]}", flush=True)
+    assert its[True] <= its[False] + 3
+
+
 def test_mg_partial_nodes_prescribed_values_and_set_bc(ctx):
     print("[case] mg roller + q_d + set_bc", flush=True)
     m = structured_mesh((8, 4, 6), lengths=(2.0, 1.0, 1.5))
