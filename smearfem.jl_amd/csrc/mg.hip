@@ -147,37 +147,41 @@ __global__ void row_maps_kernel(const uint32_t* __restrict__ flag, const uint32_
     for (int i = 0; i < bs; ++i) fixed0[(int64_t)row * bs + i] = free_of[u * bs + i] < 0 ? 1 : 0;
 }
 
-// Fine stencil from K's node blocks (one thread per stored block column):
-// st[(s*B2 + i*BS + j)*nn + row] = K[(row,i),(row+off_s,j)], 0 where either
-// dof is fixed.  A block between nodes that are not grid neighbours sets *bad.
+// Fine stencil from K's node blocks, one thread per stored block (u, in
+// column c = bcolidx[u], row r = browidx[u]): K is symmetric, so the block
+// gives row c's slot of offset r - c,
+// st[(s*B2 + i*BS + j)*nn + c] = K[(c,i),(r,j)] = bval[u*B2 + j*BS + i]
+// (exact for K stored transposed as well), 0 where either dof is fixed.  The
+// caller zeroes st first (slots without a block).  A block between nodes
+// that are not grid neighbours sets *bad.
 template <int ND>
-__global__ void stencil_from_k_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
-                                      const double* __restrict__ bval, int64_t nn, int transposed,
+__global__ void stencil_from_k_kernel(const int32_t* __restrict__ bcolidx, const int32_t* __restrict__ browidx,
+                                      const double* __restrict__ bval, int64_t nnzb, int64_t nn,
                                       const int32_t* __restrict__ free_of, Grid g, double* __restrict__ st,
                                       int* __restrict__ bad) {
     constexpr int BS = ND, B2 = BS * BS;
-    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cu >= nn) return;
-    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
-        const int64_t ru = browidx[u];
-        // K stores K[(ru,i),(cu,j)] at bval[u*B2 + i*BS + j]; transposed, K'
-        const int64_t row = transposed ? cu : ru, col = transposed ? ru : cu;
-        int64_t a[3], b[3];
-        coords_of<ND>(row, g, a);
-        coords_of<ND>(col, g, b);
-        const int64_t dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
-        if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
-            atomicOr(bad, 1);
-            continue;
-        }
-        const int s = slot_of<ND>((int)dx, (int)dy, (int)dz);
-        for (int i = 0; i < BS; ++i)
-            for (int j = 0; j < BS; ++j) {
-                const double v = transposed ? bval[u * B2 + j * BS + i] : bval[u * B2 + i * BS + j];
-                const bool masked = free_of[row * BS + i] < 0 || free_of[col * BS + j] < 0;
-                st[((int64_t)s * B2 + i * BS + j) * nn + row] = masked ? 0.0 : v;
-            }
+    const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= nnzb) return;
+    const int64_t c = bcolidx[u], r = browidx[u];
+    int64_t a[3], b[3];
+    coords_of<ND>(c, g, a);
+    coords_of<ND>(r, g, b);
+    const int64_t dx = b[0] - a[0], dy = b[1] - a[1], dz = b[2] - a[2];
+    if (dx < -1 || dx > 1 || dy < -1 || dy > 1 || dz < -1 || dz > 1) {
+        atomicOr(bad, 1);
+        return;
     }
+    const int s = slot_of<ND>((int)dx, (int)dy, (int)dz);
+    double v[B2];
+#pragma unroll
+    for (int m = 0; m < B2; ++m) v[m] = bval[u * B2 + m];
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            const bool masked = free_of[c * BS + i] < 0 || free_of[r * BS + j] < 0;
+            st[((int64_t)s * B2 + i * BS + j) * nn + c] = masked ? 0.0 : v[j * BS + i];
+        }
 }
 
 // Coarse dof fixed <=> the fine dof at the same node is fixed.
@@ -295,19 +299,21 @@ __global__ void stencil_binv_kernel(const double* __restrict__ st64, const uint8
     for (int k = 0; k < B2; ++k) binv[(int64_t)k * nn + node] = (float)inv[k];
 }
 
-// max |a| over the FP64 sliced values (exact: the bit patterns of
-// non-negative doubles order as unsigned integers)
-__global__ void maxabs_kernel(const double* __restrict__ v, int64_t n, unsigned long long* __restrict__ out) {
+// max |a| of K_ff: K_ff is symmetric positive definite, so |a_ij| <=
+// sqrt(a_ii a_jj) and the largest entry is a diagonal one, max 1 / dinv
+// (exact: the bit patterns of non-negative doubles order as unsigned integers)
+__global__ void maxdiag_kernel(const double* __restrict__ dinv, int64_t n, unsigned long long* __restrict__ out) {
     unsigned long long m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        m = max(m, (unsigned long long)__double_as_longlong(fabs(v[i])));
+        m = max(m, (unsigned long long)__double_as_longlong(fabs(1.0 / dinv[i])));
     for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned long long)__shfl_xor((long long)m, off, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-__global__ void to_half_kernel(const double* __restrict__ v, int64_t n, double scale, _Float16* __restrict__ h) {
+// FP16 of the FP32 values times a power of two (the scaling is exact)
+__global__ void to_half_kernel(const float* __restrict__ v, int64_t n, float scale, _Float16* __restrict__ h) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        h[i] = (_Float16)(float)(v[i] * scale);
+        h[i] = (_Float16)(v[i] * scale);
 }
 
 // Inverse of each block row's diagonal block, read from the FP16 values.
@@ -462,12 +468,15 @@ __global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restri
 
 // b_c = P' r_f on the coarse grid (0 on fixed coarse dofs); row_of_node maps
 // fine grid nodes to rows of r_f (NULL: the identity); r_f is in units of
-// 1 / rscale.
-template <int ND>
+// 1 / rscale.  FIRST: also the coarse level's first smoothing step
+// (cheb_first_kernel: r_c = b_c, d_c = c2 B^-1 b_c, x_c = d_c).
+template <int ND, bool FIRST>
 __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, float rscale, Grid gf,
                                                             const int32_t* __restrict__ row_of_node, Grid gc, int f0,
                                                             int f1, int f2, const uint8_t* __restrict__ cfix,
-                                                            float* __restrict__ bc, const int* done) {
+                                                            float* __restrict__ bc, const float* __restrict__ binv,
+                                                            float c2, float* __restrict__ rc, float* __restrict__ xc,
+                                                            float* __restrict__ dc, const int* done) {
     constexpr int BS = ND;
     if (*done) return;
     const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -495,8 +504,24 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
 #pragma unroll
                 for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
             }
+    const int64_t nnc = gc.n0 * gc.n1 * gc.n2;
 #pragma unroll
-    for (int j = 0; j < BS; ++j) bc[I * BS + j] = cfix[I * BS + j] ? 0.0f : acc[j];
+    for (int j = 0; j < BS; ++j) {
+        acc[j] = cfix[I * BS + j] ? 0.0f : acc[j];
+        bc[I * BS + j] = acc[j];
+        if (FIRST) rc[I * BS + j] = acc[j];
+    }
+    if (FIRST) {
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            float z = 0.0f;
+#pragma unroll
+            for (int i = 0; i < BS; ++i) z = fmaf(binv[(int64_t)(j * BS + i) * nnc + I], acc[i], z);
+            const float dn = c2 * z;
+            dc[I * BS + j] = dn;
+            xc[I * BS + j] = dn;
+        }
+    }
 }
 
 // e = P x_c on the fine rows (0 on fixed fine dofs), x_f += e; node_of_row
@@ -996,9 +1021,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     memset_d(ctx, stf.get(), 0, stf.bytes());
     DevBuf<int> bad(ctx, 1);
     memset_d(ctx, bad.get(), 0, sizeof(int));
-    hipLaunchKernelGGL(stencil_from_k_kernel<ND>, dim3(rows_grid(nn0)), dim3(kThreads), 0, s, K->bcolptr.get(),
-                       K->browidx.get(), K->bval.get(), nn0, K->transposed ? 1 : 0, S->free_of.get(), g0, stf.get(),
-                       bad.get());
+    hipLaunchKernelGGL(stencil_from_k_kernel<ND>, dim3(rows_grid(K->nnzb)), dim3(kThreads), 0, s, K->bcolidx.get(),
+                       K->browidx.get(), K->bval.get(), K->nnzb, nn0, S->free_of.get(), g0, stf.get(), bad.get());
     SFEM_LAUNCH_CHECK();
     int hbad = 0;
     d2h(ctx, &hbad, bad.get(), sizeof(int));
@@ -1116,18 +1140,19 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         const int64_t nv = S->sell_blocks * B2;
         DevBuf<unsigned long long> mx(ctx, 1);
         memset_d(ctx, mx.get(), 0, sizeof(unsigned long long));
-        hipLaunchKernelGGL(maxabs_kernel, dim3(1024), dim3(kThreads), 0, s, S->sval64.get(), nv, mx.get());
+        hipLaunchKernelGGL(maxdiag_kernel, dim3(256), dim3(kThreads), 0, s, S->dinv64.get(), S->nf, mx.get());
         SFEM_LAUNCH_CHECK();
         unsigned long long hm = 0;
         d2h(ctx, &hm, mx.get(), sizeof(hm));
         double amax;
         std::memcpy(&amax, &hm, sizeof(amax));
+        if (S->partial) amax = std::max(amax, 1.0);  // unit diagonal of fixed dofs in the block rows
         if (!(amax > 0.0) || !std::isfinite(amax)) fail(SFEM_ERR_SINGULAR, "multigrid: zero or non-finite operator");
         int e = 0;
         std::frexp(amax, &e);  // amax = f 2^e, f in [1/2, 1)
         H.scale0 = std::ldexp(1.0, -e);
         H.h0.reset(ctx, (size_t)nv);
-        hipLaunchKernelGGL(to_half_kernel, dim3(2048), dim3(kThreads), 0, s, S->sval64.get(), nv, H.scale0,
+        hipLaunchKernelGGL(to_half_kernel, dim3(2048), dim3(kThreads), 0, s, S->sval32.get(), nv, (float)H.scale0,
                            H.h0.get());
         SFEM_LAUNCH_CHECK();
         H.binv0.reset(ctx, (size_t)S->nfb * B2);
@@ -1245,7 +1270,9 @@ struct MgRun {
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next++ + 1], s));
     }
 
-    // x_l = V-cycle(b_l); level 0's right-hand side is the outer residual
+    // x_l = V-cycle(b_l); level 0's right-hand side is the outer residual.
+    // Levels below 0 start after the restriction that formed b_l, which also
+    // made their first smoothing step.
     void vcycle(int l, const double* r_outer) {
         MgLevel& L = H.L[(size_t)l];
         const int nl = (int)H.L.size();
@@ -1255,15 +1282,12 @@ struct MgRun {
             SFEM_LAUNCH_CHECK();
             return;
         }
-        const float* binv = l > 0 ? L.binv.get() : H.fp16 ? H.binv0.get() : S->binv32.get();
-        if (l == 0)
+        if (l == 0) {
+            const float* binv = H.fp16 ? H.binv0.get() : S->binv32.get();
             hipLaunchKernelGGL((cheb_first_kernel<BS, double>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, r_outer,
                                H.scale0, L.nrows, binv, L.c2[0], L.r.get(), L.x.get(), L.da.get(), done);
-        else
-            hipLaunchKernelGGL((cheb_first_kernel<BS, float>), dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s,
-                               (const float*)L.b.get(), 1.0f, L.nrows, binv, L.c2[0], L.r.get(), L.x.get(),
-                               L.da.get(), done);
-        SFEM_LAUNCH_CHECK();
+            SFEM_LAUNCH_CHECK();
+        }
         float* dp = L.da.get();
         float* dn = L.db.get();
         for (int k = 1; k < H.degree; ++k) {
@@ -1273,9 +1297,15 @@ struct MgRun {
         step(l, 1, dp, nullptr, nullptr, 0.0f, 0.0f);
         MgLevel& C = H.L[(size_t)l + 1];
         const Grid gf = grid_of(L), gc = grid_of(C);
-        hipLaunchKernelGGL(restrict_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, (const float*)L.r.get(),
-                           l == 0 ? (float)(1.0 / H.scale0) : 1.0f, gf, l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
-                           (const uint8_t*)C.fixed.get(), C.b.get(), done);
+        auto rk = [&](auto kern) {
+            hipLaunchKernelGGL(kern, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, (const float*)L.r.get(),
+                               l == 0 ? (float)(1.0 / H.scale0) : 1.0f, gf,
+                               l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
+                               (const uint8_t*)C.fixed.get(), C.b.get(), (const float*)C.binv.get(), C.c2[0],
+                               C.r.get(), C.x.get(), C.da.get(), done);
+        };
+        if (l + 1 < nl - 1) rk(restrict_kernel<ND, true>);
+        else rk(restrict_kernel<ND, false>);
         SFEM_LAUNCH_CHECK();
         vcycle(l + 1, nullptr);
         hipLaunchKernelGGL(prolong_kernel<ND>, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s,
@@ -1320,6 +1350,10 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     init.best_true = HUGE_VAL;
     init.need_true = 1;
     init.restart = 1;
+    if (o->x0_zero) {  // x = 0: the residual is rhs, no operator pass
+        SFEM_HIP(hipMemcpyAsync(r.get(), rhs, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        init.need_true = 0;
+    }
     h2d(ctx, dst.get(), &init, sizeof(init));
     MgState* st = dst.get();
     MgRun<ND, TV> run{ctx, S, H, s, ls, &st->done, ctx->tune_nt};
