@@ -155,6 +155,18 @@ def test_displacement_sweep_vs_oracle(ctx, precision):
             assert st["stop_reason"] == 1 and relerr(q, w) <= ERR
 
 
+def test_displacement_sweep_multigrid_vs_oracle(ctx):
+    """The sweep with the multigrid hierarchy of the 8^3 cylinder grid (K_bar
+    includes the surface penalty; the z-planes fix only u_z)."""
+    print("[case] sweep ne=8 multigrid", flush=True)
+    deltas = [0.001, 0.011, 0.3]
+    res = displacement_sweep(8, deltas, rtol=1e-12, preconditioner="multigrid", ctx=ctx)
+    want = oracle_sweep(8, deltas)
+    for q, w, st in zip(res.q, want, res.stats):
+        assert st["stop_reason"] == 1 and st["iterations"] <= 40
+        assert relerr(q, w) <= ERR
+
+
 def test_set_bc_matches_fresh_elimination(ctx):
     print("[case] set_bc vs fresh", flush=True)
     mesh, faces = meshgrid(0, 1, 0, 1, 0, 1, 3, 3, inflate=True, Young=40.0, nu=0.4, ctx=ctx)
