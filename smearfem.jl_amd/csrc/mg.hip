@@ -247,40 +247,50 @@ __global__ void galerkin_dir_kernel(const double* __restrict__ in, Grid gi, int 
 }
 
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
-// side), zero the rows and columns of fixed dofs (st64, the next Galerkin
-// product's input) and put a unit diagonal on them (st32, the smoother's
-// operator).
+// side), zero the rows and columns of fixed dofs (st64, slot-major, the next
+// Galerkin product's input) and put a unit diagonal on them (st32,
+// node-major, the smoother's operator).  A block handles TI consecutive
+// nodes: thread (s, node) writes st64 (coalesced along nodes), and the st32
+// tile goes through LDS so that it is written contiguously.
 template <int ND>
-__global__ void sym_mask_kernel(const double* __restrict__ raw, Grid g, const uint8_t* __restrict__ fixed,
-                                double* __restrict__ st64, float* __restrict__ st32) {
-    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+__global__ __launch_bounds__(kThreads) void sym_mask_kernel(const double* __restrict__ raw, Grid g,
+                                                            const uint8_t* __restrict__ fixed,
+                                                            double* __restrict__ st64, float* __restrict__ st32) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, TI = kThreads / NS;
+    __shared__ float tile[TI * NS * B2];
     const int64_t nn = g.n0 * g.n1 * g.n2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn * NS) return;
-    const int s = (int)(t / nn);
-    const int64_t I = t - (int64_t)s * nn;
-    int64_t c[3];
-    coords_of<ND>(I, g, c);
-    int dx, dy, dz;
-    offset_of<ND>(s, dx, dy, dz);
-    const bool valid = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
-    const int64_t J = valid ? node_at(g, c[0] + dx, c[1] + dy, c[2] + dz) : I;
-    const int sp = NS - 1 - s;
-    for (int a = 0; a < BS; ++a)
-        for (int b = 0; b < BS; ++b) {
-            double v = 0.0;
-            bool unit = false;
-            if (valid) {
-                const bool masked = fixed[I * BS + a] || fixed[J * BS + b];
-                v = masked ? 0.0
-                           : 0.5 * (raw[((int64_t)s * B2 + a * BS + b) * nn + I] +
-                                    raw[((int64_t)sp * B2 + b * BS + a) * nn + J]);
-                unit = masked && s == Sten<ND>::C && a == b;
-            }
-            const int64_t o = ((int64_t)s * B2 + a * BS + b) * nn + I;
-            st64[o] = v;
-            st32[o] = unit ? 1.0f : (float)v;
+    const int64_t I0 = (int64_t)blockIdx.x * TI;
+    const int t = threadIdx.x;
+    if (t < TI * NS) {
+        const int s = t / TI, li = t % TI;
+        const int64_t I = I0 + li;
+        if (I < nn) {
+            int64_t c[3];
+            coords_of<ND>(I, g, c);
+            int dx, dy, dz;
+            offset_of<ND>(s, dx, dy, dz);
+            const bool valid = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
+            const int64_t J = valid ? node_at(g, c[0] + dx, c[1] + dy, c[2] + dz) : I;
+            const int sp = NS - 1 - s;
+            for (int a = 0; a < BS; ++a)
+                for (int b = 0; b < BS; ++b) {
+                    double v = 0.0;
+                    bool unit = false;
+                    if (valid) {
+                        const bool masked = fixed[I * BS + a] || fixed[J * BS + b];
+                        v = masked ? 0.0
+                                   : 0.5 * (raw[((int64_t)s * B2 + a * BS + b) * nn + I] +
+                                            raw[((int64_t)sp * B2 + b * BS + a) * nn + J]);
+                        unit = masked && s == Sten<ND>::C && a == b;
+                    }
+                    st64[((int64_t)s * B2 + a * BS + b) * nn + I] = v;
+                    tile[(li * NS + s) * B2 + a * BS + b] = unit ? 1.0f : (float)v;
+                }
         }
+    }
+    __syncthreads();
+    const int64_t nt = (min(nn, I0 + TI) - I0) * NS * B2;
+    for (int64_t e = t; e < nt; e += kThreads) st32[I0 * NS * B2 + e] = tile[e];
 }
 
 template <int ND>
@@ -357,29 +367,46 @@ struct StOp {
     int64_t nn;
 };
 
+// Lanes per node in the coarse-level kernels (one per stencil slot).
 template <int ND>
-__device__ __forceinline__ void stencil_row(const StOp& op, int64_t node, const float* __restrict__ v,
-                                            float (&out)[ND]) {
-    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
-    int64_t c[3];
-    coords_of<ND>(node, op.g, c);
+struct Lanes {
+    static constexpr int G = ND == 3 ? 32 : 16;
+};
+
+// Row `node` of a coarse level's operator times v, computed by a group of G
+// lanes: lane s < NS multiplies slot s's block (node-major FP32 stencil,
+// val[(node*NS + s)*B2 + m], one contiguous run per node) by the neighbour's
+// values, then a fixed butterfly over the group sums the slots; the group's
+// first lane holds the result.  Every lane of the group must call it.
+template <int ND>
+__device__ __forceinline__ void stencil_row_group(const StOp& op, int64_t node, int s, const float* __restrict__ v,
+                                                  float (&out)[ND]) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, G = Lanes<ND>::G;
 #pragma unroll
     for (int j = 0; j < BS; ++j) out[j] = 0.0f;
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
+    if (s < NS && node < op.nn) {
+        int64_t c[3];
+        coords_of<ND>(node, op.g, c);
         int dx, dy, dz;
         offset_of<ND>(s, dx, dy, dz);
-        const bool ok = inside<ND>(op.g, c[0] + dx, c[1] + dy, c[2] + dz);
-        const int64_t m = ok ? node + dx + op.g.n0 * (dy + op.g.n1 * (int64_t)dz) : node;
-        float xv[BS];
+        if (inside<ND>(op.g, c[0] + dx, c[1] + dy, c[2] + dz)) {
+            const int64_t m = node + dx + op.g.n0 * (dy + op.g.n1 * (int64_t)dz);
+            const float* a = op.val + ((int64_t)node * NS + s) * B2;
+            float av[B2], xv[BS];
 #pragma unroll
-        for (int i = 0; i < BS; ++i) xv[i] = ok ? v[m * BS + i] : 0.0f;
+            for (int k = 0; k < B2; ++k) av[k] = a[k];
 #pragma unroll
-        for (int j = 0; j < BS; ++j)
+            for (int i = 0; i < BS; ++i) xv[i] = v[m * BS + i];
 #pragma unroll
-            for (int i = 0; i < BS; ++i)
-                out[j] = fmaf(op.val[((int64_t)s * B2 + j * BS + i) * op.nn + node], xv[i], out[j]);
+            for (int j = 0; j < BS; ++j)
+#pragma unroll
+                for (int i = 0; i < BS; ++i) out[j] = fmaf(av[j * BS + i], xv[i], out[j]);
+        }
     }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) out[j] += __shfl_xor(out[j], off, G);
 }
 
 // MODE 0: r -= A v; d_out = c1 d_in + c2 B^-1 r; x += d_out (a Chebyshev step)
@@ -431,12 +458,14 @@ __global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp op, const f
                                                                 float* __restrict__ x, const float* d_in,
                                                                 float* __restrict__ d_out, float c1, float c2,
                                                                 const float* __restrict__ binv, const int* done) {
+    constexpr int G = Lanes<ND>::G;
     if (*done) return;
-    const int64_t node = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (node >= op.nn) return;
+    const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t node = t / G;
+    const int s = (int)(t % G);
     float out[ND];
-    stencil_row<ND>(op, node, v, out);
-    cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+    stencil_row_group<ND>(op, node, s, v, out);
+    if (s == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
 }
 
 // First step from x = 0: r = b, d = c2 B^-1 b, x = d.  bscale: the units of
@@ -680,16 +709,21 @@ template <int ND>
 __global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp op, const float* __restrict__ v,
                                                                const float* __restrict__ binv, float* __restrict__ w,
                                                                PowState* ps, double* __restrict__ partials) {
+    constexpr int G = Lanes<ND>::G;
     __shared__ double sh[8];
     const float scale = (float)ps->inv;
-    double s = 0.0;
-    for (int64_t node = (int64_t)blockIdx.x * kThreads + threadIdx.x; node < op.nn;
-         node += (int64_t)gridDim.x * kThreads) {
+    double sum = 0.0;
+    // every lane of a group runs the same trip count: the stride and the
+    // bound are multiples of G
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < op.nn * G;
+         t += (int64_t)gridDim.x * kThreads) {
+        const int64_t node = t / G;
+        const int s = (int)(t % G);
         float out[ND];
-        stencil_row<ND>(op, node, v, out);
-        s += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
+        stencil_row_group<ND>(op, node, s, v, out);
+        if (s == 0) sum += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
     }
-    const double b = block_sum_256(s, sh);
+    const double b = block_sum_256(sum, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
     if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
 }
@@ -1083,7 +1117,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         }
         C.st64.reset(ctx, (size_t)C.nn * NS * B2);
         C.st32.reset(ctx, (size_t)C.nn * NS * B2);
-        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn * NS)), dim3(kThreads), 0, s, raw.get(), gc,
+        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3((unsigned)((C.nn + kThreads / NS - 1) / (kThreads / NS))),
+                           dim3(kThreads), 0, s, raw.get(), gc,
                            C.fixed.get(), C.st64.get(), C.st32.get());
         SFEM_LAUNCH_CHECK();
         C.binv.reset(ctx, (size_t)C.nn * B2);
@@ -1180,7 +1215,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     const Launch ls = plan_sell(S->nslices);
     auto init_grid = [&](int l) { return std::min<unsigned>(rows_grid(H.L[(size_t)l].nrows * BS), 1024); };
     unsigned maxgrid = ls.grid;
-    for (int l = 0; l < nl; ++l) maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn), init_grid(l)});
+    for (int l = 0; l < nl; ++l)
+        maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn * Lanes<ND>::G), init_grid(l)});
     DevBuf<double> partials(ctx, (size_t)maxgrid);
     for (int l = 0; l + 1 < nl; ++l) {
         MgLevel& v = H.L[l];
@@ -1204,7 +1240,9 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
                                    S->binv32.get(), vout, ps.get(), partials.get());
             } else {
                 const StOp op{v.st32.get(), grid_of(v), v.nn};
-                hipLaunchKernelGGL(pow_stencil_kernel<ND>, dim3(rows_grid(v.nn)), dim3(kThreads), 0, s, op, vin,
+                // grid-stride over the nodes: few blocks meet at the last-block count
+                hipLaunchKernelGGL(pow_stencil_kernel<ND>, dim3(std::min(rows_grid(v.nn * Lanes<ND>::G), 1024u)),
+                                   dim3(kThreads), 0, s, op, vin,
                                    v.binv.get(), vout, ps.get() + l, partials.get());
             }
             SFEM_LAUNCH_CHECK();
@@ -1260,8 +1298,8 @@ struct MgRun {
         } else {
             const StOp op{L.st32.get(), grid_of(L), L.nn};
             auto go = [&](auto k) {
-                hipLaunchKernelGGL(k, dim3(rows_grid(L.nn)), dim3(kThreads), 0, s, op, v, L.r.get(), L.x.get(), d_in,
-                                   d_out, c1, c2, (const float*)L.binv.get(), done);
+                hipLaunchKernelGGL(k, dim3(rows_grid(L.nn * Lanes<ND>::G)), dim3(kThreads), 0, s, op, v, L.r.get(),
+                                   L.x.get(), d_in, d_out, c1, c2, (const float*)L.binv.get(), done);
             };
             if (mode == 0) go(cheb_stencil_kernel<ND, 0>);
             else go(cheb_stencil_kernel<ND, 1>);
