@@ -46,7 +46,9 @@ ERR_BAR = 1e-8
 # HBM bytes per CG-SpMV launch on this workload, from rocprofv3 FETCH_SIZE and
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.00, write
 # x1.00); profiles/traffic_spmv_cg_hex8_256x64x64.md has the commands and data.
-MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9, "split/bsr": 1.7175e9}
+MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9, "split/bsr": 1.7175e9,
+                    # finest-level multigrid smoothing step (FP16 operator), profiles/traffic_mg_smooth_hex8_256x64x64.md
+                    "mg-smooth": 7.4977e8}
 
 
 def log(msg):
@@ -478,7 +480,8 @@ def main():
     if is_mg:
         # dominant kernel of a multigrid iteration: the finest-level Chebyshev
         # smoothing step (3 of the 5 finest-level operator passes)
-        rl = {"kernel": "cheb_sell_kernel (finest-level smoothing step, FP32 node-block operator)",
+        rl = {"kernel": "cheb_sell_kernel (finest-level Chebyshev smoothing step: FP16 node-block operator, "
+                        "FP32 vectors)",
               "achieved": best["smooth_GBs"], "bytes_per_launch": best["smooth_bytes"],
               "avg_launch_ms": best["smooth_avg_ms"], "launches_timed": best["smooth_timed"],
               "traffic": MEASURED_TRAFFIC.get("mg-smooth")}
@@ -497,7 +500,8 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": dtype[best_name.split("/")[0]] + (" (f32 multigrid V-cycle)" if is_mg else ""),
+        "dtype": dtype[best_name.split("/")[0]] + (" (multigrid V-cycle: f16 finest-level operator, f32 coarse "
+                                                   "levels and vectors)" if is_mg else ""),
         "data": "synthetic",
         "config": {
             "workload": f"hex8-{DIMS[0]}x{DIMS[1]}x{DIMS[2]}-cantilever assemble+eliminate+PCG {best_name}",
