@@ -26,6 +26,8 @@
  *   sfem_bc_planes            <- setboundaryCond            examples/vector3D.jl:133-173
  *   sfem_system_set_bc / sfem_system_solve_many
  *                             <- the displacement sweep     examples/vector3D.jl:310-322
+ *   sfem_system_set_grid      <- no counterpart: the multigrid preconditioner of the solve that
+ *                                replaces inv(Matrix(K_free)) (examples/vector3D.jl:318)
  *
  * Threading: a context owns one HIP stream.  Calls on one context are not
  * re-entrant; different contexts may be used from different threads.

@@ -153,20 +153,51 @@ end
 
 precision_code(p::Symbol) = p === :fp64 ? 0 : p === :mixed ? 1 : p === :fp32_matrix ? 2 : p === :split ? 3 :
                             throw(ArgumentError("unknown precision $p"))
-preconditioner_code(p::Symbol) = p === :jacobi ? 0 : p === :block_jacobi ? 1 :
+preconditioner_code(p::Symbol) = p === :jacobi ? 0 : p === :block_jacobi ? 1 : p === :multigrid ? 2 :
                                  throw(ArgumentError("unknown preconditioner $p"))
+
+# the one-shot solve of sfem_solve, split so that the multigrid hierarchy of a
+# structured grid (dims elements per direction) can be built in between
+function solve_with_grid(ctx, Kp, fv, mask, qd, opts, dims)
+    n = length(fv)
+    bc, sys = Ref{Ptr{Cvoid}}(C_NULL), Ref{Ptr{Cvoid}}(C_NULL)
+    stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
+    q = zeros(n)
+    d = Int64[dims...]
+    try
+        check(ctx, ccall((:sfem_bc_create, libsfem), Cint,
+                         (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                         ctx, n, fv, mask, qd, bc))
+        check(ctx, ccall((:sfem_eliminate, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
+                         ctx, Kp, bc[], sys))
+        check(ctx, ccall((:sfem_system_set_grid, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}),
+                         ctx, sys[], length(d), d, C_NULL))
+        check(ctx, ccall((:sfem_system_solve, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}),
+                         ctx, sys[], opts, stats))
+        check(ctx, ccall((:sfem_system_solution, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
+                         ctx, sys[], q, C_NULL))
+    finally
+        sys[] != C_NULL && ccall((:sfem_system_destroy, libsfem), Cvoid, (Ptr{Cvoid},), sys[])
+        bc[] != C_NULL && ccall((:sfem_bc_destroy, libsfem), Cvoid, (Ptr{Cvoid},), bc[])
+    end
+    return q
+end
 
 """
     assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
-                       precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000)
+                       precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000,
+                       grid=nothing)
 
 assemble_system (src/fem.jl:135) and the example's elimination and solve
 (examples/vector3D.jl:314-322) in one call; K stays on the device, so the
 node-block operator formats (split precision, block-Jacobi) are available when
-ID is node-interleaved and every node is fully fixed or fully free.
+ID is node-interleaved.  `preconditioner=:multigrid` with `grid=(nx, ny, nz)`
+(elements per direction of the meshgrid numbering the nodes follow) uses the
+geometric multigrid preconditioner (precision :fp64 or :split).
 """
 function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
-                            precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000)
+                            precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000,
+                            grid=nothing)
     ctx = context()
     nodes = Matrix{Float64}(NodeList)
     ien = Matrix{Int64}(IEN)
@@ -189,6 +220,14 @@ function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Yo
     qd = q_d === nothing ? zeros(n) : Vector{Float64}(vec(q_d))
     opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1,
                      preconditioner_code(preconditioner))
+    if preconditioner === :multigrid
+        grid === nothing && throw(ArgumentError("multigrid needs grid = elements per direction"))
+        try
+            return solve_with_grid(ctx, Kp[], fv, mask, qd, opts, grid)
+        finally
+            ccall((:sfem_csr_destroy, libsfem), Cvoid, (Ptr{Cvoid},), Kp[])
+        end
+    end
     stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
     q = zeros(n)
     try
@@ -241,16 +280,20 @@ struct ColumnStats
 end
 
 """
-    displacement_sweep(ne, deltas; Young=40, ν=0.4, β=100, rtol=1e-10)
+    displacement_sweep(ne, deltas; Young=40, ν=0.4, β=100, rtol=1e-10, multigrid=iseven(ne))
 
 The example's solve loop (examples/vector3D.jl:270-322) on the GPU: the unit
 cube meshed with ne^3 Hex8 and inflated to the cylinder (K and b both on it,
 as inflate_sphere overwrites NodeList there), K_bar = K + β b, u_z = 0 on
 z = 0 and u_z = -d on z = 1, q(d) = q_d + C K_free^-1 C'(-K_bar q_d) for every d.
-The displacements are solved together, up to 64 per pass over K_free.
-Returns a vector of q (3 (ne+1)^3 values each, node-interleaved as ID).
+With `multigrid` the displacements are solved one after the other with the
+geometric multigrid preconditioner of the ne^3 grid (built once; each solve
+starts from the previous solution); otherwise together, up to 64 per pass
+over K_free.  Returns a vector of q (3 (ne+1)^3 values each, node-interleaved
+as ID).
 """
-function displacement_sweep(ne, deltas; Young=40.0, ν=0.4, β=100.0, rtol=1e-10, max_iter=200000)
+function displacement_sweep(ne, deltas; Young=40.0, ν=0.4, β=100.0, rtol=1e-10, max_iter=200000,
+                            multigrid=iseven(ne))
     ctx = context()
     mesh, faces, K, b, Kbar, bc, sys = (Ref{Ptr{Cvoid}}(C_NULL) for _ in 1:7)
     dims = Int64[ne, ne, ne]
@@ -277,6 +320,36 @@ function displacement_sweep(ne, deltas; Young=40.0, ν=0.4, β=100.0, rtol=1e-10
                          ctx, mesh[], n, 2, axis, coord, dof, value, C_NULL, bc))
         check(ctx, ccall((:sfem_eliminate, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
                          ctx, Kbar[], bc[], sys))
+        if multigrid
+            check(ctx, ccall((:sfem_system_set_grid, libsfem), Cint,
+                             (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}), ctx, sys[], 3, dims, C_NULL))
+            qs = Vector{Vector{Float64}}()
+            stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
+            for (i, d) in enumerate(deltas)
+                if i > 1
+                    bcd = Ref{Ptr{Cvoid}}(C_NULL)
+                    value[2] = -d
+                    check(ctx, ccall((:sfem_bc_planes, libsfem), Cint,
+                                     (Ptr{Cvoid}, Ptr{Cvoid}, Int64, Cint, Ptr{Cint}, Ptr{Float64}, Ptr{Cint},
+                                      Ptr{Float64}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
+                                     ctx, mesh[], n, 2, axis, coord, dof, value, C_NULL, bcd))
+                    try
+                        check(ctx, ccall((:sfem_system_set_bc, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}),
+                                         ctx, sys[], bcd[]))
+                    finally
+                        ccall((:sfem_bc_destroy, libsfem), Cvoid, (Ptr{Cvoid},), bcd[])
+                    end
+                end
+                mgopts = SolveOpts(0, 0, rtol, max_iter, 0.1, 8, 0, i == 1 ? 1 : 0, 2)
+                check(ctx, ccall((:sfem_system_solve, libsfem), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}), ctx, sys[], mgopts, stats))
+                q = zeros(n)
+                check(ctx, ccall((:sfem_system_solution, libsfem), Cint,
+                                 (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}), ctx, sys[], q, C_NULL))
+                push!(qs, q)
+            end
+            return qs
+        end
         # node z coordinates of the generated mesh (x fastest, then y, then z)
         z = [Float64(k) / ne for k in 0:ne for j in 0:ne for i in 0:ne]
         z[end - (ne + 1)^2 + 1:end] .= 1.0
