@@ -419,7 +419,14 @@ void sfem_batch_destroy(sfem_batch* b);
  * iters_per_step iterations of every active request in one persistent
  * launch, and retires the finished ones (solution copied to the host, device
  * memory released).  A request's result does not depend on what else is
- * active: it is bitwise the result it gets alone. */
+ * active: it is bitwise the result it gets alone.
+ * With preconditioner = SFEM_PC_MULTIGRID, requests submitted with their
+ * grid (sfem_server_submit_grid) are solved run-to-completion instead: each
+ * step takes the oldest such queued request, assembles, eliminates, builds
+ * its multigrid hierarchy and solves it (FP64 CG, sfem_system_set_grid),
+ * besides one persistent launch for the other active requests.  A grid that
+ * cannot be coarsened, and requests without a grid, go to the persistent
+ * launch with node-block Jacobi. */
 typedef struct sfem_server sfem_server;
 
 typedef struct {
@@ -447,8 +454,15 @@ int  sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server
 int  sfem_server_submit(sfem_server* s, int ndim, const double* node_list, int64_t n_nodes,
                         const int64_t* ien, int64_t n_elem, const double* young, const double* nu,
                         const double* f, const uint8_t* fixed, const double* q_d, int64_t* id);
+/* sfem_server_submit for a mesh whose nodes are the structured grid of the
+ * meshgrid numbering (examples/vector3D.jl:67-101) with dims[d] elements
+ * along direction d. */
+int  sfem_server_submit_grid(sfem_server* s, int ndim, const int64_t* dims, const double* node_list,
+                             int64_t n_nodes, const int64_t* ien, int64_t n_elem, const double* young,
+                             const double* nu, const double* f, const uint8_t* fixed, const double* q_d,
+                             int64_t* id);
 /* One admit / iterate / retire step.  Outputs (may be NULL): requests that
- * finished in this step, and device ms of the step's solver launch. */
+ * finished in this step, and device ms of the step's solver work. */
 int  sfem_server_step(sfem_server* s, int* n_finished, double* launch_ms);
 /* State of a request (SFEM_REQ_*), its stats once done, and host times in
  * ms since the server was created: [submitted, admitted, finished]. */

@@ -121,6 +121,25 @@ __host__ __device__ __forceinline__ void offset_of(int s, int& dx, int& dy, int&
     dz = ND == 3 ? s / 9 - 1 : 0;
 }
 
+// Nested coarsening along one direction with e fine elements: coarse node c
+// sits on fine node min(2c, e), so e elements become ceil(e/2) (for odd e the
+// last coarse element spans one fine element).  child() gives the k-th
+// (k = 0, 1, 2) fine node of coarse node c's interpolation support and its
+// weight: 2c-1, 2c, 2c+1 with 1/2, 1, 1/2, except that for odd e the last fine
+// node belongs to the last coarse node alone (weight 1).
+__host__ __device__ __forceinline__ bool child(int64_t c, int k, int64_t e, int64_t& i, double& w) {
+    if ((e & 1) && c == (e + 1) / 2) {
+        i = e;
+        w = 1.0;
+        return k == 1;
+    }
+    const int a = k - 1;
+    i = 2 * c + a;
+    if (i < 0 || i > e || ((e & 1) && i == e)) return false;
+    w = a ? 0.5 : 1.0;
+    return true;
+}
+
 // ------------------------------------------------------- hierarchy --
 // Level-0 maps: a node with any free dof is a block row (rank in node order,
 // as the elimination numbers them).
@@ -193,18 +212,21 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
     if (I >= gc.n0 * gc.n1 * gc.n2) return;
     int64_t c[3];
     coords_of<ND>(I, gc, c);
-    const int64_t fn = node_at(gf, f0 * c[0], f1 * c[1], f2 * c[2]);
+    // coarse node c sits on fine node min(2c, e) along a coarsened direction
+    const int64_t x = f0 == 2 ? min(2 * c[0], gf.n0 - 1) : c[0];
+    const int64_t y = f1 == 2 ? min(2 * c[1], gf.n1 - 1) : c[1];
+    const int64_t z = f2 == 2 ? min(2 * c[2], gf.n2 - 1) : c[2];
+    const int64_t fn = node_at(gf, x, y, z);
     for (int a = 0; a < BS; ++a) cfix[I * BS + a] = ffix[fn * BS + a];
 }
 
 // Galerkin product along one direction: P = P_x (x) P_y (x) P_z, so P'AP
-// is three one-direction products, each coarsening direction d by 2 and
-// keeping the others.  One thread per (output node I, slot s):
-// A'(I, I+D)[a][b] = sum over the pairs (alpha, beta) of direction d of
-// w(alpha) w(beta) A(i, i+o)[a][b], with i = I except i_d = 2 I_d + alpha,
-// o = D except o_d = 2 (I_d + D_d) + beta - i_d (pairs with |o_d| > 1 or a
-// node off the grid add nothing); alpha, beta in {-1, 0, 1}, w(0) = 1,
-// w(+-1) = 1/2, summed in a fixed order.
+// is three one-direction products, each coarsening direction d (child())
+// and keeping the others.  One thread per (output node I, slot s):
+// A'(I, I+D)[a][b] = sum over the support pairs (i_d of I_d, j_d of I_d+D_d)
+// of w_i w_j A(i, i+o)[a][b], with i = I except along d, o = D except
+// o_d = j_d - i_d (pairs more than one apart add nothing), summed in a
+// fixed order.
 template <int ND>
 __global__ void galerkin_dir_kernel(const double* __restrict__ in, Grid gi, int d, Grid go,
                                     double* __restrict__ out) {
@@ -222,19 +244,22 @@ __global__ void galerkin_dir_kernel(const double* __restrict__ in, Grid gi, int 
 #pragma unroll
     for (int m = 0; m < B2; ++m) acc[m] = 0.0;
     if (inside<ND>(go, c[0] + D[0], c[1] + D[1], c[2] + D[2])) {
-        const int64_t nid = d == 0 ? gi.n0 : d == 1 ? gi.n1 : gi.n2;
-        for (int a = -1; a <= 1; ++a) {
-            const int64_t ia = 2 * c[d] + a;
-            if (ia < 0 || ia >= nid) continue;
-            for (int b = -1; b <= 1; ++b) {
-                const int64_t jb = 2 * (c[d] + D[d]) + b;
+        const int64_t e = (d == 0 ? gi.n0 : d == 1 ? gi.n1 : gi.n2) - 1;
+        for (int ka = 0; ka < 3; ++ka) {
+            int64_t ia;
+            double wa;
+            if (!child(c[d], ka, e, ia, wa)) continue;
+            for (int kb = 0; kb < 3; ++kb) {
+                int64_t jb;
+                double wb;
+                if (!child(c[d] + D[d], kb, e, jb, wb)) continue;
                 const int64_t o = jb - ia;
-                if (jb < 0 || jb >= nid || o < -1 || o > 1) continue;
+                if (o < -1 || o > 1) continue;
                 int64_t ic[3] = {c[0], c[1], c[2]};
                 ic[d] = ia;
                 int od[3] = {D[0], D[1], D[2]};
                 od[d] = (int)o;
-                const double w = (a ? 0.5 : 1.0) * (b ? 0.5 : 1.0);
+                const double w = wa * wb;
                 const int sf = slot_of<ND>(od[0], od[1], od[2]);
                 const double* src = in + (int64_t)sf * B2 * nni + node_at(gi, ic[0], ic[1], ic[2]);
 #pragma unroll
@@ -512,24 +537,41 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     if (I >= gc.n0 * gc.n1 * gc.n2) return;
     int64_t ci[3];
     coords_of<ND>(I, gc, ci);
+    // the interpolation support of I per direction (child(); the identity
+    // along directions that are not coarsened)
     const int f[3] = {f0, f1, f2};
-    int lo[3], hi[3];
+    const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
+    int64_t fi[3][3];
+    float fw[3][3];
+    int nk[3];
     for (int d = 0; d < 3; ++d) {
-        lo[d] = f[d] == 2 ? -1 : 0;
-        hi[d] = f[d] == 2 ? 1 : 0;
+        nk[d] = 0;
+        if (f[d] != 2) {
+            fi[d][0] = ci[d];
+            fw[d][0] = 1.0f;
+            nk[d] = 1;
+            continue;
+        }
+        for (int k = 0; k < 3; ++k) {
+            int64_t i;
+            double w;
+            if (child(ci[d], k, ef[d], i, w)) {
+                fi[d][nk[d]] = i;
+                fw[d][nk[d]] = (float)w;
+                ++nk[d];
+            }
+        }
     }
     float acc[BS];
 #pragma unroll
     for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
-    for (int az = lo[2]; az <= hi[2]; ++az)
-        for (int ay = lo[1]; ay <= hi[1]; ++ay)
-            for (int ax = lo[0]; ax <= hi[0]; ++ax) {
-                const int64_t ix = f0 * ci[0] + ax, iy = f1 * ci[1] + ay, iz = f2 * ci[2] + az;
-                if (!inside<ND>(gf, ix, iy, iz)) continue;
-                const int64_t fn = node_at(gf, ix, iy, iz);
+    for (int kz = 0; kz < nk[2]; ++kz)
+        for (int ky = 0; ky < nk[1]; ++ky)
+            for (int kx = 0; kx < nk[0]; ++kx) {
+                const int64_t fn = node_at(gf, fi[0][kx], fi[1][ky], fi[2][kz]);
                 const int64_t row = row_of_node ? (int64_t)row_of_node[fn] : fn;
                 if (row < 0) continue;
-                const float w = (ax ? 0.5f : 1.0f) * (ay ? 0.5f : 1.0f) * (az ? 0.5f : 1.0f);
+                const float w = fw[0][kx] * fw[1][ky] * fw[2][kz];
 #pragma unroll
                 for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
             }
@@ -572,8 +614,13 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
     int64_t p[3][2];
     float w[3][2];
     int np[3];
+    const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
     for (int d = 0; d < 3; ++d) {
-        if (f[d] == 2 && (cf[d] & 1)) {
+        if (f[d] == 2 && (ef[d] & 1) && cf[d] == ef[d]) {  // odd e: the last node has one parent
+            p[d][0] = p[d][1] = (ef[d] + 1) / 2;
+            w[d][0] = w[d][1] = 1.0f;
+            np[d] = 1;
+        } else if (f[d] == 2 && (cf[d] & 1)) {
             p[d][0] = (cf[d] - 1) / 2;
             p[d][1] = p[d][0] + 1;
             w[d][0] = w[d][1] = 0.5f;
@@ -1070,8 +1117,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         int f[3] = {1, 1, 1};
         bool any = false;
         for (int d = 0; d < ND; ++d) {
-            const int64_t e = P.n[d] - 1;
-            if (e >= 2 && e % 2 == 0) {
+            if (P.n[d] - 1 >= 2) {  // any element count from 2 halves (rounded up)
                 f[d] = 2;
                 any = true;
             }
@@ -1081,7 +1127,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         MgLevel C;
         for (int d = 0; d < 3; ++d) {
             C.f[d] = f[d];
-            C.n[d] = d < ND ? (P.n[d] - 1) / f[d] + 1 : 1;
+            C.n[d] = d < ND ? (f[d] == 2 ? P.n[d] / 2 + 1 : P.n[d]) : 1;  // ceil(e / 2) elements
         }
         C.nn = C.n[0] * C.n[1] * C.n[2];
         C.nrows = C.nn;
@@ -1101,7 +1147,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             if (f[d] != 2) continue;
             Grid gn = gcur;
             int64_t* nd = d == 0 ? &gn.n0 : d == 1 ? &gn.n1 : &gn.n2;
-            *nd = (*nd - 1) / 2 + 1;
+            *nd = *nd / 2 + 1;  // e = n - 1 elements -> ceil(e / 2)
             const int64_t nno = gn.n0 * gn.n1 * gn.n2;
             DevBuf<double> o(ctx, (size_t)nno * NS * B2);
             hipLaunchKernelGGL(galerkin_dir_kernel<ND>, dim3(rows_grid(nno * NS)), dim3(kThreads), 0, s, cur, gcur, d,
@@ -1133,12 +1179,12 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     sync(ctx);
     stf.release();
     if (H.L.size() < 2)
-        fail(SFEM_ERR_UNSUPPORTED, "multigrid: no direction has an even element count to coarsen");
+        fail(SFEM_ERR_UNSUPPORTED, "multigrid: no direction has 2 or more elements to coarsen");
     MgLevel& Lc = H.L.back();
     if (Lc.nn > kMgMaxCoarseNodes)
         fail(SFEM_ERR_UNSUPPORTED, "multigrid: the grid coarsens only to " + std::to_string(Lc.nn) +
                                        " nodes (at most " + std::to_string(kMgMaxCoarseNodes) +
-                                       "); element counts need more factors of 2");
+                                       ")");
 
     // coarsest: dense inverse on the host
     {

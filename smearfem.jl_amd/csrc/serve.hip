@@ -14,6 +14,10 @@
 // precision (per-problem reductions over its own items, state in memory), so
 // its result is bitwise the same alone or in any mix.
 //
+// Multigrid servers solve requests that come with their grid one at a time
+// to completion (mg.hip): the oldest such queued request per step, besides
+// the persistent launch; a grid that cannot be coarsened joins the launch.
+//
 // No counterpart in the reference (one blocking solve per call there,
 // examples/vector3D.jl:314-322).
 #include <algorithm>
@@ -33,6 +37,8 @@ struct Request {
     int64_t id = 0;
     int state = SFEM_REQ_QUEUED;
     int ndim = 0;
+    bool grid = false;      // nodes form the structured grid of dims
+    int64_t dims[3] = {0, 0, 0};
     int64_t n_nodes = 0, n_elem = 0;
     std::vector<double> nodes, young, nu, f, qd;
     std::vector<int64_t> ien;
@@ -52,7 +58,9 @@ struct sfem_server {
     sfem_ctx* ctx = nullptr;
     sfem_server_opts o{};
     std::map<int64_t, Request> reqs;
-    std::deque<int64_t> queue;
+    std::deque<int64_t> queue;   // requests for the persistent launch
+    std::deque<int64_t> gqueue;  // multigrid servers: requests solved one at a time
+    int table_pc = SFEM_PC_JACOBI;
     std::vector<int64_t> active;
     std::vector<int> free_slots;
     sfem::DevBuf<sfem::ProbState> pool;
@@ -191,10 +199,123 @@ void retire(sfem_server* S, int* n_finished) {
     S->active.swap(still);
 }
 
+// One request on its own: assemble, eliminate, multigrid hierarchy, FP64 CG.
+// Returns false (request unchanged, its inputs kept) when its grid cannot be
+// coarsened, so that it can join the persistent launch instead.
+bool solve_grid_request(sfem_server* S, Request& r, double* ms) {
+    sfem_ctx* ctx = S->ctx;
+    const int nd = r.ndim, nen = 1 << nd;
+    const int64_t N = r.n_nodes, E = r.n_elem, n = N * nd;
+    sfem_mesh* mesh = nullptr;
+    sfem_csr* K = nullptr;
+    sfem_bc* bc = nullptr;
+    sfem_system* sys = nullptr;
+    auto release = [&] {
+        if (sys) sfem_system_destroy(sys);
+        if (bc) sfem_bc_destroy(bc);
+        if (K) sfem_csr_destroy(K);
+        if (mesh) sfem_mesh_destroy(mesh);
+    };
+    auto ok = [&](int st) {
+        if (st) fail(st, ctx->last_error);
+    };
+    bool coarsens = true;
+    const auto t0 = std::chrono::steady_clock::now();
+    try {
+        std::vector<int64_t> gid((size_t)n);
+        for (int64_t m = 0; m < N; ++m)
+            for (int l = 0; l < nd; ++l) gid[(size_t)((int64_t)l * N + m)] = (int64_t)nd * m + l + 1;
+        ok(sfem_mesh_create(ctx, nd, SFEM_Q1, nd, r.nodes.data(), N, r.ien.data(), E, nen, E, gid.data(), N,
+                            r.young.data(), E, 0.3, &mesh));
+        mesh->nu_e.reset(ctx, (size_t)E);
+        h2d(ctx, mesh->nu_e.get(), r.nu.data(), sizeof(double) * E);
+        ok(sfem_assemble_mesh(ctx, mesh, SFEM_ASM_DEFAULT, &K));
+        if (K->n != n || K->bs != nd) fail(SFEM_ERR_ARGUMENT, "server: every node must belong to an element");
+        ok(sfem_bc_create(ctx, n, r.f.empty() ? nullptr : r.f.data(), r.fixed.data(),
+                          r.qd.empty() ? nullptr : r.qd.data(), &bc));
+        ok(sfem_eliminate(ctx, K, bc, &sys));
+        try {
+            system_set_grid(ctx, sys, nd, r.dims, nullptr);
+        } catch (const Error& e) {
+            if (e.code != SFEM_ERR_UNSUPPORTED) throw;
+            coarsens = false;
+        }
+        if (coarsens) {
+            sfem_solve_opts o;
+            sfem_solve_opts_default(&o);
+            o.precision = SFEM_PREC_FP64;
+            o.format = SFEM_FMT_BSR;
+            o.rtol = S->o.rtol;
+            o.max_iter = S->o.max_iter;
+            o.x0_zero = 1;
+            o.preconditioner = SFEM_PC_MULTIGRID;
+            sfem_solve_stats st{};
+            try {
+                system_solve(ctx, sys, &o, &st);
+            } catch (const Error& e) {
+                if (e.code != SFEM_ERR_NOT_CONVERGED) throw;
+            }
+            r.result.resize((size_t)n);
+            system_solution(ctx, sys, r.result.data(), nullptr);
+            r.stats.iterations = st.iterations;
+            r.stats.fp64_updates = st.fp64_updates;
+            r.stats.rel_residual = st.rel_residual;
+            r.stats.n_free = sys->nf;
+            r.stats.stop_reason = st.stop_reason;
+            r.stats.floor_residual = st.floor_residual;
+            r.state = st.converged ? SFEM_REQ_DONE : SFEM_REQ_FAILED;
+            if (!st.converged) r.error = "request did not converge (multigrid CG)";
+        }
+    } catch (const Error& e) {
+        release();
+        r.state = SFEM_REQ_FAILED;
+        r.error = e.msg;
+        ctx->last_error = e.msg;
+        r.t_fin = S->now_ms();
+        return true;
+    }
+    release();
+    if (!coarsens) return false;
+    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    r.t_fin = S->now_ms();
+    std::vector<double>().swap(r.nodes);
+    std::vector<int64_t>().swap(r.ien);
+    return true;
+}
+
+// The oldest grid request that fits the budget beside the active ones.
+void step_grid(sfem_server* S, int* n_finished, double* ms) {
+    while (!S->gqueue.empty()) {
+        Request& r = S->reqs.at(S->gqueue.front());
+        // the request's build and hierarchy are held only while it is solved
+        const int64_t need = r.est_transient + r.est_transient / 2;
+        if (S->bytes_in_use + need > S->o.hbm_budget) {
+            if (!S->active.empty()) return;  // wait for memory to free up
+            r.state = SFEM_REQ_FAILED;
+            r.error = "request does not fit the HBM budget";
+            r.t_fin = S->now_ms();
+            S->gqueue.pop_front();
+            continue;
+        }
+        S->gqueue.pop_front();
+        r.t_adm = S->now_ms();
+        if (!solve_grid_request(S, r, ms)) {  // no multigrid hierarchy: the persistent launch
+            r.t_adm = 0;
+            S->queue.push_back(r.id);
+            continue;
+        }
+        if (r.state == SFEM_REQ_DONE && n_finished) ++*n_finished;
+        return;
+    }
+}
+
 void step(sfem_server* S, int* n_finished, double* ms) {
     sfem_ctx* ctx = S->ctx;
     if (n_finished) *n_finished = 0;
     if (ms) *ms = 0;
+    double grid_ms = 0;
+    if (S->o.preconditioner == SFEM_PC_MULTIGRID) step_grid(S, n_finished, &grid_ms);
+    if (ms) *ms = grid_ms;
     admit(S);
     if (S->active.empty()) return;
     std::vector<ProbDesc> descs;
@@ -209,7 +330,7 @@ void step(sfem_server* S, int* n_finished, double* ms) {
             w.prob = (int32_t)descs.size();
             items.push_back(w);
         }
-        descs.push_back(batch_desc(r.B, S->o.precision, S->o.preconditioner, S->pool.get() + r.slot, base,
+        descs.push_back(batch_desc(r.B, S->o.precision, S->table_pc, S->pool.get() + r.slot, base,
                                    (int)items.size()));
     }
     if (S->ddesc.size() < descs.size()) S->ddesc.reset(ctx, descs.size() * 2);
@@ -232,7 +353,7 @@ void step(sfem_server* S, int* n_finished, double* ms) {
     table_launch(ctx, S->o.precision, a, S->grid);
     SFEM_HIP(hipEventRecord(S->e1, ctx->stream));
     d2h(ctx, S->hpool.data(), S->pool.get(), sizeof(ProbState) * S->hpool.size());
-    if (ms) *ms = launch_ms(S);
+    if (ms) *ms = grid_ms + launch_ms(S);
     retire(S, n_finished);
 }
 
@@ -270,8 +391,10 @@ int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server*
             fail(SFEM_ERR_ARGUMENT, "server_create: max_active, iters_per_step, item_slices and hbm_budget must be > 0");
         if (S->o.precision < SFEM_PREC_FP64 || S->o.precision > SFEM_PREC_SPLIT)
             fail(SFEM_ERR_ARGUMENT, "server_create: unknown precision");
-        if (S->o.preconditioner != SFEM_PC_JACOBI && S->o.preconditioner != SFEM_PC_BLOCK_JACOBI)
+        if (S->o.preconditioner != SFEM_PC_JACOBI && S->o.preconditioner != SFEM_PC_BLOCK_JACOBI &&
+            S->o.preconditioner != SFEM_PC_MULTIGRID)
             fail(SFEM_ERR_ARGUMENT, "server_create: unknown preconditioner");
+        S->table_pc = S->o.preconditioner == SFEM_PC_MULTIGRID ? SFEM_PC_BLOCK_JACOBI : S->o.preconditioner;
         S->ctx = ctx;
         S->grid = table_grid(ctx, S->o.precision, S->o.blocks_per_cu);
         S->pool.reset(ctx, (size_t)S->o.max_active);
@@ -287,18 +410,29 @@ int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server*
     return SFEM_OK;
 }
 
-int sfem_server_submit(sfem_server* S, int ndim, const double* node_list, int64_t n_nodes, const int64_t* ien,
-                       int64_t n_elem, const double* young, const double* nu, const double* f, const uint8_t* fixed,
-                       const double* q_d, int64_t* id) {
+int sfem_server_submit_grid(sfem_server* S, int ndim, const int64_t* dims, const double* node_list, int64_t n_nodes,
+                            const int64_t* ien, int64_t n_elem, const double* young, const double* nu,
+                            const double* f, const uint8_t* fixed, const double* q_d, int64_t* id) {
     if (!S) return SFEM_ERR_ARGUMENT;
     return guard(S->ctx, [&] {
         if (ndim != 2 && ndim != 3) fail(SFEM_ERR_UNSUPPORTED, "server_submit: Q4 (2D) or Hex8 (3D) only");
+        int64_t grid_nodes = 1;
+        if (dims)
+            for (int d = 0; d < ndim; ++d) {
+                if (dims[d] < 1) fail(SFEM_ERR_ARGUMENT, "server_submit: element counts must be >= 1");
+                grid_nodes *= dims[d] + 1;
+            }
+        if (dims && grid_nodes != n_nodes) fail(SFEM_ERR_ARGUMENT, "server_submit: dims do not match n_nodes");
         if (!node_list || !ien || !young || !nu || !fixed || n_nodes <= 0 || n_elem <= 0)
             fail(SFEM_ERR_ARGUMENT, "server_submit: NULL array or empty mesh");
         const int64_t nen = (int64_t)1 << ndim, n = n_nodes * ndim;
         Request r;
         r.id = S->next_id++;
         r.ndim = ndim;
+        if (dims) {
+            r.grid = true;
+            for (int d = 0; d < ndim; ++d) r.dims[d] = dims[d];
+        }
         r.n_nodes = n_nodes;
         r.n_elem = n_elem;
         r.nodes.assign(node_list, node_list + ndim * n_nodes);
@@ -311,10 +445,17 @@ int sfem_server_submit(sfem_server* S, int ndim, const double* node_list, int64_
         estimate_bytes(r, S->o.precision);
         r.t_sub = S->now_ms();
         const int64_t rid = r.id;
+        const bool to_grid = r.grid && S->o.preconditioner == SFEM_PC_MULTIGRID;
         S->reqs.emplace(rid, std::move(r));
-        S->queue.push_back(rid);
+        (to_grid ? S->gqueue : S->queue).push_back(rid);
         if (id) *id = rid;
     });
+}
+
+int sfem_server_submit(sfem_server* S, int ndim, const double* node_list, int64_t n_nodes, const int64_t* ien,
+                       int64_t n_elem, const double* young, const double* nu, const double* f, const uint8_t* fixed,
+                       const double* q_d, int64_t* id) {
+    return sfem_server_submit_grid(S, ndim, nullptr, node_list, n_nodes, ien, n_elem, young, nu, f, fixed, q_d, id);
 }
 
 int sfem_server_step(sfem_server* S, int* n_finished, double* launch_ms) {
@@ -365,7 +506,7 @@ int sfem_server_take(sfem_server* S, int64_t id, double* q) {
 
 int sfem_server_info(const sfem_server* S, int64_t* queued, int64_t* active, int64_t* bytes) {
     if (!S) return SFEM_ERR_ARGUMENT;
-    if (queued) *queued = (int64_t)S->queue.size();
+    if (queued) *queued = (int64_t)(S->queue.size() + S->gqueue.size());
     if (active) *active = (int64_t)S->active.size();
     if (bytes) *bytes = S->bytes_in_use;
     return SFEM_OK;

@@ -192,6 +192,8 @@ _PROTOS = [
     ("sfem_server_create", C.c_int, [c_vp, C.POINTER(ServerOpts), C.POINTER(c_vp)]),
     ("sfem_server_submit", C.c_int,
      [c_vp, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, c_dblp, c_dblp, c_dblp, c_u8p, c_dblp, c_i64p]),
+    ("sfem_server_submit_grid", C.c_int,
+     [c_vp, C.c_int, c_i64p, c_dblp, C.c_int64, c_i64p, C.c_int64, c_dblp, c_dblp, c_dblp, c_u8p, c_dblp, c_i64p]),
     ("sfem_server_step", C.c_int, [c_vp, c_intp, c_dblp]),
     ("sfem_server_poll", C.c_int, [c_vp, C.c_int64, c_intp, C.POINTER(BatchProblemStats), c_dblp]),
     ("sfem_server_take", C.c_int, [c_vp, C.c_int64, c_dblp]),

@@ -62,10 +62,16 @@ class Server:
         fixed = np.ascontiguousarray(p.fixed, dtype=np.uint8)
         qd = None if p.q_d is None else np.ascontiguousarray(p.q_d, dtype=np.float64)
         rid = C.c_int64()
-        self.ctx.check(self.ctx._lib.sfem_server_submit(
-            self.handle, m.ndim, _ptr(nodes, C.c_double), m.n_nodes, _ptr(ien, C.c_int64), m.n_elem,
-            _ptr(young, C.c_double), _ptr(nu, C.c_double), _ptr(f, C.c_double), _ptr(fixed, C.c_uint8),
-            None if qd is None else _ptr(qd, C.c_double), C.byref(rid)))
+        args = (_ptr(nodes, C.c_double), m.n_nodes, _ptr(ien, C.c_int64), m.n_elem, _ptr(young, C.c_double),
+                _ptr(nu, C.c_double), _ptr(f, C.c_double), _ptr(fixed, C.c_uint8),
+                None if qd is None else _ptr(qd, C.c_double), C.byref(rid))
+        if self.opts["preconditioner"] == PRECONDITIONERS["multigrid"] and getattr(m, "dims", None):
+            # structured mesh: solved on its own with the multigrid preconditioner
+            dims = np.ascontiguousarray(m.dims, dtype=np.int64)
+            self.ctx.check(self.ctx._lib.sfem_server_submit_grid(self.handle, m.ndim, _ptr(dims, C.c_int64),
+                                                                 *args))
+        else:
+            self.ctx.check(self.ctx._lib.sfem_server_submit(self.handle, m.ndim, *args))
         return rid.value
 
     def step(self):

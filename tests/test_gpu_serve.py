@@ -95,6 +95,33 @@ def test_partially_fixed_request_in_mix(ctx):
     S.close()
 
 
+def test_multigrid_server(ctx):
+    """Requests with a grid are solved one at a time with the multigrid
+    preconditioner (a grid with no even element count falls back to the
+    persistent launch); every result meets the oracle bar, and a request's
+    result is bitwise the same alone and in a mix."""
+    print("[case] serve multigrid", flush=True)
+    reqs = [request((8, 4, 4), 21), request((12, 6), 22), request((5, 3, 3), 23), request((16, 4, 6), 24),
+            request((1, 1, 1), 25)]
+    S = Server(ctx, precision="fp64", rtol=1e-11, iters_per_step=16, item_slices=2, preconditioner="multigrid")
+    rid = S.submit(reqs[0])
+    S.drain()
+    alone = S.take(rid, reqs[0].n_dof)
+    ids = [S.submit(p) for p in reqs]
+    S.step()
+    S.drain()
+    for i, p in zip(ids, reqs):
+        st = S.poll(i)
+        assert st["state"] == "done" and st["stop_reason"] == 1, st
+        if p.mesh.dims != (1, 1, 1):  # one element: no hierarchy, node-block Jacobi
+            assert st["iterations"] <= 40, st
+        q = S.take(i, p.n_dof)
+        assert relerr(q, oracle_solution(p)) <= ERR
+        if i == ids[0]:
+            assert np.array_equal(q, alone)
+    S.close()
+
+
 def test_hbm_budget_admission(ctx):
     print("[case] serve budget", flush=True)
     reqs = [request((6, 4, 4), s) for s in range(4)]
