@@ -348,7 +348,7 @@ def run_serve(ctx, precision, rates, nreq=48, seed=11, preconditioner="block_jac
         s = res.summary()
         s["rate_per_s"] = lam
         out["rates"].append(s)
-        log(f"serve {precision} rate {lam}/s: p50 {s['p50_ms']:.1f}ms p99 {s['p99_ms']:.1f}ms, "
+        log(f"serve {precision}/{preconditioner} rate {lam}/s: p50 {s['p50_ms']:.1f}ms p99 {s['p99_ms']:.1f}ms, "
             f"{s['solved_dof_per_s']:.3e} DOF/s, max active {s['max_active']}, failed {s['failed']}")
     return out
 
@@ -465,6 +465,8 @@ def main():
         if not args.no_serve:
             rates = [float(v) for v in args.serve_rates.split(",") if v]
             print(json.dumps({"serving": run_serve(ctx, args.serve_precision, rates)}), flush=True)
+            print(json.dumps({"serving_multigrid": run_serve(ctx, "fp64", rates, preconditioner="multigrid")}),
+                  flush=True)
         if not args.no_sweep:
             print(json.dumps({"sweep": run_sweep(ctx)}), flush=True)
         return
@@ -537,6 +539,9 @@ def main():
     if not args.no_serve:
         rates = [float(v) for v in args.serve_rates.split(",") if v]
         out["serving"] = run_serve(ctx, args.serve_precision, rates)
+        # the same request streams, each request solved on its own with the
+        # multigrid preconditioner (FP64)
+        out["serving_multigrid"] = run_serve(ctx, "fp64", rates, preconditioner="multigrid")
     if not args.no_sweep:
         out["sweep"] = run_sweep(ctx)
     if not args.no_cpu:
