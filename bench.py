@@ -5,9 +5,9 @@ request stream, p50/p99 latency) forms.
 Headline workload: 3D Hex8 block 256x64x64 elements on [0,4]x[0,1]x[0,1]
 (3,244,800 free dofs), per-element E = 10^U(-1,1) (seed 1234), nu = 0.3, x = 0
 face clamped, iid N(0,1) nodal loads on every free node of the top face z = 1
-(seed 5678).  One step = device assembly (Ke, keys, sort, segmented
-reduction) + Dirichlet elimination + PCG to relative residual 1e-10 from
-resident inputs.
+(seed 5678).  One step = device assembly (the grid check, Ke
+and the grid-pattern gather, bitwise the sort-based result) + Dirichlet
+elimination + PCG to relative residual 1e-10 from resident inputs.
 
 Solver paths timed on the same inputs: the plain FP64 path (scalar CSR), FP64
 on the node-block sliced format, FP32 operator values with FP64 vectors, the
@@ -76,6 +76,7 @@ def timed_step(ctx, mesh, bc, precision, fmt, pc, rtol, time_kernels):
 
     ctx.synchronize()
     t0 = time.perf_counter()
+    mesh.set_grid(DIMS)  # the mesh is the 256x64x64 grid (checked on the device): grid-pattern assembly
     K = mesh.assemble()
     S = System(K, bc)
     mg = S.set_grid(DIMS) if pc == "multigrid" else None

@@ -103,6 +103,15 @@ int sfem_mesh_create(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
                      const double* young, int64_t n_young, double nu,
                      sfem_mesh** out);
 void sfem_mesh_destroy(sfem_mesh* mesh);
+/* Declare a Q4 / Hex8 vector mesh (n_dof = ndim) the structured grid of the
+ * reference's meshgrid (examples/vector3D.jl:10-130): ne[d] elements along
+ * direction d, node i + (nx+1)(j + (ny+1)k) (0-based) at grid point (i,j,k),
+ * elements numbered x fastest with one corner order for all, ID the
+ * node-interleaved numbering ndof*(node-1)+l.  Checked on the device
+ * (SFEM_ERR_ARGUMENT if the mesh is not that grid).  sfem_assemble_mesh then
+ * takes every node pair's elements from the grid instead of sorting keys;
+ * the result is bitwise the same.  sfem_meshgrid meshes are declared already. */
+int  sfem_mesh_set_grid(sfem_ctx* ctx, sfem_mesh* mesh, int ndim, const int64_t* ne);
 
 /* flags for sfem_assemble_mesh */
 #define SFEM_ASM_DEFAULT       0

@@ -10,6 +10,12 @@
 // Pipeline (one stream, stage times recorded with HIP events):
 //   validate -> element Ke (COO values) -> keys -> stable radix sort ->
 //   segment heads -> scan -> in-order segment sums -> block CSC pointers.
+// On a mesh declared a structured grid (mesh_set_grid) the pattern is the
+// grid's: block counts -> scan -> one gather of every node pair's element
+// blocks in ascending element order (the same sums, bit for bit).
+#include <algorithm>
+#include <vector>
+
 #include "sfem_objects.hpp"
 
 namespace sfem {
@@ -165,6 +171,181 @@ const char* name_of_code(int code) {
 
 }  // namespace
 
+namespace {
+
+// ------------------------------------------------ structured assembly --
+// On a structured grid every node pair (r, c) that shares an element shares
+// the cells between them: per direction the two cells around the node when
+// r and c have the same coordinate, else the one cell between them.  Their
+// contributions are summed in ascending element order starting from the
+// first, as the stable sort and segmented sum do, so K is bitwise the same.
+struct GridAsm {
+    int64_t n[3];   // nodes per direction (1 beyond ndim)
+    int64_t e[3];   // elements per direction (1 beyond ndim)
+    int corner[8];  // local index of the cell corner with code ox + 2 oy + 4 oz
+};
+
+template <int ND>
+__device__ __forceinline__ void grid_coords(int64_t node, const GridAsm& g, int64_t c[3]) {
+    c[0] = node % g.n[0];
+    c[1] = (node / g.n[0]) % g.n[1];
+    c[2] = ND == 3 ? node / (g.n[0] * g.n[1]) : 0;
+}
+
+// blocks per node column: product over directions of its valid offsets
+template <int ND>
+__global__ __launch_bounds__(256) void grid_counts_kernel(GridAsm g, int64_t nn, int64_t* __restrict__ cnt) {
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c > nn) return;
+    if (c == nn) {
+        cnt[nn] = 0;
+        return;
+    }
+    int64_t a[3];
+    grid_coords<ND>(c, g, a);
+    int64_t k = 1;
+    for (int d = 0; d < ND; ++d) k *= 1 + (a[d] > 0) + (a[d] < g.n[d] - 1);
+    cnt[c] = k;
+}
+
+// One thread per (node column c, stencil slot s); slots in ascending row
+// order (z, then y, then x offset), so the k-th valid slot is block k.
+template <int ND>
+__global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
+                                                          const int64_t* __restrict__ bcolptr,
+                                                          double* __restrict__ bval, int32_t* __restrict__ browidx,
+                                                          int32_t* __restrict__ bcolidx) {
+    constexpr int BS = ND, B2 = BS * BS, NS = ND == 3 ? 27 : 9, NEN = 1 << ND;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t c = t / NS;
+    const int s = (int)(t % NS);
+    if (c >= nn) return;
+    int64_t a[3];
+    grid_coords<ND>(c, g, a);
+    auto valid = [&](int sl, int64_t b[3]) {
+        const int o[3] = {sl % 3 - 1, (sl / 3) % 3 - 1, ND == 3 ? sl / 9 - 1 : 0};
+        bool ok = true;
+        for (int d = 0; d < 3; ++d) {
+            b[d] = a[d] + o[d];
+            ok = ok && b[d] >= 0 && b[d] < g.n[d];
+        }
+        return ok;
+    };
+    int64_t b[3];
+    if (!valid(s, b)) return;
+    int k = 0;
+    for (int sl = 0; sl < s; ++sl) {
+        int64_t bb[3];
+        k += valid(sl, bb) ? 1 : 0;
+    }
+    const int64_t u = bcolptr[c] + k;
+    const int64_t r = b[0] + g.n[0] * (b[1] + g.n[1] * b[2]);
+    // the shared cells per direction, ascending
+    int64_t cl[3][2];
+    int nc[3];
+    for (int d = 0; d < 3; ++d) {
+        nc[d] = 0;
+        if (d >= ND) {
+            cl[d][nc[d]++] = 0;
+        } else if (b[d] == a[d]) {
+            if (a[d] - 1 >= 0) cl[d][nc[d]++] = a[d] - 1;
+            if (a[d] < g.e[d]) cl[d][nc[d]++] = a[d];
+        } else {
+            cl[d][nc[d]++] = min(a[d], b[d]);
+        }
+    }
+    double acc[B2];
+    bool first = true;
+    for (int kz = 0; kz < nc[2]; ++kz)
+        for (int ky = 0; ky < nc[1]; ++ky)
+            for (int kx = 0; kx < nc[0]; ++kx) {
+                const int64_t cx = cl[0][kx], cy = cl[1][ky], cz = cl[2][kz];
+                const int64_t e = cx + g.e[0] * (cy + g.e[1] * cz);
+                const int ar = g.corner[(int)(b[0] - cx) + 2 * (int)(b[1] - cy) + 4 * (int)(b[2] - cz)];
+                const int ac = g.corner[(int)(a[0] - cx) + 2 * (int)(a[1] - cy) + 4 * (int)(a[2] - cz)];
+                // element e's block (row node ar, column node ac)
+                const double* p = V + e * (int64_t)(NEN * NEN * B2) + (int64_t)(ar * NEN + ac) * B2;
+#pragma unroll
+                for (int m = 0; m < B2; ++m) acc[m] = first ? p[m] : acc[m] + p[m];
+                first = false;
+            }
+#pragma unroll
+    for (int m = 0; m < B2; ++m) bval[u * B2 + m] = acc[m];
+    browidx[u] = (int32_t)r;
+    bcolidx[u] = (int32_t)c;
+}
+
+// Is IEN the meshgrid connectivity (corner order of element 0) and the node
+// numbering the identity interleave (dof of node m, component 0, = ndof m)?
+template <int ND>
+__global__ __launch_bounds__(256) void grid_check_kernel(const int32_t* __restrict__ ien,
+                                                         const int32_t* __restrict__ dof, GridAsm g, int64_t n_elem,
+                                                         int64_t nn, int ndof, int* __restrict__ bad) {
+    constexpr int NEN = 1 << ND;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n_elem) {
+        const int64_t cx = t % g.e[0], cy = (t / g.e[0]) % g.e[1], cz = ND == 3 ? t / (g.e[0] * g.e[1]) : 0;
+        for (int code = 0; code < NEN; ++code) {
+            const int64_t node = (cx + (code & 1)) + g.n[0] * ((cy + ((code >> 1) & 1)) + g.n[1] * (cz + (code >> 2)));
+            if (ien[t * NEN + g.corner[code]] != node) atomicOr(bad, 1);
+        }
+    }
+    if (t < nn && dof[t * ndof] != ndof * t) atomicOr(bad, 2);
+}
+
+}  // namespace
+
+void mesh_set_grid(sfem_ctx* ctx, sfem_mesh* m, int ndim, const int64_t* ne) {
+    if (!ne) fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: NULL element counts");
+    if (m->face || m->cls != SFEM_Q1 || m->ndim != ndim || (ndim != 2 && ndim != 3) || m->nen != (1 << ndim) ||
+        m->ndof != ndim)
+        fail(SFEM_ERR_UNSUPPORTED, "mesh_set_grid: Q4 / Hex8 vector meshes (n_dof = ndim) only");
+    GridAsm g{};
+    int64_t ncount = 1, ecount = 1;
+    for (int d = 0; d < 3; ++d) {
+        g.e[d] = d < ndim ? ne[d] : 1;
+        g.n[d] = d < ndim ? ne[d] + 1 : 1;
+        if (g.e[d] < 1) fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: element counts must be >= 1");
+        ncount *= g.n[d];
+        ecount *= g.e[d];
+    }
+    if (ecount != m->n_elem || ncount != m->n_nodes || m->id_rows < m->n_nodes)
+        fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: element or node count differs from the mesh");
+    // corner order from element 0 (cell (0, 0, 0))
+    const int nen = m->nen;
+    std::vector<int32_t> e0((size_t)nen);
+    d2h(ctx, e0.data(), m->ien.get(), sizeof(int32_t) * nen);
+    int seen = 0;
+    for (int a = 0; a < nen; ++a) {
+        const int64_t v = e0[(size_t)a];
+        const int64_t x = v >= 0 ? v % g.n[0] : -1, y = v >= 0 ? (v / g.n[0]) % g.n[1] : -1;
+        const int64_t z = v >= 0 ? v / (g.n[0] * g.n[1]) : -1;
+        if (x < 0 || x > 1 || y < 0 || y > 1 || z < 0 || z > 1)
+            fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: element 1 is not the first cell of the grid");
+        const int code = (int)(x + 2 * y + 4 * z);
+        if (seen & (1 << code)) fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: element 1 repeats a corner");
+        seen |= 1 << code;
+        g.corner[code] = a;
+    }
+    DevBuf<int> bad(ctx, 1);
+    memset_d(ctx, bad.get(), 0, sizeof(int));
+    const int64_t nt = std::max(m->n_elem, m->n_nodes);
+    if (ndim == 3)
+        hipLaunchKernelGGL(grid_check_kernel<3>, dim3(grid_for(nt, 256)), dim3(256), 0, ctx->stream, m->ien.get(),
+                           m->dof.get(), g, m->n_elem, m->n_nodes, m->ndof, bad.get());
+    else
+        hipLaunchKernelGGL(grid_check_kernel<2>, dim3(grid_for(nt, 256)), dim3(256), 0, ctx->stream, m->ien.get(),
+                           m->dof.get(), g, m->n_elem, m->n_nodes, m->ndof, bad.get());
+    SFEM_LAUNCH_CHECK();
+    int hb = 0;
+    d2h(ctx, &hb, bad.get(), sizeof(int));
+    if (hb & 1) fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: IEN is not the meshgrid connectivity of these element counts");
+    if (hb & 2) fail(SFEM_ERR_ARGUMENT, "mesh_set_grid: ID is not the node-interleaved numbering ndof*(node-1)+l");
+    m->grid_nd = ndim;
+    for (int d = 0; d < 3; ++d) m->grid_ne[d] = g.e[d];
+    for (int c = 0; c < 8; ++c) m->grid_corner[c] = g.corner[c];
+}
+
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     K->ctx = ctx;
     Timer tm(ctx->stream);
@@ -215,6 +396,50 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     K->m = K->n = K->nunits * bs;
     const int row_bits = bits_for(K->nunits);
     if (2 * row_bits > 64) fail(SFEM_ERR_UNSUPPORTED, "too many dofs for 64-bit keys");
+
+    if (m->grid_nd && bs == ndof && ndof == m->ndim && K->nunits == m->n_nodes && m->n_elem > 0) {
+        // structured grid: the pattern is the grid's, no keys or sort
+        GridAsm g{};
+        for (int d = 0; d < 3; ++d) {
+            g.e[d] = m->grid_ne[d];
+            g.n[d] = d < m->grid_nd ? m->grid_ne[d] + 1 : 1;
+        }
+        for (int c = 0; c < 8; ++c) g.corner[c] = m->grid_corner[c];
+        const int64_t nn = K->nunits;
+        DevBuf<int64_t> cnt(ctx, (size_t)nn + 1);
+        K->bcolptr.reset(ctx, (size_t)nn + 1);
+        if (m->grid_nd == 3)
+            hipLaunchKernelGGL(grid_counts_kernel<3>, dim3(grid_for(nn + 1, 256)), dim3(256), 0, ctx->stream, g, nn,
+                               cnt.get());
+        else
+            hipLaunchKernelGGL(grid_counts_kernel<2>, dim3(grid_for(nn + 1, 256)), dim3(256), 0, ctx->stream, g, nn,
+                               cnt.get());
+        SFEM_LAUNCH_CHECK();
+        const int64_t nnzb = exclusive_scan_i64(ctx, cnt.get(), K->bcolptr.get(), nn + 1);
+        tm.mark();
+        K->nnzb = nnzb;
+        K->nnz = nnzb * bs * bs;
+        K->browidx.reset(ctx, (size_t)nnzb);
+        K->bcolidx.reset(ctx, (size_t)nnzb);
+        K->bval.reset(ctx, (size_t)nnzb * bs * bs);
+        const int ns = m->grid_nd == 3 ? 27 : 9;
+        if (m->grid_nd == 3)
+            hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(grid_for(nn * ns, 256)), dim3(256), 0, ctx->stream,
+                               V.get(), g, nn, K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
+        else
+            hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(grid_for(nn * ns, 256)), dim3(256), 0, ctx->stream,
+                               V.get(), g, nn, K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
+        SFEM_LAUNCH_CHECK();
+        tm.mark();
+        sync(ctx);
+        K->t_validate = tm.ms(0, 1);
+        K->t_element = tm.ms(1, 2);
+        K->t_keys = tm.ms(2, 3);  // block counts and their scan
+        K->t_sort = 0.0;
+        K->t_reduce = tm.ms(3, 4);
+        K->t_total = tm.ms(0, 4);
+        return;
+    }
 
     const int64_t npairs = (bs == 1 && ndof > 1) ? ncoo : m->n_elem * nen * nen;
     DevBuf<uint64_t> keys(ctx, (size_t)npairs), keys_alt(ctx, (size_t)npairs);

@@ -367,6 +367,12 @@ int sfem_mesh_create(sfem_ctx* ctx, int ndim, int function_class, int n_dof, con
     return SFEM_OK;
 }
 
+int sfem_mesh_set_grid(sfem_ctx* ctx, sfem_mesh* m, int ndim, const int64_t* ne) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!m) { ctx->last_error = "mesh_set_grid: NULL mesh"; return SFEM_ERR_ARGUMENT; }
+    return guard(ctx, [&] { mesh_set_grid(ctx, m, ndim, ne); });
+}
+
 void sfem_mesh_destroy(sfem_mesh* m) {
     if (!m) return;
     sfem_ctx* ctx = m->ctx;

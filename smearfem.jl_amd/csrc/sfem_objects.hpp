@@ -27,6 +27,14 @@ struct sfem_mesh {
     // element % key_period.
     bool face = false;
     int64_t key_period = 0;
+    // Structured grid (sfem_mesh_set_grid): grid_ne elements per direction,
+    // nodes and elements in the meshgrid numbering (x fastest), the local
+    // index of each cell corner (code ox + 2 oy + 4 oz), identity node ids.
+    // Assembly then takes every node pair's elements from the grid instead
+    // of sorting keys.
+    int grid_nd = 0;
+    int64_t grid_ne[3] = {0, 0, 0};
+    int grid_corner[8] = {0, 0, 0, 0, 0, 0, 0, 0};
 };
 
 // Assembled K, kept as a block CSC: column-unit c holds blocks (row-unit r,
@@ -90,6 +98,7 @@ void launch_face_mass(sfem_ctx* ctx, const sfem_mesh* m, double* V);
 
 // ---- assemble.hip ----
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K);
+void mesh_set_grid(sfem_ctx* ctx, sfem_mesh* m, int ndim, const int64_t* ne);
 void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval);
 void upload_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval,
                  const double* nzval, sfem_csr* K);

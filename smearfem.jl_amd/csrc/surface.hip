@@ -482,6 +482,8 @@ int sfem_meshgrid(sfem_ctx* ctx, int ndim, const int64_t* ne, const double* box,
             surface_from_device(ctx, fm, m->coords.get(), m->dof.get(), n_nodes, n_nodes, nf);
             fm->ien = std::move(fien);
         }
+        const int64_t dims[3] = {nx, ny, nz};
+        mesh_set_grid(ctx, m, ndim, dims);  // the generated mesh is its grid
         sync(ctx);
     });
     if (st) { delete m; delete fm; return st; }

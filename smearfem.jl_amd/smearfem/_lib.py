@@ -136,6 +136,7 @@ _PROTOS = [
      [c_vp, C.c_int, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, C.c_int, C.c_int64,
       c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
     ("sfem_mesh_destroy", None, [c_vp]),
+    ("sfem_mesh_set_grid", C.c_int, [c_vp, c_vp, C.c_int, c_i64p]),
     ("sfem_assemble_mesh", C.c_int, [c_vp, c_vp, C.c_int, C.POINTER(c_vp)]),
     ("sfem_assemble", C.c_int,
      [c_vp, C.c_int, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, C.c_int, C.c_int64,

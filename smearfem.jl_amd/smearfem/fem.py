@@ -194,6 +194,15 @@ class Mesh:
             _ptr(young, C.c_double), young.size, float(nu), C.byref(h)))
         self.handle = h
 
+    def set_grid(self, dims) -> None:
+        """Declare the mesh the structured grid of the meshgrid numbering with
+        ``dims`` elements per direction (``sfem_mesh_set_grid``, checked on
+        the device); assembly then takes node pairs from the grid instead of
+        sorting keys, with bitwise the same K."""
+        d = np.ascontiguousarray(dims, dtype=np.int64).ravel()
+        self.ctx.check(self.ctx._lib.sfem_mesh_set_grid(self.ctx.handle, self.handle, int(d.size),
+                                                        _ptr(d, C.c_int64)))
+
     def assemble(self, force_scalar: bool = False) -> StiffnessMatrix:
         h = C.c_void_p()
         self.ctx.check(self.ctx._lib.sfem_assemble_mesh(self.ctx.handle, self.handle, 1 if force_scalar else 0,

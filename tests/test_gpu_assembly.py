@@ -179,3 +179,66 @@ def test_error_behaviour_matches_reference(ctx):
     assert both(m.NodeList, m.IEN, 2, "Q3", 1, None) == (-5, -5)
     assert both(m.NodeList[:1], m.IEN[:, :2], 1, "Q1", 1, None) == (-3, -3)
     assert both(m.NodeList, m.IEN, 2, "Q1", 1, None, n_elem=12) == (-2, -2)  # ne^ndim > size(IEN,1)
+
+
+def _mesh_assemble(ctx, m, young, coords=None, grid=None, nu=0.3):
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList if coords is None else coords, m.IEN, m.ndim, "Q1", m.ndim,
+                         m.ID, young, nu)
+    if grid is not None:
+        mesh.set_grid(grid)
+    K = mesh.assemble()
+    out = K.csc_arrays()
+    K.close()
+    mesh.close()
+    return out
+
+
+@pytest.mark.parametrize("dims", [(6, 4, 3), (2, 1, 1), (1, 3, 2), (7, 5), (1, 1)])
+def test_structured_assembly_bitwise_equals_sorted(ctx, dims):
+    """Declaring the grid replaces keys / sort / segment sums by a gather from
+    the grid; the pattern and every value must be bitwise those of the
+    sorted path (per-element E, distorted coordinates)."""
+    print(f"[case] structured assembly {dims}", flush=True)
+    m = structured_mesh(dims)
+    rng = np.random.default_rng(len(dims) * 100 + sum(dims))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=7)
+    coords = m.NodeList + 0.1 / max(dims) * rng.uniform(-1, 1, m.NodeList.shape)
+    ref = _mesh_assemble(ctx, m, young, coords)
+    got = _mesh_assemble(ctx, m, young, coords, grid=dims)
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+
+
+def test_structured_assembly_meshgrid_and_errors(ctx):
+    print("[case] structured assembly meshgrid + errors", flush=True)
+    from smearfem.surface import meshgrid
+    # sfem_meshgrid meshes are declared grids: equal to the same mesh uploaded and sorted
+    dm, _ = meshgrid(0, 1, 0, 1, 0, 1, 5, 3, inflate=True, Young=40.0, nu=0.4, ctx=ctx)
+    nodes, ien, idm = dm.export()
+    Kg = dm.assemble()
+    got = Kg.csc_arrays()
+    Kg.close()
+    dm.close()
+    up = smearfem.Mesh(ctx, ien.shape[0], nodes, ien, 3, "Q1", 3, idm, 40.0, 0.4)
+    Ku = up.assemble()
+    ref = Ku.csc_arrays()
+    Ku.close()
+    for a, b in zip(got, ref):
+        assert np.array_equal(a, b)
+    # not the grid: wrong counts, a permuted element, a permuted numbering
+    m = structured_mesh((4, 3, 2))
+    with pytest.raises(smearfem.ArgumentError):
+        up.set_grid((4, 3, 3))
+    up.close()
+    bad = m.IEN.copy()
+    bad[[3, 5]] = bad[[5, 3]]  # elements 4 and 6 swapped
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, bad, 3, "Q1", 3, m.ID, 1.0, 0.3)
+    with pytest.raises(smearfem.ArgumentError):
+        mesh.set_grid((4, 3, 2))
+    mesh.close()
+    idp = m.ID.copy()
+    idp[[0, 1]] = idp[[1, 0]]  # nodes 1 and 2 exchange their dofs
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, idp, 1.0, 0.3)
+    with pytest.raises(smearfem.ArgumentError):
+        mesh.set_grid((4, 3, 2))
+    mesh.close()
