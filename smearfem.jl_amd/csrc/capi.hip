@@ -527,9 +527,13 @@ int sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_sys
 int sfem_system_info(const sfem_system* S, int64_t* n_free, int64_t* nnz_ff, int* block) {
     if (!S) return SFEM_ERR_ARGUMENT;
     if (n_free) *n_free = S->nf;
-    if (nnz_ff) *nnz_ff = S->nnz;
     if (block) *block = S->has_block ? S->bs : 1;
-    return SFEM_OK;
+    if (!nnz_ff) return SFEM_OK;
+    // the row counts are formed on first request (the CSR path needs them too)
+    return guard(S->ctx, [&] {
+        ensure_rowptr(S->ctx, const_cast<sfem_system*>(S));
+        *nnz_ff = S->nnz;
+    });
 }
 
 int sfem_system_eliminate_ms(const sfem_system* S, double* ms) {

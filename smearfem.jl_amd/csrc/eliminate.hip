@@ -61,14 +61,14 @@ __global__ void unit_fixed_kernel(const uint32_t* __restrict__ ufree, uint8_t* _
 }
 
 __global__ void csr_count_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
-                                 const uint8_t* __restrict__ fixed, const int32_t* __restrict__ free_of,
-                                 int64_t nunits, int bs, uint32_t* __restrict__ rowcnt) {
+                                 const int32_t* __restrict__ free_of, int64_t nunits, int bs,
+                                 uint32_t* __restrict__ rowcnt) {
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
     uint32_t cnt = 0;
     for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
         const int64_t r = browidx[u];
-        for (int i = 0; i < bs; ++i) cnt += fixed[r * bs + i] ? 0u : 1u;
+        for (int i = 0; i < bs; ++i) cnt += free_of[r * bs + i] < 0 ? 0u : 1u;
     }
     for (int j = 0; j < bs; ++j) {
         const int32_t row = free_of[cu * bs + j];
@@ -407,26 +407,12 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     d2h(ctx, &nf, tot.get(), sizeof(nf));
     S->nf = nf;
 
-    // scalar CSR
-    DevBuf<uint32_t> rowptr(ctx, (size_t)nf + 1);
-    memset_d(ctx, rowptr.get(), 0, sizeof(uint32_t) * (nf + 1));
+    // scalar CSR: row pointers, columns and values on first use (ensure_rowptr, ensure_csr)
     const int64_t nunits = K->nunits;
     const dim3 gu(grid_for(nunits, 256));
-    hipLaunchKernelGGL(csr_count_kernel, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
-                       bc->fixed.get(), S->free_of.get(), nunits, bs, rowptr.get());
-    SFEM_LAUNCH_CHECK();
-    {
-        DevBuf<uint32_t> t2(ctx, 1);
-        exclusive_scan_u32_dev(ctx, rowptr.get(), rowptr.get(), (int64_t)nf + 1, t2.get());
-    }
-    uint32_t nnz = 0;
-    if (nf) d2h(ctx, &nnz, rowptr.get() + nf, sizeof(nnz));
-    if ((uint64_t)nnz >= (1ull << 31)) fail(SFEM_ERR_UNSUPPORTED, "K_ff has 2^31 or more entries");
-    S->nnz = nnz;
-    S->rowptr.reset(ctx, (size_t)nf + 1);
-    SFEM_HIP(hipMemcpyAsync(S->rowptr.get(), rowptr.get(), sizeof(int32_t) * (nf + 1), hipMemcpyDeviceToDevice,
-                            ctx->stream));
-    S->col.release();  // the CSR arrays are filled on first use (ensure_csr)
+    S->nnz = -1;
+    S->rowptr.release();
+    S->col.release();
     S->val64.release();
     S->val32.release();
     S->rhs.reset(ctx, nf);
@@ -476,13 +462,15 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             const int64_t nslices = (nfb + 63) / 64;
             S->nfb = nfb;
             S->nslices = nslices;
-            if (S->partial) {
+            {  // node blocks of K_ff (with the masked blocks of partially fixed nodes)
                 DevBuf<uint32_t> btot(ctx, 1);
                 DevBuf<uint32_t> bscan(ctx, nfb);
                 exclusive_scan_u32_dev(ctx, (const uint32_t*)blen.get(), bscan.get(), nfb, btot.get());
                 uint32_t nb = 0;
                 d2h(ctx, &nb, btot.get(), sizeof(nb));
                 S->nnzb = nb;
+            }
+            if (S->partial) {
                 S->pad_of.reset(ctx, (size_t)nfb * bs);
                 S->prhs.reset(ctx, (size_t)nfb * bs);
                 S->px.reset(ctx, (size_t)nfb * bs);
@@ -491,8 +479,6 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
                 hipLaunchKernelGGL(pad_map_kernel, gu, dim3(256), 0, ctx->stream, ufree_of.get(), S->free_of.get(),
                                    nunits, bs, S->pad_of.get());
                 SFEM_LAUNCH_CHECK();
-            } else {
-                S->nnzb = nnz / (bs * bs);
             }
             double* pdinv = S->partial ? S->pdinv64.get() : nullptr;
             S->sl_ptr.reset(ctx, (size_t)nslices + 1);
@@ -549,7 +535,31 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     (void)hipEventDestroy(e1);
 }
 
+void ensure_rowptr(sfem_ctx* ctx, sfem_system* S) {
+    if (S->nnz >= 0) return;
+    const sfem_csr* K = S->K;
+    const int64_t nf = S->nf;
+    DevBuf<uint32_t> rowptr(ctx, (size_t)nf + 1);
+    memset_d(ctx, rowptr.get(), 0, sizeof(uint32_t) * (nf + 1));
+    hipLaunchKernelGGL(csr_count_kernel, dim3(grid_for(K->nunits, 256)), dim3(256), 0, ctx->stream,
+                       K->bcolptr.get(), K->browidx.get(), S->free_of.get(), K->nunits, S->bs, rowptr.get());
+    SFEM_LAUNCH_CHECK();
+    {
+        DevBuf<uint32_t> t2(ctx, 1);
+        exclusive_scan_u32_dev(ctx, rowptr.get(), rowptr.get(), nf + 1, t2.get());
+    }
+    uint32_t nnz = 0;
+    if (nf) d2h(ctx, &nnz, rowptr.get() + nf, sizeof(nnz));
+    if ((uint64_t)nnz >= (1ull << 31)) fail(SFEM_ERR_UNSUPPORTED, "K_ff has 2^31 or more entries");
+    S->rowptr.reset(ctx, (size_t)nf + 1);
+    SFEM_HIP(hipMemcpyAsync(S->rowptr.get(), rowptr.get(), sizeof(int32_t) * (nf + 1), hipMemcpyDeviceToDevice,
+                            ctx->stream));
+    sync(ctx);
+    S->nnz = nnz;
+}
+
 void ensure_csr(sfem_ctx* ctx, sfem_system* S) {
+    ensure_rowptr(ctx, S);
     if (S->nnz == 0 || S->col.size() == (size_t)S->nnz) return;
     const sfem_csr* K = S->K;
     S->col.reset(ctx, (size_t)S->nnz);

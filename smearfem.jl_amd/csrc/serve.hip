@@ -229,6 +229,11 @@ bool solve_grid_request(sfem_server* S, Request& r, double* ms) {
                             r.young.data(), E, 0.3, &mesh));
         mesh->nu_e.reset(ctx, (size_t)E);
         h2d(ctx, mesh->nu_e.get(), r.nu.data(), sizeof(double) * E);
+        try {
+            mesh_set_grid(ctx, mesh, nd, r.dims);  // grid-pattern assembly when the mesh is the grid
+        } catch (const Error& e) {
+            if (e.code != SFEM_ERR_ARGUMENT) throw;
+        }
         ok(sfem_assemble_mesh(ctx, mesh, SFEM_ASM_DEFAULT, &K));
         if (K->n != n || K->bs != nd) fail(SFEM_ERR_ARGUMENT, "server: every node must belong to an element");
         ok(sfem_bc_create(ctx, n, r.f.empty() ? nullptr : r.f.data(), r.fixed.data(),

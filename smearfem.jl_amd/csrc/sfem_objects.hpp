@@ -141,8 +141,9 @@ struct sfem_system {
     sfem::DevBuf<double> qd, f;    // n
     sfem::DevBuf<double> rhs;      // nf: C'(f - K q_d)
     sfem::DevBuf<double> x;        // nf: FP64 solution
-    // scalar CSR of K_ff: rowptr always, col / val64 on first use (ensure_csr)
-    int64_t nnz = 0;
+    // scalar CSR of K_ff, built on first use: rowptr and nnz (-1 until then,
+    // ensure_rowptr), col / val64 (ensure_csr)
+    int64_t nnz = -1;
     sfem::DevBuf<int32_t> rowptr, col;
     sfem::DevBuf<double> val64;
     sfem::DevBuf<float> val32;     // built on first mixed CSR use
@@ -181,7 +182,9 @@ namespace sfem {
 void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system* S);
 void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* reactions);
 void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc);
-// the scalar CSR arrays of K_ff (col, val64), filled on first use
+// the scalar CSR of K_ff: row pointers and nnz (ensure_rowptr), columns and
+// FP64 values (ensure_csr), built on first use
+void ensure_rowptr(sfem_ctx* ctx, sfem_system* S);
 void ensure_csr(sfem_ctx* ctx, sfem_system* S);
 // padded block vector <- compact free-dof vector (0 on padded fixed dofs), and back
 void pad_gather(sfem_ctx* ctx, const sfem_system* S, const double* compact, double* padded);

@@ -74,15 +74,21 @@ class System:
         h = C.c_void_p()
         self.ctx.check(self.ctx._lib.sfem_eliminate(self.ctx.handle, K.handle, bc.handle, C.byref(h)))
         self.handle = h
-        nf, nnz, bs = C.c_int64(), C.c_int64(), C.c_int()
-        self.ctx.check(self.ctx._lib.sfem_system_info(h, C.byref(nf), C.byref(nnz), C.byref(bs)))
+        nf, bs = C.c_int64(), C.c_int()
+        self.ctx.check(self.ctx._lib.sfem_system_info(h, C.byref(nf), None, C.byref(bs)))
         ms = C.c_double()
         self.ctx.check(self.ctx._lib.sfem_system_eliminate_ms(h, C.byref(ms)))
         self.eliminate_ms = ms.value
         self.n = K.shape[0]
         self.n_free = nf.value
-        self.nnz = nnz.value
         self.block = bs.value
+
+    @property
+    def nnz(self) -> int:
+        """Entries of K_ff (the scalar row counts are formed on first request)."""
+        nnz = C.c_int64()
+        self.ctx.check(self.ctx._lib.sfem_system_info(self.handle, None, C.byref(nnz), None))
+        return nnz.value
 
     def solve(self, precision="fp64", fmt="auto", rtol=1e-10, max_iter=200000, reliable_delta=0.1,
               check_every=64, time_kernels=False, x0_zero=True, raise_on_fail=True,
