@@ -247,6 +247,7 @@ int  sfem_system_solution(sfem_ctx* ctx, const sfem_system* sys, double* q,
 #define SFEM_MG_MAX_LEVELS 24
 typedef struct {
     int     degree;        /* Chebyshev steps before and after the coarse correction (2)  */
+    int     coarse_degree; /* the same on levels below the finest; 0: degree (0)          */
     double  lo, hi;        /* smoothing interval, x lambda_max (0.1, 1.1)                 */
     int     power_iters;   /* power iterations per level for lambda_max (8)               */
     int64_t coarse_nodes;  /* stop coarsening at this many nodes (64)                     */

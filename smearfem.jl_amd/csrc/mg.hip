@@ -54,6 +54,7 @@ struct MgLevel {
     DevBuf<uint8_t> fixed;     // nrows*bs: the dof is fixed
     DevBuf<float> r, x, da, db, b;
     double lmax = 0;
+    int degree = 2;  // Chebyshev steps before and after the coarse correction
     float c1[kMgMaxDegree] = {}, c2[kMgMaxDegree] = {};
 };
 
@@ -1301,7 +1302,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         H.L[l].lmax = hps[(size_t)l].lam;
         if (!(H.L[l].lmax > 0.0) || !std::isfinite(H.L[l].lmax))
             fail(SFEM_ERR_SINGULAR, "multigrid: no spectrum estimate on level " + std::to_string(l));
-        cheb_coefficients(H.L[l], H.degree, o.lo, o.hi);
+        H.L[l].degree = l == 0 || o.coarse_degree == 0 ? o.degree : o.coarse_degree;
+        cheb_coefficients(H.L[l], H.L[l].degree, o.lo, o.hi);
     }
 }
 
@@ -1374,7 +1376,7 @@ struct MgRun {
         }
         float* dp = L.da.get();
         float* dn = L.db.get();
-        for (int k = 1; k < H.degree; ++k) {
+        for (int k = 1; k < L.degree; ++k) {
             step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
             std::swap(dp, dn);
         }
@@ -1400,7 +1402,7 @@ struct MgRun {
         step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0]);
         dp = L.db.get();
         dn = L.da.get();
-        for (int k = 1; k < H.degree; ++k) {
+        for (int k = 1; k < L.degree; ++k) {
             step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
             std::swap(dp, dn);
         }
@@ -1461,7 +1463,7 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     int64_t enq = 0, counted = 0, timed = 0;
     double smooth_ms = 0.0;
     const int chunk_max = std::max(1, std::min(o->check_every, 8));
-    const int per_iter = 2 * H.degree - 1;  // finest-level smoothing steps per V-cycle
+    const int per_iter = 2 * H.L[0].degree - 1;  // finest-level smoothing steps per V-cycle
     std::vector<hipEvent_t> ev;
     if (o->time_kernels) {
         ev.resize(2 * (size_t)per_iter * chunk_max);
@@ -1565,6 +1567,7 @@ void mg_solve_nd(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_s
 
 void mg_opts_default(sfem_mg_opts* o) {
     o->degree = 2;
+    o->coarse_degree = 0;
     o->lo = 0.1;
     o->hi = 1.1;
     o->power_iters = 8;
@@ -1578,7 +1581,8 @@ void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dim
     sfem_mg_opts o;
     mg_opts_default(&o);
     if (opts) o = *opts;
-    if (o.degree < 1 || o.degree > kMgMaxDegree || !(o.lo > 0.0) || !(o.hi > o.lo) || o.power_iters < 1 ||
+    if (o.degree < 1 || o.degree > kMgMaxDegree || o.coarse_degree < 0 || o.coarse_degree > kMgMaxDegree ||
+        !(o.lo > 0.0) || !(o.hi > o.lo) || o.power_iters < 1 ||
         o.coarse_nodes < 1)
         fail(SFEM_ERR_ARGUMENT, "set_grid: bad multigrid options");
     if (!S->has_block || S->bs != ndim || !S->K || S->K->bs != ndim)

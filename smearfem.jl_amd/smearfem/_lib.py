@@ -60,6 +60,7 @@ MG_MAX_LEVELS = 24
 class MgOpts(C.Structure):
     _fields_ = [
         ("degree", C.c_int),
+        ("coarse_degree", C.c_int),
         ("lo", C.c_double),
         ("hi", C.c_double),
         ("power_iters", C.c_int),

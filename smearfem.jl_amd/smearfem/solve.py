@@ -122,7 +122,8 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
-    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True) -> dict:
+    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True,
+                 coarse_degree=0) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
@@ -131,6 +132,7 @@ class System:
         o = _lib.MgOpts()
         self.ctx._lib.sfem_mg_opts_default(C.byref(o))
         o.degree, o.lo, o.hi = int(degree), float(lo), float(hi)
+        o.coarse_degree = int(coarse_degree)
         o.power_iters, o.coarse_nodes = int(power_iters), int(coarse_nodes)
         o.fine_fp16 = 1 if fine_fp16 else 0
         self.ctx.check(self.ctx._lib.sfem_system_set_grid(self.ctx.handle, self.handle, int(d.size),
