@@ -253,13 +253,14 @@ typedef struct {
     int64_t coarse_nodes;  /* stop coarsening at this many nodes (64)                     */
     int     fine_fp16;     /* 1: the finest-level smoother reads FP16 values of K_ff / 2^e
                               (max |a| / 2^e in (1/2, 1]) instead of FP32 (1)             */
+    int     coarse_fp16;   /* 1: the same for the levels between finest and coarsest (1)  */
 } sfem_mg_opts;
 
 typedef struct {
     int     levels;
     int     degree;
     int     fine_fp16;
-    int     pad;
+    int     coarse_fp16;                     /* FP16 smoothers below the finest level   */
     int64_t nodes[SFEM_MG_MAX_LEVELS];
     int64_t dims[SFEM_MG_MAX_LEVELS][3];     /* elements per direction (0 beyond ndim)  */
     double  lambda_max[SFEM_MG_MAX_LEVELS];  /* estimate; 0 on the coarsest level         */

@@ -50,6 +50,8 @@ struct MgLevel {
     int64_t nrows = 0;         // rows of the level's vectors (level 0: block rows)
     DevBuf<double> st64;       // masked symmetric FP64 stencil (levels >= 1)
     DevBuf<float> st32;        // smoothing operator: st64 with a unit diagonal on fixed dofs
+    DevBuf<_Float16> st16;     // or (FP16 coarse levels) its FP16 values times `scale`
+    double scale = 1.0;        // 2^-e: the level's smoother works on residuals times scale
     DevBuf<float> binv;        // inverse diagonal blocks, [(j*BS+i)*nrows + row]
     DevBuf<uint8_t> fixed;     // nrows*bs: the dof is fixed
     DevBuf<float> r, x, da, db, b;
@@ -375,6 +377,35 @@ __global__ void binv_half_kernel(const int64_t* __restrict__ sl_ptr, const int32
     for (int k = 0; k < B2; ++k) binv[(int64_t)k * nrows + row] = (float)inv[k];
 }
 
+// max over the diagonal of a node-major FP32 stencil (non-negative: SPD
+// operator with unit diagonals on fixed dofs), as ordered bit patterns
+template <int ND>
+__global__ void stencil_maxdiag_kernel(const float* __restrict__ st32, int64_t nn, unsigned* __restrict__ out) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, C = Sten<ND>::C;
+    unsigned m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn * BS;
+         i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t node = i / BS;
+        const int a = (int)(i % BS);
+        m = max(m, (unsigned)__float_as_int(fabsf(st32[(node * NS + C) * B2 + a * BS + a])));
+    }
+    for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
+    if ((threadIdx.x & 63) == 0) atomicMax(out, m);
+}
+
+// inverse diagonal blocks of a node-major FP16 stencil (the values as stored,
+// i.e. times the level's scale), planar [(j*BS+i)*nn + node]
+template <int ND>
+__global__ void stencil_binv_half_kernel(const _Float16* __restrict__ st16, int64_t nn, float* __restrict__ binv) {
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, C = Sten<ND>::C;
+    const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (node >= nn) return;
+    double m[B2], inv[B2];
+    for (int k = 0; k < B2; ++k) m[k] = (double)st16[(node * NS + C) * B2 + k];
+    invert_block<BS>(m, inv);
+    for (int k = 0; k < B2; ++k) binv[(int64_t)k * nn + node] = (float)inv[k];
+}
+
 // ------------------------------------------------------- V-cycle ----
 // finest-level smoothing operator: the system's sliced format with FP32
 // values, or FP16 values of the operator scaled by 1 / max |a|
@@ -387,8 +418,10 @@ struct SellOp {
     int64_t nslices, nrows, per_block;
 };
 
+// coarse level operator: node-major stencil values, FP32 or FP16 (scaled)
+template <typename TV>
 struct StOp {
-    const float* val;
+    const TV* val;
     Grid g;
     int64_t nn;
 };
@@ -404,8 +437,8 @@ struct Lanes {
 // val[(node*NS + s)*B2 + m], one contiguous run per node) by the neighbour's
 // values, then a fixed butterfly over the group sums the slots; the group's
 // first lane holds the result.  Every lane of the group must call it.
-template <int ND>
-__device__ __forceinline__ void stencil_row_group(const StOp& op, int64_t node, int s, const float* __restrict__ v,
+template <int ND, typename TV>
+__device__ __forceinline__ void stencil_row_group(const StOp<TV>& op, int64_t node, int s, const float* __restrict__ v,
                                                   float (&out)[ND]) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, G = Lanes<ND>::G;
 #pragma unroll
@@ -417,10 +450,10 @@ __device__ __forceinline__ void stencil_row_group(const StOp& op, int64_t node, 
         offset_of<ND>(s, dx, dy, dz);
         if (inside<ND>(op.g, c[0] + dx, c[1] + dy, c[2] + dz)) {
             const int64_t m = node + dx + op.g.n0 * (dy + op.g.n1 * (int64_t)dz);
-            const float* a = op.val + ((int64_t)node * NS + s) * B2;
+            const TV* a = op.val + ((int64_t)node * NS + s) * B2;
             float av[B2], xv[BS];
 #pragma unroll
-            for (int k = 0; k < B2; ++k) av[k] = a[k];
+            for (int k = 0; k < B2; ++k) av[k] = (float)a[k];
 #pragma unroll
             for (int i = 0; i < BS; ++i) xv[i] = v[m * BS + i];
 #pragma unroll
@@ -479,8 +512,8 @@ __global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp<TV> op, cons
     }
 }
 
-template <int ND, int MODE>
-__global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp op, const float* v, float* __restrict__ r,
+template <int ND, int MODE, typename TV>
+__global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp<TV> op, const float* v, float* __restrict__ r,
                                                                 float* __restrict__ x, const float* d_in,
                                                                 float* __restrict__ d_out, float c1, float c2,
                                                                 const float* __restrict__ binv, const int* done) {
@@ -490,7 +523,7 @@ __global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp op, const f
     const int64_t node = t / G;
     const int s = (int)(t % G);
     float out[ND];
-    stencil_row_group<ND>(op, node, s, v, out);
+    stencil_row_group<ND, TV>(op, node, s, v, out);
     if (s == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
 }
 
@@ -530,8 +563,9 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
                                                             const int32_t* __restrict__ row_of_node, Grid gc, int f0,
                                                             int f1, int f2, const uint8_t* __restrict__ cfix,
                                                             float* __restrict__ bc, const float* __restrict__ binv,
-                                                            float c2, float* __restrict__ rc, float* __restrict__ xc,
-                                                            float* __restrict__ dc, const int* done) {
+                                                            float c2, float cscale, float* __restrict__ rc,
+                                                            float* __restrict__ xc, float* __restrict__ dc,
+                                                            const int* done) {
     constexpr int BS = ND;
     if (*done) return;
     const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
@@ -581,7 +615,10 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     for (int j = 0; j < BS; ++j) {
         acc[j] = cfix[I * BS + j] ? 0.0f : acc[j];
         bc[I * BS + j] = acc[j];
-        if (FIRST) rc[I * BS + j] = acc[j];
+        if (FIRST) {
+            acc[j] *= cscale;  // the coarse level's units
+            rc[I * BS + j] = acc[j];
+        }
     }
     if (FIRST) {
 #pragma unroll
@@ -753,8 +790,8 @@ __global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp<TV> op, const
     if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
 }
 
-template <int ND>
-__global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp op, const float* __restrict__ v,
+template <int ND, typename TV>
+__global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp<TV> op, const float* __restrict__ v,
                                                                const float* __restrict__ binv, float* __restrict__ w,
                                                                PowState* ps, double* __restrict__ partials) {
     constexpr int G = Lanes<ND>::G;
@@ -768,7 +805,7 @@ __global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp op, const fl
         const int64_t node = t / G;
         const int s = (int)(t % G);
         float out[ND];
-        stencil_row_group<ND>(op, node, s, v, out);
+        stencil_row_group<ND, TV>(op, node, s, v, out);
         if (s == 0) sum += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
     }
     const double b = block_sum_256(sum, sh);
@@ -1217,6 +1254,34 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         h2d(ctx, H.cinv.get(), inv.data(), inv.size() * sizeof(double));
     }
 
+    // FP16 smoothing operators on the levels between the finest and the coarsest
+    if (o.coarse_fp16)
+        for (size_t l = 1; l + 1 < H.L.size(); ++l) {
+            MgLevel& v = H.L[l];
+            const int64_t nv = v.nn * NS * B2;
+            DevBuf<unsigned> mx(ctx, 1);
+            memset_d(ctx, mx.get(), 0, sizeof(unsigned));
+            hipLaunchKernelGGL(stencil_maxdiag_kernel<ND>, dim3(std::min(rows_grid(v.nn * BS), 256u)), dim3(kThreads),
+                               0, s, (const float*)v.st32.get(), v.nn, mx.get());
+            SFEM_LAUNCH_CHECK();
+            unsigned hm = 0;
+            d2h(ctx, &hm, mx.get(), sizeof(hm));
+            float amax;
+            std::memcpy(&amax, &hm, sizeof(amax));
+            if (!(amax > 0.0f) || !std::isfinite(amax)) fail(SFEM_ERR_SINGULAR, "multigrid: zero coarse operator");
+            int e = 0;
+            std::frexp((double)amax, &e);
+            v.scale = std::ldexp(1.0, -e);
+            v.st16.reset(ctx, (size_t)nv);
+            hipLaunchKernelGGL(to_half_kernel, dim3(std::min(rows_grid(nv), 2048u)), dim3(kThreads), 0, s,
+                               (const float*)v.st32.get(), nv, (float)v.scale, v.st16.get());
+            SFEM_LAUNCH_CHECK();
+            hipLaunchKernelGGL(stencil_binv_half_kernel<ND>, dim3(rows_grid(v.nn)), dim3(kThreads), 0, s,
+                               (const _Float16*)v.st16.get(), v.nn, v.binv.get());
+            SFEM_LAUNCH_CHECK();
+            v.st32.release();
+        }
+
     // FP16 finest-level smoothing operator
     if (o.fine_fp16) {
         const int64_t nv = S->sell_blocks * B2;
@@ -1286,11 +1351,17 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
                 hipLaunchKernelGGL((pow_sell_kernel<BS, float>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
                                    S->binv32.get(), vout, ps.get(), partials.get());
             } else {
-                const StOp op{v.st32.get(), grid_of(v), v.nn};
                 // grid-stride over the nodes: few blocks meet at the last-block count
-                hipLaunchKernelGGL(pow_stencil_kernel<ND>, dim3(std::min(rows_grid(v.nn * Lanes<ND>::G), 1024u)),
-                                   dim3(kThreads), 0, s, op, vin,
-                                   v.binv.get(), vout, ps.get() + l, partials.get());
+                const dim3 gp(std::min(rows_grid(v.nn * Lanes<ND>::G), 1024u));
+                if (v.st16.size()) {
+                    const StOp<_Float16> op{v.st16.get(), grid_of(v), v.nn};
+                    hipLaunchKernelGGL((pow_stencil_kernel<ND, _Float16>), gp, dim3(kThreads), 0, s, op, vin,
+                                       v.binv.get(), vout, ps.get() + l, partials.get());
+                } else {
+                    const StOp<float> op{v.st32.get(), grid_of(v), v.nn};
+                    hipLaunchKernelGGL((pow_stencil_kernel<ND, float>), gp, dim3(kThreads), 0, s, op, vin,
+                                       v.binv.get(), vout, ps.get() + l, partials.get());
+                }
             }
             SFEM_LAUNCH_CHECK();
             std::swap(vin, vout);
@@ -1344,13 +1415,19 @@ struct MgRun {
             if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true, float>) : go(cheb_sell_kernel<BS, 0, false, float>);
             else nt ? go(cheb_sell_kernel<BS, 1, true, float>) : go(cheb_sell_kernel<BS, 1, false, float>);
         } else {
-            const StOp op{L.st32.get(), grid_of(L), L.nn};
-            auto go = [&](auto k) {
+            auto go = [&](auto k, auto op) {
                 hipLaunchKernelGGL(k, dim3(rows_grid(L.nn * Lanes<ND>::G)), dim3(kThreads), 0, s, op, v, L.r.get(),
                                    L.x.get(), d_in, d_out, c1, c2, (const float*)L.binv.get(), done);
             };
-            if (mode == 0) go(cheb_stencil_kernel<ND, 0>);
-            else go(cheb_stencil_kernel<ND, 1>);
+            if (L.st16.size()) {
+                const StOp<_Float16> op{L.st16.get(), grid_of(L), L.nn};
+                if (mode == 0) go(cheb_stencil_kernel<ND, 0, _Float16>, op);
+                else go(cheb_stencil_kernel<ND, 1, _Float16>, op);
+            } else {
+                const StOp<float> op{L.st32.get(), grid_of(L), L.nn};
+                if (mode == 0) go(cheb_stencil_kernel<ND, 0, float>, op);
+                else go(cheb_stencil_kernel<ND, 1, float>, op);
+            }
         }
         SFEM_LAUNCH_CHECK();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next++ + 1], s));
@@ -1385,10 +1462,10 @@ struct MgRun {
         const Grid gf = grid_of(L), gc = grid_of(C);
         auto rk = [&](auto kern) {
             hipLaunchKernelGGL(kern, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, (const float*)L.r.get(),
-                               l == 0 ? (float)(1.0 / H.scale0) : 1.0f, gf,
+                               (float)(1.0 / (l == 0 ? H.scale0 : L.scale)), gf,
                                l == 0 ? (const int32_t*)H.row_of_node.get() : nullptr, gc, C.f[0], C.f[1], C.f[2],
                                (const uint8_t*)C.fixed.get(), C.b.get(), (const float*)C.binv.get(), C.c2[0],
-                               C.r.get(), C.x.get(), C.da.get(), done);
+                               (float)C.scale, C.r.get(), C.x.get(), C.da.get(), done);
         };
         if (l + 1 < nl - 1) rk(restrict_kernel<ND, true>);
         else rk(restrict_kernel<ND, false>);
@@ -1573,6 +1650,7 @@ void mg_opts_default(sfem_mg_opts* o) {
     o->power_iters = 8;
     o->coarse_nodes = 64;
     o->fine_fp16 = 1;
+    o->coarse_fp16 = 1;
 }
 
 void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
@@ -1636,6 +1714,9 @@ void mg_info(const sfem_system* S, sfem_mg_info* info) {
     info->setup_ms = H.setup_ms;
     info->degree = H.degree;
     info->fine_fp16 = H.fp16 ? 1 : 0;
+    info->coarse_fp16 = 0;
+    for (size_t l = 1; l + 1 < H.L.size(); ++l)
+        if (H.L[l].st16.size()) info->coarse_fp16 = 1;
 }
 
 void mg_stencil(sfem_ctx* ctx, const sfem_system* S, int level, double* stencil, uint8_t* fixed) {

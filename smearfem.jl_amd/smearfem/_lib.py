@@ -66,6 +66,7 @@ class MgOpts(C.Structure):
         ("power_iters", C.c_int),
         ("coarse_nodes", C.c_int64),
         ("fine_fp16", C.c_int),
+        ("coarse_fp16", C.c_int),
     ]
 
 
@@ -74,7 +75,7 @@ class MgInfo(C.Structure):
         ("levels", C.c_int),
         ("degree", C.c_int),
         ("fine_fp16", C.c_int),
-        ("pad", C.c_int),
+        ("coarse_fp16", C.c_int),
         ("nodes", C.c_int64 * MG_MAX_LEVELS),
         ("dims", (C.c_int64 * 3) * MG_MAX_LEVELS),
         ("lambda_max", C.c_double * MG_MAX_LEVELS),

@@ -123,7 +123,7 @@ class System:
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
     def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True,
-                 coarse_degree=0) -> dict:
+                 coarse_degree=0, coarse_fp16=True) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
@@ -135,6 +135,7 @@ class System:
         o.coarse_degree = int(coarse_degree)
         o.power_iters, o.coarse_nodes = int(power_iters), int(coarse_nodes)
         o.fine_fp16 = 1 if fine_fp16 else 0
+        o.coarse_fp16 = 1 if coarse_fp16 else 0
         self.ctx.check(self.ctx._lib.sfem_system_set_grid(self.ctx.handle, self.handle, int(d.size),
                                                           _ptr(d, C.c_int64), C.byref(o)))
         self.grid = tuple(int(v) for v in d)
@@ -145,7 +146,8 @@ class System:
         self.ctx.check(self.ctx._lib.sfem_system_mg_info(self.handle, C.byref(info)))
         nl = info.levels
         nd = len(getattr(self, "grid", (0, 0, 0)))
-        return {"levels": nl, "degree": info.degree, "fine_fp16": bool(info.fine_fp16), "nodes": [info.nodes[i] for i in range(nl)],
+        return {"levels": nl, "degree": info.degree, "fine_fp16": bool(info.fine_fp16),
+                "coarse_fp16": bool(info.coarse_fp16), "nodes": [info.nodes[i] for i in range(nl)],
                 "dims": [tuple(info.dims[i][d] for d in range(nd)) for i in range(nl)],
                 "lambda_max": [info.lambda_max[i] for i in range(nl)], "coarse_dofs": info.coarse_dofs,
                 "setup_ms": info.setup_ms}

@@ -166,8 +166,9 @@ def test_mg_solve_vs_oracle(ctx, precision):
 
 
 def test_mg_fp32_smoother_matches_fp16_smoother_solution(ctx):
-    """Both finest-level smoother operators (FP16 scaled, FP32) reach the
-    oracle's solution; FP16 values change the preconditioner only."""
+    """Every choice of smoother values (FP16 scaled or FP32, on the finest
+    level and on the levels below it) reaches the oracle's solution; FP16
+    values change the preconditioner only."""
     print("[case] mg fp16 vs fp32 smoother", flush=True)
     m = structured_mesh((16, 4, 8), lengths=(2.0, 0.5, 1.0))
     young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=9)
@@ -176,20 +177,22 @@ def test_mg_fp32_smoother_matches_fp16_smoother_solution(ctx):
     Kd = device_K(ctx, m, young)
     ref = oracle_direct(oracle_K(m, young), fixed, f)
     its = {}
-    for fp16 in (True, False):
+    for fp16, cfp16 in ((True, True), (True, False), (False, True), (False, False)):
         bc = BoundaryConditions(ctx, m.n_dof, f, fixed)
         S = System(Kd, bc)
         bc.close()
-        info = S.set_grid(m.dims, fine_fp16=fp16)
+        info = S.set_grid(m.dims, fine_fp16=fp16, coarse_fp16=cfp16)
         assert info["fine_fp16"] == fp16
+        assert info["levels"] >= 3  # a smoothed level below the finest
+        assert info["coarse_fp16"] == cfp16
         st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
         assert st["stop_reason"] == 1
         assert relerr(S.solution(), ref) <= ERR
-        its[fp16] = st["iterations"]
+        its[fp16, cfp16] = st["iterations"]
         S.close()
     Kd.close()
-    print(f"[mg] iterations fp16 {its[True]} fp32 {its[False]}", flush=True)
-    assert its[True] <= its[False] + 3
+    print(f"[mg] iterations (fine fp16, coarse fp16): {its}", flush=True)
+    assert max(its.values()) <= its[False, False] + 3
 
 
 def test_mg_partial_nodes_prescribed_values_and_set_bc(ctx):
