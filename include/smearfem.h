@@ -254,6 +254,10 @@ typedef struct {
     int     fine_fp16;     /* 1: the finest-level smoother reads FP16 values of K_ff / 2^e
                               (max |a| / 2^e in (1/2, 1]) instead of FP32 (1)             */
     int     coarse_fp16;   /* 1: the same for the levels between finest and coarsest (1)  */
+    int     matrix_free;   /* finest level applied from the element scales when K was
+                              assembled on this grid (sfem_mesh_set_grid) and every element
+                              matrix is s_e Ke(0): 1 the smoother (deviation <= 1e-6),
+                              2 also the FP64 outer operator (<= 1e-12); 0 off (2)        */
 } sfem_mg_opts;
 
 typedef struct {
@@ -266,6 +270,10 @@ typedef struct {
     double  lambda_max[SFEM_MG_MAX_LEVELS];  /* estimate; 0 on the coarsest level         */
     int64_t coarse_dofs;
     double  setup_ms;                        /* host wall time of the build             */
+    int     matrix_free;                     /* 0, 1, 2 as sfem_mg_opts.matrix_free     */
+    int     pad2;
+    double  ke_deviation;                    /* max |Ke(e)/s_e - Ke(0)| / max |Ke(0)|;
+                                                +inf when K was not assembled on the grid */
 } sfem_mg_info;
 
 void sfem_mg_opts_default(sfem_mg_opts* opts);

@@ -14,6 +14,8 @@
 // grid's: block counts -> scan -> one gather of every node pair's element
 // blocks in ascending element order (the same sums, bit for bit).
 #include <algorithm>
+#include <cmath>
+#include <cstring>
 #include <vector>
 
 #include "sfem_objects.hpp"
@@ -208,18 +210,77 @@ __global__ __launch_bounds__(256) void grid_counts_kernel(GridAsm g, int64_t nn,
     cnt[c] = k;
 }
 
+// s_e = Ke(e)[0][0] / Ke(0)[0][0]: the multiple of element 0's matrix that
+// element e's would be on a uniform grid with one Poisson ratio; and 1 / s_e
+__global__ __launch_bounds__(256) void grid_scale_kernel(const double* __restrict__ V, int64_t n_elem, int64_t ke,
+                                                         double* __restrict__ escale, double* __restrict__ inv) {
+    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= n_elem) return;
+    const double s = V[e * ke] / V[0];
+    escale[e] = s;
+    inv[e] = 1.0 / s;
+}
+
+// the larger of a and b, NaN when either is
+__device__ __forceinline__ double max_nan(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+
+template <int ND>
+__device__ __forceinline__ double grid_gather_slot(const double* __restrict__ V, const GridAsm& g, int64_t c, int s,
+                                                   const int64_t* __restrict__ bcolptr, double* __restrict__ bval,
+                                                   int32_t* __restrict__ browidx, int32_t* __restrict__ bcolidx,
+                                                   const double* __restrict__ inv_scale);
+
 // One thread per (node column c, stencil slot s); slots in ascending row
 // order (z, then y, then x offset), so the k-th valid slot is block k.
+// Each element block read is also compared with s_e times element 0's:
+// bmax[block] = max |Ke(e) (1 / s_e) - Ke(0)| over the block's reads (NaN
+// if any is).
 template <int ND>
 __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
                                                           const int64_t* __restrict__ bcolptr,
                                                           double* __restrict__ bval, int32_t* __restrict__ browidx,
-                                                          int32_t* __restrict__ bcolidx) {
-    constexpr int BS = ND, B2 = BS * BS, NS = ND == 3 ? 27 : 9, NEN = 1 << ND;
+                                                          int32_t* __restrict__ bcolidx,
+                                                          const double* __restrict__ inv_scale,
+                                                          double* __restrict__ bmax) {
+    __shared__ double wmax[4];
+    constexpr int BS = ND, B2 = BS * BS, NS = ND == 3 ? 27 : 9;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = t / NS;
     const int s = (int)(t % NS);
-    if (c >= nn) return;
+    double dmax = c < nn ? grid_gather_slot<ND>(V, g, c, s, bcolptr, bval, browidx, bcolidx, inv_scale) : 0.0;
+    // every lane arrives here: block maximum, one store per block
+    for (int off = 32; off > 0; off >>= 1) dmax = max_nan(dmax, __shfl_xor(dmax, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = dmax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max_nan(b, wmax[w]);
+        bmax[blockIdx.x] = b;
+    }
+}
+
+// one block: out = max of the n block maxima (NaN propagates)
+__global__ __launch_bounds__(256) void max_reduce_kernel(const double* __restrict__ in, int64_t n,
+                                                         double* __restrict__ out) {
+    __shared__ double wmax[4];
+    double m = 0.0;
+    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = max_nan(m, in[i]);
+    for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
+    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double b = wmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max_nan(b, wmax[w]);
+        *out = b;
+    }
+}
+
+template <int ND>
+__device__ __forceinline__ double grid_gather_slot(const double* __restrict__ V, const GridAsm& g, int64_t c, int s,
+                                                   const int64_t* __restrict__ bcolptr, double* __restrict__ bval,
+                                                   int32_t* __restrict__ browidx, int32_t* __restrict__ bcolidx,
+                                                   const double* __restrict__ inv_scale) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
     int64_t a[3];
     grid_coords<ND>(c, g, a);
     auto valid = [&](int sl, int64_t b[3]) {
@@ -232,7 +293,7 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restri
         return ok;
     };
     int64_t b[3];
-    if (!valid(s, b)) return;
+    if (!valid(s, b)) return 0.0;
     int k = 0;
     for (int sl = 0; sl < s; ++sl) {
         int64_t bb[3];
@@ -255,6 +316,7 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restri
         }
     }
     double acc[B2];
+    double dmax = 0.0;
     bool first = true;
     for (int kz = 0; kz < nc[2]; ++kz)
         for (int ky = 0; ky < nc[1]; ++ky)
@@ -265,14 +327,20 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restri
                 const int ac = g.corner[(int)(a[0] - cx) + 2 * (int)(a[1] - cy) + 4 * (int)(a[2] - cz)];
                 // element e's block (row node ar, column node ac)
                 const double* p = V + e * (int64_t)(NEN * NEN * B2) + (int64_t)(ar * NEN + ac) * B2;
+                const double* p0 = V + (int64_t)(ar * NEN + ac) * B2;
+                const double ise = inv_scale[e];
 #pragma unroll
-                for (int m = 0; m < B2; ++m) acc[m] = first ? p[m] : acc[m] + p[m];
+                for (int m = 0; m < B2; ++m) {
+                    acc[m] = first ? p[m] : acc[m] + p[m];
+                    dmax = max_nan(dmax, fabs(p[m] * ise - p0[m]));
+                }
                 first = false;
             }
 #pragma unroll
     for (int m = 0; m < B2; ++m) bval[u * B2 + m] = acc[m];
     browidx[u] = (int32_t)r;
     bcolidx[u] = (int32_t)c;
+    return dmax;
 }
 
 // Is IEN the meshgrid connectivity (corner order of element 0) and the node
@@ -423,14 +491,36 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         K->bcolidx.reset(ctx, (size_t)nnzb);
         K->bval.reset(ctx, (size_t)nnzb * bs * bs);
         const int ns = m->grid_nd == 3 ? 27 : 9;
+        K->escale.reset(ctx, (size_t)m->n_elem);
+        DevBuf<double> inv(ctx, (size_t)m->n_elem);
+        hipLaunchKernelGGL(grid_scale_kernel, dim3(grid_for(m->n_elem, 256)), dim3(256), 0, ctx->stream, V.get(),
+                           m->n_elem, KE, K->escale.get(), inv.get());
+        SFEM_LAUNCH_CHECK();
+        const unsigned gblocks = grid_for(nn * ns, 256);
+        DevBuf<double> bmax(ctx, gblocks), dev(ctx, 1);
         if (m->grid_nd == 3)
-            hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(grid_for(nn * ns, 256)), dim3(256), 0, ctx->stream,
-                               V.get(), g, nn, K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
+            hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
+                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get(),
+                               (const double*)inv.get(), bmax.get());
         else
-            hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(grid_for(nn * ns, 256)), dim3(256), 0, ctx->stream,
-                               V.get(), g, nn, K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
+            hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
+                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get(),
+                               (const double*)inv.get(), bmax.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double*)bmax.get(),
+                           (int64_t)gblocks, dev.get());
         SFEM_LAUNCH_CHECK();
         tm.mark();
+        K->ke0.resize((size_t)KE);
+        d2h(ctx, K->ke0.data(), V.get(), sizeof(double) * KE);
+        double dabs = 0.0;
+        d2h(ctx, &dabs, dev.get(), sizeof(dabs));
+        double kmax = 0.0;
+        for (double v : K->ke0) kmax = std::max(kmax, std::fabs(v));
+        K->ke_dev = (K->ke0[0] > 0.0 && kmax > 0.0 && std::isfinite(dabs)) ? dabs / kmax : HUGE_VAL;
+        K->grid_nd = m->grid_nd;
+        for (int d = 0; d < 3; ++d) K->grid_ne[d] = m->grid_ne[d];
+        for (int cc = 0; cc < 8; ++cc) K->grid_corner[cc] = m->grid_corner[cc];
         sync(ctx);
         K->t_validate = tm.ms(0, 1);
         K->t_element = tm.ms(1, 2);

@@ -76,6 +76,14 @@ struct MgHierarchy {
     double scale0 = 1.0;
     DevBuf<_Float16> h0;
     DevBuf<float> binv0;
+    // Matrix-free finest level on a grid whose element matrices are s_e Ke(0)
+    // (K's ke_dev): 1 the smoother, 2 also the outer FP64 operator.  kc:
+    // Ke(0) by corner codes; es32: the scales in FP32 (FP64: K's escale).
+    int mf = 0;
+    double ke_dev = HUGE_VAL;
+    DevBuf<float> kc32, es32;     // es: the scales with a ghost layer
+    DevBuf<double> kc64, es64;
+    DevBuf<int32_t> mf_rows;      // row_of_node with a ghost layer
 };
 
 void MgDelete::operator()(MgHierarchy* h) const { delete h; }
@@ -527,6 +535,142 @@ __global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp<TV> op, con
     if (s == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
 }
 
+// ------------------------------------------- matrix-free finest level --
+// K_ff v = sum over cells e of s_e Ke(0) v_e, per block row (a node): the
+// node's 3^ND neighbours are gathered once, each contributes to the cells it
+// shares with the node, and the 2^ND cell sums are scaled by their s_e.
+// kc: Ke(0) blocks (a*NEN + b) for corner codes a, b (ox + 2 oy + 4 oz).
+// Inputs are zero on fixed dofs (the V-cycle and CG keep them so); a fixed
+// dof of the row returns its input (the unit diagonal of the masked K_ff).
+// The node rows and cell scales are read from copies with a ghost layer
+// (one node / cell more on every side of the grid: row -1, scale 0), so
+// every neighbour and cell is the node's padded index plus a fixed offset;
+// rows of fixed nodes read the vectors' zero pad row `nrows`.
+template <typename T>
+struct MfOp {
+    const T* kc;
+    const T* es;                 // cells (e0+2) x (e1+2) [x (e2+2)], meshgrid order
+    const int32_t* node_of_row;
+    const int32_t* rows;         // nodes (n0+2) x (n1+2) [x (n2+2)]: block row or -1
+    const uint8_t* fixed;        // per row dof
+    uint32_t n0, n1;             // nodes per direction x, y
+    int64_t nrows;
+};
+
+// kc: Ke(0), read uniformly (scalar loads).  Node indices fit 31 bits (the
+// row maps are int32) and the vectors 4 GB (offsets are 32-bit byte counts).
+// Every load is unconditional and in bounds.  The neighbours are gathered
+// and applied one plane (z offset) at a time: a third of them in registers.
+template <int ND, typename T, typename VT>
+__device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t row, const VT* __restrict__ v,
+                                       T (&out)[ND]) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, NP = ND == 3 ? 3 : 1;
+    const uint32_t n0 = op.n0, n1 = op.n1;
+    const uint32_t node = (uint32_t)op.node_of_row[row];
+    const uint32_t cx = node % n0, yz = node / n0;
+    const uint32_t cy = ND == 3 ? yz % n1 : yz, cz = ND == 3 ? yz / n1 : 0;
+    const int P0 = (int)n0 + 2, P1 = (int)n1 + 2;  // padded nodes per direction
+    const int Q0 = (int)n0 + 1, Q1 = (int)n1 + 1;  // padded cells per direction
+    const int pn = (int)(cx + 1) + P0 * ((int)(cy + 1) + (ND == 3 ? P1 * (int)(cz + 1) : 0));
+    const int pc = (int)cx + Q0 * ((int)cy + (ND == 3 ? Q1 * (int)cz : 0));  // cell c - 1, padded
+    const char* vb = (const char*)v;
+    T se[NEN];
+#pragma unroll
+    for (int q = 0; q < NEN; ++q)  // cell c - 1 + (qx, qy, qz)
+        se[q] = op.es[(uint32_t)(pc + (q & 1) + Q0 * (((q >> 1) & 1) + (ND == 3 ? Q1 * (q >> 2) : 0)))];
+    T t[NEN][BS];
+#pragma unroll
+    for (int q = 0; q < NEN; ++q)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) t[q][i] = T(0);
+    T last = T(0);  // the previous block's last sum
+#pragma unroll
+    for (int pz = 0; pz < NP; ++pz) {
+        const int sz = ND == 3 ? pz : 1;  // z offset + 1
+        T vs[9][BS];
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            const int dx = s % 3 - 1, dy = s / 3 - 1, dz = sz - 1;
+            const int32_t r = op.rows[(uint32_t)(pn + dx + P0 * (dy + P1 * dz))];
+            const uint32_t rr = r >= 0 ? (uint32_t)r : (uint32_t)op.nrows;
+#pragma unroll
+            for (int j = 0; j < BS; ++j) vs[s][j] = (T)*(const VT*)(vb + (rr * BS + j) * (uint32_t)sizeof(VT));
+        }
+#pragma unroll
+        for (int s = 0; s < 9; ++s) {
+            const int sx = s % 3, sy = s / 3;
+#pragma unroll
+            for (int q = 0; q < NEN; ++q) {
+                const int qx = q & 1, qy = (q >> 1) & 1, qz = ND == 3 ? (q >> 2) & 1 : 0;
+                const int bx = sx - qx, by = sy - qy, bz = ND == 3 ? sz - qz : 0;
+                if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // not in cell q
+                const int a = (1 - qx) + 2 * (1 - qy) + (ND == 3 ? 4 * (1 - qz) : 0);
+                const int b = bx + 2 * by + 4 * bz;
+                // Ke(0)'s block (a, b): scalar loads (constant address space:
+                // unchanged during the kernel) next to their use.  Hoisted all
+                // together they exceed the scalar registers and spill; the
+                // address takes a zero the compiler cannot see through, made
+                // after the previous block's sums, so they stay in order.
+                int zero;
+                asm volatile("s_mov_b32 %0, 0" : "=s"(zero) : "v"(last));
+                const __attribute__((address_space(4))) T* k =
+                    (const __attribute__((address_space(4))) T*)kc + zero + (a * NEN + b) * B2;
+#pragma unroll
+                for (int i = 0; i < BS; ++i)
+#pragma unroll
+                    for (int j = 0; j < BS; ++j) t[q][i] = fma(k[i * BS + j], vs[s][j], t[q][i]);
+                last = t[q][BS - 1];
+            }
+        }
+    }
+#pragma unroll
+    for (int i = 0; i < BS; ++i) {
+        T acc = T(0);
+#pragma unroll
+        for (int q = 0; q < NEN; ++q) acc = fma(se[q], t[q][i], acc);
+        const T own = (T)v[row * BS + i];
+        out[i] = op.fixed[row * BS + i] ? own : acc;
+    }
+}
+
+// ghost-layer copies for the matrix-free operator: node rows (-1 outside)
+// and cell scales (0 outside); n: nodes or cells per direction, the copy
+// has n + 2 (directions beyond ND: 1, no layer)
+template <int ND>
+__global__ void ghost_rows_kernel(const int32_t* __restrict__ in, int64_t n0, int64_t n1, int64_t n2,
+                                  int32_t* __restrict__ out) {
+    const int64_t p0 = n0 + 2, p1 = n1 + 2, p2 = ND == 3 ? n2 + 2 : 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p0 * p1 * p2) return;
+    const int64_t x = i % p0 - 1, y = (i / p0) % p1 - 1, z = ND == 3 ? i / (p0 * p1) - 1 : 0;
+    const bool in_ = x >= 0 && x < n0 && y >= 0 && y < n1 && z >= 0 && z < (ND == 3 ? n2 : 1);
+    out[i] = in_ ? in[x + n0 * (y + n1 * z)] : -1;
+}
+
+template <int ND, typename T>
+__global__ void ghost_cells_kernel(const double* __restrict__ in, int64_t e0, int64_t e1, int64_t e2,
+                                   T* __restrict__ out) {
+    const int64_t p0 = e0 + 2, p1 = e1 + 2, p2 = ND == 3 ? e2 + 2 : 1;
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= p0 * p1 * p2) return;
+    const int64_t x = i % p0 - 1, y = (i / p0) % p1 - 1, z = ND == 3 ? i / (p0 * p1) - 1 : 0;
+    const bool in_ = x >= 0 && x < e0 && y >= 0 && y < e1 && z >= 0 && z < (ND == 3 ? e2 : 1);
+    out[i] = in_ ? (T)in[x + e0 * (y + e1 * z)] : T(0);
+}
+
+template <int ND, int MODE>
+__global__ __launch_bounds__(kThreads) void cheb_mf_kernel(MfOp<float> op, const float* v, float* __restrict__ r,
+                                                           float* __restrict__ x, const float* d_in,
+                                                           float* __restrict__ d_out, float c1, float c2,
+                                                           const float* __restrict__ binv, const int* done) {
+    if (*done) return;
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (row >= op.nrows) return;
+    float out[ND];
+    mf_row<ND, float, float>(op, op.kc, row, v, out);
+    cheb_epilogue<ND, MODE>(row, op.nrows, out, r, x, d_in, d_out, c1, c2, binv);
+}
+
 // First step from x = 0: r = b, d = c2 B^-1 b, x = d.  bscale: the units of
 // the level's operator (the FP16 finest level works on b / max |a|).
 template <int BS, typename TB>
@@ -813,6 +957,24 @@ __global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp<TV> op, cons
     if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
 }
 
+template <int ND>
+__global__ __launch_bounds__(kThreads) void pow_mf_kernel(MfOp<float> op, const float* __restrict__ v,
+                                                          const float* __restrict__ binv, float* __restrict__ w,
+                                                          PowState* ps, double* __restrict__ partials) {
+    __shared__ double sh[8];
+    const float scale = (float)ps->inv;
+    double sum = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < op.nrows;
+         row += (int64_t)gridDim.x * kThreads) {
+        float out[ND];
+        mf_row<ND, float, float>(op, op.kc, row, v, out);
+        sum += pow_epilogue<ND>(row, op.nrows, out, scale, binv, w);
+    }
+    const double b = block_sum_256(sum, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+}
+
 // ------------------------------------------------------- outer CG ----
 struct MgState {
     double bb, tol2bb;
@@ -1000,6 +1162,37 @@ __global__ __launch_bounds__(kThreads) void mg_spmv_kernel(const int64_t* __rest
     }
 }
 
+// q = K_ff p matrix-free in FP64, p.q -> alpha
+template <int ND>
+__global__ __launch_bounds__(kThreads) void mg_spmv_mf_kernel(MfOp<double> op, const double* __restrict__ p,
+                                                              double* __restrict__ q, MgState* st,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    double dot = 0.0;
+    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < op.nrows;
+         row += (int64_t)gridDim.x * kThreads) {
+        double out[ND];
+        mf_row<ND, double, double>(op, op.kc, row, p, out);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) {
+            const int64_t d = row * ND + j;
+            q[d] = out[j];
+            dot = fma(p[d], out[j], dot);
+        }
+    }
+    const double b = block_sum_256(dot, sh);
+    if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
+    if (last_block_arrive(&st->counter)) {
+        const double pq = sum_parts(partials, gridDim.x, sh);
+        if (threadIdx.x == 0) {
+            st->alpha = st->rz_old / pq;
+            if (!(pq > 0.0) || !isfinite(st->alpha)) st->done = 2;
+            st->counter = 0;
+        }
+    }
+}
+
 // x += alpha p, r -= alpha q, r.r; a recurrence at the tolerance asks for
 // the FP64 check
 __global__ __launch_bounds__(kThreads) void mg_update_kernel(double* __restrict__ x, double* __restrict__ r,
@@ -1059,6 +1252,23 @@ Launch plan_sell(int64_t nslices) {
 unsigned rows_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
 
 Grid grid_of(const MgLevel& l) { return Grid{l.n[0], l.n[1], l.n[2]}; }
+
+// Element-matrix deviations (K's ke_dev) up to which the finest level is
+// applied from the element scales: the smoother computes in FP32; the outer
+// FP64 operator then differs from the assembled K by the rounding of the
+// elements' coordinates (the stop still tests the assembled K's residual).
+constexpr double kMfSmootherDev = 1e-6;
+constexpr double kMfOuterDev = 1e-12;
+
+template <typename T>
+MfOp<T> mf_op(const sfem_system* S, const MgHierarchy& H, const T* kc, const T* es) {
+    return {kc, es, H.node_of_row.get(), H.mf_rows.get(), H.L[0].fixed.get(), (uint32_t)H.L[0].n[0],
+            (uint32_t)H.L[0].n[1], S->nfb};
+}
+
+MfOp<float> mf_op32(const sfem_system* S, const MgHierarchy& H) {
+    return mf_op<float>(S, H, H.kc32.get(), H.es32.get());
+}
 
 // Gauss-Jordan inverse with partial pivoting (FP64); false if singular.
 bool dense_inverse(std::vector<double>& a, int64_t n, std::vector<double>& inv) {
@@ -1282,8 +1492,47 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             v.st32.release();
         }
 
+    // matrix-free finest level: K assembled on this grid with element matrices
+    // s_e Ke(0) (within FP32 rounding for the smoother, within a few FP64
+    // roundings for the outer operator)
+    bool same_grid = K->grid_nd == ND;
+    for (int d = 0; d < ND; ++d) same_grid = same_grid && K->grid_ne[d] == L0.n[d] - 1;
+    H.ke_dev = same_grid ? K->ke_dev : HUGE_VAL;
+    if (o.matrix_free > 0 && same_grid && K->ke_dev <= kMfSmootherDev) {
+        constexpr int NEN = 1 << ND;
+        H.mf = o.matrix_free >= 2 && K->ke_dev <= kMfOuterDev ? 2 : 1;
+        std::vector<double> kc((size_t)NEN * NEN * B2);
+        std::vector<float> kc32(kc.size());
+        for (int a = 0; a < NEN; ++a)
+            for (int b = 0; b < NEN; ++b)
+                for (int m = 0; m < B2; ++m) {
+                    const size_t src = ((size_t)K->grid_corner[a] * NEN + K->grid_corner[b]) * B2 + m;
+                    kc[((size_t)a * NEN + b) * B2 + m] = K->ke0[src];
+                    kc32[((size_t)a * NEN + b) * B2 + m] = (float)K->ke0[src];
+                }
+        H.kc64.reset(ctx, kc.size());
+        h2d(ctx, H.kc64.get(), kc.data(), kc.size() * sizeof(double));
+        H.kc32.reset(ctx, kc32.size());
+        h2d(ctx, H.kc32.get(), kc32.data(), kc32.size() * sizeof(float));
+        const int64_t n0 = L0.n[0], n1 = L0.n[1], n2 = L0.n[2];
+        const int64_t pn = (n0 + 2) * (n1 + 2) * (ND == 3 ? n2 + 2 : 1);
+        const int64_t pc = (n0 + 1) * (n1 + 1) * (ND == 3 ? n2 + 1 : 1);
+        H.mf_rows.reset(ctx, (size_t)pn);
+        H.es32.reset(ctx, (size_t)pc);
+        H.es64.reset(ctx, (size_t)pc);
+        hipLaunchKernelGGL(ghost_rows_kernel<ND>, dim3(rows_grid(pn)), dim3(kThreads), 0, s,
+                           (const int32_t*)H.row_of_node.get(), n0, n1, n2, H.mf_rows.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((ghost_cells_kernel<ND, float>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
+                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es32.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((ghost_cells_kernel<ND, double>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
+                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es64.get());
+        SFEM_LAUNCH_CHECK();
+    }
+
     // FP16 finest-level smoothing operator
-    if (o.fine_fp16) {
+    if (o.fine_fp16 && !H.mf) {
         const int64_t nv = S->sell_blocks * B2;
         DevBuf<unsigned long long> mx(ctx, 1);
         memset_d(ctx, mx.get(), 0, sizeof(unsigned long long));
@@ -1309,14 +1558,17 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         H.fp16 = true;
     }
 
-    // work vectors
+    // work vectors (level 0: and a zero row, read for fixed nodes by the
+    // matrix-free operator)
     for (size_t l = 0; l < H.L.size(); ++l) {
         MgLevel& v = H.L[l];
-        const size_t n = (size_t)v.nrows * BS;
-        v.r.reset(ctx, n);
-        v.x.reset(ctx, n);
-        v.da.reset(ctx, n);
-        v.db.reset(ctx, n);
+        const size_t n = (size_t)v.nrows * BS, na = l == 0 ? n + BS : n;
+        v.r.reset(ctx, na);
+        v.x.reset(ctx, na);
+        v.da.reset(ctx, na);
+        v.db.reset(ctx, na);
+        if (l == 0)
+            for (float* b : {v.r.get(), v.x.get(), v.da.get(), v.db.get()}) memset_d(ctx, b, 0, na * sizeof(float));
         if (l > 0) v.b.reset(ctx, n);
     }
 
@@ -1340,7 +1592,10 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         float* vin = v.da.get();
         float* vout = v.db.get();
         for (int k = 0; k < o.power_iters; ++k) {
-            if (l == 0 && H.fp16) {
+            if (l == 0 && H.mf) {
+                hipLaunchKernelGGL(pow_mf_kernel<ND>, dim3(std::min(rows_grid(v.nrows), 1024u)), dim3(kThreads), 0,
+                                   s, mf_op32(S, H), vin, S->binv32.get(), vout, ps.get(), partials.get());
+            } else if (l == 0 && H.fp16) {
                 const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices,
                                           S->nfb, ls.per_block};
                 hipLaunchKernelGGL((pow_sell_kernel<BS, _Float16>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
@@ -1396,7 +1651,14 @@ struct MgRun {
         MgLevel& L = H.L[(size_t)l];
         const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
-        if (l == 0 && H.fp16) {
+        if (l == 0 && H.mf) {
+            auto go = [&](auto k) {
+                hipLaunchKernelGGL(k, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H), v, L.r.get(),
+                                   L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), done);
+            };
+            if (mode == 0) go(cheb_mf_kernel<ND, 0>);
+            else go(cheb_mf_kernel<ND, 1>);
+        } else if (l == 0 && H.fp16) {
             const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices, S->nfb,
                                       ls.per_block};
             auto go = [&](auto k) {
@@ -1503,12 +1765,18 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     const double* rhs = padded ? S->prhs.get() : S->rhs.get();
     const Launch ls = plan_sell(S->nslices);
     const Launch lv = plan_vec(n, (unsigned)std::max(8, ctx->n_cu * 4));
-    DevBuf<double> r(ctx, (size_t)n), p(ctx, (size_t)n), q(ctx, (size_t)n);
-    DevBuf<double> partials(ctx, 2 * (size_t)std::max(ls.grid, lv.grid));
+    // p: and a zero row, read for fixed nodes by the matrix-free operator
+    DevBuf<double> r(ctx, (size_t)n), p(ctx, (size_t)n + BS), q(ctx, (size_t)n);
+    // FP64 outer operator matrix-free when the hierarchy allows it
+    const bool mf_outer = H.mf == 2 && std::is_same<TV, double>::value;
+    const unsigned mf_grid = std::min(rows_grid(S->nfb), 8192u);
+    MfOp<double> mf64{};
+    if (mf_outer) mf64 = mf_op<double>(S, H, H.kc64.get(), H.es64.get());
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid}));
     DevBuf<MgState> dst(ctx, 1);
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x);
-    memset_d(ctx, p.get(), 0, sizeof(double) * n);
+    memset_d(ctx, p.get(), 0, sizeof(double) * (n + BS));
     MgState init{};
     init.best_true = HUGE_VAL;
     init.need_true = 1;
@@ -1565,7 +1833,9 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
                                    S->scol.get(), vals, S->nslices, S->nfb, ls.per_block, (const double*)p.get(),
                                    q.get(), st, partials.get());
             };
-            if (ctx->tune_nt) go(mg_spmv_kernel<BS, TV, true>);
+            if (mf_outer) hipLaunchKernelGGL(mg_spmv_mf_kernel<ND>, dim3(mf_grid), dim3(kThreads), 0, s, mf64,
+                                             (const double*)p.get(), q.get(), st, partials.get());
+            else if (ctx->tune_nt) go(mg_spmv_kernel<BS, TV, true>);
             else go(mg_spmv_kernel<BS, TV, false>);
             SFEM_LAUNCH_CHECK();
             hipLaunchKernelGGL(mg_update_kernel, dim3(lv.grid), dim3(kThreads), 0, s, x, r.get(),
@@ -1651,6 +1921,7 @@ void mg_opts_default(sfem_mg_opts* o) {
     o->coarse_nodes = 64;
     o->fine_fp16 = 1;
     o->coarse_fp16 = 1;
+    o->matrix_free = 2;
 }
 
 void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
@@ -1714,6 +1985,8 @@ void mg_info(const sfem_system* S, sfem_mg_info* info) {
     info->setup_ms = H.setup_ms;
     info->degree = H.degree;
     info->fine_fp16 = H.fp16 ? 1 : 0;
+    info->matrix_free = H.mf;
+    info->ke_deviation = H.ke_dev;
     info->coarse_fp16 = 0;
     for (size_t l = 1; l + 1 < H.L.size(); ++l)
         if (H.L[l].st16.size()) info->coarse_fp16 = 1;

@@ -50,6 +50,19 @@ struct sfem_csr {
     sfem::DevBuf<int32_t> browidx;  // nnzb
     sfem::DevBuf<int32_t> bcolidx;  // nnzb (column unit of each block)
     sfem::DevBuf<double> bval;      // nnzb * bs * bs
+    // Assembled on a structured grid (sfem_mesh_set_grid): its element counts
+    // and corner order, and how far the element matrices are from multiples
+    // of the first one, Ke(e) = s_e Ke(0): ke_dev = max over elements and
+    // entries of |Ke(e) / s_e - Ke(0)| / max |Ke(0)| (+inf when not measured).
+    // On a uniform grid with one Poisson ratio it is rounding (the elements'
+    // coordinates differ in their last bits); multigrid then applies the
+    // finest-level operator from s_e and Ke(0) without reading K.
+    int grid_nd = 0;
+    int64_t grid_ne[3] = {0, 0, 0};
+    int grid_corner[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    double ke_dev = HUGE_VAL;
+    sfem::DevBuf<double> escale;    // n_elem: s_e = Ke(e)[0][0] / Ke(0)[0][0]
+    std::vector<double> ke0;        // Ke(0), blocks (a*nen + b)*bs*bs + i*bs + j (local corners a, b)
     // device time per assembly stage (ms)
     double t_validate = 0, t_element = 0, t_keys = 0, t_sort = 0, t_reduce = 0, t_total = 0;
 };

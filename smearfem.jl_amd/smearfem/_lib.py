@@ -67,6 +67,7 @@ class MgOpts(C.Structure):
         ("coarse_nodes", C.c_int64),
         ("fine_fp16", C.c_int),
         ("coarse_fp16", C.c_int),
+        ("matrix_free", C.c_int),
     ]
 
 
@@ -81,6 +82,9 @@ class MgInfo(C.Structure):
         ("lambda_max", C.c_double * MG_MAX_LEVELS),
         ("coarse_dofs", C.c_int64),
         ("setup_ms", C.c_double),
+        ("matrix_free", C.c_int),
+        ("pad2", C.c_int),
+        ("ke_deviation", C.c_double),
     ]
 
 

@@ -123,7 +123,7 @@ class System:
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
     def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True,
-                 coarse_degree=0, coarse_fp16=True) -> dict:
+                 coarse_degree=0, coarse_fp16=True, matrix_free=2) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
@@ -136,6 +136,7 @@ class System:
         o.power_iters, o.coarse_nodes = int(power_iters), int(coarse_nodes)
         o.fine_fp16 = 1 if fine_fp16 else 0
         o.coarse_fp16 = 1 if coarse_fp16 else 0
+        o.matrix_free = int(matrix_free)
         self.ctx.check(self.ctx._lib.sfem_system_set_grid(self.ctx.handle, self.handle, int(d.size),
                                                           _ptr(d, C.c_int64), C.byref(o)))
         self.grid = tuple(int(v) for v in d)
@@ -150,7 +151,7 @@ class System:
                 "coarse_fp16": bool(info.coarse_fp16), "nodes": [info.nodes[i] for i in range(nl)],
                 "dims": [tuple(info.dims[i][d] for d in range(nd)) for i in range(nl)],
                 "lambda_max": [info.lambda_max[i] for i in range(nl)], "coarse_dofs": info.coarse_dofs,
-                "setup_ms": info.setup_ms}
+                "setup_ms": info.setup_ms, "matrix_free": info.matrix_free, "ke_deviation": info.ke_deviation}
 
     def mg_stencil(self, level: int):
         """Level >= 1 Galerkin operator as stored (FP64, fixed rows / columns

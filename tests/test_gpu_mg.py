@@ -281,3 +281,108 @@ def test_mg_bitwise_reproducible_and_errors(ctx):
         S3.set_grid((1, 1, 1))  # nothing to coarsen
     S3.close()
     K3.close()
+
+
+# ------------------------------------------------ matrix-free finest level --
+# Assembled on the grid (sfem_mesh_set_grid), K carries how far every element
+# matrix is from a multiple of the first one; the hierarchy then applies the
+# finest level from the element scales: the smoother (deviation <= 1e-6) and
+# the FP64 outer operator (<= 1e-12).  The deviation is pinned against the
+# oracle's element matrices; the operator through solves (with the FP64 outer
+# operator matrix-free the solution is the oracle's only if that operator is
+# K_ff: the stop tests the assembled K's residual).
+
+def grid_K(ctx, m, young, nu=0.3):
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, m.ndim, "Q1", m.ndim, m.ID, young, nu)
+    mesh.set_grid(m.dims)
+    K = mesh.assemble()
+    mesh.close()
+    return K
+
+
+def jittered(m, amount, seed):
+    """The mesh with interior nodes moved by up to amount x the element size
+    (grid topology kept): element matrices no longer multiples of one."""
+    rng = np.random.default_rng(seed)
+    h = np.array(m.lengths) / np.array(m.dims)
+    X = m.NodeList.copy()
+    gi = m.node_grid_index()
+    inner = np.all([(g > 0) & (g < d) for g, d in zip(gi, m.dims)], axis=0)
+    X[:, inner] += amount * h[:, None] * rng.uniform(-1, 1, (m.ndim, int(inner.sum())))
+    return type(m)(m.ndim, m.dims, m.lengths, X, m.IEN, m.ID)
+
+
+def oracle_ke_deviation(m, young, nu=0.3):
+    """max over elements and entries of |Ke(e) / s_e - Ke(0)| / max |Ke(0)|,
+    s_e = Ke(e)[0, 0] / Ke(0)[0, 0], from the oracle's element matrices."""
+    kes = [O.element_stiffness(m.ndim, "Q1", m.ndim, m.NodeList[:, m.IEN[e] - 1], young[e], nu)
+           for e in range(m.n_elem)]
+    k0 = kes[0]
+    return max(np.abs(k / (k[0, 0] / k0[0, 0]) - k0).max() for k in kes) / np.abs(k0).max()
+
+
+@pytest.mark.parametrize("dims,amount,mode", [((6, 4, 5), 0.0, 2), ((6, 4, 5), 1e-9, 1), ((6, 4, 5), 1e-3, 0),
+                                              ((9, 7), 0.0, 2), ((9, 7), 1e-3, 0)])
+def test_mg_ke_deviation_matches_oracle(ctx, dims, amount, mode):
+    print(f"[case] mg ke deviation {dims} jitter {amount}", flush=True)
+    m = structured_mesh(dims, lengths=(1.5, 1.0, 1.25)[: len(dims)])
+    if amount:
+        m = jittered(m, amount, seed=3)
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=4)
+    ref = oracle_ke_deviation(m, young)
+    K = grid_K(ctx, m, young)
+    S = System(K, BoundaryConditions(ctx, m.n_dof, np.zeros(m.n_dof), clamp_x0(m)))
+    info = S.set_grid(m.dims)
+    dev = info["ke_deviation"]
+    print(f"[mg] deviation device {dev:.3e} oracle {ref:.3e} -> matrix_free {info['matrix_free']}", flush=True)
+    if ref > 1e-12:
+        assert abs(dev - ref) <= 1e-6 * ref
+    else:  # rounding of the coordinates only: both at the level of FP64 rounding
+        assert ref <= 1e-13 and dev <= 1e-13
+    assert info["matrix_free"] == mode
+    S.close()
+    K.close()
+
+
+@pytest.mark.parametrize("dims", [(8, 4, 6), (7, 5, 3), (12, 6)])
+def test_mg_matrix_free_solves_match_oracle(ctx, dims):
+    print(f"[case] mg matrix-free solves {dims}", flush=True)
+    m = structured_mesh(dims, lengths=(2.0, 1.0, 1.5)[: len(dims)])
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=11)
+    fixed = roller(m, axis=m.ndim - 1)
+    f = random_face_load(m, m.ndim - 1, 1, fixed, seed=11)
+    q_d = np.zeros(m.n_dof)
+    gi = m.node_grid_index()
+    top = np.nonzero(gi[m.ndim - 1] == m.dims[-1])[0]
+    fixed[m.ndim * top + m.ndim - 1] = 1  # partially fixed nodes with a prescribed value
+    q_d[m.ndim * top + m.ndim - 1] = -0.01
+    ref = oracle_direct(oracle_K(m, young), fixed, f, q_d)
+    K = grid_K(ctx, m, young)
+    its = {}
+    for mode in (2, 1, 0):
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed, q_d)
+        S = System(K, bc)
+        bc.close()
+        info = S.set_grid(m.dims, matrix_free=mode)
+        assert info["matrix_free"] == mode
+        st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+        assert st["stop_reason"] == 1
+        assert relerr(S.solution(), ref) <= ERR
+        its[mode] = st["iterations"]
+        S.close()
+    K.close()
+    print(f"[mg] iterations by matrix_free mode {its}", flush=True)
+    assert its[1] <= its[0] + 3 and its[2] <= its[0] + 3
+
+
+def test_mg_matrix_free_needs_grid_assembly(ctx):
+    """K assembled without the grid (sorted keys): no element data, the
+    finest level reads K."""
+    m = structured_mesh((8, 4, 4))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=2)
+    Kd = device_K(ctx, m, young)
+    S = System(Kd, BoundaryConditions(ctx, m.n_dof, np.zeros(m.n_dof), clamp_x0(m)))
+    info = S.set_grid(m.dims)
+    assert info["matrix_free"] == 0 and info["ke_deviation"] == float("inf")
+    S.close()
+    Kd.close()
