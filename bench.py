@@ -47,8 +47,15 @@ ERR_BAR = 1e-8
 # WRITE_SIZE passes scaled by the copy-kernel calibration (fetch x2.00, write
 # x1.00); profiles/traffic_spmv_cg_hex8_256x64x64.md has the commands and data.
 MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9, "split/bsr": 1.7175e9,
-                    # finest-level multigrid smoothing step (FP16 operator), profiles/traffic_mg_smooth_hex8_256x64x64.md
-                    "mg-smooth": 7.4977e8}
+                    # finest-level multigrid smoothing step (FP16 operator; matrix-free),
+                    # profiles/traffic_mg_smooth_hex8_256x64x64.md
+                    "mg-smooth": 7.4977e8, "mg-smooth-mf": 2.1329e8}
+PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32, published (counts packed FP32 issue)
+# Algorithmic FLOPs of one matrix-free finest-level smoothing step per block
+# row (Hex8): 8 cells x 8 corners x 3x3 FMA for the cell sums, 8 x 3 FMA for
+# the cell scales, then r -= Av (3), z = B^-1 r (9 FMA), d = c1 d + c2 z
+# (3 FMA + 3 mul), x += d (3)
+MF_FLOPS_PER_ROW = 2 * (64 * 9 + 8 * 3 + 9 + 3) + 3 + 3 + 3
 
 
 def log(msg):
@@ -172,6 +179,8 @@ def kernel_probe(ctx, mesh, bc, path):
     precision, fmt = parts[0], parts[1]
     mg = len(parts) > 2 and parts[2] == "mg"
     stream_copy_bench(ctx, 1 << 30, 1)
+    if mg:  # grid-pattern assembly, as in the timed step (element scales: matrix-free finest level)
+        mesh.set_grid(DIMS)
     K = mesh.assemble()
     S = System(K, bc)
     if mg:  # hierarchy, then 10 multigrid-PCG iterations (30 finest-level smoothing steps)
@@ -265,8 +274,13 @@ def run_headline(args, ctx, m, young, fixed, f):
                 "eliminate_ms": r["eliminate_ms"], "download_s": r["download_s"]}
         if st.get("mg"):
             sm = st["smooth_ms"] / max(1, st["smooth_timed"])
+            dev = st["mg"]["ke_deviation"]
             out.update({"mg_setup_ms": st["mg"]["setup_ms"], "mg_levels": st["mg"]["levels"],
+                        "mg_matrix_free": int(st["mg"]["matrix_free"]),
+                        "mg_ke_deviation": dev if np.isfinite(dev) else None,
                         "mg_lambda_max": st["mg"]["lambda_max"][:-1], "smooth_avg_ms": sm,
+                        # the cantilever's fixed nodes are fully fixed: one block row per 3 free dofs
+                        "smooth_flops": MF_FLOPS_PER_ROW * (r["n_free"] // 3),
                         "smooth_GBs": st["smooth_bytes"] / (sm * 1e-3) / 1e9 if sm > 0 else None,
                         "smooth_bytes": st["smooth_bytes"], "smooth_timed": int(st["smooth_timed"])})
         return out
@@ -480,7 +494,19 @@ def main():
     dtype = {"fp64": "f64", "fp32_matrix": "f32 operator / f64 vectors", "mixed": "f32 CG / f64 residual",
              "split": "f64 vectors / f32+16-bit operator"}
     is_mg = best_name.endswith("/mg")
-    if is_mg:
+    mf = is_mg and best.get("mg_matrix_free", 0) >= 1
+    if mf:
+        # dominant kernel of a multigrid iteration: the finest-level Chebyshev
+        # smoothing step, matrix-free (element scales x Ke(0)): vector-ALU bound
+        sm = best["smooth_avg_ms"]
+        rl = {"bound": "valu", "kernel": "cheb_mf_kernel<3,0> (finest-level Chebyshev smoothing step, matrix-free: "
+                                         "FP32 element-scale x Ke(0) sums, FP32 vectors)",
+              "achieved": best["smooth_flops"] / (sm * 1e-3) / 1e12 if sm > 0 else None, "peak": PEAK_FP32_TFLOPS,
+              "unit": "TFLOP/s", "flops_per_launch": best["smooth_flops"], "bytes_per_launch": best["smooth_bytes"],
+              "avg_launch_ms": sm, "launches_timed": best["smooth_timed"],
+              "traffic": MEASURED_TRAFFIC.get("mg-smooth-mf"),
+              "attainable_without_packed_fp32_TFLOPs": PEAK_FP32_TFLOPS / 2}
+    elif is_mg:
         # dominant kernel of a multigrid iteration: the finest-level Chebyshev
         # smoothing step (3 of the 5 finest-level operator passes)
         rl = {"kernel": "cheb_sell_kernel (finest-level Chebyshev smoothing step: FP16 node-block operator, "
@@ -503,8 +529,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": dtype[best_name.split("/")[0]] + (" (multigrid V-cycle: f16 finest-level operator, f32 coarse "
-                                                   "levels and vectors)" if is_mg else ""),
+        "dtype": dtype[best_name.split("/")[0]] + (
+            (" (outer operator f64 matrix-free;" if best["mg_matrix_free"] >= 2 else " (") +
+            " multigrid V-cycle: f32 matrix-free finest level, f16 coarse operators, f32 vectors)" if mf else
+            " (multigrid V-cycle: f16 finest-level operator, f32 coarse levels and vectors)" if is_mg else ""),
         "data": "synthetic",
         "config": {
             "workload": f"hex8-{DIMS[0]}x{DIMS[1]}x{DIMS[2]}-cantilever assemble+eliminate+PCG {best_name}",
@@ -513,13 +541,14 @@ def main():
             "ref_rel_residual": hinfo["ref_rel_residual"], "device": info["name"],
         },
         "roofline": {
-            "bound": "hbm",
+            "bound": rl.get("bound", "hbm"),
             "kernel": rl["kernel"],
             "achieved": rl["achieved"],
-            "peak": PEAK_HBM_GBS,
-            "unit": "GB/s",
-            "frac": rl["achieved"] / PEAK_HBM_GBS,
+            "peak": rl.get("peak", PEAK_HBM_GBS),
+            "unit": rl.get("unit", "GB/s"),
+            "frac": rl["achieved"] / rl.get("peak", PEAK_HBM_GBS) if rl["achieved"] else None,
             "traffic": rl["traffic"],
+            **{k: rl[k] for k in ("flops_per_launch", "attainable_without_packed_fp32_TFLOPs") if k in rl},
             "bytes_per_launch": rl["bytes_per_launch"],
             "avg_launch_ms": rl["avg_launch_ms"],
             "launches_timed": rl["launches_timed"],

@@ -259,10 +259,10 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restri
     }
 }
 
-// one block: out = max of the n block maxima (NaN propagates)
-__global__ __launch_bounds__(256) void max_reduce_kernel(const double* __restrict__ in, int64_t n,
-                                                         double* __restrict__ out) {
-    __shared__ double wmax[4];
+// one block of 1024: out = max of the n block maxima (NaN propagates)
+__global__ __launch_bounds__(1024) void max_reduce_kernel(const double* __restrict__ in, int64_t n,
+                                                          double* __restrict__ out) {
+    __shared__ double wmax[16];
     double m = 0.0;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = max_nan(m, in[i]);
     for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
@@ -507,7 +507,7 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
                                K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get(),
                                (const double*)inv.get(), bmax.get());
         SFEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(256), 0, ctx->stream, (const double*)bmax.get(),
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
                            (int64_t)gblocks, dev.get());
         SFEM_LAUNCH_CHECK();
         tm.mark();

@@ -1889,12 +1889,17 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     stats->format = SFEM_FMT_BSR;
     stats->stop_reason = host.done == 0 ? 4 : host.done;
     stats->floor_residual = host.bb > 0 ? std::sqrt(host.floor_rr / 16.0 / host.bb) : 0.0;
-    // a smoothing step: FP16 or FP32 operator once, v gathered, r and x read and
-    // written, d read and written, the 3x3 (2x2) inverse blocks read
+    // a smoothing step: FP16 or FP32 operator once (matrix-free: the node map,
+    // ghost-layer row and one cell scale per row, the fixed flags), v
+    // gathered, r and x read and written, d read and written, the 3x3 (2x2)
+    // inverse blocks read
     stats->smooth_ms = smooth_ms;
     stats->smooth_timed = timed;
-    stats->smooth_bytes = (double)S->nnzb * (BS * BS * (H.fp16 ? 2.0 : 4.0) + 4.0) + (double)(S->nslices + 1) * 8.0 +
-                          (double)S->nslices * 4.0 + (double)n * 4.0 * (7.0 + BS);
+    if (H.mf)
+        stats->smooth_bytes = (double)S->nfb * (3 * 4.0 + BS) + (double)n * 4.0 * (6.0 + BS);
+    else
+        stats->smooth_bytes = (double)S->nnzb * (BS * BS * (H.fp16 ? 2.0 : 4.0) + 4.0) +
+                              (double)(S->nslices + 1) * 8.0 + (double)S->nslices * 4.0 + (double)n * 4.0 * (7.0 + BS);
 }
 
 template <int ND>
