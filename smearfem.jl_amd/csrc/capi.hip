@@ -487,14 +487,18 @@ int sfem_bc_create(sfem_ctx* ctx, int64_t n, const double* f, const uint8_t* fix
         else memset_d(ctx, bc->f.get(), 0, sizeof(double) * n);
         if (fixed) h2d(ctx, bc->fixed.get(), fixed, (size_t)n);
         else memset_d(ctx, bc->fixed.get(), 0, (size_t)n);
+        bool any = false;
         if (q_d && fixed) {  // prescribed values only on fixed dofs
             qd0.assign(q_d, q_d + n);
-            for (int64_t i = 0; i < n; ++i)
+            for (int64_t i = 0; i < n; ++i) {
                 if (!fixed[i]) qd0[i] = 0.0;
+                any = any || qd0[i] != 0.0;
+            }
             h2d(ctx, bc->qd.get(), qd0.data(), sizeof(double) * n);
         } else {
             memset_d(ctx, bc->qd.get(), 0, sizeof(double) * n);
         }
+        bc->qd_zero = !any;
         sync(ctx);
     });
     if (st) { delete bc; return st; }

@@ -113,32 +113,66 @@ __global__ void csr_fill_kernel(const int64_t* __restrict__ bcolptr, const int32
 
 // rhs and 1/diag of csr_fill_kernel without the CSR arrays (same loop, same
 // order, so the same bits); the arrays are built on first use (ensure_csr).
-__global__ void rhs_dinv_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
-                                const double* __restrict__ bval, const int32_t* __restrict__ free_of,
-                                const double* __restrict__ qd, const double* __restrict__ f, int64_t n, int bs,
-                                double* __restrict__ rhs, double* __restrict__ dinv) {
+// rhs = f on the free dofs (no prescribed value is nonzero)
+__global__ void rhs_gather_kernel(const int32_t* __restrict__ free_of, const double* __restrict__ f, int64_t n,
+                                  double* __restrict__ rhs) {
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int32_t row = free_of[s];
+    if (row >= 0) rhs[row] = f[s];
+}
+
+// point-Jacobi 1 / K_ss of every free dof (1 where the diagonal is zero)
+__global__ void dinv_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                            const double* __restrict__ bval, const int32_t* __restrict__ free_of, int64_t n, int bs,
+                            double* __restrict__ dinv) {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int32_t row = free_of[s];
     if (row < 0) return;
     const int64_t cu = s / bs;
     const int j = (int)(s - cu * bs);
-    const int bs2 = bs * bs;
-    double lift = 0.0, diag = 0.0;
-    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) {
-        const int64_t r = browidx[u];
-        for (int i = 0; i < bs; ++i) {
-            const int64_t d = r * bs + i;
-            const double v = bval[u * bs2 + i * bs + j];
-            if (free_of[d] >= 0) {
-                if (d == s) diag = v;
-            } else {
-                lift += v * qd[d];
-            }
-        }
-    }
-    rhs[row] = f[s] - lift;
+    double diag = 0.0;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u)
+        if (browidx[u] == cu) diag = bval[u * bs * bs + j * bs + j];
     dinv[row] = diag != 0.0 ? 1.0 / diag : 1.0;
+}
+
+// Inverse node diagonal blocks of the block rows (and, for padded vectors,
+// the point-Jacobi 1/diag with a unit diagonal on fixed dofs): the diagonal
+// block masked as sell_fill_kernel masks it, so both give the same values.
+template <int BS>
+__global__ void diag_blocks_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
+                                   const double* __restrict__ bval, const int32_t* __restrict__ ufree_of,
+                                   const int32_t* __restrict__ free_of, int64_t nunits, int64_t nfb,
+                                   double* __restrict__ binv64, float* __restrict__ binv32,
+                                   double* __restrict__ pdinv64) {
+    constexpr int B2 = BS * BS;
+    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (cu >= nunits) return;
+    const int32_t row = ufree_of[cu];
+    if (row < 0) return;
+    int64_t ud = -1;
+    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u)
+        if (browidx[u] == cu) ud = u;
+    double m[B2];
+#pragma unroll
+    for (int j = 0; j < BS; ++j)
+#pragma unroll
+        for (int i = 0; i < BS; ++i) {
+            const bool masked = free_of[cu * BS + j] < 0 || free_of[cu * BS + i] < 0;
+            m[j * BS + i] = masked || ud < 0 ? (i == j && masked ? 1.0 : 0.0) : bval[ud * B2 + i * BS + j];
+        }
+    if (pdinv64)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) pdinv64[(int64_t)row * BS + j] = m[j * BS + j] != 0.0 ? 1.0 / m[j * BS + j] : 1.0;
+    double inv[B2];
+    invert_block<BS>(m, inv);
+#pragma unroll
+    for (int v = 0; v < B2; ++v) {
+        binv64[v * nfb + row] = inv[v];
+        binv32[v * nfb + row] = (float)inv[v];
+    }
 }
 
 // rhs[row] = f[s] - (K q_d)[s] over the fixed columns, the sum csr_fill_kernel
@@ -231,7 +265,7 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
                                  int32_t* __restrict__ scol, double* __restrict__ sval64,
                                  float* __restrict__ sval32, int16_t* __restrict__ sval16, int64_t nfb,
                                  double* __restrict__ binv64, float* __restrict__ binv32,
-                                 const uint8_t* __restrict__ fixed, double* __restrict__ pdinv64) {
+                                 const int32_t* __restrict__ free_of, double* __restrict__ pdinv64) {
     constexpr int B2 = BS * BS;
     const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (cu >= nunits) return;
@@ -254,7 +288,7 @@ __global__ void sell_fill_kernel(const int64_t* __restrict__ bcolptr, const int3
         for (int j = 0; j < BS; ++j)
 #pragma unroll
             for (int i = 0; i < BS; ++i) {
-                const bool masked = fixed[cu * BS + j] || fixed[ru * BS + i];
+                const bool masked = free_of[cu * BS + j] < 0 || free_of[ru * BS + i] < 0;
                 m[j * BS + i] = masked ? ((ru == cu && i == j) ? 1.0 : 0.0) : bval[u * B2 + i * BS + j];
             }
 #pragma unroll
@@ -417,22 +451,27 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     S->val32.release();
     S->rhs.reset(ctx, nf);
     S->x.reset(ctx, nf);
-    S->dinv64.reset(ctx, nf);
-    S->dinv32.reset(ctx, nf);
+    S->dinv64.release();  // point Jacobi's 1/diag: on first use (ensure_dinv)
+    S->dinv32.release();
     memset_d(ctx, S->x.get(), 0, sizeof(double) * nf);
-    hipLaunchKernelGGL(rhs_dinv_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
-                       K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, bs, S->rhs.get(),
-                       S->dinv64.get());
+    if (bc->qd_zero)
+        hipLaunchKernelGGL(rhs_gather_kernel, gn, dim3(256), 0, ctx->stream, S->free_of.get(), S->f.get(), n,
+                           S->rhs.get());
+    else
+        hipLaunchKernelGGL(rhs_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                           K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, bs, S->rhs.get());
     SFEM_LAUNCH_CHECK();
-    if (nf) {
-        hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(nf, 256)), dim3(256), 0, ctx->stream, S->dinv64.get(),
-                           S->dinv32.get(), (int64_t)nf);
-        SFEM_LAUNCH_CHECK();
-    }
 
-    // node-block sliced format: every node with a free dof is a block row
+    // node-block sliced format: every node with a free dof is a block row;
+    // here the slices' shape, the row map and the inverse diagonal blocks,
+    // the values on first use (ensure_sell)
     S->has_block = false;
     S->partial = false;
+    S->sell_ready = false;
+    S->scol.release();
+    S->sval64.release();
+    S->sval32.release();
+    S->sval16.release();
     if (bs > 1 && nf > 0) {
         DevBuf<uint32_t> ufree(ctx, (size_t)nunits), upos(ctx, (size_t)nunits), utot(ctx, 1);
         DevBuf<int> dpartial(ctx, 1);
@@ -447,17 +486,18 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             exclusive_scan_u32_dev(ctx, ufree.get(), upos.get(), nunits, utot.get());
             uint32_t nfb = 0;
             d2h(ctx, &nfb, utot.get(), sizeof(nfb));
-            DevBuf<int32_t> ufree_of(ctx, (size_t)nunits);
+            S->ufree_of.reset(ctx, (size_t)nunits);
+            int32_t* ufree_of = S->ufree_of.get();
             // unit map: fixed unit <=> ufree == 0
             DevBuf<uint8_t> ufixed(ctx, (size_t)nunits);
             hipLaunchKernelGGL(unit_fixed_kernel, gu, dim3(256), 0, ctx->stream, ufree.get(), ufixed.get(), nunits);
             SFEM_LAUNCH_CHECK();
             hipLaunchKernelGGL(free_map_kernel, gu, dim3(256), 0, ctx->stream, ufixed.get(), upos.get(), nunits,
-                               ufree_of.get());
+                               ufree_of);
             SFEM_LAUNCH_CHECK();
             DevBuf<int32_t> blen(ctx, nfb);
             hipLaunchKernelGGL(blk_count_kernel, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
-                               ufree_of.get(), nunits, blen.get());
+                               (const int32_t*)ufree_of, nunits, blen.get());
             SFEM_LAUNCH_CHECK();
             const int64_t nslices = (nfb + 63) / 64;
             S->nfb = nfb;
@@ -476,8 +516,8 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
                 S->px.reset(ctx, (size_t)nfb * bs);
                 S->pdinv64.reset(ctx, (size_t)nfb * bs);
                 S->pdinv32.reset(ctx, (size_t)nfb * bs);
-                hipLaunchKernelGGL(pad_map_kernel, gu, dim3(256), 0, ctx->stream, ufree_of.get(), S->free_of.get(),
-                                   nunits, bs, S->pad_of.get());
+                hipLaunchKernelGGL(pad_map_kernel, gu, dim3(256), 0, ctx->stream, (const int32_t*)ufree_of,
+                                   S->free_of.get(), nunits, bs, S->pad_of.get());
                 SFEM_LAUNCH_CHECK();
             }
             double* pdinv = S->partial ? S->pdinv64.get() : nullptr;
@@ -491,31 +531,16 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
             const int64_t total = exclusive_scan_i64(ctx, cnt.get(), S->sl_ptr.get(), nslices + 1);
             if (total >= (int64_t(1) << 31)) fail(SFEM_ERR_UNSUPPORTED, "block format too large");
             S->sell_blocks = total;
-            S->scol.reset(ctx, (size_t)total);
-            S->sval64.reset(ctx, (size_t)total * bs * bs);
-            S->sval32.reset(ctx, (size_t)total * bs * bs);
-            S->sval16.reset(ctx, (size_t)total * bs * bs);
             S->binv64.reset(ctx, (size_t)nfb * bs * bs);
             S->binv32.reset(ctx, (size_t)nfb * bs * bs);
-            if (bs == 3) {
-                hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
-                                   K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
-                                   (int64_t)nfb, S->binv64.get(), S->binv32.get(), bc->fixed.get(), pdinv);
-                SFEM_LAUNCH_CHECK();
-                hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
-                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
-                                   S->sval16.get());
-            } else {
-                hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
-                                   K->browidx.get(), K->bval.get(), ufree_of.get(), nunits, S->sl_ptr.get(),
-                                   S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(), S->sval16.get(),
-                                   (int64_t)nfb, S->binv64.get(), S->binv32.get(), bc->fixed.get(), pdinv);
-                SFEM_LAUNCH_CHECK();
-                hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, (int64_t)nfb, nslices,
-                                   S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
-                                   S->sval16.get());
-            }
+            if (bs == 3)
+                hipLaunchKernelGGL(diag_blocks_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                                   K->browidx.get(), K->bval.get(), (const int32_t*)ufree_of, S->free_of.get(),
+                                   nunits, (int64_t)nfb, S->binv64.get(), S->binv32.get(), pdinv);
+            else
+                hipLaunchKernelGGL(diag_blocks_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                                   K->browidx.get(), K->bval.get(), (const int32_t*)ufree_of, S->free_of.get(),
+                                   nunits, (int64_t)nfb, S->binv64.get(), S->binv32.get(), pdinv);
             SFEM_LAUNCH_CHECK();
             if (S->partial) {
                 hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for((int64_t)nfb * bs, 256)), dim3(256), 0, ctx->stream,
@@ -533,6 +558,54 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
     S->t_eliminate_ms = ms;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
+}
+
+void ensure_sell(sfem_ctx* ctx, sfem_system* S) {
+    if (!S->has_block || S->sell_ready) return;
+    const sfem_csr* K = S->K;
+    const int bs = S->bs;
+    const int64_t nunits = K->nunits, total = S->sell_blocks;
+    const dim3 gu(grid_for(nunits, 256));
+    S->scol.reset(ctx, (size_t)total);
+    S->sval64.reset(ctx, (size_t)total * bs * bs);
+    S->sval32.reset(ctx, (size_t)total * bs * bs);
+    S->sval16.reset(ctx, (size_t)total * bs * bs);
+    // the inverse blocks (and padded 1/diag) are written again, the same values
+    double* pdinv = S->partial ? S->pdinv64.get() : nullptr;
+    if (bs == 3) {
+        hipLaunchKernelGGL(sell_fill_kernel<3>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                           K->bval.get(), S->ufree_of.get(), nunits, S->sl_ptr.get(), S->sl_w.get(), S->scol.get(),
+                           S->sval64.get(), S->sval32.get(), S->sval16.get(), S->nfb, S->binv64.get(),
+                           S->binv32.get(), S->free_of.get(), pdinv);
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(sell_pad_tail_kernel<3>, dim3(1), dim3(64), 0, ctx->stream, S->nfb, S->nslices,
+                           S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
+                           S->sval16.get());
+    } else {
+        hipLaunchKernelGGL(sell_fill_kernel<2>, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                           K->bval.get(), S->ufree_of.get(), nunits, S->sl_ptr.get(), S->sl_w.get(), S->scol.get(),
+                           S->sval64.get(), S->sval32.get(), S->sval16.get(), S->nfb, S->binv64.get(),
+                           S->binv32.get(), S->free_of.get(), pdinv);
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(sell_pad_tail_kernel<2>, dim3(1), dim3(64), 0, ctx->stream, S->nfb, S->nslices,
+                           S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval64.get(), S->sval32.get(),
+                           S->sval16.get());
+    }
+    SFEM_LAUNCH_CHECK();
+    S->sell_ready = true;
+}
+
+void ensure_dinv(sfem_ctx* ctx, sfem_system* S) {
+    if (S->dinv64.size() || S->nf == 0) return;
+    const sfem_csr* K = S->K;
+    S->dinv64.reset(ctx, (size_t)S->nf);
+    S->dinv32.reset(ctx, (size_t)S->nf);
+    hipLaunchKernelGGL(dinv_kernel, dim3(grid_for(S->n, 256)), dim3(256), 0, ctx->stream, K->bcolptr.get(),
+                       K->browidx.get(), K->bval.get(), S->free_of.get(), S->n, S->bs, S->dinv64.get());
+    SFEM_LAUNCH_CHECK();
+    hipLaunchKernelGGL(to_f32_kernel, dim3(grid_for(S->nf, 256)), dim3(256), 0, ctx->stream, S->dinv64.get(),
+                       S->dinv32.get(), S->nf);
+    SFEM_LAUNCH_CHECK();
 }
 
 void ensure_rowptr(sfem_ctx* ctx, sfem_system* S) {
@@ -587,8 +660,12 @@ void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc) {
     SFEM_HIP(hipMemcpyAsync(S->qd.get(), bc->qd.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     SFEM_HIP(hipMemcpyAsync(S->f.get(), bc->f.get(), sizeof(double) * n, hipMemcpyDeviceToDevice, ctx->stream));
     const sfem_csr* K = S->K;
-    hipLaunchKernelGGL(rhs_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(), K->bval.get(),
-                       S->free_of.get(), S->qd.get(), S->f.get(), n, S->bs, S->rhs.get());
+    if (bc->qd_zero)
+        hipLaunchKernelGGL(rhs_gather_kernel, gn, dim3(256), 0, ctx->stream, S->free_of.get(), S->f.get(), n,
+                           S->rhs.get());
+    else
+        hipLaunchKernelGGL(rhs_kernel, gn, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
+                           K->bval.get(), S->free_of.get(), S->qd.get(), S->f.get(), n, S->bs, S->rhs.get());
     SFEM_LAUNCH_CHECK();
     if (S->partial && S->has_block) pad_gather(ctx, S, S->rhs.get(), S->prhs.get());
     sync(ctx);

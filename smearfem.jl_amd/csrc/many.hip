@@ -374,6 +374,7 @@ struct ManyRun {
         DevBuf<ColumnState> cs(ctx, KP);
         memset_d(ctx, X.get(), 0, sizeof(double) * nv);
         launch_rhs_many(ctx, S, k, KP, dqd, df, RHS.get());
+        ensure_dinv(ctx, S);
         const double* dinv = S->dinv64.get();
         hipEvent_t e0, e1;
         SFEM_HIP(hipEventCreate(&e0));

@@ -154,6 +154,53 @@ __host__ __device__ __forceinline__ bool child(int64_t c, int k, int64_t e, int6
 // ------------------------------------------------------- hierarchy --
 // Level-0 maps: a node with any free dof is a block row (rank in node order,
 // as the elimination numbers them).
+// The fine stencil from the element scales (matrix-free finest level): slot
+// s of node I is the sum, over the cells I shares with its neighbour, of
+// s_e Ke(0)'s block (I's corner, the neighbour's corner); fixed rows and
+// columns zero, as stencil_from_k.  One thread per (slot, node), nodes
+// fastest; es: the scales with a ghost layer of zero cells.
+template <int ND>
+__global__ void stencil_from_mf_kernel(const double* __restrict__ kc, const double* __restrict__ es,
+                                       const int32_t* __restrict__ free_of, Grid g, double* __restrict__ st) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, NS = Sten<ND>::NS;
+    const int64_t nn = g.n0 * g.n1 * g.n2;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nn * NS) return;
+    const int s = (int)(t / nn);
+    const int64_t node = t % nn;
+    int64_t c[3];
+    coords_of<ND>(node, g, c);
+    int d[3];
+    offset_of<ND>(s, d[0], d[1], d[2]);
+    const bool in = c[0] + d[0] >= 0 && c[0] + d[0] < g.n0 && c[1] + d[1] >= 0 && c[1] + d[1] < g.n1 &&
+                    (ND == 2 || (c[2] + d[2] >= 0 && c[2] + d[2] < g.n2));
+    double acc[B2];
+#pragma unroll
+    for (int m = 0; m < B2; ++m) acc[m] = 0.0;
+    if (in) {
+        const int64_t q0 = g.n0 + 1, q1 = g.n1 + 1;  // padded cells per direction
+#pragma unroll
+        for (int q = 0; q < NEN; ++q) {  // cell c - 1 + (qx, qy, qz)
+            const int qx = q & 1, qy = (q >> 1) & 1, qz = ND == 3 ? (q >> 2) & 1 : 0;
+            const int bx = d[0] + 1 - qx, by = d[1] + 1 - qy, bz = ND == 3 ? d[2] + 1 - qz : 0;
+            if (bx < 0 || bx > 1 || by < 0 || by > 1 || bz < 0 || bz > 1) continue;  // not in cell q
+            const int a = (1 - qx) + 2 * (1 - qy) + (ND == 3 ? 4 * (1 - qz) : 0);
+            const int b = bx + 2 * by + 4 * bz;
+            const double se = es[(c[0] + qx) + q0 * ((c[1] + qy) + (ND == 3 ? q1 * (c[2] + qz) : 0))];
+#pragma unroll
+            for (int m = 0; m < B2; ++m) acc[m] = fma(se, kc[(a * NEN + b) * B2 + m], acc[m]);
+        }
+    }
+    const int64_t nb = node + d[0] + g.n0 * (d[1] + g.n1 * d[2]);
+#pragma unroll
+    for (int i = 0; i < BS; ++i)
+#pragma unroll
+        for (int j = 0; j < BS; ++j) {
+            const bool masked = !in || free_of[node * BS + i] < 0 || free_of[nb * BS + j] < 0;
+            st[((int64_t)s * B2 + i * BS + j) * nn + node] = masked ? 0.0 : acc[i * BS + j];
+        }
+}
+
 __global__ void node_free_kernel(const int32_t* __restrict__ free_of, int64_t nn, int bs,
                                  uint32_t* __restrict__ flag) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1012,6 +1059,77 @@ __global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restr
     }
 }
 
+__device__ void true_finish(MgState* st, double* __restrict__ partials, double* sh, double rr, double ff);
+
+// r = rhs - K_ff x in FP64 from K's block columns (K is exactly symmetric:
+// column c's blocks are row c's, transposed), ||r||^2 and the scale of the
+// rounding floor; fixed dofs of a row keep the unit diagonal.  32 lanes per
+// block row read its contiguous run of block values, coalesced, and sum by
+// a fixed butterfly.
+template <int BS>
+__global__ __launch_bounds__(kThreads) void mg_true_k_kernel(const int64_t* __restrict__ bcolptr,
+                                                             const int32_t* __restrict__ browidx,
+                                                             const double* __restrict__ bval,
+                                                             const int32_t* __restrict__ node_of_row,
+                                                             const int32_t* __restrict__ row_of_node,
+                                                             const int32_t* __restrict__ free_of, int64_t nrows,
+                                                             const double* __restrict__ x,
+                                                             const double* __restrict__ rhs, double* __restrict__ r,
+                                                             MgState* st, double* __restrict__ partials) {
+    constexpr int B2 = BS * BS, G = 32;
+    __shared__ double sh[8];
+    if (st->done || !st->need_true) return;
+    double rr = 0.0, ff = 0.0;
+    const int lane = threadIdx.x % G;
+    // every lane of a group runs the same trip count: stride and bound are
+    // multiples of G
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < nrows * G;
+         t += (int64_t)gridDim.x * kThreads) {
+        const int64_t row = t / G;
+        const int64_t cu = node_of_row[row];
+        const int64_t v0 = bcolptr[cu] * B2, v1 = bcolptr[cu + 1] * B2;
+        double out[BS], aout[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) out[j] = aout[j] = 0.0;
+        for (int64_t v = v0 + lane; v < v1; v += G) {
+            const int64_t u = v / B2;
+            const int m = (int)(v - u * B2), i = m / BS, j = m - i * BS;
+            const int64_t ru = browidx[u];
+            const int32_t c = row_of_node[ru];
+            if (c < 0 || free_of[ru * BS + i] < 0) continue;  // fixed column
+            const double p = bval[v] * x[(int64_t)c * BS + i];
+#pragma unroll
+            for (int jj = 0; jj < BS; ++jj) {
+                out[jj] += jj == j ? p : 0.0;
+                aout[jj] += jj == j ? fabs(p) : 0.0;
+            }
+        }
+#pragma unroll
+        for (int off = G / 2; off > 0; off >>= 1)
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                out[j] += __shfl_xor(out[j], off, G);
+                aout[j] += __shfl_xor(aout[j], off, G);
+            }
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < BS; ++j) {
+                const int64_t d = row * BS + j;
+                if (free_of[cu * BS + j] < 0) {  // unit diagonal
+                    out[j] = x[d];
+                    aout[j] = fabs(x[d]);
+                }
+                const double res = rhs[d] - out[j];
+                r[d] = res;
+                rr = fma(res, res, rr);
+                const double sc = fabs(rhs[d]) + aout[j];
+                ff = fma(sc, sc, ff);
+            }
+        }
+    }
+    true_finish(st, partials, sh, rr, ff);
+}
+
 // r = rhs - A x in FP64 when a check is due (else exits at once); decides
 // convergence, or replaces the residual and restarts the directions.
 template <int BS>
@@ -1046,6 +1164,12 @@ __global__ __launch_bounds__(kThreads) void mg_true_kernel(const int64_t* __rest
             }
         }
     }
+    true_finish(st, partials, sh, rr, ff);
+}
+
+// Block partials of ||r||^2 and the floor scale; the last block to arrive
+// forms the totals and decides: converged, breakdown, stagnation or restart.
+__device__ void true_finish(MgState* st, double* __restrict__ partials, double* sh, double rr, double ff) {
     const double b0 = block_sum_256(rr, sh);
     const double b1 = block_sum_256(ff, sh + 4);
     if (threadIdx.x == 0) {
@@ -1345,17 +1469,68 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
                        fixnode.get());
     SFEM_LAUNCH_CHECK();
 
-    // fine stencil (FP64, fixed rows and columns zero)
+    // matrix-free finest level: K assembled on this grid with element matrices
+    // s_e Ke(0) (within FP32 rounding for the smoother, within a few FP64
+    // roundings for the outer operator)
+    bool same_grid = K->grid_nd == ND;
+    for (int d = 0; d < ND; ++d) same_grid = same_grid && K->grid_ne[d] == L0.n[d] - 1;
+    H.ke_dev = same_grid ? K->ke_dev : HUGE_VAL;
+    if (o.matrix_free > 0 && same_grid && K->ke_dev <= kMfSmootherDev) {
+        constexpr int NEN = 1 << ND;
+        H.mf = o.matrix_free >= 2 && K->ke_dev <= kMfOuterDev ? 2 : 1;
+        std::vector<double> kc((size_t)NEN * NEN * B2);
+        std::vector<float> kc32(kc.size());
+        for (int a = 0; a < NEN; ++a)
+            for (int b = 0; b < NEN; ++b)
+                for (int m = 0; m < B2; ++m) {
+                    const size_t src = ((size_t)K->grid_corner[a] * NEN + K->grid_corner[b]) * B2 + m;
+                    kc[((size_t)a * NEN + b) * B2 + m] = K->ke0[src];
+                    kc32[((size_t)a * NEN + b) * B2 + m] = (float)K->ke0[src];
+                }
+        H.kc64.reset(ctx, kc.size());
+        h2d(ctx, H.kc64.get(), kc.data(), kc.size() * sizeof(double));
+        H.kc32.reset(ctx, kc32.size());
+        h2d(ctx, H.kc32.get(), kc32.data(), kc32.size() * sizeof(float));
+        const int64_t n0 = L0.n[0], n1 = L0.n[1], n2 = L0.n[2];
+        const int64_t pn = (n0 + 2) * (n1 + 2) * (ND == 3 ? n2 + 2 : 1);
+        const int64_t pc = (n0 + 1) * (n1 + 1) * (ND == 3 ? n2 + 1 : 1);
+        H.mf_rows.reset(ctx, (size_t)pn);
+        H.es32.reset(ctx, (size_t)pc);
+        H.es64.reset(ctx, (size_t)pc);
+        hipLaunchKernelGGL(ghost_rows_kernel<ND>, dim3(rows_grid(pn)), dim3(kThreads), 0, s,
+                           (const int32_t*)H.row_of_node.get(), n0, n1, n2, H.mf_rows.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((ghost_cells_kernel<ND, float>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
+                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es32.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL((ghost_cells_kernel<ND, double>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
+                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es64.get());
+        SFEM_LAUNCH_CHECK();
+    }
+
+    // the sliced values of K_ff: read by the finest level unless it is matrix-free
+    if (!H.mf) ensure_sell(ctx, S);
+
+    // fine stencil (FP64, fixed rows and columns zero): from the element
+    // scales when the finest level is matrix-free (K on this grid couples
+    // grid neighbours only), else from K
     DevBuf<double> stf(ctx, (size_t)nn0 * NS * B2);
-    memset_d(ctx, stf.get(), 0, stf.bytes());
-    DevBuf<int> bad(ctx, 1);
-    memset_d(ctx, bad.get(), 0, sizeof(int));
-    hipLaunchKernelGGL(stencil_from_k_kernel<ND>, dim3(rows_grid(K->nnzb)), dim3(kThreads), 0, s, K->bcolidx.get(),
-                       K->browidx.get(), K->bval.get(), K->nnzb, nn0, S->free_of.get(), g0, stf.get(), bad.get());
-    SFEM_LAUNCH_CHECK();
-    int hbad = 0;
-    d2h(ctx, &hbad, bad.get(), sizeof(int));
-    if (hbad) fail(SFEM_ERR_ARGUMENT, "multigrid: K couples nodes that are not neighbours on the given grid");
+    if (H.mf) {
+        hipLaunchKernelGGL(stencil_from_mf_kernel<ND>, dim3(rows_grid(nn0 * NS)), dim3(kThreads), 0, s,
+                           (const double*)H.kc64.get(), (const double*)H.es64.get(), S->free_of.get(), g0, stf.get());
+        SFEM_LAUNCH_CHECK();
+    } else {
+        memset_d(ctx, stf.get(), 0, stf.bytes());
+        DevBuf<int> bad(ctx, 1);
+        memset_d(ctx, bad.get(), 0, sizeof(int));
+        hipLaunchKernelGGL(stencil_from_k_kernel<ND>, dim3(rows_grid(K->nnzb)), dim3(kThreads), 0, s,
+                           K->bcolidx.get(), K->browidx.get(), K->bval.get(), K->nnzb, nn0, S->free_of.get(), g0,
+                           stf.get(), bad.get());
+        SFEM_LAUNCH_CHECK();
+        int hbad = 0;
+        d2h(ctx, &hbad, bad.get(), sizeof(int));
+        if (hbad) fail(SFEM_ERR_ARGUMENT, "multigrid: K couples nodes that are not neighbours on the given grid");
+    }
 
     // coarse levels
     const double* prev_st = stf.get();
@@ -1492,48 +1667,10 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             v.st32.release();
         }
 
-    // matrix-free finest level: K assembled on this grid with element matrices
-    // s_e Ke(0) (within FP32 rounding for the smoother, within a few FP64
-    // roundings for the outer operator)
-    bool same_grid = K->grid_nd == ND;
-    for (int d = 0; d < ND; ++d) same_grid = same_grid && K->grid_ne[d] == L0.n[d] - 1;
-    H.ke_dev = same_grid ? K->ke_dev : HUGE_VAL;
-    if (o.matrix_free > 0 && same_grid && K->ke_dev <= kMfSmootherDev) {
-        constexpr int NEN = 1 << ND;
-        H.mf = o.matrix_free >= 2 && K->ke_dev <= kMfOuterDev ? 2 : 1;
-        std::vector<double> kc((size_t)NEN * NEN * B2);
-        std::vector<float> kc32(kc.size());
-        for (int a = 0; a < NEN; ++a)
-            for (int b = 0; b < NEN; ++b)
-                for (int m = 0; m < B2; ++m) {
-                    const size_t src = ((size_t)K->grid_corner[a] * NEN + K->grid_corner[b]) * B2 + m;
-                    kc[((size_t)a * NEN + b) * B2 + m] = K->ke0[src];
-                    kc32[((size_t)a * NEN + b) * B2 + m] = (float)K->ke0[src];
-                }
-        H.kc64.reset(ctx, kc.size());
-        h2d(ctx, H.kc64.get(), kc.data(), kc.size() * sizeof(double));
-        H.kc32.reset(ctx, kc32.size());
-        h2d(ctx, H.kc32.get(), kc32.data(), kc32.size() * sizeof(float));
-        const int64_t n0 = L0.n[0], n1 = L0.n[1], n2 = L0.n[2];
-        const int64_t pn = (n0 + 2) * (n1 + 2) * (ND == 3 ? n2 + 2 : 1);
-        const int64_t pc = (n0 + 1) * (n1 + 1) * (ND == 3 ? n2 + 1 : 1);
-        H.mf_rows.reset(ctx, (size_t)pn);
-        H.es32.reset(ctx, (size_t)pc);
-        H.es64.reset(ctx, (size_t)pc);
-        hipLaunchKernelGGL(ghost_rows_kernel<ND>, dim3(rows_grid(pn)), dim3(kThreads), 0, s,
-                           (const int32_t*)H.row_of_node.get(), n0, n1, n2, H.mf_rows.get());
-        SFEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL((ghost_cells_kernel<ND, float>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
-                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es32.get());
-        SFEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL((ghost_cells_kernel<ND, double>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
-                           K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es64.get());
-        SFEM_LAUNCH_CHECK();
-    }
-
     // FP16 finest-level smoothing operator
     if (o.fine_fp16 && !H.mf) {
         const int64_t nv = S->sell_blocks * B2;
+        ensure_dinv(ctx, S);  // its largest entry sets the scale
         DevBuf<unsigned long long> mx(ctx, 1);
         memset_d(ctx, mx.get(), 0, sizeof(unsigned long long));
         hipLaunchKernelGGL(maxdiag_kernel, dim3(256), dim3(kThreads), 0, s, S->dinv64.get(), S->nf, mx.get());
@@ -1772,7 +1909,8 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     const unsigned mf_grid = std::min(rows_grid(S->nfb), 8192u);
     MfOp<double> mf64{};
     if (mf_outer) mf64 = mf_op<double>(S, H, H.kc64.get(), H.es64.get());
-    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid}));
+    else ensure_sell(ctx, S);  // the outer operator reads the sliced values
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u}));
     DevBuf<MgState> dst(ctx, 1);
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x);
@@ -1798,10 +1936,19 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     hipLaunchKernelGGL(mg_norm_kernel, dim3(lv.grid), dim3(kThreads), 0, s, rhs, n, lv.per_block, st, partials.get(),
                        o->rtol);
     SFEM_LAUNCH_CHECK();
+    // FP64 residual of the assembled K: from K's block columns when the
+    // finest level is matrix-free (K assembled on the grid, exactly
+    // symmetric; the sliced values need not exist), else the sliced values
+    const unsigned tk_grid = std::min(rows_grid(S->nfb * 32), 2048u);  // 32 lanes per row
     auto true_resid = [&] {
-        hipLaunchKernelGGL(mg_true_kernel<BS>, dim3(ls.grid), dim3(kThreads), 0, s, S->sl_ptr.get(), S->sl_w.get(),
-                           S->scol.get(), S->sval64.get(), S->nslices, S->nfb, ls.per_block, (const double*)x, rhs,
-                           r.get(), st, partials.get());
+        if (H.mf)
+            hipLaunchKernelGGL(mg_true_k_kernel<BS>, dim3(tk_grid), dim3(kThreads), 0, s, S->K->bcolptr.get(),
+                               S->K->browidx.get(), S->K->bval.get(), H.node_of_row.get(), H.row_of_node.get(),
+                               S->free_of.get(), S->nfb, (const double*)x, rhs, r.get(), st, partials.get());
+        else
+            hipLaunchKernelGGL(mg_true_kernel<BS>, dim3(ls.grid), dim3(kThreads), 0, s, S->sl_ptr.get(),
+                               S->sl_w.get(), S->scol.get(), S->sval64.get(), S->nslices, S->nfb, ls.per_block,
+                               (const double*)x, rhs, r.get(), st, partials.get());
         SFEM_LAUNCH_CHECK();
     };
     MgState host{};

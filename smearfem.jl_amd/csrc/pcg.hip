@@ -809,6 +809,10 @@ void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_
     if (!(o->rtol >= 0.0) || o->max_iter < 0) fail(SFEM_ERR_ARGUMENT, "solve: bad rtol or max_iter");
     const int fmt = resolve_format(S, o->format);
     if (fmt == SFEM_FMT_CSR) ensure_csr(ctx, S);
+    if (o->preconditioner != SFEM_PC_MULTIGRID) {  // multigrid builds what its levels read
+        if (fmt == SFEM_FMT_BSR) ensure_sell(ctx, S);
+        ensure_dinv(ctx, S);
+    }
     if (o->preconditioner == SFEM_PC_MULTIGRID) {
         if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "multigrid: needs the node-block format");
         mg_solve(ctx, S, o, st);
@@ -893,6 +897,7 @@ void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format,
                        double* bytes) {
     const int fmt = resolve_format(S, format);
     if (fmt == SFEM_FMT_CSR) ensure_csr(ctx, S);
+    if (fmt == SFEM_FMT_BSR) ensure_sell(ctx, S);
     if (precision != SFEM_PREC_FP64 && fmt == SFEM_FMT_CSR) ensure_val32(ctx, S);
     switch (precision) {
         case SFEM_PREC_FP64:

@@ -73,6 +73,7 @@ struct sfem_bc {
     sfem::DevBuf<double> f;        // n
     sfem::DevBuf<uint8_t> fixed;   // n
     sfem::DevBuf<double> qd;       // n (0 on free dofs)
+    bool qd_zero = false;          // known: no prescribed value is nonzero (rhs is f on the free dofs)
 };
 
 namespace sfem {
@@ -178,6 +179,11 @@ struct sfem_system {
     // vectors of nfb*bs values, mapped to the free dofs by pad_of (-1 = fixed).
     bool partial = false;
     sfem::DevBuf<int32_t> pad_of;
+    // node -> block row (-1: every dof fixed); the sliced values (scol,
+    // sval*) are filled on first use (ensure_sell), the point-Jacobi
+    // 1/diag too (ensure_dinv)
+    sfem::DevBuf<int32_t> ufree_of;
+    bool sell_ready = false;
     sfem::DevBuf<double> prhs, px, pdinv64;
     sfem::DevBuf<float> pdinv32;
     double t_eliminate_ms = 0;
@@ -197,6 +203,8 @@ void system_solution(sfem_ctx* ctx, const sfem_system* S, double* q, double* rea
 void system_set_bc(sfem_ctx* ctx, sfem_system* S, const sfem_bc* bc);
 // the scalar CSR of K_ff: row pointers and nnz (ensure_rowptr), columns and
 // FP64 values (ensure_csr), built on first use
+void ensure_sell(sfem_ctx* ctx, sfem_system* S);
+void ensure_dinv(sfem_ctx* ctx, sfem_system* S);
 void ensure_rowptr(sfem_ctx* ctx, sfem_system* S);
 void ensure_csr(sfem_ctx* ctx, sfem_system* S);
 // padded block vector <- compact free-dof vector (0 on padded fixed dofs), and back

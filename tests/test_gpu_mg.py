@@ -358,13 +358,15 @@ def test_mg_matrix_free_solves_match_oracle(ctx, dims):
     q_d[m.ndim * top + m.ndim - 1] = -0.01
     ref = oracle_direct(oracle_K(m, young), fixed, f, q_d)
     K = grid_K(ctx, m, young)
-    its = {}
+    its, coarse = {}, {}
     for mode in (2, 1, 0):
         bc = BoundaryConditions(ctx, m.n_dof, f, fixed, q_d)
         S = System(K, bc)
         bc.close()
         info = S.set_grid(m.dims, matrix_free=mode)
         assert info["matrix_free"] == mode
+        # the hierarchy starts from the element scales (modes 1, 2) or from K
+        coarse[mode] = S.mg_stencil(1)[0]
         st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
         assert st["stop_reason"] == 1
         assert relerr(S.solution(), ref) <= ERR
@@ -373,6 +375,8 @@ def test_mg_matrix_free_solves_match_oracle(ctx, dims):
     K.close()
     print(f"[mg] iterations by matrix_free mode {its}", flush=True)
     assert its[1] <= its[0] + 3 and its[2] <= its[0] + 3
+    for mode in (1, 2):
+        assert np.linalg.norm(coarse[mode] - coarse[0]) <= 1e-12 * np.linalg.norm(coarse[0])
 
 
 def test_mg_matrix_free_needs_grid_assembly(ctx):
