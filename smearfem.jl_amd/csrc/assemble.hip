@@ -325,13 +325,17 @@ __device__ __forceinline__ double grid_gather_slot(const double* __restrict__ V,
                 const int64_t e = cx + g.e[0] * (cy + g.e[1] * cz);
                 const int ar = g.corner[(int)(b[0] - cx) + 2 * (int)(b[1] - cy) + 4 * (int)(b[2] - cz)];
                 const int ac = g.corner[(int)(a[0] - cx) + 2 * (int)(a[1] - cy) + 4 * (int)(a[2] - cz)];
-                // element e's block (row node ar, column node ac)
-                const double* p = V + e * (int64_t)(NEN * NEN * B2) + (int64_t)(ar * NEN + ac) * B2;
-                const double* p0 = V + (int64_t)(ar * NEN + ac) * B2;
+                // element e's block (row node ar, column node ac), read as the
+                // transpose of block (ac, ar): element matrices are bitwise
+                // symmetric, and the blocks of one column corner ac lie
+                // together (the slots sharing this cell read adjacent blocks)
+                const double* p = V + e * (int64_t)(NEN * NEN * B2) + (int64_t)(ac * NEN + ar) * B2;
+                const double* p0 = V + (int64_t)(ac * NEN + ar) * B2;
                 const double ise = inv_scale[e];
 #pragma unroll
                 for (int m = 0; m < B2; ++m) {
-                    acc[m] = first ? p[m] : acc[m] + p[m];
+                    const double pv = p[(m % BS) * BS + m / BS];
+                    acc[m] = first ? pv : acc[m] + pv;
                     dmax = max_nan(dmax, fabs(p[m] * ise - p0[m]));
                 }
                 first = false;
