@@ -152,9 +152,22 @@ __global__ void diag_blocks_kernel(const int64_t* __restrict__ bcolptr, const in
     if (cu >= nunits) return;
     const int32_t row = ufree_of[cu];
     if (row < 0) return;
-    int64_t ud = -1;
-    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u)
-        if (browidx[u] == cu) ud = u;
+    // assembled columns list their rows in ascending order: binary search for
+    // the diagonal block, a scan when that misses (an uploaded, unsorted CSC)
+    int64_t lo = bcolptr[cu], hi = bcolptr[cu + 1], ud = -1;
+    while (lo < hi) {
+        const int64_t mid = (lo + hi) / 2;
+        const int64_t rm = browidx[mid];
+        if (rm == cu) {
+            ud = mid;
+            break;
+        }
+        if (rm < cu) lo = mid + 1;
+        else hi = mid;
+    }
+    if (ud < 0)
+        for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u)
+            if (browidx[u] == cu) ud = u;
     double m[B2];
 #pragma unroll
     for (int j = 0; j < BS; ++j)

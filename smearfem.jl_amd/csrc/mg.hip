@@ -631,15 +631,27 @@ __device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t r
 #pragma unroll
         for (int i = 0; i < BS; ++i) t[q][i] = T(0);
     T last = T(0);  // the previous block's last sum
+    // FP64: every neighbour's row first (one round trip, not one per plane);
+    // FP32 per plane (hoisted, the scalar registers spill)
+    constexpr bool HOIST = sizeof(T) == 8;
+    auto row_at = [&](int pz, int s) {
+        const int dx = s % 3 - 1, dy = s / 3 - 1, dz = ND == 3 ? pz - 1 : 0;
+        const int32_t r = op.rows[(uint32_t)(pn + dx + P0 * (dy + P1 * dz))];
+        return r >= 0 ? (uint32_t)r : (uint32_t)op.nrows;
+    };
+    uint32_t rn[NP][9];
+    if constexpr (HOIST)
+#pragma unroll
+        for (int pz = 0; pz < NP; ++pz)
+#pragma unroll
+            for (int s = 0; s < 9; ++s) rn[pz][s] = row_at(pz, s);
 #pragma unroll
     for (int pz = 0; pz < NP; ++pz) {
         const int sz = ND == 3 ? pz : 1;  // z offset + 1
         T vs[9][BS];
 #pragma unroll
         for (int s = 0; s < 9; ++s) {
-            const int dx = s % 3 - 1, dy = s / 3 - 1, dz = sz - 1;
-            const int32_t r = op.rows[(uint32_t)(pn + dx + P0 * (dy + P1 * dz))];
-            const uint32_t rr = r >= 0 ? (uint32_t)r : (uint32_t)op.nrows;
+            const uint32_t rr = HOIST ? rn[pz][s] : row_at(pz, s);
 #pragma unroll
             for (int j = 0; j < BS; ++j) vs[s][j] = (T)*(const VT*)(vb + (rr * BS + j) * (uint32_t)sizeof(VT));
         }
@@ -1064,8 +1076,8 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
 // r = rhs - K_ff x in FP64 from K's block columns (K is exactly symmetric:
 // column c's blocks are row c's, transposed), ||r||^2 and the scale of the
 // rounding floor; fixed dofs of a row keep the unit diagonal.  32 lanes per
-// block row read its contiguous run of block values, coalesced, and sum by
-// a fixed butterfly.
+// block row, one block per lane (adjacent blocks: coalesced), summed by a
+// fixed butterfly.
 template <int BS>
 __global__ __launch_bounds__(kThreads) void mg_true_k_kernel(const int64_t* __restrict__ bcolptr,
                                                              const int32_t* __restrict__ browidx,
@@ -1087,21 +1099,23 @@ __global__ __launch_bounds__(kThreads) void mg_true_k_kernel(const int64_t* __re
          t += (int64_t)gridDim.x * kThreads) {
         const int64_t row = t / G;
         const int64_t cu = node_of_row[row];
-        const int64_t v0 = bcolptr[cu] * B2, v1 = bcolptr[cu + 1] * B2;
         double out[BS], aout[BS];
 #pragma unroll
         for (int j = 0; j < BS; ++j) out[j] = aout[j] = 0.0;
-        for (int64_t v = v0 + lane; v < v1; v += G) {
-            const int64_t u = v / B2;
-            const int m = (int)(v - u * B2), i = m / BS, j = m - i * BS;
+        for (int64_t u = bcolptr[cu] + lane; u < bcolptr[cu + 1]; u += G) {
             const int64_t ru = browidx[u];
             const int32_t c = row_of_node[ru];
-            if (c < 0 || free_of[ru * BS + i] < 0) continue;  // fixed column
-            const double p = bval[v] * x[(int64_t)c * BS + i];
+            if (c < 0) continue;
 #pragma unroll
-            for (int jj = 0; jj < BS; ++jj) {
-                out[jj] += jj == j ? p : 0.0;
-                aout[jj] += jj == j ? fabs(p) : 0.0;
+            for (int i = 0; i < BS; ++i) {
+                if (free_of[ru * BS + i] < 0) continue;  // fixed column
+                const double xv = x[(int64_t)c * BS + i];
+#pragma unroll
+                for (int j = 0; j < BS; ++j) {
+                    const double p = bval[u * B2 + i * BS + j] * xv;
+                    out[j] += p;
+                    aout[j] += fabs(p);
+                }
             }
         }
 #pragma unroll
@@ -1286,9 +1300,10 @@ __global__ __launch_bounds__(kThreads) void mg_spmv_kernel(const int64_t* __rest
     }
 }
 
-// q = K_ff p matrix-free in FP64, p.q -> alpha
+// q = K_ff p matrix-free in FP64, p.q -> alpha (4 waves per SIMD: <= 128
+// registers)
 template <int ND>
-__global__ __launch_bounds__(kThreads) void mg_spmv_mf_kernel(MfOp<double> op, const double* __restrict__ p,
+__global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) void mg_spmv_mf_kernel(MfOp<double> op, const double* __restrict__ p,
                                                               double* __restrict__ q, MgState* st,
                                                               double* __restrict__ partials) {
     __shared__ double sh[8];
