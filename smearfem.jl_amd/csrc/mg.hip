@@ -1226,14 +1226,8 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
 }
 
 // r.z -> beta (0 after a restart)
-__global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restrict__ r, const float* __restrict__ z,
-                                                         int64_t n, int64_t per_block, MgState* st,
-                                                         double* __restrict__ partials) {
-    __shared__ double sh[8];
-    if (st->done) return;
-    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
-    double s = 0.0;
-    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(r[i], (double)z[i], s);
+// block partials of r.z; the last block to arrive forms rz and beta
+__device__ void rz_finish(MgState* st, double* __restrict__ partials, double* sh, double s) {
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
     if (last_block_arrive(&st->counter)) {
@@ -1250,6 +1244,41 @@ __global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restric
             st->counter = 0;
         }
     }
+}
+
+__global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restrict__ r, const float* __restrict__ z,
+                                                         int64_t n, int64_t per_block, MgState* st,
+                                                         double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+    double s = 0.0;
+    for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(r[i], (double)z[i], s);
+    rz_finish(st, partials, sh, s);
+}
+
+// The V-cycle's last finest-level smoothing step (matrix-free) with r.z of
+// the outer CG: each row adds r_outer . z for the z it has just written, so
+// the separate pass over r and z goes.  Every thread reaches the reduction.
+template <int ND>
+__global__ __launch_bounds__(kThreads) void cheb_mf_rz_kernel(MfOp<float> op, const float* v, float* __restrict__ r,
+                                                              float* __restrict__ x, const float* d_in,
+                                                              float* __restrict__ d_out, float c1, float c2,
+                                                              const float* __restrict__ binv,
+                                                              const double* __restrict__ r_outer, MgState* st,
+                                                              double* __restrict__ partials) {
+    __shared__ double sh[8];
+    if (st->done) return;
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    double s = 0.0;
+    if (row < op.nrows) {
+        float out[ND];
+        mf_row<ND, float, float>(op, op.kc, row, v, out);
+        cheb_epilogue<ND, 0>(row, op.nrows, out, r, x, d_in, d_out, c1, c2, binv);
+#pragma unroll
+        for (int j = 0; j < ND; ++j) s = fma(r_outer[row * ND + j], (double)x[row * ND + j], s);
+    }
+    rz_finish(st, partials, sh, s);
 }
 
 __global__ __launch_bounds__(kThreads) void mg_p_kernel(double* __restrict__ p, const float* __restrict__ z, int64_t n,
@@ -1798,12 +1827,23 @@ struct MgRun {
     bool nt;
     std::vector<hipEvent_t>* ev = nullptr;  // non-null: time the finest-level smoothing steps
     size_t ev_next = 0;
+    // non-null: the last finest-level step also forms the outer r.z
+    MgState* rz_st = nullptr;
+    const double* rz_r = nullptr;
+    double* rz_partials = nullptr;
 
-    void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2) {
+    void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2,
+              bool last = false) {
         MgLevel& L = H.L[(size_t)l];
-        const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
+        const bool fused = l == 0 && H.mf && last && rz_st;
+        // timed: the plain finest-level smoothing steps
+        const bool timed = ev && l == 0 && mode == 0 && !fused && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
-        if (l == 0 && H.mf) {
+        if (fused) {
+            hipLaunchKernelGGL(cheb_mf_rz_kernel<ND>, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H),
+                               v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), rz_r,
+                               rz_st, rz_partials);
+        } else if (l == 0 && H.mf) {
             auto go = [&](auto k) {
                 hipLaunchKernelGGL(k, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H), v, L.r.get(),
                                    L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), done);
@@ -1890,11 +1930,11 @@ struct MgRun {
                            l == 0 ? (const int32_t*)H.node_of_row.get() : nullptr, L.nrows,
                            (const uint8_t*)L.fixed.get(), L.x.get(), L.da.get(), done);
         SFEM_LAUNCH_CHECK();
-        step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0]);
+        step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0], L.degree == 1);
         dp = L.db.get();
         dn = L.da.get();
         for (int k = 1; k < L.degree; ++k) {
-            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
+            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k], k == L.degree - 1);
             std::swap(dp, dn);
         }
     }
@@ -1925,7 +1965,7 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     MfOp<double> mf64{};
     if (mf_outer) mf64 = mf_op<double>(S, H, H.kc64.get(), H.es64.get());
     else ensure_sell(ctx, S);  // the outer operator reads the sliced values
-    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u}));
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u, rows_grid(S->nfb)}));
     DevBuf<MgState> dst(ctx, 1);
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x);
@@ -1941,6 +1981,11 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     h2d(ctx, dst.get(), &init, sizeof(init));
     MgState* st = dst.get();
     MgRun<ND, TV> run{ctx, S, H, s, ls, &st->done, ctx->tune_nt};
+    if (H.mf) {  // the last finest-level smoothing step also forms r.z
+        run.rz_st = st;
+        run.rz_r = r.get();
+        run.rz_partials = partials.get();
+    }
     const float* z = H.L[0].x.get();
     Vals<TV> vals = outer_vals<TV>(S);
 
@@ -1970,7 +2015,8 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     int64_t enq = 0, counted = 0, timed = 0;
     double smooth_ms = 0.0;
     const int chunk_max = std::max(1, std::min(o->check_every, 8));
-    const int per_iter = 2 * H.L[0].degree - 1;  // finest-level smoothing steps per V-cycle
+    // plain finest-level smoothing steps per V-cycle (the fused last one untimed)
+    const int per_iter = 2 * H.L[0].degree - 1 - (run.rz_st ? 1 : 0);
     std::vector<hipEvent_t> ev;
     if (o->time_kernels) {
         ev.resize(2 * (size_t)per_iter * chunk_max);
@@ -1984,9 +2030,11 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         for (int64_t k = 0; k < chunk; ++k) {
             true_resid();
             run.vcycle(0, r.get());
-            hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
-                               lv.per_block, st, partials.get());
-            SFEM_LAUNCH_CHECK();
+            if (!run.rz_st) {  // else the V-cycle's last step formed r.z
+                hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
+                                   lv.per_block, st, partials.get());
+                SFEM_LAUNCH_CHECK();
+            }
             hipLaunchKernelGGL(mg_p_kernel, dim3(lv.grid), dim3(kThreads), 0, s, p.get(), z, n, lv.per_block,
                                (const MgState*)st);
             SFEM_LAUNCH_CHECK();
