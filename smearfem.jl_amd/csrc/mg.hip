@@ -152,8 +152,6 @@ __host__ __device__ __forceinline__ bool child(int64_t c, int k, int64_t e, int6
 }
 
 // ------------------------------------------------------- hierarchy --
-// Level-0 maps: a node with any free dof is a block row (rank in node order,
-// as the elimination numbers them).
 // The fine stencil from the element scales (matrix-free finest level): slot
 // s of node I is the sum, over the cells I shares with its neighbour, of
 // s_e Ke(0)'s block (I's corner, the neighbour's corner); fixed rows and
@@ -201,6 +199,8 @@ __global__ void stencil_from_mf_kernel(const double* __restrict__ kc, const doub
         }
 }
 
+// Level-0 maps: a node with any free dof is a block row (rank in node order,
+// as the elimination numbers them).
 __global__ void node_free_kernel(const int32_t* __restrict__ free_of, int64_t nn, int bs,
                                  uint32_t* __restrict__ flag) {
     const int64_t u = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -280,53 +280,91 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
 
 // Galerkin product along one direction: P = P_x (x) P_y (x) P_z, so P'AP
 // is three one-direction products, each coarsening direction d (child())
-// and keeping the others.  One thread per (output node I, slot s):
+// and keeping the others:
 // A'(I, I+D)[a][b] = sum over the support pairs (i_d of I_d, j_d of I_d+D_d)
 // of w_i w_j A(i, i+o)[a][b], with i = I except along d, o = D except
-// o_d = j_d - i_d (pairs more than one apart add nothing), summed in a
-// fixed order.
+// o_d = j_d - i_d (pairs more than one apart add nothing).  One thread per
+// (coarse node, line of the 3 slots that differ only along d): the line's
+// outputs read the same <= 9 input blocks (3 fine children x 3 offsets along
+// d), each once.  Each output sums its terms in ascending order of the child
+// index, then of the neighbour's fine index.
 template <int ND>
-__global__ void galerkin_dir_kernel(const double* __restrict__ in, Grid gi, int d, Grid go,
-                                    double* __restrict__ out) {
-    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
+__global__ void galerkin_line_kernel(const double* __restrict__ in, Grid gi, int d, Grid go,
+                                     double* __restrict__ out) {
+    constexpr int BS = ND, B2 = BS * BS, NL = Sten<ND>::NS / 3;
     const int64_t nno = go.n0 * go.n1 * go.n2, nni = gi.n0 * gi.n1 * gi.n2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nno * NS) return;
-    const int s = (int)(t / nno);
-    const int64_t I = t - (int64_t)s * nno;
+    if (t >= nno * NL) return;
+    const int l = (int)(t / nno);
+    const int64_t I = t - (int64_t)l * nno;
     int64_t c[3];
     coords_of<ND>(I, go, c);
-    int D[3];
-    offset_of<ND>(s, D[0], D[1], D[2]);
-    double acc[B2];
+    int D[3] = {0, 0, 0};
+    {
+        int q = l;
+        for (int dd = 0; dd < ND; ++dd) {
+            if (dd == d) continue;
+            D[dd] = q % 3 - 1;
+            q /= 3;
+        }
+    }
+    bool ok[3];
+    for (int k = 0; k < 3; ++k) {
+        int64_t cc[3] = {c[0] + D[0], c[1] + D[1], c[2] + D[2]};
+        cc[d] = c[d] + k - 1;
+        ok[k] = inside<ND>(go, cc[0], cc[1], cc[2]);
+    }
+    double acc[3][B2];
 #pragma unroll
-    for (int m = 0; m < B2; ++m) acc[m] = 0.0;
-    if (inside<ND>(go, c[0] + D[0], c[1] + D[1], c[2] + D[2])) {
-        const int64_t e = (d == 0 ? gi.n0 : d == 1 ? gi.n1 : gi.n2) - 1;
+    for (int k = 0; k < 3; ++k)
+#pragma unroll
+        for (int m = 0; m < B2; ++m) acc[k][m] = 0.0;
+    const int64_t e = (d == 0 ? gi.n0 : d == 1 ? gi.n1 : gi.n2) - 1;
+    if (ok[0] || ok[1] || ok[2]) {
         for (int ka = 0; ka < 3; ++ka) {
             int64_t ia;
             double wa;
             if (!child(c[d], ka, e, ia, wa)) continue;
-            for (int kb = 0; kb < 3; ++kb) {
-                int64_t jb;
-                double wb;
-                if (!child(c[d] + D[d], kb, e, jb, wb)) continue;
-                const int64_t o = jb - ia;
-                if (o < -1 || o > 1) continue;
+            for (int o = -1; o <= 1; ++o) {
+                const int64_t jb = ia + o;
+                if (jb < 0 || jb > e) continue;
                 int64_t ic[3] = {c[0], c[1], c[2]};
                 ic[d] = ia;
                 int od[3] = {D[0], D[1], D[2]};
-                od[d] = (int)o;
-                const double w = wa * wb;
-                const int sf = slot_of<ND>(od[0], od[1], od[2]);
-                const double* src = in + (int64_t)sf * B2 * nni + node_at(gi, ic[0], ic[1], ic[2]);
+                od[d] = o;
+                const double* src = in + (int64_t)slot_of<ND>(od[0], od[1], od[2]) * B2 * nni +
+                                    node_at(gi, ic[0], ic[1], ic[2]);
+                double v[B2];
+                bool loaded = false;
 #pragma unroll
-                for (int m = 0; m < B2; ++m) acc[m] = fma(w, src[(int64_t)m * nni], acc[m]);
+                for (int k = 0; k < 3; ++k) {
+                    if (!ok[k]) continue;
+                    // jb a child of the coarse neighbour c[d] + k - 1, with weight wb
+                    for (int kb = 0; kb < 3; ++kb) {
+                        int64_t jj;
+                        double wb;
+                        if (!child(c[d] + k - 1, kb, e, jj, wb) || jj != jb) continue;
+                        if (!loaded) {
+#pragma unroll
+                            for (int m = 0; m < B2; ++m) v[m] = src[(int64_t)m * nni];
+                            loaded = true;
+                        }
+                        const double w = wa * wb;
+#pragma unroll
+                        for (int m = 0; m < B2; ++m) acc[k][m] = fma(w, v[m], acc[k][m]);
+                    }
+                }
             }
         }
     }
 #pragma unroll
-    for (int m = 0; m < B2; ++m) out[((int64_t)s * B2 + m) * nno + I] = acc[m];
+    for (int k = 0; k < 3; ++k) {
+        int od[3] = {D[0], D[1], D[2]};
+        od[d] = k - 1;
+        const int s = slot_of<ND>(od[0], od[1], od[2]);
+#pragma unroll
+        for (int m = 0; m < B2; ++m) out[((int64_t)s * B2 + m) * nno + I] = acc[k][m];
+    }
 }
 
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
@@ -1617,8 +1655,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             *nd = *nd / 2 + 1;  // e = n - 1 elements -> ceil(e / 2)
             const int64_t nno = gn.n0 * gn.n1 * gn.n2;
             DevBuf<double> o(ctx, (size_t)nno * NS * B2);
-            hipLaunchKernelGGL(galerkin_dir_kernel<ND>, dim3(rows_grid(nno * NS)), dim3(kThreads), 0, s, cur, gcur, d,
-                               gn, o.get());
+            hipLaunchKernelGGL(galerkin_line_kernel<ND>, dim3(rows_grid(nno * (NS / 3))), dim3(kThreads), 0, s, cur,
+                               gcur, d, gn, o.get());
             SFEM_LAUNCH_CHECK();
             if (d == last) {
                 raw = std::move(o);
