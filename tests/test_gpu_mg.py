@@ -372,11 +372,23 @@ def test_mg_matrix_free_solves_match_oracle(ctx, dims):
         assert relerr(S.solution(), ref) <= ERR
         its[mode] = st["iterations"]
         S.close()
-    K.close()
     print(f"[mg] iterations by matrix_free mode {its}", flush=True)
     assert its[1] <= its[0] + 3 and its[2] <= its[0] + 3
     for mode in (1, 2):
         assert np.linalg.norm(coarse[mode] - coarse[0]) <= 1e-12 * np.linalg.norm(coarse[0])
+    # the split outer operator reads the sliced values (built on first use by
+    # the solve); FP32 coarse operators under the matrix-free finest level
+    for precision, cfp16 in (("split", True), ("fp64", False)):
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed, q_d)
+        S = System(K, bc)
+        bc.close()
+        info = S.set_grid(m.dims, coarse_fp16=cfp16)
+        assert info["matrix_free"] == 2 and info["coarse_fp16"] == (cfp16 and info["levels"] > 2)
+        st = S.solve(precision, rtol=1e-12, preconditioner="multigrid")
+        assert st["stop_reason"] == 1
+        assert relerr(S.solution(), ref) <= ERR
+        S.close()
+    K.close()
 
 
 def test_mg_matrix_free_needs_grid_assembly(ctx):
