@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restri
                                                           const double* __restrict__ inv_scale,
                                                           double* __restrict__ bmax) {
     __shared__ double wmax[4];
-    constexpr int BS = ND, B2 = BS * BS, NS = ND == 3 ? 27 : 9;
+    constexpr int NS = ND == 3 ? 27 : 9;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int64_t c = t / NS;
     const int s = (int)(t % NS);

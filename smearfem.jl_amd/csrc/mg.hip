@@ -1557,7 +1557,9 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     bool same_grid = K->grid_nd == ND;
     for (int d = 0; d < ND; ++d) same_grid = same_grid && K->grid_ne[d] == L0.n[d] - 1;
     H.ke_dev = same_grid ? K->ke_dev : HUGE_VAL;
-    if (o.matrix_free > 0 && same_grid && K->ke_dev <= kMfSmootherDev) {
+    // the matrix-free kernels address vectors with 32-bit byte offsets
+    const bool mf_fits = (uint64_t)(S->nfb + 1) * BS * sizeof(double) < (uint64_t(1) << 32);
+    if (o.matrix_free > 0 && same_grid && mf_fits && K->ke_dev <= kMfSmootherDev) {
         constexpr int NEN = 1 << ND;
         H.mf = o.matrix_free >= 2 && K->ke_dev <= kMfOuterDev ? 2 : 1;
         std::vector<double> kc((size_t)NEN * NEN * B2);
