@@ -817,8 +817,8 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     // along directions that are not coarsened)
     const int f[3] = {f0, f1, f2};
     const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
-    int64_t fi[3][3];
-    float fw[3][3];
+    int64_t fi[3][3] = {};  // unused entries: node 0, weight 0
+    float fw[3][3] = {};
     int nk[3];
     for (int d = 0; d < 3; ++d) {
         nk[d] = 0;
@@ -841,13 +841,21 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     float acc[BS];
 #pragma unroll
     for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
-    for (int kz = 0; kz < nk[2]; ++kz)
-        for (int ky = 0; ky < nk[1]; ++ky)
-            for (int kx = 0; kx < nk[0]; ++kx) {
+    // a fixed 3x3x3 loop, so all loads can be in flight at once: the missing
+    // support points (and fully fixed fine nodes) read the first one with
+    // weight 0, which leaves the sums exactly as the skipped terms would
+#pragma unroll
+    for (int kz = 0; kz < 3; ++kz)
+#pragma unroll
+        for (int ky = 0; ky < 3; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 3; ++kx) {
+                const bool in = kx < nk[0] && ky < nk[1] && kz < nk[2];
                 const int64_t fn = node_at(gf, fi[0][kx], fi[1][ky], fi[2][kz]);
-                const int64_t row = row_of_node ? (int64_t)row_of_node[fn] : fn;
-                if (row < 0) continue;
-                const float w = fw[0][kx] * fw[1][ky] * fw[2][kz];
+                const int64_t mrow = row_of_node ? (int64_t)row_of_node[fn] : fn;
+                const bool use = in && mrow >= 0;
+                const int64_t row = use ? mrow : 0;
+                const float w = use ? fw[0][kx] * fw[1][ky] * fw[2][kz] : 0.0f;
 #pragma unroll
                 for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
             }
