@@ -921,11 +921,17 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
     float acc[BS];
 #pragma unroll
     for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
-    for (int kz = 0; kz < np[2]; ++kz)
-        for (int ky = 0; ky < np[1]; ++ky)
-            for (int kx = 0; kx < np[0]; ++kx) {
+    // a fixed 2x2x2 loop, all loads in flight: a second parent that does not
+    // exist is the first one again with weight 0 (the sums are unchanged)
+#pragma unroll
+    for (int kz = 0; kz < 2; ++kz)
+#pragma unroll
+        for (int ky = 0; ky < 2; ++ky)
+#pragma unroll
+            for (int kx = 0; kx < 2; ++kx) {
                 const int64_t I = node_at(gc, p[0][kx], p[1][ky], p[2][kz]);
-                const float ww = w[0][kx] * w[1][ky] * w[2][kz];
+                const bool in = kx < np[0] && ky < np[1] && kz < np[2];
+                const float ww = in ? w[0][kx] * w[1][ky] * w[2][kz] : 0.0f;
 #pragma unroll
                 for (int j = 0; j < BS; ++j) acc[j] = fmaf(ww, xc[I * BS + j], acc[j]);
             }
