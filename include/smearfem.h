@@ -112,6 +112,14 @@ void sfem_mesh_destroy(sfem_mesh* mesh);
  * takes every node pair's elements from the grid instead of sorting keys;
  * the result is bitwise the same.  sfem_meshgrid meshes are declared already. */
 int  sfem_mesh_set_grid(sfem_ctx* ctx, sfem_mesh* mesh, int ndim, const int64_t* ne);
+/* The same declaration without the element counts: they are read from element
+ * 1's corners (nodes 0, 1, nx+1, nx+2, ... of the meshgrid numbering), then
+ * the mesh is checked as by sfem_mesh_set_grid.  *ndim = 0 (and SFEM_OK) when
+ * the mesh is not such a grid; the mesh is then left as it was.  ndim / ne
+ * (3 values, 0 beyond ndim) may be NULL.  No counterpart in the reference:
+ * sfem_assemble calls it, so assemble_system's own signature (src/fem.jl:135)
+ * reaches the grid-pattern assembly and the matrix-free multigrid levels. */
+int  sfem_mesh_detect_grid(sfem_ctx* ctx, sfem_mesh* mesh, int* ndim, int64_t* ne);
 
 /* flags for sfem_assemble_mesh */
 #define SFEM_ASM_DEFAULT       0
@@ -123,7 +131,8 @@ int  sfem_mesh_set_grid(sfem_ctx* ctx, sfem_mesh* mesh, int ndim, const int64_t*
  * Julia's sparse(I,J,V) called at src/fem.jl:253. */
 int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* mesh, int flags, sfem_csr** out);
 
-/* One-shot drop-in for assemble_system: sfem_mesh_create + sfem_assemble_mesh. */
+/* One-shot drop-in for assemble_system: sfem_mesh_create +
+ * sfem_mesh_detect_grid + sfem_assemble_mesh (the same K either way). */
 int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
                   const double* node_list, int64_t n_nodes,
                   const int64_t* ien, int64_t ien_rows, int nen, int64_t n_elem,
@@ -134,6 +143,10 @@ int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
 /* m, n, stored entries, and the block size the device copy is kept in
  * (n_dof when ID is node-blocked, else 1).  Any pointer may be NULL. */
 int  sfem_csr_info(const sfem_csr* K, int64_t* m, int64_t* n, int64_t* nnz, int* block);
+/* The grid K was assembled on (sfem_mesh_set_grid / sfem_mesh_detect_grid):
+ * *ndim = 0 when none; ne = 3 element counts (0 beyond ndim).  No counterpart
+ * in the reference (its K carries no mesh). */
+int  sfem_csr_grid(const sfem_csr* K, int* ndim, int64_t* ne);
 
 /* Fill caller-allocated host buffers with the 1-based CSC arrays of
  * SparseMatrixCSC{Float64,Int64}: colptr (n+1), rowval (nnz), nzval (nnz). */
@@ -161,7 +174,10 @@ void sfem_bc_destroy(sfem_bc* bc);
  * formats are built here: the FP64 CSR row pointers (its columns and values
  * on the first solve that uses the CSR format), and, when K is node-blocked,
  * the node-block formats (a node with some dofs fixed keeps them as a zero
- * row / column with a unit diagonal, so they stay exactly 0 in the solve). */
+ * row / column with a unit diagonal, so they stay exactly 0 in the solve).
+ * The system holds a reference on K (its solves and reactions read K), so the
+ * caller may call sfem_csr_destroy(K) at any time; K's memory is released
+ * when the last system eliminated from it is destroyed. */
 int  sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system** out);
 int  sfem_system_info(const sfem_system* sys, int64_t* n_free, int64_t* nnz_ff, int* block);
 /* Device time of the elimination that built the system (ms). */
@@ -277,7 +293,8 @@ typedef struct {
 } sfem_mg_info;
 
 void sfem_mg_opts_default(sfem_mg_opts* opts);
-/* Builds the hierarchy (opts NULL: defaults). */
+/* Builds the hierarchy (opts NULL: defaults).  ndim = 0 with dims = NULL: the
+ * grid K was assembled on (SFEM_ERR_ARGUMENT if it was not). */
 int  sfem_system_set_grid(sfem_ctx* ctx, sfem_system* sys, int ndim, const int64_t* dims,
                           const sfem_mg_opts* opts);
 int  sfem_system_mg_info(const sfem_system* sys, sfem_mg_info* info);

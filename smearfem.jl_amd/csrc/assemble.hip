@@ -418,6 +418,54 @@ void mesh_set_grid(sfem_ctx* ctx, sfem_mesh* m, int ndim, const int64_t* ne) {
     for (int c = 0; c < 8; ++c) m->grid_corner[c] = g.corner[c];
 }
 
+// The element counts of the meshgrid numbering that element 1's corners
+// imply: they are the nodes {0, 1, n0, n0+1} (+ n0 n1 + the same in 3D), n_d
+// nodes along direction d.  Then the full device check of mesh_set_grid.
+// Returns false, and leaves the mesh as it was, when the mesh is not that
+// grid.
+bool mesh_detect_grid(sfem_ctx* ctx, sfem_mesh* m, int64_t* ne_out) {
+    const int nd = m->ndim;
+    if (m->grid_nd) {
+        for (int d = 0; d < 3; ++d) ne_out[d] = m->grid_ne[d];
+        return true;
+    }
+    if (m->face || m->cls != SFEM_Q1 || (nd != 2 && nd != 3) || m->nen != (1 << nd) || m->ndof != nd ||
+        m->n_elem < 1 || m->pre_code || m->dof.size() == 0 || m->id_rows < m->n_nodes)
+        return false;
+    std::vector<int32_t> c((size_t)m->nen);
+    d2h(ctx, c.data(), m->ien.get(), sizeof(int32_t) * m->nen);
+    std::sort(c.begin(), c.end());
+    if (c[0] != 0 || c[1] != 1) return false;
+    const int64_t n0 = c[2];
+    if (n0 < 2 || c[3] != n0 + 1) return false;
+    int64_t n[3] = {n0, 1, 1};
+    if (nd == 2) {
+        if (m->n_nodes % n0) return false;
+        n[1] = m->n_nodes / n0;
+    } else {
+        const int64_t n01 = c[4];
+        if (n01 % n0 || c[5] != n01 + 1 || c[6] != n01 + n0 || c[7] != n01 + n0 + 1 || m->n_nodes % n01)
+            return false;
+        n[1] = n01 / n0;
+        n[2] = m->n_nodes / n01;
+    }
+    int64_t ne[3], ecount = 1;
+    for (int d = 0; d < nd; ++d) {
+        if (n[d] < 2) return false;
+        ne[d] = n[d] - 1;
+        ecount *= ne[d];
+    }
+    if (ecount != m->n_elem) return false;
+    try {
+        mesh_set_grid(ctx, m, nd, ne);
+    } catch (const Error& e) {
+        if (e.code != SFEM_ERR_ARGUMENT) throw;
+        return false;
+    }
+    for (int d = 0; d < 3; ++d) ne_out[d] = d < nd ? ne[d] : 0;
+    return true;
+}
+
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     K->ctx = ctx;
     Timer tm(ctx->stream);

@@ -405,6 +405,18 @@ int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr** 
     return SFEM_OK;
 }
 
+int sfem_mesh_detect_grid(sfem_ctx* ctx, sfem_mesh* m, int* ndim, int64_t* ne) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!m) { ctx->last_error = "mesh_detect_grid: NULL mesh"; return SFEM_ERR_ARGUMENT; }
+    return guard(ctx, [&] {
+        int64_t d[3] = {0, 0, 0};
+        const bool found = mesh_detect_grid(ctx, m, d);
+        if (ndim) *ndim = found ? m->ndim : 0;
+        if (ne)
+            for (int k = 0; k < 3; ++k) ne[k] = found ? d[k] : 0;
+    });
+}
+
 int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof, const double* node_list,
                   int64_t n_nodes, const int64_t* ien, int64_t ien_rows, int nen, int64_t n_elem,
                   const int64_t* id, int64_t id_rows, const double* young, int64_t n_young, double nu,
@@ -413,7 +425,10 @@ int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof, const 
     int st = sfem_mesh_create(ctx, ndim, function_class, n_dof, node_list, n_nodes, ien, ien_rows, nen, n_elem,
                               id, id_rows, young, n_young, nu, &m);
     if (st) return st;
-    st = sfem_assemble_mesh(ctx, m, SFEM_ASM_DEFAULT, out);
+    // a mesh in the meshgrid numbering is assembled from the grid (bitwise the
+    // same K, no sort) and K keeps the element scales the multigrid reads
+    st = sfem_mesh_detect_grid(ctx, m, nullptr, nullptr);
+    if (!st) st = sfem_assemble_mesh(ctx, m, SFEM_ASM_DEFAULT, out);
     sfem_mesh_destroy(m);
     return st;
 }
@@ -424,6 +439,14 @@ int sfem_csr_info(const sfem_csr* K, int64_t* m, int64_t* n, int64_t* nnz, int* 
     if (n) *n = K->n;
     if (nnz) *nnz = K->nnz;
     if (block) *block = K->bs;
+    return SFEM_OK;
+}
+
+int sfem_csr_grid(const sfem_csr* K, int* ndim, int64_t* ne) {
+    if (!K) return SFEM_ERR_ARGUMENT;
+    if (ndim) *ndim = K->grid_nd;
+    if (ne)
+        for (int d = 0; d < 3; ++d) ne[d] = d < K->grid_nd ? K->grid_ne[d] : 0;
     return SFEM_OK;
 }
 
@@ -448,6 +471,7 @@ int sfem_export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t*
 
 void sfem_csr_destroy(sfem_csr* K) {
     if (!K) return;
+    if (K->refs.fetch_sub(1) != 1) return;  // a system eliminated from K still reads it
     sfem_ctx* ctx = K->ctx;
     delete K;
     ctx_release(ctx);
@@ -524,6 +548,7 @@ int sfem_eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_sys
     if (st) { delete S; return st; }
     S->ctx = ctx;
     ctx_retain(ctx);
+    K->refs.fetch_add(1);  // dropped by sfem_system_destroy
     *out = S;
     return SFEM_OK;
 }
@@ -549,7 +574,9 @@ int sfem_system_eliminate_ms(const sfem_system* S, double* ms) {
 void sfem_system_destroy(sfem_system* S) {
     if (!S) return;
     sfem_ctx* ctx = S->ctx;
+    sfem_csr* K = const_cast<sfem_csr*>(S->K);
     delete S;
+    sfem_csr_destroy(K);  // the reference sfem_eliminate took
     ctx_release(ctx);
 }
 

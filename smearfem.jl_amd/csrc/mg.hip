@@ -817,7 +817,7 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     // along directions that are not coarsened)
     const int f[3] = {f0, f1, f2};
     const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
-    int64_t fi[3][3] = {};  // unused entries: node 0, weight 0
+    int64_t fi[3][3] = {};  // unused entries: node 0 (their terms are not added)
     float fw[3][3] = {};
     int nk[3];
     for (int d = 0; d < 3; ++d) {
@@ -842,8 +842,9 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
 #pragma unroll
     for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
     // a fixed 3x3x3 loop, so all loads can be in flight at once: the missing
-    // support points (and fully fixed fine nodes) read the first one with
-    // weight 0, which leaves the sums exactly as the skipped terms would
+    // support points (and fully fixed fine nodes) load the first row and
+    // their term is then dropped by a select, so the sums are exactly those
+    // of the support points alone, whatever that row holds (Inf / NaN too)
 #pragma unroll
     for (int kz = 0; kz < 3; ++kz)
 #pragma unroll
@@ -855,9 +856,12 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
                 const int64_t mrow = row_of_node ? (int64_t)row_of_node[fn] : fn;
                 const bool use = in && mrow >= 0;
                 const int64_t row = use ? mrow : 0;
-                const float w = use ? fw[0][kx] * fw[1][ky] * fw[2][kz] : 0.0f;
+                const float w = fw[0][kx] * fw[1][ky] * fw[2][kz];
 #pragma unroll
-                for (int j = 0; j < BS; ++j) acc[j] = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
+                for (int j = 0; j < BS; ++j) {
+                    const float t = fmaf(w, rf[row * BS + j] * rscale, acc[j]);
+                    acc[j] = use ? t : acc[j];
+                }
             }
     const int64_t nnc = gc.n0 * gc.n1 * gc.n2;
 #pragma unroll
@@ -922,7 +926,8 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
 #pragma unroll
     for (int j = 0; j < BS; ++j) acc[j] = 0.0f;
     // a fixed 2x2x2 loop, all loads in flight: a second parent that does not
-    // exist is the first one again with weight 0 (the sums are unchanged)
+    // exist loads the first one again and its term is dropped by a select
+    // (the sums are those of the real parents, whatever the loaded value)
 #pragma unroll
     for (int kz = 0; kz < 2; ++kz)
 #pragma unroll
@@ -931,9 +936,12 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
             for (int kx = 0; kx < 2; ++kx) {
                 const int64_t I = node_at(gc, p[0][kx], p[1][ky], p[2][kz]);
                 const bool in = kx < np[0] && ky < np[1] && kz < np[2];
-                const float ww = in ? w[0][kx] * w[1][ky] * w[2][kz] : 0.0f;
+                const float ww = w[0][kx] * w[1][ky] * w[2][kz];
 #pragma unroll
-                for (int j = 0; j < BS; ++j) acc[j] = fmaf(ww, xc[I * BS + j], acc[j]);
+                for (int j = 0; j < BS; ++j) {
+                    const float t = fmaf(ww, xc[I * BS + j], acc[j]);
+                    acc[j] = in ? t : acc[j];
+                }
             }
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
@@ -2194,6 +2202,12 @@ void mg_opts_default(sfem_mg_opts* o) {
 }
 
 void system_set_grid(sfem_ctx* ctx, sfem_system* S, int ndim, const int64_t* dims, const sfem_mg_opts* opts) {
+    if (ndim == 0 && !dims) {  // the grid K was assembled on
+        if (!S->K || !S->K->grid_nd)
+            fail(SFEM_ERR_ARGUMENT, "set_grid: no element counts given and K was not assembled on a grid");
+        ndim = S->K->grid_nd;
+        dims = S->K->grid_ne;
+    }
     if (ndim != 2 && ndim != 3) fail(SFEM_ERR_DIMENSION, "set_grid: ndim must be 2 or 3");
     if (!dims) fail(SFEM_ERR_ARGUMENT, "set_grid: dims is NULL");
     sfem_mg_opts o;

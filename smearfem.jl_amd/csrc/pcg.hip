@@ -30,7 +30,9 @@
 // block combine, per-block partials, and the last block to arrive sums the
 // partials in a fixed order.  The grid is a fixed function of the problem
 // size, so results are bitwise reproducible run to run.
+#include <chrono>
 #include <cmath>
+#include <string>
 #include <vector>
 
 #include "sfem_objects.hpp"
@@ -654,11 +656,56 @@ int resolve_format(const sfem_system* S, int fmt) {
     return fmt;
 }
 
+// Device workspace of one solve when the caller provides it (the solve that
+// allocates nothing): four CG vectors, the FP32 solution increment, the
+// per-block partial sums and the state.
+struct CgWork {
+    void *r, *z, *q, *p, *xs;
+    double* partials;
+    size_t max_partials;  // doubles
+    CgState* st;
+};
+
+// Zeroes n elements of a vector (a kernel, not hipMemsetAsync, so that the
+// solve that allocates nothing issues kernels only).
+template <typename T>
+__global__ __launch_bounds__(kThreads) void zero_kernel(T* __restrict__ v, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads)
+        v[i] = T(0);
+}
+
+__global__ void cg_state_init_kernel(CgState* st) {
+    if (threadIdx.x != 0) return;
+    CgState init{};
+    init.rr = HUGE_VAL;
+    init.rz_prev = HUGE_VAL;
+    init.zcoef = 1.0;
+    init.best_rr = HUGE_VAL;
+    init.best_true = HUGE_VAL;
+    init.need_update = 1;
+    *st = init;
+}
+
+// The iteration cap was reached: the true residual of the final iterate is due.
+__global__ void cg_state_flag_update_kernel(CgState* st) {
+    if (threadIdx.x == 0) st->need_update = 1;
+}
+
+template <typename T>
+void zero_dev(sfem_ctx* ctx, T* v, int64_t n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(zero_kernel<T>, dim3(std::min<unsigned>(grid_for(n, kThreads), 2048)), dim3(kThreads), 0,
+                       ctx->stream, v, n);
+    SFEM_LAUNCH_CHECK();
+}
+
 // TV: operator value type, T: CG vector type.  FOLD (T == float): the
 // solution increment lives in FP32 and is folded into the FP64 solution at
-// every residual replacement.
+// every residual replacement.  ws: the caller's workspace (no device
+// allocation, no events, no host-to-device copies), or NULL.
 template <typename TV, typename T>
-void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* stats, int fmt) {
+void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* stats, int fmt,
+             const CgWork* ws = nullptr) {
     constexpr bool FOLD = std::is_same<T, float>::value;
     const bool padded = fmt == SFEM_FMT_BSR && S->partial;
     const int64_t nf = padded ? S->nfb * S->bs : S->nf;  // length of the solve's vectors
@@ -675,48 +722,63 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     ops.vec = plan(nf, kThreads, std::max(8, ctx->n_cu * 4));
     const unsigned maxgrid = std::max({ops.csr.grid, ops.sell.grid, ops.vec.grid, ops.node.grid});
 
-    DevBuf<T> r(ctx, nf), z(ctx, nf), q(ctx, nf), pv(ctx, nf);
-    DevBuf<T> xsbuf;
-    DevBuf<double> partials(ctx, 4 * (size_t)maxgrid);
-    DevBuf<CgState> dst(ctx, 1);
+    DevBuf<T> rb, zb, qb, pb, xsb;
+    DevBuf<double> partb;
+    DevBuf<CgState> dst;
+    T *r, *z, *q, *p, *xsv = nullptr;
+    double* partials;
+    CgState* st;
+    if (ws) {
+        if (4 * (size_t)maxgrid > ws->max_partials) fail(SFEM_ERR_UNSUPPORTED, "solve: workspace too small");
+        r = (T*)ws->r, z = (T*)ws->z, q = (T*)ws->q, p = (T*)ws->p;
+        if (FOLD) xsv = (T*)ws->xs;
+        partials = ws->partials;
+        st = ws->st;
+    } else {
+        rb.reset(ctx, nf), zb.reset(ctx, nf), qb.reset(ctx, nf), pb.reset(ctx, nf);
+        r = rb.get(), z = zb.get(), q = qb.get(), p = pb.get();
+        if (FOLD) {
+            xsb.reset(ctx, nf);
+            xsv = xsb.get();
+        }
+        partb.reset(ctx, 4 * (size_t)maxgrid);
+        partials = partb.get();
+        dst.reset(ctx, 1);
+        st = dst.get();
+    }
     const T* dinv = padded ? (FOLD ? (const T*)S->pdinv32.get() : (const T*)S->pdinv64.get())
                            : (FOLD ? (const T*)S->dinv32.get() : (const T*)S->dinv64.get());
     const T* binv = FOLD ? (const T*)S->binv32.get() : (const T*)S->binv64.get();
     T* x = (T*)x64;  // FP64 paths update the FP64 solution directly
     if (FOLD) {
-        xsbuf.reset(ctx, nf);
-        memset_d(ctx, xsbuf.get(), 0, sizeof(T) * nf);
-        x = xsbuf.get();
+        zero_dev(ctx, xsv, nf);
+        x = xsv;
     }
-    const T* xs = FOLD ? xsbuf.get() : nullptr;
-    if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
+    const T* xs = FOLD ? xsv : nullptr;
+    if (o->x0_zero) zero_dev(ctx, S->x.get(), S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x64);  // start from the current solution
-    memset_d(ctx, pv.get(), 0, sizeof(T) * nf);
-    CgState init{};
-    init.rr = HUGE_VAL;
-    init.rz_prev = HUGE_VAL;
-    init.zcoef = 1.0;
-    init.best_rr = HUGE_VAL;
-    init.best_true = HUGE_VAL;
-    init.need_update = 1;
-    h2d(ctx, dst.get(), &init, sizeof(init));
-    CgState* st = dst.get();
+    zero_dev(ctx, p, nf);
+    hipLaunchKernelGGL(cg_state_init_kernel, dim3(1), dim3(64), 0, ctx->stream, st);
+    SFEM_LAUNCH_CHECK();
 
-    hipEvent_t t0, t1;
-    SFEM_HIP(hipEventCreate(&t0));
-    SFEM_HIP(hipEventCreate(&t1));
+    const bool timed_run = ws == nullptr;  // events are runtime objects: none in the no-allocation solve
+    hipEvent_t t0 = nullptr, t1 = nullptr;
+    if (timed_run) {
+        SFEM_HIP(hipEventCreate(&t0));
+        SFEM_HIP(hipEventCreate(&t1));
+    }
+    const auto w0 = std::chrono::steady_clock::now();
     const int check = std::max(1, o->check_every);
     std::vector<hipEvent_t> ev;
-    if (o->time_kernels) {
+    if (o->time_kernels && timed_run) {
         ev.resize(2 * (size_t)check);
         for (auto& e : ev) SFEM_HIP(hipEventCreate(&e));
     }
-    SFEM_HIP(hipEventRecord(t0, ctx->stream));
+    if (timed_run) SFEM_HIP(hipEventRecord(t0, ctx->stream));
     hipLaunchKernelGGL(norm2_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, ops.rhs, nf,
-                       ops.vec.per_block, st, partials.get(), o->rtol, o->reliable_delta);
+                       ops.vec.per_block, st, partials, o->rtol, o->reliable_delta);
     SFEM_LAUNCH_CHECK();
-    T* p = pv.get();
-    launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
+    launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r, z, st, partials);
 
     CgState host{};
     int64_t enq = 0, counted = 0, timed = 0;
@@ -725,30 +787,27 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
     while (!host.done && enq < o->max_iter) {
         const int64_t chunk = std::min<int64_t>(check, o->max_iter - enq);
         for (int64_t k = 0; k < chunk; ++k) {
-            hipLaunchKernelGGL(p_update_kernel<T>, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, p, z.get(), nf,
+            hipLaunchKernelGGL(p_update_kernel<T>, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, p, z, nf,
                                ops.vec.per_block, st);
             SFEM_LAUNCH_CHECK();
-            if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k], ctx->stream));
-            launch_spmv_cg<TV, T>(ops, p, q.get(), st, partials.get());
-            if (o->time_kernels) SFEM_HIP(hipEventRecord(ev[2 * k + 1], ctx->stream));
+            if (!ev.empty()) SFEM_HIP(hipEventRecord(ev[2 * k], ctx->stream));
+            launch_spmv_cg<TV, T>(ops, p, q, st, partials);
+            if (!ev.empty()) SFEM_HIP(hipEventRecord(ev[2 * k + 1], ctx->stream));
             if (!block_pc)
                 hipLaunchKernelGGL((cg_update_kernel<T, FOLD>), dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream,
-                                   x, x64, r.get(), z.get(), p, q.get(), dinv, nf, ops.vec.per_block, st,
-                                   partials.get());
+                                   x, x64, r, z, p, q, dinv, nf, ops.vec.per_block, st, partials);
             else if (S->bs == 3)
                 hipLaunchKernelGGL((cg_update_block_kernel<3, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
-                                   ctx->stream, x, x64, r.get(), z.get(), p, q.get(), binv, S->nfb,
-                                   ops.node.per_block, st, partials.get());
+                                   ctx->stream, x, x64, r, z, p, q, binv, S->nfb, ops.node.per_block, st, partials);
             else
                 hipLaunchKernelGGL((cg_update_block_kernel<2, T, FOLD>), dim3(ops.node.grid), dim3(kThreads), 0,
-                                   ctx->stream, x, x64, r.get(), z.get(), p, q.get(), binv, S->nfb,
-                                   ops.node.per_block, st, partials.get());
+                                   ctx->stream, x, x64, r, z, p, q, binv, S->nfb, ops.node.per_block, st, partials);
             SFEM_LAUNCH_CHECK();
-            launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
+            launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r, z, st, partials);
         }
         enq += chunk;
         d2h(ctx, &host, st, sizeof(host));
-        if (o->time_kernels) {
+        if (!ev.empty()) {
             // launches after the stop exit at once; time only the real ones
             const int64_t real = std::min<int64_t>(chunk, std::max<int64_t>(0, host.iters - counted));
             for (int64_t k = 0; k < real; ++k) {
@@ -761,25 +820,28 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
         counted = host.iters;
     }
     if (!host.done) {  // iteration cap: evaluate the true residual of the final iterate
-        CgState h2 = host;
-        h2.need_update = 1;
-        h2d(ctx, st, &h2, sizeof(h2));
-        launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r.get(), z.get(), st, partials.get());
+        hipLaunchKernelGGL(cg_state_flag_update_kernel, dim3(1), dim3(64), 0, ctx->stream, st);
+        SFEM_LAUNCH_CHECK();
+        launch_resid<T, FOLD>(ops, xs, dinv, binv, p, r, z, st, partials);
         d2h(ctx, &host, st, sizeof(host));
     }
     if (FOLD) {  // x = x64 + xs
-        hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x64,
-                           (float*)xsbuf.get(), nf);
+        hipLaunchKernelGGL(fold_kernel, dim3(ops.vec.grid), dim3(kThreads), 0, ctx->stream, x64, (float*)xsv, nf);
         SFEM_LAUNCH_CHECK();
     }
     if (padded) pad_scatter(ctx, S, x64, S->x.get());
-    SFEM_HIP(hipEventRecord(t1, ctx->stream));
-    sync(ctx);
     float total = 0;
-    SFEM_HIP(hipEventElapsedTime(&total, t0, t1));
-    for (auto& e : ev) (void)hipEventDestroy(e);
-    (void)hipEventDestroy(t0);
-    (void)hipEventDestroy(t1);
+    if (timed_run) {
+        SFEM_HIP(hipEventRecord(t1, ctx->stream));
+        sync(ctx);
+        SFEM_HIP(hipEventElapsedTime(&total, t0, t1));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+    } else {
+        sync(ctx);
+        total = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - w0).count();
+    }
 
     stats->iterations = host.iters;
     stats->fp64_updates = host.updates;

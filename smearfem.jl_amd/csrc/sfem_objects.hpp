@@ -42,6 +42,10 @@ struct sfem_mesh {
 // node-blocked (ID[m,l] = ndof*g(m) + l), else 1.
 struct sfem_csr {
     sfem_ctx* ctx = nullptr;
+    // Handle references: the caller's (sfem_csr_destroy drops it) plus one per
+    // system eliminated from this matrix (sfem_system_destroy drops it), so a
+    // caller may destroy K right after sfem_eliminate.
+    mutable std::atomic<int> refs{1};
     int64_t m = 0, n = 0, nnz = 0;
     int bs = 1;
     bool transposed = false;  // stores K' (uploaded from a host CSC)
@@ -113,6 +117,9 @@ void launch_face_mass(sfem_ctx* ctx, const sfem_mesh* m, double* V);
 // ---- assemble.hip ----
 void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K);
 void mesh_set_grid(sfem_ctx* ctx, sfem_mesh* m, int ndim, const int64_t* ne);
+// Declares the grid when the mesh is one (element counts from element 1's
+// corners, then mesh_set_grid's check); false otherwise, mesh unchanged.
+bool mesh_detect_grid(sfem_ctx* ctx, sfem_mesh* m, int64_t* ne);
 void export_csc1(sfem_ctx* ctx, const sfem_csr* K, int64_t* colptr, int64_t* rowval, double* nzval);
 void upload_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, const int64_t* rowval,
                  const double* nzval, sfem_csr* K);
@@ -148,7 +155,7 @@ struct MgDelete {
 
 struct sfem_system {
     sfem_ctx* ctx = nullptr;
-    const sfem_csr* K = nullptr;   // must outlive the system (reactions)
+    const sfem_csr* K = nullptr;   // referenced (sfem_csr::refs) until the system is destroyed
     int64_t n = 0, nf = 0;         // all dofs, free dofs
     int bs = 1;                    // block size of K
     sfem::DevBuf<int32_t> free_of; // n: free index or -1

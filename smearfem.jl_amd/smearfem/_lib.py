@@ -8,6 +8,7 @@ from __future__ import annotations
 
 import ctypes as C
 import os
+import sys
 from pathlib import Path
 
 _LIB_PATH = Path(__file__).resolve().parent / "libsmearfem_hip.so"
@@ -143,11 +144,13 @@ _PROTOS = [
       c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
     ("sfem_mesh_destroy", None, [c_vp]),
     ("sfem_mesh_set_grid", C.c_int, [c_vp, c_vp, C.c_int, c_i64p]),
+    ("sfem_mesh_detect_grid", C.c_int, [c_vp, c_vp, c_intp, c_i64p]),
     ("sfem_assemble_mesh", C.c_int, [c_vp, c_vp, C.c_int, C.POINTER(c_vp)]),
     ("sfem_assemble", C.c_int,
      [c_vp, C.c_int, C.c_int, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, C.c_int, C.c_int64,
       c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
     ("sfem_csr_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_intp]),
+    ("sfem_csr_grid", C.c_int, [c_vp, c_intp, c_i64p]),
     ("sfem_csr_stage_ms", C.c_int, [c_vp, c_dblp]),
     ("sfem_export_csc1", C.c_int, [c_vp, c_vp, c_i64p, c_i64p, c_dblp]),
     ("sfem_csr_destroy", None, [c_vp]),
@@ -211,6 +214,13 @@ _PROTOS = [
 EXPORTED = [p[0] for p in _PROTOS]
 
 _lib = None
+
+
+def finalizing() -> bool:
+    """True while the interpreter shuts down.  Handles still open then are
+    left to process exit: their destroy calls would reach a HIP runtime (and
+    a profiler) that may already be finalised."""
+    return sys.is_finalizing()
 
 
 def lib_path() -> Path:

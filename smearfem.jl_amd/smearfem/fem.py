@@ -62,6 +62,8 @@ class Context:
             self.handle = None
 
     def __del__(self):  # pragma: no cover - interpreter shutdown order
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:
@@ -122,6 +124,15 @@ class StiffnessMatrix:
         self.nnz = nnz.value
         self.block = bs.value
 
+    @property
+    def grid(self):
+        """Elements per direction of the grid K was assembled on
+        (``sfem_csr_grid``), or None."""
+        nd = C.c_int()
+        ne = np.zeros(3, dtype=np.int64)
+        self.ctx.check(self.ctx._lib.sfem_csr_grid(self.handle, C.byref(nd), _ptr(ne, C.c_int64)))
+        return tuple(int(v) for v in ne[: nd.value]) if nd.value else None
+
     def stage_ms(self) -> dict:
         ms = np.zeros(6)
         self.ctx.check(self.ctx._lib.sfem_csr_stage_ms(self.handle, _ptr(ms, C.c_double)))
@@ -159,6 +170,8 @@ class StiffnessMatrix:
             self.handle = None
 
     def __del__(self):  # pragma: no cover
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:
@@ -203,6 +216,16 @@ class Mesh:
         self.ctx.check(self.ctx._lib.sfem_mesh_set_grid(self.ctx.handle, self.handle, int(d.size),
                                                         _ptr(d, C.c_int64)))
 
+    def detect_grid(self):
+        """Declare the mesh a structured grid if it is one, with the element
+        counts read from element 1's corners (``sfem_mesh_detect_grid``).
+        Returns the element counts, or None (the mesh is then unchanged)."""
+        nd = C.c_int()
+        ne = np.zeros(3, dtype=np.int64)
+        self.ctx.check(self.ctx._lib.sfem_mesh_detect_grid(self.ctx.handle, self.handle, C.byref(nd),
+                                                           _ptr(ne, C.c_int64)))
+        return tuple(int(v) for v in ne[: nd.value]) if nd.value else None
+
     def assemble(self, force_scalar: bool = False) -> StiffnessMatrix:
         h = C.c_void_p()
         self.ctx.check(self.ctx._lib.sfem_assemble_mesh(self.ctx.handle, self.handle, 1 if force_scalar else 0,
@@ -215,6 +238,8 @@ class Mesh:
             self.handle = None
 
     def __del__(self):  # pragma: no cover
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:
@@ -223,19 +248,24 @@ class Mesh:
 
 def assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=None, Young=1, nu=0.3, *,
                     ctx: Optional[Context] = None, device: bool = False, force_scalar: bool = False,
-                    n_elem: Optional[int] = None):
+                    n_elem: Optional[int] = None, detect_grid: bool = True):
     """Assemble the global stiffness matrix on the GPU.
 
     Mirrors ``assemble_system`` (src/fem.jl:135): element count ``ne^ndim``
     (``n_elem`` overrides it for non-cubic meshes), ``Young`` scalar or one
     value per element.  Returns ``scipy.sparse.csc_matrix`` (or the device
-    matrix with ``device=True``).
+    matrix with ``device=True``).  As ``sfem_assemble``, a mesh in the
+    meshgrid numbering is assembled from its grid (bitwise the same K; the
+    device matrix then knows its grid, ``K.grid``); ``detect_grid=False``
+    keeps the sort-based pipeline.
     """
     ctx = ctx or default_context()
     if n_elem is None:
         n_elem = int(ne) ** int(ndim) if int(ndim) >= 0 else 0
     mesh = Mesh(ctx, n_elem, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, nu)
     try:
+        if detect_grid and not force_scalar:  # as sfem_assemble
+            mesh.detect_grid()
         K = mesh.assemble(force_scalar=force_scalar)
     finally:
         mesh.close()

@@ -114,6 +114,8 @@ class Server:
             self.handle = None
 
     def __del__(self):  # pragma: no cover
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:

@@ -58,6 +58,8 @@ class BoundaryConditions:
             self.handle = None
 
     def __del__(self):  # pragma: no cover
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:
@@ -122,12 +124,17 @@ class System:
         the next ``solve(x0_zero=False)``."""
         self.ctx.check(self.ctx._lib.sfem_system_set_bc(self.ctx.handle, self.handle, bc.handle))
 
-    def set_grid(self, dims, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True,
+    def set_grid(self, dims=None, degree=2, lo=0.1, hi=1.1, power_iters=8, coarse_nodes=64, fine_fp16=True,
                  coarse_degree=0, coarse_fp16=True, matrix_free=2) -> dict:
         """Declare the nodes a structured grid with ``dims`` elements per
         direction (meshgrid numbering, x fastest) and build the geometric
         multigrid hierarchy (``sfem_system_set_grid``); then
-        ``solve(preconditioner="multigrid")``.  Returns ``mg_info()``."""
+        ``solve(preconditioner="multigrid")``.  ``dims=None``: the grid K was
+        assembled on.  Returns ``mg_info()``."""
+        if dims is None:
+            dims = self.K.grid
+            if dims is None:
+                raise ValueError("set_grid: K was not assembled on a grid; pass dims")
         d = np.asarray(dims, dtype=np.int64).ravel()
         o = _lib.MgOpts()
         self.ctx._lib.sfem_mg_opts_default(C.byref(o))
@@ -213,6 +220,8 @@ class System:
             self.handle = None
 
     def __del__(self):  # pragma: no cover
+        if _lib.finalizing():  # the HIP runtime may be gone: leave it to process exit
+            return
         try:
             self.close()
         except Exception:
@@ -223,10 +232,17 @@ def solve_system(K: StiffnessMatrix, f=None, fixed=None, q_d=None, *, precision=
                  rtol=1e-10, max_iter=200000, reliable_delta=0.1, reactions=False, return_stats=False,
                  preconditioner="jacobi", grid=None):
     """q = q_d + C K_ff^{-1} C'(f - K q_d) on the GPU for a device matrix K.
-    preconditioner="multigrid" needs ``grid``: elements per direction of the
-    structured mesh the nodes come from."""
+    preconditioner="multigrid" needs the structured grid the nodes come from:
+    ``grid`` (elements per direction), else the grid K was assembled on.
+    preconditioner="auto": multigrid when K was assembled on a grid (or
+    ``grid`` is given) and the precision allows it, else point Jacobi."""
+    if grid is None:
+        grid = K.grid
+    if preconditioner == "auto":
+        preconditioner = "multigrid" if grid is not None and precision in ("fp64", "split", "fp32_matrix") \
+            else "jacobi"
     if preconditioner == "multigrid" and grid is None:
-        raise ValueError("multigrid needs grid= (elements per direction)")
+        raise ValueError("multigrid needs grid= (elements per direction) or K assembled on a grid")
     bc = BoundaryConditions(K.ctx, K.shape[0], f, fixed, q_d)
     try:
         S = System(K, bc)

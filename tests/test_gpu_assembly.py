@@ -16,11 +16,16 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-12
 
 
-def gpu_csc(ctx, m, ndim, cls="Q1", ndof=None, young=1.0, nu=0.3, force_scalar=False, coords=None, ID="mesh"):
+def gpu_csc(ctx, m, ndim, cls="Q1", ndof=None, young=1.0, nu=0.3, force_scalar=False, coords=None, ID="mesh",
+            detect_grid=True):
     ndof = ndim if ndof is None else ndof
     idm = (m.ID if ndof > 1 else None) if isinstance(ID, str) else ID
     K = smearfem.assemble_system(0, m.NodeList if coords is None else coords, m.IEN, ndim, cls, ndof, idm, young,
-                                 nu, ctx=ctx, device=True, force_scalar=force_scalar, n_elem=m.n_elem)
+                                 nu, ctx=ctx, device=True, force_scalar=force_scalar, n_elem=m.n_elem,
+                                 detect_grid=detect_grid)
+    # the drop-in assembles meshgrid-numbered vector meshes from their grid
+    expect_grid = detect_grid and not force_scalar and cls == "Q1" and ndof == ndim and isinstance(ID, str)
+    assert (K.grid is not None) == expect_grid, (K.grid, expect_grid)
     arrays = K.csc_arrays()
     shape, bs = K.shape, K.block
     K.close()
@@ -61,15 +66,16 @@ CASES = [
 ]
 
 
+@pytest.mark.parametrize("detect_grid", [True, False], ids=["grid", "sorted"])
 @pytest.mark.parametrize("name,dims,cls,ndof", CASES, ids=[c[0] for c in CASES])
-def test_assembly_parity(ctx, name, dims, cls, ndof):
-    print(f"[case] {name}", flush=True)
+def test_assembly_parity(ctx, name, dims, cls, ndof, detect_grid):
+    print(f"[case] {name} detect_grid={detect_grid}", flush=True)
     m = structured_mesh(dims, lengths=[1.0] * len(dims))
     ndim = len(dims)
     E, nu = 1.0, 0.3
     if name.startswith("cfg1"):  # plane strain through the plane-stress code
         E, nu = 1.0 / (1 - 0.3 ** 2), 0.3 / (1 - 0.3)
-    g = gpu_csc(ctx, m, ndim, cls, ndof, E, nu)
+    g = gpu_csc(ctx, m, ndim, cls, ndof, E, nu, detect_grid=detect_grid)
     assert_parity(g, oracle_csc(m, ndim, cls, ndof, E, nu))
     assert symmetric_bitwise(g[0], *g[2])
 
@@ -102,11 +108,12 @@ def test_distorted_per_element_young_and_block_vs_scalar_paths(ctx):
     rng = np.random.default_rng(9)
     coords = m.NodeList + 0.03 * rng.standard_normal(m.NodeList.shape)
     young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=4)
-    gb = gpu_csc(ctx, m, 3, young=young, coords=coords)
+    gb = gpu_csc(ctx, m, 3, young=young, coords=coords, detect_grid=False)
     gs = gpu_csc(ctx, m, 3, young=young, coords=coords, force_scalar=True)
-    assert gb[1] == 3 and gs[1] == 1
-    for a, b in zip(gb[2], gs[2]):  # both paths sum in the same order
-        assert np.array_equal(a, b)
+    gg = gpu_csc(ctx, m, 3, young=young, coords=coords)  # the grid the mesh is detected as
+    assert gb[1] == 3 and gs[1] == 1 and gg[1] == 3
+    for a, b, c in zip(gb[2], gs[2], gg[2]):  # the three paths sum in the same order
+        assert np.array_equal(a, b) and np.array_equal(a, c)
     assert_parity(gb, oracle_csc(m, 3, young=young, coords=coords))
     assert symmetric_bitwise(gb[0], *gb[2])
 

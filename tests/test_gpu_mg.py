@@ -27,9 +27,12 @@ def relerr(a, b):
     return np.linalg.norm(a - b) / np.linalg.norm(b)
 
 
-def device_K(ctx, m, young, nu=0.3):
+def device_K(ctx, m, young, nu=0.3, detect_grid=False):
+    # the sort-based K by default: the assembled finest level (FP16 / FP32
+    # sliced smoother); the grid-assembled K of the drop-in takes the
+    # matrix-free finest level, which the tests below declare explicitly
     return smearfem.assemble_system(0, m.NodeList, m.IEN, m.ndim, "Q1", m.ndim, m.ID, young, nu, ctx=ctx,
-                                    device=True, n_elem=m.n_elem)
+                                    device=True, n_elem=m.n_elem, detect_grid=detect_grid)
 
 
 def oracle_K(m, young, nu=0.3):
