@@ -237,6 +237,21 @@ void sfem_solve_opts_default(sfem_solve_opts* opts);
 
 int  sfem_system_solve(sfem_ctx* ctx, sfem_system* sys, const sfem_solve_opts* opts,
                        sfem_solve_stats* stats);
+/* The same solve, allocating nothing: no counterpart in the reference.  For
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The CG vectors,
+ * partial sums and state live in one file-scope device array per GPU of
+ * 40 MiB (held until the process ends), so the system's solver vectors may
+ * have at most 2^20 = 1,048,576 entries (free dofs; with partially fixed
+ * nodes, node rows x n_dof), else SFEM_ERR_UNSUPPORTED.  Point or block
+ * Jacobi, any precision; multigrid is refused (its V-cycle allocates).  The
+ * operator format the options read must exist: the first sfem_system_solve
+ * with the same precision and format builds it (else SFEM_ERR_STATE).  No
+ * kernel timing (stats->spmv_ms is 0; solve_ms is host wall time).  Every
+ * call on one GPU uses that GPU's one array: calls that overlap on a GPU
+ * (from different contexts or threads) race and must not be made. */
+int  sfem_system_solve_noalloc(sfem_ctx* ctx, sfem_system* sys, const sfem_solve_opts* opts,
+                               sfem_solve_stats* stats);
 /* q = q_d + C q_f (n values, FP64), and optionally reactions K q - f on the
  * fixed dofs (n values; zero on free dofs). */
 int  sfem_system_solution(sfem_ctx* ctx, const sfem_system* sys, double* q,

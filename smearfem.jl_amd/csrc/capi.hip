@@ -195,6 +195,9 @@ int sfem_ctx_create(int device, sfem_ctx** out) {
         if (bt) ctx->bar_timeout_ticks = (unsigned long long)std::max(0LL, std::atoll(bt)) * 100ull;
         const char* bg = std::getenv("SFEM_BARRIER_GROUPED");
         if (bg) ctx->bar_grouped = bg[0] != '0';
+        // the no-allocation solve's workspace: each GPU holds its own copy of
+        // the array, so its address is taken with this device current
+        ctx->scratch = noalloc_scratch_address();
     });
     if (st) { delete ctx; return st; }
     *out = ctx;
@@ -578,6 +581,13 @@ void sfem_system_destroy(sfem_system* S) {
     delete S;
     sfem_csr_destroy(K);  // the reference sfem_eliminate took
     ctx_release(ctx);
+}
+
+int sfem_system_solve_noalloc(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* opts, sfem_solve_stats* stats) {
+    if (!ctx) return SFEM_ERR_ARGUMENT;
+    if (!S || !opts || !stats) { ctx->last_error = "solve_noalloc: NULL argument"; return SFEM_ERR_ARGUMENT; }
+    if (S->ctx != ctx) { ctx->last_error = "solve_noalloc: system belongs to another context"; return SFEM_ERR_STATE; }
+    return guard(ctx, [&] { system_solve_noalloc(ctx, S, opts, stats); });
 }
 
 void sfem_solve_opts_default(sfem_solve_opts* o) {

@@ -861,6 +861,77 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
 
 }  // namespace
 
+// ---------------------------------------------- the no-allocation solve --
+// Workspace of sfem_system_solve_noalloc: a file-scope device array, one copy
+// per GPU, held until the process ends.  Layout: five CG vectors of
+// kNoAllocMaxDofs doubles (r, z, q, p, FP32 increment), the per-block
+// partial sums, the CG state.  Host code reaches it only through the address
+// hipGetSymbolAddress returns for the current device (sfem_ctx::scratch).
+namespace {
+constexpr size_t kNoAllocVec = sizeof(double) * (size_t)kNoAllocMaxDofs;
+constexpr size_t kNoAllocPartials = 4 * 16384;  // doubles: 4 per block of the largest plan
+constexpr size_t kNoAllocBytes = 5 * kNoAllocVec + sizeof(double) * kNoAllocPartials + 256;
+static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
+}  // namespace
+
+void* noalloc_scratch_address() {
+    void* ptr = nullptr;
+    SFEM_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    return ptr;
+}
+
+size_t noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+void system_solve_noalloc(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st) {
+    *st = sfem_solve_stats{};
+    if (!ctx->scratch) fail(SFEM_ERR_STATE, "solve_noalloc: no workspace address for this device");
+    if (!(o->rtol >= 0.0) || o->max_iter < 0) fail(SFEM_ERR_ARGUMENT, "solve: bad rtol or max_iter");
+    if (o->preconditioner == SFEM_PC_MULTIGRID)
+        fail(SFEM_ERR_UNSUPPORTED, "solve_noalloc: point or block Jacobi only (the multigrid V-cycle allocates)");
+    const int fmt = resolve_format(S, o->format);
+    const int64_t nv = (fmt == SFEM_FMT_BSR && S->partial) ? S->nfb * S->bs : S->nf;
+    if (nv > kNoAllocMaxDofs)
+        fail(SFEM_ERR_UNSUPPORTED, "solve_noalloc: " + std::to_string(nv) + " solver dofs, at most " +
+                                       std::to_string(kNoAllocMaxDofs));
+    if (S->nf == 0) {
+        st->converged = 1;
+        st->stop_reason = 1;
+        return;
+    }
+    // the operator formats are built by the first allocating solve; this one
+    // only reads them
+    const bool f32 = o->precision == SFEM_PREC_MIXED || o->precision == SFEM_PREC_FP32_MATRIX;
+    const bool ready = S->dinv64.size() &&
+                       (fmt == SFEM_FMT_BSR ? S->sell_ready
+                                            : (S->col.size() == (size_t)S->nnz && (!f32 || S->val32.size())));
+    if (!ready)
+        fail(SFEM_ERR_STATE, "solve_noalloc: the operator format these options read is not built yet; one "
+                             "sfem_system_solve with the same precision and format builds it");
+    unsigned char* base = static_cast<unsigned char*>(ctx->scratch);
+    CgWork ws{base, base + kNoAllocVec, base + 2 * kNoAllocVec, base + 3 * kNoAllocVec, base + 4 * kNoAllocVec,
+              reinterpret_cast<double*>(base + 5 * kNoAllocVec), kNoAllocPartials,
+              reinterpret_cast<CgState*>(base + 5 * kNoAllocVec + sizeof(double) * kNoAllocPartials)};
+    sfem_solve_opts oo = *o;
+    oo.time_kernels = 0;
+    switch (o->precision) {
+        case SFEM_PREC_FP64: run_pcg<double, double>(ctx, S, &oo, st, fmt, &ws); break;
+        case SFEM_PREC_MIXED: run_pcg<float, float>(ctx, S, &oo, st, fmt, &ws); break;
+        case SFEM_PREC_FP32_MATRIX: run_pcg<float, double>(ctx, S, &oo, st, fmt, &ws); break;
+        case SFEM_PREC_SPLIT:
+            if (fmt != SFEM_FMT_BSR) fail(SFEM_ERR_UNSUPPORTED, "split precision needs the node-block format");
+            run_pcg<SplitVal, double>(ctx, S, &oo, st, fmt, &ws);
+            break;
+        default: fail(SFEM_ERR_ARGUMENT, "solve: unknown precision");
+    }
+    if (!st->converged) {
+        static const char* why[] = {"", "", "breakdown", "stagnation", "iteration cap reached"};
+        char buf[200];
+        std::snprintf(buf, sizeof buf, "solve: %s after %lld iterations (relative residual %.3e)",
+                      why[st->stop_reason], (long long)st->iterations, st->rel_residual);
+        fail(SFEM_ERR_NOT_CONVERGED, buf);
+    }
+}
+
 void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st) {
     *st = sfem_solve_stats{};
     if (S->nf == 0) {

@@ -84,6 +84,9 @@ struct sfem_ctx {
     unsigned long long bar_timeout_ticks = 10ull * 100000000ull;
     bool bar_grouped = true;
     bool bar_last_grouped = false;  // form the last launch used
+    // Device address of the no-allocation solve's workspace array on this
+    // context's device (hipGetSymbolAddress at creation).
+    void* scratch = nullptr;
 };
 
 namespace sfem {

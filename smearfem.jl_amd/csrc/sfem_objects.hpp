@@ -222,6 +222,11 @@ void launch_rhs_many(sfem_ctx* ctx, const sfem_system* S, int k, int kp, const d
                      double* rhs);
 // ---- pcg.hip ----
 void system_solve(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
+// the solve on the per-device workspace array (no device allocation)
+constexpr int64_t kNoAllocMaxDofs = int64_t(1) << 20;
+void system_solve_noalloc(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve_stats* st);
+void* noalloc_scratch_address();  // for the current device
+size_t noalloc_scratch_bytes();
 void system_spmv_bench(sfem_ctx* ctx, sfem_system* S, int precision, int format, int reps, double* avg_ms,
                        double* bytes);
 void stream_copy_bench(sfem_ctx* ctx, int64_t bytes, int reps, double* avg_ms);

@@ -163,6 +163,7 @@ _PROTOS = [
     ("sfem_system_destroy", None, [c_vp]),
     ("sfem_solve_opts_default", None, [C.POINTER(SolveOpts)]),
     ("sfem_system_solve", C.c_int, [c_vp, c_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
+    ("sfem_system_solve_noalloc", C.c_int, [c_vp, c_vp, C.POINTER(SolveOpts), C.POINTER(SolveStats)]),
     ("sfem_system_solution", C.c_int, [c_vp, c_vp, c_dblp, c_dblp]),
     ("sfem_mg_opts_default", None, [C.POINTER(MgOpts)]),
     ("sfem_system_set_grid", C.c_int, [c_vp, c_vp, C.c_int, c_i64p, C.POINTER(MgOpts)]),
