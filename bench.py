@@ -118,38 +118,44 @@ def cpu_model():
     return {"model": name, "logical_cpus": os.cpu_count()}
 
 
-def cpu_baseline(n_free_full, iters_full, nz_layers=8, k_it=400, k_it_mt=3000):
-    """The CPU port (oracle/sfem_oracle.c) on a slab of the same mesh:
-    nz_layers of the 64 element layers.  Assembly and elimination (single
-    thread, as the reference) scale with the element count, one PCG iteration
-    with nnz; the total for the full problem uses the FP64 Jacobi iteration
-    count of the GPU run (same algorithm).  The PCG iteration is timed on one
-    thread and on all host threads; value is the all-thread figure."""
+def host_memory_gb():
+    try:
+        with open("/proc/meminfo") as fh:
+            for line in fh:
+                if line.startswith("MemTotal"):
+                    return int(line.split()[1]) / 2 ** 20
+    except OSError:
+        pass
+    return None
+
+
+def cpu_baseline(n_free_full, iters_full, k_it=4, k_it_mt=20):
+    """The CPU port (oracle/sfem_oracle.c) on the full 256x64x64 workload:
+    assembly and elimination on one thread (as the reference runs), then
+    FP64 Jacobi-PCG iterations on the full K_ff, timed on one thread (k_it)
+    and on all host threads (k_it_mt, OpenMP).  The whole solve would take
+    the GPU's FP64 Jacobi iteration count (the same algorithm), so the solve
+    time is that count times the measured time per iteration; assembly and
+    elimination are measured whole.  value is the all-thread figure."""
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
-    from smearfem.mesh import clamp_x0, element_young_loguniform, random_face_load, structured_mesh
 
     O.lib()
-    dims = (DIMS[0], DIMS[1], nz_layers)
-    m = structured_mesh(dims, lengths=(LENGTHS[0], LENGTHS[1], LENGTHS[2] * nz_layers / DIMS[2]))
-    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=SEED_E)
-    fixed = clamp_x0(m)
-    f = random_face_load(m, 2, 1, fixed, seed=SEED_F)
+    m, young, fixed, f = make_inputs()
     t0 = time.perf_counter()
     K = O.assemble_system(0, m.NodeList, m.IEN, 3, "Q1", 3, m.ID, young, 0.3, n_elem=m.n_elem)
     t1 = time.perf_counter()
     Kff, rhs = O.eliminate(K, fixed, f)
     t2 = time.perf_counter()
+    del K
     _, it, _ = O.pcg(Kff, rhs, rtol=0.0, max_iter=k_it)
     t3 = time.perf_counter()
     nth = cpu_threads()
     _, it_mt, _ = O.pcg_threads(Kff, rhs, nth, rtol=0.0, max_iter=k_it_mt)
     t4 = time.perf_counter()
-    scale_e = (DIMS[2]) / nz_layers
-    full_nnz = 256794606.0
-    t_iter = (t3 - t2) / max(it, 1) * full_nnz / Kff.nnz
-    t_iter_mt = (t4 - t3) / max(it_mt, 1) * full_nnz / Kff.nnz
-    t_build = (t1 - t0 + t2 - t1) * scale_e
+    t_iter = (t3 - t2) / max(it, 1)
+    t_iter_mt = (t4 - t3) / max(it_mt, 1)
+    t_build = t2 - t0
     total_1 = t_build + t_iter * iters_full
     total = t_build + t_iter_mt * iters_full
     return {
@@ -157,16 +163,16 @@ def cpu_baseline(n_free_full, iters_full, nz_layers=8, k_it=400, k_it_mt=3000):
         "unit": "DOF/s",
         "cores": nth,
         "kind": "port",
-        "sample": (f"oracle (C restatement) on a {dims[0]}x{dims[1]}x{dims[2]} slab: assembly "
-                   f"{t1 - t0:.2f}s + elimination {t2 - t1:.2f}s (1 thread) scaled x{scale_e:.0f} by elements, "
-                   f"{k_it} PCG iterations on 1 thread {t3 - t2:.2f}s and {k_it_mt} on {nth} threads "
-                   f"{t4 - t3:.2f}s, scaled by nnz to {t_iter_mt * 1e3:.1f} ms/iteration x {iters_full} "
-                   f"iterations (FP64 GPU count)"),
+        "sample": (f"oracle (C restatement) on the full 256x64x64 workload: assembly {t1 - t0:.1f}s + "
+                   f"elimination {t2 - t1:.1f}s (1 thread, measured), FP64 Jacobi-PCG {t_iter * 1e3:.0f} ms per "
+                   f"iteration on 1 thread ({it} timed) and {t_iter_mt * 1e3:.1f} ms on {nth} threads "
+                   f"({it_mt} timed), times {iters_full} iterations (the GPU's FP64 Jacobi count)"),
         "single_thread_value": n_free_full / total_1,
-        "single_thread_ms_per_iteration": t_iter * 1e3,
+        "assembly_s": t1 - t0, "elimination_s": t2 - t1,
+        "ms_per_iteration_1_thread": t_iter * 1e3, "ms_per_iteration_all_threads": t_iter_mt * 1e3,
         "extrapolated_seconds": total,
         "sample_seconds": t4 - t0,
-        "host": cpu_model(),
+        "host": dict(cpu_model(), memory_gb=host_memory_gb()),
     }
 
 
@@ -221,8 +227,13 @@ def run_headline(args, ctx, m, young, fixed, f):
     st_ref = {"rel_residual": float("nan"), "iterations": 0}
     for vi, (precision, fmt, pc) in enumerate(variants):
         steps = []
-        nwarm = args.warmup if (vi == 0 or pc == "multigrid") else 0
-        for i in range(nwarm + args.steps):
+        # the multigrid paths (the headline) get the warmup and K timed steps;
+        # the Jacobi-type paths (seconds per step, for the comparison) one
+        # untimed warm step on the first path and one timed step each
+        mg = pc == "multigrid"
+        nwarm = args.warmup if mg else (1 if vi == 0 else 0)
+        nsteps = args.steps if mg else 1
+        for i in range(nwarm + nsteps):
             dt, st, K, S = timed_step(ctx, mesh, bc, precision, fmt, pc, RTOL, time_kernels=True)
             phase = "warmup" if i < nwarm else "timed"
             log(f"{path_name((precision, fmt, pc))} {phase} step {i}: {dt:.3f}s iters={st['iterations']} "
@@ -231,7 +242,7 @@ def run_headline(args, ctx, m, young, fixed, f):
                 + (f" mg_setup={st['mg']['setup_ms']:.1f}ms" if st.get("mg") else ""))
             if i >= nwarm:
                 steps.append((dt, st))
-            if i == nwarm + args.steps - 1:
+            if i == nwarm + nsteps - 1:
                 t0 = time.perf_counter()
                 q = S.solution()
                 download_s = time.perf_counter() - t0
@@ -292,22 +303,22 @@ def run_headline(args, ctx, m, young, fixed, f):
 
 
 def run_batch(ctx, nprob=512, seed=4):
-    """512 uneven Hex8 cantilevers in one persistent launch per solve."""
+    """512 uneven Hex8 cantilevers in one persistent launch per solve.  The
+    problems are generated on the device from specs (the same draws as the
+    host generator), so the build is device work only."""
     import smearfem
-    from smearfem.batch import Batch, uneven_hex8_problems
+    from smearfem.batch import Batch, uneven_hex8_specs
 
-    t0 = time.perf_counter()
-    probs = uneven_hex8_problems(nprob, seed)
-    t_gen = time.perf_counter() - t0
+    specs = uneven_hex8_specs(nprob, seed)
     ctx.synchronize()
     t0 = time.perf_counter()
-    B = Batch(probs, ctx)
+    B = Batch.from_specs(specs, ctx)
     ctx.synchronize()
     t_build = time.perf_counter() - t0
-    n_free = int(sum(int((p.fixed == 0).sum()) for p in probs))
-    log(f"batch: {nprob} problems, {n_free} free dofs, host gen {t_gen:.2f}s, build {t_build * 1e3:.0f}ms "
+    n_free = int(sum(sp.n_free for sp in specs))
+    log(f"batch: {nprob} problems, {n_free} free dofs, build from specs {t_build * 1e3:.0f}ms "
         f"(assembly {B.assemble_ms:.1f}ms, elimination {B.eliminate_ms:.1f}ms)")
-    out = {"problems": nprob, "seed": seed, "n_free": n_free, "build_ms": t_build * 1e3,
+    out = {"problems": nprob, "seed": seed, "n_free": n_free, "build_ms": t_build * 1e3, "generated": "device",
            "assemble_ms": B.assemble_ms, "eliminate_ms": B.eliminate_ms, "forms": {}}
     ref = None
     for precision, pc in (("fp64", "jacobi"), ("fp64", "block_jacobi"), ("mixed", "block_jacobi"),
@@ -343,28 +354,54 @@ def run_batch(ctx, nprob=512, seed=4):
     return out
 
 
-def run_serve(ctx, precision, rates, nreq=48, seed=11, preconditioner="block_jacobi"):
+def run_serve(ctx, precision, rates, nreq=1000, seed=11, preconditioner="multigrid", workers=4,
+              n_check=16):
     """Poisson request streams of Q4/Hex8 cantilevers (DOF log-uniform in
-    [1e3, 1e6]); the same requests at every arrival rate."""
-    from smearfem.serve import Server, random_request, run_poisson
+    [1e3, 1e6], E log-uniform [0.1, 10] per element, N(0,1) loads on the end
+    face), submitted as device specs (no host preparation per request); the
+    same requests at every arrival rate.  Each served result of the first
+    n_check requests under 3e4 dofs is checked against the oracle's direct
+    solve (relative displacement error)."""
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import scipy.sparse.linalg as spla
+
+    import oracle as O
+    from smearfem.serve import Server, run_poisson
+    from smearfem.spec import random_request_spec
 
     rng = np.random.Generator(np.random.PCG64(seed))
-    t0 = time.perf_counter()
-    reqs = [random_request(rng, 1e3, 1e6) for _ in range(nreq)]
-    log(f"serve: {nreq} requests generated in {time.perf_counter() - t0:.1f}s, "
-        f"{sum(r.n_dof for r in reqs)} dofs total")
-    out = {"precision": precision, "preconditioner": preconditioner, "requests": nreq, "dof_range": [1e3, 1e6],
-           "seed": seed, "rates": []}
+    reqs = [random_request_spec(rng, 1e3, 1e6) for _ in range(nreq)]
+    checked = [i for i, r in enumerate(reqs) if r.n_dof <= 30000][:n_check]
+    refs = {}
+    for i in checked:
+        r = O.spec_inputs(reqs[i].as_dict())
+        K = O.assemble_system(0, r["NodeList"], r["IEN"], reqs[i].ndim, "Q1", reqs[i].ndim, r["ID"], r["young"],
+                              reqs[i].nu, n_elem=r["n_elem"])
+        Kff, rhs = O.eliminate(K, r["fixed"], r["f"])
+        refs[i] = O.recombine(r["fixed"], None, spla.spsolve(Kff.tocsc(), rhs))
+    log(f"serve: {nreq} request specs, {sum(r.n_dof for r in reqs)} dofs total, {len(refs)} checked "
+        f"against the oracle")
+    out = {"precision": precision, "preconditioner": preconditioner, "mg_workers": workers, "requests": nreq,
+           "dof_range": [1e3, 1e6], "seed": seed, "submitted": "device specs", "rates": []}
     for lam in rates:
         S = Server(ctx, precision=precision, rtol=RTOL, iters_per_step=64, hbm_budget=64 << 30, max_active=64,
-                   preconditioner=preconditioner)
-        res = run_poisson(S, reqs, rate_per_s=lam, seed=seed + 1)
+                   preconditioner=preconditioner, mg_workers=workers)
+        errs = []
+
+        def check(idx, q, st):
+            if idx in refs:
+                errs.append(float(np.linalg.norm(q - refs[idx]) / np.linalg.norm(refs[idx])))
+
+        res = run_poisson(S, reqs, rate_per_s=lam, seed=seed + 1, check=check)
         S.close()
         s = res.summary()
         s["rate_per_s"] = lam
+        s["checked_requests"] = len(errs)
+        s["max_rel_err_vs_direct"] = max(errs) if errs else None
         out["rates"].append(s)
-        log(f"serve {precision}/{preconditioner} rate {lam}/s: p50 {s['p50_ms']:.1f}ms p99 {s['p99_ms']:.1f}ms, "
-            f"{s['solved_dof_per_s']:.3e} DOF/s, max active {s['max_active']}, failed {s['failed']}")
+        log(f"serve {precision}/{preconditioner} x{workers} rate {lam}/s: p50 {s['p50_ms']:.1f}ms "
+            f"p99 {s['p99_ms']:.1f}ms, {s['solved_dof_per_s']:.3e} DOF/s, {s['requests_per_s']:.0f} req/s, "
+            f"max active {s['max_active']}, failed {s['failed']}, max err {s['max_rel_err_vs_direct']}")
     return out
 
 
@@ -444,7 +481,8 @@ def main():
     ap.add_argument("--forms-only", action="store_true",
                     help="run only the batch, serving and sweep forms (no headline, no JSON line)")
     ap.add_argument("--serve-precision", default="fp64")
-    ap.add_argument("--serve-rates", default="4,16,64")
+    ap.add_argument("--serve-rates", default="100,300,1000")
+    ap.add_argument("--serve-requests", type=int, default=1000)
     ap.add_argument("--only", type=str, default=None, help="comma list of precision/format paths to time")
     ap.add_argument("--no-ref", action="store_true", help="skip the FP64 reference solve (profiling runs)")
     ap.add_argument("--kernel-probe", type=str, default=None,
@@ -493,9 +531,7 @@ def run(args, ctx, info, m, young, fixed, f, copy_gbs):
             print(json.dumps({"batch": run_batch(ctx)}), flush=True)
         if not args.no_serve:
             rates = [float(v) for v in args.serve_rates.split(",") if v]
-            print(json.dumps({"serving": run_serve(ctx, args.serve_precision, rates)}), flush=True)
-            print(json.dumps({"serving_multigrid": run_serve(ctx, "fp64", rates, preconditioner="multigrid")}),
-                  flush=True)
+            print(json.dumps({"serving": run_serve(ctx, "fp64", rates, nreq=args.serve_requests)}), flush=True)
         if not args.no_sweep:
             print(json.dumps({"sweep": run_sweep(ctx)}), flush=True)
         return
@@ -587,10 +623,11 @@ def run(args, ctx, info, m, young, fixed, f, copy_gbs):
         out["batch"] = run_batch(ctx)
     if not args.no_serve:
         rates = [float(v) for v in args.serve_rates.split(",") if v]
-        out["serving"] = run_serve(ctx, args.serve_precision, rates)
-        # the same request streams, each request solved on its own with the
-        # multigrid preconditioner (FP64)
-        out["serving_multigrid"] = run_serve(ctx, "fp64", rates, preconditioner="multigrid")
+        # the multigrid workers (several requests on the device at once)
+        out["serving"] = run_serve(ctx, "fp64", rates, nreq=args.serve_requests)
+        # the persistent node-block-Jacobi table server on a shorter stream
+        out["serving_table"] = run_serve(ctx, args.serve_precision, [rates[0]], nreq=48,
+                                         preconditioner="block_jacobi")
     if not args.no_sweep:
         out["sweep"] = run_sweep(ctx)
     if not args.no_cpu:

@@ -391,6 +391,53 @@ int  sfem_bc_planes(sfem_ctx* ctx, const sfem_mesh* mesh, int64_t n_total, int n
  * different fixed set is SFEM_ERR_ARGUMENT. */
 int  sfem_system_set_bc(sfem_ctx* ctx, sfem_system* sys, const sfem_bc* bc);
 
+/* ---- problems generated on the device from a compact spec -------------
+ * The load and constraint front end of the example (meshgrid,
+ * examples/vector3D.jl:10-130; setboundaryCond, :133-173) moved to the
+ * device, plus consistent loads, which the reference passes as arrays.  A
+ * structured Q4 (ndim 2) / Hex8 (ndim 3) elasticity problem:
+ *   mesh     meshgrid of box with dims elements per direction (the numbering,
+ *            corner order and coordinates of sfem_meshgrid)
+ *   material E per element log-uniform in [young_lo, young_hi] from seed
+ *            (E = young_lo when equal), nu for every element
+ *   clamp    every dof of the nodes on the face x_clamp_axis = box min
+ *            (clamp_side 0) or max (1)
+ *   loads    consistent nodal loads of a uniform traction (force per area)
+ *            on the face load_axis / load_side and of a body force (per
+ *            volume), 2-point Gauss per cell / face, each node summing its
+ *            cells in ascending order; plus N(0, noise^2) on every free dof
+ *            of the loaded face.
+ * Random draws are counter-based on (seed, stream, index):
+ *   z = splitmix64 finaliser (x ^ x>>30, * 0xBF58476D1CE4E5B9, ^ >>27,
+ *       * 0x94D049BB133111EB, ^ >>31) of (seed ^ stream*0xD1B54A32D192ED03
+ *       ^ index*0x9E3779B97F4A7C15) + 0x9E3779B97F4A7C15 (mod 2^64),
+ *   u = ((z >> 12) + 1/2) 2^-52;
+ *   E_e = exp(log lo + u(1, e) (log hi - log lo));
+ *   noise on local dof k = noise sqrt(-2 log u(2, 2k)) cos(2 pi u(2, 2k+1)). */
+typedef struct {
+    int      ndim;
+    int      pad0;
+    int64_t  dims[3];        /* elements per direction (dims[2] unused in 2D)            */
+    double   box[6];         /* x0, x1, y0, y1[, z0, z1]                                 */
+    double   young_lo, young_hi;
+    double   nu;
+    uint64_t seed;
+    int      clamp_axis, clamp_side;
+    int      load_axis, load_side;
+    double   traction[3];    /* on the loaded face, force per unit area                  */
+    double   body[3];        /* force per unit volume                                    */
+    double   noise;          /* sigma of the random nodal loads on the loaded face       */
+} sfem_problem_spec;
+
+/* The problem's mesh (declared its grid) and loads / constraints on the
+ * device; then sfem_assemble_mesh + sfem_eliminate as for any mesh. */
+int  sfem_spec_build(sfem_ctx* ctx, const sfem_problem_spec* spec, sfem_mesh** mesh, sfem_bc** bc);
+/* The same inputs copied to the host (for checking or output): NodeList
+ * (ndim x n_nodes), Young per element, f and the fixed mask (ndim*n_nodes).
+ * Any pointer may be NULL. */
+int  sfem_spec_export(sfem_ctx* ctx, const sfem_problem_spec* spec, double* node_list, double* young, double* f,
+                      uint8_t* fixed);
+
 
 /* ---- batches: many meshes, one persistent solver launch ---------------
  * No counterpart in the reference (each assemble_system / backslash call
@@ -445,6 +492,9 @@ int  sfem_batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, const 
                        const int64_t* node_offsets, const int64_t* ien, int nen,
                        const int64_t* elem_offsets, const double* young, const double* nu,
                        const double* f, const uint8_t* fixed, const double* q_d, sfem_batch** out);
+/* A batch of problems generated on the device from specs (one dimension for
+ * all): no host-side mesh arrays are built or copied. */
+int  sfem_batch_create_specs(sfem_ctx* ctx, int64_t nprob, const sfem_problem_spec* specs, sfem_batch** out);
 /* n_total = ndim*N_total; n_padded = dofs of the padded operator; slices of 64 node rows. */
 int  sfem_batch_info(const sfem_batch* b, int64_t* n_total, int64_t* n_padded, int64_t* slices,
                      double* assemble_ms, double* eliminate_ms);
@@ -472,12 +522,16 @@ void sfem_batch_destroy(sfem_batch* b);
  * memory released).  A request's result does not depend on what else is
  * active: it is bitwise the result it gets alone.
  * With preconditioner = SFEM_PC_MULTIGRID, requests submitted with their
- * grid (sfem_server_submit_grid) are solved run-to-completion instead: each
- * step takes the oldest such queued request, assembles, eliminates, builds
- * its multigrid hierarchy and solves it (FP64 CG, sfem_system_set_grid),
- * besides one persistent launch for the other active requests.  A grid that
- * cannot be coarsened, and requests without a grid, go to the persistent
- * launch with node-block Jacobi. */
+ * grid (sfem_server_submit_grid, sfem_server_submit_spec) are solved
+ * run-to-completion instead, up to mg_workers at once: each step hands the
+ * oldest queued such requests that fit the HBM budget to idle worker
+ * threads, each of which assembles, eliminates, builds the multigrid
+ * hierarchy and solves (FP64 CG, sfem_system_set_grid) on its own context
+ * and stream, so their kernels overlap on the device beside the persistent
+ * launch for the other active requests.  Steps with nothing else to do wait
+ * up to 1 ms for a worker to finish.  A grid that cannot be coarsened, and
+ * requests without a grid, go to the persistent launch with node-block
+ * Jacobi. */
 typedef struct sfem_server sfem_server;
 
 typedef struct {
@@ -491,6 +545,8 @@ typedef struct {
     int     max_active;       /* cap on concurrently active requests                      */
     int     iters_per_step;   /* CG iterations per persistent launch                      */
     int     preconditioner;   /* SFEM_PC_*                                                */
+    int     mg_workers;       /* multigrid servers: requests solved at once, each by a host
+                                 thread with its own stream (4)                            */
 } sfem_server_opts;
 
 #define SFEM_REQ_QUEUED  0
@@ -512,6 +568,10 @@ int  sfem_server_submit_grid(sfem_server* s, int ndim, const int64_t* dims, cons
                              int64_t n_nodes, const int64_t* ien, int64_t n_elem, const double* young,
                              const double* nu, const double* f, const uint8_t* fixed, const double* q_d,
                              int64_t* id);
+/* A request generated on the device from a spec (sfem_problem_spec): no
+ * host arrays are built or copied; the spec is its whole input.  Solved
+ * like a grid request. */
+int  sfem_server_submit_spec(sfem_server* s, const sfem_problem_spec* spec, int64_t* id);
 /* One admit / iterate / retire step.  Outputs (may be NULL): requests that
  * finished in this step, and device ms of the step's solver work. */
 int  sfem_server_step(sfem_server* s, int* n_finished, double* launch_ms);

@@ -394,3 +394,204 @@ def recombine(fixed, q_d, x):
     q = np.where(fixed, 0.0 if q_d is None else q_d, 0.0).astype(np.float64)
     q[~fixed] = x
     return q
+
+
+# ------------------------------------------------------------------ specs --
+# Restatement of the device problem generator (sfem_problem_spec) with numpy:
+# mesh as meshgrid (examples/vector3D.jl:10-130, generalised to nx x ny x nz
+# elements on a box), clamp as setboundaryCond's fixed face (:133-173), and the
+# consistent loads and counter-based random draws the header describes (loads
+# are not in the reference: they are pinned by closed forms in the tests).
+_U64 = np.uint64
+_SM1, _SM2 = _U64(0xBF58476D1CE4E5B9), _U64(0x94D049BB133111EB)
+_SC1, _SC2 = _U64(0xD1B54A32D192ED03), _U64(0x9E3779B97F4A7C15)
+
+
+def spec_u01(seed, stream, index):
+    """u in (0, 1) for (seed, stream, index array): splitmix64 finaliser."""
+    idx = np.asarray(index, dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        z = (_U64(seed) ^ (_U64(stream) * _SC1) ^ (idx * _SC2)) + _SC2
+        z = (z ^ (z >> _U64(30))) * _SM1
+        z = (z ^ (z >> _U64(27))) * _SM2
+        z = z ^ (z >> _U64(31))
+    return ((z >> _U64(12)).astype(np.float64) + 0.5) * 2.0 ** -52
+
+
+def _lin(x0, x1, ne, i):
+    i = np.asarray(i)
+    v = i.astype(np.float64) * ((x1 - x0) / float(ne)) + x0
+    return np.where(i == ne, x1, v)
+
+
+_Q1SX = np.array([-1, 1, 1, -1, -1, 1, 1, -1], dtype=np.float64)
+_Q1SY = np.array([-1, -1, 1, 1, -1, -1, 1, 1], dtype=np.float64)
+_Q1SZ = np.array([-1, -1, -1, -1, 1, 1, 1, 1], dtype=np.float64)
+
+
+def _gauss_points(ndim):
+    s = 2.0 / (2.0 * np.sqrt(3.0))
+    return [(_Q1SX[g] * s, _Q1SY[g] * s, _Q1SZ[g] * s) for g in range(1 << ndim)]
+
+
+def _cell_integral(X, ndim):
+    """int N_a dV (dA in 2D) per corner a of cells with corner coordinates X
+    [cells, nen, ndim], 2-point Gauss, the device's order of roundings."""
+    nen = 1 << ndim
+    out = np.zeros((X.shape[0], nen))
+    for (xi, eta, zeta) in _gauss_points(ndim):
+        J = np.zeros((X.shape[0], ndim, ndim))
+        for b in range(nen):
+            if ndim == 3:
+                fx, fy, fz = 1 + _Q1SX[b] * xi, 1 + _Q1SY[b] * eta, 1 + _Q1SZ[b] * zeta
+                dn = (_Q1SX[b] * (fy * fz) / 8, _Q1SY[b] * (fx * fz) / 8, _Q1SZ[b] * (fx * fy) / 8)
+            else:
+                dn = (_Q1SX[b] * (1 + _Q1SY[b] * eta) / 4, _Q1SY[b] * (1 + _Q1SX[b] * xi) / 4)
+            for i in range(ndim):
+                for j in range(ndim):
+                    J[:, i, j] = J[:, i, j] + X[:, b, i] * dn[j]
+        if ndim == 3:
+            det = (J[:, 0, 0] * (J[:, 1, 1] * J[:, 2, 2] - J[:, 1, 2] * J[:, 2, 1]) +
+                   J[:, 0, 1] * (J[:, 1, 2] * J[:, 2, 0] - J[:, 1, 0] * J[:, 2, 2]) +
+                   J[:, 0, 2] * (J[:, 1, 0] * J[:, 2, 1] - J[:, 1, 1] * J[:, 2, 0]))
+        else:
+            det = J[:, 0, 0] * J[:, 1, 1] - J[:, 0, 1] * J[:, 1, 0]
+        for a in range(nen):
+            if ndim == 3:
+                N = (1 + _Q1SX[a] * xi) * (1 + _Q1SY[a] * eta) * (1 + _Q1SZ[a] * zeta) / 8
+            else:
+                N = (1 + _Q1SX[a] * xi) * (1 + _Q1SY[a] * eta) / 4
+            out[:, a] = out[:, a] + N * np.abs(det)
+    return out
+
+
+def _face_integral(X):
+    """int N_a dA per corner a of bilinear faces X [faces, 4, 3], 2x2 Gauss."""
+    out = np.zeros((X.shape[0], 4))
+    for (xi, eta, _) in _gauss_points(2):
+        t1 = np.zeros((X.shape[0], 3))
+        t2 = np.zeros((X.shape[0], 3))
+        for b in range(4):
+            d0 = _Q1SX[b] * (1 + _Q1SY[b] * eta) / 4
+            d1 = _Q1SY[b] * (1 + _Q1SX[b] * xi) / 4
+            for d in range(3):
+                t1[:, d] = t1[:, d] + X[:, b, d] * d0
+                t2[:, d] = t2[:, d] + X[:, b, d] * d1
+        c0 = t1[:, 1] * t2[:, 2] - t1[:, 2] * t2[:, 1]
+        c1 = t1[:, 2] * t2[:, 0] - t1[:, 0] * t2[:, 2]
+        c2 = t1[:, 0] * t2[:, 1] - t1[:, 1] * t2[:, 0]
+        area = np.sqrt(c0 * c0 + c1 * c1 + c2 * c2)
+        for a in range(4):
+            out[:, a] = out[:, a] + (1 + _Q1SX[a] * xi) * (1 + _Q1SY[a] * eta) / 4 * area
+    return out
+
+
+def _edge_integral(X):
+    """int N_a ds per end a of straight edges X [edges, 2, 2], 2-point Gauss."""
+    s = 2.0 / (2.0 * np.sqrt(3.0))
+    t0 = (X[:, 1, 0] - X[:, 0, 0]) / 2
+    t1 = (X[:, 1, 1] - X[:, 0, 1]) / 2
+    length = np.sqrt(t0 * t0 + t1 * t1)
+    out = np.zeros((X.shape[0], 2))
+    for xi in (-s, s):
+        out[:, 0] = out[:, 0] + (1 - xi) / 2 * length
+        out[:, 1] = out[:, 1] + (1 + xi) / 2 * length
+    return out
+
+
+def spec_inputs(spec):
+    """(NodeList ndim x n, IEN n_elem x nen 1-based, ID n x ndim, young,
+    nu, f, fixed) of a problem spec: a dict with the fields of
+    sfem_problem_spec (ndim, dims, box, young_lo, young_hi, nu, seed,
+    clamp_axis, clamp_side, load_axis, load_side, traction, body, noise)."""
+    nd = int(spec["ndim"])
+    ne = [int(spec["dims"][d]) if d < nd else 1 for d in range(3)]
+    nn = [ne[d] + 1 if d < nd else 1 for d in range(3)]
+    box = list(spec["box"])
+    n_nodes = nn[0] * nn[1] * nn[2]
+    n_elem = ne[0] * ne[1] * ne[2]
+    m = np.arange(n_nodes)
+    idx = np.stack([m % nn[0], (m // nn[0]) % nn[1], m // (nn[0] * nn[1])])
+    coord = [_lin(box[2 * d], box[2 * d + 1], ne[d], np.arange(nn[d])) for d in range(nd)]
+    NodeList = np.stack([coord[d][idx[d]] for d in range(nd)])
+    # meshgrid connectivity in the reference's corner order
+    e = np.arange(n_elem)
+    ci = np.stack([e % ne[0], (e // ne[0]) % ne[1], e // (ne[0] * ne[1])])
+    nen = 1 << nd
+    IEN = np.zeros((n_elem, nen), dtype=np.int64)
+    for a in range(nen):
+        ox, oy, oz = int(_Q1SX[a] > 0), int(_Q1SY[a] > 0), int(_Q1SZ[a] > 0) if nd == 3 else 0
+        IEN[:, a] = (ci[0] + ox) + nn[0] * ((ci[1] + oy) + nn[1] * (ci[2] + oz)) + 1
+    ID = (nd * m[:, None] + np.arange(nd)[None, :] + 1).astype(np.int64)
+    lo, hi = float(spec["young_lo"]), float(spec["young_hi"])
+    if hi == lo:
+        young = np.full(n_elem, lo)
+    else:
+        young = np.exp(np.log(lo) + spec_u01(spec["seed"], 1, e) * (np.log(hi) - np.log(lo)))
+    nu = np.full(n_elem, float(spec["nu"]))
+    ca, cs = int(spec["clamp_axis"]), int(spec["clamp_side"])
+    la, ls = int(spec["load_axis"]), int(spec["load_side"])
+    clamped = idx[ca] == (nn[ca] - 1 if cs else 0)
+    loaded = idx[la] == (nn[la] - 1 if ls else 0)
+    load = np.zeros((n_nodes, nd))
+    body = np.array(spec["body"][:nd], dtype=np.float64)
+    if np.any(body != 0):
+        Xc = NodeList[:, IEN - 1].transpose(1, 2, 0)  # [cells, nen, nd]
+        w_cell = _cell_integral(Xc, nd)
+        w = np.zeros(n_nodes)
+        for oz in ((-1, 0) if nd == 3 else (0,)):
+            for oy in (-1, 0):
+                for ox in (-1, 0):
+                    c = [idx[0] + ox, idx[1] + oy, idx[2] + oz]
+                    ok = np.ones(n_nodes, dtype=bool)
+                    for d in range(3):
+                        ok &= (c[d] >= 0) & (c[d] < ne[d])
+                    a = ((-ox) ^ (-oy)) + 2 * (-oy) + (4 * (-oz) if nd == 3 else 0)
+                    cell = c[0] + ne[0] * (c[1] + ne[1] * c[2])
+                    w = np.where(ok, w + w_cell[np.where(ok, cell, 0), a], w)
+        load = load + body[None, :] * w[:, None]
+    trac = np.array(spec["traction"][:nd], dtype=np.float64)
+    if np.any(trac != 0):
+        w = np.zeros(n_nodes)
+        if nd == 3:
+            du, dv = (1, 2) if la == 0 else ((0, 2) if la == 1 else (0, 1))
+            for ov in (-1, 0):
+                for ou in (-1, 0):
+                    iu, iv = idx[du] + ou, idx[dv] + ov
+                    ok = loaded & (iu >= 0) & (iu < ne[du]) & (iv >= 0) & (iv < ne[dv])
+                    X = np.zeros((n_nodes, 4, 3))
+                    for b in range(4):
+                        ii = [None, None, None]
+                        ii[la] = idx[la]
+                        ii[du] = iu + int(_Q1SX[b] > 0)
+                        ii[dv] = iv + int(_Q1SY[b] > 0)
+                        for d in range(3):
+                            X[:, b, d] = _lin(box[2 * d], box[2 * d + 1], ne[d], np.clip(ii[d], 0, ne[d]))
+                    a = ((-ou) ^ (-ov)) + 2 * (-ov)
+                    w = np.where(ok, w + _face_integral(X)[:, a], w)
+        else:
+            du = 1 - la
+            for ou in (-1, 0):
+                iu = idx[du] + ou
+                ok = loaded & (iu >= 0) & (iu < ne[du])
+                X = np.zeros((n_nodes, 2, 2))
+                for b in range(2):
+                    ii = [None, None]
+                    ii[la] = idx[la]
+                    ii[du] = iu + b
+                    for d in range(2):
+                        X[:, b, d] = _lin(box[2 * d], box[2 * d + 1], ne[d], np.clip(ii[d], 0, ne[d]))
+                w = np.where(ok, w + _edge_integral(X)[:, -ou], w)
+        load = load + trac[None, :] * w[:, None]
+    f = load.reshape(-1).copy()
+    fixed = np.repeat(clamped, nd).astype(np.uint8)
+    sigma = float(spec["noise"])
+    if sigma != 0.0:
+        k = np.arange(n_nodes * nd, dtype=np.uint64)
+        u1 = spec_u01(spec["seed"], 2, 2 * k)
+        u2 = spec_u01(spec["seed"], 2, 2 * k + 1)
+        noise = sigma * (np.sqrt(-2.0 * np.log(u1)) * np.cos(6.283185307179586 * u2))
+        on = np.repeat(loaded & ~clamped, nd)
+        f = np.where(on, f + noise, f)
+    return {"NodeList": NodeList, "IEN": IEN, "ID": ID, "young": young, "nu": nu, "f": f, "fixed": fixed,
+            "n_elem": n_elem}

@@ -935,6 +935,47 @@ void batch_solution(sfem_ctx* ctx, const sfem_batch* B, double* q) {
     d2h(ctx, q, dq.get(), sizeof(double) * B->n);
 }
 
+// The padded operator of problems already on the device as one
+// concatenated mesh (node ids offset per problem) with its loads and
+// constraints.  Takes ownership of mesh and bc (both are destroyed once the
+// operator exists, or on failure).  nfree: free dofs per problem.
+sfem_batch* batch_from_device(sfem_ctx* ctx, int64_t nprob, sfem_mesh* mesh, sfem_bc* bc,
+                              const std::vector<int64_t>& node_offsets, const std::vector<int64_t>& nfree) {
+    std::unique_ptr<sfem_batch> B(new sfem_batch());
+    B->ctx = ctx;
+    B->mesh = mesh;
+    B->bc = bc;
+    B->nprob = nprob;
+    B->ndim = mesh->ndim;
+    B->ndof = mesh->ndof;
+    B->node_off = node_offsets;
+    const int64_t N = mesh->n_nodes;
+    const int n_dof = mesh->ndof;
+    sfem_csr* K = nullptr;
+    int s = sfem_assemble_mesh(ctx, mesh, SFEM_ASM_DEFAULT, &K);
+    if (s) fail(s, ctx->last_error);
+    B->K = K;
+    B->t_assemble_ms = K->t_total;
+    if (K->n != N * n_dof || K->bs != n_dof)
+        fail(SFEM_ERR_ARGUMENT, "batch_create: every node must belong to an element");
+    if (bc->n != K->n) fail(SFEM_ERR_ARGUMENT, "batch_create: loads do not match the mesh");
+    batch_build(ctx, B.get());
+    B->qd.reset(ctx, (size_t)B->n);
+    SFEM_HIP(hipMemcpyAsync(B->qd.get(), bc->qd.get(), sizeof(double) * B->n, hipMemcpyDeviceToDevice, ctx->stream));
+    B->fixed.reset(ctx, (size_t)B->n);
+    SFEM_HIP(hipMemcpyAsync(B->fixed.get(), bc->fixed.get(), (size_t)B->n, hipMemcpyDeviceToDevice, ctx->stream));
+    // free dofs per problem (a partially fixed node keeps only its free dofs)
+    for (int64_t p = 0; p < nprob; ++p) B->hprobs[(size_t)p].n_free = nfree[(size_t)p];
+    sync(ctx);
+    sfem_csr_destroy(B->K);
+    B->K = nullptr;
+    sfem_bc_destroy(B->bc);
+    B->bc = nullptr;
+    sfem_mesh_destroy(B->mesh);
+    B->mesh = nullptr;
+    return B.release();
+}
+
 sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, const double* node_list,
                          const int64_t* node_offsets, const int64_t* ien, int nen, const int64_t* elem_offsets,
                          const double* young, const double* nu, const double* f, const uint8_t* fixed,
@@ -943,11 +984,6 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
         fail(SFEM_ERR_ARGUMENT, "batch_create: NULL array or empty batch");
     if (n_dof != ndim || (ndim != 2 && ndim != 3) || nen != (1 << ndim))
         fail(SFEM_ERR_UNSUPPORTED, "batch_create: Q4 (2D) or Hex8 (3D) vector problems only");
-    std::unique_ptr<sfem_batch> B(new sfem_batch());
-    B->ctx = ctx;
-    B->nprob = nprob;
-    B->ndim = ndim;
-    B->ndof = n_dof;
     if (node_offsets[0] != 0 || elem_offsets[0] != 0)
         fail(SFEM_ERR_ARGUMENT, "batch_create: offsets must start at 0");
     for (int64_t p = 0; p < nprob; ++p)
@@ -955,7 +991,6 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
             fail(SFEM_ERR_ARGUMENT, "batch_create: every problem needs nodes and elements (problem " +
                                         std::to_string(p) + ")");
     const int64_t N = node_offsets[nprob], E = elem_offsets[nprob];
-    B->node_off.assign(node_offsets, node_offsets + nprob + 1);
     std::vector<int64_t> gien((size_t)(E * nen)), gid((size_t)(N * n_dof));
     for (int64_t p = 0; p < nprob; ++p)
         for (int64_t e = elem_offsets[p]; e < elem_offsets[p + 1]; ++e)
@@ -972,39 +1007,24 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
     int s = sfem_mesh_create(ctx, ndim, SFEM_Q1, n_dof, node_list, N, gien.data(), E, nen, E, gid.data(), N, young, E,
                              0.3, &mesh);
     if (s) fail(s, ctx->last_error);
-    B->mesh = mesh;
-    mesh->nu_e.reset(ctx, (size_t)E);
-    h2d(ctx, mesh->nu_e.get(), nu, sizeof(double) * E);
-    sfem_csr* K = nullptr;
-    s = sfem_assemble_mesh(ctx, mesh, SFEM_ASM_DEFAULT, &K);
-    if (s) fail(s, ctx->last_error);
-    B->K = K;
-    B->t_assemble_ms = K->t_total;
-    if (K->n != N * n_dof || K->bs != n_dof)
-        fail(SFEM_ERR_ARGUMENT, "batch_create: every node must belong to an element");
-    sfem_bc* bc = nullptr;
-    s = sfem_bc_create(ctx, K->n, f, fixed, q_d, &bc);
-    if (s) fail(s, ctx->last_error);
-    B->bc = bc;
-    batch_build(ctx, B.get());
-    B->qd.reset(ctx, (size_t)B->n);
-    SFEM_HIP(hipMemcpyAsync(B->qd.get(), bc->qd.get(), sizeof(double) * B->n, hipMemcpyDeviceToDevice, ctx->stream));
-    B->fixed.reset(ctx, (size_t)B->n);
-    SFEM_HIP(hipMemcpyAsync(B->fixed.get(), bc->fixed.get(), (size_t)B->n, hipMemcpyDeviceToDevice, ctx->stream));
-    // free dofs per problem (a partially fixed node keeps only its free dofs)
-    for (int64_t p = 0; p < nprob; ++p) {
-        int64_t nfree = 0;
-        for (int64_t d = node_offsets[p] * n_dof; d < node_offsets[p + 1] * n_dof; ++d) nfree += fixed[d] ? 0 : 1;
-        B->hprobs[(size_t)p].n_free = nfree;
+    try {
+        mesh->nu_e.reset(ctx, (size_t)E);
+        h2d(ctx, mesh->nu_e.get(), nu, sizeof(double) * E);
+    } catch (...) {
+        sfem_mesh_destroy(mesh);
+        throw;
     }
-    sync(ctx);
-    sfem_csr_destroy(B->K);
-    B->K = nullptr;
-    sfem_bc_destroy(B->bc);
-    B->bc = nullptr;
-    sfem_mesh_destroy(B->mesh);
-    B->mesh = nullptr;
-    return B.release();
+    sfem_bc* bc = nullptr;
+    s = sfem_bc_create(ctx, N * n_dof, f, fixed, q_d, &bc);
+    if (s) {
+        sfem_mesh_destroy(mesh);
+        fail(s, ctx->last_error);
+    }
+    std::vector<int64_t> noff(node_offsets, node_offsets + nprob + 1), nfree((size_t)nprob, 0);
+    for (int64_t p = 0; p < nprob; ++p)
+        for (int64_t d = node_offsets[p] * n_dof; d < node_offsets[p + 1] * n_dof; ++d)
+            nfree[(size_t)p] += fixed[d] ? 0 : 1;
+    return batch_from_device(ctx, nprob, mesh, bc, noff, nfree);
 }
 
 int64_t batch_device_bytes(const sfem_batch* B, int precision) {

@@ -112,6 +112,15 @@ sfem_batch* batch_create(sfem_ctx* ctx, int64_t nprob, int ndim, int n_dof, cons
                          const int64_t* node_offsets, const int64_t* ien, int nen, const int64_t* elem_offsets,
                          const double* young, const double* nu, const double* f, const uint8_t* fixed,
                          const double* q_d);
+// problems already on the device (one concatenated mesh, node ids offset per
+// problem, and its bc); takes ownership of mesh and bc
+sfem_batch* batch_from_device(sfem_ctx* ctx, int64_t nprob, sfem_mesh* mesh, sfem_bc* bc,
+                              const std::vector<int64_t>& node_offsets, const std::vector<int64_t>& nfree);
+// structured problems generated from specs (spec.hip): one concatenated
+// device mesh and bc; one problem: the mesh is declared its grid
+void spec_build(sfem_ctx* ctx, int64_t nprob, const sfem_problem_spec* specs, sfem_mesh** mesh, sfem_bc** bc,
+                std::vector<int64_t>* node_off);
+int64_t spec_free_dofs(const sfem_problem_spec& s);
 void batch_items(const sfem_batch* B, int item_slices, std::vector<WorkItem>* items,
                  std::vector<std::pair<int, int>>* ranges);
 void batch_ensure_work(sfem_ctx* ctx, sfem_batch* B, int precision);

@@ -761,7 +761,7 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_kernel(MfOp<float> op, const
                                                            float* __restrict__ d_out, float c1, float c2,
                                                            const float* __restrict__ binv, const int* done) {
     if (*done) return;
-    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t row = logical_block() * kThreads + threadIdx.x;
     if (row >= op.nrows) return;
     float out[ND];
     mf_row<ND, float, float>(op, op.kc, row, v, out);
@@ -1329,7 +1329,7 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_rz_kernel(MfOp<float> op, co
                                                               double* __restrict__ partials) {
     __shared__ double sh[8];
     if (st->done) return;
-    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t row = logical_block() * kThreads + threadIdx.x;
     double s = 0.0;
     if (row < op.nrows) {
         float out[ND];
@@ -1398,7 +1398,7 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     __shared__ double sh[8];
     if (st->done) return;
     double dot = 0.0;
-    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < op.nrows;
+    for (int64_t row = logical_block() * kThreads + threadIdx.x; row < op.nrows;
          row += (int64_t)gridDim.x * kThreads) {
         double out[ND];
         mf_row<ND, double, double>(op, op.kc, row, p, out);
@@ -1478,6 +1478,11 @@ Launch plan_sell(int64_t nslices) {
 }
 
 unsigned rows_grid(int64_t n) { return (unsigned)std::max<int64_t>(1, (n + kThreads - 1) / kThreads); }
+// The same rounded up to a multiple of the 8 XCDs, for kernels that take
+// their rows from logical_block(): each XCD then works a contiguous range of
+// rows (a z-slab of the grid), and the neighbour rows it gathers are mostly
+// in its own L2.  Blocks past the last row return.
+unsigned xcd_rows_grid(int64_t n) { return (rows_grid(n) + 7u) / 8u * 8u; }
 
 Grid grid_of(const MgLevel& l) { return Grid{l.n[0], l.n[1], l.n[2]}; }
 
@@ -1902,12 +1907,12 @@ struct MgRun {
         const bool timed = ev && l == 0 && mode == 0 && !fused && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
         if (fused) {
-            hipLaunchKernelGGL(cheb_mf_rz_kernel<ND>, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H),
+            hipLaunchKernelGGL(cheb_mf_rz_kernel<ND>, dim3(xcd_rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H),
                                v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), rz_r,
                                rz_st, rz_partials);
         } else if (l == 0 && H.mf) {
             auto go = [&](auto k) {
-                hipLaunchKernelGGL(k, dim3(rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H), v, L.r.get(),
+                hipLaunchKernelGGL(k, dim3(xcd_rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H), v, L.r.get(),
                                    L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), done);
             };
             if (mode == 0) go(cheb_mf_kernel<ND, 0>);
@@ -2023,11 +2028,11 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     DevBuf<double> r(ctx, (size_t)n), p(ctx, (size_t)n + BS), q(ctx, (size_t)n);
     // FP64 outer operator matrix-free when the hierarchy allows it
     const bool mf_outer = H.mf == 2 && std::is_same<TV, double>::value;
-    const unsigned mf_grid = std::min(rows_grid(S->nfb), 8192u);
+    const unsigned mf_grid = std::min(xcd_rows_grid(S->nfb), 8192u);
     MfOp<double> mf64{};
     if (mf_outer) mf64 = mf_op<double>(S, H, H.kc64.get(), H.es64.get());
     else ensure_sell(ctx, S);  // the outer operator reads the sliced values
-    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u, rows_grid(S->nfb)}));
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u, xcd_rows_grid(S->nfb)}));
     DevBuf<MgState> dst(ctx, 1);
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x);

@@ -14,19 +14,27 @@
 // precision (per-problem reductions over its own items, state in memory), so
 // its result is bitwise the same alone or in any mix.
 //
-// Multigrid servers solve requests that come with their grid one at a time
-// to completion (mg.hip): the oldest such queued request per step, besides
-// the persistent launch; a grid that cannot be coarsened joins the launch.
+// Multigrid servers solve requests that come with their grid (or as a spec,
+// generated on the device) to completion with the multigrid preconditioner
+// (mg.hip), several at once: mg_workers host threads, each with its own
+// context (stream and allocator), take queued grid requests in arrival
+// order while the HBM budget allows, so the requests' kernels overlap on the
+// device beside the persistent launch.  A worker's arithmetic depends only on
+// its request, so results are bitwise the same alone or in any mix.  A grid
+// that cannot be coarsened joins the persistent launch.
 //
 // No counterpart in the reference (one blocking solve per call there,
 // examples/vector3D.jl:314-322).
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <condition_variable>
 #include <deque>
 #include <map>
 #include <memory>
+#include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "batch.hpp"
@@ -38,6 +46,8 @@ struct Request {
     int state = SFEM_REQ_QUEUED;
     int ndim = 0;
     bool grid = false;      // nodes form the structured grid of dims
+    bool spec = false;      // generated on the device from sp (no host arrays)
+    sfem_problem_spec sp{};
     int64_t dims[3] = {0, 0, 0};
     int64_t n_nodes = 0, n_elem = 0;
     std::vector<double> nodes, young, nu, f, qd;
@@ -52,11 +62,27 @@ struct Request {
     std::string error;
 };
 
+// A host thread solving multigrid requests on its own context.
+struct Worker {
+    std::thread th;
+    sfem_ctx* ctx = nullptr;
+    int64_t rid = -1;      // assigned request, -1 when idle
+    int64_t reserved = 0;  // HBM budget bytes held for it
+    bool finished = false; // solved, not yet harvested by a step
+    bool coarsens = true;  // false: the grid has no multigrid hierarchy
+    double ms = 0;         // wall time of the solve
+};
+
 }  // namespace
 
 struct sfem_server {
     sfem_ctx* ctx = nullptr;
     sfem_server_opts o{};
+    std::vector<std::unique_ptr<Worker>> workers;
+    mutable std::mutex mu;             // request map and states, worker slots
+    std::condition_variable cv_work;   // a worker has a request, or the server stops
+    std::condition_variable cv_done;   // a worker finished
+    bool stopping = false;
     std::map<int64_t, Request> reqs;
     std::deque<int64_t> queue;   // requests for the persistent launch
     std::deque<int64_t> gqueue;  // multigrid servers: requests solved one at a time
@@ -73,6 +99,15 @@ struct sfem_server {
     hipEvent_t e0 = nullptr, e1 = nullptr;
     std::chrono::steady_clock::time_point t0;
     ~sfem_server() {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stopping = true;
+        }
+        cv_work.notify_all();
+        for (auto& w : workers) {
+            if (w->th.joinable()) w->th.join();
+            if (w->ctx) sfem_ctx_destroy(w->ctx);
+        }
         for (auto& kv : reqs)
             if (kv.second.B) sfem_batch_destroy(kv.second.B);
         if (e0) (void)hipEventDestroy(e0);
@@ -92,13 +127,18 @@ namespace {
 // assembly buffers (element matrices, sort keys, block CSC).
 void estimate_bytes(Request& r, int precision) {
     const int nen = 1 << r.ndim, bs = r.ndim;
-    std::vector<int32_t> cnt((size_t)r.n_nodes, 0);
-    for (int64_t i = 0; i < r.n_elem * nen; ++i) {
-        const int64_t v = r.ien[(size_t)i];
-        if (v >= 1 && v <= r.n_nodes) ++cnt[(size_t)(v - 1)];
-    }
     int64_t maxdeg = 1;
-    for (int32_t c : cnt) maxdeg = std::max<int64_t>(maxdeg, std::min<int64_t>(r.n_nodes, (int64_t)c * (nen - 1) + 1));
+    if (r.spec) {  // a structured grid: at most 3^ndim neighbours
+        maxdeg = std::min<int64_t>(r.n_nodes, r.ndim == 3 ? 27 : 9);
+    } else {
+        std::vector<int32_t> cnt((size_t)r.n_nodes, 0);
+        for (int64_t i = 0; i < r.n_elem * nen; ++i) {
+            const int64_t v = r.ien[(size_t)i];
+            if (v >= 1 && v <= r.n_nodes) ++cnt[(size_t)(v - 1)];
+        }
+        for (int32_t c : cnt)
+            maxdeg = std::max<int64_t>(maxdeg, std::min<int64_t>(r.n_nodes, (int64_t)c * (nen - 1) + 1));
+    }
     const int64_t rows = (r.n_nodes + 63) / 64 * 64;
     const int64_t slots = rows * maxdeg;
     const int64_t b2 = (int64_t)bs * bs, vb = (int64_t)prec_vec_bytes(precision);
@@ -133,6 +173,16 @@ void admit(sfem_server* S) {
         const int64_t noff[2] = {0, r.n_nodes}, eoff[2] = {0, r.n_elem};
         sfem_batch* B = nullptr;
         const int st = guard(ctx, [&] {
+            if (r.spec) {  // generated on the device
+                sfem_mesh* mesh = nullptr;
+                sfem_bc* bc = nullptr;
+                std::vector<int64_t> off;
+                spec_build(ctx, 1, &r.sp, &mesh, &bc, &off);
+                ctx_retain(ctx);  // released by the destroy calls batch_from_device makes
+                ctx_retain(ctx);
+                B = batch_from_device(ctx, 1, mesh, bc, off, {spec_free_dofs(r.sp)});
+                return;
+            }
             B = batch_create(ctx, 1, r.ndim, r.ndim, r.nodes.data(), noff, r.ien.data(), (int)nen, eoff,
                              r.young.data(), r.nu.data(), r.f.empty() ? nullptr : r.f.data(), r.fixed.data(),
                              r.qd.empty() ? nullptr : r.qd.data());
@@ -199,11 +249,12 @@ void retire(sfem_server* S, int* n_finished) {
     S->active.swap(still);
 }
 
-// One request on its own: assemble, eliminate, multigrid hierarchy, FP64 CG.
-// Returns false (request unchanged, its inputs kept) when its grid cannot be
-// coarsened, so that it can join the persistent launch instead.
-bool solve_grid_request(sfem_server* S, Request& r, double* ms) {
-    sfem_ctx* ctx = S->ctx;
+// One request on its own on ctx (a worker's context): build, assemble,
+// eliminate, multigrid hierarchy, FP64 CG.  Writes the result, stats and
+// error into r and returns the request state (SFEM_REQ_DONE / FAILED), or
+// SFEM_REQ_QUEUED when the grid has no multigrid hierarchy (the request then
+// joins the persistent launch, its inputs kept).  Touches no other request.
+int solve_grid_request(sfem_server* S, sfem_ctx* ctx, Request& r, double* ms) {
     const int nd = r.ndim, nen = 1 << nd;
     const int64_t N = r.n_nodes, E = r.n_elem, n = N * nd;
     sfem_mesh* mesh = nullptr;
@@ -220,24 +271,29 @@ bool solve_grid_request(sfem_server* S, Request& r, double* ms) {
         if (st) fail(st, ctx->last_error);
     };
     bool coarsens = true;
+    int state = SFEM_REQ_FAILED;
     const auto t0 = std::chrono::steady_clock::now();
     try {
-        std::vector<int64_t> gid((size_t)n);
-        for (int64_t m = 0; m < N; ++m)
-            for (int l = 0; l < nd; ++l) gid[(size_t)((int64_t)l * N + m)] = (int64_t)nd * m + l + 1;
-        ok(sfem_mesh_create(ctx, nd, SFEM_Q1, nd, r.nodes.data(), N, r.ien.data(), E, nen, E, gid.data(), N,
-                            r.young.data(), E, 0.3, &mesh));
-        mesh->nu_e.reset(ctx, (size_t)E);
-        h2d(ctx, mesh->nu_e.get(), r.nu.data(), sizeof(double) * E);
-        try {
-            mesh_set_grid(ctx, mesh, nd, r.dims);  // grid-pattern assembly when the mesh is the grid
-        } catch (const Error& e) {
-            if (e.code != SFEM_ERR_ARGUMENT) throw;
+        if (r.spec) {
+            ok(sfem_spec_build(ctx, &r.sp, &mesh, &bc));
+        } else {
+            std::vector<int64_t> gid((size_t)n);
+            for (int64_t m = 0; m < N; ++m)
+                for (int l = 0; l < nd; ++l) gid[(size_t)((int64_t)l * N + m)] = (int64_t)nd * m + l + 1;
+            ok(sfem_mesh_create(ctx, nd, SFEM_Q1, nd, r.nodes.data(), N, r.ien.data(), E, nen, E, gid.data(), N,
+                                r.young.data(), E, 0.3, &mesh));
+            mesh->nu_e.reset(ctx, (size_t)E);
+            h2d(ctx, mesh->nu_e.get(), r.nu.data(), sizeof(double) * E);
+            try {
+                mesh_set_grid(ctx, mesh, nd, r.dims);  // grid-pattern assembly when the mesh is the grid
+            } catch (const Error& e) {
+                if (e.code != SFEM_ERR_ARGUMENT) throw;
+            }
+            ok(sfem_bc_create(ctx, n, r.f.empty() ? nullptr : r.f.data(), r.fixed.data(),
+                              r.qd.empty() ? nullptr : r.qd.data(), &bc));
         }
         ok(sfem_assemble_mesh(ctx, mesh, SFEM_ASM_DEFAULT, &K));
         if (K->n != n || K->bs != nd) fail(SFEM_ERR_ARGUMENT, "server: every node must belong to an element");
-        ok(sfem_bc_create(ctx, n, r.f.empty() ? nullptr : r.f.data(), r.fixed.data(),
-                          r.qd.empty() ? nullptr : r.qd.data(), &bc));
         ok(sfem_eliminate(ctx, K, bc, &sys));
         try {
             system_set_grid(ctx, sys, nd, r.dims, nullptr);
@@ -268,34 +324,80 @@ bool solve_grid_request(sfem_server* S, Request& r, double* ms) {
             r.stats.n_free = sys->nf;
             r.stats.stop_reason = st.stop_reason;
             r.stats.floor_residual = st.floor_residual;
-            r.state = st.converged ? SFEM_REQ_DONE : SFEM_REQ_FAILED;
+            state = st.converged ? SFEM_REQ_DONE : SFEM_REQ_FAILED;
             if (!st.converged) r.error = "request did not converge (multigrid CG)";
         }
     } catch (const Error& e) {
         release();
-        r.state = SFEM_REQ_FAILED;
         r.error = e.msg;
-        ctx->last_error = e.msg;
-        r.t_fin = S->now_ms();
-        return true;
+        return SFEM_REQ_FAILED;
     }
     release();
-    if (!coarsens) return false;
-    *ms += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    r.t_fin = S->now_ms();
-    std::vector<double>().swap(r.nodes);
-    std::vector<int64_t>().swap(r.ien);
-    return true;
+    if (!coarsens) return SFEM_REQ_QUEUED;
+    *ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    return state;
 }
 
-// The oldest grid request that fits the budget beside the active ones.
+// A worker: waits for a request, solves it on its own context, reports back.
+void worker_loop(sfem_server* S, Worker* W) {
+    (void)hipSetDevice(S->ctx->device);
+    std::unique_lock<std::mutex> lk(S->mu);
+    for (;;) {
+        S->cv_work.wait(lk, [&] { return S->stopping || (W->rid >= 0 && !W->finished); });
+        if (S->stopping) return;
+        Request& r = S->reqs.at(W->rid);  // map nodes stay put; no one else touches r meanwhile
+        lk.unlock();
+        double ms = 0;
+        const int state = solve_grid_request(S, W->ctx, r, &ms);
+        lk.lock();
+        W->coarsens = state != SFEM_REQ_QUEUED;
+        if (W->coarsens) {
+            r.state = state;
+            r.t_fin = S->now_ms();
+        }
+        W->ms = ms;
+        W->finished = true;
+        S->cv_done.notify_all();
+    }
+}
+
+// Harvest finished workers, then hand queued grid requests to idle workers
+// in arrival order while the HBM budget allows.  Caller holds S->mu.
 void step_grid(sfem_server* S, int* n_finished, double* ms) {
+    for (auto& w : S->workers) {
+        if (w->rid < 0 || !w->finished) continue;
+        Request& r = S->reqs.at(w->rid);
+        S->bytes_in_use -= w->reserved;
+        if (w->coarsens) {
+            *ms += w->ms;
+            if (r.state == SFEM_REQ_DONE && n_finished) ++*n_finished;
+            if (r.state == SFEM_REQ_DONE) {
+                std::vector<double>().swap(r.nodes);
+                std::vector<int64_t>().swap(r.ien);
+            }
+        } else {  // no multigrid hierarchy: the persistent launch
+            r.state = SFEM_REQ_QUEUED;
+            r.t_adm = 0;
+            S->queue.push_back(r.id);
+        }
+        w->rid = -1;
+        w->reserved = 0;
+        w->finished = false;
+    }
+    bool assigned = false;
     while (!S->gqueue.empty()) {
+        Worker* idle = nullptr;
+        bool busy = false;
+        for (auto& w : S->workers) {
+            if (w->rid < 0) idle = idle ? idle : w.get();
+            else busy = true;
+        }
+        if (!idle) break;
         Request& r = S->reqs.at(S->gqueue.front());
         // the request's build and hierarchy are held only while it is solved
         const int64_t need = r.est_transient + r.est_transient / 2;
         if (S->bytes_in_use + need > S->o.hbm_budget) {
-            if (!S->active.empty()) return;  // wait for memory to free up
+            if (!S->active.empty() || busy) break;  // wait for memory to free up
             r.state = SFEM_REQ_FAILED;
             r.error = "request does not fit the HBM budget";
             r.t_fin = S->now_ms();
@@ -303,15 +405,21 @@ void step_grid(sfem_server* S, int* n_finished, double* ms) {
             continue;
         }
         S->gqueue.pop_front();
+        r.state = SFEM_REQ_ACTIVE;
         r.t_adm = S->now_ms();
-        if (!solve_grid_request(S, r, ms)) {  // no multigrid hierarchy: the persistent launch
-            r.t_adm = 0;
-            S->queue.push_back(r.id);
-            continue;
-        }
-        if (r.state == SFEM_REQ_DONE && n_finished) ++*n_finished;
-        return;
+        S->bytes_in_use += need;
+        idle->rid = r.id;
+        idle->reserved = need;
+        idle->finished = false;
+        assigned = true;
     }
+    if (assigned) S->cv_work.notify_all();
+}
+
+int64_t busy_workers(const sfem_server* S) {
+    int64_t k = 0;
+    for (const auto& w : S->workers) k += w->rid >= 0 ? 1 : 0;
+    return k;
 }
 
 void step(sfem_server* S, int* n_finished, double* ms) {
@@ -319,9 +427,19 @@ void step(sfem_server* S, int* n_finished, double* ms) {
     if (n_finished) *n_finished = 0;
     if (ms) *ms = 0;
     double grid_ms = 0;
-    if (S->o.preconditioner == SFEM_PC_MULTIGRID) step_grid(S, n_finished, &grid_ms);
+    std::unique_lock<std::mutex> lk(S->mu);
+    if (!S->workers.empty()) {
+        step_grid(S, n_finished, &grid_ms);
+        // nothing for the persistent launch while workers solve: wait (briefly)
+        // for one of them, so that a polling caller does not spin
+        if (S->queue.empty() && S->active.empty() && busy_workers(S) > 0 && !(n_finished && *n_finished)) {
+            S->cv_done.wait_for(lk, std::chrono::milliseconds(1));
+            step_grid(S, n_finished, &grid_ms);
+        }
+    }
     if (ms) *ms = grid_ms;
     admit(S);
+    lk.unlock();
     if (S->active.empty()) return;
     std::vector<ProbDesc> descs;
     std::vector<WorkItem> items;
@@ -381,6 +499,7 @@ void sfem_server_opts_default(sfem_server_opts* o) {
     o->max_active = 256;
     o->iters_per_step = 64;
     o->preconditioner = SFEM_PC_JACOBI;
+    o->mg_workers = 4;
 }
 
 int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server** out) {
@@ -408,6 +527,17 @@ int sfem_server_create(sfem_ctx* ctx, const sfem_server_opts* opts, sfem_server*
         SFEM_HIP(hipEventCreate(&S->e0));
         SFEM_HIP(hipEventCreate(&S->e1));
         S->t0 = std::chrono::steady_clock::now();
+        if (S->o.preconditioner == SFEM_PC_MULTIGRID) {
+            if (S->o.mg_workers < 1 || S->o.mg_workers > 64)
+                fail(SFEM_ERR_ARGUMENT, "server_create: mg_workers must lie in [1, 64]");
+            for (int i = 0; i < S->o.mg_workers; ++i) {
+                std::unique_ptr<Worker> w(new Worker());
+                const int cs = sfem_ctx_create(ctx->device, &w->ctx);
+                if (cs) fail(cs, "server_create: worker context");
+                S->workers.push_back(std::move(w));
+            }
+            for (auto& w : S->workers) w->th = std::thread(worker_loop, S.get(), w.get());
+        }
     });
     if (st) return st;
     ctx_retain(ctx);
@@ -451,6 +581,38 @@ int sfem_server_submit_grid(sfem_server* S, int ndim, const int64_t* dims, const
         r.t_sub = S->now_ms();
         const int64_t rid = r.id;
         const bool to_grid = r.grid && S->o.preconditioner == SFEM_PC_MULTIGRID;
+        std::lock_guard<std::mutex> lk(S->mu);
+        S->reqs.emplace(rid, std::move(r));
+        (to_grid ? S->gqueue : S->queue).push_back(rid);
+        if (id) *id = rid;
+    });
+}
+
+int sfem_server_submit_spec(sfem_server* S, const sfem_problem_spec* spec, int64_t* id) {
+    if (!S) return SFEM_ERR_ARGUMENT;
+    return guard(S->ctx, [&] {
+        if (!spec) fail(SFEM_ERR_ARGUMENT, "server_submit_spec: NULL spec");
+        const sfem_problem_spec& sp = *spec;
+        if (sp.ndim != 2 && sp.ndim != 3) fail(SFEM_ERR_UNSUPPORTED, "server_submit: Q4 (2D) or Hex8 (3D) only");
+        Request r;
+        r.id = S->next_id++;
+        r.ndim = sp.ndim;
+        r.grid = true;
+        r.spec = true;
+        r.sp = sp;
+        r.n_nodes = 1;
+        r.n_elem = 1;
+        for (int d = 0; d < sp.ndim; ++d) {
+            if (sp.dims[d] < 1) fail(SFEM_ERR_ARGUMENT, "server_submit: element counts must be >= 1");
+            r.dims[d] = sp.dims[d];
+            r.n_nodes *= sp.dims[d] + 1;
+            r.n_elem *= sp.dims[d];
+        }
+        estimate_bytes(r, S->o.precision);
+        r.t_sub = S->now_ms();
+        const int64_t rid = r.id;
+        const bool to_grid = S->o.preconditioner == SFEM_PC_MULTIGRID;
+        std::lock_guard<std::mutex> lk(S->mu);
         S->reqs.emplace(rid, std::move(r));
         (to_grid ? S->gqueue : S->queue).push_back(rid);
         if (id) *id = rid;
@@ -470,6 +632,7 @@ int sfem_server_step(sfem_server* S, int* n_finished, double* launch_ms) {
 
 int sfem_server_poll(const sfem_server* S, int64_t id, int* state, sfem_batch_problem_stats* st, double* times3) {
     if (!S) return SFEM_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) {
         S->ctx->last_error = "server_poll: unknown request id";
@@ -489,6 +652,7 @@ int sfem_server_poll(const sfem_server* S, int64_t id, int* state, sfem_batch_pr
 
 int sfem_server_take(sfem_server* S, int64_t id, double* q) {
     if (!S) return SFEM_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
     auto it = S->reqs.find(id);
     if (it == S->reqs.end()) {
         S->ctx->last_error = "server_take: unknown request id";
@@ -511,8 +675,9 @@ int sfem_server_take(sfem_server* S, int64_t id, double* q) {
 
 int sfem_server_info(const sfem_server* S, int64_t* queued, int64_t* active, int64_t* bytes) {
     if (!S) return SFEM_ERR_ARGUMENT;
+    std::lock_guard<std::mutex> lk(S->mu);
     if (queued) *queued = (int64_t)(S->queue.size() + S->gqueue.size());
-    if (active) *active = (int64_t)S->active.size();
+    if (active) *active = (int64_t)S->active.size() + busy_workers(S);
     if (bytes) *bytes = S->bytes_in_use;
     return SFEM_OK;
 }

@@ -125,6 +125,28 @@ class ServerOpts(C.Structure):
         ("max_active", C.c_int),
         ("iters_per_step", C.c_int),
         ("preconditioner", C.c_int),
+        ("mg_workers", C.c_int),
+    ]
+
+
+class ProblemSpec(C.Structure):
+    """sfem_problem_spec: a structured elasticity problem generated on the device."""
+    _fields_ = [
+        ("ndim", C.c_int),
+        ("pad0", C.c_int),
+        ("dims", C.c_int64 * 3),
+        ("box", C.c_double * 6),
+        ("young_lo", C.c_double),
+        ("young_hi", C.c_double),
+        ("nu", C.c_double),
+        ("seed", C.c_uint64),
+        ("clamp_axis", C.c_int),
+        ("clamp_side", C.c_int),
+        ("load_axis", C.c_int),
+        ("load_side", C.c_int),
+        ("traction", C.c_double * 3),
+        ("body", C.c_double * 3),
+        ("noise", C.c_double),
     ]
 
 
@@ -205,6 +227,10 @@ _PROTOS = [
      [c_vp, C.c_int, c_dblp, C.c_int64, c_i64p, C.c_int64, c_dblp, c_dblp, c_dblp, c_u8p, c_dblp, c_i64p]),
     ("sfem_server_submit_grid", C.c_int,
      [c_vp, C.c_int, c_i64p, c_dblp, C.c_int64, c_i64p, C.c_int64, c_dblp, c_dblp, c_dblp, c_u8p, c_dblp, c_i64p]),
+    ("sfem_server_submit_spec", C.c_int, [c_vp, C.POINTER(ProblemSpec), c_i64p]),
+    ("sfem_spec_build", C.c_int, [c_vp, C.POINTER(ProblemSpec), C.POINTER(c_vp), C.POINTER(c_vp)]),
+    ("sfem_spec_export", C.c_int, [c_vp, C.POINTER(ProblemSpec), c_dblp, c_dblp, c_dblp, c_u8p]),
+    ("sfem_batch_create_specs", C.c_int, [c_vp, C.c_int64, C.POINTER(ProblemSpec), C.POINTER(c_vp)]),
     ("sfem_server_step", C.c_int, [c_vp, c_intp, c_dblp]),
     ("sfem_server_poll", C.c_int, [c_vp, C.c_int64, c_intp, C.POINTER(BatchProblemStats), c_dblp]),
     ("sfem_server_take", C.c_int, [c_vp, C.c_int64, c_dblp]),

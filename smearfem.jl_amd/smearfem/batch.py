@@ -63,6 +63,22 @@ def uneven_hex8_problems(n: int, seed: int, lo: int = 4, hi: int = 32) -> List[P
     return out
 
 
+def uneven_hex8_specs(n: int, seed: int, lo: int = 4, hi: int = 32):
+    """The problems of ``uneven_hex8_problems`` (the same draws: dims, E and nu
+    per problem) as device specs: the unit end load is the uniform traction
+    -1 / (face area) in z, whose consistent nodal loads are face_load's."""
+    from .spec import cantilever_spec
+
+    rng = np.random.Generator(np.random.PCG64(seed))
+    out = []
+    for _ in range(n):
+        dims = tuple(int(d) for d in rng.integers(lo, hi + 1, size=3))
+        E = 10.0 ** rng.uniform(-1.0, 1.0)
+        nu = rng.uniform(0.2, 0.4)
+        out.append(cantilever_spec(dims, young=(E, E), nu=nu, end_load=(0.0, 0.0, -1.0)))
+    return out
+
+
 class Batch:
     """``sfem_batch``: the problems' padded operator, rhs and solution on the device."""
 
@@ -105,6 +121,33 @@ class Batch:
         self.n, self.n_padded, self.slices = n.value, npad.value, sl.value
         self.assemble_ms, self.eliminate_ms = am.value, em.value
         self.signature = {"ndim": ndim, "nprob": self.nprob, "slices": self.slices}
+
+    @classmethod
+    def from_specs(cls, specs, ctx: Optional[Context] = None) -> "Batch":
+        """A batch generated on the device from ProblemSpecs
+        (``sfem_batch_create_specs``): no host mesh arrays."""
+        if not specs:
+            raise ValueError("empty batch")
+        self = cls.__new__(cls)
+        self.ctx = ctx or default_context()
+        arr = (_lib.ProblemSpec * len(specs))(*[sp.to_c() for sp in specs])
+        h = C.c_void_p()
+        self.ctx.check(self.ctx._lib.sfem_batch_create_specs(self.ctx.handle, len(specs), arr, C.byref(h)))
+        self.handle = h
+        self.nprob = len(specs)
+        self.ndim = specs[0].ndim
+        self.dof_off = np.concatenate([[0], np.cumsum([sp.n_dof for sp in specs])]).astype(np.int64)
+        self._info()
+        return self
+
+    def _info(self) -> None:
+        n, npad, sl = C.c_int64(), C.c_int64(), C.c_int64()
+        am, em = C.c_double(), C.c_double()
+        self.ctx.check(self.ctx._lib.sfem_batch_info(self.handle, C.byref(n), C.byref(npad), C.byref(sl),
+                                                     C.byref(am), C.byref(em)))
+        self.n, self.n_padded, self.slices = n.value, npad.value, sl.value
+        self.assemble_ms, self.eliminate_ms = am.value, em.value
+        self.signature = {"ndim": self.ndim, "nprob": self.nprob, "slices": self.slices}
 
     def solve(self, precision="mixed", rtol=1e-10, max_iter=100000, item_slices=8, blocks_per_cu=0,
               check_every=0, reliable_delta=0.1, raise_on_fail=True, preconditioner="jacobi"):

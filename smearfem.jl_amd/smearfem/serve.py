@@ -24,6 +24,7 @@ from .batch import Problem, make_problem
 from .fem import Context, _ptr, default_context
 from .mesh import clamp_x0, structured_mesh
 from .solve import PRECISIONS, PRECONDITIONERS
+from .spec import ProblemSpec
 
 REQ_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed"}
 
@@ -31,7 +32,7 @@ REQ_STATES = {0: "queued", 1: "active", 2: "done", 3: "failed"}
 class Server:
     def __init__(self, ctx: Optional[Context] = None, precision="mixed", rtol=1e-10, max_iter=100000,
                  reliable_delta=0.1, item_slices=8, blocks_per_cu=0, hbm_budget: Optional[int] = None,
-                 max_active=256, iters_per_step=64, preconditioner="jacobi"):
+                 max_active=256, iters_per_step=64, preconditioner="jacobi", mg_workers=4):
         self.ctx = ctx or default_context()
         lib = self.ctx._lib
         o = _lib.ServerOpts()
@@ -47,12 +48,19 @@ class Server:
         o.max_active = int(max_active)
         o.iters_per_step = int(iters_per_step)
         o.preconditioner = PRECONDITIONERS[preconditioner]
+        o.mg_workers = int(mg_workers)
         h = C.c_void_p()
         self.ctx.check(lib.sfem_server_create(self.ctx.handle, C.byref(o), C.byref(h)))
         self.handle = h
         self.opts = {name: getattr(o, name) for name, _ in _lib.ServerOpts._fields_}
 
-    def submit(self, p: Problem) -> int:
+    def submit(self, p) -> int:
+        """Queue a Problem (host arrays) or a ProblemSpec (generated on the device)."""
+        if isinstance(p, ProblemSpec):
+            rid = C.c_int64()
+            sp = p.to_c()
+            self.ctx.check(self.ctx._lib.sfem_server_submit_spec(self.handle, C.byref(sp), C.byref(rid)))
+            return rid.value
         m = p.mesh
         nodes = np.asfortranarray(m.NodeList, dtype=np.float64)
         ien = np.asfortranarray(m.IEN, dtype=np.int64)
@@ -179,10 +187,13 @@ class StreamResult:
         }
 
 
-def run_poisson(server: Server, requests: Sequence[Problem], rate_per_s: float, seed: int,
+def run_poisson(server: Server, requests: Sequence, rate_per_s: float, seed: int,
                 check=None) -> StreamResult:
-    """Submit ``requests`` at Poisson arrival times (rate per second, wall
-    clock) while stepping the server; latency = finish - arrival."""
+    """Submit ``requests`` (Problems or ProblemSpecs) at Poisson arrival
+    times (rate per second, wall clock) while stepping the server.  Latency
+    = finish - arrival: the wait from the arrival to the submission on the
+    host plus the server's own submit-to-finish time (stamped by whichever
+    thread finished the request), so polling does not add to it."""
     rng = np.random.Generator(np.random.PCG64(seed))
     arrivals = np.cumsum(rng.exponential(1.0 / rate_per_s, len(requests)))
     res = StreamResult()
@@ -193,7 +204,7 @@ def run_poisson(server: Server, requests: Sequence[Problem], rate_per_s: float, 
         now = time.perf_counter() - t0
         while nxt < len(requests) and arrivals[nxt] <= now:
             rid = server.submit(requests[nxt])
-            pending[rid] = (nxt, t0 + arrivals[nxt])
+            pending[rid] = (nxt, time.perf_counter() - t0 - arrivals[nxt])
             nxt += 1
         info = server.info()
         res.max_active = max(res.max_active, info["active"])
@@ -207,8 +218,7 @@ def run_poisson(server: Server, requests: Sequence[Problem], rate_per_s: float, 
         for rid in list(pending):
             p = server.poll(rid)
             if p["state"] in ("done", "failed"):
-                idx, t_arr = pending.pop(rid)
-                t_done = time.perf_counter()
+                idx, sub_delay = pending.pop(rid)
                 # solved: converged to rtol (1), or stopped at the FP64
                 # rounding floor of the system (3) when rtol lies below it
                 if p["state"] == "failed" or p["stop_reason"] not in (1, 3):
@@ -223,7 +233,7 @@ def run_poisson(server: Server, requests: Sequence[Problem], rate_per_s: float, 
                 q = server.take(rid, requests[idx].n_dof)
                 if check is not None:
                     check(idx, q, p)
-                res.latencies_ms.append(1e3 * (t_done - t_arr))
+                res.latencies_ms.append(1e3 * sub_delay + (p["t_finish_ms"] - p["t_submit_ms"]))
                 res.dofs.append(int(p["n_free"]))
                 res.iterations.append(int(p["iterations"]))
     res.wall_s = time.perf_counter() - t0
