@@ -355,7 +355,7 @@ def run_batch(ctx, nprob=512, seed=4):
 
 
 def run_serve(ctx, precision, rates, nreq=1000, seed=11, preconditioner="multigrid", workers=4,
-              n_check=16):
+              n_check=16, vtk_dir=None):
     """Poisson request streams of Q4/Hex8 cantilevers (DOF log-uniform in
     [1e3, 1e6], E log-uniform [0.1, 10] per element, N(0,1) loads on the end
     face), submitted as device specs (no host preparation per request); the
@@ -368,6 +368,7 @@ def run_serve(ctx, precision, rates, nreq=1000, seed=11, preconditioner="multigr
     import oracle as O
     from smearfem.serve import Server, run_poisson
     from smearfem.spec import random_request_spec
+    from smearfem.vtk import write_solution
 
     rng = np.random.Generator(np.random.PCG64(seed))
     reqs = [random_request_spec(rng, 1e3, 1e6) for _ in range(nreq)]
@@ -387,16 +388,22 @@ def run_serve(ctx, precision, rates, nreq=1000, seed=11, preconditioner="multigr
         S = Server(ctx, precision=precision, rtol=RTOL, iters_per_step=64, hbm_budget=64 << 30, max_active=64,
                    preconditioner=preconditioner, mg_workers=workers)
         errs = []
+        written = []
 
         def check(idx, q, st):
             if idx in refs:
                 errs.append(float(np.linalg.norm(q - refs[idx]) / np.linalg.norm(refs[idx])))
+            if vtk_dir and len(written) < 4 and lam == rates[0]:  # served fields as VTK files
+                written.append(write_solution(os.path.join(vtk_dir, f"request_{idx}"), reqs[idx], q))
 
         res = run_poisson(S, reqs, rate_per_s=lam, seed=seed + 1, check=check)
         S.close()
         s = res.summary()
         s["rate_per_s"] = lam
         s["checked_requests"] = len(errs)
+        if written:
+            s["vtk_files"] = [os.path.basename(w) for w in written]
+            s["vtk_bytes"] = int(sum(os.path.getsize(w) for w in written))
         s["max_rel_err_vs_direct"] = max(errs) if errs else None
         out["rates"].append(s)
         log(f"serve {precision}/{preconditioner} x{workers} rate {lam}/s: p50 {s['p50_ms']:.1f}ms "
@@ -624,7 +631,10 @@ def run(args, ctx, info, m, young, fixed, f, copy_gbs):
     if not args.no_serve:
         rates = [float(v) for v in args.serve_rates.split(",") if v]
         # the multigrid workers (several requests on the device at once)
-        out["serving"] = run_serve(ctx, "fp64", rates, nreq=args.serve_requests)
+        import tempfile
+
+        with tempfile.TemporaryDirectory() as vtk_dir:
+            out["serving"] = run_serve(ctx, "fp64", rates, nreq=args.serve_requests, vtk_dir=vtk_dir)
         # the persistent node-block-Jacobi table server on a shorter stream
         out["serving_table"] = run_serve(ctx, args.serve_precision, [rates[0]], nreq=48,
                                          preconditioner="block_jacobi")

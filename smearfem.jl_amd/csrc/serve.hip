@@ -70,6 +70,7 @@ struct Worker {
     int64_t reserved = 0;  // HBM budget bytes held for it
     bool finished = false; // solved, not yet harvested by a step
     bool coarsens = true;  // false: the grid has no multigrid hierarchy
+    int result = SFEM_REQ_ACTIVE;  // the request's state once harvested (DONE / FAILED)
     double ms = 0;         // wall time of the solve
 };
 
@@ -324,8 +325,12 @@ int solve_grid_request(sfem_server* S, sfem_ctx* ctx, Request& r, double* ms) {
             r.stats.n_free = sys->nf;
             r.stats.stop_reason = st.stop_reason;
             r.stats.floor_residual = st.floor_residual;
-            state = st.converged ? SFEM_REQ_DONE : SFEM_REQ_FAILED;
-            if (!st.converged) r.error = "request did not converge (multigrid CG)";
+            // stagnation at the FP64 floor keeps the best attainable answer: done,
+            // as in the persistent launch (stop_reason 3 tells the caller)
+            state = st.converged || st.stop_reason == 3 ? SFEM_REQ_DONE : SFEM_REQ_FAILED;
+            if (state == SFEM_REQ_FAILED)
+                r.error = "request did not converge (multigrid CG, stop reason " + std::to_string(st.stop_reason) +
+                          ", relative residual " + std::to_string(st.rel_residual) + ")";
         }
     } catch (const Error& e) {
         release();
@@ -350,11 +355,11 @@ void worker_loop(sfem_server* S, Worker* W) {
         double ms = 0;
         const int state = solve_grid_request(S, W->ctx, r, &ms);
         lk.lock();
+        // the request stays ACTIVE until a step harvests it (a caller polling
+        // it then finds its slot released as well); its finish time is now
         W->coarsens = state != SFEM_REQ_QUEUED;
-        if (W->coarsens) {
-            r.state = state;
-            r.t_fin = S->now_ms();
-        }
+        W->result = state;
+        if (W->coarsens) r.t_fin = S->now_ms();
         W->ms = ms;
         W->finished = true;
         S->cv_done.notify_all();
@@ -370,6 +375,7 @@ void step_grid(sfem_server* S, int* n_finished, double* ms) {
         S->bytes_in_use -= w->reserved;
         if (w->coarsens) {
             *ms += w->ms;
+            r.state = w->result;
             if (r.state == SFEM_REQ_DONE && n_finished) ++*n_finished;
             if (r.state == SFEM_REQ_DONE) {
                 std::vector<double>().swap(r.nodes);

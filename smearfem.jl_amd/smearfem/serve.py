@@ -172,6 +172,7 @@ class StreamResult:
     at_floor: int = 0
     rel_residuals: List[float] = field(default_factory=list)
     max_active: int = 0
+    errors: List[str] = field(default_factory=list)
 
     def summary(self) -> dict:
         lat = np.array(self.latencies_ms) if self.latencies_ms else np.array([np.nan])
@@ -183,7 +184,9 @@ class StreamResult:
             "solved_dof_per_s": float(np.sum(self.dofs) / self.wall_s) if self.wall_s > 0 else 0.0,
             "requests_per_s": len(self.latencies_ms) / self.wall_s if self.wall_s > 0 else 0.0,
             "steps": self.steps, "launch_ms": self.launch_ms, "max_active": self.max_active,
-            "gpu_busy_frac": self.launch_ms / (1e3 * self.wall_s) if self.wall_s > 0 else 0.0,
+            # device work over wall time (above 1: requests solved concurrently by the workers)
+            "busy_ratio": self.launch_ms / (1e3 * self.wall_s) if self.wall_s > 0 else 0.0,
+            "errors": self.errors[:5],
         }
 
 
@@ -225,8 +228,9 @@ def run_poisson(server: Server, requests: Sequence, rate_per_s: float, seed: int
                     res.failed += 1
                     try:
                         server.take(rid, requests[idx].n_dof)
-                    except Exception:
-                        pass
+                        res.errors.append(f"request {idx}: stop reason {p['stop_reason']}")
+                    except Exception as e:  # noqa: BLE001 - the server's message for the failure
+                        res.errors.append(f"request {idx}: {e}")
                     continue
                 res.at_floor += p["stop_reason"] == 3
                 res.rel_residuals.append(p["rel_residual"])

@@ -127,6 +127,27 @@ def write_scene(fileName, NodeList, IEN, ne, ndim, fields: Sequence, encoding="r
     return pvd
 
 
+def write_solution(fileName, request, q, fieldName="u", encoding="raw") -> str:
+    """``fileName.vtu`` of a solved problem: a ``Problem`` (its mesh) or a
+    ``ProblemSpec`` (the meshgrid of its box and element counts, the mesh the
+    device generated), with the displacement q (n_dof values, node-interleaved,
+    as a server returns it) as the point vector ``fieldName``."""
+    mesh = getattr(request, "mesh", None)
+    if mesh is None:  # a spec: the same numbering and coordinates as the device's meshgrid
+        from .mesh import structured_mesh
+
+        nd = request.ndim
+        box = request.box
+        mesh = structured_mesh(request.dims[:nd], lengths=[box[2 * d + 1] - box[2 * d] for d in range(nd)],
+                               origin=[box[2 * d] for d in range(nd)])
+    n_points = mesh.NodeList.shape[1]
+    path = f"{fileName}.vtu"
+    with open(path, "wb") as fh:
+        fh.write(_grid(mesh.NodeList, mesh.IEN, mesh.IEN.shape[0], mesh.ndim,
+                       [(fieldName, _point_field(q, n_points))], encoding))
+    return path
+
+
 def read_vtu(path) -> dict:
     """Points, connectivity, offsets, types and point fields of a file written
     here (for checks; raw or ASCII)."""

@@ -19,6 +19,7 @@ from smearfem.batch import Batch, uneven_hex8_problems, uneven_hex8_specs
 from smearfem.serve import Server
 from smearfem.solve import System
 from smearfem.spec import ProblemSpec, cantilever_spec, random_request_spec
+from smearfem.vtk import read_vtu, write_solution
 
 pytestmark = pytest.mark.gpu
 ERR = 1e-8
@@ -113,7 +114,7 @@ def test_batch_from_specs_equals_host_batch(ctx):
 
 
 @pytest.mark.parametrize("workers", [1, 4])
-def test_spec_server_alone_equals_mix(ctx, workers):
+def test_spec_server_alone_equals_mix(ctx, workers, tmp_path):
     """Spec requests served by the multigrid workers: each result is bitwise
     the same alone and in a mix of concurrently solved requests, and meets
     the oracle bar."""
@@ -144,6 +145,10 @@ def test_spec_server_alone_equals_mix(ctx, workers):
             assert np.array_equal(q, alone[k]), k
         if sp.n_dof <= 20000:
             assert relerr(q, direct(sp)) <= ERR, k
+        if k < 3:  # the served field as a VTK file, read back
+            g = read_vtu(write_solution(str(tmp_path / f"req{k}"), sp, q))
+            assert np.array_equal(g["fields"]["u"], q.reshape(-1, sp.ndim))
+            assert g["points"].shape[0] == sp.n_nodes
     S.close()
     if workers > 1:
         assert peak >= 2  # requests were solved at the same time

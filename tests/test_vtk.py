@@ -59,3 +59,37 @@ def test_write_vtk_rejects_bad_cells(tmp_path):
         write_vtk(str(tmp_path / "bad"), "q", NL, bad, 2, 3, np.zeros(NL.shape[1]))
     with pytest.raises(ValueError):
         write_vtk(str(tmp_path / "bad"), "q", NL, IEN, 2, 3, np.zeros(NL.shape[1] + 1))
+
+
+def test_layout_matches_golden_files(tmp_path):
+    """The ASCII files byte for byte against tests/golden (spelled out from the
+    VTK XML format by make_vtk_fixture.py, not by this writer): one Hex8
+    cell, 0-based connectivity in the reference's IEN order, offsets, type
+    12, the point vector "u"; the collection's file names and time steps."""
+    from pathlib import Path
+
+    gold = Path(__file__).resolve().parent / "golden"
+    NL = np.array([[0, 1, 0, 1, 0, 1, 0, 1], [0, 0, 1, 1, 0, 0, 1, 1], [0, 0, 0, 0, 1, 1, 1, 1]], dtype=float)
+    IEN = np.array([[1, 2, 4, 3, 5, 6, 8, 7]])
+    u = np.vstack([NL[0], 2 * NL[1], -NL[2]])
+    path = write_vtk(str(tmp_path / "hex1"), "u", NL, IEN, 1, 3, u, encoding="ascii")
+    assert open(path, "rb").read() == (gold / "vtk_hex1.vtu").read_bytes()
+    pvd = write_scene(str(tmp_path / "scene_"), NL, IEN, 1, 3, [u, u], encoding="ascii")
+    assert open(pvd, "rb").read() == (gold / "vtk_scene.pvd").read_bytes()
+    assert open(str(tmp_path / "scene_timestep_1.vtu"), "rb").read() == (gold / "vtk_hex1.vtu").read_bytes()
+
+
+def test_write_solution_of_a_spec(tmp_path):
+    """A served solution (node-interleaved dofs) of a spec request: the file's
+    grid is the spec's meshgrid (the oracle's restatement of the device
+    generator), the field the solution."""
+    from smearfem.spec import cantilever_spec
+    from smearfem.vtk import write_solution
+
+    sp = cantilever_spec((4, 3, 2), lengths=(2.0, 1.0, 0.5))
+    ref = O.spec_inputs(sp.as_dict())
+    q = np.arange(sp.n_dof, dtype=float) / 7.0
+    g = read_vtu(write_solution(str(tmp_path / "req"), sp, q))
+    assert np.allclose(g["points"], ref["NodeList"].T, rtol=1e-15, atol=1e-15)
+    assert np.array_equal(g["connectivity"].reshape(-1, 8), ref["IEN"] - 1)
+    assert np.array_equal(g["fields"]["u"], q.reshape(-1, 3))
