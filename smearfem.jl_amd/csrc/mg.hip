@@ -31,6 +31,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -84,6 +85,11 @@ struct MgHierarchy {
     DevBuf<float> kc32, es32;     // es: the scales with a ghost layer
     DevBuf<double> kc64, es64;
     DevBuf<int32_t> mf_rows;      // row_of_node with a ghost layer
+    // the smoothing steps on the matrix cores (3D): Ke(0) 2^-ek as FP16 hi / lo
+    // fragments of the 32x32x8 instruction, [part][k step][lane][4]
+    bool mfma = false;
+    DevBuf<_Float16> kfrag;
+    float kscale = 1.0f;          // 2^ek
 };
 
 void MgDelete::operator()(MgHierarchy* h) const { delete h; }
@@ -1341,6 +1347,204 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_rz_kernel(MfOp<float> op, co
     rz_finish(st, partials, sh, s);
 }
 
+// ------------------------------- matrix-free finest level on MFMA (3D) --
+// K_ff v as sum over cells of s_e Ke(0) v_e, formed cell by cell on the
+// matrix cores.  A workgroup owns a tile of kTX x kTY x kTZ grid nodes:
+//   A  the values of the tile's nodes and of a one-node halo go to LDS, scaled
+//      by 2^-ex (ex from the tile's largest |v|) and split into FP16 hi + lo
+//      (hi = fp16(y), lo = fp16(y - hi): 22 significant bits together);
+//   B  every cell touching the tile, (kTX+1)(kTY+1)(kTZ+1) of them, gets
+//      Z_e = Ke(0) v_e from v_mfma_f32_32x32x8f16 (32 cells per instruction
+//      column block, Ke(0)'s 24 rows padded to 32, K = 24 in 3 steps), with
+//      Ke(0) split the same way and the FP32 sum hi*hi + hi*lo + lo*hi; the
+//      cell's column is scaled by s_e 2^ex 2^ek into LDS;
+//   C  each tile node sums its 8 cells' rows in a fixed order, then the
+//      Chebyshev epilogue of the VALU kernel (cheb_mf_kernel).
+// Halo cells are formed by both tiles that touch them (1.66x the cell
+// products); the matrix cores leave the kernel bound by its memory stream.
+// The products carry about FP32 accuracy; the V-cycle stays a fixed linear
+// operator.  Lane layout of the 32x32x8 instruction: A[i][k] in lane
+// i + 32 (k / 4), element k % 4; B[k][j] in lane j + 32 (k / 4), element
+// k % 4; D[i][j] in lane j + 32 ((i / 4) % 2), element 4 (i / 8) + i % 4.
+constexpr int kTX = 16, kTY = 4, kTZ = 4;                      // tile nodes
+constexpr int kHX = kTX + 2, kHY = kTY + 2, kHZ = kTZ + 2;     // with the halo
+constexpr int kCX = kTX + 1, kCY = kTY + 1, kCZ = kTZ + 1;     // cells touching the tile
+constexpr int kHN = kHX * kHY * kHZ;
+constexpr int kCN = kCX * kCY * kCZ;
+constexpr int kCG = (kCN + 31) / 32;                           // column blocks of 32 cells
+constexpr int kZS = 24;                                        // floats per cell in LDS
+
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+typedef float floatx16_t __attribute__((ext_vector_type(16)));
+
+struct MfTiles {
+    int64_t n0, n1, n2;  // grid nodes per direction
+    int64_t tx, ty, tz;  // tiles per direction
+};
+
+template <int MODE, bool RZ>
+__global__ __launch_bounds__(kThreads) void cheb_mf_mfma_kernel(
+    MfOp<float> op, MfTiles T, const half4_t* __restrict__ kfrag, float kscale, const float* v,
+    float* __restrict__ r, float* __restrict__ x, const float* d_in, float* __restrict__ d_out, float c1, float c2,
+    const float* __restrict__ binv, const int* done, const double* __restrict__ r_outer, MgState* st,
+    double* __restrict__ partials) {
+    __shared__ _Float16 xh[kHN * 3], xl[kHN * 3];
+    __shared__ float zc[kCN * kZS];
+    __shared__ float scell[kCN];
+    __shared__ float wmax[kThreads / 64];
+    __shared__ double sh[8];
+    if (RZ ? st->done != 0 : *done != 0) return;  // uniform over the grid
+    const int tid = threadIdx.x;
+    const int64_t t = logical_block();
+    const bool live = t < T.tx * T.ty * T.tz;
+    int64_t x0 = 0, y0 = 0, z0 = 0;
+    if (live) {
+        x0 = (t % T.tx) * kTX;
+        y0 = (t / T.tx % T.ty) * kTY;
+        z0 = t / (T.tx * T.ty) * kTZ;
+    }
+    const int64_t P0 = T.n0 + 2, P1 = T.n1 + 2, Q0 = T.n0 + 1, Q1 = T.n1 + 1;
+    // this thread's tile node: its row and everything the epilogue reads,
+    // loaded first so that their latency overlaps the cell products
+    const int tx = tid % kTX, ty = tid / kTX % kTY, tz = tid / (kTX * kTY);
+    const int64_t gx = x0 + tx, gy = y0 + ty, gz = z0 + tz;
+    int32_t row = -1;
+    if (live && gx < T.n0 && gy < T.n1 && gz < T.n2) row = op.rows[(gx + 1) + P0 * ((gy + 1) + P1 * (gz + 1))];
+    float rv[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, own[3] = {0, 0, 0}, bv[9] = {};
+    uint8_t fx[3] = {0, 0, 0};
+    double ro[3] = {0, 0, 0};
+    if (row >= 0) {
+        const int64_t r3 = (int64_t)row * 3;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            rv[j] = r[r3 + j];
+            own[j] = v[r3 + j];
+            fx[j] = op.fixed[r3 + j];
+            if (MODE == 0) {
+                xv[j] = x[r3 + j];
+                dv[j] = d_in ? d_in[r3 + j] : 0.0f;
+            }
+            if (RZ) ro[j] = r_outer[r3 + j];
+        }
+        if (MODE == 0)
+#pragma unroll
+            for (int m = 0; m < 9; ++m) bv[m] = binv[(int64_t)m * op.nrows + row];
+    }
+    // A: halo values (zero outside the grid and on fixed nodes), staged as
+    // FP32 in zc, and the tile's largest magnitude; the cell scales
+    float amax = 0.0f;
+    for (int i = tid; i < kHN; i += kThreads) {
+        const int64_t hx = x0 - 1 + i % kHX, hy = y0 - 1 + i / kHX % kHY, hz = z0 - 1 + i / (kHX * kHY);
+        int32_t hr = -1;
+        if (live && hx <= T.n0 && hy <= T.n1 && hz <= T.n2) hr = op.rows[(hx + 1) + P0 * ((hy + 1) + P1 * (hz + 1))];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float val = hr >= 0 ? v[(int64_t)hr * 3 + j] : 0.0f;
+            zc[i * 3 + j] = val;
+            amax = fmaxf(amax, fabsf(val));
+        }
+    }
+    float se[2] = {0.0f, 0.0f};
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+        const int c = tid + k * kThreads;
+        const int64_t cx = x0 - 1 + c % kCX, cy = y0 - 1 + c / kCX % kCY, cz = z0 - 1 + c / (kCX * kCY);
+        if (c < kCN && live && cx <= T.n0 - 1 && cy <= T.n1 - 1 && cz <= T.n2 - 1)
+            se[k] = op.es[(cx + 1) + Q0 * ((cy + 1) + Q1 * (cz + 1))];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) amax = fmaxf(amax, __shfl_xor(amax, off, 64));
+    if ((tid & 63) == 0) wmax[tid >> 6] = amax;
+    __syncthreads();
+    amax = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+    int ex = 0;
+    (void)frexpf(amax, &ex);  // amax 2^-ex in [1/2, 1); 0 for a zero tile
+    const float xs = ldexpf(1.0f, -ex);
+    for (int i = tid; i < kHN * 3; i += kThreads) {
+        const float y = zc[i] * xs;
+        const _Float16 h = (_Float16)y;
+        xh[i] = h;
+        xl[i] = (_Float16)(y - (float)h);
+    }
+    const float unscale = ldexpf(kscale, ex);
+#pragma unroll
+    for (int k = 0; k < 2; ++k)
+        if (tid + k * kThreads < kCN) scell[tid + k * kThreads] = se[k] * unscale;
+    __syncthreads();
+    // B: Z_e = s_e Ke(0) v_e on the matrix cores, 32 cells per column block
+    const int w = tid >> 6, lane = tid & 63, half = lane >> 5;
+    half4_t ahi[3], alo[3];
+#pragma unroll
+    for (int kk = 0; kk < 3; ++kk) {
+        ahi[kk] = kfrag[kk * 64 + lane];
+        alo[kk] = kfrag[(3 + kk) * 64 + lane];
+    }
+    for (int g = w; g < kCG; g += kThreads / 64) {
+        const int cell = g * 32 + (lane & 31);
+        const bool cv = cell < kCN;
+        const int cx = cell % kCX, cy = cell / kCX % kCY, cz = cell / (kCX * kCY);
+        floatx16_t acc = {};
+#pragma unroll
+        for (int kk = 0; kk < 3; ++kk) {
+            half4_t bh, bl;
+#pragma unroll
+            for (int e = 0; e < 4; ++e) {
+                const int k = 8 * kk + 4 * half + e;  // dof of the cell: corner k / 3, component k % 3
+                const int b = k / 3, j = k % 3;
+                const int hn = (cx + (b & 1)) + kHX * ((cy + ((b >> 1) & 1)) + kHY * (cz + (b >> 2)));
+                bh[e] = cv ? xh[hn * 3 + j] : (_Float16)0.0f;
+                bl[e] = cv ? xl[hn * 3 + j] : (_Float16)0.0f;
+            }
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(ahi[kk], bh, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(ahi[kk], bl, acc, 0, 0, 0);
+            acc = __builtin_amdgcn_mfma_f32_32x32x8f16(alo[kk], bh, acc, 0, 0, 0);
+        }
+        if (cv) {
+            const float sc = scell[cell];
+#pragma unroll
+            for (int e = 0; e < 12; ++e) {  // rows 24..31 are the padding
+                const int i = 8 * (e >> 2) + 4 * half + (e & 3);
+                zc[cell * kZS + i] = acc[e] * sc;
+            }
+        }
+    }
+    __syncthreads();
+    // C: a tile node sums its 8 cells (ascending corner code), then the
+    // Chebyshev epilogue on the values loaded at the start
+    double s = 0.0;
+    if (row >= 0) {
+        float acc[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+            const int qx = q & 1, qy = (q >> 1) & 1, qz = q >> 2;
+            const int cell = (tx + qx) + kCX * ((ty + qy) + kCY * (tz + qz));
+            const int a = (1 - qx) + 2 * (1 - qy) + 4 * (1 - qz);
+#pragma unroll
+            for (int i = 0; i < 3; ++i) acc[i] += zc[cell * kZS + 3 * a + i];
+        }
+        const int64_t r3 = (int64_t)row * 3;
+        float res[3];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            res[j] = rv[j] - (fx[j] ? own[j] : acc[j]);
+            r[r3 + j] = res[j];
+        }
+        if (MODE == 0)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float z = 0.0f;
+#pragma unroll
+                for (int i = 0; i < 3; ++i) z = fmaf(bv[j * 3 + i], res[i], z);
+                const float dn = fmaf(c2, z, d_in ? c1 * dv[j] : 0.0f);
+                d_out[r3 + j] = dn;
+                const float xn = xv[j] + dn;
+                x[r3 + j] = xn;
+                if (RZ) s = fma(ro[j], (double)xn, s);
+            }
+    }
+    if (RZ) rz_finish(st, partials, sh, s);
+}
+
 __global__ __launch_bounds__(kThreads) void mg_p_kernel(double* __restrict__ p, const float* __restrict__ z, int64_t n,
                                                         int64_t per_block, const MgState* st) {
     if (st->done) return;
@@ -1503,6 +1707,12 @@ MfOp<float> mf_op32(const sfem_system* S, const MgHierarchy& H) {
     return mf_op<float>(S, H, H.kc32.get(), H.es32.get());
 }
 
+MfTiles mf_tiles(const MgHierarchy& H) {
+    const MgLevel& L0 = H.L[0];
+    return {L0.n[0], L0.n[1], L0.n[2], (L0.n[0] + kTX - 1) / kTX, (L0.n[1] + kTY - 1) / kTY,
+            (L0.n[2] + kTZ - 1) / kTZ};
+}
+
 // Gauss-Jordan inverse with partial pivoting (FP64); false if singular.
 bool dense_inverse(std::vector<double>& a, int64_t n, std::vector<double>& inv) {
     inv.assign((size_t)(n * n), 0.0);
@@ -1617,6 +1827,35 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         hipLaunchKernelGGL((ghost_cells_kernel<ND, double>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
                            K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es64.get());
         SFEM_LAUNCH_CHECK();
+        // SFEM_MF_MFMA=1: the smoothing steps on the matrix cores.  Off by
+        // default: on MI355X the VALU kernel is faster at 257 x 65 x 65 nodes
+        // (59 us a step against 66-94 us; the tile's halo, 2.5 x its nodes,
+        // and 3 workgroups per CU leave the loads exposed)
+        const char* mm = std::getenv("SFEM_MF_MFMA");
+        if (ND == 3 && mm && mm[0] == '1') {
+            double kmax = 0.0;
+            for (double vv : kc) kmax = std::max(kmax, std::fabs(vv));
+            int ek = 0;
+            (void)std::frexp(kmax, &ek);
+            std::vector<_Float16> fr((size_t)2 * 3 * 64 * 4);
+            for (int kk = 0; kk < 3; ++kk)
+                for (int lane = 0; lane < 64; ++lane)
+                    for (int e = 0; e < 4; ++e) {
+                        const int i = lane % 32, k = 8 * kk + 4 * (lane / 32) + e;  // A[i][k], i < 24 real
+                        float y = 0.0f;
+                        if (i < 24) {
+                            const int a = i / 3, ii = i % 3, b = k / 3, jj = k % 3;
+                            y = (float)std::ldexp(kc[((size_t)a * NEN + b) * B2 + ii * 3 + jj], -ek);
+                        }
+                        const _Float16 hi = (_Float16)y;
+                        fr[((size_t)(0 * 3 + kk) * 64 + lane) * 4 + e] = hi;
+                        fr[((size_t)(1 * 3 + kk) * 64 + lane) * 4 + e] = (_Float16)(y - (float)hi);
+                    }
+            H.kfrag.reset(ctx, fr.size());
+            h2d(ctx, H.kfrag.get(), fr.data(), fr.size() * sizeof(_Float16));
+            H.kscale = (float)std::ldexp(1.0, ek);
+            H.mfma = true;
+        }
     }
 
     // the sliced values of K_ff: read by the finest level unless it is matrix-free
@@ -1906,7 +2145,24 @@ struct MgRun {
         // timed: the plain finest-level smoothing steps
         const bool timed = ev && l == 0 && mode == 0 && !fused && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
-        if (fused) {
+        if (l == 0 && H.mf && H.mfma) {  // the matrix cores (3D)
+            const MfTiles T = mf_tiles(H);
+            const dim3 grid((unsigned)((T.tx * T.ty * T.tz + 7) / 8 * 8));
+            const half4_t* kf = (const half4_t*)H.kfrag.get();
+            const float* bi = (const float*)S->binv32.get();
+            if (fused)
+                hipLaunchKernelGGL((cheb_mf_mfma_kernel<0, true>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
+                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, rz_r, rz_st,
+                                   rz_partials);
+            else if (mode == 0)
+                hipLaunchKernelGGL((cheb_mf_mfma_kernel<0, false>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
+                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, nullptr,
+                                   nullptr, nullptr);
+            else
+                hipLaunchKernelGGL((cheb_mf_mfma_kernel<1, false>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
+                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, nullptr,
+                                   nullptr, nullptr);
+        } else if (fused) {
             hipLaunchKernelGGL(cheb_mf_rz_kernel<ND>, dim3(xcd_rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H),
                                v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), rz_r,
                                rz_st, rz_partials);
@@ -2032,7 +2288,13 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     MfOp<double> mf64{};
     if (mf_outer) mf64 = mf_op<double>(S, H, H.kc64.get(), H.es64.get());
     else ensure_sell(ctx, S);  // the outer operator reads the sliced values
-    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u, xcd_rows_grid(S->nfb)}));
+    unsigned tiles = 0;
+    if (H.mfma) {
+        const MfTiles T = mf_tiles(H);
+        tiles = (unsigned)((T.tx * T.ty * T.tz + 7) / 8 * 8);
+    }
+    DevBuf<double> partials(ctx, 2 * (size_t)std::max({ls.grid, lv.grid, mf_grid, 2048u, xcd_rows_grid(S->nfb),
+                                                        tiles}));
     DevBuf<MgState> dst(ctx, 1);
     if (o->x0_zero) memset_d(ctx, S->x.get(), 0, sizeof(double) * S->nf);
     if (padded) pad_gather(ctx, S, S->x.get(), x);
@@ -2091,8 +2353,13 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         run.ev = &ev;
     }
     d2h(ctx, &host, st, sizeof(host));
+    // iterations enqueued per poll: chunk_max, then the count the observed
+    // contraction of r.r predicts to the stop (V-cycles enqueued past the
+    // stop exit at once but still cost their launches)
+    int64_t next = chunk_max, seen_iters = 0;
+    double seen_rr = host.bb;
     while (!host.done && enq < o->max_iter) {
-        const int64_t chunk = std::min<int64_t>(chunk_max, o->max_iter - enq);
+        const int64_t chunk = std::min<int64_t>(next, o->max_iter - enq);
         run.ev_next = 0;
         for (int64_t k = 0; k < chunk; ++k) {
             true_resid();
@@ -2133,6 +2400,15 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
             }
         }
         counted = host.iters;
+        next = chunk_max;
+        if (host.iters > seen_iters && host.rr > 0.0 && host.rr < seen_rr && host.tol2bb > 0.0 &&
+            std::isfinite(host.rr)) {
+            const double per_iter_log = std::log(host.rr / seen_rr) / (double)(host.iters - seen_iters);
+            const double left = std::log(host.tol2bb / host.rr) / per_iter_log;
+            if (std::isfinite(left)) next = std::max<int64_t>(1, std::min<int64_t>(chunk_max, (int64_t)std::ceil(left)));
+        }
+        seen_iters = host.iters;
+        seen_rr = host.rr;
     }
     for (auto& e : ev) (void)hipEventDestroy(e);
     if (!host.done) {  // iteration cap: the true residual of the final iterate

@@ -405,3 +405,44 @@ def test_mg_matrix_free_needs_grid_assembly(ctx):
     assert info["matrix_free"] == 0 and info["ke_deviation"] == float("inf")
     S.close()
     Kd.close()
+
+
+@pytest.mark.parametrize("dims", [(20, 9, 6), (33, 12, 10)])
+def test_mg_matrix_free_mfma_matches_valu(ctx, dims, monkeypatch):
+    """The matrix-free smoothing steps on the matrix cores (split FP16
+    products, tiles of 16 x 4 x 4 nodes: these grids leave partial tiles in
+    every direction) against the VALU kernel: both preconditioners reach the
+    oracle's solution, in the same number of iterations up to one, and the
+    MFMA one is bitwise reproducible."""
+    print(f"[case] mg mfma vs valu {dims}", flush=True)
+    m = structured_mesh(dims, lengths=(2.0, 1.0, 0.8))
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=17)
+    fixed = roller(m, axis=2)
+    f = random_face_load(m, 2, 1, fixed, seed=17)
+    q_d = np.zeros(m.n_dof)
+    gi = m.node_grid_index()
+    top = np.nonzero(gi[2] == m.dims[2])[0]
+    fixed[3 * top + 2] = 1  # partially fixed nodes with a prescribed value
+    q_d[3 * top + 2] = -0.01
+    import scipy.sparse.linalg as spla
+
+    Kff, rhs = O.eliminate(oracle_K(m, young), fixed, f, q_d)
+    ref = O.recombine(fixed, q_d, spla.spsolve(Kff.tocsc(), rhs))
+    K = grid_K(ctx, m, young)
+    its, sols = {}, {}
+    for flag in ("0", "1", "1"):
+        monkeypatch.setenv("SFEM_MF_MFMA", flag)
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed, q_d)
+        S = System(K, bc)
+        bc.close()
+        assert S.set_grid(m.dims)["matrix_free"] == 2
+        st = S.solve("fp64", rtol=1e-12, preconditioner="multigrid")
+        q = S.solution()
+        assert st["stop_reason"] == 1 and relerr(q, ref) <= ERR, flag
+        if flag in sols:
+            assert np.array_equal(q, sols[flag])  # run to run
+        its[flag], sols[flag] = st["iterations"], q
+        S.close()
+    K.close()
+    print(f"[mg] iterations valu {its['0']} mfma {its['1']}", flush=True)
+    assert abs(its["1"] - its["0"]) <= 1
