@@ -52,6 +52,7 @@ struct MgLevel {
     DevBuf<double> st64;       // masked symmetric FP64 stencil (levels >= 1)
     DevBuf<float> st32;        // smoothing operator: st64 with a unit diagonal on fixed dofs
     DevBuf<_Float16> st16;     // or (FP16 coarse levels) its FP16 values times `scale`
+    bool node_major = false;   // st32 / st16 layout (st_index): lane-group kernels
     double scale = 1.0;        // 2^-e: the level's smoother works on residuals times scale
     DevBuf<float> binv;        // inverse diagonal blocks, [(j*BS+i)*nrows + row]
     DevBuf<uint8_t> fixed;     // nrows*bs: the dof is fixed
@@ -373,51 +374,55 @@ __global__ void galerkin_line_kernel(const double* __restrict__ in, Grid gi, int
     }
 }
 
+// A coarse smoothing stencil's value (slot s, entry m) of a node: slot-major
+// [(s*B2 + m)*nn + node] on the large levels (one thread per node, loads
+// coalesced along nodes), node-major [(node*NS + s)*B2 + m] on the small ones
+// (a lane group per node reads one contiguous run).
+template <int ND>
+__host__ __device__ __forceinline__ int64_t st_index(bool node_major, int64_t nn, int64_t node, int s, int m) {
+    constexpr int B2 = ND * ND, NS = Sten<ND>::NS;
+    return node_major ? (node * NS + s) * B2 + m : ((int64_t)s * B2 + m) * nn + node;
+}
+
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
 // side), zero the rows and columns of fixed dofs (st64, slot-major, the next
-// Galerkin product's input) and put a unit diagonal on them (st32,
-// node-major, the smoother's operator).  A block handles TI consecutive
-// nodes: thread (s, node) writes st64 (coalesced along nodes), and the st32
-// tile goes through LDS so that it is written contiguously.
+// Galerkin product's input) and put a unit diagonal on them (st32, the
+// smoother's operator, in the level's layout).  Thread (s, node) writes
+// coalesced along nodes.
 template <int ND>
 __global__ __launch_bounds__(kThreads) void sym_mask_kernel(const double* __restrict__ raw, Grid g,
                                                             const uint8_t* __restrict__ fixed,
-                                                            double* __restrict__ st64, float* __restrict__ st32) {
+                                                            double* __restrict__ st64, float* __restrict__ st32,
+                                                            bool node_major) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, TI = kThreads / NS;
-    __shared__ float tile[TI * NS * B2];
     const int64_t nn = g.n0 * g.n1 * g.n2;
     const int64_t I0 = (int64_t)blockIdx.x * TI;
     const int t = threadIdx.x;
-    if (t < TI * NS) {
-        const int s = t / TI, li = t % TI;
-        const int64_t I = I0 + li;
-        if (I < nn) {
-            int64_t c[3];
-            coords_of<ND>(I, g, c);
-            int dx, dy, dz;
-            offset_of<ND>(s, dx, dy, dz);
-            const bool valid = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
-            const int64_t J = valid ? node_at(g, c[0] + dx, c[1] + dy, c[2] + dz) : I;
-            const int sp = NS - 1 - s;
-            for (int a = 0; a < BS; ++a)
-                for (int b = 0; b < BS; ++b) {
-                    double v = 0.0;
-                    bool unit = false;
-                    if (valid) {
-                        const bool masked = fixed[I * BS + a] || fixed[J * BS + b];
-                        v = masked ? 0.0
-                                   : 0.5 * (raw[((int64_t)s * B2 + a * BS + b) * nn + I] +
-                                            raw[((int64_t)sp * B2 + b * BS + a) * nn + J]);
-                        unit = masked && s == Sten<ND>::C && a == b;
-                    }
-                    st64[((int64_t)s * B2 + a * BS + b) * nn + I] = v;
-                    tile[(li * NS + s) * B2 + a * BS + b] = unit ? 1.0f : (float)v;
-                }
+    if (t >= TI * NS) return;
+    const int s = t / TI, li = t % TI;
+    const int64_t I = I0 + li;
+    if (I >= nn) return;
+    int64_t c[3];
+    coords_of<ND>(I, g, c);
+    int dx, dy, dz;
+    offset_of<ND>(s, dx, dy, dz);
+    const bool valid = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
+    const int64_t J = valid ? node_at(g, c[0] + dx, c[1] + dy, c[2] + dz) : I;
+    const int sp = NS - 1 - s;
+    for (int a = 0; a < BS; ++a)
+        for (int b = 0; b < BS; ++b) {
+            double v = 0.0;
+            bool unit = false;
+            if (valid) {
+                const bool masked = fixed[I * BS + a] || fixed[J * BS + b];
+                v = masked ? 0.0
+                           : 0.5 * (raw[((int64_t)s * B2 + a * BS + b) * nn + I] +
+                                    raw[((int64_t)sp * B2 + b * BS + a) * nn + J]);
+                unit = masked && s == Sten<ND>::C && a == b;
+            }
+            st64[((int64_t)s * B2 + a * BS + b) * nn + I] = v;
+            st32[st_index<ND>(node_major, nn, I, s, a * BS + b)] = unit ? 1.0f : (float)v;
         }
-    }
-    __syncthreads();
-    const int64_t nt = (min(nn, I0 + TI) - I0) * NS * B2;
-    for (int64_t e = t; e < nt; e += kThreads) st32[I0 * NS * B2 + e] = tile[e];
 }
 
 template <int ND>
@@ -476,31 +481,33 @@ __global__ void binv_half_kernel(const int64_t* __restrict__ sl_ptr, const int32
     for (int k = 0; k < B2; ++k) binv[(int64_t)k * nrows + row] = (float)inv[k];
 }
 
-// max over the diagonal of a node-major FP32 stencil (non-negative: SPD
+// max over the diagonal of an FP32 smoothing stencil (non-negative: SPD
 // operator with unit diagonals on fixed dofs), as ordered bit patterns
 template <int ND>
-__global__ void stencil_maxdiag_kernel(const float* __restrict__ st32, int64_t nn, unsigned* __restrict__ out) {
+__global__ void stencil_maxdiag_kernel(const float* __restrict__ st32, int64_t nn, bool node_major,
+                                       unsigned* __restrict__ out) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, C = Sten<ND>::C;
     unsigned m = 0;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < nn * BS;
          i += (int64_t)gridDim.x * blockDim.x) {
         const int64_t node = i / BS;
         const int a = (int)(i % BS);
-        m = max(m, (unsigned)__float_as_int(fabsf(st32[(node * NS + C) * B2 + a * BS + a])));
+        m = max(m, (unsigned)__float_as_int(fabsf(st32[st_index<ND>(node_major, nn, node, C, a * BS + a)])));
     }
     for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off, 64));
     if ((threadIdx.x & 63) == 0) atomicMax(out, m);
 }
 
-// inverse diagonal blocks of a node-major FP16 stencil (the values as stored,
-// i.e. times the level's scale), planar [(j*BS+i)*nn + node]
+// inverse diagonal blocks of an FP16 smoothing stencil (the values as
+// stored, i.e. times the level's scale), planar [(j*BS+i)*nn + node]
 template <int ND>
-__global__ void stencil_binv_half_kernel(const _Float16* __restrict__ st16, int64_t nn, float* __restrict__ binv) {
+__global__ void stencil_binv_half_kernel(const _Float16* __restrict__ st16, int64_t nn, bool node_major,
+                                         float* __restrict__ binv) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, C = Sten<ND>::C;
     const int64_t node = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (node >= nn) return;
     double m[B2], inv[B2];
-    for (int k = 0; k < B2; ++k) m[k] = (double)st16[(node * NS + C) * B2 + k];
+    for (int k = 0; k < B2; ++k) m[k] = (double)st16[st_index<ND>(node_major, nn, node, C, k)];
     invert_block<BS>(m, inv);
     for (int k = 0; k < B2; ++k) binv[(int64_t)k * nn + node] = (float)inv[k];
 }
@@ -517,7 +524,8 @@ struct SellOp {
     int64_t nslices, nrows, per_block;
 };
 
-// coarse level operator: node-major stencil values, FP32 or FP16 (scaled)
+// coarse level operator: stencil values (st_index: slot-major for one thread
+// per node, node-major for lane groups), FP32 or FP16 (scaled)
 template <typename TV>
 struct StOp {
     const TV* val;
@@ -525,46 +533,160 @@ struct StOp {
     int64_t nn;
 };
 
-// Lanes per node in the coarse-level kernels (one per stencil slot).
+// Row `node` of a coarse level's operator times v, one thread per node: slot
+// s's block (read coalesced along nodes) times the neighbour's values, and
+// the G = 32 (3D) / 16 (2D) slot products (slots past the stencil's NS and
+// outside the grid are zero) summed by a fixed pairwise tree, p[i] += p[i+h]
+// for h = G/2, ..., 1.  The tree is walked depth first over its leaves in
+// bit-reversed order (leaf k is slot bitrev(k), and leaves 2m, 2m+1 are
+// slots i, i + G/2), four leaves per loop trip, so that few products are
+// live at a time; a stack holds the pending subtree sums.  Leaves of slots
+// >= NS are all right children of present ones and are left out: no product
+// or sum is -0 (each is a rounded fma chain from +0), so x + 0 = x and
+// leaving them out changes no bit.  A slot outside the grid loads the
+// node's own values (in bounds) and its product is dropped by a select.
+// Offsets are 32-bit byte counts from uniform bases (the stencil and the
+// vector under 4 GB each, checked when the level is built).
 template <int ND>
 struct Lanes {
     static constexpr int G = ND == 3 ? 32 : 16;
+    static constexpr int BITS = ND == 3 ? 5 : 4;
 };
 
-// Row `node` of a coarse level's operator times v, computed by a group of G
-// lanes: lane s < NS multiplies slot s's block (node-major FP32 stencil,
-// val[(node*NS + s)*B2 + m], one contiguous run per node) by the neighbour's
-// values, then a fixed butterfly over the group sums the slots; the group's
-// first lane holds the result.  Every lane of the group must call it.
+template <int ND>
+__device__ __forceinline__ int stencil_leaf_slot(int k) {
+    return (int)(__builtin_bitreverse32((uint32_t)k) >> (32 - Lanes<ND>::BITS));
+}
+
 template <int ND, typename TV>
-__device__ __forceinline__ void stencil_row_group(const StOp<TV>& op, int64_t node, int s, const float* __restrict__ v,
+__device__ __forceinline__ void stencil_leaf(const char* __restrict__ val, const char* __restrict__ v, const Grid& g,
+                                             uint32_t nb, uint32_t own, int node, const int (&c)[3], int sl,
+                                             float (&t)[ND]) {
+    constexpr int BS = ND, B2 = BS * BS;
+    int dx, dy, dz;
+    offset_of<ND>(sl, dx, dy, dz);
+    const bool in = inside<ND>(g, c[0] + dx, c[1] + dy, c[2] + dz);
+    const int m = in ? node + dx + (int)g.n0 * (dy + (int)g.n1 * dz) : node;
+    const uint32_t mo = (uint32_t)m * (uint32_t)(BS * sizeof(float));
+    const uint32_t so = own + (uint32_t)(sl * B2) * nb;
+    float av[B2], xv[BS];
+#pragma unroll
+    for (int k = 0; k < B2; ++k) av[k] = (float)*(const TV*)(val + (so + (uint32_t)k * nb));
+#pragma unroll
+    for (int i = 0; i < BS; ++i) xv[i] = *(const float*)(v + (mo + (uint32_t)(i * sizeof(float))));
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float a = 0.0f;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) a = fmaf(av[j * BS + i], xv[i], a);
+        t[j] = in ? a : 0.0f;
+    }
+}
+
+template <int ND, typename TV>
+__device__ __forceinline__ void stencil_row(const StOp<TV>& op, int64_t node64, const float* __restrict__ v,
+                                            float (&out)[ND]) {
+    constexpr int BS = ND, NS = Sten<ND>::NS, NG = Lanes<ND>::G / 4;
+    const char* val = (const char*)op.val;
+    const char* vb = (const char*)v;
+    const uint32_t nb = (uint32_t)(op.nn * (int64_t)sizeof(TV));
+    const uint32_t own = (uint32_t)(node64 * (int64_t)sizeof(TV));
+    const int node = (int)node64;
+    int c[3];
+    c[0] = node % (int)op.g.n0;
+    c[1] = (node / (int)op.g.n0) % (int)op.g.n1;
+    c[2] = ND == 3 ? node / ((int)op.g.n0 * (int)op.g.n1) : 0;
+    float st[3][BS];  // pending sums of 4, 8 and 16 leaves
+#pragma unroll 1
+    for (int q = 0; q < NG; ++q) {
+        float l[4][BS];
+#pragma unroll
+        for (int h = 0; h < 4; ++h) {
+            const int sl = stencil_leaf_slot<ND>(4 * q + h);
+            if (sl < NS) stencil_leaf<ND, TV>(val, vb, op.g, nb, own, node, c, sl, l[h]);  // uniform
+        }
+        const bool r1 = stencil_leaf_slot<ND>(4 * q + 1) < NS, r3 = stencil_leaf_slot<ND>(4 * q + 3) < NS;
+        float w[BS];
+#pragma unroll
+        for (int j = 0; j < BS; ++j) w[j] = (r1 ? l[0][j] + l[1][j] : l[0][j]) + (r3 ? l[2][j] + l[3][j] : l[2][j]);
+        // combine with the pending subtrees of this one's ancestors (q's trailing ones)
+        if (!(q & 1)) {
+#pragma unroll
+            for (int j = 0; j < BS; ++j) st[0][j] = w[j];
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < BS; ++j) w[j] = st[0][j] + w[j];
+        if (!(q & 2)) {
+#pragma unroll
+            for (int j = 0; j < BS; ++j) st[1][j] = w[j];
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < BS; ++j) w[j] = st[1][j] + w[j];
+        if (NG == 4 || !(q & 4)) {  // 2D: 16 leaves, the root
+#pragma unroll
+            for (int j = 0; j < BS; ++j) st[2][j] = w[j];
+            continue;
+        }
+#pragma unroll
+        for (int j = 0; j < BS; ++j) st[2][j] = st[2][j] + w[j];  // 3D: the root
+    }
+#pragma unroll
+    for (int j = 0; j < BS; ++j) out[j] = st[2][j];
+}
+
+// The same row by a group of G lanes on a node-major stencil (small levels,
+// where one thread per node leaves the device idle and its 8 rounds of loads
+// exposed): lane s < NS forms slot s's product from the node's contiguous
+// run, then a butterfly over the group, p[i] += p[i ^ h] for h = G/2, ...,
+// 1 -- the tree above, so both forms give the same bits.  The group's first
+// lane holds the sum; every lane of the group must call it.
+template <int ND, typename TV>
+__device__ __forceinline__ void stencil_row_group(const StOp<TV>& op, int64_t node, int sl, const float* __restrict__ v,
                                                   float (&out)[ND]) {
     constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, G = Lanes<ND>::G;
 #pragma unroll
     for (int j = 0; j < BS; ++j) out[j] = 0.0f;
-    if (s < NS && node < op.nn) {
-        int64_t c[3];
-        coords_of<ND>(node, op.g, c);
+    if (sl < NS && node < op.nn) {
+        const int nd = (int)node;
+        int c[3];
+        c[0] = nd % (int)op.g.n0;
+        c[1] = (nd / (int)op.g.n0) % (int)op.g.n1;
+        c[2] = ND == 3 ? nd / ((int)op.g.n0 * (int)op.g.n1) : 0;
         int dx, dy, dz;
-        offset_of<ND>(s, dx, dy, dz);
+        offset_of<ND>(sl, dx, dy, dz);
         if (inside<ND>(op.g, c[0] + dx, c[1] + dy, c[2] + dz)) {
-            const int64_t m = node + dx + op.g.n0 * (dy + op.g.n1 * (int64_t)dz);
-            const TV* a = op.val + ((int64_t)node * NS + s) * B2;
+            const int m = nd + dx + (int)op.g.n0 * (dy + (int)op.g.n1 * dz);
+            const TV* a = op.val + ((int64_t)nd * NS + sl) * B2;
             float av[B2], xv[BS];
 #pragma unroll
             for (int k = 0; k < B2; ++k) av[k] = (float)a[k];
 #pragma unroll
-            for (int i = 0; i < BS; ++i) xv[i] = v[m * BS + i];
+            for (int i = 0; i < BS; ++i) xv[i] = v[(int64_t)m * BS + i];
 #pragma unroll
-            for (int j = 0; j < BS; ++j)
+            for (int j = 0; j < BS; ++j) {
+                float t = 0.0f;
 #pragma unroll
-                for (int i = 0; i < BS; ++i) out[j] = fmaf(av[j * BS + i], xv[i], out[j]);
+                for (int i = 0; i < BS; ++i) t = fmaf(av[j * BS + i], xv[i], t);
+                out[j] = t;
+            }
         }
     }
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1)
 #pragma unroll
         for (int j = 0; j < BS; ++j) out[j] += __shfl_xor(out[j], off, G);
+}
+
+// Levels with fewer nodes than this store their stencil node-major and run
+// the lane-group form (SFEM_MG_GROUP_BELOW overrides it, read when the
+// hierarchy is built: both forms give the same bits).
+constexpr int64_t kStencilGroupBelow = 65536;
+
+int64_t stencil_group_below() {
+    const char* e = std::getenv("SFEM_MG_GROUP_BELOW");
+    return e && *e ? std::atoll(e) : kStencilGroupBelow;
 }
 
 // MODE 0: r -= A v; d_out = c1 d_in + c2 B^-1 r; x += d_out (a Chebyshev step)
@@ -611,19 +733,24 @@ __global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp<TV> op, cons
     }
 }
 
-template <int ND, int MODE, typename TV>
-__global__ __launch_bounds__(kThreads) void cheb_stencil_kernel(StOp<TV> op, const float* v, float* __restrict__ r,
-                                                                float* __restrict__ x, const float* d_in,
-                                                                float* __restrict__ d_out, float c1, float c2,
-                                                                const float* __restrict__ binv, const int* done) {
+template <int ND, int MODE, bool GROUP, typename TV>
+__global__ __launch_bounds__(kThreads, 4) void cheb_stencil_kernel(StOp<TV> op, const float* v, float* __restrict__ r,
+                                                                   float* __restrict__ x, const float* d_in,
+                                                                   float* __restrict__ d_out, float c1, float c2,
+                                                                   const float* __restrict__ binv, const int* done) {
     constexpr int G = Lanes<ND>::G;
     if (*done) return;
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    const int64_t node = t / G;
-    const int s = (int)(t % G);
     float out[ND];
-    stencil_row_group<ND, TV>(op, node, s, v, out);
-    if (s == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+    if (GROUP) {
+        const int64_t node = t / G;
+        stencil_row_group<ND, TV>(op, node, (int)(t % G), v, out);
+        if (t % G == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+    } else {
+        if (t >= op.nn) return;
+        stencil_row<ND, TV>(op, t, v, out);
+        cheb_epilogue<ND, MODE>(t, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+    }
 }
 
 // ------------------------------------------- matrix-free finest level --
@@ -806,17 +933,12 @@ __global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restri
 // 1 / rscale.  FIRST: also the coarse level's first smoothing step
 // (cheb_first_kernel: r_c = b_c, d_c = c2 B^-1 b_c, x_c = d_c).
 template <int ND, bool FIRST>
-__global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, float rscale, Grid gf,
-                                                            const int32_t* __restrict__ row_of_node, Grid gc, int f0,
-                                                            int f1, int f2, const uint8_t* __restrict__ cfix,
-                                                            float* __restrict__ bc, const float* __restrict__ binv,
-                                                            float c2, float cscale, float* __restrict__ rc,
-                                                            float* __restrict__ xc, float* __restrict__ dc,
-                                                            const int* done) {
+__device__ __forceinline__ void restrict_row(int64_t I, const float* __restrict__ rf, float rscale, const Grid& gf,
+                                             const int32_t* __restrict__ row_of_node, const Grid& gc, int f0, int f1,
+                                             int f2, const uint8_t* __restrict__ cfix, float* __restrict__ bc,
+                                             const float* __restrict__ binv, float c2, float cscale,
+                                             float* __restrict__ rc, float* __restrict__ xc, float* __restrict__ dc) {
     constexpr int BS = ND;
-    if (*done) return;
-    const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (I >= gc.n0 * gc.n1 * gc.n2) return;
     int64_t ci[3];
     coords_of<ND>(I, gc, ci);
     // the interpolation support of I per direction (child(); the identity
@@ -892,18 +1014,28 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
     }
 }
 
+template <int ND, bool FIRST>
+__global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restrict__ rf, float rscale, Grid gf,
+                                                            const int32_t* __restrict__ row_of_node, Grid gc, int f0,
+                                                            int f1, int f2, const uint8_t* __restrict__ cfix,
+                                                            float* __restrict__ bc, const float* __restrict__ binv,
+                                                            float c2, float cscale, float* __restrict__ rc,
+                                                            float* __restrict__ xc, float* __restrict__ dc,
+                                                            const int* done) {
+    if (*done) return;
+    const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (I >= gc.n0 * gc.n1 * gc.n2) return;
+    restrict_row<ND, FIRST>(I, rf, rscale, gf, row_of_node, gc, f0, f1, f2, cfix, bc, binv, c2, cscale, rc, xc, dc);
+}
+
 // e = P x_c on the fine rows (0 on fixed fine dofs), x_f += e; node_of_row
 // maps rows to fine grid nodes (NULL: the identity).
 template <int ND>
-__global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restrict__ xc, Grid gc, Grid gf, int f0,
-                                                           int f1, int f2, const int32_t* __restrict__ node_of_row,
-                                                           int64_t nrows, const uint8_t* __restrict__ ffix,
-                                                           float* __restrict__ xf, float* __restrict__ e,
-                                                           const int* done) {
+__device__ __forceinline__ void prolong_row(int64_t row, const float* __restrict__ xc, const Grid& gc, const Grid& gf,
+                                            int f0, int f1, int f2, const int32_t* __restrict__ node_of_row,
+                                            const uint8_t* __restrict__ ffix, float* __restrict__ xf,
+                                            float* __restrict__ e) {
     constexpr int BS = ND;
-    if (*done) return;
-    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
-    if (row >= nrows) return;
     const int64_t node = node_of_row ? (int64_t)node_of_row[row] : row;
     int64_t cf[3];
     coords_of<ND>(node, gf, cf);
@@ -956,6 +1088,18 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
         e[d] = ev;
         xf[d] += ev;
     }
+}
+
+template <int ND>
+__global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restrict__ xc, Grid gc, Grid gf, int f0,
+                                                           int f1, int f2, const int32_t* __restrict__ node_of_row,
+                                                           int64_t nrows, const uint8_t* __restrict__ ffix,
+                                                           float* __restrict__ xf, float* __restrict__ e,
+                                                           const int* done) {
+    if (*done) return;
+    const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (row >= nrows) return;
+    prolong_row<ND>(row, xc, gc, gf, f0, f1, f2, node_of_row, ffix, xf, e);
 }
 
 // coarsest level: x = A^-1 b with the dense FP64 inverse (one block)
@@ -1059,11 +1203,12 @@ __global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp<TV> op, const
     if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
 }
 
-template <int ND, typename TV>
-__global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp<TV> op, const float* __restrict__ v,
-                                                               const float* __restrict__ binv, float* __restrict__ w,
-                                                               PowState* ps, double* __restrict__ partials) {
-    constexpr int G = Lanes<ND>::G;
+template <int ND, bool GROUP, typename TV>
+__global__ __launch_bounds__(kThreads, 4) void pow_stencil_kernel(StOp<TV> op, const float* __restrict__ v,
+                                                                  const float* __restrict__ binv,
+                                                                  float* __restrict__ w, PowState* ps,
+                                                                  double* __restrict__ partials) {
+    constexpr int G = GROUP ? Lanes<ND>::G : 1;
     __shared__ double sh[8];
     const float scale = (float)ps->inv;
     double sum = 0.0;
@@ -1072,10 +1217,10 @@ __global__ __launch_bounds__(kThreads) void pow_stencil_kernel(StOp<TV> op, cons
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < op.nn * G;
          t += (int64_t)gridDim.x * kThreads) {
         const int64_t node = t / G;
-        const int s = (int)(t % G);
         float out[ND];
-        stencil_row_group<ND, TV>(op, node, s, v, out);
-        if (s == 0) sum += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
+        if (GROUP) stencil_row_group<ND, TV>(op, node, (int)(t % G), v, out);
+        else stencil_row<ND, TV>(op, node, v, out);
+        if (t % G == 0) sum += pow_epilogue<ND>(node, op.nn, out, scale, binv, w);
     }
     const double b = block_sum_256(sum, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
@@ -1934,11 +2079,16 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             }
             gcur = gn;
         }
+        // the smoothing kernels address a coarse stencil with 32-bit byte offsets
+        if ((uint64_t)C.nn * NS * B2 * sizeof(float) >= (1ull << 32))
+            fail(SFEM_ERR_UNSUPPORTED, "multigrid: a coarse level's stencil exceeds 4 GiB (" + std::to_string(C.nn) +
+                                           " nodes)");
         C.st64.reset(ctx, (size_t)C.nn * NS * B2);
         C.st32.reset(ctx, (size_t)C.nn * NS * B2);
+        C.node_major = C.nn < stencil_group_below();
         hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3((unsigned)((C.nn + kThreads / NS - 1) / (kThreads / NS))),
                            dim3(kThreads), 0, s, raw.get(), gc,
-                           C.fixed.get(), C.st64.get(), C.st32.get());
+                           C.fixed.get(), C.st64.get(), C.st32.get(), C.node_major);
         SFEM_LAUNCH_CHECK();
         C.binv.reset(ctx, (size_t)C.nn * B2);
         hipLaunchKernelGGL(stencil_binv_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, C.st64.get(),
@@ -1997,7 +2147,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             DevBuf<unsigned> mx(ctx, 1);
             memset_d(ctx, mx.get(), 0, sizeof(unsigned));
             hipLaunchKernelGGL(stencil_maxdiag_kernel<ND>, dim3(std::min(rows_grid(v.nn * BS), 256u)), dim3(kThreads),
-                               0, s, (const float*)v.st32.get(), v.nn, mx.get());
+                               0, s, (const float*)v.st32.get(), v.nn, v.node_major, mx.get());
             SFEM_LAUNCH_CHECK();
             unsigned hm = 0;
             d2h(ctx, &hm, mx.get(), sizeof(hm));
@@ -2012,7 +2162,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
                                (const float*)v.st32.get(), nv, (float)v.scale, v.st16.get());
             SFEM_LAUNCH_CHECK();
             hipLaunchKernelGGL(stencil_binv_half_kernel<ND>, dim3(rows_grid(v.nn)), dim3(kThreads), 0, s,
-                               (const _Float16*)v.st16.get(), v.nn, v.binv.get());
+                               (const _Float16*)v.st16.get(), v.nn, v.node_major, v.binv.get());
             SFEM_LAUNCH_CHECK();
             v.st32.release();
         }
@@ -2067,7 +2217,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     auto init_grid = [&](int l) { return std::min<unsigned>(rows_grid(H.L[(size_t)l].nrows * BS), 1024); };
     unsigned maxgrid = ls.grid;
     for (int l = 0; l < nl; ++l)
-        maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn * Lanes<ND>::G), init_grid(l)});
+        maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn), init_grid(l)});
     DevBuf<double> partials(ctx, (size_t)maxgrid);
     for (int l = 0; l + 1 < nl; ++l) {
         MgLevel& v = H.L[l];
@@ -2094,15 +2244,18 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
                                    S->binv32.get(), vout, ps.get(), partials.get());
             } else {
                 // grid-stride over the nodes: few blocks meet at the last-block count
-                const dim3 gp(std::min(rows_grid(v.nn * Lanes<ND>::G), 1024u));
+                const bool grp = v.node_major;
+                const dim3 gp(std::min(rows_grid(v.nn * (grp ? Lanes<ND>::G : 1)), 1024u));
+                auto go = [&](auto k, auto op) {
+                    hipLaunchKernelGGL(k, gp, dim3(kThreads), 0, s, op, vin, v.binv.get(), vout, ps.get() + l,
+                                       partials.get());
+                };
                 if (v.st16.size()) {
                     const StOp<_Float16> op{v.st16.get(), grid_of(v), v.nn};
-                    hipLaunchKernelGGL((pow_stencil_kernel<ND, _Float16>), gp, dim3(kThreads), 0, s, op, vin,
-                                       v.binv.get(), vout, ps.get() + l, partials.get());
+                    grp ? go(pow_stencil_kernel<ND, true, _Float16>, op) : go(pow_stencil_kernel<ND, false, _Float16>, op);
                 } else {
                     const StOp<float> op{v.st32.get(), grid_of(v), v.nn};
-                    hipLaunchKernelGGL((pow_stencil_kernel<ND, float>), gp, dim3(kThreads), 0, s, op, vin,
-                                       v.binv.get(), vout, ps.get() + l, partials.get());
+                    grp ? go(pow_stencil_kernel<ND, true, float>, op) : go(pow_stencil_kernel<ND, false, float>, op);
                 }
             }
             SFEM_LAUNCH_CHECK();
@@ -2137,7 +2290,6 @@ struct MgRun {
     MgState* rz_st = nullptr;
     const double* rz_r = nullptr;
     double* rz_partials = nullptr;
-
     void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2,
               bool last = false) {
         MgLevel& L = H.L[(size_t)l];
@@ -2192,19 +2344,18 @@ struct MgRun {
             if (mode == 0) nt ? go(cheb_sell_kernel<BS, 0, true, float>) : go(cheb_sell_kernel<BS, 0, false, float>);
             else nt ? go(cheb_sell_kernel<BS, 1, true, float>) : go(cheb_sell_kernel<BS, 1, false, float>);
         } else {
+            const bool grp = L.node_major;
             auto go = [&](auto k, auto op) {
-                hipLaunchKernelGGL(k, dim3(rows_grid(L.nn * Lanes<ND>::G)), dim3(kThreads), 0, s, op, v, L.r.get(),
-                                   L.x.get(), d_in, d_out, c1, c2, (const float*)L.binv.get(), done);
+                hipLaunchKernelGGL(k, dim3(rows_grid(L.nn * (grp ? Lanes<ND>::G : 1))), dim3(kThreads), 0, s, op, v,
+                                   L.r.get(), L.x.get(), d_in, d_out, c1, c2, (const float*)L.binv.get(), done);
             };
-            if (L.st16.size()) {
-                const StOp<_Float16> op{L.st16.get(), grid_of(L), L.nn};
-                if (mode == 0) go(cheb_stencil_kernel<ND, 0, _Float16>, op);
-                else go(cheb_stencil_kernel<ND, 1, _Float16>, op);
-            } else {
-                const StOp<float> op{L.st32.get(), grid_of(L), L.nn};
-                if (mode == 0) go(cheb_stencil_kernel<ND, 0, float>, op);
-                else go(cheb_stencil_kernel<ND, 1, float>, op);
-            }
+            auto pick = [&](auto op, auto tv) {
+                using TS = decltype(tv);
+                if (grp) mode == 0 ? go(cheb_stencil_kernel<ND, 0, true, TS>, op) : go(cheb_stencil_kernel<ND, 1, true, TS>, op);
+                else mode == 0 ? go(cheb_stencil_kernel<ND, 0, false, TS>, op) : go(cheb_stencil_kernel<ND, 1, false, TS>, op);
+            };
+            if (L.st16.size()) pick(StOp<_Float16>{L.st16.get(), grid_of(L), L.nn}, _Float16(0));
+            else pick(StOp<float>{L.st32.get(), grid_of(L), L.nn}, 0.0f);
         }
         SFEM_LAUNCH_CHECK();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next++ + 1], s));

@@ -446,3 +446,39 @@ def test_mg_matrix_free_mfma_matches_valu(ctx, dims, monkeypatch):
     K.close()
     print(f"[mg] iterations valu {its['0']} mfma {its['1']}", flush=True)
     assert abs(its["1"] - its["0"]) <= 1
+
+
+@pytest.mark.parametrize("dims", [(40, 20, 14), (96, 50), (23, 9, 7)])
+def test_mg_coarse_kernel_forms_same_bits(ctx, dims, monkeypatch):
+    """The coarse levels' two kernel forms -- one thread per node on a
+    slot-major stencil, a lane group per node on a node-major one -- sum each
+    stencil row by the same tree, so solutions and iteration counts are
+    bitwise the same whichever form runs on which level; and they meet the
+    oracle."""
+    print(f"[case] mg coarse forms {dims}", flush=True)
+    m = structured_mesh(dims)
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=23)
+    fixed = clamp_x0(m)
+    f = random_face_load(m, 0, 1, fixed, seed=23)
+    q_d = np.zeros(m.n_dof)
+    K = device_K(ctx, m, young, detect_grid=True)
+    forms = {"threads": "0", "groups": "1000000000", "default": None}
+    sols, its = {}, {}
+    for name, below in forms.items():
+        if below is None:
+            monkeypatch.delenv("SFEM_MG_GROUP_BELOW", raising=False)
+        else:
+            monkeypatch.setenv("SFEM_MG_GROUP_BELOW", below)
+        bc = BoundaryConditions(ctx, m.n_dof, f, fixed, q_d)
+        S = System(K, bc)
+        bc.close()
+        S.set_grid(dims)
+        st = S.solve("fp64", rtol=1e-11, preconditioner="multigrid")
+        sols[name], its[name] = S.solution(), st["iterations"]
+        S.close()
+    K.close()
+    print(f"[mg] coarse forms iterations {its}", flush=True)
+    for name in forms:
+        assert its[name] == its["threads"] and np.array_equal(sols[name], sols["threads"]), name
+    if m.n_dof <= 6000:
+        assert relerr(sols["default"], oracle_direct(oracle_K(m, young), fixed, f)) <= ERR
