@@ -85,6 +85,112 @@ __global__ __launch_bounds__(256) void validate_kernel(const int32_t* __restrict
     }
 }
 
+// ------------------------------------------------ element arithmetic --
+// The steps of one element's quadrature.
+
+// Jacobian at Gauss point g from the element's node coordinates (Jac =
+// coords * dN, :192), its inverse (zero when det = 0, the caller reports the
+// SingularException, :195) and the weight |det| (weights are 1.0, :194).
+template <int NDIM, int NEN>
+__device__ __forceinline__ double elem_jacobian(const double (*x)[NDIM], int g, double* inv, double& det) {
+    double xi, eta, zeta;
+    gauss_point(g, xi, eta, zeta);
+    double J[NDIM][NDIM];
+#pragma unroll
+    for (int i = 0; i < NDIM; ++i)
+#pragma unroll
+        for (int j = 0; j < NDIM; ++j) J[i][j] = 0.0;
+    for (int a = 0; a < NEN; ++a) {
+        double dn[NDIM];
+        dshape<NDIM, NEN>(a, xi, eta, zeta, dn);
+#pragma unroll
+        for (int i = 0; i < NDIM; ++i)
+#pragma unroll
+            for (int j = 0; j < NDIM; ++j) J[i][j] = J[i][j] + x[a][i] * dn[j];
+    }
+    if constexpr (NDIM == 2) {
+        det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
+        const double r = 1.0 / det;
+        inv[0] = J[1][1] * r;  inv[1] = -J[0][1] * r;
+        inv[2] = -J[1][0] * r; inv[3] = J[0][0] * r;
+    } else {
+        const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
+        const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
+        const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
+        det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
+        const double r = 1.0 / det;
+        inv[0] = c00 * r;
+        inv[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
+        inv[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
+        inv[3] = c01 * r;
+        inv[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
+        inv[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
+        inv[6] = c02 * r;
+        inv[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
+        inv[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
+    }
+    if (det == 0.0)
+#pragma unroll
+        for (int i = 0; i < NDIM * NDIM; ++i) inv[i] = 0.0;
+    return 1.0 * fabs(det);
+}
+
+// dN_a/dX at Gauss point g (dNdX = dN * invJ, :196)
+template <int NDIM, int NEN>
+__device__ __forceinline__ void elem_dndx(int a, int g, const double* inv, double* out) {
+    double xi, eta, zeta;
+    gauss_point(g, xi, eta, zeta);
+    double dn[NDIM];
+    dshape<NDIM, NEN>(a, xi, eta, zeta, dn);
+#pragma unroll
+    for (int k = 0; k < NDIM; ++k) {
+        double s = 0.0;
+#pragma unroll
+        for (int j = 0; j < NDIM; ++j) s = s + dn[j] * inv[j * NDIM + k];
+        out[k] = s;
+    }
+}
+
+// the isotropic elasticity constants: plane stress (:217), 3D (:230)
+template <int NDIM>
+__device__ __forceinline__ void elem_d(double E, double nu, double& D11, double& D12, double& D44) {
+    if constexpr (NDIM == 2) {
+        D11 = E / (1 - nu * nu);
+        D12 = nu * E / (1 - nu * nu);
+        D44 = E / (2 * (1 + nu));
+    } else {
+        const double c = E / ((1 + nu) * (1 - 2 * nu));
+        D11 = (1 - nu) * c;
+        D12 = nu * c;
+        D44 = (1 - 2 * nu) / 2 * c;
+    }
+}
+
+// one Gauss point's w B_a' D B_b added to the NDIM x NDIM block (a, b), A =
+// dN_a/dX, B = dN_b/dX, written out per entry (block (a,b) entry (i,j) and
+// block (b,a) entry (j,i) are the same rounded expression)
+template <int NDIM>
+__device__ __forceinline__ void elem_block_gp(const double* A, const double* B, double w, double D11, double D12,
+                                              double D44, double (&acc)[NDIM][NDIM]) {
+#pragma unroll
+    for (int i = 0; i < NDIM; ++i)
+#pragma unroll
+        for (int j = 0; j < NDIM; ++j) {
+            double ent;
+            if (i == j) {
+                double o = 0.0;
+                bool first = true;
+#pragma unroll
+                for (int k = 0; k < NDIM; ++k)
+                    if (k != i) { o = first ? A[k] * B[k] : o + A[k] * B[k]; first = false; }
+                ent = D11 * (A[i] * B[i]) + D44 * o;
+            } else {
+                ent = D12 * (A[i] * B[j]) + D44 * (A[j] * B[i]);
+            }
+            acc[i][j] = acc[i][j] + ent * w;
+        }
+}
+
 // ------------------------------------------------------- element kernel --
 // A block holds EPB elements and one thread per (element, node a, node b).
 //   phase 0: gather the element coordinates into LDS
@@ -131,49 +237,11 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
 
     for (int t = tid; t < EPB * NGP; t += blockDim.x) {
         const int el = t / NGP, g = t % NGP;
-        double xi, eta, zeta;
-        gauss_point(g, xi, eta, zeta);
-        double J[NDIM][NDIM];
-#pragma unroll
-        for (int i = 0; i < NDIM; ++i)
-#pragma unroll
-            for (int j = 0; j < NDIM; ++j) J[i][j] = 0.0;
-        for (int a = 0; a < NEN; ++a) {  // Jac = coords * ΔN, :192
-            double dn[NDIM];
-            dshape<NDIM, NEN>(a, xi, eta, zeta, dn);
-#pragma unroll
-            for (int i = 0; i < NDIM; ++i)
-#pragma unroll
-                for (int j = 0; j < NDIM; ++j) J[i][j] = J[i][j] + sx[el][a][i] * dn[j];
-        }
         double det, inv[NDIM * NDIM];
-        if constexpr (NDIM == 2) {
-            det = J[0][0] * J[1][1] - J[0][1] * J[1][0];
-            const double r = 1.0 / det;
-            inv[0] = J[1][1] * r;  inv[1] = -J[0][1] * r;
-            inv[2] = -J[1][0] * r; inv[3] = J[0][0] * r;
-        } else {
-            const double c00 = J[1][1] * J[2][2] - J[1][2] * J[2][1];
-            const double c01 = J[1][2] * J[2][0] - J[1][0] * J[2][2];
-            const double c02 = J[1][0] * J[2][1] - J[1][1] * J[2][0];
-            det = J[0][0] * c00 + J[0][1] * c01 + J[0][2] * c02;
-            const double r = 1.0 / det;
-            inv[0] = c00 * r;
-            inv[1] = (J[0][2] * J[2][1] - J[0][1] * J[2][2]) * r;
-            inv[2] = (J[0][1] * J[1][2] - J[0][2] * J[1][1]) * r;
-            inv[3] = c01 * r;
-            inv[4] = (J[0][0] * J[2][2] - J[0][2] * J[2][0]) * r;
-            inv[5] = (J[0][2] * J[1][0] - J[0][0] * J[1][2]) * r;
-            inv[6] = c02 * r;
-            inv[7] = (J[0][1] * J[2][0] - J[0][0] * J[2][1]) * r;
-            inv[8] = (J[0][0] * J[1][1] - J[0][1] * J[1][0]) * r;
-        }
-        if (det == 0.0) {  // inv(Jac) throws SingularException, :195
-            if (svalid[el]) atomicMin(&out->err_key, (unsigned long long)(e0 + el) * 8 + (g == 0 ? 2 : 5));
-#pragma unroll
-            for (int i = 0; i < NDIM * NDIM; ++i) inv[i] = 0.0;
-        }
-        sw[el][g] = 1.0 * fabs(det);  // w = wpoints[gp]*abs(det(Jac)), weights are 1.0, :194
+        const double w = elem_jacobian<NDIM, NEN>(sx[el], g, inv, det);
+        if (det == 0.0 && svalid[el])  // inv(Jac) throws SingularException, :195
+            atomicMin(&out->err_key, (unsigned long long)(e0 + el) * 8 + (g == 0 ? 2 : 5));
+        sw[el][g] = w;
 #pragma unroll
         for (int i = 0; i < NDIM * NDIM; ++i) sinv[el][g][i] = inv[i];
     }
@@ -181,17 +249,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
 
     for (int t = tid; t < EPB * NGP * NEN; t += blockDim.x) {  // dNdX = ΔN * invJ, :196
         const int el = t / (NGP * NEN), g = (t / NEN) % NGP, a = t % NEN;
-        double xi, eta, zeta;
-        gauss_point(g, xi, eta, zeta);
-        double dn[NDIM];
-        dshape<NDIM, NEN>(a, xi, eta, zeta, dn);
-#pragma unroll
-        for (int k = 0; k < NDIM; ++k) {
-            double s = 0.0;
-#pragma unroll
-            for (int j = 0; j < NDIM; ++j) s = s + dn[j] * sinv[el][g][j * NDIM + k];
-            sd[el][g][a][k] = s;
-        }
+        elem_dndx<NDIM, NEN>(a, g, sinv[el][g], sd[el][g][a]);
     }
     __syncthreads();
 
@@ -207,16 +265,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
         if constexpr (NDOF > 1) {
             const double E = (e < n_elem) ? young[n_young == 1 ? 0 : e] : 0.0;
             const double nu = (nu_e && e < n_elem) ? nu_e[e] : nu_scalar;
-            if constexpr (NDIM == 2) {  // plane stress, :217
-                D11 = E / (1 - nu * nu);
-                D12 = nu * E / (1 - nu * nu);
-                D44 = E / (2 * (1 + nu));
-            } else {  // 3D isotropic, :230
-                const double c = E / ((1 + nu) * (1 - 2 * nu));
-                D11 = (1 - nu) * c;
-                D12 = nu * c;
-                D44 = (1 - 2 * nu) / 2 * c;
-            }
+            elem_d<NDIM>(E, nu, D11, D12, D44);
         }
         for (int g = 0; g < NGP; ++g) {
             const double w = sw[el][g];
@@ -229,23 +278,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
                 for (int k = 1; k < NDIM; ++k) s = s + A[k] * B[k];
                 acc[0][0] = acc[0][0] + s * w;
             } else {
-#pragma unroll
-                for (int i = 0; i < NDIM; ++i)
-#pragma unroll
-                    for (int j = 0; j < NDIM; ++j) {
-                        double ent;
-                        if (i == j) {
-                            double o = 0.0;
-                            bool first = true;
-#pragma unroll
-                            for (int k = 0; k < NDIM; ++k)
-                                if (k != i) { o = first ? A[k] * B[k] : o + A[k] * B[k]; first = false; }
-                            ent = D11 * (A[i] * B[i]) + D44 * o;
-                        } else {
-                            ent = D12 * (A[i] * B[j]) + D44 * (A[j] * B[i]);
-                        }
-                        acc[i][j] = acc[i][j] + ent * w;
-                    }
+                elem_block_gp<NDIM>(A, B, w, D11, D12, D44, acc);
             }
         }
 #pragma unroll
