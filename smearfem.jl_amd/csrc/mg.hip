@@ -162,17 +162,13 @@ __host__ __device__ __forceinline__ bool child(int64_t c, int k, int64_t e, int6
 // The fine stencil from the element scales (matrix-free finest level): slot
 // s of node I is the sum, over the cells I shares with its neighbour, of
 // s_e Ke(0)'s block (I's corner, the neighbour's corner); fixed rows and
-// columns zero, as stencil_from_k.  One thread per (slot, node), nodes
-// fastest; es: the scales with a ghost layer of zero cells.
+// columns zero, as stencil_from_k.  es: the scales with a ghost layer of
+// zero cells.
 template <int ND>
-__global__ void stencil_from_mf_kernel(const double* __restrict__ kc, const double* __restrict__ es,
-                                       const int32_t* __restrict__ free_of, Grid g, double* __restrict__ st) {
-    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, NS = Sten<ND>::NS;
-    const int64_t nn = g.n0 * g.n1 * g.n2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nn * NS) return;
-    const int s = (int)(t / nn);
-    const int64_t node = t % nn;
+__device__ __forceinline__ void mf_stencil_block(const double* __restrict__ kc, const double* __restrict__ es,
+                                                 const int32_t* __restrict__ free_of, const Grid& g, int s,
+                                                 int64_t node, double (&v)[ND * ND]) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
     int64_t c[3];
     coords_of<ND>(node, g, c);
     int d[3];
@@ -196,15 +192,40 @@ __global__ void stencil_from_mf_kernel(const double* __restrict__ kc, const doub
             for (int m = 0; m < B2; ++m) acc[m] = fma(se, kc[(a * NEN + b) * B2 + m], acc[m]);
         }
     }
-    const int64_t nb = node + d[0] + g.n0 * (d[1] + g.n1 * d[2]);
+    const int64_t nb = in ? node + d[0] + g.n0 * (d[1] + g.n1 * d[2]) : node;
 #pragma unroll
     for (int i = 0; i < BS; ++i)
 #pragma unroll
         for (int j = 0; j < BS; ++j) {
             const bool masked = !in || free_of[node * BS + i] < 0 || free_of[nb * BS + j] < 0;
-            st[((int64_t)s * B2 + i * BS + j) * nn + node] = masked ? 0.0 : acc[i * BS + j];
+            v[i * BS + j] = masked ? 0.0 : acc[i * BS + j];
         }
 }
+
+// The input of a Galerkin line product: a stored slot-major stencil, or the
+// matrix-free finest level's blocks (mf_stencil_block), formed where they are
+// read instead of stored.
+template <int ND>
+struct StencilMem {
+    const double* st;
+    int64_t nn;
+    __device__ __forceinline__ void load(int s, int64_t node, double (&v)[ND * ND]) const {
+        const double* src = st + (int64_t)s * ND * ND * nn + node;
+#pragma unroll
+        for (int m = 0; m < ND * ND; ++m) v[m] = src[(int64_t)m * nn];
+    }
+};
+
+template <int ND>
+struct StencilMf {
+    const double* kc;
+    const double* es;
+    const int32_t* free_of;
+    Grid g;
+    __device__ __forceinline__ void load(int s, int64_t node, double (&v)[ND * ND]) const {
+        mf_stencil_block<ND>(kc, es, free_of, g, s, node, v);
+    }
+};
 
 // Level-0 maps: a node with any free dof is a block row (rank in node order,
 // as the elimination numbers them).
@@ -295,11 +316,11 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
 // outputs read the same <= 9 input blocks (3 fine children x 3 offsets along
 // d), each once.  Each output sums its terms in ascending order of the child
 // index, then of the neighbour's fine index.
-template <int ND>
-__global__ void galerkin_line_kernel(const double* __restrict__ in, Grid gi, int d, Grid go,
-                                     double* __restrict__ out) {
+template <int ND, typename SRC>
+__global__ __launch_bounds__(kThreads) void galerkin_line_kernel(SRC in, Grid gi, int d, Grid go,
+                                                              double* __restrict__ out) {
     constexpr int BS = ND, B2 = BS * BS, NL = Sten<ND>::NS / 3;
-    const int64_t nno = go.n0 * go.n1 * go.n2, nni = gi.n0 * gi.n1 * gi.n2;
+    const int64_t nno = go.n0 * go.n1 * go.n2;
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nno * NL) return;
     const int l = (int)(t / nno);
@@ -339,8 +360,8 @@ __global__ void galerkin_line_kernel(const double* __restrict__ in, Grid gi, int
                 ic[d] = ia;
                 int od[3] = {D[0], D[1], D[2]};
                 od[d] = o;
-                const double* src = in + (int64_t)slot_of<ND>(od[0], od[1], od[2]) * B2 * nni +
-                                    node_at(gi, ic[0], ic[1], ic[2]);
+                const int src_slot = slot_of<ND>(od[0], od[1], od[2]);
+                const int64_t src_node = node_at(gi, ic[0], ic[1], ic[2]);
                 double v[B2];
                 bool loaded = false;
 #pragma unroll
@@ -352,8 +373,7 @@ __global__ void galerkin_line_kernel(const double* __restrict__ in, Grid gi, int
                         double wb;
                         if (!child(c[d] + k - 1, kb, e, jj, wb) || jj != jb) continue;
                         if (!loaded) {
-#pragma unroll
-                            for (int m = 0; m < B2; ++m) v[m] = src[(int64_t)m * nni];
+                            in.load(src_slot, src_node, v);
                             loaded = true;
                         }
                         const double w = wa * wb;
@@ -2008,13 +2028,11 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
 
     // fine stencil (FP64, fixed rows and columns zero): from the element
     // scales when the finest level is matrix-free (K on this grid couples
-    // grid neighbours only), else from K
-    DevBuf<double> stf(ctx, (size_t)nn0 * NS * B2);
-    if (H.mf) {
-        hipLaunchKernelGGL(stencil_from_mf_kernel<ND>, dim3(rows_grid(nn0 * NS)), dim3(kThreads), 0, s,
-                           (const double*)H.kc64.get(), (const double*)H.es64.get(), S->free_of.get(), g0, stf.get());
-        SFEM_LAUNCH_CHECK();
-    } else {
+    // grid neighbours only) -- formed inside the first Galerkin product, not
+    // stored -- else from K
+    DevBuf<double> stf;
+    if (!H.mf) {
+        stf.reset(ctx, (size_t)nn0 * NS * B2);
         memset_d(ctx, stf.get(), 0, stf.bytes());
         DevBuf<int> bad(ctx, 1);
         memset_d(ctx, bad.get(), 0, sizeof(int));
@@ -2028,7 +2046,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     }
 
     // coarse levels
-    const double* prev_st = stf.get();
+    const double* prev_st = H.mf ? nullptr : stf.get();
     const uint8_t* prev_fix = fixnode.get();
     for (;;) {
         const MgLevel& P = H.L.back();
@@ -2068,8 +2086,14 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             *nd = *nd / 2 + 1;  // e = n - 1 elements -> ceil(e / 2)
             const int64_t nno = gn.n0 * gn.n1 * gn.n2;
             DevBuf<double> o(ctx, (size_t)nno * NS * B2);
-            hipLaunchKernelGGL(galerkin_line_kernel<ND>, dim3(rows_grid(nno * (NS / 3))), dim3(kThreads), 0, s, cur,
-                               gcur, d, gn, o.get());
+            const dim3 gl(rows_grid(nno * (NS / 3)));
+            if (!cur)  // the matrix-free finest level
+                hipLaunchKernelGGL((galerkin_line_kernel<ND, StencilMf<ND>>), gl, dim3(kThreads), 0, s,
+                                   StencilMf<ND>{H.kc64.get(), H.es64.get(), S->free_of.get(), gcur}, gcur, d, gn,
+                                   o.get());
+            else
+                hipLaunchKernelGGL((galerkin_line_kernel<ND, StencilMem<ND>>), gl, dim3(kThreads), 0, s,
+                                   StencilMem<ND>{cur, gcur.n0 * gcur.n1 * gcur.n2}, gcur, d, gn, o.get());
             SFEM_LAUNCH_CHECK();
             if (d == last) {
                 raw = std::move(o);
