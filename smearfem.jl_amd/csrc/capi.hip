@@ -102,8 +102,19 @@ void ctx_release(sfem_ctx* ctx) {
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     if (ctx->bar_ws) ctx->alloc->free(ctx->bar_ws);
     delete ctx->alloc;
+    for (hipStream_t& t : ctx->side)
+        if (t) {
+            (void)hipStreamSynchronize(t);
+            (void)hipStreamDestroy(t);
+        }
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
     delete ctx;
+}
+
+hipStream_t side_stream(sfem_ctx* ctx, int i) {
+    hipStream_t& t = ctx->side[i % sfem_ctx::kSideStreams];
+    if (!t) SFEM_HIP(hipStreamCreateWithFlags(&t, hipStreamNonBlocking));
+    return t;
 }
 
 namespace {

@@ -2239,40 +2239,50 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
     memset_d(ctx, ps.get(), 0, sizeof(PowState) * nl);
     const Launch ls = plan_sell(S->nslices);
     auto init_grid = [&](int l) { return std::min<unsigned>(rows_grid(H.L[(size_t)l].nrows * BS), 1024); };
-    unsigned maxgrid = ls.grid;
+    // the levels are independent: level 0 on the context's stream, the
+    // others on side streams, each with its own partial sums
+    std::vector<size_t> part_off((size_t)nl + 1, 0);
     for (int l = 0; l < nl; ++l)
-        maxgrid = std::max({maxgrid, rows_grid(H.L[(size_t)l].nn), init_grid(l)});
-    DevBuf<double> partials(ctx, (size_t)maxgrid);
+        part_off[(size_t)l + 1] =
+            part_off[(size_t)l] + std::max({ls.grid, rows_grid(H.L[(size_t)l].nn), init_grid(l), 1024u});
+    DevBuf<double> partials_all(ctx, part_off[(size_t)nl]);
+    hipEvent_t forked;
+    SFEM_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
+    SFEM_HIP(hipEventRecord(forked, ctx->stream));
+    std::vector<hipEvent_t> joined;
     for (int l = 0; l + 1 < nl; ++l) {
         MgLevel& v = H.L[l];
         const int64_t n = v.nrows * BS;
         const unsigned gi = init_grid(l);
+        const hipStream_t s = l == 0 ? ctx->stream : side_stream(ctx, l - 1);
+        if (l > 0) SFEM_HIP(hipStreamWaitEvent(s, forked, 0));
+        double* partials = partials_all.get() + part_off[(size_t)l];
         hipLaunchKernelGGL(pow_init_kernel, dim3(gi), dim3(kThreads), 0, s, n, v.fixed.get(), v.da.get(),
-                           ps.get() + l, partials.get());
+                           ps.get() + l, partials);
         SFEM_LAUNCH_CHECK();
         float* vin = v.da.get();
         float* vout = v.db.get();
         for (int k = 0; k < o.power_iters; ++k) {
             if (l == 0 && H.mf) {
                 hipLaunchKernelGGL(pow_mf_kernel<ND>, dim3(std::min(rows_grid(v.nrows), 1024u)), dim3(kThreads), 0,
-                                   s, mf_op32(S, H), vin, S->binv32.get(), vout, ps.get(), partials.get());
+                                   s, mf_op32(S, H), vin, S->binv32.get(), vout, ps.get(), partials);
             } else if (l == 0 && H.fp16) {
                 const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices,
                                           S->nfb, ls.per_block};
                 hipLaunchKernelGGL((pow_sell_kernel<BS, _Float16>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
-                                   (const float*)H.binv0.get(), vout, ps.get(), partials.get());
+                                   (const float*)H.binv0.get(), vout, ps.get(), partials);
             } else if (l == 0) {
                 const SellOp<float> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices,
                                        S->nfb, ls.per_block};
                 hipLaunchKernelGGL((pow_sell_kernel<BS, float>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
-                                   S->binv32.get(), vout, ps.get(), partials.get());
+                                   S->binv32.get(), vout, ps.get(), partials);
             } else {
                 // grid-stride over the nodes: few blocks meet at the last-block count
                 const bool grp = v.node_major;
                 const dim3 gp(std::min(rows_grid(v.nn * (grp ? Lanes<ND>::G : 1)), 1024u));
                 auto go = [&](auto k, auto op) {
                     hipLaunchKernelGGL(k, gp, dim3(kThreads), 0, s, op, vin, v.binv.get(), vout, ps.get() + l,
-                                       partials.get());
+                                       partials);
                 };
                 if (v.st16.size()) {
                     const StOp<_Float16> op{v.st16.get(), grid_of(v), v.nn};
@@ -2285,9 +2295,18 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             SFEM_LAUNCH_CHECK();
             std::swap(vin, vout);
         }
+        if (l > 0) {
+            hipEvent_t done_l;
+            SFEM_HIP(hipEventCreateWithFlags(&done_l, hipEventDisableTiming));
+            joined.push_back(done_l);
+            SFEM_HIP(hipEventRecord(done_l, s));
+            SFEM_HIP(hipStreamWaitEvent(ctx->stream, done_l, 0));
+        }
     }
     std::vector<PowState> hps((size_t)nl);
-    d2h(ctx, hps.data(), ps.get(), sizeof(PowState) * nl);
+    d2h(ctx, hps.data(), ps.get(), sizeof(PowState) * nl);  // the side streams joined: every level done
+    (void)hipEventDestroy(forked);
+    for (hipEvent_t e : joined) (void)hipEventDestroy(e);
     for (int l = 0; l + 1 < nl; ++l) {
         H.L[l].lmax = hps[(size_t)l].lam;
         if (!(H.L[l].lmax > 0.0) || !std::isfinite(H.L[l].lmax))

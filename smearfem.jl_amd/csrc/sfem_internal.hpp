@@ -87,12 +87,18 @@ struct sfem_ctx {
     // Device address of the no-allocation solve's workspace array on this
     // context's device (hipGetSymbolAddress at creation).
     void* scratch = nullptr;
+    // Streams for independent work beside `stream` (default priority, made
+    // on first use; joined back to `stream` by events before its results
+    // are read or its buffers released).
+    static constexpr int kSideStreams = 4;
+    hipStream_t side[kSideStreams] = {};
 };
 
 namespace sfem {
 
 void ctx_retain(sfem_ctx* ctx);
 void ctx_release(sfem_ctx* ctx);
+hipStream_t side_stream(sfem_ctx* ctx, int i);
 
 // RAII device buffer backed by the context allocator.
 template <typename T>
