@@ -157,21 +157,24 @@ preconditioner_code(p::Symbol) = p === :jacobi ? 0 : p === :block_jacobi ? 1 : p
                                  throw(ArgumentError("unknown preconditioner $p"))
 
 # the one-shot solve of sfem_solve, split so that the multigrid hierarchy of a
-# structured grid (dims elements per direction) can be built in between
+# structured grid (dims elements per direction; nothing: the grid sfem_assemble
+# found K's mesh to be) can be built in between
 function solve_with_grid(ctx, Kp, fv, mask, qd, opts, dims)
     n = length(fv)
     bc, sys = Ref{Ptr{Cvoid}}(C_NULL), Ref{Ptr{Cvoid}}(C_NULL)
     stats = SolveStats(0, 0, 0, 0, 0.0, 0.0, 0.0, 0.0, 0, 0, 0, 0, 0.0, 0.0, 0, 0.0)
     q = zeros(n)
-    d = Int64[dims...]
+    d = dims === nothing ? Int64[] : Int64[dims...]
     try
         check(ctx, ccall((:sfem_bc_create, libsfem), Cint,
                          (Ptr{Cvoid}, Int64, Ptr{Float64}, Ptr{UInt8}, Ptr{Float64}, Ref{Ptr{Cvoid}}),
                          ctx, n, fv, mask, qd, bc))
         check(ctx, ccall((:sfem_eliminate, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Cvoid}, Ref{Ptr{Cvoid}}),
                          ctx, Kp, bc[], sys))
-        check(ctx, ccall((:sfem_system_set_grid, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}),
-                         ctx, sys[], length(d), d, C_NULL))
+        GC.@preserve d check(ctx, ccall((:sfem_system_set_grid, libsfem), Cint,
+                                        (Ptr{Cvoid}, Ptr{Cvoid}, Cint, Ptr{Int64}, Ptr{Cvoid}),
+                                        ctx, sys[], length(d), isempty(d) ? Ptr{Int64}(C_NULL) : pointer(d),
+                                        C_NULL))
         check(ctx, ccall((:sfem_system_solve, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ref{SolveOpts}, Ref{SolveStats}),
                          ctx, sys[], opts, stats))
         check(ctx, ccall((:sfem_system_solution, libsfem), Cint, (Ptr{Cvoid}, Ptr{Cvoid}, Ptr{Float64}, Ptr{Float64}),
@@ -191,9 +194,10 @@ end
 assemble_system (src/fem.jl:135) and the example's elimination and solve
 (examples/vector3D.jl:314-322) in one call; K stays on the device, so the
 node-block operator formats (split precision, block-Jacobi) are available when
-ID is node-interleaved.  `preconditioner=:multigrid` with `grid=(nx, ny, nz)`
-(elements per direction of the meshgrid numbering the nodes follow) uses the
-geometric multigrid preconditioner (precision :fp64 or :split).
+ID is node-interleaved.  `preconditioner=:multigrid` uses the geometric
+multigrid preconditioner (precision :fp64 or :split) on the grid of the
+meshgrid numbering the nodes follow: the one sfem_assemble recognised
+(grid=nothing), or `grid=(nx, ny, nz)` elements per direction.
 """
 function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Young, ν, f, fixed, q_d;
                             precision=:split, preconditioner=:block_jacobi, rtol=1e-10, max_iter=200000,
@@ -221,7 +225,6 @@ function assemble_and_solve(ne, NodeList, IEN, ndim, FunctionClass, nDof, ID, Yo
     opts = SolveOpts(precision_code(precision), 0, rtol, max_iter, 0.1, 64, 0, 1,
                      preconditioner_code(preconditioner))
     if preconditioner === :multigrid
-        grid === nothing && throw(ArgumentError("multigrid needs grid = elements per direction"))
         try
             return solve_with_grid(ctx, Kp[], fv, mask, qd, opts, grid)
         finally
