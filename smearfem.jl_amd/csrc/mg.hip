@@ -319,12 +319,13 @@ __global__ void coarse_fixed_kernel(const uint8_t* __restrict__ ffix, Grid gf, G
 template <int ND, typename SRC>
 __global__ __launch_bounds__(kThreads) void galerkin_line_kernel(SRC in, Grid gi, int d, Grid go,
                                                               double* __restrict__ out) {
-    constexpr int BS = ND, B2 = BS * BS, NL = Sten<ND>::NS / 3;
+    constexpr int BS = ND, B2 = BS * BS;
     const int64_t nno = go.n0 * go.n1 * go.n2;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nno * NL) return;
-    const int l = (int)(t / nno);
-    const int64_t I = t - (int64_t)l * nno;
+    // the line (the offsets off direction d) is the block's y index: uniform,
+    // so the slots and Ke(0) blocks it addresses are scalar loads
+    const int l = (int)blockIdx.y;
+    const int64_t I = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (I >= nno) return;
     int64_t c[3];
     coords_of<ND>(I, go, c);
     int D[3] = {0, 0, 0};
@@ -2086,7 +2087,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             *nd = *nd / 2 + 1;  // e = n - 1 elements -> ceil(e / 2)
             const int64_t nno = gn.n0 * gn.n1 * gn.n2;
             DevBuf<double> o(ctx, (size_t)nno * NS * B2);
-            const dim3 gl(rows_grid(nno * (NS / 3)));
+            const dim3 gl(rows_grid(nno), NS / 3);
             if (!cur)  // the matrix-free finest level
                 hipLaunchKernelGGL((galerkin_line_kernel<ND, StencilMf<ND>>), gl, dim3(kThreads), 0, s,
                                    StencilMf<ND>{H.kc64.get(), H.es64.get(), S->free_of.get(), gcur}, gcur, d, gn,
