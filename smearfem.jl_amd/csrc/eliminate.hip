@@ -250,15 +250,25 @@ __global__ void mask_diff_kernel(const uint8_t* __restrict__ fixed, const int32_
     if (s < n && ((fixed[s] != 0) != (free_of[s] < 0))) atomicOr(diff, 1);
 }
 
+// Free blocks per free column: a group of 32 lanes per column reads the
+// column's row indices together (one contiguous run) and sums the count by a
+// butterfly; every lane of a group runs the same trips.
 __global__ void blk_count_kernel(const int64_t* __restrict__ bcolptr, const int32_t* __restrict__ browidx,
                                  const int32_t* __restrict__ ufree_of, int64_t nunits, int32_t* __restrict__ blen) {
-    const int64_t cu = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (cu >= nunits) return;
-    const int32_t row = ufree_of[cu];
-    if (row < 0) return;
+    constexpr int G = 32;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t cu = t / G;
+    const int lane = (int)(t % G);
+    int32_t row = -1;
     int cnt = 0;
-    for (int64_t u = bcolptr[cu]; u < bcolptr[cu + 1]; ++u) cnt += ufree_of[browidx[u]] >= 0 ? 1 : 0;
-    blen[row] = cnt;
+    if (cu < nunits) {
+        row = ufree_of[cu];
+        if (row >= 0)
+            for (int64_t u = bcolptr[cu] + lane; u < bcolptr[cu + 1]; u += G) cnt += ufree_of[browidx[u]] >= 0 ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = G / 2; off > 0; off >>= 1) cnt += __shfl_xor(cnt, off, G);
+    if (lane == 0 && row >= 0) blen[row] = cnt;
 }
 
 __global__ void slice_width_kernel(const int32_t* __restrict__ blen, int64_t nfb, int64_t nslices,
@@ -509,8 +519,8 @@ void eliminate(sfem_ctx* ctx, const sfem_csr* K, const sfem_bc* bc, sfem_system*
                                ufree_of);
             SFEM_LAUNCH_CHECK();
             DevBuf<int32_t> blen(ctx, nfb);
-            hipLaunchKernelGGL(blk_count_kernel, gu, dim3(256), 0, ctx->stream, K->bcolptr.get(), K->browidx.get(),
-                               (const int32_t*)ufree_of, nunits, blen.get());
+            hipLaunchKernelGGL(blk_count_kernel, dim3(grid_for(nunits * 32, 256)), dim3(256), 0, ctx->stream,
+                               K->bcolptr.get(), K->browidx.get(), (const int32_t*)ufree_of, nunits, blen.get());
             SFEM_LAUNCH_CHECK();
             const int64_t nslices = (nfb + 63) / 64;
             S->nfb = nfb;
