@@ -800,53 +800,69 @@ struct MfOp {
 // row maps are int32) and the vectors 4 GB (offsets are 32-bit byte counts).
 // Every load is unconditional and in bounds.  The neighbours are gathered
 // and applied one plane (z offset) at a time: a third of them in registers.
-template <int ND, typename T, typename VT>
-__device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t row, const VT* __restrict__ v,
-                                       T (&out)[ND]) {
+// NR rows per thread share Ke(0)'s scalar loads and give the FMA chains of
+// independent rows to interleave; each row's arithmetic is the same in any
+// NR (the same bits).  The smoothing kernels use NR = 1: at 2 the FP32 step
+// needs 223 VGPRs (2 waves per SIMD) and ran slower (solve 19.5 against
+// 15.1 ms at 257 x 65 x 65 nodes).
+template <int ND, int NR, typename T, typename VT>
+__device__ __forceinline__ void mf_rows(const MfOp<T>& op, const T* kc, const int64_t (&row)[NR],
+                                        const VT* __restrict__ v, T (&out)[NR][ND]) {
     constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, NP = ND == 3 ? 3 : 1;
     const uint32_t n0 = op.n0, n1 = op.n1;
-    const uint32_t node = (uint32_t)op.node_of_row[row];
-    const uint32_t cx = node % n0, yz = node / n0;
-    const uint32_t cy = ND == 3 ? yz % n1 : yz, cz = ND == 3 ? yz / n1 : 0;
     const int P0 = (int)n0 + 2, P1 = (int)n1 + 2;  // padded nodes per direction
     const int Q0 = (int)n0 + 1, Q1 = (int)n1 + 1;  // padded cells per direction
-    const int pn = (int)(cx + 1) + P0 * ((int)(cy + 1) + (ND == 3 ? P1 * (int)(cz + 1) : 0));
-    const int pc = (int)cx + Q0 * ((int)cy + (ND == 3 ? Q1 * (int)cz : 0));  // cell c - 1, padded
     const char* vb = (const char*)v;
-    T se[NEN];
+    int pn[NR];
+    T se[NR][NEN];
 #pragma unroll
-    for (int q = 0; q < NEN; ++q)  // cell c - 1 + (qx, qy, qz)
-        se[q] = op.es[(uint32_t)(pc + (q & 1) + Q0 * (((q >> 1) & 1) + (ND == 3 ? Q1 * (q >> 2) : 0)))];
-    T t[NEN][BS];
+    for (int r = 0; r < NR; ++r) {
+        const uint32_t node = (uint32_t)op.node_of_row[row[r]];
+        const uint32_t cx = node % n0, yz = node / n0;
+        const uint32_t cy = ND == 3 ? yz % n1 : yz, cz = ND == 3 ? yz / n1 : 0;
+        pn[r] = (int)(cx + 1) + P0 * ((int)(cy + 1) + (ND == 3 ? P1 * (int)(cz + 1) : 0));
+        const int pc = (int)cx + Q0 * ((int)cy + (ND == 3 ? Q1 * (int)cz : 0));  // cell c - 1, padded
 #pragma unroll
-    for (int q = 0; q < NEN; ++q)
+        for (int q = 0; q < NEN; ++q)  // cell c - 1 + (qx, qy, qz)
+            se[r][q] = op.es[(uint32_t)(pc + (q & 1) + Q0 * (((q >> 1) & 1) + (ND == 3 ? Q1 * (q >> 2) : 0)))];
+    }
+    T t[NR][NEN][BS];
 #pragma unroll
-        for (int i = 0; i < BS; ++i) t[q][i] = T(0);
+    for (int r = 0; r < NR; ++r)
+#pragma unroll
+        for (int q = 0; q < NEN; ++q)
+#pragma unroll
+            for (int i = 0; i < BS; ++i) t[r][q][i] = T(0);
     T last = T(0);  // the previous block's last sum
     // FP64: every neighbour's row first (one round trip, not one per plane);
     // FP32 per plane (hoisted, the scalar registers spill)
     constexpr bool HOIST = sizeof(T) == 8;
-    auto row_at = [&](int pz, int s) {
+    auto row_at = [&](int r, int pz, int s) {
         const int dx = s % 3 - 1, dy = s / 3 - 1, dz = ND == 3 ? pz - 1 : 0;
-        const int32_t r = op.rows[(uint32_t)(pn + dx + P0 * (dy + P1 * dz))];
-        return r >= 0 ? (uint32_t)r : (uint32_t)op.nrows;
+        const int32_t rr = op.rows[(uint32_t)(pn[r] + dx + P0 * (dy + P1 * dz))];
+        return rr >= 0 ? (uint32_t)rr : (uint32_t)op.nrows;
     };
-    uint32_t rn[NP][9];
+    uint32_t rn[HOIST ? NR : 1][NP][9];
     if constexpr (HOIST)
 #pragma unroll
-        for (int pz = 0; pz < NP; ++pz)
+        for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int s = 0; s < 9; ++s) rn[pz][s] = row_at(pz, s);
+            for (int pz = 0; pz < NP; ++pz)
+#pragma unroll
+                for (int s = 0; s < 9; ++s) rn[r][pz][s] = row_at(r, pz, s);
 #pragma unroll
     for (int pz = 0; pz < NP; ++pz) {
         const int sz = ND == 3 ? pz : 1;  // z offset + 1
-        T vs[9][BS];
+        T vs[NR][9][BS];
 #pragma unroll
-        for (int s = 0; s < 9; ++s) {
-            const uint32_t rr = HOIST ? rn[pz][s] : row_at(pz, s);
+        for (int r = 0; r < NR; ++r)
 #pragma unroll
-            for (int j = 0; j < BS; ++j) vs[s][j] = (T)*(const VT*)(vb + (rr * BS + j) * (uint32_t)sizeof(VT));
-        }
+            for (int s = 0; s < 9; ++s) {
+                const uint32_t rr = HOIST ? rn[HOIST ? r : 0][pz][s] : row_at(r, pz, s);
+#pragma unroll
+                for (int j = 0; j < BS; ++j)
+                    vs[r][s][j] = (T)*(const VT*)(vb + (rr * BS + j) * (uint32_t)sizeof(VT));
+            }
 #pragma unroll
         for (int s = 0; s < 9; ++s) {
             const int sx = s % 3, sy = s / 3;
@@ -867,21 +883,35 @@ __device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t r
                 const __attribute__((address_space(4))) T* k =
                     (const __attribute__((address_space(4))) T*)kc + zero + (a * NEN + b) * B2;
 #pragma unroll
-                for (int i = 0; i < BS; ++i)
+                for (int r = 0; r < NR; ++r)
 #pragma unroll
-                    for (int j = 0; j < BS; ++j) t[q][i] = fma(k[i * BS + j], vs[s][j], t[q][i]);
-                last = t[q][BS - 1];
+                    for (int i = 0; i < BS; ++i)
+#pragma unroll
+                        for (int j = 0; j < BS; ++j) t[r][q][i] = fma(k[i * BS + j], vs[r][s][j], t[r][q][i]);
+                last = t[NR - 1][q][BS - 1];
             }
         }
     }
 #pragma unroll
-    for (int i = 0; i < BS; ++i) {
-        T acc = T(0);
+    for (int r = 0; r < NR; ++r)
 #pragma unroll
-        for (int q = 0; q < NEN; ++q) acc = fma(se[q], t[q][i], acc);
-        const T own = (T)v[row * BS + i];
-        out[i] = op.fixed[row * BS + i] ? own : acc;
-    }
+        for (int i = 0; i < BS; ++i) {
+            T acc = T(0);
+#pragma unroll
+            for (int q = 0; q < NEN; ++q) acc = fma(se[r][q], t[r][q][i], acc);
+            const T own = (T)v[row[r] * BS + i];
+            out[r][i] = op.fixed[row[r] * BS + i] ? own : acc;
+        }
+}
+
+template <int ND, typename T, typename VT>
+__device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t row, const VT* __restrict__ v,
+                                       T (&out)[ND]) {
+    const int64_t rows[1] = {row};
+    T o[1][ND];
+    mf_rows<ND, 1, T, VT>(op, kc, rows, v, o);
+#pragma unroll
+    for (int i = 0; i < ND; ++i) out[i] = o[0][i];
 }
 
 // ghost-layer copies for the matrix-free operator: node rows (-1 outside)
