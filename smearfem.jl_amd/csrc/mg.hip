@@ -86,6 +86,7 @@ struct MgHierarchy {
     DevBuf<float> kc32, es32;     // es: the scales with a ghost layer
     DevBuf<double> kc64, es64;
     DevBuf<int32_t> mf_rows;      // row_of_node with a ghost layer
+    DevBuf<uint32_t> mf_offs;     // its rows as byte offsets in an FP32 vector
     // the smoothing steps on the matrix cores (3D): Ke(0) 2^-ek as FP16 hi / lo
     // fragments of the 32x32x8 instruction, [part][k step][lane][4]
     bool mfma = false;
@@ -791,6 +792,8 @@ struct MfOp {
     const T* es;                 // cells (e0+2) x (e1+2) [x (e2+2)], meshgrid order
     const int32_t* node_of_row;
     const int32_t* rows;         // nodes (n0+2) x (n1+2) [x (n2+2)]: block row or -1
+    const uint32_t* offs;        // the same nodes: byte offset of the row (the pad row for -1) in
+                                 // an FP32 vector, 3 x 4 B per row (FP64: twice that)
     const uint8_t* fixed;        // per row dof
     uint32_t n0, n1;             // nodes per direction x, y
     int64_t nrows;
@@ -806,8 +809,8 @@ struct MfOp {
 // needs 223 VGPRs (2 waves per SIMD) and ran slower (solve 19.5 against
 // 15.1 ms at 257 x 65 x 65 nodes).
 template <int ND, int NR, typename T, typename VT>
-__device__ __forceinline__ void mf_rows(const MfOp<T>& op, const T* kc, const int64_t (&row)[NR],
-                                        const VT* __restrict__ v, T (&out)[NR][ND]) {
+__device__ __forceinline__ void mf_block_rows(const MfOp<T>& op, const T* kc, const int64_t (&row)[NR],
+                                              const VT* __restrict__ v, T (&out)[NR][ND]) {
     constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, NP = ND == 3 ? 3 : 1;
     const uint32_t n0 = op.n0, n1 = op.n1;
     const int P0 = (int)n0 + 2, P1 = (int)n1 + 2;  // padded nodes per direction
@@ -837,10 +840,17 @@ __device__ __forceinline__ void mf_rows(const MfOp<T>& op, const T* kc, const in
     // FP64: every neighbour's row first (one round trip, not one per plane);
     // FP32 per plane (hoisted, the scalar registers spill)
     constexpr bool HOIST = sizeof(T) == 8;
+    // FP32: a neighbour's byte offset (the offset map); FP64: its row (the
+    // row map; fewer registers across the hoisted loads)
     auto row_at = [&](int r, int pz, int s) {
         const int dx = s % 3 - 1, dy = s / 3 - 1, dz = ND == 3 ? pz - 1 : 0;
-        const int32_t rr = op.rows[(uint32_t)(pn[r] + dx + P0 * (dy + P1 * dz))];
-        return rr >= 0 ? (uint32_t)rr : (uint32_t)op.nrows;
+        const uint32_t p = (uint32_t)(pn[r] + dx + P0 * (dy + P1 * dz));
+        if constexpr (sizeof(VT) == 8) {
+            const int32_t rr = op.rows[p];
+            return rr >= 0 ? (uint32_t)rr : (uint32_t)op.nrows;
+        } else {
+            return op.offs[p];
+        }
     };
     uint32_t rn[HOIST ? NR : 1][NP][9];
     if constexpr (HOIST)
@@ -858,10 +868,13 @@ __device__ __forceinline__ void mf_rows(const MfOp<T>& op, const T* kc, const in
         for (int r = 0; r < NR; ++r)
 #pragma unroll
             for (int s = 0; s < 9; ++s) {
-                const uint32_t rr = HOIST ? rn[HOIST ? r : 0][pz][s] : row_at(r, pz, s);
+                const uint32_t q = HOIST ? rn[HOIST ? r : 0][pz][s] : row_at(r, pz, s);
 #pragma unroll
-                for (int j = 0; j < BS; ++j)
-                    vs[r][s][j] = (T)*(const VT*)(vb + (rr * BS + j) * (uint32_t)sizeof(VT));
+                for (int j = 0; j < BS; ++j) {
+                    const uint32_t o = sizeof(VT) == 8 ? (q * BS + j) * (uint32_t)sizeof(VT)
+                                                       : q + j * (uint32_t)sizeof(VT);
+                    vs[r][s][j] = (T)*(const VT*)(vb + o);
+                }
             }
 #pragma unroll
         for (int s = 0; s < 9; ++s) {
@@ -909,7 +922,7 @@ __device__ __forceinline__ void mf_row(const MfOp<T>& op, const T* kc, int64_t r
                                        T (&out)[ND]) {
     const int64_t rows[1] = {row};
     T o[1][ND];
-    mf_rows<ND, 1, T, VT>(op, kc, rows, v, o);
+    mf_block_rows<ND, 1, T, VT>(op, kc, rows, v, o);
 #pragma unroll
     for (int i = 0; i < ND; ++i) out[i] = o[0][i];
 }
@@ -926,6 +939,16 @@ __global__ void ghost_rows_kernel(const int32_t* __restrict__ in, int64_t n0, in
     const int64_t x = i % p0 - 1, y = (i / p0) % p1 - 1, z = ND == 3 ? i / (p0 * p1) - 1 : 0;
     const bool in_ = x >= 0 && x < n0 && y >= 0 && y < n1 && z >= 0 && z < (ND == 3 ? n2 : 1);
     out[i] = in_ ? in[x + n0 * (y + n1 * z)] : -1;
+}
+
+// the ghost-layer row map as FP32 byte offsets (3 x 4 B per row; -1: the
+// pad row nrows)
+__global__ void ghost_offs_kernel(const int32_t* __restrict__ rows, int64_t n, int64_t nrows, int bs,
+                                  uint32_t* __restrict__ offs) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int64_t r = rows[i] >= 0 ? rows[i] : nrows;
+    offs[i] = (uint32_t)(r * bs * (int64_t)sizeof(float));
 }
 
 template <int ND, typename T>
@@ -1895,8 +1918,8 @@ constexpr double kMfOuterDev = 1e-12;
 
 template <typename T>
 MfOp<T> mf_op(const sfem_system* S, const MgHierarchy& H, const T* kc, const T* es) {
-    return {kc, es, H.node_of_row.get(), H.mf_rows.get(), H.L[0].fixed.get(), (uint32_t)H.L[0].n[0],
-            (uint32_t)H.L[0].n[1], S->nfb};
+    return {kc, es, H.node_of_row.get(), H.mf_rows.get(), H.mf_offs.get(), H.L[0].fixed.get(),
+            (uint32_t)H.L[0].n[0], (uint32_t)H.L[0].n[1], S->nfb};
 }
 
 MfOp<float> mf_op32(const sfem_system* S, const MgHierarchy& H) {
@@ -2016,6 +2039,10 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         H.es64.reset(ctx, (size_t)pc);
         hipLaunchKernelGGL(ghost_rows_kernel<ND>, dim3(rows_grid(pn)), dim3(kThreads), 0, s,
                            (const int32_t*)H.row_of_node.get(), n0, n1, n2, H.mf_rows.get());
+        SFEM_LAUNCH_CHECK();
+        H.mf_offs.reset(ctx, (size_t)pn);
+        hipLaunchKernelGGL(ghost_offs_kernel, dim3(rows_grid(pn)), dim3(kThreads), 0, s,
+                           (const int32_t*)H.mf_rows.get(), pn, S->nfb, BS, H.mf_offs.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL((ghost_cells_kernel<ND, float>), dim3(rows_grid(pc)), dim3(kThreads), 0, s,
                            K->escale.get(), n0 - 1, n1 - 1, n2 - 1, H.es32.get());
