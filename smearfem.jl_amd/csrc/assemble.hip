@@ -210,55 +210,6 @@ __global__ __launch_bounds__(256) void grid_counts_kernel(GridAsm g, int64_t nn,
     cnt[c] = k;
 }
 
-// s_e = Ke(e)[0][0] / Ke(0)[0][0]: the multiple of element 0's matrix that
-// element e's would be on a uniform grid with one Poisson ratio; and 1 / s_e
-__global__ __launch_bounds__(256) void grid_scale_kernel(const double* __restrict__ V, int64_t n_elem, int64_t ke,
-                                                         double* __restrict__ escale, double* __restrict__ inv) {
-    const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= n_elem) return;
-    const double s = V[e * ke] / V[0];
-    escale[e] = s;
-    inv[e] = 1.0 / s;
-}
-
-// the larger of a and b, NaN when either is
-__device__ __forceinline__ double max_nan(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
-
-template <int ND>
-__device__ __forceinline__ double grid_gather_slot(const double* __restrict__ V, const GridAsm& g, int64_t c, int s,
-                                                   const int64_t* __restrict__ bcolptr, double* __restrict__ bval,
-                                                   int32_t* __restrict__ browidx, int32_t* __restrict__ bcolidx,
-                                                   const double* __restrict__ inv_scale);
-
-// One thread per (node column c, stencil slot s); slots in ascending row
-// order (z, then y, then x offset), so the k-th valid slot is block k.
-// Each element block read is also compared with s_e times element 0's:
-// bmax[block] = max |Ke(e) (1 / s_e) - Ke(0)| over the block's reads (NaN
-// if any is).
-template <int ND>
-__global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
-                                                          const int64_t* __restrict__ bcolptr,
-                                                          double* __restrict__ bval, int32_t* __restrict__ browidx,
-                                                          int32_t* __restrict__ bcolidx,
-                                                          const double* __restrict__ inv_scale,
-                                                          double* __restrict__ bmax) {
-    __shared__ double wmax[4];
-    constexpr int NS = ND == 3 ? 27 : 9;
-    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int64_t c = t / NS;
-    const int s = (int)(t % NS);
-    double dmax = c < nn ? grid_gather_slot<ND>(V, g, c, s, bcolptr, bval, browidx, bcolidx, inv_scale) : 0.0;
-    // every lane arrives here: block maximum, one store per block
-    for (int off = 32; off > 0; off >>= 1) dmax = max_nan(dmax, __shfl_xor(dmax, off, 64));
-    if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = dmax;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double b = wmax[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max_nan(b, wmax[w]);
-        bmax[blockIdx.x] = b;
-    }
-}
-
 // one block of 1024: out = max of the n block maxima (NaN propagates)
 __global__ __launch_bounds__(1024) void max_reduce_kernel(const double* __restrict__ in, int64_t n,
                                                           double* __restrict__ out) {
@@ -275,76 +226,98 @@ __global__ __launch_bounds__(1024) void max_reduce_kernel(const double* __restri
     }
 }
 
+// Half a wavefront per node column c (8 columns per block of 256).
+//   phase 1: lane (cell, part) copies its part of the column node's section
+//            in one of the cells around c (the element's NEN blocks whose
+//            column corner is c, NEN * ndof^2 contiguous values) from V to LDS
+//            with 16-byte loads;
+//   phase 2: lane s (stencil slot, ascending row order: z, then y, then x
+//            offset) sums the shared cells' blocks for row node c + offset(s)
+//            in ascending element order and stores K block k, the slot's rank
+//            among the column's valid slots (each direction's valid offsets
+//            as a mixed radix, x fastest).
+// Element matrices are bitwise symmetric, so block (row ar, column c) is read
+// as the transpose of block (c, ar), which lies in c's section.
 template <int ND>
-__device__ __forceinline__ double grid_gather_slot(const double* __restrict__ V, const GridAsm& g, int64_t c, int s,
-                                                   const int64_t* __restrict__ bcolptr, double* __restrict__ bval,
-                                                   int32_t* __restrict__ browidx, int32_t* __restrict__ bcolidx,
-                                                   const double* __restrict__ inv_scale) {
-    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
-    int64_t a[3];
-    grid_coords<ND>(c, g, a);
-    auto valid = [&](int sl, int64_t b[3]) {
-        const int o[3] = {sl % 3 - 1, (sl / 3) % 3 - 1, ND == 3 ? sl / 9 - 1 : 0};
-        bool ok = true;
+__global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
+                                                          const int64_t* __restrict__ bcolptr,
+                                                          double* __restrict__ bval, int32_t* __restrict__ browidx,
+                                                          int32_t* __restrict__ bcolidx) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, SEC = NEN * B2, KE = NEN * SEC;
+    constexpr int NS = ND == 3 ? 27 : 9, CPB = 256 / 32;
+    constexpr int LPC = 32 / NEN, PER = SEC / LPC;  // lanes per cell, values per lane
+    static_assert(PER % 2 == 0 && (SEC * 8) % 16 == 0, "16-byte copies");
+    __shared__ __align__(16) double sv[CPB][NEN * SEC];
+    const int hl = threadIdx.x & 31, w = threadIdx.x >> 5;
+    const int64_t c = (int64_t)blockIdx.x * CPB + w;
+    const bool live = c < nn;
+    int64_t a[3] = {0, 0, 0};
+    if (live) grid_coords<ND>(c, g, a);
+    {
+        const int kc = hl / LPC, part = hl % LPC;
+        bool ex = live;
+        int code = 0;
+        int64_t cell[3];
+#pragma unroll
         for (int d = 0; d < 3; ++d) {
-            b[d] = a[d] + o[d];
-            ok = ok && b[d] >= 0 && b[d] < g.n[d];
+            const int k = d < ND ? (kc >> d) & 1 : 1;
+            cell[d] = a[d] - 1 + k;
+            ex = ex && cell[d] >= 0 && cell[d] < g.e[d];
+            code += (1 - k) << d;
         }
-        return ok;
-    };
+        if (ex) {
+            const int64_t e = cell[0] + g.e[0] * (cell[1] + g.e[1] * cell[2]);
+            const double2* src = reinterpret_cast<const double2*>(V + e * KE + g.corner[code] * SEC + part * PER);
+            double2* dst = reinterpret_cast<double2*>(&sv[w][kc * SEC + part * PER]);
+            double2 t[PER / 2];
+#pragma unroll
+            for (int r = 0; r < PER / 2; ++r) t[r] = src[r];
+#pragma unroll
+            for (int r = 0; r < PER / 2; ++r) dst[r] = t[r];
+        }
+    }
+    __syncthreads();
+    if (!live || hl >= NS) return;
+    const int s = hl;
+    const int o[3] = {s % 3, (s / 3) % 3, ND == 3 ? s / 9 : 1};  // offset + 1 per direction
     int64_t b[3];
-    if (!valid(s, b)) return 0.0;
-    int k = 0;
-    for (int sl = 0; sl < s; ++sl) {
-        int64_t bb[3];
-        k += valid(sl, bb) ? 1 : 0;
+    int k = 0, radix = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b[d] = a[d] + o[d] - 1;
+        if (b[d] < 0 || b[d] >= g.n[d]) return;
+        const int lo = a[d] > 0 ? 0 : 1, hi = a[d] < g.n[d] - 1 ? 2 : 1;
+        k += (o[d] - lo) * radix;
+        radix *= hi - lo + 1;
     }
     const int64_t u = bcolptr[c] + k;
-    const int64_t r = b[0] + g.n[0] * (b[1] + g.n[1] * b[2]);
-    // the shared cells per direction, ascending
-    int64_t cl[3][2];
-    int nc[3];
-    for (int d = 0; d < 3; ++d) {
-        nc[d] = 0;
-        if (d >= ND) {
-            cl[d][nc[d]++] = 0;
-        } else if (b[d] == a[d]) {
-            if (a[d] - 1 >= 0) cl[d][nc[d]++] = a[d] - 1;
-            if (a[d] < g.e[d]) cl[d][nc[d]++] = a[d];
-        } else {
-            cl[d][nc[d]++] = min(a[d], b[d]);
-        }
-    }
     double acc[B2];
-    double dmax = 0.0;
     bool first = true;
-    for (int kz = 0; kz < nc[2]; ++kz)
-        for (int ky = 0; ky < nc[1]; ++ky)
-            for (int kx = 0; kx < nc[0]; ++kx) {
-                const int64_t cx = cl[0][kx], cy = cl[1][ky], cz = cl[2][kz];
-                const int64_t e = cx + g.e[0] * (cy + g.e[1] * cz);
-                const int ar = g.corner[(int)(b[0] - cx) + 2 * (int)(b[1] - cy) + 4 * (int)(b[2] - cz)];
-                const int ac = g.corner[(int)(a[0] - cx) + 2 * (int)(a[1] - cy) + 4 * (int)(a[2] - cz)];
-                // element e's block (row node ar, column node ac), read as the
-                // transpose of block (ac, ar): element matrices are bitwise
-                // symmetric, and the blocks of one column corner ac lie
-                // together (the slots sharing this cell read adjacent blocks)
-                const double* p = V + e * (int64_t)(NEN * NEN * B2) + (int64_t)(ac * NEN + ar) * B2;
-                const double* p0 = V + (int64_t)(ac * NEN + ar) * B2;
-                const double ise = inv_scale[e];
 #pragma unroll
-                for (int m = 0; m < B2; ++m) {
-                    const double pv = p[(m % BS) * BS + m / BS];
-                    acc[m] = first ? pv : acc[m] + pv;
-                    dmax = max_nan(dmax, fabs(p[m] * ise - p0[m]));
-                }
-                first = false;
-            }
+    for (int kc = 0; kc < NEN; ++kc) {  // cell (a - 1 + k) per direction: ascending element index
+        bool ok = true;
+        int rc = 0;
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int kd = d < ND ? (kc >> d) & 1 : 1;
+            const int64_t cd = a[d] - 1 + kd;
+            const int rb = o[d] - kd;  // row node's corner bit in the cell
+            ok = ok && cd >= 0 && cd < g.e[d] && (rb == 0 || rb == 1);
+            rc += (rb & 1) << d;
+        }
+        if (!ok) continue;
+        const double* p = &sv[w][kc * SEC + g.corner[rc] * B2];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) {
+            const double pv = p[(m % BS) * BS + m / BS];
+            acc[m] = first ? pv : acc[m] + pv;
+        }
+        first = false;
+    }
 #pragma unroll
     for (int m = 0; m < B2; ++m) bval[u * B2 + m] = acc[m];
-    browidx[u] = (int32_t)r;
+    browidx[u] = (int32_t)(b[0] + g.n[0] * (b[1] + g.n[1] * b[2]));
     bcolidx[u] = (int32_t)c;
-    return dmax;
 }
 
 // Is IEN the meshgrid connectivity (corner order of element 0) and the node
@@ -482,7 +455,20 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     const int64_t ncoo = m->n_elem * KE;
     if (ncoo >= (int64_t(1) << 32)) fail(SFEM_ERR_UNSUPPORTED, "assemble_system: 2^32 or more COO entries");
     DevBuf<double> V(ctx, (size_t)ncoo);
-    launch_element_stiffness(ctx, m, V.get(), vout.get());
+    // a mesh declared a grid gets the element-matrix check inside the element
+    // kernel (used if the dof numbering below admits the grid path)
+    const bool grid_try = m->grid_nd && ndof == m->ndim && m->n_elem > 0 && !(flags & SFEM_ASM_FORCE_SCALAR);
+    DevBuf<double> ke0, escale, bmax, dev;
+    if (grid_try) {
+        ke0.reset(ctx, (size_t)KE);
+        escale.reset(ctx, (size_t)m->n_elem);
+        bmax.reset(ctx, element_launch_blocks(m));
+        dev.reset(ctx, 1);
+        launch_element_zero(ctx, m, ke0.get(), vout.get());
+        launch_element_checked(ctx, m, V.get(), vout.get(), ElemCheck{ke0.get(), escale.get(), bmax.get()});
+    } else {
+        launch_element_stiffness(ctx, m, V.get(), vout.get());
+    }
     tm.mark();
     ValidateOut vr;
     d2h(ctx, &vr, vout.get(), sizeof(vr));
@@ -517,7 +503,7 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     const int row_bits = bits_for(K->nunits);
     if (2 * row_bits > 64) fail(SFEM_ERR_UNSUPPORTED, "too many dofs for 64-bit keys");
 
-    if (m->grid_nd && bs == ndof && ndof == m->ndim && K->nunits == m->n_nodes && m->n_elem > 0) {
+    if (grid_try && bs == ndof && K->nunits == m->n_nodes) {
         // structured grid: the pattern is the grid's, no keys or sort
         GridAsm g{};
         for (int d = 0; d < 3; ++d) {
@@ -542,25 +528,17 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         K->browidx.reset(ctx, (size_t)nnzb);
         K->bcolidx.reset(ctx, (size_t)nnzb);
         K->bval.reset(ctx, (size_t)nnzb * bs * bs);
-        const int ns = m->grid_nd == 3 ? 27 : 9;
-        K->escale.reset(ctx, (size_t)m->n_elem);
-        DevBuf<double> inv(ctx, (size_t)m->n_elem);
-        hipLaunchKernelGGL(grid_scale_kernel, dim3(grid_for(m->n_elem, 256)), dim3(256), 0, ctx->stream, V.get(),
-                           m->n_elem, KE, K->escale.get(), inv.get());
-        SFEM_LAUNCH_CHECK();
-        const unsigned gblocks = grid_for(nn * ns, 256);
-        DevBuf<double> bmax(ctx, gblocks), dev(ctx, 1);
+        K->escale = std::move(escale);
+        const unsigned gblocks = grid_for(nn, 256 / 32);
         if (m->grid_nd == 3)
             hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
-                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get(),
-                               (const double*)inv.get(), bmax.get());
+                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
         else
             hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
-                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get(),
-                               (const double*)inv.get(), bmax.get());
+                               K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
         SFEM_LAUNCH_CHECK();
         hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
-                           (int64_t)gblocks, dev.get());
+                           (int64_t)bmax.size(), dev.get());
         SFEM_LAUNCH_CHECK();
         tm.mark();
         K->ke0.resize((size_t)KE);

@@ -198,6 +198,9 @@ __device__ __forceinline__ void elem_block_gp(const double* A, const double* B, 
 //            node) dN/dX
 //   phase 2: per (element, a, b) the ndof x ndof block of Ke summed over the
 //            Gauss points in order (the reference's V[inz] += Ke[i,j], :245)
+//            and, with chk.ke0 set (structured grids: Hex8 and Q4, where
+//            an element's P threads share a wavefront), compares it with
+//            s_e Ke(0) (ElemCheck)
 //   phase 3: coalesced store of the EPB elements' Ke in COO order.
 // The block formula below is B'DB written out per node pair; with the
 // reference's D matrices (:217, :230) and contraction off, block (a,b) entry
@@ -209,7 +212,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
                                                       const double* __restrict__ young,
                                                       int64_t n_young, double nu_scalar,
                                                       const double* __restrict__ nu_e, int64_t n_elem,
-                                                      double* __restrict__ V, ValidateOut* out) {
+                                                      double* __restrict__ V, ValidateOut* out, ElemCheck chk) {
     constexpr int NGP = 1 << NDIM;
     constexpr int P = NEN * NEN;
     constexpr int EPB = 256 / P;
@@ -219,6 +222,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
     __shared__ double sinv[EPB][NGP][NDIM * NDIM];
     __shared__ double sw[EPB][NGP];
     __shared__ double sd[EPB][NGP][NEN][NDIM];
+    __shared__ double swmax[4];
     __shared__ double ske[EPB * KE];
 
     const int tid = threadIdx.x;
@@ -253,6 +257,7 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
     }
     __syncthreads();
 
+    double dmax = 0.0;
     if (tid < EPB * P) {
         const int el = tid / P, p = tid % P, a = p / NEN, b = p % NEN;
         const int64_t e = e0 + el;
@@ -285,8 +290,33 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
         for (int i = 0; i < NDOF; ++i)
 #pragma unroll
             for (int j = 0; j < NDOF; ++j) ske[el * KE + p * NDOF * NDOF + i * NDOF + j] = acc[i][j];
+        if (chk.ke0) {
+            // s_e = Ke(e)[0][0] / Ke(0)[0][0] from the element's (0, 0) thread
+            // (an element's P threads are consecutive lanes, P divides 64);
+            // |Ke(e) (1 / s_e) - Ke(0)| rounded once (a fused multiply-add)
+            const double k00 = __shfl(acc[0][0], (tid & 63) & ~(P - 1), 64);
+            const double s = k00 / chk.ke0[0];
+            const double ise = 1.0 / s;
+            if (e < n_elem) {
+                if (p == 0) chk.escale[e] = s;
+#pragma unroll
+                for (int i = 0; i < NDOF; ++i)
+#pragma unroll
+                    for (int j = 0; j < NDOF; ++j)
+                        dmax = max_nan(dmax, fabs(fma(acc[i][j], ise, -chk.ke0[p * NDOF * NDOF + i * NDOF + j])));
+            }
+        }
+    }
+    if (chk.ke0) {  // block maximum, one store per block
+        for (int off = 32; off > 0; off >>= 1) dmax = max_nan(dmax, __shfl_xor(dmax, off, 64));
+        if ((tid & 63) == 0) swmax[tid >> 6] = dmax;
     }
     __syncthreads();
+    if (chk.ke0 && tid == 0) {
+        double bm = swmax[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = max_nan(bm, swmax[w]);
+        chk.bmax[blockIdx.x] = bm;
+    }
     const int64_t nel = min((int64_t)EPB, n_elem - e0);
     const int count = (int)(nel * KE);
     double* dst = V + e0 * KE;
@@ -340,13 +370,21 @@ __global__ __launch_bounds__(256) void keys_scalar_kernel(const int32_t* __restr
 }
 
 template <int NDIM, int NEN, int NDOF>
-void launch_elem(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* out) {
+void launch_elem(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* out, int64_t n_elem = -1,
+                 ElemCheck chk = {}) {
     constexpr int EPB = 256 / (NEN * NEN);
-    const unsigned grid = grid_for(m->n_elem, EPB);
+    if (n_elem < 0) n_elem = m->n_elem;
+    const unsigned grid = grid_for(n_elem, EPB);
     hipLaunchKernelGGL((element_kernel<NDIM, NEN, NDOF>), dim3(grid), dim3(256), 0, ctx->stream,
-                       m->coords.get(), m->ien.get(), m->young.get(), m->n_young, m->nu, m->nu_e.get(),
-                       m->n_elem, V, out);
+                       m->coords.get(), m->ien.get(), m->young.get(), m->n_young, m->nu, m->nu_e.get(), n_elem,
+                       V, out, chk);
     SFEM_LAUNCH_CHECK();
+}
+
+int grid_kind(const sfem_mesh* m) {
+    const int kind = m->ndim * 100 + m->nen * 10 + m->ndof;
+    if (m->face || (kind != 383 && kind != 242)) fail(SFEM_ERR_UNSUPPORTED, "element check: Hex8 / Q4 vector meshes");
+    return kind;
 }
 
 }  // namespace
@@ -376,6 +414,25 @@ void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, Vali
         case 381: launch_elem<3, 8, 1>(ctx, m, V, d_out); break;  // Hex8 Laplace
         default: fail(SFEM_ERR_DIMENSION, "no element kernel for this (ndim, nen, nDof)");
     }
+}
+
+unsigned element_launch_blocks(const sfem_mesh* m) {
+    return grid_for(m->n_elem, grid_kind(m) == 383 ? 256 / 64 : 256 / 16);
+}
+
+void launch_element_zero(sfem_ctx* ctx, const sfem_mesh* m, double* ke0, ValidateOut* d_out) {
+    if (grid_kind(m) == 383)
+        launch_elem<3, 8, 3>(ctx, m, ke0, d_out, 1);
+    else
+        launch_elem<2, 4, 2>(ctx, m, ke0, d_out, 1);
+}
+
+void launch_element_checked(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out,
+                            const ElemCheck& chk) {
+    if (grid_kind(m) == 383)
+        launch_elem<3, 8, 3>(ctx, m, V, d_out, -1, chk);
+    else
+        launch_elem<2, 4, 2>(ctx, m, V, d_out, -1, chk);
 }
 
 int64_t launch_keys(sfem_ctx* ctx, const sfem_mesh* m, int bs, int row_bits, uint64_t* keys,

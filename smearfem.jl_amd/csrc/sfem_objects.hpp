@@ -92,6 +92,22 @@ struct ValidateOut {
 };
 void launch_validate(sfem_ctx* ctx, const sfem_mesh* m, ValidateOut* d_out);
 void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out);
+// The element-matrix check of a structured-grid assembly, done while the
+// element kernel stores Ke: with ke0 = element 0's Ke (computed first, by
+// launch_element_zero), escale[e] = s_e = Ke(e)[0][0] / Ke(0)[0][0] and, per
+// launch block b of the element kernel, bmax[b] = max |Ke(e) (1 / s_e) - Ke(0)|
+// over its elements' entries (NaN if any is).  Hex8 / Q4 vector meshes only.
+// max_nan: the larger of a and b, NaN when either is.
+__device__ __forceinline__ double max_nan(double a, double b) { return (a != a || b != b) ? a + b : fmax(a, b); }
+struct ElemCheck {
+    const double* ke0;
+    double* escale;
+    double* bmax;
+};
+unsigned element_launch_blocks(const sfem_mesh* m);
+void launch_element_zero(sfem_ctx* ctx, const sfem_mesh* m, double* ke0, ValidateOut* d_out);
+void launch_element_checked(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out,
+                            const ElemCheck& chk);
 // Emit (key, payload) for every unit pair; returns pair count.
 int64_t launch_keys(sfem_ctx* ctx, const sfem_mesh* m, int bs, int row_bits, uint64_t* keys,
                     uint32_t* payload);

@@ -249,3 +249,23 @@ def test_structured_assembly_meshgrid_and_errors(ctx):
     with pytest.raises(smearfem.ArgumentError):
         mesh.set_grid((4, 3, 2))
     mesh.close()
+
+
+def test_grid_declared_non_blocked_numbering(ctx):
+    """A declared grid whose numbering is node-interleaved in component 0 only
+    (one node's components 1 and 2 swapped): the grid check accepts it, the
+    block path does not apply, so the element matrices stored by node column
+    are computed again in COO order for the scalar path: the oracle's K."""
+    print("[case] grid declared, non-blocked numbering", flush=True)
+    m = structured_mesh((4, 3, 2))
+    idp = m.ID.copy()
+    idp[5, [1, 2]] = idp[5, [2, 1]]
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=5)
+    mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList, m.IEN, 3, "Q1", 3, idp, young, 0.3)
+    mesh.set_grid((4, 3, 2))
+    K = mesh.assemble()
+    assert K.block == 1 and K.grid is None
+    g = (K.shape, K.block, K.csc_arrays())
+    K.close()
+    mesh.close()
+    assert_parity(g, oracle_csc(m, 3, young=young, ID=idp))
