@@ -210,23 +210,25 @@ __global__ __launch_bounds__(256) void grid_counts_kernel(GridAsm g, int64_t nn,
     cnt[c] = k;
 }
 
-// one block of 1024: out = max of the n block maxima (NaN propagates)
+// out[block] = max of the block's grid-stride share of in[0, n) (NaN
+// propagates); a second launch with one block reduces the block results
 __global__ __launch_bounds__(1024) void max_reduce_kernel(const double* __restrict__ in, int64_t n,
                                                           double* __restrict__ out) {
     __shared__ double wmax[16];
     double m = 0.0;
-    for (int64_t i = threadIdx.x; i < n; i += blockDim.x) m = max_nan(m, in[i]);
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        m = max_nan(m, in[i]);
     for (int off = 32; off > 0; off >>= 1) m = max_nan(m, __shfl_xor(m, off, 64));
     if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
         double b = wmax[0];
         for (int w = 1; w < (int)(blockDim.x >> 6); ++w) b = max_nan(b, wmax[w]);
-        *out = b;
+        out[blockIdx.x] = b;
     }
 }
 
-// Half a wavefront per node column c (8 columns per block of 256).
+// Half a wavefront per node column c (4 columns per block of 128).
 //   phase 1: lane (cell, part) copies its part of the column node's section
 //            in one of the cells around c (the element's NEN blocks whose
 //            column corner is c, NEN * ndof^2 contiguous values) from V to LDS
@@ -239,12 +241,12 @@ __global__ __launch_bounds__(1024) void max_reduce_kernel(const double* __restri
 // Element matrices are bitwise symmetric, so block (row ar, column c) is read
 // as the transpose of block (c, ar), which lies in c's section.
 template <int ND>
-__global__ __launch_bounds__(256) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
+__global__ __launch_bounds__(128) void grid_gather_kernel(const double* __restrict__ V, GridAsm g, int64_t nn,
                                                           const int64_t* __restrict__ bcolptr,
                                                           double* __restrict__ bval, int32_t* __restrict__ browidx,
                                                           int32_t* __restrict__ bcolidx) {
     constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, SEC = NEN * B2, KE = NEN * SEC;
-    constexpr int NS = ND == 3 ? 27 : 9, CPB = 256 / 32;
+    constexpr int NS = ND == 3 ? 27 : 9, CPB = 128 / 32;
     constexpr int LPC = 32 / NEN, PER = SEC / LPC;  // lanes per cell, values per lane
     static_assert(PER % 2 == 0 && (SEC * 8) % 16 == 0, "16-byte copies");
     __shared__ __align__(16) double sv[CPB][NEN * SEC];
@@ -529,16 +531,21 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         K->bcolidx.reset(ctx, (size_t)nnzb);
         K->bval.reset(ctx, (size_t)nnzb * bs * bs);
         K->escale = std::move(escale);
-        const unsigned gblocks = grid_for(nn, 256 / 32);
+        const unsigned gblocks = grid_for(nn, 128 / 32);
         if (m->grid_nd == 3)
-            hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
+            hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(gblocks), dim3(128), 0, ctx->stream, V.get(), g, nn,
                                K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
         else
-            hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(gblocks), dim3(256), 0, ctx->stream, V.get(), g, nn,
+            hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(gblocks), dim3(128), 0, ctx->stream, V.get(), g, nn,
                                K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
         SFEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
-                           (int64_t)bmax.size(), dev.get());
+        const unsigned rblocks = std::min(256u, grid_for((int64_t)bmax.size(), 1024));
+        DevBuf<double> part(ctx, rblocks);
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(rblocks), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
+                           (int64_t)bmax.size(), part.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)part.get(),
+                           (int64_t)rblocks, dev.get());
         SFEM_LAUNCH_CHECK();
         tm.mark();
         K->ke0.resize((size_t)KE);

@@ -192,20 +192,25 @@ __device__ __forceinline__ void elem_block_gp(const double* A, const double* B, 
 }
 
 // ------------------------------------------------------- element kernel --
-// A block holds EPB elements and one thread per (element, node a, node b).
+// Ke is bitwise symmetric: the block formula below is B'DB written out per
+// node pair, and with the reference's D matrices (:217, :230) and contraction
+// off, block (a,b) entry (i,j) and block (b,a) entry (j,i) are the same
+// rounded expression.  So only the node pairs a <= b are computed.
+// A block holds EPB elements and one thread per (element, a <= b).
 //   phase 0: gather the element coordinates into LDS
 //   phase 1: per (element, gp) Jacobian, |det|, inverse; per (element, gp,
 //            node) dN/dX
-//   phase 2: per (element, a, b) the ndof x ndof block of Ke summed over the
-//            Gauss points in order (the reference's V[inz] += Ke[i,j], :245)
-//            and, with chk.ke0 set (structured grids: Hex8 and Q4, where
-//            an element's P threads share a wavefront), compares it with
-//            s_e Ke(0) (ElemCheck)
-//   phase 3: coalesced store of the EPB elements' Ke in COO order.
-// The block formula below is B'DB written out per node pair; with the
-// reference's D matrices (:217, :230) and contraction off, block (a,b) entry
-// (i,j) and block (b,a) entry (j,i) are the same rounded expression, so Ke is
-// bitwise symmetric.
+//   phase 2: per (element, a <= b) the ndof x ndof block of Ke summed over
+//            the Gauss points in order (the reference's V[inz] += Ke[i,j],
+//            :245) into LDS; with chk.ke0 set (structured grids) each block
+//            is also compared with s_e Ke(0) (ElemCheck)
+//   phase 3: coalesced store of the EPB elements' Ke in COO order, block
+//            (b, a) as the transpose of (a, b) for a < b.
+template <int NEN>
+constexpr int elem_pairs() { return NEN * (NEN + 1) / 2; }
+template <int NEN>
+constexpr int elem_per_block() { return 256 / elem_pairs<NEN>(); }
+
 template <int NDIM, int NEN, int NDOF>
 __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__ coords,
                                                       const int32_t* __restrict__ ien,
@@ -214,20 +219,28 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
                                                       const double* __restrict__ nu_e, int64_t n_elem,
                                                       double* __restrict__ V, ValidateOut* out, ElemCheck chk) {
     constexpr int NGP = 1 << NDIM;
-    constexpr int P = NEN * NEN;
-    constexpr int EPB = 256 / P;
-    constexpr int KE = NEN * NDOF * NEN * NDOF;
+    constexpr int NPS = elem_pairs<NEN>();
+    constexpr int EPB = elem_per_block<NEN>();
+    constexpr int B2 = NDOF * NDOF;
+    constexpr int KE = NEN * NEN * B2;
     __shared__ double sx[EPB][NEN][NDIM];
     __shared__ int svalid[EPB];
     __shared__ double sinv[EPB][NGP][NDIM * NDIM];
     __shared__ double sw[EPB][NGP];
     __shared__ double sd[EPB][NGP][NEN][NDIM];
     __shared__ double swmax[4];
-    __shared__ double ske[EPB * KE];
+    __shared__ double ske[EPB * NPS * B2];  // the computed blocks, (element, a <= b)
+    __shared__ int16_t stab[KE];            // COO entry of an element -> its value in ske
 
     const int tid = threadIdx.x;
     const int64_t e0 = (int64_t)blockIdx.x * EPB;
     if (tid < EPB) svalid[tid] = (e0 + tid) < n_elem;
+    for (int q = tid; q < KE; q += blockDim.x) {
+        const int p = q / B2, m = q % B2, a = p / NEN, b = p % NEN, i = m / NDOF, j = m % NDOF;
+        const int lo = min(a, b), hi = max(a, b);
+        const int pair = lo * NEN - lo * (lo - 1) / 2 + (hi - lo);
+        stab[q] = (int16_t)(pair * B2 + (a <= b ? i * NDOF + j : j * NDOF + i));
+    }
     __syncthreads();
     for (int t = tid; t < EPB * NEN; t += blockDim.x) {
         const int el = t / NEN, a = t % NEN;
@@ -257,15 +270,21 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
     }
     __syncthreads();
 
-    double dmax = 0.0;
-    if (tid < EPB * P) {
-        const int el = tid / P, p = tid % P, a = p / NEN, b = p % NEN;
-        const int64_t e = e0 + el;
-        double acc[NDOF][NDOF];
+    const bool busy = tid < EPB * NPS;
+    const int el = busy ? tid / NPS : 0;
+    int a = 0, b = busy ? tid % NPS : 0;  // pair index -> (a, b), a <= b
+    while (b >= NEN - a) {
+        b -= NEN - a;
+        ++a;
+    }
+    b += a;
+    const int64_t e = e0 + el;
+    double acc[NDOF][NDOF];
 #pragma unroll
-        for (int i = 0; i < NDOF; ++i)
+    for (int i = 0; i < NDOF; ++i)
 #pragma unroll
-            for (int j = 0; j < NDOF; ++j) acc[i][j] = 0.0;
+        for (int j = 0; j < NDOF; ++j) acc[i][j] = 0.0;
+    if (busy) {
         double D11 = 0, D12 = 0, D44 = 0;
         if constexpr (NDOF > 1) {
             const double E = (e < n_elem) ? young[n_young == 1 ? 0 : e] : 0.0;
@@ -289,38 +308,39 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
 #pragma unroll
         for (int i = 0; i < NDOF; ++i)
 #pragma unroll
-            for (int j = 0; j < NDOF; ++j) ske[el * KE + p * NDOF * NDOF + i * NDOF + j] = acc[i][j];
-        if (chk.ke0) {
-            // s_e = Ke(e)[0][0] / Ke(0)[0][0] from the element's (0, 0) thread
-            // (an element's P threads are consecutive lanes, P divides 64);
-            // |Ke(e) (1 / s_e) - Ke(0)| rounded once (a fused multiply-add)
-            const double k00 = __shfl(acc[0][0], (tid & 63) & ~(P - 1), 64);
-            const double s = k00 / chk.ke0[0];
-            const double ise = 1.0 / s;
-            if (e < n_elem) {
-                if (p == 0) chk.escale[e] = s;
-#pragma unroll
-                for (int i = 0; i < NDOF; ++i)
-#pragma unroll
-                    for (int j = 0; j < NDOF; ++j)
-                        dmax = max_nan(dmax, fabs(fma(acc[i][j], ise, -chk.ke0[p * NDOF * NDOF + i * NDOF + j])));
-            }
-        }
-    }
-    if (chk.ke0) {  // block maximum, one store per block
-        for (int off = 32; off > 0; off >>= 1) dmax = max_nan(dmax, __shfl_xor(dmax, off, 64));
-        if ((tid & 63) == 0) swmax[tid >> 6] = dmax;
+            for (int j = 0; j < NDOF; ++j) ske[tid * B2 + i * NDOF + j] = acc[i][j];
     }
     __syncthreads();
-    if (chk.ke0 && tid == 0) {
-        double bm = swmax[0];
-        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = max_nan(bm, swmax[w]);
-        chk.bmax[blockIdx.x] = bm;
+    if (chk.ke0) {
+        // s_e = Ke(e)[0][0] / Ke(0)[0][0]; |Ke(e) (1 / s_e) - Ke(0)| rounded
+        // once (a fused multiply-add); block (b, a) holds the same values
+        double dmax = 0.0;
+        if (busy && e < n_elem) {
+            const double s = ske[el * NPS * B2] / chk.ke0[0];
+            const double ise = 1.0 / s;
+            if (a == 0 && b == 0) chk.escale[e] = s;
+#pragma unroll
+            for (int i = 0; i < NDOF; ++i)
+#pragma unroll
+                for (int j = 0; j < NDOF; ++j)
+                    dmax = max_nan(dmax, fabs(fma(acc[i][j], ise, -chk.ke0[(a * NEN + b) * B2 + i * NDOF + j])));
+        }
+        for (int off = 32; off > 0; off >>= 1) dmax = max_nan(dmax, __shfl_xor(dmax, off, 64));
+        if ((tid & 63) == 0) swmax[tid >> 6] = dmax;
+        __syncthreads();
+        if (tid == 0) {
+            double bm = swmax[0];
+            for (int w = 1; w < (int)(blockDim.x >> 6); ++w) bm = max_nan(bm, swmax[w]);
+            chk.bmax[blockIdx.x] = bm;
+        }
     }
     const int64_t nel = min((int64_t)EPB, n_elem - e0);
     const int count = (int)(nel * KE);
     double* dst = V + e0 * KE;
-    for (int t = tid; t < count; t += blockDim.x) dst[t] = ske[t];
+    for (int t = tid; t < count; t += blockDim.x) {
+        const int te = t / KE;
+        dst[t] = ske[te * (NPS * B2) + stab[t - te * KE]];
+    }
 }
 
 // ------------------------------------------------------------- key emission --
@@ -372,7 +392,7 @@ __global__ __launch_bounds__(256) void keys_scalar_kernel(const int32_t* __restr
 template <int NDIM, int NEN, int NDOF>
 void launch_elem(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* out, int64_t n_elem = -1,
                  ElemCheck chk = {}) {
-    constexpr int EPB = 256 / (NEN * NEN);
+    constexpr int EPB = elem_per_block<NEN>();
     if (n_elem < 0) n_elem = m->n_elem;
     const unsigned grid = grid_for(n_elem, EPB);
     hipLaunchKernelGGL((element_kernel<NDIM, NEN, NDOF>), dim3(grid), dim3(256), 0, ctx->stream,
@@ -417,7 +437,7 @@ void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, Vali
 }
 
 unsigned element_launch_blocks(const sfem_mesh* m) {
-    return grid_for(m->n_elem, grid_kind(m) == 383 ? 256 / 64 : 256 / 16);
+    return grid_for(m->n_elem, grid_kind(m) == 383 ? elem_per_block<8>() : elem_per_block<4>());
 }
 
 void launch_element_zero(sfem_ctx* ctx, const sfem_mesh* m, double* ke0, ValidateOut* d_out) {

@@ -480,6 +480,28 @@ __global__ void to_half_kernel(const float* __restrict__ v, int64_t n, float sca
         h[i] = (_Float16)(v[i] * scale);
 }
 
+// The scale 2^-e of a level's FP16 stencil, amax = f 2^e with f in [1/2, 1)
+// (amax as stencil_maxdiag_kernel's bit pattern); 0 when amax is zero or not
+// finite
+__global__ void half_scale_kernel(const unsigned* __restrict__ amax_bits, double* __restrict__ scale) {
+    const float amax = __uint_as_float(*amax_bits);
+    double sc = 0.0;
+    if (amax > 0.0f && isfinite(amax)) {
+        int e = 0;
+        (void)frexp((double)amax, &e);
+        sc = ldexp(1.0, -e);
+    }
+    *scale = sc;
+}
+
+// to_half_kernel with the scale read from the device
+__global__ void to_half_dev_kernel(const float* __restrict__ v, int64_t n, const double* __restrict__ scale,
+                                   _Float16* __restrict__ h) {
+    const float sc = (float)*scale;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        h[i] = (_Float16)(v[i] * sc);
+}
+
 // Inverse of each block row's diagonal block, read from the FP16 values.
 template <int BS>
 __global__ void binv_half_kernel(const int64_t* __restrict__ sl_ptr, const int32_t* __restrict__ sl_w,
@@ -2081,6 +2103,100 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         }
     }
 
+    // Power iterations (lambda_max(B^-1 A) per level, the coarsest level
+    // excepted) run on side streams as soon as a level's smoothing operator
+    // exists, while the next levels are built: the finest level's first when
+    // it is matrix-free.  Every level has its own vectors, sums and PowState;
+    // all of them live until the side streams are joined (join syncs them
+    // when setup fails).  A side stream starts after an event recorded on the
+    // context's stream once its buffers are allocated, so memory the cache
+    // hands out again is past its earlier users.
+    DevBuf<PowState> ps(ctx, SFEM_MG_MAX_LEVELS);
+    memset_d(ctx, ps.get(), 0, sizeof(PowState) * SFEM_MG_MAX_LEVELS);
+    DevBuf<double> scales(ctx, SFEM_MG_MAX_LEVELS);  // FP16 stencil scales, 0: zero operator
+    std::vector<DevBuf<double>> parts(SFEM_MG_MAX_LEVELS);
+    DevBuf<float> p0a, p0b;
+    struct SideJoin {
+        std::vector<hipStream_t> streams;
+        std::vector<hipEvent_t> done;
+        ~SideJoin() {
+            for (hipStream_t t : streams) (void)hipStreamSynchronize(t);
+            for (hipEvent_t e : done) (void)hipEventDestroy(e);
+        }
+    } join;
+    auto fork = [&](hipStream_t t) {
+        hipEvent_t f;
+        SFEM_HIP(hipEventCreateWithFlags(&f, hipEventDisableTiming));
+        SFEM_HIP(hipEventRecord(f, s));
+        SFEM_HIP(hipStreamWaitEvent(t, f, 0));
+        (void)hipEventDestroy(f);
+        if (std::find(join.streams.begin(), join.streams.end(), t) == join.streams.end()) join.streams.push_back(t);
+    };
+    auto finish = [&](hipStream_t t) {
+        hipEvent_t d;
+        SFEM_HIP(hipEventCreateWithFlags(&d, hipEventDisableTiming));
+        join.done.push_back(d);
+        SFEM_HIP(hipEventRecord(d, t));
+    };
+    auto init_grid = [&](int64_t n) { return std::min<unsigned>(rows_grid(n), 1024); };
+    // v's vectors: da, db (n + zero) floats
+    auto power_start = [&](int l, const MgLevel& v, float* va, float* vb, unsigned nparts, hipStream_t t) {
+        const int64_t n = v.nrows * BS;
+        parts[(size_t)l].reset(ctx, std::max({nparts, init_grid(n), 1024u}));
+        fork(t);
+        hipLaunchKernelGGL(pow_init_kernel, dim3(init_grid(n)), dim3(kThreads), 0, t, n, v.fixed.get(), va,
+                           ps.get() + l, parts[(size_t)l].get());
+        SFEM_LAUNCH_CHECK();
+    };
+    auto power_coarse = [&](int l, const MgLevel& v) {
+        const hipStream_t t = side_stream(ctx, (l - 1) % (sfem_ctx::kSideStreams - 1));
+        const bool grp = v.node_major;
+        const dim3 gp(std::min(rows_grid(v.nn * (grp ? Lanes<ND>::G : 1)), 1024u));
+        power_start(l, v, v.da.get(), v.db.get(), gp.x, t);
+        float* vin = v.da.get();
+        float* vout = v.db.get();
+        double* partials = parts[(size_t)l].get();
+        for (int k = 0; k < o.power_iters; ++k) {
+            // grid-stride over the nodes: few blocks meet at the last-block count
+            auto go = [&](auto kern, auto op) {
+                hipLaunchKernelGGL(kern, gp, dim3(kThreads), 0, t, op, vin, v.binv.get(), vout, ps.get() + l,
+                                   partials);
+            };
+            if (v.st16.size()) {
+                const StOp<_Float16> op{v.st16.get(), grid_of(v), v.nn};
+                grp ? go(pow_stencil_kernel<ND, true, _Float16>, op) : go(pow_stencil_kernel<ND, false, _Float16>, op);
+            } else {
+                const StOp<float> op{v.st32.get(), grid_of(v), v.nn};
+                grp ? go(pow_stencil_kernel<ND, true, float>, op) : go(pow_stencil_kernel<ND, false, float>, op);
+            }
+            SFEM_LAUNCH_CHECK();
+            std::swap(vin, vout);
+        }
+        finish(t);
+    };
+    // The finest level's, when it is matrix-free, starts once the first
+    // coarse level is built: the first Galerkin product (the longest kernel
+    // of setup) runs alone.
+    auto power_fine_mf = [&]() {
+        const int64_t n = L0.nrows * BS;
+        p0a.reset(ctx, (size_t)(n + BS));
+        p0b.reset(ctx, (size_t)(n + BS));
+        memset_d(ctx, p0a.get(), 0, p0a.bytes());
+        memset_d(ctx, p0b.get(), 0, p0b.bytes());
+        const unsigned gm = std::min(rows_grid(L0.nrows), 1024u);
+        const hipStream_t t = side_stream(ctx, sfem_ctx::kSideStreams - 1);
+        power_start(0, L0, p0a.get(), p0b.get(), gm, t);
+        float* vin = p0a.get();
+        float* vout = p0b.get();
+        for (int k = 0; k < o.power_iters; ++k) {
+            hipLaunchKernelGGL(pow_mf_kernel<ND>, dim3(gm), dim3(kThreads), 0, t, mf_op32(S, H), vin,
+                               S->binv32.get(), vout, ps.get(), parts[0].get());
+            SFEM_LAUNCH_CHECK();
+            std::swap(vin, vout);
+        }
+        finish(t);
+    };
+
     // the sliced values of K_ff: read by the finest level unless it is matrix-free
     if (!H.mf) ensure_sell(ctx, S);
 
@@ -2176,6 +2292,38 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         hipLaunchKernelGGL(stencil_binv_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, C.st64.get(),
                            C.fixed.get(), C.nn, C.binv.get());
         SFEM_LAUNCH_CHECK();
+        // unless C is the coarsest level (the loop's exit tests for the next
+        // pass; the coarsest is solved directly): its smoothing operator, FP16
+        // values with a power-of-two scale (the scale found on the device),
+        // then its power iteration on a side stream
+        bool more = false;
+        for (int d = 0; d < ND; ++d) more = more || C.n[d] - 1 >= 2;
+        const int lc = (int)H.L.size();
+        if (more && C.nn > o.coarse_nodes && lc + 1 < SFEM_MG_MAX_LEVELS) {
+            if (o.coarse_fp16) {
+                const int64_t nv = C.nn * NS * B2;
+                DevBuf<unsigned> mx(ctx, 1);
+                memset_d(ctx, mx.get(), 0, sizeof(unsigned));
+                hipLaunchKernelGGL(stencil_maxdiag_kernel<ND>, dim3(std::min(rows_grid(C.nn * BS), 256u)),
+                                   dim3(kThreads), 0, s, (const float*)C.st32.get(), C.nn, C.node_major, mx.get());
+                SFEM_LAUNCH_CHECK();
+                hipLaunchKernelGGL(half_scale_kernel, dim3(1), dim3(1), 0, s, (const unsigned*)mx.get(),
+                                   scales.get() + lc);
+                SFEM_LAUNCH_CHECK();
+                C.st16.reset(ctx, (size_t)nv);
+                hipLaunchKernelGGL(to_half_dev_kernel, dim3(std::min(rows_grid(nv), 2048u)), dim3(kThreads), 0, s,
+                                   (const float*)C.st32.get(), nv, (const double*)scales.get() + lc, C.st16.get());
+                SFEM_LAUNCH_CHECK();
+                hipLaunchKernelGGL(stencil_binv_half_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s,
+                                   (const _Float16*)C.st16.get(), C.nn, C.node_major, C.binv.get());
+                SFEM_LAUNCH_CHECK();
+                C.st32.release();
+            }
+            C.da.reset(ctx, (size_t)C.nn * BS);
+            C.db.reset(ctx, (size_t)C.nn * BS);
+            power_coarse(lc, C);
+        }
+        if (lc == 1 && H.mf) power_fine_mf();
         H.L.push_back(std::move(C));
         prev_st = H.L.back().st64.get();
         prev_fix = H.L.back().fixed.get();
@@ -2221,33 +2369,16 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         h2d(ctx, H.cinv.get(), inv.data(), inv.size() * sizeof(double));
     }
 
-    // FP16 smoothing operators on the levels between the finest and the coarsest
-    if (o.coarse_fp16)
+    // FP16 smoothing operators on the levels between the finest and the
+    // coarsest (made as the levels were built): their scales
+    if (o.coarse_fp16 && H.L.size() > 2) {
+        std::vector<double> hs(H.L.size());
+        d2h(ctx, hs.data(), scales.get(), sizeof(double) * hs.size());
         for (size_t l = 1; l + 1 < H.L.size(); ++l) {
-            MgLevel& v = H.L[l];
-            const int64_t nv = v.nn * NS * B2;
-            DevBuf<unsigned> mx(ctx, 1);
-            memset_d(ctx, mx.get(), 0, sizeof(unsigned));
-            hipLaunchKernelGGL(stencil_maxdiag_kernel<ND>, dim3(std::min(rows_grid(v.nn * BS), 256u)), dim3(kThreads),
-                               0, s, (const float*)v.st32.get(), v.nn, v.node_major, mx.get());
-            SFEM_LAUNCH_CHECK();
-            unsigned hm = 0;
-            d2h(ctx, &hm, mx.get(), sizeof(hm));
-            float amax;
-            std::memcpy(&amax, &hm, sizeof(amax));
-            if (!(amax > 0.0f) || !std::isfinite(amax)) fail(SFEM_ERR_SINGULAR, "multigrid: zero coarse operator");
-            int e = 0;
-            std::frexp((double)amax, &e);
-            v.scale = std::ldexp(1.0, -e);
-            v.st16.reset(ctx, (size_t)nv);
-            hipLaunchKernelGGL(to_half_kernel, dim3(std::min(rows_grid(nv), 2048u)), dim3(kThreads), 0, s,
-                               (const float*)v.st32.get(), nv, (float)v.scale, v.st16.get());
-            SFEM_LAUNCH_CHECK();
-            hipLaunchKernelGGL(stencil_binv_half_kernel<ND>, dim3(rows_grid(v.nn)), dim3(kThreads), 0, s,
-                               (const _Float16*)v.st16.get(), v.nn, v.node_major, v.binv.get());
-            SFEM_LAUNCH_CHECK();
-            v.st32.release();
+            if (!(hs[l] > 0.0)) fail(SFEM_ERR_SINGULAR, "multigrid: zero coarse operator");
+            H.L[l].scale = hs[l];
         }
+    }
 
     // FP16 finest-level smoothing operator
     if (o.fine_fp16 && !H.mf) {
@@ -2284,87 +2415,48 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         const size_t n = (size_t)v.nrows * BS, na = l == 0 ? n + BS : n;
         v.r.reset(ctx, na);
         v.x.reset(ctx, na);
-        v.da.reset(ctx, na);
-        v.db.reset(ctx, na);
+        if (!v.da.get()) {  // a level's power iteration may be using its own
+            v.da.reset(ctx, na);
+            v.db.reset(ctx, na);
+        }
         if (l == 0)
             for (float* b : {v.r.get(), v.x.get(), v.da.get(), v.db.get()}) memset_d(ctx, b, 0, na * sizeof(float));
         if (l > 0) v.b.reset(ctx, n);
     }
 
-    // lambda_max(B^-1 A) per level (the coarsest is solved, not smoothed)
+    // lambda_max(B^-1 A) of the finest level when it is not matrix-free (on
+    // the context's stream); then every level's result
     const int nl = (int)H.L.size();
-    DevBuf<PowState> ps(ctx, (size_t)nl);
-    memset_d(ctx, ps.get(), 0, sizeof(PowState) * nl);
-    const Launch ls = plan_sell(S->nslices);
-    auto init_grid = [&](int l) { return std::min<unsigned>(rows_grid(H.L[(size_t)l].nrows * BS), 1024); };
-    // the levels are independent: level 0 on the context's stream, the
-    // others on side streams, each with its own partial sums
-    std::vector<size_t> part_off((size_t)nl + 1, 0);
-    for (int l = 0; l < nl; ++l)
-        part_off[(size_t)l + 1] =
-            part_off[(size_t)l] + std::max({ls.grid, rows_grid(H.L[(size_t)l].nn), init_grid(l), 1024u});
-    DevBuf<double> partials_all(ctx, part_off[(size_t)nl]);
-    hipEvent_t forked;
-    SFEM_HIP(hipEventCreateWithFlags(&forked, hipEventDisableTiming));
-    SFEM_HIP(hipEventRecord(forked, ctx->stream));
-    std::vector<hipEvent_t> joined;
-    for (int l = 0; l + 1 < nl; ++l) {
-        MgLevel& v = H.L[l];
+    if (!H.mf) {
+        MgLevel& v = H.L[0];
         const int64_t n = v.nrows * BS;
-        const unsigned gi = init_grid(l);
-        const hipStream_t s = l == 0 ? ctx->stream : side_stream(ctx, l - 1);
-        if (l > 0) SFEM_HIP(hipStreamWaitEvent(s, forked, 0));
-        double* partials = partials_all.get() + part_off[(size_t)l];
-        hipLaunchKernelGGL(pow_init_kernel, dim3(gi), dim3(kThreads), 0, s, n, v.fixed.get(), v.da.get(),
-                           ps.get() + l, partials);
+        const Launch ls = plan_sell(S->nslices);
+        parts[0].reset(ctx, std::max({ls.grid, init_grid(n), 1024u}));
+        double* partials = parts[0].get();
+        hipLaunchKernelGGL(pow_init_kernel, dim3(init_grid(n)), dim3(kThreads), 0, s, n, v.fixed.get(), v.da.get(),
+                           ps.get(), partials);
         SFEM_LAUNCH_CHECK();
         float* vin = v.da.get();
         float* vout = v.db.get();
         for (int k = 0; k < o.power_iters; ++k) {
-            if (l == 0 && H.mf) {
-                hipLaunchKernelGGL(pow_mf_kernel<ND>, dim3(std::min(rows_grid(v.nrows), 1024u)), dim3(kThreads), 0,
-                                   s, mf_op32(S, H), vin, S->binv32.get(), vout, ps.get(), partials);
-            } else if (l == 0 && H.fp16) {
+            if (H.fp16) {
                 const SellOp<_Float16> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), H.h0.get(), S->nslices,
                                           S->nfb, ls.per_block};
                 hipLaunchKernelGGL((pow_sell_kernel<BS, _Float16>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
                                    (const float*)H.binv0.get(), vout, ps.get(), partials);
-            } else if (l == 0) {
+            } else {
                 const SellOp<float> op{S->sl_ptr.get(), S->sl_w.get(), S->scol.get(), S->sval32.get(), S->nslices,
                                        S->nfb, ls.per_block};
                 hipLaunchKernelGGL((pow_sell_kernel<BS, float>), dim3(ls.grid), dim3(kThreads), 0, s, op, vin,
                                    S->binv32.get(), vout, ps.get(), partials);
-            } else {
-                // grid-stride over the nodes: few blocks meet at the last-block count
-                const bool grp = v.node_major;
-                const dim3 gp(std::min(rows_grid(v.nn * (grp ? Lanes<ND>::G : 1)), 1024u));
-                auto go = [&](auto k, auto op) {
-                    hipLaunchKernelGGL(k, gp, dim3(kThreads), 0, s, op, vin, v.binv.get(), vout, ps.get() + l,
-                                       partials);
-                };
-                if (v.st16.size()) {
-                    const StOp<_Float16> op{v.st16.get(), grid_of(v), v.nn};
-                    grp ? go(pow_stencil_kernel<ND, true, _Float16>, op) : go(pow_stencil_kernel<ND, false, _Float16>, op);
-                } else {
-                    const StOp<float> op{v.st32.get(), grid_of(v), v.nn};
-                    grp ? go(pow_stencil_kernel<ND, true, float>, op) : go(pow_stencil_kernel<ND, false, float>, op);
-                }
             }
             SFEM_LAUNCH_CHECK();
             std::swap(vin, vout);
         }
-        if (l > 0) {
-            hipEvent_t done_l;
-            SFEM_HIP(hipEventCreateWithFlags(&done_l, hipEventDisableTiming));
-            joined.push_back(done_l);
-            SFEM_HIP(hipEventRecord(done_l, s));
-            SFEM_HIP(hipStreamWaitEvent(ctx->stream, done_l, 0));
-        }
     }
+    for (hipEvent_t d : join.done) SFEM_HIP(hipStreamWaitEvent(s, d, 0));
     std::vector<PowState> hps((size_t)nl);
     d2h(ctx, hps.data(), ps.get(), sizeof(PowState) * nl);  // the side streams joined: every level done
-    (void)hipEventDestroy(forked);
-    for (hipEvent_t e : joined) (void)hipEventDestroy(e);
     for (int l = 0; l + 1 < nl; ++l) {
         H.L[l].lmax = hps[(size_t)l].lam;
         if (!(H.L[l].lmax > 0.0) || !std::isfinite(H.L[l].lmax))
