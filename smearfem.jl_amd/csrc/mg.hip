@@ -409,20 +409,17 @@ __host__ __device__ __forceinline__ int64_t st_index(bool node_major, int64_t nn
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
 // side), zero the rows and columns of fixed dofs (st64, slot-major, the next
 // Galerkin product's input) and put a unit diagonal on them (st32, the
-// smoother's operator, in the level's layout).  Thread (s, node) writes
-// coalesced along nodes.
+// smoother's operator, in the level's layout).  Thread (node, slot s =
+// blockIdx.y): a wavefront reads and writes 64 consecutive nodes of a slot.
 template <int ND>
 __global__ __launch_bounds__(kThreads) void sym_mask_kernel(const double* __restrict__ raw, Grid g,
                                                             const uint8_t* __restrict__ fixed,
                                                             double* __restrict__ st64, float* __restrict__ st32,
                                                             bool node_major) {
-    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS, TI = kThreads / NS;
+    constexpr int BS = ND, B2 = BS * BS, NS = Sten<ND>::NS;
     const int64_t nn = g.n0 * g.n1 * g.n2;
-    const int64_t I0 = (int64_t)blockIdx.x * TI;
-    const int t = threadIdx.x;
-    if (t >= TI * NS) return;
-    const int s = t / TI, li = t % TI;
-    const int64_t I = I0 + li;
+    const int s = (int)blockIdx.y;
+    const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (I >= nn) return;
     int64_t c[3];
     coords_of<ND>(I, g, c);
@@ -2284,8 +2281,7 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         C.st64.reset(ctx, (size_t)C.nn * NS * B2);
         C.st32.reset(ctx, (size_t)C.nn * NS * B2);
         C.node_major = C.nn < stencil_group_below();
-        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3((unsigned)((C.nn + kThreads / NS - 1) / (kThreads / NS))),
-                           dim3(kThreads), 0, s, raw.get(), gc,
+        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn), NS), dim3(kThreads), 0, s, raw.get(), gc,
                            C.fixed.get(), C.st64.get(), C.st32.get(), C.node_major);
         SFEM_LAUNCH_CHECK();
         C.binv.reset(ctx, (size_t)C.nn * B2);
