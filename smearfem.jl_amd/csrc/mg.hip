@@ -164,14 +164,12 @@ __host__ __device__ __forceinline__ bool child(int64_t c, int k, int64_t e, int6
 // s of node I is the sum, over the cells I shares with its neighbour, of
 // s_e Ke(0)'s block (I's corner, the neighbour's corner); fixed rows and
 // columns zero, as stencil_from_k.  es: the scales with a ghost layer of
-// zero cells.
+// zero cells.  c: the node's grid coordinates.
 template <int ND>
 __device__ __forceinline__ void mf_stencil_block(const double* __restrict__ kc, const double* __restrict__ es,
                                                  const int32_t* __restrict__ free_of, const Grid& g, int s,
-                                                 int64_t node, double (&v)[ND * ND]) {
+                                                 int64_t node, const int64_t (&c)[3], double (&v)[ND * ND]) {
     constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
-    int64_t c[3];
-    coords_of<ND>(node, g, c);
     int d[3];
     offset_of<ND>(s, d[0], d[1], d[2]);
     const bool in = c[0] + d[0] >= 0 && c[0] + d[0] < g.n0 && c[1] + d[1] >= 0 && c[1] + d[1] < g.n1 &&
@@ -210,7 +208,7 @@ template <int ND>
 struct StencilMem {
     const double* st;
     int64_t nn;
-    __device__ __forceinline__ void load(int s, int64_t node, double (&v)[ND * ND]) const {
+    __device__ __forceinline__ void load(int s, int64_t node, const int64_t (&)[3], double (&v)[ND * ND]) const {
         const double* src = st + (int64_t)s * ND * ND * nn + node;
 #pragma unroll
         for (int m = 0; m < ND * ND; ++m) v[m] = src[(int64_t)m * nn];
@@ -223,8 +221,8 @@ struct StencilMf {
     const double* es;
     const int32_t* free_of;
     Grid g;
-    __device__ __forceinline__ void load(int s, int64_t node, double (&v)[ND * ND]) const {
-        mf_stencil_block<ND>(kc, es, free_of, g, s, node, v);
+    __device__ __forceinline__ void load(int s, int64_t node, const int64_t (&c)[3], double (&v)[ND * ND]) const {
+        mf_stencil_block<ND>(kc, es, free_of, g, s, node, c, v);
     }
 };
 
@@ -375,7 +373,7 @@ __global__ __launch_bounds__(kThreads) void galerkin_line_kernel(SRC in, Grid gi
                         double wb;
                         if (!child(c[d] + k - 1, kb, e, jj, wb) || jj != jb) continue;
                         if (!loaded) {
-                            in.load(src_slot, src_node, v);
+                            in.load(src_slot, src_node, ic, v);
                             loaded = true;
                         }
                         const double w = wa * wb;
