@@ -223,14 +223,20 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
     constexpr int EPB = elem_per_block<NEN>();
     constexpr int B2 = NDOF * NDOF;
     constexpr int KE = NEN * NEN * B2;
+    // the Gauss-point data (inverse Jacobians, weights, dN/dX) and, once the
+    // blocks are formed, the blocks themselves share one LDS region
+    constexpr int NINV = EPB * NGP * NDIM * NDIM, NW = EPB * NGP, ND_ = EPB * NGP * NEN * NDIM;
+    constexpr int NKE = EPB * NPS * B2;
+    constexpr int NPOOL = NINV + NW + ND_ > NKE ? NINV + NW + ND_ : NKE;
     __shared__ double sx[EPB][NEN][NDIM];
     __shared__ int svalid[EPB];
-    __shared__ double sinv[EPB][NGP][NDIM * NDIM];
-    __shared__ double sw[EPB][NGP];
-    __shared__ double sd[EPB][NGP][NEN][NDIM];
+    __shared__ double pool[NPOOL];
     __shared__ double swmax[4];
-    __shared__ double ske[EPB * NPS * B2];  // the computed blocks, (element, a <= b)
-    __shared__ int16_t stab[KE];            // COO entry of an element -> its value in ske
+    __shared__ int16_t stab[KE];  // COO entry of an element -> its value in ske
+    auto sinv = reinterpret_cast<double (*)[NGP][NDIM * NDIM]>(pool);
+    auto sw = reinterpret_cast<double (*)[NGP]>(pool + NINV);
+    auto sd = reinterpret_cast<double (*)[NGP][NEN][NDIM]>(pool + NINV + NW);
+    double* const ske = pool;  // the computed blocks, (element, a <= b), after the Gauss-point data
 
     const int tid = threadIdx.x;
     const int64_t e0 = (int64_t)blockIdx.x * EPB;
@@ -305,11 +311,13 @@ __global__ __launch_bounds__(256) void element_kernel(const double* __restrict__
                 elem_block_gp<NDIM>(A, B, w, D11, D12, D44, acc);
             }
         }
+    }
+    __syncthreads();  // every thread is past its Gauss-point reads
+    if (busy)
 #pragma unroll
         for (int i = 0; i < NDOF; ++i)
 #pragma unroll
             for (int j = 0; j < NDOF; ++j) ske[tid * B2 + i * NDOF + j] = acc[i][j];
-    }
     __syncthreads();
     if (chk.ke0) {
         // s_e = Ke(e)[0][0] / Ke(0)[0][0]; |Ke(e) (1 / s_e) - Ke(0)| rounded
