@@ -886,11 +886,22 @@ __device__ __forceinline__ void mf_block_rows(const MfOp<T>& op, const T* kc, co
 #pragma unroll
             for (int s = 0; s < 9; ++s) {
                 const uint32_t q = HOIST ? rn[HOIST ? r : 0][pz][s] : row_at(r, pz, s);
+                if constexpr (sizeof(VT) == 4 && BS == 3) {  // the row's 3 values in one load
+                    const float3 w = *(const float3*)(vb + q);
+                    vs[r][s][0] = (T)w.x;
+                    vs[r][s][1] = (T)w.y;
+                    vs[r][s][2] = (T)w.z;
+                } else if constexpr (sizeof(VT) == 4 && BS == 2) {
+                    const float2 w = *(const float2*)(vb + q);
+                    vs[r][s][0] = (T)w.x;
+                    vs[r][s][1] = (T)w.y;
+                } else {
 #pragma unroll
-                for (int j = 0; j < BS; ++j) {
-                    const uint32_t o = sizeof(VT) == 8 ? (q * BS + j) * (uint32_t)sizeof(VT)
-                                                       : q + j * (uint32_t)sizeof(VT);
-                    vs[r][s][j] = (T)*(const VT*)(vb + o);
+                    for (int j = 0; j < BS; ++j) {
+                        const uint32_t o = sizeof(VT) == 8 ? (q * BS + j) * (uint32_t)sizeof(VT)
+                                                           : q + j * (uint32_t)sizeof(VT);
+                        vs[r][s][j] = (T)*(const VT*)(vb + o);
+                    }
                 }
             }
 #pragma unroll
