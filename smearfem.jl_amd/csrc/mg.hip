@@ -1225,9 +1225,9 @@ struct PowState {
 };
 
 __device__ __forceinline__ double sum_parts(const double* partials, int nparts, double* sh) {
-    double s = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kThreads) s += ld_coherent(partials + i);
-    return block_sum_256(s, sh);
+    double s[1];
+    thread_partials<1>(partials, nparts, s);
+    return block_sum_256(s[0], sh);
 }
 
 __device__ __forceinline__ void pow_finish(PowState* ps, const double* partials, double* sh) {
@@ -1504,13 +1504,10 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
         st_coherent(partials + 2 * blockIdx.x + 1, b1);
     }
     if (last_block_arrive(&st->counter)) {
-        double s0v = 0.0, s1v = 0.0;
-        for (int i = threadIdx.x; i < (int)gridDim.x; i += kThreads) {
-            s0v += ld_coherent(partials + 2 * i);
-            s1v += ld_coherent(partials + 2 * i + 1);
-        }
-        const double trr = block_sum_256(s0v, sh);
-        const double tff = block_sum_256(s1v, sh + 4);
+        double sv[2];
+        thread_partials<2>(partials, (int)gridDim.x, sv);
+        const double trr = block_sum_256(sv[0], sh);
+        const double tff = block_sum_256(sv[1], sh + 4);
         if (threadIdx.x == 0) {
             st->rr_true = trr;
             st->rr = trr;
@@ -1570,30 +1567,6 @@ __global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restric
     rz_finish(st, partials, sh, s);
 }
 
-// The V-cycle's last finest-level smoothing step (matrix-free) with r.z of
-// the outer CG: each row adds r_outer . z for the z it has just written, so
-// the separate pass over r and z goes.  Every thread reaches the reduction.
-template <int ND>
-__global__ __launch_bounds__(kThreads) void cheb_mf_rz_kernel(MfOp<float> op, const float* v, float* __restrict__ r,
-                                                              float* __restrict__ x, const float* d_in,
-                                                              float* __restrict__ d_out, float c1, float c2,
-                                                              const float* __restrict__ binv,
-                                                              const double* __restrict__ r_outer, MgState* st,
-                                                              double* __restrict__ partials) {
-    __shared__ double sh[8];
-    if (st->done) return;
-    const int64_t row = logical_block() * kThreads + threadIdx.x;
-    double s = 0.0;
-    if (row < op.nrows) {
-        float out[ND];
-        mf_row<ND, float, float>(op, op.kc, row, v, out);
-        cheb_epilogue<ND, 0>(row, op.nrows, out, r, x, d_in, d_out, c1, c2, binv);
-#pragma unroll
-        for (int j = 0; j < ND; ++j) s = fma(r_outer[row * ND + j], (double)x[row * ND + j], s);
-    }
-    rz_finish(st, partials, sh, s);
-}
-
 // ------------------------------- matrix-free finest level on MFMA (3D) --
 // K_ff v as sum over cells of s_e Ke(0) v_e, formed cell by cell on the
 // matrix cores.  A workgroup owns a tile of kTX x kTY x kTZ grid nodes:
@@ -1629,18 +1602,16 @@ struct MfTiles {
     int64_t tx, ty, tz;  // tiles per direction
 };
 
-template <int MODE, bool RZ>
+template <int MODE>
 __global__ __launch_bounds__(kThreads) void cheb_mf_mfma_kernel(
     MfOp<float> op, MfTiles T, const half4_t* __restrict__ kfrag, float kscale, const float* v,
     float* __restrict__ r, float* __restrict__ x, const float* d_in, float* __restrict__ d_out, float c1, float c2,
-    const float* __restrict__ binv, const int* done, const double* __restrict__ r_outer, MgState* st,
-    double* __restrict__ partials) {
+    const float* __restrict__ binv, const int* done) {
     __shared__ _Float16 xh[kHN * 3], xl[kHN * 3];
     __shared__ float zc[kCN * kZS];
     __shared__ float scell[kCN];
     __shared__ float wmax[kThreads / 64];
-    __shared__ double sh[8];
-    if (RZ ? st->done != 0 : *done != 0) return;  // uniform over the grid
+    if (*done != 0) return;  // uniform over the grid
     const int tid = threadIdx.x;
     const int64_t t = logical_block();
     const bool live = t < T.tx * T.ty * T.tz;
@@ -1659,7 +1630,6 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_mfma_kernel(
     if (live && gx < T.n0 && gy < T.n1 && gz < T.n2) row = op.rows[(gx + 1) + P0 * ((gy + 1) + P1 * (gz + 1))];
     float rv[3] = {0, 0, 0}, xv[3] = {0, 0, 0}, dv[3] = {0, 0, 0}, own[3] = {0, 0, 0}, bv[9] = {};
     uint8_t fx[3] = {0, 0, 0};
-    double ro[3] = {0, 0, 0};
     if (row >= 0) {
         const int64_t r3 = (int64_t)row * 3;
 #pragma unroll
@@ -1671,7 +1641,6 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_mfma_kernel(
                 xv[j] = x[r3 + j];
                 dv[j] = d_in ? d_in[r3 + j] : 0.0f;
             }
-            if (RZ) ro[j] = r_outer[r3 + j];
         }
         if (MODE == 0)
 #pragma unroll
@@ -1784,12 +1753,9 @@ __global__ __launch_bounds__(kThreads) void cheb_mf_mfma_kernel(
                 for (int i = 0; i < 3; ++i) z = fmaf(bv[j * 3 + i], res[i], z);
                 const float dn = fmaf(c2, z, d_in ? c1 * dv[j] : 0.0f);
                 d_out[r3 + j] = dn;
-                const float xn = xv[j] + dn;
-                x[r3 + j] = xn;
-                if (RZ) s = fma(ro[j], (double)xn, s);
+                x[r3 + j] = xv[j] + dn;
             }
     }
-    if (RZ) rz_finish(st, partials, sh, s);
 }
 
 __global__ __launch_bounds__(kThreads) void mg_p_kernel(double* __restrict__ p, const float* __restrict__ z, int64_t n,
@@ -2484,38 +2450,22 @@ struct MgRun {
     bool nt;
     std::vector<hipEvent_t>* ev = nullptr;  // non-null: time the finest-level smoothing steps
     size_t ev_next = 0;
-    // non-null: the last finest-level step also forms the outer r.z
-    MgState* rz_st = nullptr;
-    const double* rz_r = nullptr;
-    double* rz_partials = nullptr;
-    void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2,
-              bool last = false) {
+    void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2) {
         MgLevel& L = H.L[(size_t)l];
-        const bool fused = l == 0 && H.mf && last && rz_st;
-        // timed: the plain finest-level smoothing steps
-        const bool timed = ev && l == 0 && mode == 0 && !fused && 2 * ev_next + 1 < ev->size();
+        // timed: the finest-level smoothing steps
+        const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
         if (l == 0 && H.mf && H.mfma) {  // the matrix cores (3D)
             const MfTiles T = mf_tiles(H);
             const dim3 grid((unsigned)((T.tx * T.ty * T.tz + 7) / 8 * 8));
             const half4_t* kf = (const half4_t*)H.kfrag.get();
             const float* bi = (const float*)S->binv32.get();
-            if (fused)
-                hipLaunchKernelGGL((cheb_mf_mfma_kernel<0, true>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
-                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, rz_r, rz_st,
-                                   rz_partials);
-            else if (mode == 0)
-                hipLaunchKernelGGL((cheb_mf_mfma_kernel<0, false>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
-                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, nullptr,
-                                   nullptr, nullptr);
+            if (mode == 0)
+                hipLaunchKernelGGL((cheb_mf_mfma_kernel<0>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
+                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done);
             else
-                hipLaunchKernelGGL((cheb_mf_mfma_kernel<1, false>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
-                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done, nullptr,
-                                   nullptr, nullptr);
-        } else if (fused) {
-            hipLaunchKernelGGL(cheb_mf_rz_kernel<ND>, dim3(xcd_rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H),
-                               v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, (const float*)S->binv32.get(), rz_r,
-                               rz_st, rz_partials);
+                hipLaunchKernelGGL((cheb_mf_mfma_kernel<1>), grid, dim3(kThreads), 0, s, mf_op32(S, H), T, kf,
+                                   H.kscale, v, L.r.get(), L.x.get(), d_in, d_out, c1, c2, bi, done);
         } else if (l == 0 && H.mf) {
             auto go = [&](auto k) {
                 hipLaunchKernelGGL(k, dim3(xcd_rows_grid(L.nrows)), dim3(kThreads), 0, s, mf_op32(S, H), v, L.r.get(),
@@ -2602,11 +2552,11 @@ struct MgRun {
                            l == 0 ? (const int32_t*)H.node_of_row.get() : nullptr, L.nrows,
                            (const uint8_t*)L.fixed.get(), L.x.get(), L.da.get(), done);
         SFEM_LAUNCH_CHECK();
-        step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0], L.degree == 1);
+        step(l, 0, L.da.get(), nullptr, L.db.get(), 0.0f, L.c2[0]);
         dp = L.db.get();
         dn = L.da.get();
         for (int k = 1; k < L.degree; ++k) {
-            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k], k == L.degree - 1);
+            step(l, 0, dp, dp, dn, L.c1[k], L.c2[k]);
             std::swap(dp, dn);
         }
     }
@@ -2659,11 +2609,6 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     h2d(ctx, dst.get(), &init, sizeof(init));
     MgState* st = dst.get();
     MgRun<ND, TV> run{ctx, S, H, s, ls, &st->done, ctx->tune_nt};
-    if (H.mf) {  // the last finest-level smoothing step also forms r.z
-        run.rz_st = st;
-        run.rz_r = r.get();
-        run.rz_partials = partials.get();
-    }
     const float* z = H.L[0].x.get();
     Vals<TV> vals = outer_vals<TV>(S);
 
@@ -2693,8 +2638,8 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     int64_t enq = 0, counted = 0, timed = 0;
     double smooth_ms = 0.0;
     const int chunk_max = std::max(1, std::min(o->check_every, 8));
-    // plain finest-level smoothing steps per V-cycle (the fused last one untimed)
-    const int per_iter = 2 * H.L[0].degree - 1 - (run.rz_st ? 1 : 0);
+    // finest-level smoothing steps per V-cycle
+    const int per_iter = 2 * H.L[0].degree - 1;
     std::vector<hipEvent_t> ev;
     if (o->time_kernels) {
         ev.resize(2 * (size_t)per_iter * chunk_max);
@@ -2713,11 +2658,9 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         for (int64_t k = 0; k < chunk; ++k) {
             true_resid();
             run.vcycle(0, r.get());
-            if (!run.rz_st) {  // else the V-cycle's last step formed r.z
-                hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
-                                   lv.per_block, st, partials.get());
-                SFEM_LAUNCH_CHECK();
-            }
+            hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
+                               lv.per_block, st, partials.get());
+            SFEM_LAUNCH_CHECK();
             hipLaunchKernelGGL(mg_p_kernel, dim3(lv.grid), dim3(kThreads), 0, s, p.get(), z, n, lv.per_block,
                                (const MgState*)st);
             SFEM_LAUNCH_CHECK();

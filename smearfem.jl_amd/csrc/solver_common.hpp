@@ -58,12 +58,35 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
     return is_last;
 }
 
+// Fixed-order sums of nparts partial W-tuples (stride W) by one block:
+// thread t adds parts t, t + 256, t + 512, ... in that order (component by
+// component), then the block sums.  The coherent loads go out 8 parts at a
+// time (each one a round trip past the L2), the additions stay in order.
+template <int W>
+__device__ __forceinline__ void thread_partials(const double* partials, int nparts, double (&s)[W]) {
+#pragma unroll
+    for (int k = 0; k < W; ++k) s[k] = 0.0;
+    int i = threadIdx.x;
+    for (; i + 7 * kThreads < nparts; i += 8 * kThreads) {
+        double v[8][W];
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < W; ++k) v[u][k] = ld_coherent(partials + (int64_t)W * (i + u * kThreads) + k);
+#pragma unroll
+        for (int u = 0; u < 8; ++u)
+#pragma unroll
+            for (int k = 0; k < W; ++k) s[k] += v[u][k];
+    }
+    for (; i < nparts; i += kThreads)
+#pragma unroll
+        for (int k = 0; k < W; ++k) s[k] += ld_coherent(partials + (int64_t)W * i + k);
+}
+
 // Fixed-order sum of nparts partial quadruples (stride 4) by one block.
 __device__ __forceinline__ void sum_partials4(const double* partials, int nparts, double out[4], double* sh) {
-    double s[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int i = threadIdx.x; i < nparts; i += kThreads)
-#pragma unroll
-        for (int k = 0; k < 4; ++k) s[k] += ld_coherent(partials + 4 * i + k);
+    double s[4];
+    thread_partials<4>(partials, nparts, s);
     out[0] = block_sum_256(s[0], sh);
     out[1] = block_sum_256(s[1], sh + 4);
     out[2] = block_sum_256(s[2], sh);
@@ -73,15 +96,11 @@ __device__ __forceinline__ void sum_partials4(const double* partials, int nparts
 // Fixed-order sum of nparts partial triples (stride 3) by one block.
 __device__ __forceinline__ void sum_partials3(const double* partials, int nparts, double* a, double* b, double* c,
                                               double* sh) {
-    double s0 = 0.0, s1 = 0.0, s2 = 0.0;
-    for (int i = threadIdx.x; i < nparts; i += kThreads) {
-        s0 += ld_coherent(partials + 3 * i);
-        s1 += ld_coherent(partials + 3 * i + 1);
-        s2 += ld_coherent(partials + 3 * i + 2);
-    }
-    *a = block_sum_256(s0, sh);
-    *b = block_sum_256(s1, sh + 4);
-    *c = block_sum_256(s2, sh);
+    double s[3];
+    thread_partials<3>(partials, nparts, s);
+    *a = block_sum_256(s[0], sh);
+    *b = block_sum_256(s[1], sh + 4);
+    *c = block_sum_256(s[2], sh);
 }
 
 // Logical block id: consecutive logical blocks land on the same XCD (the
