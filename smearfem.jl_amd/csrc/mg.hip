@@ -611,8 +611,15 @@ __device__ __forceinline__ void stencil_leaf(const char* __restrict__ val, const
     float av[B2], xv[BS];
 #pragma unroll
     for (int k = 0; k < B2; ++k) av[k] = (float)*(const TV*)(val + (so + (uint32_t)k * nb));
+    if constexpr (BS == 3) {  // the neighbour's 3 values in one load
+        const float3 w = *(const float3*)(v + mo);
+        xv[0] = w.x;
+        xv[1] = w.y;
+        xv[2] = w.z;
+    } else {
 #pragma unroll
-    for (int i = 0; i < BS; ++i) xv[i] = *(const float*)(v + (mo + (uint32_t)(i * sizeof(float))));
+        for (int i = 0; i < BS; ++i) xv[i] = *(const float*)(v + (mo + (uint32_t)(i * sizeof(float))));
+    }
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
         float a = 0.0f;
