@@ -60,21 +60,24 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
 
 // Fixed-order sums of nparts partial W-tuples (stride W) by one block:
 // thread t adds parts t, t + 256, t + 512, ... in that order (component by
-// component), then the block sums.  The coherent loads go out 8 parts at a
-// time (each one a round trip past the L2), the additions stay in order.
+// component), then the block sums.  The coherent loads go out U parts at a
+// time (each one a round trip past the L2; at most 8 values in flight, so
+// the kernels that end in this reduction keep their register budget), the
+// additions stay in order.
 template <int W>
 __device__ __forceinline__ void thread_partials(const double* partials, int nparts, double (&s)[W]) {
+    constexpr int U = W >= 8 ? 1 : 8 / W;
 #pragma unroll
     for (int k = 0; k < W; ++k) s[k] = 0.0;
     int i = threadIdx.x;
-    for (; i + 7 * kThreads < nparts; i += 8 * kThreads) {
-        double v[8][W];
+    for (; i + (U - 1) * kThreads < nparts; i += U * kThreads) {
+        double v[U][W];
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < W; ++k) v[u][k] = ld_coherent(partials + (int64_t)W * (i + u * kThreads) + k);
 #pragma unroll
-        for (int u = 0; u < 8; ++u)
+        for (int u = 0; u < U; ++u)
 #pragma unroll
             for (int k = 0; k < W; ++k) s[k] += v[u][k];
     }
