@@ -2457,10 +2457,11 @@ struct MgRun {
     bool nt;
     std::vector<hipEvent_t>* ev = nullptr;  // non-null: time the finest-level smoothing steps
     size_t ev_next = 0;
+    bool time_vc = false;  // time this V-cycle's steps
     void step(int l, int mode, const float* v, const float* d_in, float* d_out, float c1, float c2) {
         MgLevel& L = H.L[(size_t)l];
         // timed: the finest-level smoothing steps
-        const bool timed = ev && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
+        const bool timed = ev && time_vc && l == 0 && mode == 0 && 2 * ev_next + 1 < ev->size();
         if (timed) SFEM_HIP(hipEventRecord((*ev)[2 * ev_next], s));
         if (l == 0 && H.mf && H.mfma) {  // the matrix cores (3D)
             const MfTiles T = mf_tiles(H);
@@ -2645,11 +2646,13 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
     int64_t enq = 0, counted = 0, timed = 0;
     double smooth_ms = 0.0;
     const int chunk_max = std::max(1, std::min(o->check_every, 8));
-    // finest-level smoothing steps per V-cycle
+    // finest-level smoothing steps per V-cycle; with time_kernels those of
+    // every third V-cycle are timed (each timed launch adds two timestamps)
     const int per_iter = 2 * H.L[0].degree - 1;
+    constexpr int kTimeEvery = 3;
     std::vector<hipEvent_t> ev;
     if (o->time_kernels) {
-        ev.resize(2 * (size_t)per_iter * chunk_max);
+        ev.resize(2 * (size_t)per_iter * ((chunk_max + kTimeEvery - 1) / kTimeEvery));
         for (auto& e : ev) SFEM_HIP(hipEventCreate(&e));
         run.ev = &ev;
     }
@@ -2664,6 +2667,7 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         run.ev_next = 0;
         for (int64_t k = 0; k < chunk; ++k) {
             true_resid();
+            run.time_vc = k % kTimeEvery == 0;
             run.vcycle(0, r.get());
             hipLaunchKernelGGL(mg_rz_kernel, dim3(lv.grid), dim3(kThreads), 0, s, (const double*)r.get(), z, n,
                                lv.per_block, st, partials.get());
@@ -2691,7 +2695,8 @@ void mg_pcg(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_solve_opts
         if (o->time_kernels) {
             // V-cycles after the stop exit at once; time those of real iterations
             const int64_t real = std::min<int64_t>(chunk, std::max<int64_t>(0, host.iters - counted));
-            for (int64_t k = 0; k < real * per_iter && k < (int64_t)run.ev_next; ++k) {
+            const int64_t real_timed = (real + kTimeEvery - 1) / kTimeEvery;
+            for (int64_t k = 0; k < real_timed * per_iter && k < (int64_t)run.ev_next; ++k) {
                 float ms = 0;
                 SFEM_HIP(hipEventElapsedTime(&ms, ev[2 * k], ev[2 * k + 1]));
                 smooth_ms += ms;
