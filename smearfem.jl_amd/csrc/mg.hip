@@ -1374,6 +1374,7 @@ __global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restr
     __shared__ double sh[8];
     const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
     double s = 0.0;
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(rhs[i], rhs[i], s);
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
@@ -1570,6 +1571,7 @@ __global__ __launch_bounds__(kThreads) void mg_rz_kernel(const double* __restric
     if (st->done) return;
     const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
     double s = 0.0;
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(r[i], (double)z[i], s);
     rz_finish(st, partials, sh, s);
 }
@@ -1770,6 +1772,7 @@ __global__ __launch_bounds__(kThreads) void mg_p_kernel(double* __restrict__ p, 
     if (st->done) return;
     const double beta = st->beta;
     const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) p[i] = fma(beta, p[i], (double)z[i]);
 }
 
@@ -1857,6 +1860,7 @@ __global__ __launch_bounds__(kThreads) void mg_update_kernel(double* __restrict_
     const double a = st->alpha;
     const int64_t i0 = (int64_t)blockIdx.x * per_block, i1 = min(n, i0 + per_block);
     double rr = 0.0;
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
         x[i] = fma(a, p[i], x[i]);
         const double ri = fma(-a, q[i], r[i]);

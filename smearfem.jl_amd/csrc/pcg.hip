@@ -52,6 +52,7 @@ __global__ __launch_bounds__(kThreads) void p_update_kernel(T* __restrict__ p, c
     const T beta = (T)st->beta, zc = (T)st->zcoef;
     const int64_t i0 = (int64_t)blockIdx.x * per_block;
     const int64_t i1 = min(n, i0 + per_block);
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) p[i] = fmaT<T>(beta, p[i], zc * z[i]);
 }
 
@@ -178,6 +179,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
     const int64_t i0 = (int64_t)blockIdx.x * per_block;
     const int64_t i1 = min(n, i0 + per_block);
     double rz = 0.0, rr = 0.0;
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) {
         T xi = x[i];
         if (fold) {
@@ -426,6 +428,7 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double* __restric
     const int64_t i0 = (int64_t)blockIdx.x * per_block;
     const int64_t i1 = min(n, i0 + per_block);
     double s = 0.0;
+#pragma unroll 4  // four strides of loads in flight; the same order of operations
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s += v[i] * v[i];
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) {
