@@ -1227,8 +1227,7 @@ __global__ __launch_bounds__(kThreads) void dense_apply_kernel(const double* __r
 struct PowState {
     double lam;  // |B^-1 A v| for the last unit vector v
     double inv;  // 1 / |w| of the last step (scale of the next input)
-    unsigned counter;
-    unsigned pad;
+    Arrivals arrive;
 };
 
 __device__ __forceinline__ double sum_parts(const double* partials, int nparts, double* sh) {
@@ -1243,7 +1242,6 @@ __device__ __forceinline__ void pow_finish(PowState* ps, const double* partials,
         const double nrm = sqrt(s);
         ps->lam = nrm;
         ps->inv = nrm > 0.0 ? 1.0 / nrm : 0.0;
-        ps->counter = 0;
     }
 }
 
@@ -1270,7 +1268,7 @@ __global__ __launch_bounds__(kThreads) void pow_init_kernel(int64_t n, const uin
     }
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+    if (last_block_arrive(&ps->arrive)) pow_finish(ps, partials, sh);
 }
 
 template <int BS>
@@ -1309,7 +1307,7 @@ __global__ __launch_bounds__(kThreads) void pow_sell_kernel(SellOp<TV> op, const
     }
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+    if (last_block_arrive(&ps->arrive)) pow_finish(ps, partials, sh);
 }
 
 template <int ND, bool GROUP, typename TV>
@@ -1333,7 +1331,7 @@ __global__ __launch_bounds__(kThreads, 4) void pow_stencil_kernel(StOp<TV> op, c
     }
     const double b = block_sum_256(sum, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+    if (last_block_arrive(&ps->arrive)) pow_finish(ps, partials, sh);
 }
 
 template <int ND>
@@ -1351,7 +1349,7 @@ __global__ __launch_bounds__(kThreads) void pow_mf_kernel(MfOp<float> op, const 
     }
     const double b = block_sum_256(sum, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&ps->counter)) pow_finish(ps, partials, sh);
+    if (last_block_arrive(&ps->arrive)) pow_finish(ps, partials, sh);
 }
 
 // ------------------------------------------------------- outer CG ----
@@ -1364,8 +1362,7 @@ struct MgState {
     int restart;    // next direction is z (after a residual replacement)
     int need_true;  // the recurrence claims convergence: check the FP64 residual
     int stall;
-    unsigned counter;
-    unsigned pad;
+    Arrivals arrive;
 };
 
 __global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restrict__ rhs, int64_t n,
@@ -1378,7 +1375,7 @@ __global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restr
     for (int64_t i = i0 + threadIdx.x; i < i1; i += kThreads) s = fma(rhs[i], rhs[i], s);
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         const double bb = sum_parts(partials, gridDim.x, sh);
         if (threadIdx.x == 0) {
             st->bb = bb;
@@ -1387,7 +1384,6 @@ __global__ __launch_bounds__(kThreads) void mg_norm_kernel(const double* __restr
                 st->done = 1;
                 st->rr_true = 0.0;
             }
-            st->counter = 0;
         }
     }
 }
@@ -1511,7 +1507,7 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
         st_coherent(partials + 2 * blockIdx.x, b0);
         st_coherent(partials + 2 * blockIdx.x + 1, b1);
     }
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         double sv[2];
         thread_partials<2>(partials, (int)gridDim.x, sv);
         const double trr = block_sum_256(sv[0], sh);
@@ -1538,7 +1534,6 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
                     if (trr <= st->floor_rr || st->stall >= 3) st->done = 3;
                 }
             }
-            st->counter = 0;
         }
     }
 }
@@ -1548,7 +1543,7 @@ __device__ void true_finish(MgState* st, double* __restrict__ partials, double* 
 __device__ void rz_finish(MgState* st, double* __restrict__ partials, double* sh, double s) {
     const double b = block_sum_256(s, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         const double rz = sum_parts(partials, gridDim.x, sh);
         if (threadIdx.x == 0) {
             if (!(rz > 0.0) || !isfinite(rz)) {
@@ -1559,7 +1554,6 @@ __device__ void rz_finish(MgState* st, double* __restrict__ partials, double* sh
             }
             st->rz = rz;
             st->restart = 0;
-            st->counter = 0;
         }
     }
 }
@@ -1806,12 +1800,11 @@ __global__ __launch_bounds__(kThreads) void mg_spmv_kernel(const int64_t* __rest
     }
     const double b = block_sum_256(dot, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         const double pq = sum_parts(partials, gridDim.x, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz_old / pq;
             if (!(pq > 0.0) || !isfinite(st->alpha)) st->done = 2;
-            st->counter = 0;
         }
     }
 }
@@ -1838,12 +1831,11 @@ __global__ __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(4))) v
     }
     const double b = block_sum_256(dot, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         const double pq = sum_parts(partials, gridDim.x, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz_old / pq;
             if (!(pq > 0.0) || !isfinite(st->alpha)) st->done = 2;
-            st->counter = 0;
         }
     }
 }
@@ -1869,14 +1861,13 @@ __global__ __launch_bounds__(kThreads) void mg_update_kernel(double* __restrict_
     }
     const double b = block_sum_256(rr, sh);
     if (threadIdx.x == 0) st_coherent(partials + blockIdx.x, b);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         const double trr = sum_parts(partials, gridDim.x, sh);
         if (threadIdx.x == 0) {
             st->rr = trr;
             st->iters += 1;
             if (trr <= st->tol2bb) st->need_true = 1;
             if (!isfinite(trr)) st->done = 2;
-            st->counter = 0;
         }
     }
 }

@@ -58,6 +58,36 @@ __device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
     return is_last;
 }
 
+// Arrival counters spread over 8 lines: block b arrives on counter b % 8,
+// the last block of each of those groups then on the top counter.  One
+// agent-scope counter for every block serialises the arrivals (about 12 ns
+// each on MI355X, 50 us for 4,096 blocks); spread over 8 counters 256 bytes
+// apart they cost about 1.3 ns each.  The last block resets every counter.
+struct Arrivals {
+    unsigned c[9 * 64];
+};
+
+__device__ __forceinline__ bool last_block_arrive(Arrivals* a) {
+    __shared__ int is_last;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        wait_vm_complete();
+        const unsigned G = gridDim.x, g = blockIdx.x % 8u;
+        const unsigned groups = G < 8u ? G : 8u, in_group = (G - g + 7u) / 8u;
+        bool last = __hip_atomic_fetch_add(&a->c[64 * g], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                    in_group - 1u;
+        if (last)
+            last = __hip_atomic_fetch_add(&a->c[64 * 8], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) ==
+                   groups - 1u;
+        if (last)
+            for (int k = 0; k < 9; ++k)
+                __hip_atomic_store(&a->c[64 * k], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = last;
+    }
+    __syncthreads();
+    return is_last;
+}
+
 // Fixed-order sums of nparts partial W-tuples (stride W) by one block:
 // thread t adds parts t, t + 256, t + 512, ... in that order (component by
 // component), then the block sums.  The coherent loads go out U parts at a
