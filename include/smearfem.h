@@ -241,7 +241,7 @@ int  sfem_system_solve(sfem_ctx* ctx, sfem_system* sys, const sfem_solve_opts* o
  * callers that cannot allocate device memory (code under HIP graph capture,
  * tight serving loops, harnesses that forbid hipMalloc).  The CG vectors,
  * partial sums and state live in one file-scope device array per GPU of
- * 40 MiB (held until the process ends), so the system's solver vectors may
+ * 40.5 MiB (held until the process ends), so the system's solver vectors may
  * have at most 2^20 = 1,048,576 entries (free dofs; with partially fixed
  * nodes, node rows x n_dof), else SFEM_ERR_UNSUPPORTED.  Point or block
  * Jacobi, any precision; multigrid is refused (its V-cycle allocates).  The

@@ -81,13 +81,12 @@ __global__ __launch_bounds__(kThreads) void csr_spmv_cg_kernel(const int32_t* __
     }
     const double bsum = block_sum_256(dot, sh);
     if (threadIdx.x == 0) st_coherent(partials + 3 * blockIdx.x, bsum);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         double pq, u1, u2;
         sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz / pq;
             if (!(pq > 0.0) || !std::isfinite(st->alpha)) st->done = 2;  // breakdown
-            st->counter = 0;
         }
     }
 }
@@ -119,13 +118,12 @@ __global__ __launch_bounds__(kThreads) void sell_spmv_cg_kernel(
     }
     const double bsum = block_sum_256(dot, sh);
     if (threadIdx.x == 0) st_coherent(partials + 3 * blockIdx.x, bsum);
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         double pq, u1, u2;
         sum_partials3(partials, gridDim.x, &pq, &u1, &u2, sh);
         if (threadIdx.x == 0) {
             st->alpha = st->rz / pq;
             if (!(pq > 0.0) || !std::isfinite(st->alpha)) st->done = 2;
-            st->counter = 0;
         }
     }
 }
@@ -157,7 +155,6 @@ __device__ __forceinline__ void cg_update_finish(CgState* st, double* partials, 
             st->done = 3;
         }
         if (!std::isfinite(trr)) st->done = 2;
-        st->counter = 0;
     }
 }
 
@@ -200,7 +197,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_kernel(T* __restrict__ x, 
         st_coherent(partials + 3 * blockIdx.x, b0);
         st_coherent(partials + 3 * blockIdx.x + 1, b1);
     }
-    if (last_block_arrive(&st->counter)) cg_update_finish(st, partials, sh);
+    if (last_block_arrive(&st->arrive)) cg_update_finish(st, partials, sh);
 }
 
 // Node-block Jacobi form of cg_update: one thread per node; z = B^-1 r with
@@ -249,7 +246,7 @@ __global__ __launch_bounds__(kThreads) void cg_update_block_kernel(T* __restrict
         st_coherent(partials + 3 * blockIdx.x, b0);
         st_coherent(partials + 3 * blockIdx.x + 1, b1);
     }
-    if (last_block_arrive(&st->counter)) cg_update_finish(st, partials, sh);
+    if (last_block_arrive(&st->arrive)) cg_update_finish(st, partials, sh);
 }
 
 // ------------------------------------------- FP64 residual replacement --
@@ -309,7 +306,6 @@ __device__ __forceinline__ void resid_finish(CgState* st, double* partials, doub
         }
         if (trr < st->best_true) st->best_true = trr;
         if (!std::isfinite(trr)) st->done = 2;
-        st->counter = 0;
     }
 }
 
@@ -355,7 +351,7 @@ __global__ __launch_bounds__(kThreads) void csr_resid_kernel(const int32_t* __re
         st_coherent(partials + 4 * blockIdx.x + 2, b2);
         st_coherent(partials + 4 * blockIdx.x + 3, b3);
     }
-    if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
+    if (last_block_arrive(&st->arrive)) resid_finish<FOLD>(st, partials, sh);
 }
 
 // BPC: z = B^-1 r with the node's inverse diagonal block (binv, planar with
@@ -417,7 +413,7 @@ __global__ __launch_bounds__(kThreads) void sell_resid_kernel(
         st_coherent(partials + 4 * blockIdx.x + 2, b2);
         st_coherent(partials + 4 * blockIdx.x + 3, b3);
     }
-    if (last_block_arrive(&st->counter)) resid_finish<FOLD>(st, partials, sh);
+    if (last_block_arrive(&st->arrive)) resid_finish<FOLD>(st, partials, sh);
 }
 
 // bb = rhs.rhs (FP64), once per solve
@@ -436,7 +432,7 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double* __restric
         st_coherent(partials + 3 * blockIdx.x + 1, 0.0);
         st_coherent(partials + 3 * blockIdx.x + 2, 0.0);
     }
-    if (last_block_arrive(&st->counter)) {
+    if (last_block_arrive(&st->arrive)) {
         double bb, u1, u2;
         sum_partials3(partials, gridDim.x, &bb, &u1, &u2, sh);
         if (threadIdx.x == 0) {
@@ -444,7 +440,6 @@ __global__ __launch_bounds__(kThreads) void norm2_kernel(const double* __restric
             st->tol2bb = rtol * rtol * bb;
             st->delta2 = delta * delta;
             st->maxrr = bb;
-            st->counter = 0;
         }
     }
 }
@@ -873,7 +868,7 @@ void run_pcg(sfem_ctx* ctx, sfem_system* S, const sfem_solve_opts* o, sfem_solve
 namespace {
 constexpr size_t kNoAllocVec = sizeof(double) * (size_t)kNoAllocMaxDofs;
 constexpr size_t kNoAllocPartials = 4 * 16384;  // doubles: 4 per block of the largest plan
-constexpr size_t kNoAllocBytes = 5 * kNoAllocVec + sizeof(double) * kNoAllocPartials + 256;
+constexpr size_t kNoAllocBytes = 5 * kNoAllocVec + sizeof(double) * kNoAllocPartials + (sizeof(CgState) + 255) / 256 * 256;
 static __device__ __align__(256) unsigned char scratch[kNoAllocBytes];
 }  // namespace
 

@@ -142,6 +142,14 @@ void upload_csc1(sfem_ctx* ctx, int64_t m, int64_t n, const int64_t* colptr, con
 
 }  // namespace sfem
 
+namespace sfem {
+// Arrival counters of a last-block reduction (last_block_arrive): 8 group
+// counters and a top counter, each on its own 256-byte line.
+struct Arrivals {
+    unsigned c[9 * 64];
+};
+}  // namespace sfem
+
 // Device scalars of one CG solve; updated only by the last block of each
 // reduction, read by every kernel of the next step.
 struct CgState {
@@ -157,8 +165,7 @@ struct CgState {
     long long iters, updates, best_it;
     long long last_upd;  // iteration of the latest replacement
     long long best_true_it;  // iteration of best_true
-    unsigned int counter;
-    unsigned int pad;
+    sfem::Arrivals arrive;
 };
 
 // K_ff and its solver formats, plus the data to recombine the solution.

@@ -44,29 +44,14 @@ __device__ __forceinline__ void wait_vm_complete() { __builtin_amdgcn_s_waitcnt(
 
 // True in every thread of the block that arrives last.  Thread 0 must have
 // written this block's partials with st_coherent before the call; it waits for
-// those stores to complete before the counter increment.  Unlike
+// those stores to complete before its counter increment.  Unlike
 // __threadfence() this does not write back and invalidate the whole L2, which
 // would evict the vector every other block is still gathering from.
-__device__ __forceinline__ bool last_block_arrive(unsigned int* counter) {
-    __shared__ int is_last;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        wait_vm_complete();
-        is_last = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == gridDim.x - 1;
-    }
-    __syncthreads();
-    return is_last;
-}
-
-// Arrival counters spread over 8 lines: block b arrives on counter b % 8,
+// The arrivals are spread over 8 counters: block b arrives on counter b % 8,
 // the last block of each of those groups then on the top counter.  One
 // agent-scope counter for every block serialises the arrivals (about 12 ns
 // each on MI355X, 50 us for 4,096 blocks); spread over 8 counters 256 bytes
 // apart they cost about 1.3 ns each.  The last block resets every counter.
-struct Arrivals {
-    unsigned c[9 * 64];
-};
-
 __device__ __forceinline__ bool last_block_arrive(Arrivals* a) {
     __shared__ int is_last;
     __syncthreads();
