@@ -445,11 +445,10 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     K->ctx = ctx;
     Timer tm(ctx->stream);
     tm.mark();
-    DevBuf<ValidateOut> vout(ctx, 1);
-    {
-        ValidateOut init{~0ull, -1, INT32_MAX, 1, 0};
-        h2d(ctx, vout.get(), &init, sizeof(init));
-    }
+    DevBuf<ValidateOut> vout(ctx, kValidateCopies);
+    ValidateOut vcopies[kValidateCopies];
+    for (ValidateOut& c : vcopies) c = ValidateOut{~0ull, -1, INT32_MAX, 1, 0};
+    h2d(ctx, vout.get(), vcopies, sizeof(vcopies));
     launch_validate(ctx, m, vout.get());
     tm.mark();
     const int nen = m->nen, ndof = m->ndof;
@@ -472,8 +471,8 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         launch_element_stiffness(ctx, m, V.get(), vout.get());
     }
     tm.mark();
-    ValidateOut vr;
-    d2h(ctx, &vr, vout.get(), sizeof(vr));
+    d2h(ctx, vcopies, vout.get(), sizeof(vcopies));
+    const ValidateOut vr = combine_validate(vcopies);
 
     // first error in the reference's order: element-level errors, then the
     // bounds check inside sparse() on the dof ids (ArgumentError)

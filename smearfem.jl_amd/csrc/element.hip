@@ -59,7 +59,7 @@ __global__ __launch_bounds__(256) void validate_kernel(const int32_t* __restrict
             }
         }
     }
-    // wave then block reduction, one atomic per block
+    // wave then block reduction, one atomic per block into its group's copy
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         const unsigned long long k2 = __shfl_xor(key, off, 64);
@@ -78,10 +78,11 @@ __global__ __launch_bounds__(256) void validate_kernel(const int32_t* __restrict
             mn = min(mn, s_min[i]);
             blk = min(blk, s_blk[i]);
         }
-        if (key != ~0ull) atomicMin(&out->err_key, key);
-        if (mx >= 0) atomicMax(&out->max_unit, mx);
-        if (mn != INT32_MAX) atomicMin(&out->min_dof, mn);
-        if (!blk) atomicAnd(&out->blocked, 0);
+        ValidateOut* o = out + blockIdx.x % kValidateCopies;
+        if (key != ~0ull) atomicMin(&o->err_key, key);
+        if (mx >= 0) atomicMax(&o->max_unit, mx);
+        if (mn != INT32_MAX) atomicMin(&o->min_dof, mn);
+        if (!blk) atomicAnd(&o->blocked, 0);
     }
 }
 

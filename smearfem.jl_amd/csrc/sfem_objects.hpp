@@ -90,7 +90,22 @@ struct ValidateOut {
     int blocked;                  // 1 while every used node's ID is ndof*g + l
     int pad;
 };
+// d_out: kValidateCopies records; block b of the validation reduces into copy
+// b % kValidateCopies (one record for every block serialises its atomics),
+// the host combines them (combine_validate).  The element kernels report
+// singular Jacobians into copy 0.
+constexpr int kValidateCopies = 8;
 void launch_validate(sfem_ctx* ctx, const sfem_mesh* m, ValidateOut* d_out);
+inline ValidateOut combine_validate(const ValidateOut* v) {
+    ValidateOut r = v[0];
+    for (int i = 1; i < kValidateCopies; ++i) {
+        r.err_key = v[i].err_key < r.err_key ? v[i].err_key : r.err_key;
+        r.max_unit = v[i].max_unit > r.max_unit ? v[i].max_unit : r.max_unit;
+        r.min_dof = v[i].min_dof < r.min_dof ? v[i].min_dof : r.min_dof;
+        r.blocked = r.blocked && v[i].blocked;
+    }
+    return r;
+}
 void launch_element_stiffness(sfem_ctx* ctx, const sfem_mesh* m, double* V, ValidateOut* d_out);
 // The element-matrix check of a structured-grid assembly, done while the
 // element kernel stores Ke: with ke0 = element 0's Ke (computed first, by
