@@ -140,6 +140,7 @@ __device__ __forceinline__ ACC csr_row(const int32_t* __restrict__ rowptr, const
                                        const TV* __restrict__ val, int64_t row, int lane, Gather gx) {
     ACC s = 0;
     const int32_t e = rowptr[row + 1];
+#pragma unroll 4  // the loads of four strides in flight; the same order of operations
     for (int32_t k = rowptr[row] + lane; k < e; k += G) s = fmaT<ACC>((ACC)val[k], gx(col[k]), s);
 #pragma unroll
     for (int off = G / 2; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
@@ -154,6 +155,7 @@ __device__ __forceinline__ double csr_row_abs(const int32_t* __restrict__ rowptr
                                               double* aout) {
     double s = 0.0, a = 0.0;
     const int32_t e = rowptr[row + 1];
+#pragma unroll 4
     for (int32_t k = rowptr[row] + lane; k < e; k += G) {
         const double v = val[k], x = gx(col[k]);
         s = fma(v, x, s);
