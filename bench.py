@@ -49,7 +49,7 @@ ERR_BAR = 1e-8
 MEASURED_TRAFFIC = {"fp64/bsr": 2.2323e9, "mixed/bsr": 1.1738e9, "split/bsr": 1.7175e9,
                     # finest-level multigrid smoothing step (FP16 operator; matrix-free),
                     # profiles/traffic_mg_smooth_hex8_256x64x64.md
-                    "mg-smooth": 7.4977e8, "mg-smooth-mf": 1.3832e8}
+                    "mg-smooth": 7.4977e8, "mg-smooth-mf": 1.4063e8}
 PEAK_FP32_TFLOPS = 157.3  # MI355X vector FP32, published (counts packed FP32 issue)
 # Algorithmic FLOPs of one matrix-free finest-level smoothing step per block
 # row (Hex8): 8 cells x 8 corners x 3x3 FMA for the cell sums, 8 x 3 FMA for
