@@ -394,6 +394,215 @@ __global__ __launch_bounds__(kThreads) void galerkin_line_kernel(SRC in, Grid gi
     }
 }
 
+// ------------------------- first coarse level from the element scales --
+// With every direction coarsened by 2, a coarse cell E (lower corner Ec)
+// holds the fine elements 2 Ec + u, u in {0, 1}^ND, and P restricted to a
+// fine element interpolates its corners from E's corners alone: along a
+// direction, fine corner k of sub-element u sits at t = (u + k) / 2 of the
+// cell, weight 1 - t for E's lower corner and t for the upper one.  So
+// P'AP = sum over cells E, over their fine elements e, of s_e W_e' Ke(0)_m W_e
+// (Ke(0)_m: Ke(0) with the rows and columns of e's fixed dofs zero).  A cell
+// is regular when it has two fine elements along every direction and no
+// fixed dof among its fine nodes; its elements' W_e' Ke(0) W_e depend on u
+// alone: gt[u][a][b] (a, b: the corners of E, 2^ND x 2^ND blocks; computed
+// from Ke(0) by galerkin_table_kernel).  The last cell of an odd element
+// count (one fine element, the identity weights) and cells with a fixed dof
+// are irregular: their cell matrices are summed first (cell_matrix_kernel),
+// the fine elements with a fixed dof (all, in a one-element cell) from Ke(0)
+// with that element's masked weights.  This replaces the one-direction
+// products and the symmetrising pass for that level.
+
+// gt[((u NEN + a) NEN + b) B2 + m] = sum over e's corners i, then j, of
+// w_a(i) w_b(j) Ke(0)[i][j][m], for m = (r, t) with a's dof r at most b's
+// dof t; the other entries are their transposes' sums (bit for bit
+// symmetric).  One thread per entry.
+template <int ND>
+__global__ void galerkin_table_kernel(const double* __restrict__ kc, double* __restrict__ gt) {
+    constexpr int B2 = ND * ND, NEN = 1 << ND;
+    const int t = (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (t >= NEN * NEN * NEN * B2) return;
+    int m = t % B2, b = (t / B2) % NEN, a = (t / (B2 * NEN)) % NEN;
+    const int u = t / (B2 * NEN * NEN);
+    if (a * ND + m / ND > b * ND + m % ND) {  // summed as the transposed entry (b, a)
+        const int r = m / ND;
+        m = (m % ND) * ND + r;
+        const int x = a;
+        a = b;
+        b = x;
+    }
+    auto w = [u](int c, int k) {  // weight of corner c of E at fine corner k of sub-element u
+        double p = 1.0;
+        for (int d = 0; d < ND; ++d) {
+            const double x = 0.5 * (((u >> d) & 1) + ((k >> d) & 1));
+            p *= ((c >> d) & 1) ? x : 1.0 - x;
+        }
+        return p;
+    };
+    double acc = 0.0;
+    for (int i = 0; i < NEN; ++i)
+        for (int j = 0; j < NEN; ++j) acc = fma(w(a, i) * w(b, j), kc[(i * NEN + j) * B2 + m], acc);
+    gt[t] = acc;
+}
+
+// The irregular coarse cells (see above) numbered in any order: slot[E] =
+// k >= 0 and cells[k] = E for an irregular cell E, slot[E] = -1 otherwise
+template <int ND>
+__global__ void cell_irregular_kernel(const uint8_t* __restrict__ ffix, Grid gf, Grid gc, int32_t* __restrict__ slot,
+                                      int32_t* __restrict__ cells, unsigned* __restrict__ count) {
+    constexpr int BS = ND;
+    const int64_t c0 = gc.n0 - 1, c1 = gc.n1 - 1, c2 = ND == 3 ? gc.n2 - 1 : 1;
+    const int64_t E = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (E >= c0 * c1 * c2) return;
+    const int64_t ec[3] = {E % c0, (E / c0) % c1, ND == 3 ? E / (c0 * c1) : 0};
+    const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
+    bool bad = false;
+    int64_t hi[3] = {0, 0, 0};
+    for (int d = 0; d < ND; ++d) {
+        hi[d] = 2 * ec[d] + 2;
+        if (hi[d] > ef[d]) {  // one fine element
+            bad = true;
+            hi[d] = ef[d];
+        }
+    }
+    for (int64_t z = 2 * ec[2]; z <= hi[2]; ++z)
+        for (int64_t y = 2 * ec[1]; y <= hi[1]; ++y)
+            for (int64_t x = 2 * ec[0]; x <= hi[0]; ++x) {
+                const int64_t fn = node_at(gf, x, y, z);
+                for (int a = 0; a < BS; ++a) bad = bad || ffix[fn * BS + a];
+            }
+    int32_t k = -1;
+    if (bad) {
+        k = (int32_t)atomicAdd(count, 1u);
+        cells[k] = (int32_t)E;
+    }
+    slot[E] = k;
+}
+
+// The cell matrix of each irregular cell, kcell[k][a][b] = sum over its fine
+// elements (u ascending) of s_e T_u(a, b): T_u = gt[u] for an element with
+// no fixed dof in a cell of two fine elements per direction, else
+// sum over i, then j, of (w_a(i) [i's dof free]) (w_b(j) [j's dof free]) Ke(0)[i][j]
+// with the element's own weights (1 - t, t; the identity along a
+// one-element direction).  Each entry (row dof of corner a, column dof of
+// corner b) is summed in the order of the entry whose row dof is the lower
+// one, so kcell[k][a][b] and kcell[k][b][a] are transposes bit for bit.
+// One wavefront per cell, lane (a, b); fine nodes outside a one-element
+// cell are never read.
+template <int ND>
+__global__ __launch_bounds__(64) void cell_matrix_kernel(const double* __restrict__ gt, const double* __restrict__ kc,
+                                                         const double* __restrict__ es,
+                                                         const uint8_t* __restrict__ ffix, Grid gf, Grid gc,
+                                                         const int32_t* __restrict__ cells,
+                                                         double* __restrict__ kcell) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
+    const int k = (int)blockIdx.x;
+    const int lane = (int)threadIdx.x;
+    if (lane >= NEN * NEN) return;
+    const int a = lane / NEN, b = lane % NEN;
+    const int64_t E = cells[k];
+    const int64_t c0 = gc.n0 - 1, c1 = gc.n1 - 1;
+    const int64_t ec[3] = {E % c0, (E / c0) % c1, ND == 3 ? E / (c0 * c1) : 0};
+    const int64_t p0 = gf.n0 + 1, p1 = gf.n1 + 1;  // padded fine cells per direction
+    const int64_t ef[3] = {gf.n0 - 1, gf.n1 - 1, gf.n2 - 1};
+    int ns[3] = {1, 1, 1};  // fine elements along each direction
+    bool two = true;
+#pragma unroll
+    for (int d = 0; d < ND; ++d) {
+        ns[d] = 2 * ec[d] + 2 <= ef[d] ? 2 : 1;
+        two = two && ns[d] == 2;
+    }
+    double acc[B2];
+#pragma unroll
+    for (int m = 0; m < B2; ++m) acc[m] = 0.0;
+#pragma unroll 1
+    for (int u = 0; u < NEN; ++u) {
+        const int ud[3] = {u & 1, (u >> 1) & 1, ND == 3 ? (u >> 2) & 1 : 0};
+        if (ud[0] >= ns[0] || ud[1] >= ns[1] || ud[2] >= ns[2]) continue;
+        int64_t fc[3];  // the fine element's lower corner
+#pragma unroll
+        for (int d = 0; d < 3; ++d) fc[d] = d < ND ? 2 * ec[d] + ud[d] : 0;
+        const double se = es[(fc[0] + 1) + p0 * ((fc[1] + 1) + (ND == 3 ? p1 * (fc[2] + 1) : 0))];
+        uint32_t fx = 0;  // bit n BS + j: corner n's dof j fixed
+#pragma unroll
+        for (int n = 0; n < NEN; ++n) {
+            const int64_t fn =
+                node_at(gf, fc[0] + (n & 1), fc[1] + ((n >> 1) & 1), ND == 3 ? fc[2] + ((n >> 2) & 1) : 0);
+#pragma unroll
+            for (int j = 0; j < BS; ++j) fx |= (ffix[fn * BS + j] ? 1u : 0u) << (n * BS + j);
+        }
+        double T[B2];
+        if (two && !fx) {
+            const double* g = gt + (((int64_t)u * NEN + a) * NEN + b) * B2;
+#pragma unroll
+            for (int m = 0; m < B2; ++m) T[m] = g[m];
+        } else {
+            // corner c's weight at the element's corner n
+            auto wt = [&](int c, int n) {
+                double p = 1.0;
+#pragma unroll
+                for (int d = 0; d < ND; ++d) {
+                    const int nd = (n >> d) & 1, cd = (c >> d) & 1;
+                    const double t = 0.5 * (ud[d] + nd);
+                    p *= ns[d] == 2 ? (cd ? t : 1.0 - t) : (cd == nd ? 1.0 : 0.0);
+                }
+                return p;
+            };
+#pragma unroll
+            for (int m = 0; m < B2; ++m) T[m] = 0.0;
+#pragma unroll 1
+            for (int i = 0; i < NEN; ++i) {
+                const double gai = wt(a, i), gbi = wt(b, i);
+#pragma unroll 1
+                for (int j = 0; j < NEN; ++j) {
+                    const double gaj = wt(a, j), gbj = wt(b, j);
+                    const double* kb = kc + ((int64_t)i * NEN + j) * B2;
+#pragma unroll
+                    for (int r = 0; r < BS; ++r)
+#pragma unroll
+                        for (int t = 0; t < BS; ++t) {
+                            // (a, r) first: w_a(i) w_b(j) Ke[i][j][r][t]; else the
+                            // transposed entry's sum, w_b(i) w_a(j) Ke[i][j][t][r]
+                            const bool lo = a * BS + r <= b * BS + t;
+                            const bool fir = (fx >> (i * BS + r)) & 1u, fit = (fx >> (i * BS + t)) & 1u;
+                            const bool fjr = (fx >> (j * BS + r)) & 1u, fjt = (fx >> (j * BS + t)) & 1u;
+                            const double x = lo ? (fir ? 0.0 : gai) : (fit ? 0.0 : gbi);
+                            const double y = lo ? (fjt ? 0.0 : gbj) : (fjr ? 0.0 : gaj);
+                            T[r * BS + t] = fma(x * y, lo ? kb[r * BS + t] : kb[t * BS + r], T[r * BS + t]);
+                        }
+                }
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < B2; ++m) acc[m] = fma(se, T[m], acc[m]);
+    }
+    double* o = kcell + ((int64_t)k * NEN * NEN + lane) * B2;
+#pragma unroll
+    for (int m = 0; m < B2; ++m) o[m] = acc[m];
+}
+
+// The cells holding coarse node c and its neighbour c + D: q in {0,1}^ND
+// names the cell with lower corner c - 1 + q; it holds the neighbour when
+// its corner b = 1 - q + D (per direction) is 0 or 1, and exists when its
+// lower corner is inside the cell grid.  a = 1 - q: c's corner.
+template <int ND>
+__device__ __forceinline__ bool shared_cell(const Grid& gc, const int64_t (&c)[3], const int (&D)[3], int q,
+                                            int64_t (&ec)[3], int& a, int& b) {
+    a = b = 0;
+    bool ok = true;
+    for (int d = 0; d < 3; ++d) {
+        ec[d] = 0;
+        if (d >= ND) continue;
+        const int qd = (q >> d) & 1;
+        const int bd = 1 - qd + D[d];
+        const int64_t n = d == 0 ? gc.n0 : d == 1 ? gc.n1 : gc.n2;
+        ec[d] = c[d] - 1 + qd;
+        ok = ok && bd >= 0 && bd <= 1 && ec[d] >= 0 && ec[d] <= n - 2;
+        a |= (1 - qd) << d;
+        b |= bd << d;
+    }
+    return ok;
+}
+
 // A coarse smoothing stencil's value (slot s, entry m) of a node: slot-major
 // [(s*B2 + m)*nn + node] on the large levels (one thread per node, loads
 // coalesced along nodes), node-major [(node*NS + s)*B2 + m] on the small ones
@@ -402,6 +611,72 @@ template <int ND>
 __host__ __device__ __forceinline__ int64_t st_index(bool node_major, int64_t nn, int64_t node, int s, int m) {
     constexpr int B2 = ND * ND, NS = Sten<ND>::NS;
     return node_major ? (node * NS + s) * B2 + m : ((int64_t)s * B2 + m) * nn + node;
+}
+
+// The first coarse level's stencil, FP64 (slot-major) and the smoother's
+// FP32 copy, as sym_mask_kernel leaves them.  Thread (coarse node, slot =
+// blockIdx.y): the block sums, over the cells shared with the slot's
+// neighbour (q ascending), a regular cell's fine elements' s_e gt[u][a][b]
+// (u ascending) or an irregular cell's kcell block.  gt and kcell are
+// symmetric bit for bit and both blocks (I, J) and (J, I) meet the same
+// cells in the same order, so the stencil needs no symmetrising.
+template <int ND>
+__global__ __launch_bounds__(kThreads) void galerkin_cells_kernel(
+    const double* __restrict__ gt, const double* __restrict__ kcell, const double* __restrict__ es,
+    const int32_t* __restrict__ slot, Grid gf, Grid gc, const uint8_t* __restrict__ cfix,
+    double* __restrict__ st64, float* __restrict__ st32, bool node_major) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
+    const int s = (int)blockIdx.y;
+    const int64_t nno = gc.n0 * gc.n1 * gc.n2;
+    const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    if (I >= nno) return;
+    int D[3];
+    offset_of<ND>(s, D[0], D[1], D[2]);
+    const int64_t c0 = gc.n0 - 1, c1 = gc.n1 - 1;
+    const int64_t p0 = gf.n0 + 1, p1 = gf.n1 + 1;  // padded fine cells per direction
+    int64_t c[3];
+    coords_of<ND>(I, gc, c);
+    double acc[B2];
+#pragma unroll
+    for (int m = 0; m < B2; ++m) acc[m] = 0.0;
+    // (cells and elements not unrolled: the table blocks are scalar loads,
+    // a few at a time)
+#pragma unroll 1
+    for (int q = 0; q < NEN; ++q) {
+        int64_t ec[3];
+        int a, b;
+        if (!shared_cell<ND>(gc, c, D, q, ec, a, b)) continue;
+        const int32_t k = slot[ec[0] + c0 * (ec[1] + c1 * ec[2])];
+        if (k >= 0) {
+            const double* o = kcell + ((int64_t)k * NEN * NEN + a * NEN + b) * B2;
+#pragma unroll
+            for (int m = 0; m < B2; ++m) acc[m] += o[m];
+            continue;
+        }
+#pragma unroll 2
+        for (int u = 0; u < NEN; ++u) {
+            const int64_t fe = (2 * ec[0] + (u & 1) + 1) +
+                               p0 * ((2 * ec[1] + ((u >> 1) & 1) + 1) +
+                                     (ND == 3 ? p1 * (2 * ec[2] + ((u >> 2) & 1) + 1) : 0));
+            const double se = es[fe];
+            const double* g = gt + (((int64_t)u * NEN + a) * NEN + b) * B2;
+#pragma unroll
+            for (int m = 0; m < B2; ++m) acc[m] = fma(se, g[m], acc[m]);
+        }
+    }
+    // fixed coarse dofs: zero rows and columns, unit diagonal in the FP32 copy
+    const bool valid = inside<ND>(gc, c[0] + D[0], c[1] + D[1], c[2] + D[2]);
+    const int64_t J = valid ? node_at(gc, c[0] + D[0], c[1] + D[1], c[2] + D[2]) : I;
+#pragma unroll
+    for (int r = 0; r < BS; ++r)
+#pragma unroll
+        for (int t = 0; t < BS; ++t) {
+            const bool masked = cfix[I * BS + r] || cfix[J * BS + t];
+            const double v = valid && !masked ? acc[r * BS + t] : 0.0;
+            st64[((int64_t)s * B2 + r * BS + t) * nno + I] = v;
+            st32[st_index<ND>(node_major, nno, I, s, r * BS + t)] =
+                valid && masked && s == Sten<ND>::C && r == t ? 1.0f : (float)v;
+        }
 }
 
 // Symmetrise (the mean of A(I,J) and A(J,I)', the same bits from either
@@ -729,6 +1004,14 @@ __device__ __forceinline__ void stencil_row_group(const StOp<TV>& op, int64_t no
 // the lane-group form (SFEM_MG_GROUP_BELOW overrides it, read when the
 // hierarchy is built: both forms give the same bits).
 constexpr int64_t kStencilGroupBelow = 65536;
+
+// SFEM_MG_GALERKIN_LINES=1: the first coarse level of a matrix-free finest
+// level by the one-direction products too (read when the hierarchy is
+// built; the two forms agree to FP64 rounding)
+bool galerkin_lines_forced() {
+    const char* e = std::getenv("SFEM_MG_GALERKIN_LINES");
+    return e && e[0] == '1';
+}
 
 int64_t stencil_group_below() {
     const char* e = std::getenv("SFEM_MG_GROUP_BELOW");
@@ -2220,14 +2503,53 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         hipLaunchKernelGGL(coarse_fixed_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, prev_fix, gf, gc,
                            f[0], f[1], f[2], C.fixed.get());
         SFEM_LAUNCH_CHECK();
-        // P'AP one direction at a time (x, then y, then z)
+        // P'AP: cell by cell from the element scales (the first coarse level
+        // under a matrix-free finest level, every direction coarsened), else
+        // one direction at a time (x, then y, then z)
         DevBuf<double> raw, tmp;
         const double* cur = prev_st;
         Grid gcur = gf;
         int last = -1;
         for (int d = 0; d < ND; ++d)
             if (f[d] == 2) last = d;
-        for (int d = 0; d < ND; ++d) {
+        bool by_cells = !prev_st && !galerkin_lines_forced();
+        for (int d = 0; d < ND; ++d) by_cells = by_cells && f[d] == 2;
+        // the smoothing kernels address a coarse stencil with 32-bit byte offsets
+        if ((uint64_t)C.nn * NS * B2 * sizeof(float) >= (1ull << 32))
+            fail(SFEM_ERR_UNSUPPORTED, "multigrid: a coarse level's stencil exceeds 4 GiB (" + std::to_string(C.nn) +
+                                           " nodes)");
+        C.st64.reset(ctx, (size_t)C.nn * NS * B2);
+        C.st32.reset(ctx, (size_t)C.nn * NS * B2);
+        C.node_major = C.nn < stencil_group_below();
+        if (by_cells) {  // symmetric bit for bit and masked: no symmetrising pass
+            constexpr int NEN = 1 << ND;
+            DevBuf<double> gt(ctx, (size_t)NEN * NEN * NEN * B2);
+            hipLaunchKernelGGL(galerkin_table_kernel<ND>, dim3(rows_grid((int64_t)gt.size())), dim3(kThreads), 0, s,
+                               (const double*)H.kc64.get(), gt.get());
+            SFEM_LAUNCH_CHECK();
+            const int64_t ncell = (C.n[0] - 1) * (C.n[1] - 1) * (ND == 3 ? C.n[2] - 1 : 1);
+            DevBuf<int32_t> slot(ctx, (size_t)ncell), cells(ctx, (size_t)ncell);
+            DevBuf<unsigned> cnt(ctx, 1);
+            memset_d(ctx, cnt.get(), 0, sizeof(unsigned));
+            hipLaunchKernelGGL(cell_irregular_kernel<ND>, dim3(rows_grid(ncell)), dim3(kThreads), 0, s, prev_fix, gf,
+                               gc, slot.get(), cells.get(), cnt.get());
+            SFEM_LAUNCH_CHECK();
+            unsigned nirr = 0;
+            d2h(ctx, &nirr, cnt.get(), sizeof(unsigned));
+            DevBuf<double> kcell(ctx, (size_t)std::max(nirr, 1u) * NEN * NEN * B2);
+            if (nirr) {
+                hipLaunchKernelGGL(cell_matrix_kernel<ND>, dim3(nirr), dim3(64), 0, s, (const double*)gt.get(),
+                                   (const double*)H.kc64.get(), (const double*)H.es64.get(), prev_fix, gf, gc,
+                                   (const int32_t*)cells.get(), kcell.get());
+                SFEM_LAUNCH_CHECK();
+            }
+            hipLaunchKernelGGL(galerkin_cells_kernel<ND>, dim3(rows_grid(C.nn), NS), dim3(kThreads), 0, s,
+                               (const double*)gt.get(), (const double*)kcell.get(), (const double*)H.es64.get(),
+                               (const int32_t*)slot.get(), gf, gc, (const uint8_t*)C.fixed.get(), C.st64.get(),
+                               C.st32.get(), C.node_major);
+            SFEM_LAUNCH_CHECK();
+        }
+        for (int d = 0; d < ND && !by_cells; ++d) {
             if (f[d] != 2) continue;
             Grid gn = gcur;
             int64_t* nd = d == 0 ? &gn.n0 : d == 1 ? &gn.n1 : &gn.n2;
@@ -2251,16 +2573,11 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
             }
             gcur = gn;
         }
-        // the smoothing kernels address a coarse stencil with 32-bit byte offsets
-        if ((uint64_t)C.nn * NS * B2 * sizeof(float) >= (1ull << 32))
-            fail(SFEM_ERR_UNSUPPORTED, "multigrid: a coarse level's stencil exceeds 4 GiB (" + std::to_string(C.nn) +
-                                           " nodes)");
-        C.st64.reset(ctx, (size_t)C.nn * NS * B2);
-        C.st32.reset(ctx, (size_t)C.nn * NS * B2);
-        C.node_major = C.nn < stencil_group_below();
-        hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn), NS), dim3(kThreads), 0, s, raw.get(), gc,
-                           C.fixed.get(), C.st64.get(), C.st32.get(), C.node_major);
-        SFEM_LAUNCH_CHECK();
+        if (!by_cells) {
+            hipLaunchKernelGGL(sym_mask_kernel<ND>, dim3(rows_grid(C.nn), NS), dim3(kThreads), 0, s, raw.get(), gc,
+                               C.fixed.get(), C.st64.get(), C.st32.get(), C.node_major);
+            SFEM_LAUNCH_CHECK();
+        }
         C.binv.reset(ctx, (size_t)C.nn * B2);
         hipLaunchKernelGGL(stencil_binv_kernel<ND>, dim3(rows_grid(C.nn)), dim3(kThreads), 0, s, C.st64.get(),
                            C.fixed.get(), C.nn, C.binv.get());

@@ -394,6 +394,42 @@ def test_mg_matrix_free_solves_match_oracle(ctx, dims):
     K.close()
 
 
+@pytest.mark.parametrize("dims", [(8, 4, 6), (7, 5, 3), (12, 6), (9, 7)])
+def test_first_coarse_level_by_cells(ctx, dims, monkeypatch):
+    """Under a matrix-free finest level the first coarse operator is summed
+    cell by cell from the element scales: regular cells from the per
+    sub-element table, cells with a fixed dof or a single fine element (odd
+    element counts) element by element with masked weights.  It equals
+    scipy's P'AP (1e-12) and the one-direction products
+    (SFEM_MG_GALERKIN_LINES=1) to FP64 rounding, with fixed dofs scattered
+    through the grid (single components of a node too)."""
+    print(f"[case] mg first level by cells {dims}", flush=True)
+    m = structured_mesh(dims)
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=29)
+    fixed = roller(m, axis=m.ndim - 1)
+    rng = np.random.default_rng(29)
+    fixed[rng.choice(m.n_dof, size=max(3, m.n_dof // 40), replace=False)] = 1
+    K = grid_K(ctx, m, young)
+    ops = {}
+    for lines in ("0", "1"):
+        monkeypatch.setenv("SFEM_MG_GALERKIN_LINES", lines)
+        bc = BoundaryConditions(ctx, m.n_dof, np.zeros(m.n_dof), fixed)
+        S = System(K, bc)
+        bc.close()
+        info = S.set_grid(m.dims, coarse_nodes=1)
+        assert info["matrix_free"] == 2
+        ops[lines] = S.mg_stencil(1)
+        S.close()
+    K.close()
+    A, cfx, new = galerkin_chain(oracle_K(m, young), fixed, dims, 2)[0]
+    st, fx = ops["0"]
+    assert np.array_equal(fx.reshape(-1).astype(bool), cfx)
+    G = stencil_to_csr(st, new)
+    assert abs(G - G.T).max() == 0.0
+    assert sp.linalg.norm(G - A) <= 1e-12 * sp.linalg.norm(A)
+    assert np.linalg.norm(st - ops["1"][0]) <= 1e-13 * np.linalg.norm(ops["1"][0])
+
+
 def test_mg_matrix_free_needs_grid_assembly(ctx):
     """K assembled without the grid (sorted keys): no element data, the
     finest level reads K."""
