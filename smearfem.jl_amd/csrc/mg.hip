@@ -1044,6 +1044,52 @@ __device__ __forceinline__ void cheb_epilogue(int64_t row, int64_t nrows, const 
     }
 }
 
+// The same step with its operands loaded first (cheb_load, beside the row's
+// own loads, so the kernel's chain of dependent loads is one shorter), then
+// cheb_finish: the arithmetic of cheb_epilogue.
+template <int BS>
+struct ChebIn {
+    float r[BS], x[BS], din[BS], binv[BS * BS];
+};
+
+template <int BS, int MODE>
+__device__ __forceinline__ void cheb_load(int64_t row, int64_t nrows, const float* __restrict__ r,
+                                          const float* __restrict__ x, const float* d_in,
+                                          const float* __restrict__ binv, ChebIn<BS>& o) {
+#pragma unroll
+    for (int j = 0; j < BS; ++j) o.r[j] = r[row * BS + j];
+    if (MODE == 1) return;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        o.x[j] = x[row * BS + j];
+        o.din[j] = d_in ? d_in[row * BS + j] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < BS * BS; ++k) o.binv[k] = binv[(int64_t)k * nrows + row];
+}
+
+template <int BS, int MODE>
+__device__ __forceinline__ void cheb_finish(int64_t row, const ChebIn<BS>& o, const float (&av)[BS],
+                                            float* __restrict__ r, float* __restrict__ x, bool has_din,
+                                            float* __restrict__ d_out, float c1, float c2) {
+    float rv[BS];
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        rv[j] = o.r[j] - av[j];
+        r[row * BS + j] = rv[j];
+    }
+    if (MODE == 1) return;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) {
+        float z = 0.0f;
+#pragma unroll
+        for (int i = 0; i < BS; ++i) z = fmaf(o.binv[j * BS + i], rv[i], z);
+        const float dn = fmaf(c2, z, has_din ? c1 * o.din[j] : 0.0f);
+        d_out[row * BS + j] = dn;
+        x[row * BS + j] = o.x[j] + dn;
+    }
+}
+
 template <int BS, int MODE, bool NT, typename TV>
 __global__ __launch_bounds__(kThreads) void cheb_sell_kernel(SellOp<TV> op, const float* v, float* __restrict__ r,
                                                              float* __restrict__ x, const float* d_in,
@@ -1068,17 +1114,23 @@ __global__ __launch_bounds__(kThreads, 4) void cheb_stencil_kernel(StOp<TV> op, 
                                                                    float* __restrict__ d_out, float c1, float c2,
                                                                    const float* __restrict__ binv, const int* done) {
     constexpr int G = Lanes<ND>::G;
-    if (*done) return;
+    // the flag is read beside the row's loads and checked before any store
+    const bool stop = *done != 0;
     const int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     float out[ND];
+    ChebIn<ND> in;
     if (GROUP) {
         const int64_t node = t / G;
+        const bool lead = t % G == 0 && node < op.nn;
+        if (lead) cheb_load<ND, MODE>(node, op.nn, r, x, d_in, binv, in);
         stencil_row_group<ND, TV>(op, node, (int)(t % G), v, out);
-        if (t % G == 0 && node < op.nn) cheb_epilogue<ND, MODE>(node, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+        if (lead && !stop) cheb_finish<ND, MODE>(node, in, out, r, x, d_in != nullptr, d_out, c1, c2);
     } else {
         if (t >= op.nn) return;
+        cheb_load<ND, MODE>(t, op.nn, r, x, d_in, binv, in);
         stencil_row<ND, TV>(op, t, v, out);
-        cheb_epilogue<ND, MODE>(t, op.nn, out, r, x, d_in, d_out, c1, c2, binv);
+        if (stop) return;
+        cheb_finish<ND, MODE>(t, in, out, r, x, d_in != nullptr, d_out, c1, c2);
     }
 }
 
@@ -1329,7 +1381,8 @@ __device__ __forceinline__ void restrict_row(int64_t I, const float* __restrict_
                                              const int32_t* __restrict__ row_of_node, const Grid& gc, int f0, int f1,
                                              int f2, const uint8_t* __restrict__ cfix, float* __restrict__ bc,
                                              const float* __restrict__ binv, float c2, float cscale,
-                                             float* __restrict__ rc, float* __restrict__ xc, float* __restrict__ dc) {
+                                             float* __restrict__ rc, float* __restrict__ xc, float* __restrict__ dc,
+                                             bool skip) {
     constexpr int BS = ND;
     int64_t ci[3];
     coords_of<ND>(I, gc, ci);
@@ -1385,8 +1438,10 @@ __device__ __forceinline__ void restrict_row(int64_t I, const float* __restrict_
             }
     const int64_t nnc = gc.n0 * gc.n1 * gc.n2;
 #pragma unroll
+    for (int j = 0; j < BS; ++j) acc[j] = cfix[I * BS + j] ? 0.0f : acc[j];
+    if (skip) return;
+#pragma unroll
     for (int j = 0; j < BS; ++j) {
-        acc[j] = cfix[I * BS + j] ? 0.0f : acc[j];
         bc[I * BS + j] = acc[j];
         if (FIRST) {
             acc[j] *= cscale;  // the coarse level's units
@@ -1414,10 +1469,11 @@ __global__ __launch_bounds__(kThreads) void restrict_kernel(const float* __restr
                                                             float c2, float cscale, float* __restrict__ rc,
                                                             float* __restrict__ xc, float* __restrict__ dc,
                                                             const int* done) {
-    if (*done) return;
+    const bool stop = *done != 0;  // checked before the stores
     const int64_t I = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (I >= gc.n0 * gc.n1 * gc.n2) return;
-    restrict_row<ND, FIRST>(I, rf, rscale, gf, row_of_node, gc, f0, f1, f2, cfix, bc, binv, c2, cscale, rc, xc, dc);
+    restrict_row<ND, FIRST>(I, rf, rscale, gf, row_of_node, gc, f0, f1, f2, cfix, bc, binv, c2, cscale, rc, xc, dc,
+                            stop);
 }
 
 // e = P x_c on the fine rows (0 on fixed fine dofs), x_f += e; node_of_row
@@ -1426,7 +1482,7 @@ template <int ND>
 __device__ __forceinline__ void prolong_row(int64_t row, const float* __restrict__ xc, const Grid& gc, const Grid& gf,
                                             int f0, int f1, int f2, const int32_t* __restrict__ node_of_row,
                                             const uint8_t* __restrict__ ffix, float* __restrict__ xf,
-                                            float* __restrict__ e) {
+                                            float* __restrict__ e, bool skip) {
     constexpr int BS = ND;
     const int64_t node = node_of_row ? (int64_t)node_of_row[row] : row;
     int64_t cf[3];
@@ -1473,6 +1529,7 @@ __device__ __forceinline__ void prolong_row(int64_t row, const float* __restrict
                     acc[j] = in ? t : acc[j];
                 }
             }
+    if (skip) return;
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
         const int64_t d = row * BS + j;
@@ -1488,21 +1545,21 @@ __global__ __launch_bounds__(kThreads) void prolong_kernel(const float* __restri
                                                            int64_t nrows, const uint8_t* __restrict__ ffix,
                                                            float* __restrict__ xf, float* __restrict__ e,
                                                            const int* done) {
-    if (*done) return;
+    const bool stop = *done != 0;  // checked before the stores
     const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (row >= nrows) return;
-    prolong_row<ND>(row, xc, gc, gf, f0, f1, f2, node_of_row, ffix, xf, e);
+    prolong_row<ND>(row, xc, gc, gf, f0, f1, f2, node_of_row, ffix, xf, e, stop);
 }
 
 // coarsest level: x = A^-1 b with the dense FP64 inverse (one block)
 __global__ __launch_bounds__(kThreads) void dense_apply_kernel(const double* __restrict__ inv, int64_t nc,
                                                                const float* __restrict__ b, float* __restrict__ x,
                                                                const int* done) {
-    if (*done) return;
+    const bool stop = *done != 0;  // checked before the stores
     for (int64_t i = threadIdx.x; i < nc; i += kThreads) {
         double s = 0.0;
         for (int64_t j = 0; j < nc; ++j) s = fma(inv[i * nc + j], (double)b[j], s);
-        x[i] = (float)s;
+        if (!stop) x[i] = (float)s;
     }
 }
 
