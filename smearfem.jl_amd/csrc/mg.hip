@@ -1352,20 +1352,23 @@ __global__ __launch_bounds__(kThreads) void cheb_first_kernel(const TB* __restri
                                                               const float* __restrict__ binv, float c2,
                                                               float* __restrict__ r, float* __restrict__ x,
                                                               float* __restrict__ d, const int* done) {
-    if (*done) return;
+    const bool stop = *done != 0;  // checked before the stores
     const int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x;
     if (row >= nrows) return;
     float rv[BS];
 #pragma unroll
-    for (int j = 0; j < BS; ++j) {
-        rv[j] = (float)(b[row * BS + j] * bscale);
-        r[row * BS + j] = rv[j];
-    }
+    for (int j = 0; j < BS; ++j) rv[j] = (float)(b[row * BS + j] * bscale);
+    float bi[BS * BS];
+#pragma unroll
+    for (int k = 0; k < BS * BS; ++k) bi[k] = binv[(int64_t)k * nrows + row];
+    if (stop) return;
+#pragma unroll
+    for (int j = 0; j < BS; ++j) r[row * BS + j] = rv[j];
 #pragma unroll
     for (int j = 0; j < BS; ++j) {
         float z = 0.0f;
 #pragma unroll
-        for (int i = 0; i < BS; ++i) z = fmaf(binv[(int64_t)(j * BS + i) * nrows + row], rv[i], z);
+        for (int i = 0; i < BS; ++i) z = fmaf(bi[j * BS + i], rv[i], z);
         const float dn = c2 * z;
         d[row * BS + j] = dn;
         x[row * BS + j] = dn;
