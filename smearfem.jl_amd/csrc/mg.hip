@@ -1684,7 +1684,7 @@ __global__ __launch_bounds__(kThreads) void pow_mf_kernel(MfOp<float> op, const 
     __shared__ double sh[8];
     const float scale = (float)ps->inv;
     double sum = 0.0;
-    for (int64_t row = (int64_t)blockIdx.x * kThreads + threadIdx.x; row < op.nrows;
+    for (int64_t row = logical_block() * kThreads + threadIdx.x; row < op.nrows;
          row += (int64_t)gridDim.x * kThreads) {
         float out[ND];
         mf_row<ND, float, float>(op, op.kc, row, v, out);
@@ -2500,7 +2500,8 @@ void build_levels(sfem_ctx* ctx, sfem_system* S, MgHierarchy& H, const sfem_mg_o
         p0b.reset(ctx, (size_t)(n + BS));
         memset_d(ctx, p0a.get(), 0, p0a.bytes());
         memset_d(ctx, p0b.get(), 0, p0b.bytes());
-        const unsigned gm = std::min(rows_grid(L0.nrows), 1024u);
+        // a row per thread, blocks mapped XCD-contiguously as in the smoothing step
+        const unsigned gm = xcd_rows_grid(L0.nrows);
         const hipStream_t t = side_stream(ctx, sfem_ctx::kSideStreams - 1);
         power_start(0, L0, p0a.get(), p0b.get(), gm, t);
         float* vin = p0a.get();
