@@ -124,15 +124,23 @@ int  sfem_mesh_detect_grid(sfem_ctx* ctx, sfem_mesh* mesh, int* ndim, int64_t* n
 /* flags for sfem_assemble_mesh */
 #define SFEM_ASM_DEFAULT       0
 #define SFEM_ASM_FORCE_SCALAR  1   /* sort dof-pair keys even when ID is node-blocked */
+#define SFEM_ASM_EXACT_SUMS    2   /* on a declared grid, sum the element matrices themselves */
 
 /* Assemble K = sparse(E,J,V) on the device.  Pattern is the union of the
  * element couplings (explicit zeros kept), duplicates are summed in
  * ascending COO index order, dims = max row/col id -- the semantics of
- * Julia's sparse(I,J,V) called at src/fem.jl:253. */
+ * Julia's sparse(I,J,V) called at src/fem.jl:253.  On a declared grid whose
+ * element matrices are all s_e Ke(0) to within 1e-13 of Ke(0)'s largest
+ * entry (s_e = Ke(e)[1,1] / Ke(0)[1,1], every element matrix computed and
+ * compared), K's values are summed from s_e and Ke(0) in the same order
+ * instead of from the stored element matrices: the same pattern, values
+ * within that deviation, K bitwise symmetric (sfem_csr_grid_values says
+ * which form was taken; SFEM_ASM_EXACT_SUMS keeps the element sums). */
 int sfem_assemble_mesh(sfem_ctx* ctx, const sfem_mesh* mesh, int flags, sfem_csr** out);
 
 /* One-shot drop-in for assemble_system: sfem_mesh_create +
- * sfem_mesh_detect_grid + sfem_assemble_mesh (the same K either way). */
+ * sfem_mesh_detect_grid + sfem_assemble_mesh with SFEM_ASM_DEFAULT (the
+ * same pattern either way; the values as sfem_assemble_mesh says). */
 int sfem_assemble(sfem_ctx* ctx, int ndim, int function_class, int n_dof,
                   const double* node_list, int64_t n_nodes,
                   const int64_t* ien, int64_t ien_rows, int nen, int64_t n_elem,
@@ -147,6 +155,11 @@ int  sfem_csr_info(const sfem_csr* K, int64_t* m, int64_t* n, int64_t* nnz, int*
  * *ndim = 0 when none; ne = 3 element counts (0 beyond ndim).  No counterpart
  * in the reference (its K carries no mesh). */
 int  sfem_csr_grid(const sfem_csr* K, int* ndim, int64_t* ne);
+/* How K's values were formed on its grid: *from_scales = 1 when summed from
+ * s_e Ke(0), 0 from the element matrices; *ke_dev = max over elements of
+ * |Ke(e) / s_e - Ke(0)| / max |Ke(0)| (+inf when not measured).  No
+ * counterpart in the reference.  Either pointer may be NULL. */
+int  sfem_csr_grid_values(const sfem_csr* K, int* from_scales, double* ke_dev);
 
 /* Fill caller-allocated host buffers with the 1-based CSC arrays of
  * SparseMatrixCSC{Float64,Int64}: colptr (n+1), rowval (nnz), nzval (nnz). */

@@ -181,6 +181,11 @@ namespace {
 // r and c have the same coordinate, else the one cell between them.  Their
 // contributions are summed in ascending element order starting from the
 // first, as the stable sort and segmented sum do, so K is bitwise the same.
+// K summed from the element scales when every Ke(e) / s_e is within this
+// much of Ke(0) (relative to Ke(0)'s largest entry): K's values then stay
+// within it of the element sums, 10x inside the 1e-12 the assembly is held to
+constexpr double kGridScaleDev = 1e-13;
+
 struct GridAsm {
     int64_t n[3];   // nodes per direction (1 beyond ndim)
     int64_t e[3];   // elements per direction (1 beyond ndim)
@@ -322,6 +327,86 @@ __global__ __launch_bounds__(128) void grid_gather_kernel(const double* __restri
     bcolidx[u] = (int32_t)c;
 }
 
+// K from the element scales, when every element matrix is s_e Ke(0) to
+// within the measured deviation: half a wavefront per node column c as in
+// grid_gather_kernel (the same blocks in the same places), lane s summing
+// over the shared cells in ascending element order s_e Ke(0)[row corner]
+// [column corner], Ke(0) (local corner numbering) staged in LDS.  The
+// column's blocks are put in LDS by rank and written out as one contiguous
+// run (consecutive lanes, consecutive doubles).  Ke(0) is bitwise symmetric
+// and the blocks (r, c) and (c, r) meet the same cells in the same order, so
+// K is bitwise symmetric.
+constexpr int kScaleThreads = 128;
+
+template <int ND>
+__global__ __launch_bounds__(kScaleThreads) void grid_scale_kernel(const double* __restrict__ ke0,
+                                                                   const double* __restrict__ es, GridAsm g,
+                                                                   int64_t nn, const int64_t* __restrict__ bcolptr,
+                                                                   double* __restrict__ bval,
+                                                                   int32_t* __restrict__ browidx,
+                                                                   int32_t* __restrict__ bcolidx) {
+    constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND, KE = NEN * NEN * B2;
+    constexpr int NS = ND == 3 ? 27 : 9, CPB = kScaleThreads / 32;
+    __shared__ double kl[KE];
+    __shared__ double sv[CPB][NS * B2];
+    for (int t = (int)threadIdx.x; t < KE; t += kScaleThreads) kl[t] = ke0[t];
+    __syncthreads();
+    const int hl = threadIdx.x & 31, w = threadIdx.x >> 5;
+    const int64_t c = (int64_t)blockIdx.x * CPB + w;
+    bool live = c < nn && hl < NS;
+    int64_t a[3] = {0, 0, 0};
+    if (c < nn) grid_coords<ND>(c, g, a);
+    const int s = hl;
+    const int o[3] = {s % 3, (s / 3) % 3, ND == 3 ? s / 9 : 1};  // offset + 1 per direction
+    int64_t b[3];
+    int k = 0, radix = 1;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        b[d] = a[d] + o[d] - 1;
+        live = live && b[d] >= 0 && b[d] < g.n[d];
+        const int lo = a[d] > 0 ? 0 : 1, hi = a[d] < g.n[d] - 1 ? 2 : 1;
+        k += (o[d] - lo) * radix;
+        radix *= hi - lo + 1;
+    }
+    if (live) {
+        double acc[B2];
+#pragma unroll
+        for (int m = 0; m < B2; ++m) acc[m] = 0.0;
+#pragma unroll
+        for (int kc = 0; kc < NEN; ++kc) {  // cell (a - 1 + k) per direction: ascending element index
+            bool ok = true;
+            int rc = 0, cc = 0;
+            int64_t cd[3];
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                const int kd = d < ND ? (kc >> d) & 1 : 1;
+                cd[d] = a[d] - 1 + kd;
+                const int rb = o[d] - kd;  // row node's corner bit in the cell
+                ok = ok && cd[d] >= 0 && cd[d] < g.e[d] && (rb == 0 || rb == 1);
+                rc += (rb & 1) << d;
+                cc += (1 - kd) << d;
+            }
+            if (!ok) continue;
+            const double se = es[cd[0] + g.e[0] * (cd[1] + g.e[1] * cd[2])];
+            const double* p = &kl[(g.corner[rc] * NEN + g.corner[cc]) * B2];
+#pragma unroll
+            for (int m = 0; m < B2; ++m) acc[m] = fma(se, p[m], acc[m]);
+        }
+#pragma unroll
+        for (int m = 0; m < B2; ++m) sv[w][k * B2 + m] = acc[m];
+        const int64_t u = bcolptr[c] + k;
+        browidx[u] = (int32_t)(b[0] + g.n[0] * (b[1] + g.n[1] * b[2]));
+        bcolidx[u] = (int32_t)c;
+    }
+    __syncthreads();
+    if (c < nn) {
+        const int64_t u0 = bcolptr[c];
+        const int nv = (int)(bcolptr[c + 1] - u0) * B2;
+        double* dst = bval + u0 * B2;
+        for (int t = hl; t < nv; t += 32) dst[t] = sv[w][t];
+    }
+}
+
 // Is IEN the meshgrid connectivity (corner order of element 0) and the node
 // numbering the identity interleave (dof of node m, component 0, = ndof m)?
 template <int ND>
@@ -455,9 +540,10 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     const int64_t KE = (int64_t)nen * nen * ndof * ndof;
     const int64_t ncoo = m->n_elem * KE;
     if (ncoo >= (int64_t(1) << 32)) fail(SFEM_ERR_UNSUPPORTED, "assemble_system: 2^32 or more COO entries");
-    DevBuf<double> V(ctx, (size_t)ncoo);
-    // a mesh declared a grid gets the element-matrix check inside the element
-    // kernel (used if the dof numbering below admits the grid path)
+    DevBuf<double> V;  // the element matrices (not built when K is summed from the scales)
+    // a mesh declared a grid gets the element-matrix check first: Ke(0), the
+    // scales s_e and the largest deviation of Ke(e) / s_e from Ke(0), without
+    // storing the element matrices
     const bool grid_try = m->grid_nd && ndof == m->ndim && m->n_elem > 0 && !(flags & SFEM_ASM_FORCE_SCALAR);
     DevBuf<double> ke0, escale, bmax, dev;
     if (grid_try) {
@@ -466,11 +552,19 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         bmax.reset(ctx, element_launch_blocks(m));
         dev.reset(ctx, 1);
         launch_element_zero(ctx, m, ke0.get(), vout.get());
-        launch_element_checked(ctx, m, V.get(), vout.get(), ElemCheck{ke0.get(), escale.get(), bmax.get()});
+        launch_element_checked(ctx, m, nullptr, vout.get(), ElemCheck{ke0.get(), escale.get(), bmax.get()});
+        const unsigned rblocks = std::min(256u, grid_for((int64_t)bmax.size(), 1024));
+        DevBuf<double> part(ctx, rblocks);
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(rblocks), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
+                           (int64_t)bmax.size(), part.get());
+        SFEM_LAUNCH_CHECK();
+        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)part.get(),
+                           (int64_t)rblocks, dev.get());
+        SFEM_LAUNCH_CHECK();
     } else {
+        V.reset(ctx, (size_t)ncoo);
         launch_element_stiffness(ctx, m, V.get(), vout.get());
     }
-    tm.mark();
     d2h(ctx, vcopies, vout.get(), sizeof(vcopies));
     const ValidateOut vr = combine_validate(vcopies);
 
@@ -504,7 +598,28 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
     const int row_bits = bits_for(K->nunits);
     if (2 * row_bits > 64) fail(SFEM_ERR_UNSUPPORTED, "too many dofs for 64-bit keys");
 
-    if (grid_try && bs == ndof && K->nunits == m->n_nodes) {
+    // the grid's pattern when the dof numbering is the nodes'; its values from
+    // the scales when every Ke(e) is s_e Ke(0) to within kGridScaleDev of
+    // Ke(0)'s largest entry (SFEM_ASM_EXACT_SUMS: always the element sums)
+    const bool grid_asm = grid_try && bs == ndof && K->nunits == m->n_nodes;
+    double ke_dev = HUGE_VAL;
+    if (grid_try) {
+        K->ke0.resize((size_t)KE);
+        d2h(ctx, K->ke0.data(), ke0.get(), sizeof(double) * KE);
+        double dabs = 0.0;
+        d2h(ctx, &dabs, dev.get(), sizeof(dabs));
+        double kmax = 0.0;
+        for (double v : K->ke0) kmax = std::max(kmax, std::fabs(v));
+        ke_dev = (K->ke0[0] > 0.0 && kmax > 0.0 && std::isfinite(dabs)) ? dabs / kmax : HUGE_VAL;
+    }
+    const bool by_scale = grid_asm && !(flags & SFEM_ASM_EXACT_SUMS) && ke_dev <= kGridScaleDev;
+    if (grid_try && !by_scale) {
+        V.reset(ctx, (size_t)ncoo);
+        launch_element_checked(ctx, m, V.get(), vout.get(), ElemCheck{ke0.get(), escale.get(), bmax.get()});
+    }
+    tm.mark();
+
+    if (grid_asm) {
         // structured grid: the pattern is the grid's, no keys or sort
         GridAsm g{};
         for (int d = 0; d < 3; ++d) {
@@ -531,29 +646,26 @@ void assemble(sfem_ctx* ctx, const sfem_mesh* m, int flags, sfem_csr* K) {
         K->bval.reset(ctx, (size_t)nnzb * bs * bs);
         K->escale = std::move(escale);
         const unsigned gblocks = grid_for(nn, 128 / 32);
-        if (m->grid_nd == 3)
+        if (by_scale) {
+            auto go = [&](auto kern) {
+                hipLaunchKernelGGL(kern, dim3(grid_for(nn, kScaleThreads / 32)), dim3(kScaleThreads), 0,
+                                   ctx->stream, (const double*)ke0.get(),
+                                   (const double*)K->escale.get(), g, nn, (const int64_t*)K->bcolptr.get(),
+                                   K->bval.get(), K->browidx.get(), K->bcolidx.get());
+            };
+            if (m->grid_nd == 3) go(grid_scale_kernel<3>);
+            else go(grid_scale_kernel<2>);
+        } else if (m->grid_nd == 3) {
             hipLaunchKernelGGL(grid_gather_kernel<3>, dim3(gblocks), dim3(128), 0, ctx->stream, V.get(), g, nn,
                                K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
-        else
+        } else {
             hipLaunchKernelGGL(grid_gather_kernel<2>, dim3(gblocks), dim3(128), 0, ctx->stream, V.get(), g, nn,
                                K->bcolptr.get(), K->bval.get(), K->browidx.get(), K->bcolidx.get());
-        SFEM_LAUNCH_CHECK();
-        const unsigned rblocks = std::min(256u, grid_for((int64_t)bmax.size(), 1024));
-        DevBuf<double> part(ctx, rblocks);
-        hipLaunchKernelGGL(max_reduce_kernel, dim3(rblocks), dim3(1024), 0, ctx->stream, (const double*)bmax.get(),
-                           (int64_t)bmax.size(), part.get());
-        SFEM_LAUNCH_CHECK();
-        hipLaunchKernelGGL(max_reduce_kernel, dim3(1), dim3(1024), 0, ctx->stream, (const double*)part.get(),
-                           (int64_t)rblocks, dev.get());
+        }
         SFEM_LAUNCH_CHECK();
         tm.mark();
-        K->ke0.resize((size_t)KE);
-        d2h(ctx, K->ke0.data(), V.get(), sizeof(double) * KE);
-        double dabs = 0.0;
-        d2h(ctx, &dabs, dev.get(), sizeof(dabs));
-        double kmax = 0.0;
-        for (double v : K->ke0) kmax = std::max(kmax, std::fabs(v));
-        K->ke_dev = (K->ke0[0] > 0.0 && kmax > 0.0 && std::isfinite(dabs)) ? dabs / kmax : HUGE_VAL;
+        K->ke_dev = ke_dev;
+        K->from_scales = by_scale;
         K->grid_nd = m->grid_nd;
         for (int d = 0; d < 3; ++d) K->grid_ne[d] = m->grid_ne[d];
         for (int cc = 0; cc < 8; ++cc) K->grid_corner[cc] = m->grid_corner[cc];

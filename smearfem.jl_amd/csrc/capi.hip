@@ -464,6 +464,13 @@ int sfem_csr_grid(const sfem_csr* K, int* ndim, int64_t* ne) {
     return SFEM_OK;
 }
 
+int sfem_csr_grid_values(const sfem_csr* K, int* from_scales, double* ke_dev) {
+    if (!K) return SFEM_ERR_ARGUMENT;
+    if (from_scales) *from_scales = K->from_scales ? 1 : 0;
+    if (ke_dev) *ke_dev = K->ke_dev;
+    return SFEM_OK;
+}
+
 int sfem_csr_stage_ms(const sfem_csr* K, double* ms) {
     if (!K || !ms) return SFEM_ERR_ARGUMENT;
     ms[0] = K->t_validate; ms[1] = K->t_element; ms[2] = K->t_keys; ms[3] = K->t_sort;

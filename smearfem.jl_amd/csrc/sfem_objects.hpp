@@ -65,6 +65,7 @@ struct sfem_csr {
     int64_t grid_ne[3] = {0, 0, 0};
     int grid_corner[8] = {0, 0, 0, 0, 0, 0, 0, 0};
     double ke_dev = HUGE_VAL;
+    bool from_scales = false;       // values summed from s_e Ke(0), not from the element matrices
     sfem::DevBuf<double> escale;    // n_elem: s_e = Ke(e)[0][0] / Ke(0)[0][0]
     std::vector<double> ke0;        // Ke(0), blocks (a*nen + b)*bs*bs + i*bs + j (local corners a, b)
     // device time per assembly stage (ms)

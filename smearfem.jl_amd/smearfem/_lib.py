@@ -173,6 +173,7 @@ _PROTOS = [
       c_i64p, C.c_int64, c_dblp, C.c_int64, C.c_double, C.POINTER(c_vp)]),
     ("sfem_csr_info", C.c_int, [c_vp, c_i64p, c_i64p, c_i64p, c_intp]),
     ("sfem_csr_grid", C.c_int, [c_vp, c_intp, c_i64p]),
+    ("sfem_csr_grid_values", C.c_int, [c_vp, c_intp, C.POINTER(C.c_double)]),
     ("sfem_csr_stage_ms", C.c_int, [c_vp, c_dblp]),
     ("sfem_export_csc1", C.c_int, [c_vp, c_vp, c_i64p, c_i64p, c_dblp]),
     ("sfem_csr_destroy", None, [c_vp]),

@@ -133,6 +133,14 @@ class StiffnessMatrix:
         self.ctx.check(self.ctx._lib.sfem_csr_grid(self.handle, C.byref(nd), _ptr(ne, C.c_int64)))
         return tuple(int(v) for v in ne[: nd.value]) if nd.value else None
 
+    def grid_values(self) -> dict:
+        """How K's values were formed on its grid (``sfem_csr_grid_values``):
+        ``from_scales`` (summed from s_e Ke(0)) and the measured
+        ``ke_deviation``."""
+        fs, dev = C.c_int(), C.c_double()
+        self.ctx.check(self.ctx._lib.sfem_csr_grid_values(self.handle, C.byref(fs), C.byref(dev)))
+        return {"from_scales": bool(fs.value), "ke_deviation": dev.value}
+
     def stage_ms(self) -> dict:
         ms = np.zeros(6)
         self.ctx.check(self.ctx._lib.sfem_csr_stage_ms(self.handle, _ptr(ms, C.c_double)))
@@ -226,10 +234,13 @@ class Mesh:
                                                            _ptr(ne, C.c_int64)))
         return tuple(int(v) for v in ne[: nd.value]) if nd.value else None
 
-    def assemble(self, force_scalar: bool = False) -> StiffnessMatrix:
+    def assemble(self, force_scalar: bool = False, exact_sums: bool = False) -> StiffnessMatrix:
+        """``sfem_assemble_mesh``; ``exact_sums``: on a declared grid, K from
+        the element matrices even when they are all s_e Ke(0)
+        (``SFEM_ASM_EXACT_SUMS``)."""
         h = C.c_void_p()
-        self.ctx.check(self.ctx._lib.sfem_assemble_mesh(self.ctx.handle, self.handle, 1 if force_scalar else 0,
-                                                        C.byref(h)))
+        flags = (1 if force_scalar else 0) | (2 if exact_sums else 0)
+        self.ctx.check(self.ctx._lib.sfem_assemble_mesh(self.ctx.handle, self.handle, flags, C.byref(h)))
         return StiffnessMatrix(self.ctx, h)
 
     def close(self) -> None:
@@ -248,16 +259,19 @@ class Mesh:
 
 def assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=None, Young=1, nu=0.3, *,
                     ctx: Optional[Context] = None, device: bool = False, force_scalar: bool = False,
-                    n_elem: Optional[int] = None, detect_grid: bool = True):
+                    n_elem: Optional[int] = None, detect_grid: bool = True, exact_sums: bool = False):
     """Assemble the global stiffness matrix on the GPU.
 
     Mirrors ``assemble_system`` (src/fem.jl:135): element count ``ne^ndim``
     (``n_elem`` overrides it for non-cubic meshes), ``Young`` scalar or one
     value per element.  Returns ``scipy.sparse.csc_matrix`` (or the device
     matrix with ``device=True``).  As ``sfem_assemble``, a mesh in the
-    meshgrid numbering is assembled from its grid (bitwise the same K; the
-    device matrix then knows its grid, ``K.grid``); ``detect_grid=False``
-    keeps the sort-based pipeline.
+    meshgrid numbering is assembled from its grid (the same pattern; the
+    device matrix then knows its grid, ``K.grid``); when every element
+    matrix is s_e Ke(0) to within 1e-13, K's values are summed from the
+    scales (``K.grid_values()``; ``exact_sums=True``: the element sums,
+    bitwise the sort-based K); ``detect_grid=False`` keeps the sort-based
+    pipeline.
     """
     ctx = ctx or default_context()
     if n_elem is None:
@@ -266,7 +280,7 @@ def assemble_system(ne, NodeList, IEN, ndim, FunctionClass="Q1", nDof=1, ID=None
     try:
         if detect_grid and not force_scalar:  # as sfem_assemble
             mesh.detect_grid()
-        K = mesh.assemble(force_scalar=force_scalar)
+        K = mesh.assemble(force_scalar=force_scalar, exact_sums=exact_sums)
     finally:
         mesh.close()
     if device:

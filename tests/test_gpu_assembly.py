@@ -188,16 +188,51 @@ def test_error_behaviour_matches_reference(ctx):
     assert both(m.NodeList, m.IEN, 2, "Q1", 1, None, n_elem=12) == (-2, -2)  # ne^ndim > size(IEN,1)
 
 
-def _mesh_assemble(ctx, m, young, coords=None, grid=None, nu=0.3):
+def _mesh_assemble(ctx, m, young, coords=None, grid=None, nu=0.3, exact_sums=False, values=None):
     mesh = smearfem.Mesh(ctx, m.n_elem, m.NodeList if coords is None else coords, m.IEN, m.ndim, "Q1", m.ndim,
                          m.ID, young, nu)
     if grid is not None:
         mesh.set_grid(grid)
-    K = mesh.assemble()
+    K = mesh.assemble(exact_sums=exact_sums)
     out = K.csc_arrays()
+    if values is not None:
+        values.update(K.grid_values())
     K.close()
     mesh.close()
     return out
+
+
+@pytest.mark.parametrize("dims", [(12, 6, 5), (3, 1, 2), (9, 7), (2, 2)])
+def test_structured_assembly_from_scales(ctx, dims):
+    """On a declared uniform grid every element matrix is s_e Ke(0) to
+    rounding, so K's values are summed from the scales: the pattern of the
+    sorted path, values within 1e-12 of the element sums (max |dK| / max |K|;
+    bitwise those sums with exact_sums), K bitwise symmetric and the same bits
+    run to run.  Distorted elements keep the element sums."""
+    print(f"[case] structured assembly from scales {dims}", flush=True)
+    m = structured_mesh(dims, lengths=(1.7, 1.1, 0.9)[: len(dims)])
+    young = element_young_loguniform(m.n_elem, 0.1, 10.0, seed=41)
+    ref = _mesh_assemble(ctx, m, young)
+    info = {}
+    got = _mesh_assemble(ctx, m, young, grid=dims, values=info)
+    assert info["from_scales"] and info["ke_deviation"] <= 1e-13, info
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.abs(got[2] - ref[2]).max() <= 1e-12 * np.abs(ref[2]).max()
+    shape = (got[0].size - 1,) * 2
+    assert symmetric_bitwise(shape, *got)
+    again = _mesh_assemble(ctx, m, young, grid=dims)
+    assert all(np.array_equal(a, b) for a, b in zip(got, again))
+    info = {}
+    exact = _mesh_assemble(ctx, m, young, grid=dims, exact_sums=True, values=info)
+    assert not info["from_scales"]
+    assert all(np.array_equal(a, b) for a, b in zip(exact, ref))
+    # distorted elements: not multiples of Ke(0), the element sums
+    rng = np.random.default_rng(41)
+    coords = m.NodeList + 0.05 / max(dims) * rng.uniform(-1, 1, m.NodeList.shape)
+    info = {}
+    dist = _mesh_assemble(ctx, m, young, coords, grid=dims, values=info)
+    assert not info["from_scales"] and info["ke_deviation"] > 1e-13
+    assert all(np.array_equal(a, b) for a, b in zip(dist, _mesh_assemble(ctx, m, young, coords)))
 
 
 @pytest.mark.parametrize("dims", [(6, 4, 3), (2, 1, 1), (1, 3, 2), (7, 5), (1, 1)])

@@ -7,7 +7,10 @@ as they are (column-major, 1-based); the example's elimination and solve
 ``sfem_system_set_grid`` + ``sfem_system_solve`` + ``sfem_system_solution``.
 The checks:
 * sfem_assemble detects a meshgrid-numbered mesh and assembles it from the
-  grid: the K it returns is bitwise the sort-based K, it reports its grid,
+  grid: the K it returns has the sort-based K's pattern and, its element
+  matrices being multiples of the first, values summed from the element
+  scales within 1e-12 of the sort-based K's (bitwise with
+  SFEM_ASM_EXACT_SUMS); it reports its grid,
   and the multigrid built without element counts applies the finest level
   matrix-free (``matrix_free`` 2), the path the benchmark measures;
 * the system holds its own reference on K: K destroyed right after
@@ -82,8 +85,11 @@ def test_reference_call_sequence_reaches_matrix_free_multigrid(ctx, case):
     ref = Ks.csc_arrays()
     Ks.close()
     mesh.close()
-    for a, b in zip(got, ref):
-        assert np.array_equal(a, b)
+    fs, dev = C.c_int(), C.c_double()
+    ctx.check(lib.sfem_csr_grid_values(K, C.byref(fs), C.byref(dev)))
+    assert fs.value == 1 and dev.value <= 1e-13
+    assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    assert np.abs(got[2] - ref[2]).max() <= 1e-12 * np.abs(ref[2]).max()
     # loads and constraints: z = 0 face clamped, random loads elsewhere
     n = got[0].size - 1
     z = np.asarray(nodes)[2]
