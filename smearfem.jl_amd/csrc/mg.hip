@@ -487,7 +487,8 @@ __global__ void cell_irregular_kernel(const uint8_t* __restrict__ ffix, Grid gf,
 // corner b) is summed in the order of the entry whose row dof is the lower
 // one, so kcell[k][a][b] and kcell[k][b][a] are transposes bit for bit.
 // One wavefront per cell, lane (a, b); fine nodes outside a one-element
-// cell are never read.
+// cell are never read.  Ke(0) is staged in LDS; the inner corner's
+// weights are formed once per element.
 template <int ND>
 __global__ __launch_bounds__(64) void cell_matrix_kernel(const double* __restrict__ gt, const double* __restrict__ kc,
                                                          const double* __restrict__ es,
@@ -495,6 +496,9 @@ __global__ __launch_bounds__(64) void cell_matrix_kernel(const double* __restric
                                                          const int32_t* __restrict__ cells,
                                                          double* __restrict__ kcell) {
     constexpr int BS = ND, B2 = BS * BS, NEN = 1 << ND;
+    __shared__ double kl[NEN * NEN * B2];
+    for (int t = (int)threadIdx.x; t < NEN * NEN * B2; t += 64) kl[t] = kc[t];
+    __syncthreads();
     const int k = (int)blockIdx.x;
     const int lane = (int)threadIdx.x;
     if (lane >= NEN * NEN) return;
@@ -547,15 +551,21 @@ __global__ __launch_bounds__(64) void cell_matrix_kernel(const double* __restric
                 }
                 return p;
             };
+            double gaj[NEN], gbj[NEN];  // the inner corner's weights, formed once
+#pragma unroll
+            for (int j = 0; j < NEN; ++j) {
+                gaj[j] = wt(a, j);
+                gbj[j] = wt(b, j);
+            }
 #pragma unroll
             for (int m = 0; m < B2; ++m) T[m] = 0.0;
 #pragma unroll 1
             for (int i = 0; i < NEN; ++i) {
                 const double gai = wt(a, i), gbi = wt(b, i);
-#pragma unroll 1
+                const double* ki = kl + i * NEN * B2;
+#pragma unroll
                 for (int j = 0; j < NEN; ++j) {
-                    const double gaj = wt(a, j), gbj = wt(b, j);
-                    const double* kb = kc + ((int64_t)i * NEN + j) * B2;
+                    const double* kb = ki + j * B2;
 #pragma unroll
                     for (int r = 0; r < BS; ++r)
 #pragma unroll
@@ -566,7 +576,7 @@ __global__ __launch_bounds__(64) void cell_matrix_kernel(const double* __restric
                             const bool fir = (fx >> (i * BS + r)) & 1u, fit = (fx >> (i * BS + t)) & 1u;
                             const bool fjr = (fx >> (j * BS + r)) & 1u, fjt = (fx >> (j * BS + t)) & 1u;
                             const double x = lo ? (fir ? 0.0 : gai) : (fit ? 0.0 : gbi);
-                            const double y = lo ? (fjt ? 0.0 : gbj) : (fjr ? 0.0 : gaj);
+                            const double y = lo ? (fjt ? 0.0 : gbj[j]) : (fjr ? 0.0 : gaj[j]);
                             T[r * BS + t] = fma(x * y, lo ? kb[r * BS + t] : kb[t * BS + r], T[r * BS + t]);
                         }
                 }
